@@ -1,0 +1,148 @@
+#!/usr/bin/env python3
+"""Throughput of the per-move wavefunction path on one MI355X.
+
+A step is one metrop1 sweep of the whole walker population: for every
+electron a drift-diffusion proposal, psie (distances, jastrow4e, orbitalse,
+Sherman-Morrison ratio), the new drift, accept/reject and commit; then one
+hpsi (full Jastrow, orbitals, LU inverse, ECP quadrature, local energy) per
+walker.  Workload: 40 nuclei (20 Z_eff=4 with s/p non-local ECP + 20 H),
+160 electrons (80/80), 600 AOs, 12-point quadrature, 4096 walkers, FP64.
+
+Prints one JSON line (walker-moves/s = walkers * electrons * sweeps / s).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic FP64 flop counts of the kernels' loop structure (per walker):
+#   move (jastrow4e + AO):  (Ne-1)*Nc*(25T+60) + AO
+#   hpsi_nonloc (per quadrature point): (Ne-1)*(Nc*(4T+40) + 60) + Nc*40 + Nao*(40 + 2)
+# T = e-e-n terms (13), Nc = centres, Nao = AOs.
+
+
+def flops_move_jastrow(ne, nc, nt, nao, nrad_total):
+    return (ne - 1) * nc * (25 * nt + 60) + nao * 20 + nrad_total * 40
+
+
+def flops_nonloc_point(ne, nc, nt, nao):
+    return (ne - 1) * (nc * (4 * nt + 40) + 60) + nc * 40 + nao * 42
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--walkers", type=int, default=4096)
+    ap.add_argument("--system", default="c160")
+    ap.add_argument("--tau", type=float, default=0.3)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-sweeps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this engine runs the walker population on one GPU; "
+              "multi-GPU runs (walker sharding, see DESIGN.md) are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    from champ_amd import system as S
+    from champ_amd.engine import Engine
+
+    sy = S.config(args.system)
+    W = args.walkers
+    eng = Engine(sy, W)
+    eng.set_walkers(sy.initial_walkers(W, seed=1))
+    eng.seed_rng(S.seed_from_string())
+    eng.hpsi()
+    print(f"[bench] {sy.name}: Ne={sy.nelec} Nc={sy.ncent} Nao={sy.nbasis} W={W}; warmup {args.warmup}", flush=True)
+    if args.warmup:
+        eng.metrop1(args.warmup, args.tau)
+    eng.enable_timing(True)
+    t0 = time.perf_counter()
+    st = eng.metrop1(args.steps, args.tau)
+    t1 = time.perf_counter()
+    kt = eng.kernel_times()
+    eng.enable_timing(False)
+    wall = t1 - t0
+    moves = W * sy.nelec * args.steps
+    value = moves / wall
+    nxq = eng.array("nxquad")
+    print(f"[bench] {args.steps} sweeps in {wall:.3f}s; acceptance {st['accepted'] / st['proposed']:.3f}; "
+          f"<E_L> {st['esum'] / (W * args.steps):.4f}; flagged {st['nflagged']}", flush=True)
+    for k, (ms, n) in kt.items():
+        if n:
+            print(f"[bench]   {k:20s} {ms:10.2f} ms  {n:6d} launches  {ms / n:9.4f} ms/launch", flush=True)
+
+    # dominant kernel class by time + its algorithmic rate
+    nt = len(S.een_terms(sy.nordc))
+    nrad = int(sum(sy.nrbas[t] for t in sy.iwctype))
+    dom = max(kt.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dn) = dom[0], dom[1]
+    if dname == "hpsi_nonloc":
+        fl = float(np.sum(nxq)) * flops_nonloc_point(sy.nelec, sy.ncent, nt, sy.nbasis)
+    elif dname == "move_jastrow_ao":
+        fl = W * flops_move_jastrow(sy.nelec, sy.ncent, nt, sy.nbasis, nrad)
+    elif dname == "hpsi_jastrow":
+        fl = W * 0.5 * sy.nelec * (sy.nelec - 1) * sy.ncent * (54 * nt + 80)
+    else:
+        fl = None
+    roof = None
+    if fl is not None and dn:
+        avg_s = dms / dn / 1e3
+        ach = fl / avg_s / 1e12
+        roof = {"bound": "mfma", "pipe": "fp64-valu", "kernel": dname, "achieved": round(ach, 3), "peak": 78.6,
+                "unit": "TFLOP/s", "frac": round(ach / 78.6, 4), "traffic": None,
+                "avg_launch_ms": round(dms / dn, 4), "flops_per_launch": fl}
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import ffi
+        import ctypes as C
+        nth = args.cpu_threads
+        nwc = nth
+        x0 = sy.initial_walkers(nwc, seed=1)
+        streams = ffi.seeded_streams(S.seed_from_string(), nwc)
+        es, acc = np.zeros(1), np.zeros(1)
+        cs = sy.as_ctypes()
+        tc = ffi.lib().orc_vmc_run(C.byref(cs), nwc, ffi.P(x0), ffi.U(streams), args.cpu_sweeps, args.tau, nth,
+                                    ffi.P(es), ffi.P(acc))
+        cpu = {"value": round(nwc * sy.nelec * args.cpu_sweeps / tc, 2), "unit": "walker-moves/s", "cores": nth,
+               "kind": "port",
+               "sample": f"{nwc} walkers x {args.cpu_sweeps} metrop1 sweep(s) of the same system, one walker per "
+                         f"thread, {tc:.1f} s"}
+
+    out = {
+        "metric": "walker-moves/sec at 160 e- on 1 MI355X (FP64 GPU path)",
+        "value": round(value, 1),
+        "unit": "walker-moves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator: geometry, basis, MOs, Jastrow, ECP; walkers Gaussian around nuclei)",
+        "config": {"workload": f"metrop1 sweep, {sy.nelec} e, {sy.ncent} nuclei, {sy.nbasis} AOs, ECP + 12-pt "
+                               f"quadrature, {W} walkers", "walkers": W, "electrons": sy.nelec,
+                   "nuclei": sy.ncent, "aos": sy.nbasis, "precision": "fp64", "parallelism": "walkers on 1 GPU"},
+        "walker_sweeps_per_s": round(W * args.steps / wall, 2),
+        "acceptance": round(st["accepted"] / st["proposed"], 4),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

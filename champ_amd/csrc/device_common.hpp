@@ -1,0 +1,238 @@
+// device_common.hpp -- device-side building blocks shared by the engine
+// kernels: wavefront reductions, xoshiro256**, the grid-3 spline, Cartesian
+// monomials, scaled distances and the e-e-n polynomial.  Each routine cites
+// the CHAMP routine whose arithmetic it reproduces.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define WAVE 64
+
+namespace champ {
+
+// ---------------------------------------------------------------- reductions
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// --------------------------------------------------------------- xoshiro256**
+// Blackman & Vigna (2018), bound by CHAMP through random.f90:3-55.
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+__device__ __forceinline__ uint64_t xo_next(uint64_t s[4])
+{
+    const uint64_t result = rotl64(s[1] * 5, 7) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl64(s[3], 45);
+    return result;
+}
+__device__ __forceinline__ double xo_uniform(uint64_t s[4])
+{
+    return (double)(xo_next(s) >> 11) * 0x1.0p-53;
+}
+// gauss.f90:3-22: cos(2 pi u1) * sqrt(-2 ln u2)
+__device__ __forceinline__ double xo_gauss(uint64_t s[4])
+{
+    const double pi = 3.141592653589793;  // 4*atan(1) in double
+    double g = cos(2 * pi * xo_uniform(s));
+    return g * sqrt(-2 * log(xo_uniform(s)));
+}
+
+// ------------------------------------------------------------ integer power
+// gfortran's real**integer (libgcc __powidf2): binary exponentiation.
+template <typename T>
+__device__ __forceinline__ T powi(T x, int m)
+{
+    unsigned n = m < 0 ? (unsigned)(-m) : (unsigned)m;
+    T y = (n % 2) ? x : T(1);
+    while (n >>= 1) {
+        x = x * x;
+        if (n % 2) y = y * x;
+    }
+    return m < 0 ? T(1) / y : y;
+}
+
+// ------------------------------------------------------------------- tables
+struct SplineTab {
+    const double *rwf, *d2rwf, *ce, *ae, *rmaxwf;
+    int nr, nrbas_max;
+    double r0, arg, dlogag;
+};
+
+// splfit.f90:3-94, grid type 3.  ider 0 -> wfv[0]; ider >= 1 -> wfv[0..2]
+// (R''' is only needed by forces and is not produced).
+template <typename T>
+__device__ __forceinline__ void splfit(const SplineTab &tab, int it, int irb, T r, T wfv[3], int ider)
+{
+    const size_t base = ((size_t)it * tab.nrbas_max + irb);
+    const double r0 = tab.r0, arg = tab.arg;
+    double xr = log(((double)r + r0) / r0) / tab.dlogag + 1.0;
+    // compare before converting: a NaN or huge r must never become a table index
+    int jx = (xr >= 1.0 && xr < (double)tab.nr) ? (int)xr : (xr >= (double)tab.nr ? tab.nr : 0);
+    if (jx < 1) {
+        const double *ce = tab.ce + base * 5;
+        T v = (T)ce[0];
+        for (int ii = 2; ii <= 5; ii++) v += (T)ce[ii - 1] * powi(r, ii - 1);
+        wfv[0] = v;
+        if (ider >= 1) {
+            T d1 = 0, d2 = 0;
+            for (int ii = 2; ii <= 5; ii++) {
+                d1 += (T)((ii - 1) * ce[ii - 1]) * powi(r, ii - 2);
+                d2 += (T)((ii - 1) * (ii - 2) * ce[ii - 1]) * powi(r, ii - 3);
+            }
+            wfv[1] = d1;
+            wfv[2] = d2;
+        }
+    } else if (jx >= tab.nr) {
+        const double *ae = tab.ae + base * 2;
+        T a1 = (T)ae[1];
+        wfv[0] = (T)ae[0] * exp(-a1 * r);
+        wfv[1] = -a1 * wfv[0];
+        wfv[2] = -a1 * wfv[1];
+    } else {
+        const double *rwf = tab.rwf + base * tab.nr;
+        const double *d2rwf = tab.d2rwf + base * tab.nr;
+        double ref0 = r0 * powi(arg, jx - 1) - r0;
+        double ref1 = (ref0 + r0) * arg - r0;
+        T delh = (T)(ref1 - ref0);
+        T y0 = (T)rwf[jx - 1], y1 = (T)rwf[jx], s0 = (T)d2rwf[jx - 1], s1 = (T)d2rwf[jx];
+        const T sixth = T(1) / T(6);
+        T delhi = T(1) / delh;
+        T bb = (r - (T)ref0) * delhi;
+        T aa = ((T)ref1 - r) * delhi;
+        T cc = aa * (aa * aa - T(1)) * delh * delh * sixth;
+        T dd = bb * (bb * bb - T(1)) * delh * delh * sixth;
+        wfv[0] = aa * y0 + bb * y1 + cc * s0 + dd * s1;
+        if (ider >= 1) {
+            wfv[1] = (y1 - y0) * delhi + (-(T(3) * aa * aa - T(1)) * s0 + (T(3) * bb * bb - T(1)) * s1) * delh * sixth;
+            wfv[2] = aa * s0 + bb * s1;
+        }
+    }
+}
+
+// slm.f90:7-200 (ider < 3): monomial y, gradient dy, Laplacian.
+template <typename T>
+__device__ __forceinline__ void slm(int l, T x, T yy, T z, T &y, T dy[3], T &lap)
+{
+    const T cs = (T)0.28209479177387814, cp = (T)0.48860251190291992;
+    const T cd1 = (T)0.630783130505040, cd2 = (T)1.092548430592079;
+    const T cf = (T)0.74635266518023078, cf2 = (T)1.66889529453113635, cf3 = (T)2.89061144264055405;
+    dy[0] = dy[1] = dy[2] = 0;
+    lap = 0;
+    switch (l) {
+    case 1: y = cs; break;
+    case 2: y = cp * x; dy[0] = cp; break;
+    case 3: y = cp * yy; dy[1] = cp; break;
+    case 4: y = cp * z; dy[2] = cp; break;
+    case 5: y = cd1 * x * x; dy[0] = T(2) * cd1 * x; lap = T(2) * cd1; break;
+    case 6: y = cd2 * x * yy; dy[0] = cd2 * yy; dy[1] = cd2 * x; break;
+    case 7: y = cd2 * x * z; dy[0] = cd2 * z; dy[2] = cd2 * x; break;
+    case 8: y = cd1 * yy * yy; dy[1] = T(2) * cd1 * yy; lap = T(2) * cd1; break;
+    case 9: y = cd2 * yy * z; dy[1] = cd2 * z; dy[2] = cd2 * yy; break;
+    case 10: y = cd1 * z * z; dy[2] = T(2) * cd1 * z; lap = T(2) * cd1; break;
+    case 11: y = cf * x * x * x; dy[0] = cf * T(3) * x * x; lap = cf * T(6) * x; break;
+    case 12: y = cf2 * x * x * yy; dy[0] = cf2 * T(2) * x * yy; dy[1] = cf2 * x * x; lap = cf2 * T(2) * yy; break;
+    case 13: y = cf2 * x * x * z; dy[0] = cf2 * T(2) * x * z; dy[2] = cf2 * x * x; lap = cf2 * T(2) * z; break;
+    case 14: y = cf2 * yy * yy * x; dy[1] = cf2 * T(2) * yy * x; dy[0] = cf2 * yy * yy; lap = cf2 * T(2) * x; break;
+    case 15: y = cf3 * x * yy * z; dy[0] = cf3 * yy * z; dy[1] = cf3 * x * z; dy[2] = cf3 * x * yy; break;
+    case 16: y = cf2 * z * z * x; dy[0] = cf2 * z * z; dy[2] = cf2 * T(2) * z * x; lap = cf2 * T(2) * x; break;
+    case 17: y = cf * yy * yy * yy; dy[1] = cf * T(3) * yy * yy; lap = cf * T(6) * yy; break;
+    case 18: y = cf2 * yy * yy * z; dy[1] = cf2 * T(2) * yy * z; dy[2] = cf2 * yy * yy; lap = cf2 * T(2) * z; break;
+    case 19: y = cf2 * z * z * yy; dy[1] = cf2 * z * z; dy[2] = cf2 * T(2) * z * yy; lap = cf2 * T(2) * yy; break;
+    case 20: y = cf * z * z * z; dy[2] = cf * T(3) * z * z; lap = cf * T(6) * z; break;
+    default: y = 0; break;
+    }
+}
+
+// One AO at one point: basis_fns.f90:72-110 + phi_combine :217-296.
+// Returns value, gradient and (ider == 2) Laplacian.
+template <typename T>
+__device__ __forceinline__ void ao_eval(const SplineTab &tab, int it, int irad, int l, T rx, T ry, T rz, T r,
+                                        double rmax, int ider, T out[5])
+{
+    T wfv[3] = {0, 0, 0};
+    if ((double)r <= rmax) splfit<T>(tab, it, irad, r, wfv, ider);
+    T y, dy[3], lap;
+    slm<T>(l, rx, ry, rz, y, dy, lap);
+    out[0] = y * wfv[0];
+    if (ider >= 1) {
+        T ri = T(1) / r;
+        T xcri[3] = {rx * ri, ry * ri, rz * ri};
+        T dum = 0;
+        for (int jj = 0; jj < 3; jj++) {
+            out[1 + jj] = y * xcri[jj] * wfv[1] + dy[jj] * wfv[0];
+            dum = dum + dy[jj] * xcri[jj];
+        }
+        if (ider >= 2) {
+            T d2 = y * wfv[2] + y * T(2) * ri * wfv[1] + lap * wfv[0];
+            out[4] = d2 + T(2) * dum * wfv[1];
+        }
+    }
+}
+
+// ----------------------------------------------------------------- Jastrow
+#define CHAMP_MAXCT 4
+#define CHAMP_MAXT 32
+#define CHAMP_NORDP 8
+
+// ijas=4, isc=2 parameters (m_jastrow.f90:145-147) + asymptotes (set_scale_dist).
+struct JasParams {
+    int norda, nordb, nordc, nterms, nspin2b, nocuspb, nctype, pad;
+    double kap, asymp_r;
+    double a4[CHAMP_MAXCT][CHAMP_NORDP];
+    double b[2][CHAMP_NORDP];
+    double c[CHAMP_MAXCT][CHAMP_MAXT];
+    double ja[CHAMP_MAXCT];
+    double jb[2];
+    signed char tk[CHAMP_MAXT], tl[CHAMP_MAXT], tm[CHAMP_MAXT];  // (k,l,m) of each e-e-n term
+};
+
+// scale_dist2 / switch_scale (scale_dist.f90:159-331), isc = 2
+template <typename T>
+__device__ __forceinline__ T scale_r(T kap, T r) { return (T(1) - exp(-kap * r)) / kap; }
+
+// e-e-n sums for one pair and one centre: jastrow4.f90:203-262 (fc, fu, fi,
+// fj and, with want2, the second derivatives).  uu/ri/rj are the switched
+// scaled distances.  Power tables have zeros at negative indices.
+template <typename T, bool WANT2>
+struct Een {
+    T fc, fu, fi, fj, fuu, fii, fjj, fui, fuj;
+};
+
+template <typename T, bool WANT2>
+__device__ __forceinline__ Een<T, WANT2> een_eval(const JasParams &jp, const double *c, const T *uu, const T *rri,
+                                                  const T *rrj, const T *ss, const T *tt)
+{
+    Een<T, WANT2> e;
+    e.fc = e.fu = e.fi = e.fj = 0;
+    e.fuu = e.fii = e.fjj = e.fui = e.fuj = 0;
+    for (int ll = 0; ll < jp.nterms; ll++) {
+        const int k = jp.tk[ll], l = jp.tl[ll], m = jp.tm[ll];
+        const T cl = (T)c[ll];
+        const T stv = ss[l] * tt[m];
+        e.fc = e.fc + cl * uu[k] * stv;
+        e.fu = e.fu + cl * T(k) * uu[k - 1] * stv;
+        const T gi = T(l + m) * rri[l + m - 1] * rrj[m] + T(m) * rri[m - 1] * rrj[l + m];
+        const T gj = T(l + m) * rrj[l + m - 1] * rri[m] + T(m) * rrj[m - 1] * rri[l + m];
+        e.fi = e.fi + cl * uu[k] * gi;
+        e.fj = e.fj + cl * uu[k] * gj;
+        if (WANT2) {
+            e.fuu = e.fuu + cl * T(k * (k - 1)) * uu[k - 2] * stv;
+            e.fii = e.fii + cl * uu[k] * (T((l + m) * (l + m - 1)) * rri[l + m - 2] * rrj[m] + T(m * (m - 1)) * rri[m - 2] * rrj[l + m]);
+            e.fjj = e.fjj + cl * uu[k] * (T((l + m) * (l + m - 1)) * rrj[l + m - 2] * rri[m] + T(m * (m - 1)) * rrj[m - 2] * rri[l + m]);
+            e.fui = e.fui + cl * T(k) * uu[k - 1] * gi;
+            e.fuj = e.fuj + cl * T(k) * uu[k - 1] * gj;
+        }
+    }
+    return e;
+}
+
+}  // namespace champ

@@ -1,0 +1,613 @@
+// engine_api.hip -- the extern "C" boundary declared in include/champ_gpu.h.
+// Context set-up uploads the CHAMP tables once; every other entry point acts
+// on the whole walker population on the context's stream.
+#include "engine_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace champ {
+static thread_local std::string g_err;
+void set_error(const std::string &s) { g_err = s; }
+int launch_hpsi_impl(champ_ctx *c, int accumulate);
+void set_tau(double tau);
+
+void tbegin(champ_ctx *c, int cls, int *slot)
+{
+    *slot = -1;
+    if (!c->timing) return;
+    size_t idx = c->ev_used.size();
+    if (idx >= c->ev_pool.size()) {
+        Timer t;
+        hipEventCreate(&t.a);
+        hipEventCreate(&t.b);
+        c->ev_pool.push_back(t);
+    }
+    c->ev_used.push_back({cls, (int)idx});
+    hipEventRecord(c->ev_pool[idx].a, c->stream);
+    *slot = (int)idx;
+}
+void tend(champ_ctx *c, int slot)
+{
+    if (slot < 0) return;
+    hipEventRecord(c->ev_pool[slot].b, c->stream);
+}
+static void tflush(champ_ctx *c)
+{
+    if (!c->timing || c->ev_used.empty()) return;
+    hipStreamSynchronize(c->stream);
+    for (auto &u : c->ev_used) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, c->ev_pool[u.second].a, c->ev_pool[u.second].b);
+        c->class_ms[u.first] += ms;
+        c->class_n[u.first] += 1;
+    }
+    c->ev_used.clear();
+}
+
+// host xoshiro256** for seeding (Blackman & Vigna 2018)
+static inline uint64_t hrotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+static uint64_t hnext(uint64_t s[4])
+{
+    const uint64_t result = hrotl(s[1] * 5, 7) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = hrotl(s[3], 45);
+    return result;
+}
+static void hjump(uint64_t s[4])
+{
+    static const uint64_t J[4] = {0x180ec6d33cfd0abaULL, 0xd5a61266f0c9392cULL, 0xa9582618e03fc9aaULL,
+                                  0x39abdc4529b1661cULL};
+    uint64_t t[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 4; i++)
+        for (int b = 0; b < 64; b++) {
+            if (J[i] & ((uint64_t)1 << b)) for (int k = 0; k < 4; k++) t[k] ^= s[k];
+            hnext(s);
+        }
+    for (int k = 0; k < 4; k++) s[k] = t[k];
+}
+static double hpowi(double x, int m)
+{
+    unsigned n = m < 0 ? (unsigned)(-m) : (unsigned)m;
+    double y = (n % 2) ? x : 1.0;
+    while (n >>= 1) { x = x * x; if (n % 2) y = y * x; }
+    return m < 0 ? 1.0 / y : y;
+}
+
+template <typename T>
+static int dalloc(T **p, size_t n)
+{
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) { set_error(std::string("hipMalloc: ") + hipGetErrorString(e)); return CHAMP_ERR_NOMEM; }
+    return CHAMP_OK;
+}
+template <typename T>
+static int upload(T **p, const T *h, size_t n)
+{
+    int rc = dalloc(p, n);
+    if (rc) return rc;
+    if (n) HIPCHECK(hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return CHAMP_OK;
+}
+}  // namespace champ
+
+using namespace champ;
+
+#define TRY(x) do { int rc_ = (x); if (rc_) { champ_gpu_free(c); *ctxp = nullptr; return rc_; } } while (0)
+
+extern "C" const char *champ_gpu_last_error(void) { return g_err.c_str(); }
+
+extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t nwalk, int32_t precision)
+{
+    if (!ctxp || !s || nwalk <= 0) { set_error("init: bad arguments"); return CHAMP_ERR_ARG; }
+    *ctxp = nullptr;
+    if (precision != CHAMP_PREC_FP64) { set_error("init: only CHAMP_PREC_FP64 is available in this build"); return CHAMP_ERR_UNSUPPORTED; }
+    if (s->nctype > CHAMP_MAXCT || s->ncent > 128 || s->nelec > 1024 || s->nelec < 2 || s->ndet != 1 ||
+        s->norda >= CHAMP_NORDP || s->nordb >= CHAMP_NORDP || s->nordc >= CHAMP_NORDP || s->nspin2b < 1 ||
+        s->nspin2b > 2 || (s->nloc != 0 && s->nloc != 4) || s->nquad > CHAMP_MPS_QUAD) {
+        set_error("init: system outside the supported envelope (nctype<=4, ncent<=128, 2<=nelec<=1024, ndet=1, orders<8, nloc 0|4)");
+        return CHAMP_ERR_UNSUPPORTED;
+    }
+    for (int e = 0; e < s->nelec; e++)
+        if (s->iworbd[e] < 0 || s->iworbd[e] >= s->norb) { set_error("init: iworbd outside 0..norb-1"); return CHAMP_ERR_ARG; }
+    for (int ic = 0; ic < s->ncent; ic++)
+        if (s->iwctype[ic] < 0 || s->iwctype[ic] >= s->nctype) { set_error("init: iwctype outside 0..nctype-1"); return CHAMP_ERR_ARG; }
+    for (int it = 0; it < s->nctype; it++)
+        for (int i = 0; i < s->nbastyp[it]; i++) {
+            const int ir = s->iwrwf[it * s->nbastyp_max + i], l = s->iwlbas[it * s->nbastyp_max + i];
+            if (ir < 0 || ir >= s->nrbas[it] || l < 1 || l > CHAMP_MAX_SLM) { set_error("init: AO table entry out of range"); return CHAMP_ERR_ARG; }
+        }
+    if (s->nr < 2 || !s->cdet) { set_error("init: radial grid or cdet missing"); return CHAMP_ERR_ARG; }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) { set_error("init: no HIP device"); return CHAMP_ERR_HIP; }
+    champ_ctx *c = new champ_ctx();
+    *ctxp = c;
+    c->W = nwalk; c->NE = s->nelec; c->NUP = s->nup; c->NDN = s->nelec - s->nup;
+    c->NM = std::max(c->NUP, c->NDN); c->NC = s->ncent; c->NCT = s->nctype; c->NB = s->nbasis; c->NO = s->norb;
+    c->NQ = s->nloc ? s->nquad : 0; c->nloc = s->nloc; c->prec = precision; c->cdet0 = s->cdet[0];
+    if (c->NM > 400) { set_error("init: more than 400 electrons of one spin"); delete c; *ctxp = nullptr; return CHAMP_ERR_UNSUPPORTED; }
+    if (hipStreamCreate(&c->stream) != hipSuccess) { set_error("init: stream"); delete c; *ctxp = nullptr; return CHAMP_ERR_HIP; }
+    if (rocblas_create_handle(&c->blas) != rocblas_status_success) { set_error("init: rocblas"); champ_gpu_free(c); *ctxp = nullptr; return CHAMP_ERR_HIP; }
+    rocblas_set_stream(c->blas, c->stream);
+
+    const int NE = c->NE, NC = c->NC, NB = c->NB, NO = c->NO, W = c->W, NM = c->NM, NCT = c->NCT;
+    // ---- pecent (pot_nn, pot.f90:3-50)
+    double pecent = 0;
+    for (int i = 1; i < NC; i++)
+        for (int j = 0; j < i; j++) {
+            double r2 = 0;
+            for (int k = 0; k < 3; k++) { double d = s->cent[3 * i + k] - s->cent[3 * j + k]; r2 = r2 + d * d; }
+            pecent = pecent + s->znuc[s->iwctype[i]] * s->znuc[s->iwctype[j]] / std::sqrt(r2);
+        }
+    c->pecent = pecent;
+
+    TRY(upload(&c->cent, s->cent, (size_t)3 * NC));
+    TRY(upload(&c->iwctype, s->iwctype, (size_t)NC));
+    // ---- spline tables
+    const size_t ntab = (size_t)NCT * s->nrbas_max;
+    double *d_rwf, *d_d2, *d_ce, *d_ae, *d_rmax;
+    TRY(upload(&d_rwf, s->rwf, ntab * s->nr));
+    TRY(upload(&d_d2, s->d2rwf, ntab * s->nr));
+    TRY(upload(&d_ce, s->ce, ntab * CHAMP_NCOEF));
+    TRY(upload(&d_ae, s->ae, ntab * 2));
+    TRY(upload(&d_rmax, s->rmaxwf, ntab));
+    c->tab.rwf = d_rwf; c->tab.d2rwf = d_d2; c->tab.ce = d_ce; c->tab.ae = d_ae; c->tab.rmaxwf = d_rmax;
+    c->tab.nr = s->nr; c->tab.nrbas_max = s->nrbas_max; c->tab.r0 = s->grid_r0; c->tab.arg = s->grid_arg;
+    c->tab.dlogag = std::log(s->grid_arg);
+    // ---- AO table (basis_fns ordering: centres in order, AOs of the centre's type)
+    std::vector<int> aoc, aot, aor, aol;
+    std::vector<double> aorm;
+    for (int ic = 0; ic < NC; ic++) {
+        const int it = s->iwctype[ic];
+        for (int i = 0; i < s->nbastyp[it]; i++) {
+            const int ir = s->iwrwf[it * s->nbastyp_max + i];
+            aoc.push_back(ic); aot.push_back(it); aor.push_back(ir);
+            aol.push_back(s->iwlbas[it * s->nbastyp_max + i]);
+            aorm.push_back(s->rmaxwf[it * s->nrbas_max + ir]);
+        }
+    }
+    if ((int)aoc.size() != NB) { set_error("init: nbasis does not match the per-type AO counts"); champ_gpu_free(c); *ctxp = nullptr; return CHAMP_ERR_ARG; }
+    TRY(upload(&c->ao_cent, aoc.data(), NB)); TRY(upload(&c->ao_type, aot.data(), NB));
+    TRY(upload(&c->ao_rad, aor.data(), NB)); TRY(upload(&c->ao_slm, aol.data(), NB));
+    TRY(upload(&c->rmax_ao, aorm.data(), NB));
+    TRY(upload(&c->coef, s->coef, (size_t)NO * NB));
+    TRY(upload(&c->iworbd, s->iworbd, (size_t)NE));
+    for (int iab = 0; iab < 2; iab++) {
+        const int nel = iab ? c->NDN : c->NUP, ish = iab ? c->NUP : 0;
+        std::vector<double> h((size_t)std::max(nel, 1) * NB, 0.0);
+        for (int j = 0; j < nel; j++) std::memcpy(&h[(size_t)j * NB], s->coef + (size_t)s->iworbd[ish + j] * NB, sizeof(double) * NB);
+        TRY(upload(&c->cocc[iab], h.data(), h.size()));
+    }
+    // ---- Jastrow parameters + asymptotes (set_scale_dist, scale_dist.f90:3-60)
+    JasParams &jp = c->hjas;
+    std::memset(&jp, 0, sizeof(jp));
+    jp.norda = s->norda; jp.nordb = s->nordb; jp.nordc = s->nordc; jp.nspin2b = s->nspin2b; jp.nocuspb = s->nocuspb;
+    jp.nctype = NCT; jp.kap = s->scalek; jp.asymp_r = 1.0 / s->scalek;
+    int nt = 0;
+    for (int n = 2; n <= s->nordc; n++)
+        for (int k = n - 1; k >= 0; k--) {
+            const int l_hi = (k == 0) ? n - k - 2 : n - k;
+            for (int l = l_hi; l >= 0; l--) {
+                const int m = (n - k - l) / 2;
+                if (2 * m == n - k - l) {
+                    if (nt >= CHAMP_MAXT) { set_error("init: too many e-e-n terms"); champ_gpu_free(c); *ctxp = nullptr; return CHAMP_ERR_UNSUPPORTED; }
+                    jp.tk[nt] = (signed char)k; jp.tl[nt] = (signed char)l; jp.tm[nt] = (signed char)m; nt++;
+                }
+            }
+        }
+    jp.nterms = nt;
+    const double ar = jp.asymp_r;
+    for (int it = 0; it < NCT; it++) {
+        for (int p = 0; p <= s->norda; p++) jp.a4[it][p] = s->a4[it * (s->norda + 1) + p];
+        for (int p = 0; p < nt; p++) jp.c[it][p] = s->c[it * nt + p];
+        double ja = jp.a4[it][0] * ar / (1 + jp.a4[it][1] * ar);
+        for (int iord = 2; iord <= s->norda; iord++) ja = ja + jp.a4[it][iord] * hpowi(ar, iord);
+        jp.ja[it] = ja;
+    }
+    for (int isp = 0; isp < s->nspin2b; isp++)
+        for (int p = 0; p <= s->nordb; p++) jp.b[isp][p] = s->b[isp * (s->nordb + 1) + p];
+    for (int i = 0; i < 2; i++) {
+        double sspinn; int isp;
+        if (i == 0) { sspinn = 1; isp = 0; }
+        else { sspinn = (s->nspin2b == 1 && s->nocuspb == 0) ? 0.5 : 1.0; isp = s->nspin2b - 1; }
+        double jb = sspinn * jp.b[isp][0] * ar / (1 + jp.b[isp][1] * ar);
+        for (int iord = 2; iord <= s->nordb; iord++) jb = jb + jp.b[isp][iord] * hpowi(ar, iord);
+        jp.jb[i] = jb;
+    }
+    TRY(upload(&c->jas, &c->hjas, 1));
+    // ---- ECP
+    EcpParams &ep = c->hecp;
+    std::memset(&ep, 0, sizeof(ep));
+    ep.nloc = s->nloc; ep.nquad = c->NQ;
+    int nonloc_cent = 0;
+    for (int it = 0; it < NCT; it++) {
+        ep.znuc[it] = s->znuc[it];
+        ep.lpot[it] = s->nloc ? s->lpot[it] : 0;
+        if (s->nloc)
+            for (int l = 0; l < CHAMP_MPS_L; l++) {
+                ep.nterm[it][l] = s->necp_term[it * CHAMP_MPS_L + l];
+                for (int g = 0; g < CHAMP_MGAUSS; g++) {
+                    const size_t q = ((size_t)it * CHAMP_MPS_L + l) * CHAMP_MGAUSS + g;
+                    ep.power[it][l][g] = s->necp_power[q];
+                    ep.coef[it][l][g] = s->ecp_coef[q];
+                    ep.expo[it][l][g] = s->ecp_exponent[q];
+                }
+            }
+    }
+    for (int q = 0; q < c->NQ; q++) ep.wq[q] = s->wq[q];
+    for (int ic = 0; ic < NC; ic++) if (s->nloc && s->lpot[s->iwctype[ic]] > 1) nonloc_cent++;
+    TRY(upload(&c->ecp, &c->hecp, 1));
+    const int nq = std::max(c->NQ, 1);
+    c->h_xq0.assign(nq, 0.0); c->h_yq0.assign(nq, 0.0); c->h_zq0.assign(nq, 0.0);
+    for (int q = 0; q < c->NQ; q++) { c->h_xq0[q] = s->xq0[q]; c->h_yq0[q] = s->yq0[q]; c->h_zq0[q] = s->zq0[q]; }
+    TRY(upload(&c->xq0, c->h_xq0.data(), nq)); TRY(upload(&c->yq0, c->h_yq0.data(), nq)); TRY(upload(&c->zq0, c->h_zq0.data(), nq));
+
+    // ---- walker state
+    TRY(dalloc(&c->xT, (size_t)W * 3 * NE));
+    TRY(dalloc(&c->r_en, (size_t)W * NC * NE)); TRY(dalloc(&c->rr_en, (size_t)W * NC * NE));
+    TRY(dalloc(&c->fso, (size_t)W * NE * NE));
+    TRY(dalloc(&c->g1, (size_t)W * NE * 3 * NE)); TRY(dalloc(&c->g2, (size_t)W * NE * 3 * NE));
+    TRY(dalloc(&c->vjT, (size_t)W * 3 * NE));
+    TRY(dalloc(&c->fsumo, W)); TRY(dalloc(&c->d2o, W));
+    TRY(dalloc(&c->dorb, (size_t)W * NE * 3 * NO));
+    TRY(dalloc(&c->slmi, (size_t)W * 2 * NM * NM));
+    TRY(dalloc(&c->detiab, (size_t)W * 2));
+    TRY(dalloc(&c->ddx, (size_t)W * NE * 3)); TRY(dalloc(&c->d2dx2, (size_t)W * NE));
+    TRY(dalloc(&c->psido, W)); TRY(dalloc(&c->psijo, W)); TRY(dalloc(&c->psi2o, W)); TRY(dalloc(&c->eold, W));
+    TRY(dalloc(&c->ekin, W)); TRY(dalloc(&c->pe_local, W)); TRY(dalloc(&c->vpsp, (size_t)2 * W));
+    TRY(dalloc(&c->status, W)); TRY(dalloc(&c->rng, (size_t)4 * W));
+    TRY(dalloc(&c->xq, (size_t)W * nq)); TRY(dalloc(&c->yq, (size_t)W * nq)); TRY(dalloc(&c->zq, (size_t)W * nq));
+    // move scratch
+    TRY(dalloc(&c->xnew, (size_t)3 * W)); TRY(dalloc(&c->dfus2o, W));
+    TRY(dalloc(&c->rnew, (size_t)W * NC)); TRY(dalloc(&c->rrnew, (size_t)W * NC));
+    TRY(dalloc(&c->fsn_row, (size_t)W * NE));
+    TRY(dalloc(&c->g1n, (size_t)W * 3 * NE)); TRY(dalloc(&c->g2n, (size_t)W * 3 * NE)); TRY(dalloc(&c->vjnT, (size_t)W * 3 * NE));
+    TRY(dalloc(&c->fsumn, W));
+    TRY(dalloc(&c->aomove, (size_t)W * 4 * NB)); TRY(dalloc(&c->orbn, (size_t)W * 4 * NO));
+    TRY(dalloc(&c->ratio, W)); TRY(dalloc(&c->psidn, W)); TRY(dalloc(&c->psijn, W));
+    TRY(dalloc(&c->vnew, (size_t)3 * W)); TRY(dalloc(&c->vold, (size_t)3 * W));
+    TRY(dalloc(&c->acc, W)); TRY(dalloc(&c->esum_dev, 4));
+    // hpsi scratch
+    const double per_w = (double)NE * 5 * NB * 8;
+    c->hpsi_chunk = (int)std::max(1.0, std::min((double)W, 3.0e9 / per_w));
+    TRY(dalloc(&c->aoall, (size_t)c->hpsi_chunk * NE * 5 * NB));
+    TRY(dalloc(&c->orball, (size_t)c->hpsi_chunk * NE * 5 * NO));
+    c->pair_cap = std::max(1, NE * nonloc_cent);
+    if (c->nloc) {
+        TRY(dalloc(&c->wmat, (size_t)W * NE * NB));
+        TRY(dalloc(&c->pair_e, (size_t)W * c->pair_cap)); TRY(dalloc(&c->pair_c, (size_t)W * c->pair_cap));
+        TRY(dalloc(&c->pair_vps, (size_t)W * c->pair_cap * CHAMP_MPS_L));
+        TRY(dalloc(&c->pair_val, (size_t)W * c->pair_cap));
+    }
+    TRY(dalloc(&c->pair_count, W)); TRY(dalloc(&c->pair_off, (size_t)W + 1)); TRY(dalloc(&c->pair_total, 2));
+    TRY(dalloc(&c->nxquad, W));
+    HIPCHECK(hipMemset(c->status, 0, sizeof(int) * W));
+    HIPCHECK(hipMemset(c->nxquad, 0, sizeof(int) * W));
+    HIPCHECK(hipMemset(c->pair_count, 0, sizeof(int) * W));
+    HIPCHECK(hipMemset(c->esum_dev, 0, sizeof(double) * 4));
+    // quadrature starts unrotated
+    {
+        std::vector<double> hx((size_t)W * nq), hy((size_t)W * nq), hz((size_t)W * nq);
+        for (int w = 0; w < W; w++)
+            for (int q = 0; q < nq; q++) { hx[(size_t)w * nq + q] = c->h_xq0[q]; hy[(size_t)w * nq + q] = c->h_yq0[q]; hz[(size_t)w * nq + q] = c->h_zq0[q]; }
+        HIPCHECK(hipMemcpy(c->xq, hx.data(), hx.size() * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(c->yq, hy.data(), hy.size() * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(c->zq, hz.data(), hz.size() * 8, hipMemcpyHostToDevice));
+    }
+    // large LDS for the Slater inverse
+    {
+        const size_t lds = (size_t)(NM * (NM + 1) + NM) * sizeof(double);
+        if (lds > 160 * 1024) { set_error("init: Slater matrix does not fit in LDS"); champ_gpu_free(c); *ctxp = nullptr; return CHAMP_ERR_UNSUPPORTED; }
+    }
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_free(champ_ctx *c)
+{
+    if (!c) return CHAMP_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void *ptrs[] = {c->cent, c->iwctype, (void *)c->tab.rwf, (void *)c->tab.d2rwf, (void *)c->tab.ce, (void *)c->tab.ae,
+                    (void *)c->tab.rmaxwf, c->rmax_ao, c->ao_cent, c->ao_type, c->ao_rad, c->ao_slm, c->coef,
+                    c->cocc[0], c->cocc[1], c->iworbd, c->jas, c->ecp, c->xq0, c->yq0, c->zq0, c->xT, c->r_en,
+                    c->rr_en, c->fso, c->g1, c->g2, c->vjT, c->fsumo, c->d2o, c->dorb, c->slmi, c->detiab, c->ddx,
+                    c->d2dx2, c->psido, c->psijo, c->psi2o, c->eold, c->ekin, c->pe_local, c->vpsp, c->status,
+                    c->rng, c->xq, c->yq, c->zq, c->xnew, c->dfus2o, c->rnew, c->rrnew, c->fsn_row, c->g1n, c->g2n,
+                    c->vjnT, c->fsumn, c->aomove, c->orbn, c->ratio, c->psidn, c->psijn, c->vnew, c->vold, c->acc,
+                    c->esum_dev, c->aoall, c->orball, c->wmat, c->pair_count, c->pair_off, c->pair_total,
+                    c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->nxquad};
+    for (void *p : ptrs) if (p) hipFree(p);
+    for (auto &t : c->ev_pool) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
+    if (c->blas) rocblas_destroy_handle(c->blas);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_set_walkers(champ_ctx *c, const double *x)
+{
+    if (!c || !x) return CHAMP_ERR_ARG;
+    std::vector<double> h((size_t)c->W * 3 * c->NE);
+    for (int w = 0; w < c->W; w++)
+        for (int e = 0; e < c->NE; e++)
+            for (int k = 0; k < 3; k++) h[((size_t)w * 3 + k) * c->NE + e] = x[((size_t)w * c->NE + e) * 3 + k];
+    HIPCHECK(hipMemcpyAsync(c->xT, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_get_walkers(champ_ctx *c, double *x)
+{
+    if (!c || !x) return CHAMP_ERR_ARG;
+    std::vector<double> h((size_t)c->W * 3 * c->NE);
+    HIPCHECK(hipMemcpyAsync(h.data(), c->xT, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (int w = 0; w < c->W; w++)
+        for (int e = 0; e < c->NE; e++)
+            for (int k = 0; k < 3; k++) x[((size_t)w * c->NE + e) * 3 + k] = h[((size_t)w * 3 + k) * c->NE + e];
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_set_rng(champ_ctx *c, const uint64_t *st)
+{
+    if (!c || !st) return CHAMP_ERR_ARG;
+    HIPCHECK(hipMemcpy(c->rng, st, sizeof(uint64_t) * 4 * c->W, hipMemcpyHostToDevice));
+    return CHAMP_OK;
+}
+extern "C" int champ_gpu_get_rng(champ_ctx *c, uint64_t *st)
+{
+    if (!c || !st) return CHAMP_ERR_ARG;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipMemcpy(st, c->rng, sizeof(uint64_t) * 4 * c->W, hipMemcpyDeviceToHost));
+    return CHAMP_OK;
+}
+extern "C" int champ_gpu_seed_rng(champ_ctx *c, const int32_t seed[8])
+{
+    if (!c || !seed) return CHAMP_ERR_ARG;
+    uint64_t s[4];
+    for (int i = 0; i < 4; i++) s[i] = (uint64_t)(uint32_t)seed[2 * i] | ((uint64_t)(uint32_t)seed[2 * i + 1] << 32);
+    std::vector<uint64_t> h((size_t)4 * c->W);
+    for (int w = 0; w < c->W; w++) {
+        for (int k = 0; k < 4; k++) h[4 * (size_t)w + k] = s[k];
+        hjump(s);
+    }
+    return champ_gpu_set_rng(c, h.data());
+}
+
+extern "C" int champ_gpu_set_quadrature(champ_ctx *c, const double *xq, const double *yq, const double *zq)
+{
+    if (!c || !xq || !yq || !zq) return CHAMP_ERR_ARG;
+    if (c->NQ == 0) return CHAMP_OK;
+    const size_t n = (size_t)c->W * c->NQ * 8;
+    HIPCHECK(hipMemcpy(c->xq, xq, n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->yq, yq, n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->zq, zq, n, hipMemcpyHostToDevice));
+    return CHAMP_OK;
+}
+extern "C" int champ_gpu_rotqua(champ_ctx *c)
+{
+    if (!c) return CHAMP_ERR_ARG;
+    if (c->NQ == 0) return CHAMP_OK;
+    int rc = launch_rotqua(c);
+    if (rc) return rc;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return CHAMP_OK;
+}
+
+static int copy_out(champ_ctx *c, double *h, const double *d, size_t n)
+{
+    if (!h) return CHAMP_OK;
+    HIPCHECK(hipMemcpyAsync(h, d, n * 8, hipMemcpyDeviceToHost, c->stream));
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_hpsi(champ_ctx *c, double *psid, double *psij, double *ekin, double *energy)
+{
+    if (!c) return CHAMP_ERR_ARG;
+    int rc = launch_hpsi_impl(c, 0);
+    if (rc) return rc;
+    if ((rc = copy_out(c, psid, c->psido, c->W))) return rc;
+    if ((rc = copy_out(c, psij, c->psijo, c->W))) return rc;
+    if ((rc = copy_out(c, ekin, c->ekin, c->W))) return rc;
+    if ((rc = copy_out(c, energy, c->eold, c->W))) return rc;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_compute_determinante_grad(champ_ctx *c, int32_t iel, int32_t iflag_move, double *vd)
+{
+    if (!c || !vd || iel < 0 || iel >= c->NE) return CHAMP_ERR_ARG;
+    if (iflag_move == 1) {
+        int rc = launch_drift_old(c, iel);
+        if (rc) return rc;
+        HIPCHECK(hipMemcpyAsync(vd, c->vold, (size_t)3 * c->W * 8, hipMemcpyDeviceToHost, c->stream));
+    } else if (iflag_move == 0) {
+        HIPCHECK(hipMemcpyAsync(vd, c->vnew, (size_t)3 * c->W * 8, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        return CHAMP_ERR_UNSUPPORTED;
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_psie(champ_ctx *c, int32_t iel, const double *xnew, double *psidn, double *psijn, int32_t iflag)
+{
+    if (!c || !xnew || iel < 0 || iel >= c->NE) return CHAMP_ERR_ARG;
+    if (iflag != 0) { set_error("psie: only iflag=0 (metrop1) is provided"); return CHAMP_ERR_UNSUPPORTED; }
+    HIPCHECK(hipMemcpyAsync(c->xnew, xnew, (size_t)3 * c->W * 8, hipMemcpyHostToDevice, c->stream));
+    int rc;
+    if ((rc = launch_move_a(c, iel, 1))) return rc;
+    if ((rc = launch_move_gemm(c))) return rc;
+    if ((rc = launch_move_b(c, iel, 1))) return rc;
+    if ((rc = copy_out(c, psidn, c->psidn, c->W))) return rc;
+    if ((rc = copy_out(c, psijn, c->psijn, c->W))) return rc;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_accept(champ_ctx *c, int32_t iel, const uint8_t *accept)
+{
+    if (!c || !accept || iel < 0 || iel >= c->NE) return CHAMP_ERR_ARG;
+    HIPCHECK(hipMemcpyAsync(c->acc, accept, c->W, hipMemcpyHostToDevice, c->stream));
+    int rc = launch_move_b(c, iel, 2);
+    if (rc) return rc;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_metrop1(champ_ctx *c, int32_t nsweep, double tau, champ_sweep_stats *stats)
+{
+    if (!c || nsweep < 0 || !(tau > 0)) return CHAMP_ERR_ARG;
+    set_tau(tau);
+    HIPCHECK(hipMemsetAsync(c->esum_dev, 0, sizeof(double) * 4, c->stream));
+    int rc;
+    for (int isw = 0; isw < nsweep; isw++) {
+        if (c->nloc > 0 && (rc = launch_rotqua(c))) return rc;
+        for (int iel = 0; iel < c->NE; iel++) {
+            if ((rc = launch_move_a(c, iel, 0))) return rc;
+            if ((rc = launch_move_gemm(c))) return rc;
+            if ((rc = launch_move_b(c, iel, 0))) return rc;
+        }
+        if ((rc = launch_hpsi_impl(c, 1))) return rc;
+        if (c->timing && (isw % 4) == 3) tflush(c);
+    }
+    double h[4];
+    HIPCHECK(hipMemcpyAsync(h, c->esum_dev, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    tflush(c);
+    if (stats) {
+        std::vector<int> st(c->W);
+        HIPCHECK(hipMemcpy(st.data(), c->status, sizeof(int) * c->W, hipMemcpyDeviceToHost));
+        int64_t nf = 0;
+        for (int v : st) nf += v != 0;
+        stats->accepted = h[0];
+        stats->proposed = (double)nsweep * c->W * c->NE;
+        stats->esum = h[1];
+        stats->esum2 = h[2];
+        stats->nflagged = nf;
+    }
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_get_walker_scalars(champ_ctx *c, double *psido, double *psijo, double *eold, double *psi2o)
+{
+    if (!c) return CHAMP_ERR_ARG;
+    int rc;
+    if ((rc = copy_out(c, psido, c->psido, c->W))) return rc;
+    if ((rc = copy_out(c, psijo, c->psijo, c->W))) return rc;
+    if ((rc = copy_out(c, eold, c->eold, c->W))) return rc;
+    if ((rc = copy_out(c, psi2o, c->psi2o, c->W))) return rc;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_get_status(champ_ctx *c, int32_t *status)
+{
+    if (!c || !status) return CHAMP_ERR_ARG;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipMemcpy(status, c->status, sizeof(int) * c->W, hipMemcpyDeviceToHost));
+    return CHAMP_OK;
+}
+
+extern "C" int64_t champ_gpu_array_size(champ_ctx *c, int32_t which)
+{
+    if (!c) return -1;
+    const int64_t W = c->W, NE = c->NE, NM = c->NM, NO = c->NO;
+    switch (which) {
+    case CHAMP_ARR_SLMI: return W * 2 * NM * NM;
+    case CHAMP_ARR_VJ: case CHAMP_ARR_DDX: case CHAMP_ARR_VJN: return W * NE * 3;
+    case CHAMP_ARR_D2DX2: return W * NE;
+    case CHAMP_ARR_DETIAB: case CHAMP_ARR_VPSP: return W * 2;
+    case CHAMP_ARR_PELOC: case CHAMP_ARR_D2J: case CHAMP_ARR_NXQUAD: return W;
+    case CHAMP_ARR_ORBN: return W * 4 * NO;
+    default: return -1;
+    }
+}
+
+extern "C" int champ_gpu_get_array(champ_ctx *c, int32_t which, double *out)
+{
+    if (!c || !out) return CHAMP_ERR_ARG;
+    const int64_t n = champ_gpu_array_size(c, which);
+    if (n < 0) return CHAMP_ERR_ARG;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    const size_t W = c->W, NE = c->NE;
+    auto get = [&](const double *d, size_t cnt, std::vector<double> &h) -> int {
+        h.resize(cnt);
+        HIPCHECK(hipMemcpy(h.data(), d, cnt * 8, hipMemcpyDeviceToHost));
+        return CHAMP_OK;
+    };
+    std::vector<double> h;
+    int rc = CHAMP_OK;
+    switch (which) {
+    case CHAMP_ARR_SLMI: rc = get(c->slmi, n, h); if (!rc) std::memcpy(out, h.data(), n * 8); break;
+    case CHAMP_ARR_VJ:
+    case CHAMP_ARR_VJN:
+        rc = get(which == CHAMP_ARR_VJ ? c->vjT : c->vjnT, W * 3 * NE, h);
+        if (!rc) for (size_t w = 0; w < W; w++) for (size_t e = 0; e < NE; e++) for (int k = 0; k < 3; k++)
+            out[(w * NE + e) * 3 + k] = h[(w * 3 + k) * NE + e];
+        break;
+    case CHAMP_ARR_DDX: rc = get(c->ddx, n, h); if (!rc) std::memcpy(out, h.data(), n * 8); break;
+    case CHAMP_ARR_D2DX2: rc = get(c->d2dx2, n, h); if (!rc) std::memcpy(out, h.data(), n * 8); break;
+    case CHAMP_ARR_DETIAB: rc = get(c->detiab, n, h); if (!rc) std::memcpy(out, h.data(), n * 8); break;
+    case CHAMP_ARR_VPSP: rc = get(c->vpsp, n, h); if (!rc) std::memcpy(out, h.data(), n * 8); break;
+    case CHAMP_ARR_PELOC: rc = get(c->pe_local, n, h); if (!rc) std::memcpy(out, h.data(), n * 8); break;
+    case CHAMP_ARR_D2J: rc = get(c->d2o, n, h); if (!rc) std::memcpy(out, h.data(), n * 8); break;
+    case CHAMP_ARR_ORBN: rc = get(c->orbn, n, h); if (!rc) std::memcpy(out, h.data(), n * 8); break;
+    case CHAMP_ARR_NXQUAD: {
+        std::vector<int> hi(W);
+        HIPCHECK(hipMemcpy(hi.data(), c->nxquad, W * sizeof(int), hipMemcpyDeviceToHost));
+        for (size_t w = 0; w < W; w++) out[w] = hi[w];
+        break;
+    }
+    default: return CHAMP_ERR_ARG;
+    }
+    return rc;
+}
+
+extern "C" int champ_gpu_basis_fns(champ_ctx *c, int32_t npt, const double *pos, double *phin)
+{
+    if (!c || npt <= 0 || !pos || !phin) return CHAMP_ERR_ARG;
+    double *dpos = nullptr, *dout = nullptr;
+    HIPCHECK(hipMalloc(&dpos, (size_t)npt * 3 * 8));
+    if (hipMalloc(&dout, (size_t)npt * 5 * c->NB * 8) != hipSuccess) { hipFree(dpos); set_error("basis_fns: alloc"); return CHAMP_ERR_NOMEM; }
+    hipMemcpy(dpos, pos, (size_t)npt * 3 * 8, hipMemcpyHostToDevice);
+    int rc = launch_basis_points(c, npt, dpos, dout);
+    if (!rc) {
+        hipStreamSynchronize(c->stream);
+        hipMemcpy(phin, dout, (size_t)npt * 5 * c->NB * 8, hipMemcpyDeviceToHost);
+    }
+    hipFree(dpos);
+    hipFree(dout);
+    return rc;
+}
+
+static const char *kclass_name[K_NCLASS] = {"move_jastrow_ao", "move_mo_gemm", "move_ratio_commit", "hpsi_jastrow",
+                                            "hpsi_ao", "hpsi_mo_gemm", "hpsi_slater_lu", "hpsi_nonloc",
+                                            "hpsi_eloc", "other"};
+
+extern "C" int champ_gpu_kernel_times(champ_ctx *c, int32_t *n, char *names, double *ms, int64_t *launches)
+{
+    if (!c || !n) return CHAMP_ERR_ARG;
+    if (!names && !ms) {  // toggle / reset: *n > 0 enables timing, 0 disables
+        c->timing = *n > 0;
+        for (int k = 0; k < K_NCLASS; k++) { c->class_ms[k] = 0; c->class_n[k] = 0; }
+        *n = K_NCLASS;
+        return CHAMP_OK;
+    }
+    tflush(c);
+    const int m = std::min<int>(*n, K_NCLASS);
+    for (int k = 0; k < m; k++) {
+        if (names) { std::strncpy(names + 32 * k, kclass_name[k], 31); names[32 * k + 31] = 0; }
+        if (ms) ms[k] = c->class_ms[k];
+        if (launches) launches[k] = c->class_n[k];
+    }
+    *n = m;
+    return CHAMP_OK;
+}

@@ -1,0 +1,775 @@
+// engine_hpsi.hip -- FP64 once-per-sweep kernels: hpsi (hpsi.f90:3-273) over the
+// whole population.  distances + jastrow_factor4, AO values for all
+// electrons, the AO->MO GEMM, batched LU inverse of both Slater matrices,
+// nonloc_pot quadrature and the local-energy assembly.
+#include "engine_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace champ {
+
+#define HB 0.5
+
+struct HArgs {
+    int W, NE, NUP, NM, NC, NB, NO, NQ, nloc, w0, nw, cap, accumulate, pad;
+    double pecent, cdet0;
+    const double *cent;
+    const int *iwctype;
+    SplineTab tab;
+    const double *rmax_ao;
+    const int *ao_cent, *ao_type, *ao_rad, *ao_slm, *iworbd;
+    const JasParams *jas;
+    const EcpParams *ecp;
+    double *xT, *r_en, *rr_en, *fso, *g1, *g2, *vjT, *fsumo, *d2o, *dorb, *slmi, *detiab, *ddx, *d2dx2;
+    double *psido, *psijo, *psi2o, *eold, *ekin, *pe_local, *vpsp;
+    double *aoall, *orball, *wmat;
+    int *pair_count, *pair_off, *pair_total, *pair_e, *pair_c, *nxquad;
+    double *pair_vps, *pair_val;
+    double *xq, *yq, *zq;
+    double *esum_dev;
+    int *status;
+};
+
+static HArgs hargs(champ_ctx *c)
+{
+    HArgs a;
+    a.W = c->W; a.NE = c->NE; a.NUP = c->NUP; a.NM = c->NM; a.NC = c->NC; a.NB = c->NB; a.NO = c->NO; a.NQ = c->NQ;
+    a.nloc = c->nloc; a.w0 = 0; a.nw = c->W; a.cap = c->pair_cap; a.accumulate = 0; a.pad = 0;
+    a.pecent = c->pecent; a.cdet0 = c->cdet0;
+    a.cent = c->cent; a.iwctype = c->iwctype; a.tab = c->tab; a.rmax_ao = c->rmax_ao;
+    a.ao_cent = c->ao_cent; a.ao_type = c->ao_type; a.ao_rad = c->ao_rad; a.ao_slm = c->ao_slm; a.iworbd = c->iworbd;
+    a.jas = c->jas; a.ecp = c->ecp;
+    a.xT = c->xT; a.r_en = c->r_en; a.rr_en = c->rr_en; a.fso = c->fso; a.g1 = c->g1; a.g2 = c->g2; a.vjT = c->vjT;
+    a.fsumo = c->fsumo; a.d2o = c->d2o; a.dorb = c->dorb; a.slmi = c->slmi; a.detiab = c->detiab; a.ddx = c->ddx;
+    a.d2dx2 = c->d2dx2; a.psido = c->psido; a.psijo = c->psijo; a.psi2o = c->psi2o; a.eold = c->eold; a.ekin = c->ekin;
+    a.pe_local = c->pe_local; a.vpsp = c->vpsp; a.aoall = c->aoall; a.orball = c->orball; a.wmat = c->wmat;
+    a.pair_count = c->pair_count; a.pair_off = c->pair_off; a.pair_total = c->pair_total; a.pair_e = c->pair_e;
+    a.pair_c = c->pair_c; a.nxquad = c->nxquad; a.pair_vps = c->pair_vps; a.pair_val = c->pair_val;
+    a.xq = c->xq; a.yq = c->yq; a.zq = c->zq; a.esum_dev = c->esum_dev; a.status = c->status;
+    return a;
+}
+
+template <typename T>
+__device__ T block_sum(T v, T *scratch)
+{
+    const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE, nwv = blockDim.x / WAVE;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) scratch[wv] = v;
+    __syncthreads();
+    T s = 0;
+    if (threadIdx.x == 0) for (int i = 0; i < nwv; i++) s += scratch[i];
+    __syncthreads();
+    return s;  // valid on thread 0
+}
+
+// ------------------------------------------------------------------------
+// distances(0) + jastrow_factor4 (jastrow4.f90:3-415, iforce_analy = 0),
+// one 256-thread block per walker.
+__global__ void __launch_bounds__(256) k_hpsi_jastrow(HArgs a)
+{
+    __shared__ double red[16];
+    const int w = blockIdx.x, tid = threadIdx.x, bd = blockDim.x;
+    const int NE = a.NE, NC = a.NC;
+    const JasParams &jp = *a.jas;
+    const double kap = jp.kap, asymp_r = jp.asymp_r;
+    const double *xw = a.xT + (size_t)w * 3 * NE;
+    double *rw = a.r_en + (size_t)w * NC * NE, *rrw = a.rr_en + (size_t)w * NC * NE;
+    double *F = a.fso + (size_t)w * NE * NE;
+    double *G1 = a.g1 + (size_t)w * NE * 3 * NE, *G2 = a.g2 + (size_t)w * NE * 3 * NE;
+
+    for (int t = tid; t < NC * NE; t += bd) {
+        const int c = t / NE, e = t % NE;
+        const double rv0 = xw[e] - a.cent[3 * c], rv1 = xw[NE + e] - a.cent[3 * c + 1], rv2 = xw[2 * NE + e] - a.cent[3 * c + 2];
+        double r2 = 0;
+        r2 = r2 + rv0 * rv0; r2 = r2 + rv1 * rv1; r2 = r2 + rv2 * rv2;
+        const double r = sqrt(r2);
+        rw[t] = r;
+        rrw[t] = scale_r(kap, r);
+    }
+    __syncthreads();
+
+    double fsp = 0, d2p = 0;
+    // e-n terms (jastrow4.f90:300-370)
+    for (int e = tid; e < NE; e += bd) {
+        double fs = 0, g[3] = {0, 0, 0}, d2 = 0;
+        for (int c = 0; c < NC; c++) {
+            const double r = rw[(size_t)c * NE + e], rr = rrw[(size_t)c * NE + e];
+            if (rr == asymp_r) continue;
+            const double dd7 = 1.0 - kap * rr, dd9 = -kap * dd7;
+            const int it = a.iwctype[c];
+            const double *a4 = jp.a4[it];
+            double top = a4[0] * rr, topi = a4[0], topii = 0;
+            double bot = 1 + a4[1] * rr, boti = a4[1], botii = 0, bot2 = bot * bot;
+            double fen = top / bot - jp.ja[it];
+            double feni = topi / bot - boti * top / bot2;
+            double fenii = topii - (botii * top + 2 * boti * topi) / bot + 2 * boti * boti * top / bot2;
+            fenii = fenii / bot;
+            double rp1 = rr, rp2 = 1.0;  // rri(iord-1), rri(iord-2)
+            for (int iord = 2; iord <= jp.norda; iord++) {
+                const double rp = powi(rr, iord);
+                fen = fen + a4[iord] * rp;
+                feni = feni + a4[iord] * iord * rp1;
+                fenii = fenii + a4[iord] * iord * (iord - 1) * rp2;
+                rp2 = rp1; rp1 = rp;
+            }
+            fenii = fenii * dd7 * dd7 + feni * dd9;
+            feni = feni * dd7 / r;
+            fs = fs + fen;
+            g[0] += feni * (xw[e] - a.cent[3 * c]);
+            g[1] += feni * (xw[NE + e] - a.cent[3 * c + 1]);
+            g[2] += feni * (xw[2 * NE + e] - a.cent[3 * c + 2]);
+            d2 = d2 + fenii + 2 * feni;
+        }
+        F[(size_t)e * NE + e] = fs;
+        for (int k = 0; k < 3; k++) { G1[((size_t)e * 3 + k) * NE + e] = g[k]; G2[((size_t)e * 3 + k) * NE + e] = g[k]; }
+        fsp += fs; d2p += d2;
+    }
+
+    // e-e and e-e-n pair terms (jastrow4.f90:79-296)
+    const int NP = NE * (NE - 1) / 2, nordc = jp.nordc;
+    for (int p = tid; p < NP; p += bd) {
+        int i = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
+        while (i * (i - 1) / 2 > p) i--;
+        while ((i + 1) * i / 2 <= p) i++;
+        const int j = p - i * (i - 1) / 2;
+        double rv[3] = {xw[i] - xw[j], xw[NE + i] - xw[NE + j], xw[2 * NE + i] - xw[2 * NE + j]};
+        double r2 = 0;
+        r2 = r2 + rv[0] * rv[0]; r2 = r2 + rv[1] * rv[1]; r2 = r2 + rv[2] * rv[2];
+        const double rij = sqrt(r2);
+        double sspinn = 1; int isb = 0, ipar = 0;
+        if (i < a.NUP || j >= a.NUP) {
+            if (jp.nspin2b == 2) isb = 1;
+            else if (jp.nocuspb == 0) sspinn = 0.5;
+            ipar = 1;
+        }
+        double fs = 0, d2 = 0, fI[3] = {0, 0, 0}, fJ[3] = {0, 0, 0};
+        const double u = scale_r(kap, rij);
+        double dd1 = 1.0 - kap * u, dd2 = -kap * dd1;
+        if (u != asymp_r) {
+            const double *b = jp.b[isb];
+            double top = sspinn * b[0] * u, topu = sspinn * b[0], topuu = 0;
+            double bot = 1 + b[1] * u, botu = b[1], botuu = 0, bot2 = bot * bot;
+            double fee = top / bot - jp.jb[ipar];
+            double feeu = topu / bot - botu * top / bot2;
+            double feeuu = topuu - (botuu * top + 2 * botu * topu) / bot + 2 * botu * botu * top / bot2;
+            feeuu = feeuu / bot;
+            double u1 = u, u2 = 1.0;  // uu(iord-1), uu(iord-2)
+            for (int iord = 2; iord <= jp.nordb; iord++) {
+                const double un = u * u1;
+                fee = fee + b[iord] * un;
+                feeu = feeu + b[iord] * iord * u1;
+                feeuu = feeuu + b[iord] * iord * (iord - 1) * u2;
+                u2 = u1; u1 = un;
+            }
+            feeuu = feeuu * dd1 * dd1 + feeu * dd2;
+            feeu = feeu * dd1 / rij;
+            fs += fee;
+            for (int k = 0; k < 3; k++) { fI[k] += feeu * rv[k]; fJ[k] -= feeu * rv[k]; }
+            d2 += 2 * (feeuu + 2 * feeu);
+        }
+        if (nordc > 1) {
+            double uu_[CHAMP_NORDP + 4], ri_[CHAMP_NORDP + 4], rj_[CHAMP_NORDP + 4], ss[CHAMP_NORDP + 1], tt[CHAMP_NORDP + 1];
+            double *uu = uu_ + 3, *rri = ri_ + 3, *rrj = rj_ + 3;
+            uu[-3] = uu[-2] = uu[-1] = 0; rri[-3] = rri[-2] = rri[-1] = 0; rrj[-3] = rrj[-2] = rrj[-1] = 0;
+            uu[0] = 1; rri[0] = 1; rrj[0] = 1; ss[0] = 2; tt[0] = 1;
+            uu[1] = 1 - kap * u;
+            dd1 = -kap * dd1; dd2 = -kap * dd2;
+            for (int iord = 2; iord <= nordc; iord++) uu[iord] = uu[1] * uu[iord - 1];
+            for (int c = 0; c < NC; c++) {
+                const double ri = rw[(size_t)c * NE + i], rj = rw[(size_t)c * NE + j];
+                const double rri0 = rrw[(size_t)c * NE + i], rrj0 = rrw[(size_t)c * NE + j];
+                if (rri0 == asymp_r && rrj0 == asymp_r) continue;
+                const double dd7u = 1.0 - kap * rri0, dd8u = 1.0 - kap * rrj0;
+                const double dd7 = -kap * dd7u, dd9 = -kap * (-kap * dd7u), dd8 = -kap * dd8u, dd10 = -kap * (-kap * dd8u);
+                rri[1] = 1 - kap * rri0; rrj[1] = 1 - kap * rrj0;
+                const double s = ri + rj, t = ri - rj;
+                const double u2pst = rij * rij + s * t, u2mst = rij * rij - s * t;
+                for (int iord = 1; iord <= nordc; iord++) {
+                    rri[iord] = rri[1] * rri[iord - 1];
+                    rrj[iord] = rrj[1] * rrj[iord - 1];
+                    ss[iord] = rri[iord] + rrj[iord];
+                    tt[iord] = rri[iord] * rrj[iord];
+                }
+                const Een<double, true> e = een_eval<double, true>(jp, jp.c[a.iwctype[c]], uu, rri, rrj, ss, tt);
+                const double fuu = e.fuu * dd1 * dd1 + e.fu * dd2;
+                const double fu = e.fu * dd1 / rij;
+                const double fui = e.fui * dd1 * dd7, fuj = e.fuj * dd1 * dd8;
+                const double fii = e.fii * dd7 * dd7 + e.fi * dd9, fjj = e.fjj * dd8 * dd8 + e.fj * dd10;
+                const double fi = e.fi * dd7 / ri, fj = e.fj * dd8 / rj;
+                fs = fs + e.fc;
+                for (int k = 0; k < 3; k++) {
+                    const double rvi = xw[k * NE + i] - a.cent[3 * c + k], rvj = xw[k * NE + j] - a.cent[3 * c + k];
+                    fI[k] = fI[k] + fi * rvi + fu * rv[k];
+                    fJ[k] = fJ[k] + fj * rvj - fu * rv[k];
+                }
+                d2 = d2 + 2 * (fuu + 2 * fu) + fui * u2pst / (ri * rij) + fuj * u2mst / (rj * rij) + fii + 2 * fi + fjj + 2 * fj;
+            }
+        }
+        F[(size_t)i * NE + j] = fs;
+        F[(size_t)j * NE + i] = fs;
+        for (int k = 0; k < 3; k++) {
+            G1[((size_t)i * 3 + k) * NE + j] = fI[k];
+            G2[((size_t)i * 3 + k) * NE + j] = fJ[k];
+            G1[((size_t)j * 3 + k) * NE + i] = fJ[k];
+            G2[((size_t)j * 3 + k) * NE + i] = fI[k];
+        }
+        fsp += fs; d2p += d2;
+    }
+    __syncthreads();
+    double *vj = a.vjT + (size_t)w * 3 * NE;
+    for (int t = tid; t < 3 * NE; t += bd) {
+        const int e = t % NE, k = t / NE;
+        const double *row = G1 + ((size_t)e * 3 + k) * NE;
+        double s = 0;
+        for (int j = 0; j < NE; j++) s += row[j];
+        vj[(size_t)k * NE + e] = s;
+    }
+    const double fsum = block_sum(fsp, red);
+    const double d2sum = block_sum(d2p, red);
+    if (tid == 0) { a.fsumo[w] = fsum; a.d2o[w] = d2sum; }
+}
+
+// ------------------------------------------------------------------------
+// AO values, gradients and Laplacians of every electron (basis_fns with
+// ider=2 inside orbitals, orbitals_no_qmckl.f90:46): one wavefront per
+// (walker, electron), lanes over AOs.  Chunk of walkers [w0, w0+nw).
+__global__ void __launch_bounds__(256) k_hpsi_ao(HArgs a)
+{
+    const int lane = threadIdx.x % WAVE;
+    const long long gw = (long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (gw >= (long long)a.nw * a.NE) return;
+    const int wc = (int)(gw / a.NE), e = (int)(gw % a.NE), w = a.w0 + wc;
+    const int NE = a.NE;
+    const double *xw = a.xT + (size_t)w * 3 * NE;
+    const double *rw = a.r_en + (size_t)w * a.NC * NE;
+    double *out = a.aoall + (size_t)gw * 5 * a.NB;
+    const double x0 = xw[e], x1 = xw[NE + e], x2 = xw[2 * NE + e];
+    for (int m = lane; m < a.NB; m += WAVE) {
+        const int c = a.ao_cent[m];
+        double o[5];
+        ao_eval<double>(a.tab, a.ao_type[m], a.ao_rad[m], a.ao_slm[m], x0 - a.cent[3 * c], x1 - a.cent[3 * c + 1],
+                        x2 - a.cent[3 * c + 2], rw[(size_t)c * NE + e], a.rmax_ao[m], 2, o);
+        for (int k = 0; k < 5; k++) out[(size_t)k * a.NB + m] = o[k];
+    }
+}
+
+// ------------------------------------------------------------------------
+// determinant (determinant.f90:9-140): per (walker, spin) block, Slater
+// matrix in LDS, in-place Gauss-Jordan inverse with partial (row) pivoting,
+// determinant from the pivots with matinv's mantissa/exponent loop
+// (matinv.f90:65-84), then ddx / d2dx2 and the dorb rows for the moves.
+__global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
+{
+    HIP_DYNAMIC_SHARED(double, L)
+    __shared__ int perm[512];
+    __shared__ double piv[512];
+    __shared__ int s_p;
+    __shared__ int s_bad;
+    const int wc = blockIdx.x / 2, iab = blockIdx.x % 2, w = a.w0 + wc;
+    const int tid = threadIdx.x, bd = blockDim.x, lane = tid % WAVE, wv = tid / WAVE;
+    const int NE = a.NE, NO = a.NO;
+    const int nel = iab ? NE - a.NUP : a.NUP, ish = iab ? a.NUP : 0;
+    const int ld = nel + 1;
+    double *col = L + (size_t)nel * ld;
+    const double *ob = a.orball + (size_t)wc * NE * 5 * NO;
+    if (tid == 0) s_bad = 0;
+    for (int t = tid; t < nel * nel; t += bd) {
+        const int i = t / nel, j = t % nel;
+        L[i * ld + j] = ob[((size_t)(ish + i) * 5) * NO + a.iworbd[ish + j]];
+    }
+    __syncthreads();
+    for (int k = 0; k < nel; k++) {
+        if (wv == 0) {
+            double best = -1.0; int bi = k;
+            for (int i = k + lane; i < nel; i += WAVE) {
+                const double v = fabs(L[i * ld + k]);
+                if (v > best) { best = v; bi = i; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob2 = __shfl_xor(best, off, WAVE);
+                const int oi = __shfl_xor(bi, off, WAVE);
+                if (ob2 > best || (ob2 == best && oi < bi)) { best = ob2; bi = oi; }
+            }
+            if (lane == 0) { s_p = bi; perm[k] = bi; }
+        }
+        __syncthreads();
+        const int p = s_p;
+        if (p != k)
+            for (int j = tid; j < nel; j += bd) { const double t = L[k * ld + j]; L[k * ld + j] = L[p * ld + j]; L[p * ld + j] = t; }
+        __syncthreads();
+        const double pv = L[k * ld + k];
+        if (tid == 0) { piv[k] = pv; if (pv == 0.0) s_bad = 1; }
+        const double pinv = 1.0 / pv;
+        __syncthreads();
+        for (int j = tid; j < nel; j += bd) L[k * ld + j] = (j == k) ? pinv : L[k * ld + j] * pinv;
+        for (int i = tid; i < nel; i += bd) col[i] = L[i * ld + k];
+        __syncthreads();
+        for (int t = tid; t < nel * nel; t += bd) {
+            const int i = t / nel, j = t % nel;
+            if (i == k) continue;
+            const double base = (j == k) ? 0.0 : L[i * ld + j];
+            L[i * ld + j] = base - L[k * ld + j] * col[i];
+        }
+        __syncthreads();
+    }
+    for (int k = nel - 1; k >= 0; k--) {
+        const int p = perm[k];
+        if (p != k)
+            for (int i = tid; i < nel; i += bd) { const double t = L[i * ld + k]; L[i * ld + k] = L[i * ld + p]; L[i * ld + p] = t; }
+        __syncthreads();
+    }
+    // slmi(j + (i-1) n) = A^-1(j, i): row i of slmi holds column i of the inverse
+    double *S = a.slmi + ((size_t)w * 2 + iab) * a.NM * a.NM;
+    for (int t = tid; t < nel * nel; t += bd) {
+        const int i = t / nel, j = t % nel;
+        S[(size_t)i * a.NM + j] = L[j * ld + i];
+    }
+    if (tid == 0) {
+        // mantissa/exponent loops only on finite values: a NaN or Inf pivot
+        // would never leave them (it marks the walker instead)
+        double det1 = 1.0, det2 = 0.0;
+        int nonfinite = 0;
+        for (int i = 0; i < nel; i++) {
+            if (perm[i] != i) det1 = -det1;
+            det1 = piv[i] * det1;
+            if (det1 == 0.0) break;
+            if (!isfinite(det1)) { nonfinite = 1; break; }
+            for (int it = 0; it < 800 && fabs(det1) < 1.0; it++) { det1 = 10.0 * det1; det2 = det2 - 1.0; }
+            for (int it = 0; it < 800 && fabs(det1) >= 10.0; it++) { det1 = det1 / 10.0; det2 = det2 + 1.0; }
+        }
+        a.detiab[2 * w + iab] = nonfinite ? det1 : det1 * pow(10.0, det2);
+        if (s_bad) atomicOr(&a.status[w], CHAMP_WALKER_SINGULAR);
+        if (nonfinite) atomicOr(&a.status[w], CHAMP_WALKER_NONFINITE);
+    }
+    // ddx, d2dx2 (determinant.f90:81-88)
+    for (int i = tid; i < nel; i += bd) {
+        const double *orbi = ob + (size_t)(ish + i) * 5 * NO;
+        double d0 = 0, d1 = 0, d2 = 0, dd = 0;
+        for (int j = 0; j < nel; j++) {
+            const double s = L[j * ld + i];
+            const int jo = a.iworbd[ish + j];
+            d0 = d0 + s * orbi[NO + jo];
+            d1 = d1 + s * orbi[2 * NO + jo];
+            d2 = d2 + s * orbi[3 * NO + jo];
+            dd = dd + s * orbi[4 * NO + jo];
+        }
+        double *dx = a.ddx + ((size_t)w * NE + ish + i) * 3;
+        dx[0] = d0; dx[1] = d1; dx[2] = d2;
+        a.d2dx2[(size_t)w * NE + ish + i] = dd;
+    }
+    // orbital gradients of this spin's electrons for the moves (orbval dorb)
+    for (int t = tid; t < nel * 3 * NO; t += bd) {
+        const int i = t / (3 * NO), r = t % (3 * NO);
+        a.dorb[((size_t)w * NE + ish + i) * 3 * NO + r] = ob[(size_t)(ish + i) * 5 * NO + NO + r];
+    }
+}
+
+// ------------------------------------------------------------------------
+// gauss_pot (readps_gauss.f90:57-92), l is the 1-based component
+__device__ double gauss_pot(const EcpParams &ep, int ict, int l, double r)
+{
+    double v = 0, rsq = r * r;
+    const int nt = ep.nterm[ict][l - 1];
+    for (int i = 0; i < nt; i++) {
+        const int pw = ep.power[ict][l - 1][i];
+        const double ex = ep.expo[ict][l - 1][i];
+        double p = (pw != 2) ? powi(r, pw - 2) : 1.0;
+        double e = (ex * rsq > 7.0e2) ? 0.0 : ep.coef[ict][l - 1][i] * exp(-ex * rsq);
+        v = v + p * e;
+    }
+    return v;
+}
+
+// getvps_gauss + local part of nonloc_pot + active (electron, centre) list
+// in the order of nonloc.f90:126-186.  One block per walker, threads over
+// electrons, deterministic order by a block-wide scan.
+__global__ void __launch_bounds__(256) k_hpsi_vps(HArgs a)
+{
+    __shared__ int cnt[1024];
+    __shared__ double red[16];
+    const int w = blockIdx.x, tid = threadIdx.x, bd = blockDim.x;
+    const int NE = a.NE, NC = a.NC;
+    const EcpParams &ep = *a.ecp;
+    const double *rw = a.r_en + (size_t)w * NC * NE;
+    double pe = 0;
+    for (int e = tid; e < 1024; e += bd) cnt[e] = 0;
+    __syncthreads();
+    for (int e = tid; e < NE; e += bd) {
+        int n = 0;
+        for (int c = 0; c < NC; c++) {
+            const int ict = a.iwctype[c], lp = ep.lpot[ict];
+            double r = rw[(size_t)c * NE + e];
+            r = r > 1.0e-10 ? r : 1.0e-10;
+            pe += gauss_pot(ep, ict, lp, r) - ep.znuc[ict] * (1.0 / r);
+            for (int l = 1; l <= lp - 1; l++)
+                if (fabs(gauss_pot(ep, ict, l, r)) > 1.0e-4) { n++; break; }
+        }
+        cnt[e] = n;
+    }
+    __syncthreads();
+    if (tid == 0) {  // exclusive scan (NE <= 1024)
+        int s = 0;
+        for (int e = 0; e < NE; e++) { const int v = cnt[e]; cnt[e] = s; s += v; }
+        a.pair_count[w] = s;
+        a.nxquad[w] = s * a.NQ;
+    }
+    __syncthreads();
+    for (int e = tid; e < NE; e += bd) {
+        int off = cnt[e];
+        for (int c = 0; c < NC; c++) {
+            const int ict = a.iwctype[c], lp = ep.lpot[ict];
+            double r = rw[(size_t)c * NE + e];
+            r = r > 1.0e-10 ? r : 1.0e-10;
+            double v[CHAMP_MPS_L] = {0, 0, 0, 0, 0};
+            int act = 0;
+            for (int l = 1; l <= lp - 1; l++) {
+                v[l - 1] = gauss_pot(ep, ict, l, r);
+                if (fabs(v[l - 1]) > 1.0e-4) act = 1;
+            }
+            if (!act) continue;
+            const size_t slot = (size_t)w * a.cap + off;
+            a.pair_e[slot] = e;
+            a.pair_c[slot] = c;
+            for (int l = 0; l < CHAMP_MPS_L; l++) a.pair_vps[slot * CHAMP_MPS_L + l] = v[l];
+            off++;
+        }
+    }
+    const double pes = block_sum(pe, red);
+    if (tid == 0) a.pe_local[w] = pes;
+}
+
+__global__ void k_pair_scan(int W, const int *count, int *off, int *total)
+{
+    // single block: exclusive scan over walkers, reset the work counter
+    __shared__ int part[1024];
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const int per = (W + bd - 1) / bd;
+    int s = 0;
+    for (int i = 0; i < per; i++) { const int w = tid * per + i; if (w < W) s += count[w]; }
+    part[tid] = s;
+    __syncthreads();
+    if (tid == 0) { int acc = 0; for (int t = 0; t < bd; t++) { const int v = part[t]; part[t] = acc; acc += v; } total[0] = acc; total[1] = 0; }
+    __syncthreads();
+    int acc = part[tid];
+    for (int i = 0; i < per; i++) { const int w = tid * per + i; if (w < W) { off[w] = acc; acc += count[w]; } }
+    if (tid == 0) off[W] = total[0];
+}
+
+// yl0 (nonloc.f90:317-340)
+__device__ __forceinline__ double yl0(int l, double c)
+{
+    if (l == 1) return 1.0;
+    if (l == 2) return 3.0 * c;
+    if (l == 3) return 2.5 * (3 * c * c - 1);
+    if (l == 4) return 3.5 * (5 * c * c * c - 3 * c);
+    return 0.0;
+}
+
+// psinl (nonlpsi.f90:4-86, ijas=4): e-e-n value with switched scalings
+__device__ __forceinline__ double psinl(const JasParams &jp, const double *c, double u, double rri, double rrj)
+{
+    const double kap = jp.kap;
+    if (rri == jp.asymp_r && rrj == jp.asymp_r) return 0.0;
+    double uu[CHAMP_NORDP + 1], ri[CHAMP_NORDP + 1], rj[CHAMP_NORDP + 1];
+    uu[0] = 1; ri[0] = 1; rj[0] = 1;
+    uu[1] = 1 - kap * u; ri[1] = 1 - kap * rri; rj[1] = 1 - kap * rrj;
+    for (int p = 2; p <= jp.nordc; p++) { uu[p] = uu[1] * uu[p - 1]; ri[p] = ri[1] * ri[p - 1]; rj[p] = rj[1] * rj[p - 1]; }
+    double psi = 0.0;
+    for (int ll = 0; ll < jp.nterms; ll++) {
+        const int k = jp.tk[ll], l = jp.tl[ll], m = jp.tm[ll];
+        const double ssl = (l == 0) ? 2.0 : ri[l] + rj[l];
+        const double ttm = (m == 0) ? 1.0 : ri[m] * rj[m];
+        psi = psi + c[ll] * uu[k] * ssl * ttm;
+    }
+    return psi;
+}
+
+// Non-local quadrature for the active (electron, centre) pairs: work queue
+// over the flattened pair list of the whole population.  Per pair and
+// point: orbitals_quad + nonlocd_quad (ratio = phi(x_q) . (coef^T slmi_i)),
+// nonlocj_quad4 (jastrow log ratio) and the vpsp_det term (nonloc.f90:248-274).
+__global__ void __launch_bounds__(256) k_hpsi_nonloc(HArgs a)
+{
+    const int lane = threadIdx.x % WAVE;
+    const int NE = a.NE, NC = a.NC, NQ = a.NQ;
+    const JasParams &jp = *a.jas;
+    const EcpParams &ep = *a.ecp;
+    const double kap = jp.kap;
+    const int total = a.pair_total[0];
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(&a.pair_total[1], 1);
+        t = __shfl(t, 0, WAVE);
+        if (t >= total) break;
+        int lo = 0, hi = a.W - 1;
+        while (lo < hi) {  // largest w with off[w] <= t
+            const int mid = (lo + hi + 1) / 2;
+            if (a.pair_off[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        const int w = lo, p = t - a.pair_off[w];
+        const size_t slot = (size_t)w * a.cap + p;
+        const int i = a.pair_e[slot], ic = a.pair_c[slot];
+        const int ict = a.iwctype[ic], lp = ep.lpot[ict];
+        const double *xw = a.xT + (size_t)w * 3 * NE;
+        const double *rw = a.r_en + (size_t)w * NC * NE, *rrw = a.rr_en + (size_t)w * NC * NE;
+        const double *F = a.fso + (size_t)w * NE * NE;
+        const double *Wm = a.wmat + ((size_t)w * NE + i) * a.NB;
+        const double ren = rw[(size_t)ic * NE + i], ri = 1.0 / ren;
+        const double rv0 = xw[i] - a.cent[3 * ic], rv1 = xw[NE + i] - a.cent[3 * ic + 1], rv2 = xw[2 * NE + i] - a.cent[3 * ic + 2];
+        const int iup = i < a.NUP;
+        double accum = 0.0;
+        for (int iq = 0; iq < NQ; iq++) {
+            const double xqv = a.xq[(size_t)w * NQ + iq], yqv = a.yq[(size_t)w * NQ + iq], zqv = a.zq[(size_t)w * NQ + iq];
+            double costh = rv0 * xqv + rv1 * yqv + rv2 * zqv;
+            costh = costh * ri;
+            const double q0 = ren * xqv + a.cent[3 * ic], q1 = ren * yqv + a.cent[3 * ic + 1], q2 = ren * zqv + a.cent[3 * ic + 2];
+            // distances of the point to every centre: lanes own centres c and c+64
+            double rq_r[2] = {0, 0}, rr_r[2] = {0, 0}, vx_r[2] = {0, 0}, vy_r[2] = {0, 0}, vz_r[2] = {0, 0};
+            for (int h = 0; h < 2; h++) {
+                const int c = lane + h * WAVE;
+                if (c < NC) {
+                    const double d0 = q0 - a.cent[3 * c], d1 = q1 - a.cent[3 * c + 1], d2 = q2 - a.cent[3 * c + 2];
+                    double r2 = 0;
+                    r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
+                    double r = sqrt(r2);
+                    if (c == ic) r = ren;
+                    rq_r[h] = r; rr_r[h] = scale_r(kap, r); vx_r[h] = d0; vy_r[h] = d1; vz_r[h] = d2;
+                }
+            }
+            // determinant ratio through the contracted inverse row
+            double dr = 0.0;
+            for (int mb = 0; mb < a.NB; mb += WAVE) {  // uniform trip count: shuffles need every lane
+                const int m = mb + lane < a.NB ? mb + lane : a.NB - 1;
+                const int c = a.ao_cent[m], h = c / WAVE, src = c % WAVE;
+                // both register halves are shuffled; the reader picks the half of its centre
+                const double rx0 = __shfl(vx_r[0], src, WAVE), rx1 = __shfl(vx_r[1], src, WAVE);
+                const double ry0 = __shfl(vy_r[0], src, WAVE), ry1 = __shfl(vy_r[1], src, WAVE);
+                const double rz0 = __shfl(vz_r[0], src, WAVE), rz1 = __shfl(vz_r[1], src, WAVE);
+                const double rq0 = __shfl(rq_r[0], src, WAVE), rq1 = __shfl(rq_r[1], src, WAVE);
+                const double rx = h ? rx1 : rx0, ry = h ? ry1 : ry0, rz = h ? rz1 : rz0, rr = h ? rq1 : rq0;
+                double o[5];
+                ao_eval<double>(a.tab, a.ao_type[m], a.ao_rad[m], a.ao_slm[m], rx, ry, rz, rr, a.rmax_ao[m], 0, o);
+                if (mb + lane < a.NB) dr += o[0] * Wm[m];
+            }
+            dr = wave_sum(dr);
+            // jastrow log ratio (nonlocj_quad4, nonloc.f90:732-806)
+            double fj = 0.0;
+            for (int jb = 0; jb < NE; jb += WAVE) {
+                // every lane runs the centre loop (uniform shuffles); lanes past NE discard
+                const int jj = jb + lane;
+                const bool on = jj < NE && jj != i;
+                const int jq = jj < NE ? jj : 0;
+                const int I = jq > i ? jq : i, J = jq > i ? i : jq;
+                double sspinn = 1; int isb = 0, ipar = 0;
+                if (I < a.NUP || J >= a.NUP) {
+                    if (jp.nspin2b == 2) isb = 1;
+                    else if (jp.nocuspb == 0) sspinn = 0.5;
+                    ipar = 1;
+                }
+                const double dx0 = xw[jq] - q0, dx1 = xw[NE + jq] - q1, dx2 = xw[2 * NE + jq] - q2;
+                double r2 = 0;
+                r2 = r2 + dx0 * dx0; r2 = r2 + dx1 * dx1; r2 = r2 + dx2 * dx2;
+                const double rij = sqrt(r2);
+                const double u = scale_r(kap, rij);
+                double fsn = 0.0;
+                if (u != jp.asymp_r) {
+                    const double *b = jp.b[isb];
+                    const double fee = b[0] * u / (1 + b[1] * u);
+                    fsn = sspinn * fee - jp.jb[ipar];
+                    double up = u;
+                    for (int o = 2; o <= jp.nordb; o++) { up = u * up; fsn = fsn + b[o] * up; }
+                }
+                if (jp.nordc > 1)
+                    for (int c = 0; c < NC; c++) {
+                        const double rrqc = c < WAVE ? __shfl(rr_r[0], c, WAVE) : __shfl(rr_r[1], c - WAVE, WAVE);
+                        fsn = fsn + psinl(jp, jp.c[a.iwctype[c]], u, rrqc, rrw[(size_t)c * NE + jq]);
+                    }
+                if (on) fj += fsn - F[(size_t)I * NE + J];
+            }
+            double fd = 0.0;
+            for (int h = 0; h < 2; h++) {
+                const int c = lane + h * WAVE;
+                if (c < NC) {
+                    const double rr = rr_r[h];
+                    if (rr != jp.asymp_r) {
+                        const int it = a.iwctype[c];
+                        double psi = jp.a4[it][0] * rr / (1.0 + jp.a4[it][1] * rr) - jp.ja[it];
+                        double rrip = rr;
+                        for (int o = 2; o <= jp.norda; o++) { rrip = rrip * rr; psi = psi + jp.a4[it][o] * rrip; }
+                        fd += psi;
+                    }
+                }
+            }
+            fj = wave_sum(fj);
+            fd = wave_sum(fd);
+            const double fsumn = fj + fd - F[(size_t)i * NE + i];
+            double term_radial = 0.0;
+            for (int l = 1; l <= lp - 1; l++) term_radial = term_radial + yl0(l, costh) * a.pair_vps[slot * CHAMP_MPS_L + l - 1];
+            const double trj = term_radial * ep.wq[iq] * exp(fsumn);
+            accum = accum + trj * dr;
+        }
+        if (lane == 0) a.pair_val[slot] = accum;
+        (void)iup;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Local energy assembly, pot_local and psi (hpsi.f90:177-246,
+// pot_local.f90:3-78, determinant_psit.f90:3-26).  One block per walker.
+__global__ void __launch_bounds__(256) k_hpsi_eloc(HArgs a)
+{
+    __shared__ double red[16];
+    const int w = blockIdx.x, tid = threadIdx.x, bd = blockDim.x;
+    const int NE = a.NE, NC = a.NC;
+    const double *xw = a.xT + (size_t)w * 3 * NE;
+    const double *rw = a.r_en + (size_t)w * NC * NE;
+    double pe = 0;
+    if (a.nloc == 0)
+        for (int t = tid; t < NC * NE; t += bd) pe = pe + (-a.ecp->znuc[a.iwctype[t / NE]] / rw[t]);
+    const int NP = NE * (NE - 1) / 2;
+    for (int p = tid; p < NP; p += bd) {
+        int i = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
+        while (i * (i - 1) / 2 > p) i--;
+        while ((i + 1) * i / 2 <= p) i++;
+        const int j = p - i * (i - 1) / 2;
+        const double d0 = xw[i] - xw[j], d1 = xw[NE + i] - xw[NE + j], d2 = xw[2 * NE + i] - xw[2 * NE + j];
+        double r2 = 0;
+        r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
+        pe = pe + 1 / sqrt(r2);
+    }
+    const double *vj = a.vjT + (size_t)w * 3 * NE;
+    double kin[2] = {0, 0}, vv = 0;
+    for (int e = tid; e < NE; e += bd) {
+        const double *d = a.ddx + ((size_t)w * NE + e) * 3;
+        const double v0 = vj[e], v1 = vj[NE + e], v2 = vj[2 * NE + e];
+        const double tmp = -HB * (a.d2dx2[(size_t)w * NE + e] + 2.0 * (v0 * d[0] + v1 * d[1] + v2 * d[2]));
+        kin[e < a.NUP ? 0 : 1] += tmp;
+        vv = vv + (-HB * (v0 * v0 + v1 * v1 + v2 * v2));
+    }
+    double vp[2] = {0, 0};
+    if (a.nloc > 0) {
+        const int np = a.pair_count[w];
+        for (int p = tid; p < np; p += bd) {
+            const size_t slot = (size_t)w * a.cap + p;
+            vp[a.pair_e[slot] < a.NUP ? 0 : 1] += a.pair_val[slot];
+        }
+    }
+    const double pes = block_sum(pe, red);
+    const double k0 = block_sum(kin[0], red), k1 = block_sum(kin[1], red);
+    const double vvs = block_sum(vv, red);
+    const double vp0 = block_sum(vp[0], red), vp1 = block_sum(vp[1], red);
+    if (tid == 0) {
+        const double pe_local = a.pecent + pes + (a.nloc > 0 ? a.pe_local[w] : 0.0);
+        const double ekin_other = -HB * a.d2o[w] + vvs;
+        const double e_other = ekin_other + pe_local;
+        const double eloc0 = k0 + vp0, eloc1 = k1 + vp1;
+        const double energy = 0.0 + eloc0 + eloc1 + e_other;
+        const double ek = 0.0 + k0 + k1 + ekin_other;
+        const double psid = 0.0 + a.detiab[2 * w] * a.detiab[2 * w + 1] * a.cdet0;
+        const double psij = a.fsumo[w];
+        a.psido[w] = psid; a.psijo[w] = psij; a.eold[w] = energy; a.ekin[w] = ek;
+        a.psi2o[w] = 2 * (log(fabs(psid)) + psij);
+        a.pe_local[w] = pe_local;
+        a.vpsp[2 * w] = vp0; a.vpsp[2 * w + 1] = vp1;
+        if (!isfinite(energy) || !isfinite(psij) || psid == 0.0) atomicOr(&a.status[w], CHAMP_WALKER_NONFINITE);
+        if (a.accumulate) {
+            atomicAdd(&a.esum_dev[1], energy);
+            atomicAdd(&a.esum_dev[2], energy * energy);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+int launch_hpsi_impl(champ_ctx *c, int accumulate)
+{
+    HArgs a = hargs(c);
+    a.accumulate = accumulate;
+    int slot;
+    tbegin(c, K_HPSI_JAS, &slot);
+    hipLaunchKernelGGL(k_hpsi_jastrow, dim3(c->W), dim3(256), 0, c->stream, a);
+    tend(c, slot);
+    HIPCHECK(hipGetLastError());
+    const double one = 1.0, zero = 0.0;
+    for (int w0 = 0; w0 < c->W; w0 += c->hpsi_chunk) {
+        a.w0 = w0;
+        a.nw = std::min(c->hpsi_chunk, c->W - w0);
+        const long long nwave = (long long)a.nw * c->NE;
+        tbegin(c, K_HPSI_AO, &slot);
+        hipLaunchKernelGGL(k_hpsi_ao, dim3((unsigned)((nwave + 3) / 4)), dim3(256), 0, c->stream, a);
+        tend(c, slot);
+        HIPCHECK(hipGetLastError());
+        tbegin(c, K_HPSI_GEMM, &slot);
+        rocblas_status st = rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->NO,
+                                          (int)(nwave * 5), c->NB, &one, c->coef, c->NB, c->aoall, c->NB, &zero,
+                                          c->orball, c->NO);
+        tend(c, slot);
+        if (st != rocblas_status_success) { set_error("rocblas_dgemm (orbitals) failed"); return CHAMP_ERR_HIP; }
+        const int nmx = std::max(c->NUP, c->NDN);
+        const size_t lds = (size_t)(nmx * (nmx + 1) + nmx) * sizeof(double);
+        if (lds > 64 * 1024) HIPCHECK(hipFuncSetAttribute((const void *)k_hpsi_slater, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        tbegin(c, K_HPSI_SLATER, &slot);
+        hipLaunchKernelGGL(k_hpsi_slater, dim3(2 * a.nw), dim3(256), lds, c->stream, a);
+        tend(c, slot);
+        HIPCHECK(hipGetLastError());
+    }
+    a.w0 = 0; a.nw = c->W;
+    if (c->nloc > 0) {
+        tbegin(c, K_HPSI_NONLOC, &slot);
+        hipLaunchKernelGGL(k_hpsi_vps, dim3(c->W), dim3(256), 0, c->stream, a);
+        HIPCHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_pair_scan, dim3(1), dim3(1024), 0, c->stream, c->W, c->pair_count, c->pair_off, c->pair_total);
+        HIPCHECK(hipGetLastError());
+        // wmat[w][i][m] = sum_j slmi[w][spin][i][j] coef[iworbd(j)][m], per spin
+        for (int iab = 0; iab < 2; iab++) {
+            const int nel = iab ? c->NDN : c->NUP, ish = iab ? c->NUP : 0;
+            if (nel == 0) continue;
+            rocblas_status st = rocblas_dgemm_strided_batched(
+                c->blas, rocblas_operation_none, rocblas_operation_none, c->NB, nel, nel, &one, c->cocc[iab], c->NB, 0,
+                c->slmi + (size_t)iab * c->NM * c->NM, c->NM, (rocblas_stride)2 * c->NM * c->NM, &zero,
+                c->wmat + (size_t)ish * c->NB, c->NB, (rocblas_stride)c->NE * c->NB, c->W);
+            if (st != rocblas_status_success) { set_error("rocblas_dgemm_strided_batched (wmat) failed"); return CHAMP_ERR_HIP; }
+        }
+        int dev = 0, ncu = 256;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        hipLaunchKernelGGL(k_hpsi_nonloc, dim3(ncu * 8), dim3(256), 0, c->stream, a);
+        tend(c, slot);
+        HIPCHECK(hipGetLastError());
+    }
+    tbegin(c, K_HPSI_ELOC, &slot);
+    hipLaunchKernelGGL(k_hpsi_eloc, dim3(c->W), dim3(256), 0, c->stream, a);
+    tend(c, slot);
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
+int launch_hpsi(champ_ctx *c) { return launch_hpsi_impl(c, 0); }
+
+// basis_fns at arbitrary points (checking entry point): one wave per point.
+__global__ void k_basis_points(int npt, const double *pos, HArgs a, double *out)
+{
+    const int lane = threadIdx.x % WAVE;
+    const int p = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (p >= npt) return;
+    for (int m = lane; m < a.NB; m += WAVE) {
+        const int c = a.ao_cent[m];
+        const double d0 = pos[3 * p] - a.cent[3 * c], d1 = pos[3 * p + 1] - a.cent[3 * c + 1], d2 = pos[3 * p + 2] - a.cent[3 * c + 2];
+        double r2 = 0;
+        r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
+        double o[5];
+        ao_eval<double>(a.tab, a.ao_type[m], a.ao_rad[m], a.ao_slm[m], d0, d1, d2, sqrt(r2), a.rmax_ao[m], 2, o);
+        for (int k = 0; k < 5; k++) out[((size_t)p * 5 + k) * a.NB + m] = o[k];
+    }
+}
+
+int launch_basis_points(champ_ctx *c, int npt, const double *dpos, double *dout)
+{
+    HArgs a = hargs(c);
+    hipLaunchKernelGGL(k_basis_points, dim3((npt + 3) / 4), dim3(256), 0, c->stream, npt, dpos, a, dout);
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
+}  // namespace champ

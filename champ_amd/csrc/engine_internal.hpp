@@ -1,0 +1,134 @@
+// engine_internal.hpp -- the engine context (device tables + walker state)
+// shared by the kernel translation units and the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/champ_gpu.h"
+#include "device_common.hpp"
+
+namespace champ {
+
+struct EcpParams {
+    int lpot[CHAMP_MAXCT];
+    int nterm[CHAMP_MAXCT][CHAMP_MPS_L];
+    int power[CHAMP_MAXCT][CHAMP_MPS_L][CHAMP_MGAUSS];
+    double coef[CHAMP_MAXCT][CHAMP_MPS_L][CHAMP_MGAUSS];
+    double expo[CHAMP_MAXCT][CHAMP_MPS_L][CHAMP_MGAUSS];
+    double znuc[CHAMP_MAXCT];
+    double wq[CHAMP_MPS_QUAD];
+    int nquad, nloc;
+};
+
+// Per-kernel-class timing with HIP events on the context stream.
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+enum KClass { K_MOVE_A = 0, K_MOVE_GEMM, K_MOVE_B, K_HPSI_JAS, K_HPSI_AO, K_HPSI_GEMM, K_HPSI_SLATER,
+              K_HPSI_NONLOC, K_HPSI_ELOC, K_OTHER, K_NCLASS };
+
+}  // namespace champ
+
+struct champ_ctx {
+    // ---- dimensions
+    int W = 0, NE = 0, NUP = 0, NDN = 0, NM = 0, NC = 0, NCT = 0, NB = 0, NO = 0, NQ = 0;
+    int nloc = 0, prec = 0, hpsi_chunk = 0, pair_cap = 0;
+    double pecent = 0, cdet0 = 1;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+
+    // ---- tables
+    double *cent = nullptr;          // [NC][3]
+    int *iwctype = nullptr;          // [NC]
+    champ::SplineTab tab{};
+    double *rmax_ao = nullptr;       // [NB] rmaxwf of each AO's shell
+    int *ao_cent = nullptr, *ao_type = nullptr, *ao_rad = nullptr, *ao_slm = nullptr;  // [NB]
+    double *coef = nullptr;          // [NO][NB]
+    double *cocc[2] = {nullptr, nullptr};  // [nel][NB] coef rows of occupied orbitals
+    int *iworbd = nullptr;           // [NE]
+    champ::JasParams hjas{};
+    champ::JasParams *jas = nullptr;
+    champ::EcpParams hecp{};
+    champ::EcpParams *ecp = nullptr;
+    double *xq0 = nullptr, *yq0 = nullptr, *zq0 = nullptr;  // [NQ]
+    std::vector<double> h_xq0, h_yq0, h_zq0;
+
+    // ---- walker state
+    double *xT = nullptr;      // [W][3][NE]
+    double *r_en = nullptr;    // [W][NC][NE]
+    double *rr_en = nullptr;   // [W][NC][NE]  scale_dist (A/B scaling)
+    double *fso = nullptr;     // [W][NE][NE]  symmetric, diagonal = e-n
+    double *g1 = nullptr;      // [W][NE][3][NE] g1[i][k][j] = grad_i of pair(i,j)
+    double *g2 = nullptr;      // [W][NE][3][NE] g2[i][k][j] = grad_j of pair(i,j)
+    double *vjT = nullptr;     // [W][3][NE]
+    double *fsumo = nullptr;   // [W]
+    double *d2o = nullptr;     // [W]
+    double *dorb = nullptr;    // [W][NE][3][NO]
+    double *slmi = nullptr;    // [W][2][NM][NM]
+    double *detiab = nullptr;  // [W][2]
+    double *ddx = nullptr;     // [W][NE][3]
+    double *d2dx2 = nullptr;   // [W][NE]
+    double *psido = nullptr, *psijo = nullptr, *psi2o = nullptr, *eold = nullptr, *ekin = nullptr;
+    double *pe_local = nullptr, *vpsp = nullptr /*[W][2]*/;
+    int *status = nullptr;
+    uint64_t *rng = nullptr;   // [W][4]
+    double *xq = nullptr, *yq = nullptr, *zq = nullptr;  // [W][NQ]
+
+    // ---- move scratch
+    double *xnew = nullptr;    // [W][3]
+    double *dfus2o = nullptr;  // [W]
+    double *rnew = nullptr, *rrnew = nullptr;  // [W][NC]
+    double *fsn_row = nullptr; // [W][NE]
+    double *g1n = nullptr, *g2n = nullptr;     // [W][3][NE]
+    double *vjnT = nullptr;    // [W][3][NE]
+    double *fsumn = nullptr;   // [W]
+    double *aomove = nullptr;  // [W][4][NB]
+    double *orbn = nullptr;    // [W][4][NO]
+    double *ratio = nullptr, *psidn = nullptr, *psijn = nullptr, *vnew = nullptr /*[W][3]*/, *vold = nullptr;
+    unsigned char *acc = nullptr;
+    double *esum_dev = nullptr;  // [4] accepted, esum, esum2, nflag
+
+    // ---- hpsi scratch
+    double *aoall = nullptr;   // [chunk][NE][5][NB]
+    double *orball = nullptr;  // [chunk][NE][5][NO]
+    double *wmat = nullptr;    // [W][NE][NB]
+    int *pair_count = nullptr, *pair_off = nullptr, *pair_total = nullptr;  // [W], [W+1], [2]
+    int *pair_e = nullptr, *pair_c = nullptr;  // [W*pair_cap]
+    double *pair_vps = nullptr;                // [W*pair_cap][MPS_L]
+    double *pair_val = nullptr;                // [W*pair_cap]
+    int *nxquad = nullptr;                     // [W]
+
+    // ---- timing
+    bool timing = false;
+    std::vector<champ::Timer> ev_pool;
+    std::vector<std::pair<int, int>> ev_used;  // (class, pool index)
+    double class_ms[champ::K_NCLASS] = {0};
+    long long class_n[champ::K_NCLASS] = {0};
+};
+
+namespace champ {
+// kernel launchers (engine_kernels.hip)
+int launch_move_a(champ_ctx *c, int iel, int mode);
+int launch_move_gemm(champ_ctx *c);
+int launch_move_b(champ_ctx *c, int iel, int mode);
+int launch_drift_old(champ_ctx *c, int iel);
+int launch_hpsi(champ_ctx *c);
+int launch_rotqua(champ_ctx *c);
+int launch_basis_points(champ_ctx *c, int npt, const double *dpos, double *dout);
+void set_error(const std::string &s);
+void tbegin(champ_ctx *c, int cls, int *slot);
+void tend(champ_ctx *c, int slot);
+}  // namespace champ
+
+#define HIPCHECK(x)                                                                       \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess) {                                                           \
+            champ::set_error(std::string(#x) + ": " + hipGetErrorString(e_));             \
+            return CHAMP_ERR_HIP;                                                         \
+        }                                                                                 \
+    } while (0)

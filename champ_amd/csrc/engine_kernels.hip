@@ -1,0 +1,528 @@
+// engine_kernels.hip -- FP64 walker-population kernels of the per-move path.
+//
+// Layout (walker-major, lanes run over the innermost index):
+//   xT[w][3][NE], r_en/rr_en[w][NC][NE], fso[w][NE][NE] (symmetric),
+//   g1/g2[w][NE][3][NE] (pair gradients on the first/second electron),
+//   vjT[w][3][NE], slmi[w][2][NM][NM] (row i = electron, CHAMP slmi(j+(i-1)n)),
+//   dorb[w][NE][3][NO].
+// One 64-lane wavefront serves one walker in the per-move kernels; the
+// AO->MO contractions are population-wide GEMMs.
+#include "engine_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace champ {
+
+struct MoveArgs {
+    int W, NE, NUP, NM, NC, NB, NO, iel, mode, pad;
+    double tau, cdet0;
+    const double *cent;
+    const int *iwctype;
+    SplineTab tab;
+    const double *rmax_ao;
+    const int *ao_cent, *ao_type, *ao_rad, *ao_slm, *iworbd;
+    const JasParams *jas;
+    double *xT, *r_en, *rr_en, *fso, *g1, *g2, *vjT, *fsumo, *dorb, *slmi, *detiab, *psido, *psijo, *psi2o;
+    uint64_t *rng;
+    double *xnew, *dfus2o, *rnew, *rrnew, *fsn_row, *g1n, *g2n, *vjnT, *fsumn, *aomove, *orbn;
+    double *ratio, *psidn, *psijn, *vnew, *vold, *esum_dev;
+    unsigned char *acc;
+    int *status;
+};
+
+static MoveArgs move_args(champ_ctx *c, int iel, int mode, double tau)
+{
+    MoveArgs a;
+    a.W = c->W; a.NE = c->NE; a.NUP = c->NUP; a.NM = c->NM; a.NC = c->NC; a.NB = c->NB; a.NO = c->NO;
+    a.iel = iel; a.mode = mode; a.pad = 0; a.tau = tau; a.cdet0 = c->cdet0;
+    a.cent = c->cent; a.iwctype = c->iwctype; a.tab = c->tab; a.rmax_ao = c->rmax_ao;
+    a.ao_cent = c->ao_cent; a.ao_type = c->ao_type; a.ao_rad = c->ao_rad; a.ao_slm = c->ao_slm;
+    a.iworbd = c->iworbd; a.jas = c->jas;
+    a.xT = c->xT; a.r_en = c->r_en; a.rr_en = c->rr_en; a.fso = c->fso; a.g1 = c->g1; a.g2 = c->g2;
+    a.vjT = c->vjT; a.fsumo = c->fsumo; a.dorb = c->dorb; a.slmi = c->slmi; a.detiab = c->detiab;
+    a.psido = c->psido; a.psijo = c->psijo; a.psi2o = c->psi2o; a.rng = c->rng;
+    a.xnew = c->xnew; a.dfus2o = c->dfus2o; a.rnew = c->rnew; a.rrnew = c->rrnew; a.fsn_row = c->fsn_row;
+    a.g1n = c->g1n; a.g2n = c->g2n; a.vjnT = c->vjnT; a.fsumn = c->fsumn; a.aomove = c->aomove; a.orbn = c->orbn;
+    a.ratio = c->ratio; a.psidn = c->psidn; a.psijn = c->psijn; a.vnew = c->vnew; a.vold = c->vold;
+    a.esum_dev = c->esum_dev; a.acc = c->acc; a.status = c->status;
+    return a;
+}
+
+// run-time tau for the metropolis kernels
+static double g_tau = 0.5;
+
+// ------------------------------------------------------------------------
+// Drift at the current position of iel: compute_determinante_grad with
+// iflag_move=1 (determinante.f90:127-157, determinante_ref_grad :285-328).
+__device__ __forceinline__ void drift_old(const MoveArgs &a, int w, int lane, double v[3])
+{
+    const int iel = a.iel, iab = iel < a.NUP ? 0 : 1, nel = iab ? a.NE - a.NUP : a.NUP, ish = iab ? a.NUP : 0;
+    const double *srow = a.slmi + ((size_t)w * 2 + iab) * a.NM * a.NM + (size_t)(iel - ish) * a.NM;
+    const double *dor = a.dorb + ((size_t)w * a.NE + iel) * 3 * a.NO;
+    double v0 = 0, v1 = 0, v2 = 0;
+    for (int j = lane; j < nel; j += WAVE) {
+        const double s = srow[j];
+        const int jo = a.iworbd[ish + j];
+        v0 += s * dor[jo];
+        v1 += s * dor[a.NO + jo];
+        v2 += s * dor[2 * a.NO + jo];
+    }
+    v0 = wave_sum(v0); v1 = wave_sum(v1); v2 = wave_sum(v2);
+    const double *vj = a.vjT + (size_t)w * 3 * a.NE;
+    v[0] = 0.0 + v0 + vj[iel];
+    v[1] = 0.0 + v1 + vj[a.NE + iel];
+    v[2] = 0.0 + v2 + vj[2 * a.NE + iel];
+}
+
+__global__ void __launch_bounds__(256) k_drift_old(MoveArgs a)
+{
+    const int lane = threadIdx.x % WAVE;
+    const int w = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (w >= a.W) return;
+    double v[3];
+    drift_old(a, w, lane, v);
+    if (lane == 0) for (int k = 0; k < 3; k++) a.vold[3 * w + k] = v[k];
+}
+
+// ------------------------------------------------------------------------
+// Move kernel A, one wavefront per walker:
+//   mode 0: metrop1 proposal (drift at old position, 3 gauss draws;
+//           metrop_mov1_driftdif.f90:105-117); mode 1: xnew given.
+//   then distances(iel) (distances.f90:3-167), jastrow4e with iflag=0
+//   (jastrow4e.f90:3-358) and basis_fns(iel, ider=1) (basis_fns.f90:8-214)
+//   writing the AO block that the MO GEMM contracts (orbitalse).
+__global__ void __launch_bounds__(256) k_move_a(MoveArgs a)
+{
+    HIP_DYNAMIC_SHARED(double, lds)
+    const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int w = blockIdx.x * (blockDim.x / WAVE) + wv;
+    const bool active = w < a.W;
+    const int NE = a.NE, NC = a.NC, iel = a.iel;
+    const JasParams &jp = *a.jas;
+    const double kap = jp.kap, asymp_r = jp.asymp_r;
+    double *Lr = lds + (size_t)wv * 7 * NC;
+    double *Lrr = Lr + NC, *Lsw = Lr + 2 * NC, *Ldd = Lr + 3 * NC, *Lrv = Lr + 4 * NC;
+
+    double xn0 = 0, xn1 = 0, xn2 = 0;
+    if (active) {
+        const double *xw = a.xT + (size_t)w * 3 * NE;
+        if (a.mode == 0) {
+            double vo[3];
+            drift_old(a, w, lane, vo);
+            if (lane == 0) {
+                const double tau = a.tau, rttau = sqrt(tau), adrift = 0.5;
+                uint64_t s[4];
+                for (int k = 0; k < 4; k++) s[k] = a.rng[4 * (size_t)w + k];
+                double v2old = vo[0] * vo[0] + vo[1] * vo[1] + vo[2] * vo[2];
+                double vavvt = (sqrt(1.0 + 2.0 * adrift * v2old * tau) - 1.0) / (adrift * v2old);
+                double dfus2o = 0.0, xn[3];
+                for (int k = 0; k < 3; k++) {
+                    double drift = vavvt * vo[k] * tau;
+                    double dfus = xo_gauss(s) * rttau;
+                    double dx = drift + dfus;
+                    dfus2o = dfus2o + dfus * dfus;
+                    xn[k] = xw[k * NE + iel] + dx;
+                }
+                for (int k = 0; k < 4; k++) a.rng[4 * (size_t)w + k] = s[k];
+                for (int k = 0; k < 3; k++) { a.xnew[3 * w + k] = xn[k]; a.vold[3 * w + k] = vo[k]; }
+                a.dfus2o[w] = dfus2o;
+                xn0 = xn[0]; xn1 = xn[1]; xn2 = xn[2];
+            }
+            xn0 = __shfl(xn0, 0, WAVE); xn1 = __shfl(xn1, 0, WAVE); xn2 = __shfl(xn2, 0, WAVE);
+        } else {
+            xn0 = a.xnew[3 * w]; xn1 = a.xnew[3 * w + 1]; xn2 = a.xnew[3 * w + 2];
+        }
+    }
+
+    // ---- e-n distances of the moved electron + its e-n Jastrow term
+    double fs_en = 0, ge0 = 0, ge1 = 0, ge2 = 0;
+    if (active) {
+        for (int c = lane; c < NC; c += WAVE) {
+            const double rv0 = xn0 - a.cent[3 * c], rv1 = xn1 - a.cent[3 * c + 1], rv2 = xn2 - a.cent[3 * c + 2];
+            double r2 = 0;
+            r2 = r2 + rv0 * rv0; r2 = r2 + rv1 * rv1; r2 = r2 + rv2 * rv2;
+            const double r = sqrt(r2);
+            const double rr = scale_r(kap, r);
+            const double dd7 = 1.0 - kap * rr;
+            Lr[c] = r; Lrr[c] = rr; Lsw[c] = 1.0 - kap * rr; Ldd[c] = -kap * dd7;
+            Lrv[3 * c] = rv0; Lrv[3 * c + 1] = rv1; Lrv[3 * c + 2] = rv2;
+            a.rnew[(size_t)w * NC + c] = r;
+            a.rrnew[(size_t)w * NC + c] = rr;
+            if (rr != asymp_r) {
+                const int it = a.iwctype[c];
+                const double *a4 = jp.a4[it];
+                double top = a4[0] * rr, topi = a4[0];
+                double bot = 1 + a4[1] * rr, boti = a4[1], bot2 = bot * bot;
+                double fen = top / bot - jp.ja[it];
+                double feni = topi / bot - boti * top / bot2;
+                double rprev = rr;  // rri(iord-1)
+                for (int iord = 2; iord <= jp.norda; iord++) {
+                    double rp = powi(rr, iord);
+                    fen = fen + a4[iord] * rp;
+                    feni = feni + a4[iord] * iord * rprev;
+                    rprev = rp;
+                }
+                feni = feni * dd7 / r;
+                fs_en += fen;
+                ge0 += feni * rv0; ge1 += feni * rv1; ge2 += feni * rv2;
+            }
+        }
+        fs_en = wave_sum(fs_en); ge0 = wave_sum(ge0); ge1 = wave_sum(ge1); ge2 = wave_sum(ge2);
+    }
+    __syncthreads();
+
+    if (active) {
+        const double *xw = a.xT + (size_t)w * 3 * NE;
+        const double *fso_row = a.fso + ((size_t)w * NE + iel) * NE;
+        const double *g1_row = a.g1 + ((size_t)w * NE + iel) * 3 * NE;
+        const double *g2_row = a.g2 + ((size_t)w * NE + iel) * 3 * NE;
+        const double *vj = a.vjT + (size_t)w * 3 * NE;
+        double *vjn = a.vjnT + (size_t)w * 3 * NE;
+        double *fsn = a.fsn_row + (size_t)w * NE;
+        double *g1n = a.g1n + (size_t)w * 3 * NE, *g2n = a.g2n + (size_t)w * 3 * NE;
+        const double *rw = a.r_en + (size_t)w * NC * NE, *rrw = a.rr_en + (size_t)w * NC * NE;
+        double dfs = 0, dg0 = 0, dg1 = 0, dg2 = 0;
+        const int nordc = jp.nordc;
+        for (int j = lane; j < NE; j += WAVE) {
+            if (j == iel) {
+                fsn[iel] = fs_en;
+                g1n[iel] = ge0; g1n[NE + iel] = ge1; g1n[2 * NE + iel] = ge2;
+                g2n[iel] = ge0; g2n[NE + iel] = ge1; g2n[2 * NE + iel] = ge2;
+                dfs += fs_en - fso_row[iel];
+                dg0 += ge0 - g1_row[iel]; dg1 += ge1 - g1_row[NE + iel]; dg2 += ge2 - g1_row[2 * NE + iel];
+                continue;
+            }
+            const bool inew = iel > j;  // moved electron is the first of the pair (i > j)
+            const int I = inew ? iel : j, J = inew ? j : iel;
+            const double xj0 = xw[j], xj1 = xw[NE + j], xj2 = xw[2 * NE + j];
+            double rv[3];
+            if (inew) { rv[0] = xn0 - xj0; rv[1] = xn1 - xj1; rv[2] = xn2 - xj2; }
+            else { rv[0] = xj0 - xn0; rv[1] = xj1 - xn1; rv[2] = xj2 - xn2; }
+            double r2 = 0;
+            r2 = r2 + rv[0] * rv[0]; r2 = r2 + rv[1] * rv[1]; r2 = r2 + rv[2] * rv[2];
+            const double rij = sqrt(r2);
+            double sspinn = 1; int isb = 0, ipar = 0;
+            if (I < a.NUP || J >= a.NUP) {
+                if (jp.nspin2b == 2) isb = 1;
+                else if (jp.nocuspb == 0) sspinn = 0.5;
+                ipar = 1;
+            }
+            const double u = scale_r(kap, rij);
+            const double dd1 = 1.0 - kap * u;
+            double fs = 0, fI[3] = {0, 0, 0}, fJ[3] = {0, 0, 0};
+            if (u != asymp_r) {
+                const double *b = jp.b[isb];
+                double top = sspinn * b[0] * u, topu = sspinn * b[0];
+                double bot = 1 + b[1] * u, botu = b[1], bot2 = bot * bot;
+                double fee = top / bot - jp.jb[ipar];
+                double feeu = topu / bot - botu * top / bot2;
+                double up = u;
+                for (int iord = 2; iord <= jp.nordb; iord++) {
+                    double un = u * up;
+                    fee = fee + b[iord] * un;
+                    feeu = feeu + b[iord] * iord * up;
+                    up = un;
+                }
+                feeu = feeu * dd1 / rij;
+                fs += fee;
+                for (int k = 0; k < 3; k++) { fI[k] += feeu * rv[k]; fJ[k] -= feeu * rv[k]; }
+            }
+            if (nordc > 1) {
+                double uu_[CHAMP_NORDP + 4], ri_[CHAMP_NORDP + 4], rj_[CHAMP_NORDP + 4], ss[CHAMP_NORDP + 1], tt[CHAMP_NORDP + 1];
+                double *uu = uu_ + 3, *rri = ri_ + 3, *rrj = rj_ + 3;
+                uu[-3] = uu[-2] = uu[-1] = 0; rri[-3] = rri[-2] = rri[-1] = 0; rrj[-3] = rrj[-2] = rrj[-1] = 0;
+                uu[0] = 1; rri[0] = 1; rrj[0] = 1; ss[0] = 2; tt[0] = 1;
+                uu[1] = 1 - kap * u;
+                const double dd1s = -kap * dd1;
+                for (int iord = 2; iord <= nordc; iord++) uu[iord] = uu[1] * uu[iord - 1];
+                for (int c = 0; c < NC; c++) {
+                    const double rjo = rw[(size_t)c * NE + j], rrjo = rrw[(size_t)c * NE + j];
+                    const double swjo = 1 - kap * rrjo, ddjo = -kap * (1.0 - kap * rrjo);
+                    double rvj[3] = {xj0 - a.cent[3 * c], xj1 - a.cent[3 * c + 1], xj2 - a.cent[3 * c + 2]};
+                    double rI, rrI, swI, ddI, rJ, rrJ, swJ, ddJ, rvI[3], rvJ[3];
+                    if (inew) {
+                        rI = Lr[c]; rrI = Lrr[c]; swI = Lsw[c]; ddI = Ldd[c];
+                        rvI[0] = Lrv[3 * c]; rvI[1] = Lrv[3 * c + 1]; rvI[2] = Lrv[3 * c + 2];
+                        rJ = rjo; rrJ = rrjo; swJ = swjo; ddJ = ddjo; rvJ[0] = rvj[0]; rvJ[1] = rvj[1]; rvJ[2] = rvj[2];
+                    } else {
+                        rJ = Lr[c]; rrJ = Lrr[c]; swJ = Lsw[c]; ddJ = Ldd[c];
+                        rvJ[0] = Lrv[3 * c]; rvJ[1] = Lrv[3 * c + 1]; rvJ[2] = Lrv[3 * c + 2];
+                        rI = rjo; rrI = rrjo; swI = swjo; ddI = ddjo; rvI[0] = rvj[0]; rvI[1] = rvj[1]; rvI[2] = rvj[2];
+                    }
+                    if (rrI == asymp_r && rrJ == asymp_r) continue;
+                    rri[1] = swI; rrj[1] = swJ;
+                    for (int iord = 1; iord <= nordc; iord++) {
+                        rri[iord] = rri[1] * rri[iord - 1];
+                        rrj[iord] = rrj[1] * rrj[iord - 1];
+                        ss[iord] = rri[iord] + rrj[iord];
+                        tt[iord] = rri[iord] * rrj[iord];
+                    }
+                    const Een<double, false> e = een_eval<double, false>(jp, jp.c[a.iwctype[c]], uu, rri, rrj, ss, tt);
+                    const double fu = e.fu * dd1s / rij, fi = e.fi * ddI / rI, fj = e.fj * ddJ / rJ;
+                    fs += e.fc;
+                    for (int k = 0; k < 3; k++) {
+                        fI[k] = fI[k] + fi * rvI[k] + fu * rv[k];
+                        fJ[k] = fJ[k] + fj * rvJ[k] - fu * rv[k];
+                    }
+                }
+            }
+            const double *gi = inew ? fI : fJ, *gj = inew ? fJ : fI;
+            fsn[j] = fs;
+            dfs += fs - fso_row[j];
+            for (int k = 0; k < 3; k++) {
+                g1n[k * NE + j] = gi[k];
+                g2n[k * NE + j] = gj[k];
+                vjn[k * NE + j] = vj[k * NE + j] + gj[k] - g2_row[k * NE + j];
+            }
+            dg0 += gi[0] - g1_row[j]; dg1 += gi[1] - g1_row[NE + j]; dg2 += gi[2] - g1_row[2 * NE + j];
+        }
+        dfs = wave_sum(dfs); dg0 = wave_sum(dg0); dg1 = wave_sum(dg1); dg2 = wave_sum(dg2);
+        if (lane == 0) {
+            const double fsn_tot = a.fsumo[w] + dfs;
+            a.fsumn[w] = fsn_tot;
+            a.psijn[w] = fsn_tot;
+            vjn[iel] = vj[iel] + dg0; vjn[NE + iel] = vj[NE + iel] + dg1; vjn[2 * NE + iel] = vj[2 * NE + iel] + dg2;
+        }
+
+        // ---- AOs of the moved electron (ider = 1) into the GEMM operand
+        double *ao = a.aomove + (size_t)w * 4 * a.NB;
+        for (int m = lane; m < a.NB; m += WAVE) {
+            const int c = a.ao_cent[m];
+            double out[5];
+            ao_eval<double>(a.tab, a.ao_type[m], a.ao_rad[m], a.ao_slm[m], Lrv[3 * c], Lrv[3 * c + 1], Lrv[3 * c + 2], Lr[c],
+                            a.rmax_ao[m], 1, out);
+            ao[m] = out[0]; ao[a.NB + m] = out[1]; ao[2 * a.NB + m] = out[2]; ao[3 * a.NB + m] = out[3];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Move kernel B, one 256-thread block per walker: determinante ratio
+// (determinante.f90:38-46), drift at the new position (iflag_move=0,
+// :158-199), metropolis decision (metrop_mov1_driftdif.f90:166-205) and on
+// acceptance jassav + detsav with the Sherman-Morrison update applied in
+// place (determinante.f90:47-62 + detsav.f90:42-93).
+//   mode 0: decide with the walker's stream; 1: compute only; 2: commit acc[w].
+__global__ void __launch_bounds__(256) k_move_b(MoveArgs a)
+{
+    HIP_DYNAMIC_SHARED(double, lds)
+    const int w = blockIdx.x, tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE, nwv = blockDim.x / WAVE;
+    const int NE = a.NE, NO = a.NO, iel = a.iel;
+    const int iab = iel < a.NUP ? 0 : 1, nel = iab ? NE - a.NUP : a.NUP, ish = iab ? a.NUP : 0, row = iel - ish;
+    double *S = a.slmi + ((size_t)w * 2 + iab) * a.NM * a.NM;
+    const double *ob = a.orbn + (size_t)w * 4 * NO;
+    double *s_row = lds, *s_orb = lds + a.NM;
+    __shared__ double s_ratio, s_detn;
+    __shared__ int s_acc;
+
+    if (wv == 0) {
+        const double *srow = S + (size_t)row * a.NM;
+        double rt = 0;
+        for (int j = lane; j < nel; j += WAVE) rt += srow[j] * ob[a.iworbd[ish + j]];
+        rt = wave_sum(rt);
+        double v0 = 0, v1 = 0, v2 = 0;
+        for (int j = lane; j < nel; j += WAVE) {
+            const double sn = srow[j] / rt;
+            const int jo = a.iworbd[ish + j];
+            v0 += sn * ob[NO + jo]; v1 += sn * ob[2 * NO + jo]; v2 += sn * ob[3 * NO + jo];
+        }
+        v0 = wave_sum(v0); v1 = wave_sum(v1); v2 = wave_sum(v2);
+        if (lane == 0) {
+            const double *vjn = a.vjnT + (size_t)w * 3 * NE;
+            double vn[3] = {0.0 + v0 + vjn[iel], 0.0 + v1 + vjn[NE + iel], 0.0 + v2 + vjn[2 * NE + iel]};
+            const double detn = a.detiab[2 * w + iab] * rt;
+            const double psidn = (rt == 0.0) ? 0.0 : 0.0 + detn * a.detiab[2 * w + 1 - iab] * a.cdet0;
+            const double psijn = a.psijn[w];
+            int accept = 0;
+            if (a.mode == 0) {
+                const double tau = a.tau, adrift = 0.5;
+                const double psi2n = 2 * (log(fabs(psidn)) + psijn);
+                const double v2new = vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2];
+                const double vavvt = (sqrt(1.0 + 2.0 * adrift * v2new * tau) - 1.0) / (adrift * v2new);
+                const double *xw = a.xT + (size_t)w * 3 * NE;
+                double dfus2n = 0.0;
+                for (int k = 0; k < 3; k++) {
+                    double drift = vavvt * vn[k] * tau;
+                    double xbac = a.xnew[3 * w + k] + drift;
+                    double dfus = xbac - xw[k * NE + iel];
+                    dfus2n = dfus2n + dfus * dfus;
+                }
+                double p = exp(psi2n - a.psi2o[w]) * exp(-0.5 * (dfus2n - a.dfus2o[w]) / tau);
+                if (psidn == 0.0) p = 0.0;
+                p = p < 1.0 ? p : 1.0;
+                uint64_t s[4];
+                for (int k = 0; k < 4; k++) s[k] = a.rng[4 * (size_t)w + k];
+                const double uacc = xo_uniform(s);
+                for (int k = 0; k < 4; k++) a.rng[4 * (size_t)w + k] = s[k];
+                accept = uacc < p;
+                if (accept) {
+                    a.psi2o[w] = psi2n;
+                    atomicAdd(&a.esum_dev[0], 1.0);
+                }
+                if (rt == 0.0) atomicOr(&a.status[w], CHAMP_WALKER_ZERO_RATIO);
+            } else if (a.mode == 2) {
+                accept = a.acc[w] != 0;
+                if (accept) a.psi2o[w] = 2 * (log(fabs(psidn)) + psijn);
+            }
+            a.ratio[w] = rt; a.psidn[w] = psidn;
+            for (int k = 0; k < 3; k++) a.vnew[3 * w + k] = vn[k];
+            if (accept) { a.psido[w] = psidn; a.psijo[w] = psijn; }
+            s_ratio = rt; s_detn = detn; s_acc = accept;
+        }
+    }
+    __syncthreads();
+    if (!s_acc) return;
+    const double rt = s_ratio;
+    for (int j = tid; j < nel; j += blockDim.x) {
+        s_row[j] = S[(size_t)row * a.NM + j];
+        s_orb[j] = ob[a.iworbd[ish + j]];
+    }
+    __syncthreads();
+    // Sherman-Morrison: rows i != iel
+    for (int i = wv; i < nel; i += nwv) {
+        if (i == row) continue;
+        double *Si = S + (size_t)i * a.NM;
+        double sum = 0;
+        for (int j = lane; j < nel; j += WAVE) sum += Si[j] * s_orb[j];
+        sum = wave_sum(sum);
+        sum = sum / rt;
+        for (int j = lane; j < nel; j += WAVE) Si[j] = Si[j] - s_row[j] * sum;
+    }
+    for (int j = tid; j < nel; j += blockDim.x) S[(size_t)row * a.NM + j] = s_row[j] / rt;
+    // detsav: orbital gradients of iel, spin determinant
+    double *dor = a.dorb + ((size_t)w * NE + iel) * 3 * NO;
+    for (int t = tid; t < 3 * NO; t += blockDim.x) dor[t] = ob[NO + t];
+    if (tid == 0) a.detiab[2 * w + iab] = s_detn;
+    // position and distances
+    if (tid < 3) a.xT[((size_t)w * 3 + tid) * NE + iel] = a.xnew[3 * w + tid];
+    for (int c = tid; c < a.NC; c += blockDim.x) {
+        a.r_en[((size_t)w * a.NC + c) * NE + iel] = a.rnew[(size_t)w * a.NC + c];
+        a.rr_en[((size_t)w * a.NC + c) * NE + iel] = a.rrnew[(size_t)w * a.NC + c];
+    }
+    // jassav (iflag = 0)
+    const double *fsn = a.fsn_row + (size_t)w * NE;
+    const double *g1n = a.g1n + (size_t)w * 3 * NE, *g2n = a.g2n + (size_t)w * 3 * NE;
+    const double *vjn = a.vjnT + (size_t)w * 3 * NE;
+    double *vj = a.vjT + (size_t)w * 3 * NE;
+    double *F = a.fso + (size_t)w * NE * NE;
+    double *G1 = a.g1 + (size_t)w * NE * 3 * NE, *G2 = a.g2 + (size_t)w * NE * 3 * NE;
+    for (int j = tid; j < NE; j += blockDim.x) {
+        const double v = fsn[j];
+        F[(size_t)iel * NE + j] = v;
+        F[(size_t)j * NE + iel] = v;
+        for (int k = 0; k < 3; k++) {
+            G1[((size_t)iel * 3 + k) * NE + j] = g1n[k * NE + j];
+            G2[((size_t)iel * 3 + k) * NE + j] = g2n[k * NE + j];
+            G1[((size_t)j * 3 + k) * NE + iel] = g2n[k * NE + j];
+            G2[((size_t)j * 3 + k) * NE + iel] = g1n[k * NE + j];
+            vj[k * NE + j] = vjn[k * NE + j];
+        }
+    }
+    if (tid == 0) a.fsumo[w] = a.fsumn[w];
+}
+
+// ------------------------------------------------------------------------
+// rotqua (rotqua.f90:357-430) per walker with its own stream.
+__global__ void k_rotqua(int W, int nq, uint64_t *rng, const double *xq0, const double *yq0, const double *zq0,
+                         double *xq, double *yq, double *zq)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    uint64_t s[4];
+    for (int k = 0; k < 4; k++) s[k] = rng[4 * (size_t)w + k];
+    // rejection sampling as in rotqua; the trip cap (probability of reaching
+    // it ~ 0.215^1000) only guarantees that every thread finishes
+    double x1 = 0, x2 = 0, x3, xsum = 0, cfi, sfi, sthet;
+    for (int tries = 0; tries < 1000; tries++) {
+        x1 = 1.0 - 2.0 * xo_uniform(s);
+        x2 = 1.0 - 2.0 * xo_uniform(s);
+        xsum = x1 * x1 + x2 * x2;
+        if (xsum < 1.0) break;
+    }
+    double xsum2 = 2.0 * sqrt(fabs(1.0 - xsum));
+    x1 = x1 * xsum2; x2 = x2 * xsum2; x3 = 1.0 - 2.0 * xsum;
+    double theta = acos(x3);
+    sthet = sin(theta);
+    if (sthet < 1.0e-05) { cfi = 1.0; sfi = 0.0; x1 = 0.0; x2 = 0.0; x3 = 1.0; sthet = 0.0; }
+    else { cfi = x1 / sthet; sfi = x2 / sthet; }
+    double yy1 = x3 * cfi, yy2 = x3 * sfi, yy3 = -sthet, zz1 = -sfi, zz2 = cfi, zz3 = 0.0;
+    double u1 = 1, u2 = 0, usum = 1;
+    for (int tries = 0; tries < 1000; tries++) {
+        u1 = xo_uniform(s) * 2.0 - 1.0;
+        u2 = xo_uniform(s) * 2.0 - 1.0;
+        usum = u1 * u1 + u2 * u2;
+        if (usum < 1.0) break;
+    }
+    double uu = sqrt(usum);
+    u1 = u1 / uu; u2 = u2 / uu;
+    double y1 = yy1 * u1 + zz1 * u2, y2 = yy2 * u1 + zz2 * u2, y3 = yy3 * u1 + zz3 * u2;
+    double z1 = yy1 * u2 - zz1 * u1, z2 = yy2 * u2 - zz2 * u1, z3 = yy3 * u2 - zz3 * u1;
+    for (int iq = 0; iq < nq; iq++) {
+        xq[(size_t)w * nq + iq] = xq0[iq] * x1 + yq0[iq] * y1 + zq0[iq] * z1;
+        yq[(size_t)w * nq + iq] = xq0[iq] * x2 + yq0[iq] * y2 + zq0[iq] * z2;
+        zq[(size_t)w * nq + iq] = xq0[iq] * x3 + yq0[iq] * y3 + zq0[iq] * z3;
+    }
+    for (int k = 0; k < 4; k++) rng[4 * (size_t)w + k] = s[k];
+}
+
+// ------------------------------------------------------------------------
+// launchers
+static inline int nblocks(int n, int per) { return (n + per - 1) / per; }
+
+int launch_drift_old(champ_ctx *c, int iel)
+{
+    MoveArgs a = move_args(c, iel, 1, g_tau);
+    hipLaunchKernelGGL(k_drift_old, dim3(nblocks(c->W, 4)), dim3(256), 0, c->stream, a);
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
+int launch_move_a(champ_ctx *c, int iel, int mode)
+{
+    MoveArgs a = move_args(c, iel, mode, g_tau);
+    size_t lds = (size_t)4 * 7 * c->NC * sizeof(double);
+    int slot;
+    tbegin(c, K_MOVE_A, &slot);
+    hipLaunchKernelGGL(k_move_a, dim3(nblocks(c->W, 4)), dim3(256), lds, c->stream, a);
+    tend(c, slot);
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
+int launch_move_gemm(champ_ctx *c)
+{
+    // orbn[w][comp][o] = sum_m aomove[w][comp][m] * coef[o][m]
+    const double one = 1.0, zero = 0.0;
+    int slot;
+    tbegin(c, K_MOVE_GEMM, &slot);
+    rocblas_status st = rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->NO, 4 * c->W,
+                                      c->NB, &one, c->coef, c->NB, c->aomove, c->NB, &zero, c->orbn, c->NO);
+    tend(c, slot);
+    if (st != rocblas_status_success) { set_error("rocblas_dgemm (move) failed"); return CHAMP_ERR_HIP; }
+    return CHAMP_OK;
+}
+
+int launch_move_b(champ_ctx *c, int iel, int mode)
+{
+    MoveArgs a = move_args(c, iel, mode, g_tau);
+    size_t lds = (size_t)2 * c->NM * sizeof(double);
+    int slot;
+    tbegin(c, K_MOVE_B, &slot);
+    hipLaunchKernelGGL(k_move_b, dim3(c->W), dim3(256), lds, c->stream, a);
+    tend(c, slot);
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
+int launch_rotqua(champ_ctx *c)
+{
+    hipLaunchKernelGGL(k_rotqua, dim3(nblocks(c->W, 256)), dim3(256), 0, c->stream, c->W, c->NQ, c->rng, c->xq0,
+                       c->yq0, c->zq0, c->xq, c->yq, c->zq);
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
+void set_tau(double tau) { g_tau = tau; }
+
+}  // namespace champ

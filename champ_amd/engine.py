@@ -1,0 +1,212 @@
+"""Python host side of the engine: ctypes bindings of include/champ_gpu.h.
+
+The methods keep the names and argument meaning of the CHAMP routines they
+stand for (hpsi, psie, compute_determinante_grad, jassav/detsav via accept,
+metrop1, rotqua), each acting on a whole walker population.  There is no CPU
+fallback: constructing an :class:`Engine` without the built library or
+without a GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+from .system import CSystem, System
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libchamp_gpu.so")
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+ARR = dict(slmi=1, vj=2, ddx=3, d2dx2=4, detiab=5, vpsp=6, peloc=7, d2j=8, orbn=9, vjn=10, nxquad=11)
+
+EXPORTS = [
+    "champ_gpu_init", "champ_gpu_free", "champ_gpu_last_error", "champ_gpu_set_walkers", "champ_gpu_get_walkers",
+    "champ_gpu_set_rng", "champ_gpu_get_rng", "champ_gpu_seed_rng", "champ_gpu_set_quadrature", "champ_gpu_rotqua",
+    "champ_gpu_hpsi", "champ_gpu_compute_determinante_grad", "champ_gpu_psie", "champ_gpu_accept",
+    "champ_gpu_metrop1", "champ_gpu_get_walker_scalars", "champ_gpu_get_status", "champ_gpu_array_size",
+    "champ_gpu_get_array", "champ_gpu_basis_fns", "champ_gpu_kernel_times",
+]
+
+
+class SweepStats(C.Structure):
+    _fields_ = [("accepted", C.c_double), ("proposed", C.c_double), ("esum", C.c_double), ("esum2", C.c_double),
+                ("nflagged", C.c_int64)]
+
+
+_libs: Dict[str, C.CDLL] = {}
+
+
+def load_library(path: str = LIBPATH):
+    """Load libchamp_gpu.so (or another build of the same C-ABI); raises if it
+    has not been built."""
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m champ_amd.build`")
+    L = C.CDLL(path)
+    vp, dp, i32 = C.c_void_p, C.POINTER(C.c_double), C.c_int32
+    L.champ_gpu_init.argtypes = [C.POINTER(vp), C.POINTER(CSystem), i32, i32]
+    L.champ_gpu_free.argtypes = [vp]
+    L.champ_gpu_last_error.restype = C.c_char_p
+    L.champ_gpu_set_walkers.argtypes = [vp, dp]
+    L.champ_gpu_get_walkers.argtypes = [vp, dp]
+    L.champ_gpu_set_rng.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.champ_gpu_get_rng.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.champ_gpu_seed_rng.argtypes = [vp, C.POINTER(C.c_int32)]
+    L.champ_gpu_set_quadrature.argtypes = [vp, dp, dp, dp]
+    L.champ_gpu_rotqua.argtypes = [vp]
+    L.champ_gpu_hpsi.argtypes = [vp, dp, dp, dp, dp]
+    L.champ_gpu_compute_determinante_grad.argtypes = [vp, i32, i32, dp]
+    L.champ_gpu_psie.argtypes = [vp, i32, dp, dp, dp, i32]
+    L.champ_gpu_accept.argtypes = [vp, i32, C.POINTER(C.c_uint8)]
+    L.champ_gpu_metrop1.argtypes = [vp, i32, C.c_double, C.POINTER(SweepStats)]
+    L.champ_gpu_get_walker_scalars.argtypes = [vp, dp, dp, dp, dp]
+    L.champ_gpu_get_status.argtypes = [vp, C.POINTER(C.c_int32)]
+    L.champ_gpu_array_size.argtypes = [vp, i32]
+    L.champ_gpu_array_size.restype = C.c_int64
+    L.champ_gpu_get_array.argtypes = [vp, i32, dp]
+    L.champ_gpu_basis_fns.argtypes = [vp, i32, dp, dp]
+    L.champ_gpu_kernel_times.argtypes = [vp, C.POINTER(i32), C.c_char_p, dp, C.POINTER(C.c_int64)]
+    _libs[path] = L
+    return L
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Engine:
+    """A walker population of one system resident on the current GPU."""
+
+    def __init__(self, system: System, nwalk: int, precision: int = PREC_FP64, lib_path: str = LIBPATH):
+        self.L = load_library(lib_path)
+        self.system = system
+        self.nwalk = int(nwalk)
+        self.csys = system.as_ctypes()
+        self.ptr = C.c_void_p()
+        rc = self.L.champ_gpu_init(C.byref(self.ptr), C.byref(self.csys), self.nwalk, int(precision))
+        self._check(rc, "champ_gpu_init")
+        self.nelec, self.norb = system.nelec, system.norb
+        self.nmax = max(system.nup, system.ndn)
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {self.L.champ_gpu_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            self.L.champ_gpu_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- state
+    def set_walkers(self, x: np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.nwalk, self.nelec, 3)
+        self._check(self.L.champ_gpu_set_walkers(self.ptr, _p(x)), "set_walkers")
+
+    def get_walkers(self) -> np.ndarray:
+        x = np.zeros((self.nwalk, self.nelec, 3))
+        self._check(self.L.champ_gpu_get_walkers(self.ptr, _p(x)), "get_walkers")
+        return x
+
+    def seed_rng(self, seed8):
+        s = np.ascontiguousarray(seed8, dtype=np.int32)
+        self._check(self.L.champ_gpu_seed_rng(self.ptr, s.ctypes.data_as(C.POINTER(C.c_int32))), "seed_rng")
+
+    def set_rng(self, st: np.ndarray):
+        st = np.ascontiguousarray(st, dtype=np.uint64).reshape(self.nwalk, 4)
+        self._check(self.L.champ_gpu_set_rng(self.ptr, st.ctypes.data_as(C.POINTER(C.c_uint64))), "set_rng")
+
+    def get_rng(self) -> np.ndarray:
+        st = np.zeros((self.nwalk, 4), dtype=np.uint64)
+        self._check(self.L.champ_gpu_get_rng(self.ptr, st.ctypes.data_as(C.POINTER(C.c_uint64))), "get_rng")
+        return st
+
+    def set_quadrature(self, xq, yq, zq):
+        a = [np.ascontiguousarray(np.broadcast_to(v, (self.nwalk, len(self.system.wq))), dtype=np.float64)
+             for v in (xq, yq, zq)]
+        self._check(self.L.champ_gpu_set_quadrature(self.ptr, _p(a[0]), _p(a[1]), _p(a[2])), "set_quadrature")
+
+    def rotqua(self):
+        self._check(self.L.champ_gpu_rotqua(self.ptr), "rotqua")
+
+    # ------------------------------------------------------- CHAMP routines
+    def hpsi(self) -> Dict[str, np.ndarray]:
+        o = [np.zeros(self.nwalk) for _ in range(4)]
+        self._check(self.L.champ_gpu_hpsi(self.ptr, *map(_p, o)), "hpsi")
+        return dict(psid=o[0], psij=o[1], ekin=o[2], energy=o[3])
+
+    def compute_determinante_grad(self, iel: int, iflag_move: int) -> np.ndarray:
+        v = np.zeros((self.nwalk, 3))
+        self._check(self.L.champ_gpu_compute_determinante_grad(self.ptr, int(iel), int(iflag_move), _p(v)),
+                    "compute_determinante_grad")
+        return v
+
+    def psie(self, iel: int, xnew: np.ndarray, iflag: int = 0):
+        xnew = np.ascontiguousarray(xnew, dtype=np.float64).reshape(self.nwalk, 3)
+        pd, pj = np.zeros(self.nwalk), np.zeros(self.nwalk)
+        self._check(self.L.champ_gpu_psie(self.ptr, int(iel), _p(xnew), _p(pd), _p(pj), int(iflag)), "psie")
+        return pd, pj
+
+    def accept(self, iel: int, mask: np.ndarray):
+        m = np.ascontiguousarray(mask, dtype=np.uint8).reshape(self.nwalk)
+        self._check(self.L.champ_gpu_accept(self.ptr, int(iel), m.ctypes.data_as(C.POINTER(C.c_uint8))), "accept")
+
+    def metrop1(self, nsweep: int, tau: float = 0.5) -> Dict[str, float]:
+        st = SweepStats()
+        self._check(self.L.champ_gpu_metrop1(self.ptr, int(nsweep), float(tau), C.byref(st)), "metrop1")
+        return dict(accepted=st.accepted, proposed=st.proposed, esum=st.esum, esum2=st.esum2, nflagged=st.nflagged)
+
+    def scalars(self) -> Dict[str, np.ndarray]:
+        o = [np.zeros(self.nwalk) for _ in range(4)]
+        self._check(self.L.champ_gpu_get_walker_scalars(self.ptr, *map(_p, o)), "get_walker_scalars")
+        return dict(psido=o[0], psijo=o[1], eold=o[2], psi2o=o[3])
+
+    def status(self) -> np.ndarray:
+        s = np.zeros(self.nwalk, dtype=np.int32)
+        self._check(self.L.champ_gpu_get_status(self.ptr, s.ctypes.data_as(C.POINTER(C.c_int32))), "get_status")
+        return s
+
+    def array(self, name: str) -> np.ndarray:
+        which = ARR[name]
+        n = self.L.champ_gpu_array_size(self.ptr, which)
+        out = np.zeros(n)
+        self._check(self.L.champ_gpu_get_array(self.ptr, which, _p(out)), f"get_array({name})")
+        W, NE, NM, NO = self.nwalk, self.nelec, self.nmax, self.norb
+        shapes = dict(slmi=(W, 2, NM, NM), vj=(W, NE, 3), vjn=(W, NE, 3), ddx=(W, NE, 3), d2dx2=(W, NE),
+                      detiab=(W, 2), vpsp=(W, 2), peloc=(W,), d2j=(W,), orbn=(W, 4, NO), nxquad=(W,))
+        return out.reshape(shapes[name])
+
+    def basis_fns(self, pos: np.ndarray) -> np.ndarray:
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros((pos.shape[0], 5, self.system.nbasis))
+        self._check(self.L.champ_gpu_basis_fns(self.ptr, pos.shape[0], _p(pos), _p(out)), "basis_fns")
+        return out
+
+    # ---------------------------------------------------------------- timing
+    def enable_timing(self, on: bool = True):
+        n = C.c_int32(1 if on else 0)
+        self._check(self.L.champ_gpu_kernel_times(self.ptr, C.byref(n), None, None, None), "kernel_times")
+
+    def kernel_times(self) -> Dict[str, tuple]:
+        n = C.c_int32(16)
+        names = C.create_string_buffer(32 * 16)
+        ms = np.zeros(16)
+        cnt = np.zeros(16, dtype=np.int64)
+        self._check(self.L.champ_gpu_kernel_times(self.ptr, C.byref(n), names, _p(ms),
+                                                  cnt.ctypes.data_as(C.POINTER(C.c_int64))), "kernel_times")
+        out = {}
+        for k in range(n.value):
+            nm = names.raw[32 * k:32 * k + 32].split(b"\0", 1)[0].decode()
+            out[nm] = (float(ms[k]), int(cnt[k]))
+        return out

@@ -1,0 +1,505 @@
+"""Synthetic CHAMP systems and their tables.
+
+Builds, from a seed, the input a CHAMP run would read (geometry, Gaussian
+basis, MO coefficients, ijas=4 Jastrow, Gaussian ECP, quadrature rule) and
+turns it into the tables the per-move path consumes, the way CHAMP does at
+input time:
+
+* Gaussian contractions tabulated on the grid-3 radial mesh with the small-r
+  polynomial fit, the cut radius scan, the exponential tail fit and the cubic
+  spline second derivatives (src/vmc/read_data_trexio.f90:991-1179,
+  src/vmc/fitting.f90:11-74, src/vmc/spline2.f90:3-80, gnorm in
+  src/module/m_basis.f90:515-536);
+* e-e-n coefficients made cusp-free by the dependency rule of
+  src/vmc/cuspinit4.f90:3-70 and src/vmc/cuspexact4.f90:3-60;
+* the 12-point icosahedral quadrature of src/vmc/rotqua.f90:3-253.
+
+The resulting :class:`System` is plain numpy; :meth:`System.as_ctypes` gives
+the ``champ_system_t`` descriptor of include/champ_gpu.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import math
+from typing import Dict, List, Sequence
+
+import numpy as np
+
+NCOEF = 5
+MPS_L = 5
+MGAUSS = 8
+MPS_QUAD = 86
+
+SHELL_L = {"s": 0, "p": 1, "d": 2, "f": 3}
+# Cartesian monomial (slm) indices per angular momentum, src/vmc/slm.f90:22-28
+SLM_OF_L = {0: [1], 1: [2, 3, 4], 2: [5, 6, 7, 8, 9, 10], 3: list(range(11, 21))}
+
+
+# --------------------------------------------------------------------------
+# radial tables
+def gnorm(alpha: float, l: int) -> float:
+    """Primitive normalisation, src/module/m_basis.f90:515-536."""
+    pi = 4.0 * math.atan(1.0)
+    q = 1.0 / (pi ** 0.25)
+    if l == 0:
+        return (2.0 * alpha) ** 0.75 * 2.0 * q
+    if l == 1:
+        return (2.0 * alpha) ** 1.25 * math.sqrt(8.0 / 3.0) * q
+    if l == 2:
+        return (2.0 * alpha) ** 1.75 * math.sqrt(16.0 / 15.0) * q
+    if l == 3:
+        return (2.0 * alpha) ** 2.25 * math.sqrt(32.0 / 105.0) * q
+    if l == 4:
+        return (2.0 * alpha) ** 2.75 * math.sqrt(64.0 / 945.0) * q
+    return 1.0
+
+
+def grid3(nr: int = 2000, arg: float = 1.003, rmax: float = 20.0):
+    """Shifted exponential grid, read_data_trexio.f90:991-1010."""
+    r0 = rmax / (arg ** (nr - 1) - 1)
+    x = np.array([r0 * arg ** i - r0 for i in range(nr)])
+    return r0, x
+
+
+def exp_fit(x: np.ndarray, y: np.ndarray):
+    """Least-squares fit y = a exp(-b x) on log|y|, src/vmc/fitting.f90:11-74."""
+    n = len(x)
+    w = np.log(np.abs(y))
+    x_sum = float(np.sum(x))
+    w_sum = float(np.sum(w))
+    xw_sum = float(np.sum(x * w))
+    xx_sum = float(np.sum(x * x))
+    x_mean, w_mean = x_sum / n, w_sum / n
+    denom = xx_sum - n * x_mean * x_mean
+    b = (n * x_mean * w_mean - xw_sum) / denom
+    a = math.exp((w_mean * xx_sum - x_mean * xw_sum) / denom)
+    if y[0] < 0:
+        a = -a
+    return a, b
+
+
+def spline2(x: np.ndarray, y: np.ndarray, yp1: float, ypn: float) -> np.ndarray:
+    """Second-derivative table of the cubic spline, src/vmc/spline2.f90:3-80."""
+    n = len(x)
+    y2 = np.zeros(n)
+    work = np.zeros(n)
+    if yp1 > 1e30:
+        y2[0] = 0.0
+        work[0] = 0.0
+    else:
+        y2[0] = -0.5
+        work[0] = (3.0 / (x[1] - x[0])) * ((y[1] - y[0]) / (x[1] - x[0]) - yp1)
+    xl, yl = x.tolist(), y.tolist()
+    y2l, wl = y2.tolist(), work.tolist()
+    for i in range(1, n - 1):
+        sig = (xl[i] - xl[i - 1]) / (xl[i + 1] - xl[i - 1])
+        p = sig * y2l[i - 1] + 2.0
+        y2l[i] = (sig - 1.0) / p
+        wl[i] = (6.0 * ((yl[i + 1] - yl[i]) / (xl[i + 1] - xl[i]) - (yl[i] - yl[i - 1])
+                        / (xl[i] - xl[i - 1])) / (xl[i + 1] - xl[i - 1]) - sig * wl[i - 1]) / p
+    if ypn > 1e30:
+        qn = 0.0
+        workn = 0.0
+    else:
+        qn = 0.5
+        workn = (3.0 / (xl[n - 1] - xl[n - 2])) * (ypn - (yl[n - 1] - yl[n - 2]) / (xl[n - 1] - xl[n - 2]))
+    y2l[n - 1] = (workn - qn * wl[n - 2]) / (qn * y2l[n - 2] + 1.0)
+    for k in range(n - 2, -1, -1):
+        y2l[k] = y2l[k] * y2l[k + 1] + wl[k]
+    return np.array(y2l)
+
+
+def tabulate_shell(x: np.ndarray, alphas: Sequence[float], coefs: Sequence[float], l: int,
+                   cutoff_rmax: float = 1.0e-12):
+    """One radial shell on the grid: rwf, d2rwf, ce, ae, rmaxwf
+    (read_data_trexio.f90:1055-1179, icusp=0 branch)."""
+    r2 = x * x
+    rwf = np.zeros_like(x)
+    for a, cf in zip(alphas, coefs):
+        rwf = rwf + gnorm(a, l) * cf * np.exp(-a * r2)
+    # small-r polynomial through the first NCOEF nodes (dgesv on a Vandermonde)
+    V = np.array([[x[i] ** j for j in range(NCOEF)] for i in range(NCOEF)])
+    ce = np.linalg.solve(V, rwf[:NCOEF].copy())
+    dwf1 = 0.0
+    for icoef in range(2, NCOEF + 1):
+        dwf1 += (icoef - 1) * ce[icoef - 1] * x[0] ** (icoef - 2)
+    nr = len(x)
+    rmax = x[-1]
+    for ir in range(nr - 1, -1, -1):
+        if abs(rwf[ir]) > cutoff_rmax:
+            rmax = x[ir]
+            break
+    ae = np.zeros(2)
+    if abs(rmax - x[-1]) < 1.0e-10:
+        a, b = exp_fit(x[nr - 10:], rwf[nr - 10:])
+        if b < 0:
+            raise ValueError("BASIS_READ_NUM: ak<0")
+        ae[:] = (a, b)
+        rmax = -math.log(cutoff_rmax / abs(a)) / b
+        dwfn = -b * a * math.exp(-b * x[-1])
+    else:
+        dwfn = 0.0
+    d2 = spline2(x, rwf, dwf1, dwfn)
+    return rwf, d2, ce, ae, rmax
+
+
+# --------------------------------------------------------------------------
+# Jastrow helpers
+def nterms4(nord: int) -> int:
+    """src/vmc/jastrow4.f90:418-445."""
+    return len(een_terms(nord))
+
+
+def een_terms(nord: int):
+    """(k, l, m) of every e-e-n term in the order of jastrow4.f90:203-213."""
+    out = []
+    for n in range(2, nord + 1):
+        for k in range(n - 1, -1, -1):
+            l_hi = n - k - 2 if k == 0 else n - k
+            for l in range(l_hi, -1, -1):
+                m = (n - k - l) // 2
+                if 2 * m == n - k - l:
+                    out.append((k, l, m))
+    return out
+
+
+def cusp_fix_c(c: np.ndarray, nordc: int) -> np.ndarray:
+    """Impose the e-e-n cusp-free conditions: d matrix and pivot terms of
+    cuspinit4.f90:3-70, elimination order of cuspexact4.f90:3-60."""
+    terms = een_terms(nordc)
+    nt = len(terms)
+    neq = 2 * (nordc - 1)
+    d = np.zeros((neq + 1, nt + 1))  # 1-based
+    iwc4 = [0] * (neq + 1)
+    for i, (k, l, m) in enumerate(terms, start=1):
+        n = k + l + 2 * m
+        if k == 1 and iwc4[n - 1] == 0:
+            iwc4[n - 1] = i
+        if k == 0 and iwc4[n + nordc - 2] == 0:
+            iwc4[n + nordc - 2] = i
+        iord = l + 2 * m
+        d[iord, i] -= 2 * k
+        if l + m > 0:
+            d[nordc - 1 + k + m, i] -= (l + m)
+        if m > 0:
+            d[nordc - 1 + k + l + m, i] -= m
+    c = c.copy()
+    order = list(range(1, nordc)) + list(range(nordc + 1, neq + 1)) + ([nordc] if nordc > 1 else [])
+    for it in range(c.shape[0]):
+        cc = np.concatenate([[0.0], c[it]])
+        for i in order:
+            s = 0.0
+            for j in range(1, nt + 1):
+                if j != iwc4[i]:
+                    s += d[i, j] * cc[j]
+            cc[iwc4[i]] = -s / d[i, iwc4[i]]
+        c[it] = cc[1:]
+    return c
+
+
+# --------------------------------------------------------------------------
+def icosa12():
+    """gesqua icosahedral rule, rotqua.f90:199-253 (pi as written there)."""
+    pi = 3.14159265359
+    xq = np.zeros(12); yq = np.zeros(12); zq = np.zeros(12)
+    zq[0], zq[1] = 1.0, -1.0
+    fi0 = pi / 5.0
+    s5 = math.sqrt(5.0)
+    cstha, sntha = 1.0 / s5, 2.0 / s5
+    for j in range(1, 6):
+        rk2 = 2.0 * (j - 1)
+        xq[j + 1] = sntha * math.cos(rk2 * fi0); yq[j + 1] = sntha * math.sin(rk2 * fi0); zq[j + 1] = cstha
+        rk21 = rk2 + 1.0
+        xq[j + 6] = sntha * math.cos(rk21 * fi0); yq[j + 6] = sntha * math.sin(rk21 * fi0); zq[j + 6] = -cstha
+    return xq, yq, zq, np.full(12, 1.0 / 12.0)
+
+
+class CSystem(C.Structure):
+    """ctypes mirror of champ_system_t (include/champ_gpu.h)."""
+    _fields_ = [
+        ("nelec", C.c_int32), ("nup", C.c_int32), ("ncent", C.c_int32), ("nctype", C.c_int32),
+        ("cent", C.c_void_p), ("iwctype", C.c_void_p), ("znuc", C.c_void_p),
+        ("nr", C.c_int32), ("grid_r0", C.c_double), ("grid_arg", C.c_double),
+        ("nrbas_max", C.c_int32), ("nrbas", C.c_void_p), ("rwf", C.c_void_p), ("d2rwf", C.c_void_p),
+        ("ce", C.c_void_p), ("ae", C.c_void_p), ("rmaxwf", C.c_void_p),
+        ("nbastyp_max", C.c_int32), ("nbastyp", C.c_void_p), ("iwrwf", C.c_void_p), ("iwlbas", C.c_void_p),
+        ("nbasis", C.c_int32),
+        ("norb", C.c_int32), ("coef", C.c_void_p), ("ndet", C.c_int32), ("iworbd", C.c_void_p),
+        ("cdet", C.c_void_p),
+        ("norda", C.c_int32), ("nordb", C.c_int32), ("nordc", C.c_int32),
+        ("nspin2b", C.c_int32), ("nocuspb", C.c_int32), ("scalek", C.c_double),
+        ("a4", C.c_void_p), ("b", C.c_void_p), ("c", C.c_void_p),
+        ("nloc", C.c_int32), ("lpot", C.c_void_p), ("necp_term", C.c_void_p), ("necp_power", C.c_void_p),
+        ("ecp_coef", C.c_void_p), ("ecp_exponent", C.c_void_p),
+        ("nquad", C.c_int32), ("xq0", C.c_void_p), ("yq0", C.c_void_p), ("zq0", C.c_void_p),
+        ("wq", C.c_void_p),
+    ]
+
+
+@dataclasses.dataclass
+class System:
+    name: str
+    nelec: int
+    nup: int
+    cent: np.ndarray            # [ncent,3]
+    iwctype: np.ndarray         # [ncent] int32
+    znuc: np.ndarray            # [nctype]
+    grid_r0: float
+    grid_arg: float
+    nr: int
+    nrbas: np.ndarray           # [nctype]
+    rwf: np.ndarray             # [nctype,nrbas_max,nr]
+    d2rwf: np.ndarray
+    ce: np.ndarray              # [nctype,nrbas_max,NCOEF]
+    ae: np.ndarray              # [nctype,nrbas_max,2]
+    rmaxwf: np.ndarray          # [nctype,nrbas_max]
+    nbastyp: np.ndarray         # [nctype]
+    iwrwf: np.ndarray           # [nctype,nbastyp_max]
+    iwlbas: np.ndarray          # [nctype,nbastyp_max]
+    coef: np.ndarray            # [norb,nbasis]
+    iworbd: np.ndarray          # [ndet,nelec]
+    cdet: np.ndarray            # [ndet]
+    norda: int
+    nordb: int
+    nordc: int
+    scalek: float
+    a4: np.ndarray              # [nctype,norda+1]
+    b: np.ndarray               # [nspin2b,nordb+1]
+    c: np.ndarray               # [nctype,nterms]
+    nloc: int
+    lpot: np.ndarray            # [nctype]
+    necp_term: np.ndarray       # [nctype,MPS_L]
+    necp_power: np.ndarray      # [nctype,MPS_L,MGAUSS]
+    ecp_coef: np.ndarray
+    ecp_exponent: np.ndarray
+    xq0: np.ndarray
+    yq0: np.ndarray
+    zq0: np.ndarray
+    wq: np.ndarray
+    nspin2b: int = 1
+    nocuspb: int = 0
+    shells: Dict[int, list] = dataclasses.field(default_factory=dict)  # analytic contractions
+    # (alphas, coefs, l) per [type][shell] for checking the tables
+
+    @property
+    def ncent(self) -> int:
+        return int(self.cent.shape[0])
+
+    @property
+    def nctype(self) -> int:
+        return int(self.znuc.shape[0])
+
+    @property
+    def nbasis(self) -> int:
+        return int(sum(int(self.nbastyp[t]) for t in self.iwctype))
+
+    @property
+    def norb(self) -> int:
+        return int(self.coef.shape[0])
+
+    @property
+    def ndn(self) -> int:
+        return self.nelec - self.nup
+
+    def as_ctypes(self) -> CSystem:
+        """Contiguous typed copies (kept alive on the struct) + descriptor."""
+        keep: List[np.ndarray] = []
+
+        def p(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data
+
+        f8, i4 = np.float64, np.int32
+        s = CSystem()
+        s.nelec, s.nup, s.ncent, s.nctype = self.nelec, self.nup, self.ncent, self.nctype
+        s.cent, s.iwctype, s.znuc = p(self.cent, f8), p(self.iwctype, i4), p(self.znuc, f8)
+        s.nr, s.grid_r0, s.grid_arg = self.nr, self.grid_r0, self.grid_arg
+        s.nrbas_max = self.rwf.shape[1]
+        s.nrbas, s.rwf, s.d2rwf = p(self.nrbas, i4), p(self.rwf, f8), p(self.d2rwf, f8)
+        s.ce, s.ae, s.rmaxwf = p(self.ce, f8), p(self.ae, f8), p(self.rmaxwf, f8)
+        s.nbastyp_max = self.iwrwf.shape[1]
+        s.nbastyp, s.iwrwf, s.iwlbas = p(self.nbastyp, i4), p(self.iwrwf, i4), p(self.iwlbas, i4)
+        s.nbasis = self.nbasis
+        s.norb, s.coef = self.norb, p(self.coef, f8)
+        s.ndet, s.iworbd, s.cdet = self.iworbd.shape[0], p(self.iworbd, i4), p(self.cdet, f8)
+        s.norda, s.nordb, s.nordc = self.norda, self.nordb, self.nordc
+        s.nspin2b, s.nocuspb, s.scalek = self.nspin2b, self.nocuspb, self.scalek
+        s.a4, s.b, s.c = p(self.a4, f8), p(self.b, f8), p(self.c, f8)
+        s.nloc = self.nloc
+        s.lpot, s.necp_term, s.necp_power = p(self.lpot, i4), p(self.necp_term, i4), p(self.necp_power, i4)
+        s.ecp_coef, s.ecp_exponent = p(self.ecp_coef, f8), p(self.ecp_exponent, f8)
+        s.nquad = len(self.wq)
+        s.xq0, s.yq0, s.zq0, s.wq = p(self.xq0, f8), p(self.yq0, f8), p(self.zq0, f8), p(self.wq, f8)
+        s._keep = keep  # type: ignore[attr-defined]
+        return s
+
+    # ------------------------------------------------------------------
+    def initial_walkers(self, nwalk: int, seed: int = 7, sigma: float = 1.0) -> np.ndarray:
+        """Electrons drawn N(0, sigma) around nuclei; nucleus k hosts
+        round(Z_k) electrons, extra electrons go round-robin over nuclei;
+        up and down spins alternate over the assignment list."""
+        rng = np.random.default_rng(seed)
+        sites: List[int] = []
+        for ic in range(self.ncent):
+            sites += [ic] * int(round(float(self.znuc[self.iwctype[ic]])))
+        k = 0
+        while len(sites) < self.nelec:
+            sites.append(k % self.ncent)
+            k += 1
+        sites = sites[: self.nelec]
+        order = np.array(sites)
+        # spin assignment: interleave so both spins spread over all nuclei
+        up_sites = order[0::2]
+        dn_sites = order[1::2]
+        allsites = np.concatenate([up_sites, dn_sites])
+        nu = self.nup
+        if len(up_sites) != nu:  # uneven split: rebalance
+            allsites = np.concatenate([order[:nu], order[nu:]])
+        x = self.cent[allsites][None, :, :] + sigma * rng.standard_normal((nwalk, self.nelec, 3))
+        return np.ascontiguousarray(x)
+
+
+# --------------------------------------------------------------------------
+def _zigzag_chain(nheavy: int, nh: int, bond: float = 1.4, hbond: float = 2.0):
+    """Heavy atoms on a zig-zag chain (bond length `bond`, 120 degrees); H
+    atoms `hbond` away from their heavy atom, terminal atoms first taking an
+    axial and a second perpendicular H, interior atoms an outward one."""
+    heavy = np.array([[k * bond * math.sin(math.pi / 3), (k % 2) * bond * math.cos(math.pi / 3), 0.0]
+                      for k in range(nheavy)])
+    # hydrogen slots in fill order
+    slots = []
+    for k in range(nheavy):
+        out = -1.0 if k % 2 == 0 else 1.0
+        slots.append((k, np.array([0.0, out, 0.0])))
+    if nheavy > 1:
+        slots.append((0, np.array([-1.0, 0.0, 0.0])))
+        slots.append((nheavy - 1, np.array([1.0, 0.0, 0.0])))
+    for k in range(nheavy):
+        slots.append((k, np.array([0.0, 0.0, 1.0])))
+        slots.append((k, np.array([0.0, 0.0, -1.0])))
+    # terminal atoms get their extra H before interior ones (butadiene-like)
+    if nh > nheavy and nheavy > 1:
+        slots = [slots[nheavy], slots[nheavy + 1]] + slots[:nheavy] + slots[nheavy + 2:]
+    hs = []
+    for i in range(nh):
+        k, d = slots[i]
+        hs.append(heavy[k] + hbond * d / np.linalg.norm(d))
+    return heavy, np.array(hs).reshape(-1, 3)
+
+
+def make_system(name: str, nheavy: int, nh: int, nelec: int, heavy_shells: str, h_shells: str,
+                zheavy: float = 4.0, nloc: int = 0, seed: int = 1234, norb: int | None = None,
+                kappa: float = 0.6, norda: int = 5, nordb: int = 5, nordc: int = 4) -> System:
+    """A synthetic molecule in CHAMP's table form.
+
+    heavy_shells / h_shells: shell recipe such as "ssssppp d" or "sssspppdd"
+    (each letter one contraction of 3 primitives, exponents log-uniform in
+    [0.1, 20], contraction coefficients U(0.3, 1))."""
+    rng = np.random.default_rng(seed)
+    heavy, hyd = _zigzag_chain(nheavy, nh)
+    cent = np.concatenate([heavy, hyd]) if nh else heavy
+    iwctype = np.array([0] * nheavy + [1] * nh, dtype=np.int32)
+    znuc = np.array([zheavy, 1.0])
+    nup = (nelec + 1) // 2
+
+    r0, x = grid3()
+    recipes = [heavy_shells.replace(" ", ""), h_shells.replace(" ", "")]
+    nrbas = np.array([len(r) for r in recipes], dtype=np.int32)
+    nrmax = int(nrbas.max())
+    nr = len(x)
+    rwf = np.zeros((2, nrmax, nr)); d2rwf = np.zeros((2, nrmax, nr))
+    ce = np.zeros((2, nrmax, NCOEF)); ae = np.zeros((2, nrmax, 2)); rmaxwf = np.zeros((2, nrmax))
+    shells: Dict[int, list] = {0: [], 1: []}
+    ao_rad: List[List[int]] = [[], []]
+    ao_slm: List[List[int]] = [[], []]
+    for it in range(2):
+        for irb, ch in enumerate(recipes[it]):
+            l = SHELL_L[ch]
+            alphas = np.exp(rng.uniform(math.log(0.1), math.log(20.0), size=3))
+            coefs = rng.uniform(0.3, 1.0, size=3)
+            shells[it].append((alphas, coefs, l))
+            rwf[it, irb], d2rwf[it, irb], ce[it, irb], ae[it, irb], rmaxwf[it, irb] = tabulate_shell(x, alphas, coefs, l)
+            for lm in SLM_OF_L[l]:
+                ao_rad[it].append(irb)
+                ao_slm[it].append(lm)
+    nbastyp = np.array([len(a) for a in ao_rad], dtype=np.int32)
+    nbmax = int(nbastyp.max())
+    iwrwf = np.zeros((2, nbmax), dtype=np.int32); iwlbas = np.ones((2, nbmax), dtype=np.int32)
+    for it in range(2):
+        iwrwf[it, :nbastyp[it]] = ao_rad[it]
+        iwlbas[it, :nbastyp[it]] = ao_slm[it]
+    nbasis = int(sum(nbastyp[t] for t in iwctype))
+
+    nocc = max(nup, nelec - nup)
+    norb = norb or nocc
+    coef = rng.normal(0.0, 0.3, size=(norb, nbasis))
+    iworbd = np.zeros((1, nelec), dtype=np.int32)
+    iworbd[0, :nup] = np.arange(nup)
+    iworbd[0, nup:] = np.arange(nelec - nup)
+    cdet = np.ones(1)
+
+    a4 = rng.uniform(-0.1, 0.1, size=(2, norda + 1))
+    b = rng.uniform(-0.1, 0.1, size=(1, nordb + 1))
+    c = rng.uniform(-0.1, 0.1, size=(2, nterms4(nordc)))
+    b[0, 0] = 0.5                   # e-e cusp (antiparallel 1/2, parallel 1/4)
+    if nloc == 0:
+        a4[:, 0] = -znuc            # e-n cusp carried by the Jastrow (Gaussian orbitals)
+    c = cusp_fix_c(c, nordc)
+
+    lpot = np.ones(2, dtype=np.int32)
+    necp_term = np.zeros((2, MPS_L), dtype=np.int32)
+    necp_power = np.zeros((2, MPS_L, MGAUSS), dtype=np.int32)
+    ecp_coef = np.zeros((2, MPS_L, MGAUSS)); ecp_exponent = np.zeros((2, MPS_L, MGAUSS))
+    if nloc:
+        # BFD-style heavy atom: s and p non-local, local component last
+        # (file order local first, stored at index lpot; read_bas_num.f90:400-440)
+        lpot[0] = 3
+        loc = 2  # 0-based index of component lpot
+        necp_term[0, loc] = 3
+        necp_power[0, loc, :3] = [1, 3, 2]
+        ecp_exponent[0, loc, :3] = rng.uniform(4.0, 9.0, size=3)
+        ecp_coef[0, loc, :3] = [zheavy, zheavy * ecp_exponent[0, loc, 0], rng.uniform(-25.0, -10.0)]
+        for comp in (0, 1):
+            necp_term[0, comp] = 3
+            necp_power[0, comp, :3] = 2
+            ecp_exponent[0, comp, :3] = rng.uniform(4.0, 10.0, size=3)
+            ecp_coef[0, comp, :3] = rng.uniform(-5.0, 25.0, size=3) * (0.6 if comp else 1.0)
+        lpot[1] = 1  # hydrogen: local -Z/r only (zero Gaussian terms)
+    xq0, yq0, zq0, wq = icosa12()
+
+    return System(name=name, nelec=nelec, nup=nup, cent=cent, iwctype=iwctype, znuc=znuc,
+                  grid_r0=r0, grid_arg=1.003, nr=nr, nrbas=nrbas, rwf=rwf, d2rwf=d2rwf, ce=ce, ae=ae,
+                  rmaxwf=rmaxwf, nbastyp=nbastyp, iwrwf=iwrwf, iwlbas=iwlbas, coef=coef, iworbd=iworbd,
+                  cdet=cdet, norda=norda, nordb=nordb, nordc=nordc, scalek=kappa, a4=a4, b=b, c=c,
+                  nloc=nloc, lpot=lpot, necp_term=necp_term, necp_power=necp_power, ecp_coef=ecp_coef,
+                  ecp_exponent=ecp_exponent, xq0=xq0, yq0=yq0, zq0=zq0, wq=wq, shells=shells)
+
+
+def config(name: str, seed: int = 1234) -> System:
+    """Named workloads.
+
+    c22  : 4 heavy (Z=4) + 6 H, 22 e, all-electron, Nao = 100 (heavy 4s3p1d, H 1s1p)
+    c160 : 20 heavy (Z_eff=4, s/p non-local ECP) + 20 H, 160 e, Nao = 600
+           (heavy 4s3p2d, H 2s1p); 12-point icosahedral quadrature
+    small: 2 heavy + 2 H, 10 e, ECP - fast checks
+    """
+    if name == "c22":
+        return make_system("c22", 4, 6, 22, "sssspppd", "sp", nloc=0, seed=seed)
+    if name == "c160":
+        return make_system("c160", 20, 20, 160, "sssspppdd", "ssp", nloc=4, seed=seed)
+    if name == "small":
+        return make_system("small", 2, 2, 10, "ssppd", "sp", nloc=4, seed=seed)
+    if name == "small_ae":
+        return make_system("small_ae", 2, 2, 10, "ssppd", "sp", nloc=0, seed=seed)
+    if name == "c60":
+        return make_system("c60", 8, 6, 60, "sssspppdd", "ssp", nloc=4, seed=seed)
+    raise KeyError(name)
+
+
+def seed_from_string(cseed: str = "18123437465549275475255231234865") -> np.ndarray:
+    """The 32-character CHAMP seed read as 8 x i4 (parser.f90:315,654)."""
+    if len(cseed) < 32:
+        raise ValueError("INPUT: Random number seed is shorter than 32 characters.")
+    return np.array([int(cseed[4 * i:4 * i + 4]) for i in range(8)], dtype=np.int32)

@@ -1,0 +1,217 @@
+/*
+ * champ_gpu.h -- C-ABI of the MI355X engine for CHAMP's per-move wavefunction
+ * evaluation (walker-population batched).
+ *
+ * Every entry point acts on a whole walker population held on one GPU by an
+ * engine context.  The routine names, argument order and meaning follow the
+ * CHAMP Fortran subroutines they replace (cited per declaration, paths relative
+ * to the CHAMP source tree); the one-walker module-global state of the
+ * reference becomes per-walker device state inside the context, and every
+ * per-walker output gains a leading walker index.
+ *
+ * Host arrays are C row-major; sizes are given per declaration.  Indices of
+ * electrons, centres and orbitals are 0-based here (the Fortran side adds 1).
+ * All calls are ordered on the context's single HIP stream and return when
+ * their host outputs are written.  Errors are reported by the return code;
+ * a walker whose state went bad (singular Slater matrix, zero ratio,
+ * non-finite value) is flagged in its status word, never aborts the batch.
+ */
+#ifndef CHAMP_GPU_H
+#define CHAMP_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- limits (match CHAMP's compile-time parameters) --------------------- */
+#define CHAMP_NCOEF     5   /* small-r polynomial terms, m_vmc.f90:96          */
+#define CHAMP_MPS_L     5   /* max pseudopotential components, m_pseudo.f90:3 */
+#define CHAMP_MPS_QUAD  86  /* max quadrature points, m_pseudo.f90:4          */
+#define CHAMP_MGAUSS    8   /* max Gaussian terms per ECP component (CHAMP: 100) */
+#define CHAMP_MAX_SLM   20  /* Cartesian monomials up to f, slm.f90:22-28      */
+#define CHAMP_MAX_NORD  8   /* max Jastrow polynomial order                    */
+
+/* ---- return codes -------------------------------------------------------- */
+#define CHAMP_OK               0
+#define CHAMP_ERR_ARG         -1
+#define CHAMP_ERR_HIP         -2
+#define CHAMP_ERR_UNSUPPORTED -3
+#define CHAMP_ERR_NOMEM       -4
+#define CHAMP_ERR_STATE       -5
+
+/* ---- per-walker status bits ---------------------------------------------- */
+#define CHAMP_WALKER_SINGULAR   1  /* LU pivot was exactly zero (matinv.f90:59)   */
+#define CHAMP_WALKER_ZERO_RATIO 2  /* determinant ratio exactly 0 (determinante.f90:46) */
+#define CHAMP_WALKER_NONFINITE  4  /* NaN/Inf in psi, drift or local energy       */
+
+/* ---- precision modes ----------------------------------------------------- */
+#define CHAMP_PREC_FP64  0     /* every stage in IEEE double                      */
+#define CHAMP_PREC_MIXED 1     /* FP32 AO/Jastrow/MO, FP64 inverse updates and E_L */
+
+/*
+ * System description (read-only tables shared by all walkers).  Mirrors the
+ * module globals CHAMP fills at input time: `system` (cent, znuc, iwctype),
+ * `numbas`/`numexp`/`numbas1` (grid-3 splines, read_data_trexio.f90:991-1179),
+ * `coefs`/`slater` (MO coefficients, determinants), `jastrow` (ijas=4, isc=2;
+ * m_jastrow.f90:145-147), `gauss_ecp`/`pseudo` (nloc=4) and `qua`.
+ */
+typedef struct champ_system {
+    int32_t nelec, nup;             /* ndn = nelec - nup                          */
+    int32_t ncent, nctype;
+    const double  *cent;            /* [ncent][3]                                 */
+    const int32_t *iwctype;         /* [ncent] centre -> type (0-based)           */
+    const double  *znuc;            /* [nctype] (effective) nuclear charge        */
+
+    /* numerical radial basis on grid type 3: r_i = r0*arg^(i-1) - r0            */
+    int32_t nr;                     /* grid points (2000)                         */
+    double  grid_r0, grid_arg;      /* r0 already divided by (arg^(nr-1)-1)       */
+    int32_t nrbas_max;              /* leading dimension for radial shells        */
+    const int32_t *nrbas;           /* [nctype] radial shells per type            */
+    const double  *rwf;             /* [nctype][nrbas_max][nr]  R(r_i)            */
+    const double  *d2rwf;           /* [nctype][nrbas_max][nr]  spline R''        */
+    const double  *ce;              /* [nctype][nrbas_max][NCOEF] small-r poly    */
+    const double  *ae;              /* [nctype][nrbas_max][2]  a*exp(-b r) tail   */
+    const double  *rmaxwf;          /* [nctype][nrbas_max] cut radius             */
+    int32_t nbastyp_max;            /* leading dimension for AOs per type         */
+    const int32_t *nbastyp;         /* [nctype] AOs on a centre of this type      */
+    const int32_t *iwrwf;           /* [nctype][nbastyp_max] radial shell (0-based) */
+    const int32_t *iwlbas;          /* [nctype][nbastyp_max] slm index (1-based, 1..20) */
+    int32_t nbasis;                 /* sum over centres of nbastyp(iwctype)       */
+
+    /* orbitals and determinants                                                 */
+    int32_t norb;                   /* MOs referenced by any determinant          */
+    const double  *coef;            /* [norb][nbasis]  coef(m,iorb) of CHAMP      */
+    int32_t ndet;                   /* determinants (1 = single Slater det)       */
+    const int32_t *iworbd;          /* [ndet][nelec] orbital of electron slot (0-based) */
+    const double  *cdet;            /* [ndet] CI coefficients                     */
+
+    /* Jastrow ijas=4, isc=2                                                     */
+    int32_t norda, nordb, nordc;
+    int32_t nspin2b, nocuspb;       /* CHAMP defaults 1 and 0                     */
+    double  scalek;                 /* kappa                                      */
+    const double *a4;               /* [nctype][norda+1]                          */
+    const double *b;                /* [nspin2b][nordb+1]                         */
+    const double *c;                /* [nctype][nterms4(nordc)]                   */
+
+    /* Gaussian ECP (nloc = 4) or all-electron (nloc = 0)                        */
+    int32_t nloc;
+    const int32_t *lpot;            /* [nctype] local component index (1-based)   */
+    const int32_t *necp_term;       /* [nctype][MPS_L] terms of component l (l=1..) */
+    const int32_t *necp_power;      /* [nctype][MPS_L][MGAUSS]                    */
+    const double  *ecp_coef;        /* [nctype][MPS_L][MGAUSS]                    */
+    const double  *ecp_exponent;    /* [nctype][MPS_L][MGAUSS]                    */
+    int32_t nquad;                  /* quadrature points                          */
+    const double  *xq0, *yq0, *zq0, *wq;  /* [nquad] unrotated rule (gesqua)     */
+} champ_system_t;
+
+/* Opaque engine context: tables + walker population resident in HBM.        */
+typedef struct champ_ctx champ_ctx;
+
+/* Sweep statistics returned by the population driver.                       */
+typedef struct champ_sweep_stats {
+    double accepted;        /* accepted single-electron moves (all walkers)      */
+    double proposed;        /* proposed moves                                    */
+    double esum;            /* sum over walkers and sweeps of E_L (FP64)         */
+    double esum2;           /* sum of E_L^2                                      */
+    int64_t nflagged;       /* walkers with a non-zero status word               */
+} champ_sweep_stats;
+
+/* --------------------------------------------------------------------------
+ * Context life cycle.  Replaces the input-time uploads of CHAMP's module
+ * globals (read_data_trexio.f90:991-1179 tables, readps_gauss, set_scale_dist
+ * scale_dist.f90:3-60) by one upload of `sys` for a population of `nwalk`.
+ * ------------------------------------------------------------------------ */
+int champ_gpu_init(champ_ctx **ctx, const champ_system_t *sys, int32_t nwalk,
+                   int32_t precision);
+int champ_gpu_free(champ_ctx *ctx);
+const char *champ_gpu_last_error(void);
+
+/* Walker positions xold, [nwalk][nelec][3].  mc_configs.f90:33-150 reads the
+ * same quantity from `mc_configs_start`.                                      */
+int champ_gpu_set_walkers(champ_ctx *ctx, const double *x);
+int champ_gpu_get_walkers(champ_ctx *ctx, double *x);
+
+/* Per-walker xoshiro256** state, [nwalk][4] uint64.  seed_rng reproduces
+ * parser.f90:651-661: the 32-character seed read as 8 x i4 (setrn), then
+ * walker w takes the stream jump()ed w times (one walker per MPI rank).     */
+int champ_gpu_set_rng(champ_ctx *ctx, const uint64_t *state);
+int champ_gpu_get_rng(champ_ctx *ctx, uint64_t *state);
+int champ_gpu_seed_rng(champ_ctx *ctx, const int32_t seed[8]);
+
+/* Rotated quadrature per walker, [nwalk][nquad] each: the state rotqua
+ * (rotqua.f90:357-430) leaves in xq,yq,zq.  rotqua draws the rotation from
+ * each walker's own stream exactly as the reference does once per step.      */
+int champ_gpu_set_quadrature(champ_ctx *ctx, const double *xq, const double *yq,
+                             const double *zq);
+int champ_gpu_rotqua(champ_ctx *ctx);
+
+/* hpsi (hpsi.f90:3-273): full recompute of distances, pot_local, jastrow_factor4,
+ * determinant (orbitals + LU inverse), nonloc_pot and E_L at the current xold,
+ * refreshing every cached per-walker quantity.  Outputs per walker [nwalk]
+ * (any may be NULL): psid (Slater part), psij (log Jastrow), ekin, energy.     */
+int champ_gpu_hpsi(champ_ctx *ctx, double *psid, double *psij, double *ekin,
+                   double *energy);
+
+/* compute_determinante_grad (determinante.f90:69-283), single determinant:
+ * iflag_move=1 -> drift of electron iel at its current position,
+ * iflag_move=0 -> drift at the position last proposed by psie.
+ * vd: [nwalk][3].                                                             */
+int champ_gpu_compute_determinante_grad(champ_ctx *ctx, int32_t iel,
+                                        int32_t iflag_move, double *vd);
+
+/* psie (hpsie.f90:3-79): distances(iel), jastrowe/jastrow4e, determinante
+ * (orbitalse + Sherman-Morrison ratio) for electron iel moved to xnew
+ * ([nwalk][3]) in every walker.  psidn: new Slater value, psijn: new log
+ * Jastrow ([nwalk] each).  iflag=0 as in metrop1 (no Laplacians).            */
+int champ_gpu_psie(champ_ctx *ctx, int32_t iel, const double *xnew,
+                   double *psidn, double *psijn, int32_t iflag);
+
+/* jassav + detsav (jassav.f90:3-78, detsav.f90:3-94) for walkers with
+ * accept[w] != 0, distancese_restore (distances.f90:169-211) for the others.  */
+int champ_gpu_accept(champ_ctx *ctx, int32_t iel, const uint8_t *accept);
+
+/* metrop1 (metrop_mov1_driftdif.f90:4-363) over the whole population for
+ * nsweep sweeps: rotqua (if nloc>0), Ne drift-diffusion single-electron moves
+ * with accept/reject drawn from each walker's stream, then hpsi.  All state
+ * stays on the GPU; stats accumulate the local energies of every sweep.      */
+int champ_gpu_metrop1(champ_ctx *ctx, int32_t nsweep, double tau,
+                      champ_sweep_stats *stats);
+
+/* Per-walker state readback for checking: psido, psijo, eold, psi2o [nwalk]
+ * and the status words.                                                       */
+int champ_gpu_get_walker_scalars(champ_ctx *ctx, double *psido, double *psijo,
+                                 double *eold, double *psi2o);
+int champ_gpu_get_status(champ_ctx *ctx, int32_t *status);
+
+/* Array readback for checking (host buffer sized by champ_gpu_array_size).   */
+#define CHAMP_ARR_SLMI   1   /* [nwalk][2][nmax][nmax] transpose inverse rows  */
+#define CHAMP_ARR_VJ     2   /* [nwalk][nelec][3]  Jastrow gradient (vj)       */
+#define CHAMP_ARR_DDX    3   /* [nwalk][nelec][3]  grad ln D                   */
+#define CHAMP_ARR_D2DX2  4   /* [nwalk][nelec]     lap D / D                   */
+#define CHAMP_ARR_DETIAB 5   /* [nwalk][2]         spin determinants           */
+#define CHAMP_ARR_VPSP   6   /* [nwalk][2]         nonlocal energy per spin    */
+#define CHAMP_ARR_PELOC  7   /* [nwalk]            local potential             */
+#define CHAMP_ARR_D2J    8   /* [nwalk]            Jastrow Laplacian sum       */
+#define CHAMP_ARR_ORBN   9   /* [nwalk][4][norb]   orbn,dorbn of last psie     */
+#define CHAMP_ARR_VJN   10   /* [nwalk][nelec][3]  vjn of last psie            */
+#define CHAMP_ARR_NXQUAD 11  /* [nwalk]            quadrature points used      */
+int64_t champ_gpu_array_size(champ_ctx *ctx, int32_t which);
+int champ_gpu_get_array(champ_ctx *ctx, int32_t which, double *out);
+
+/* basis_fns (basis_fns.f90:8-214) at arbitrary points: npt positions
+ * [npt][3] -> phin [npt][5][nbasis] (value, d/dx, d/dy, d/dz, Laplacian),
+ * evaluated with ider=2.                                                      */
+int champ_gpu_basis_fns(champ_ctx *ctx, int32_t npt, const double *pos,
+                        double *phin);
+
+/* Timing of the last metrop1 call: per-kernel-class milliseconds measured
+ * with HIP events on the context stream.  names/ms sized by *n on input.      */
+int champ_gpu_kernel_times(champ_ctx *ctx, int32_t *n, char *names /* n*32 */,
+                           double *ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHAMP_GPU_H */

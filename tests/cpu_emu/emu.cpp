@@ -1,0 +1,185 @@
+// Host runtime of the CPU emulation declared in include/hip/hip_runtime.h.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <barrier>
+#include <chrono>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace emu {
+thread_local ThreadCtx tctx;
+
+struct WaveSync {
+    std::barrier<> bar;
+    uint64_t buf[64];
+    explicit WaveSync(int n) : bar(n) {}
+};
+
+struct Block {
+    std::barrier<> bar;
+    std::vector<std::unique_ptr<WaveSync>> waves;
+    std::vector<char> shmem;
+    explicit Block(int n) : bar(n) {}
+};
+
+static std::mutex g_atomic;
+static std::atomic<long long> g_launch_id{0};
+static std::atomic<long long> g_watch_id{-1};
+static std::atomic<double> g_watch_t0{0};
+static const char *g_watch_name = "";
+
+static double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static void watchdog()
+{
+    static std::once_flag once;
+    std::call_once(once, [] {
+        std::thread([] {
+            const char *env = getenv("EMU_TIMEOUT");
+            const double lim = env ? atof(env) : 120.0;
+            for (;;) {
+                std::this_thread::sleep_for(std::chrono::milliseconds(500));
+                if (g_watch_id.load() >= 0 && now_s() - g_watch_t0.load() > lim) {
+                    fprintf(stderr, "EMU: kernel %s did not finish within %.0f s (deadlock or endless loop)\n",
+                            g_watch_name, lim);
+                    fflush(stderr);
+                    std::abort();
+                }
+            }
+        }).detach();
+    });
+}
+
+void *dyn_shared() { return ((Block *)tctx.block)->shmem.data(); }
+
+void syncthreads() { ((Block *)tctx.block)->bar.arrive_and_wait(); }
+
+void wave_exchange(uint64_t v, uint64_t *all)
+{
+    Block *b = (Block *)tctx.block;
+    WaveSync &w = *b->waves[tctx.wave];
+    w.buf[tctx.lane] = v;
+    w.bar.arrive_and_wait();
+    memcpy(all, w.buf, sizeof(w.buf));
+    w.bar.arrive_and_wait();
+}
+
+int atomic_add_i(int *p, int v) { std::lock_guard<std::mutex> g(g_atomic); int o = *p; *p = o + v; return o; }
+double atomic_add_d(double *p, double v) { std::lock_guard<std::mutex> g(g_atomic); double o = *p; *p = o + v; return o; }
+int atomic_or_i(int *p, int v) { std::lock_guard<std::mutex> g(g_atomic); int o = *p; *p = o | v; return o; }
+
+void run_grid(const char *name, dim3 grid, dim3 block, size_t shmem, void (*thunk)(void *), void *ctx)
+{
+    watchdog();
+    const int nt = block.x * block.y * block.z;
+    const long long id = g_launch_id++;
+    g_watch_name = name;
+    g_watch_t0 = now_s();
+    g_watch_id = id;
+    for (unsigned bz = 0; bz < grid.z; bz++)
+        for (unsigned by = 0; by < grid.y; by++)
+            for (unsigned bx = 0; bx < grid.x; bx++) {
+                Block B(nt);
+                B.shmem.assign(shmem ? shmem : 8, (char)0xFF);  // poisoned like uninitialised LDS
+                const int nw = (nt + 63) / 64;
+                for (int wv = 0; wv < nw; wv++) B.waves.emplace_back(new WaveSync(std::min(64, nt - 64 * wv)));
+                std::vector<std::thread> th;
+                th.reserve(nt);
+                for (int t = 0; t < nt; t++) {
+                    th.emplace_back([&, t] {
+                        tctx.tid = dim3(t % block.x, (t / block.x) % block.y, t / (block.x * block.y));
+                        tctx.bid = dim3(bx, by, bz);
+                        tctx.bdim = block;
+                        tctx.gdim = grid;
+                        tctx.lane = t % 64;
+                        tctx.wave = t / 64;
+                        tctx.block = &B;
+                        thunk(ctx);
+                        // a finished thread leaves every barrier it could have joined
+                        B.waves[t / 64]->bar.arrive_and_drop();
+                        B.bar.arrive_and_drop();
+                    });
+                }
+                for (auto &x : th) x.join();
+            }
+    g_watch_id = -1;
+}
+}  // namespace emu
+
+hipError_t hipMalloc(void **p, size_t n)
+{
+    *p = malloc(n ? n : 1);
+    if (!*p) return hipErrorOutOfMemory;
+    memset(*p, 0xFF, n);  // NaN pattern for doubles: use-before-write shows up
+    return hipSuccess;
+}
+hipError_t hipFree(void *p) { free(p); return hipSuccess; }
+hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { memcpy(d, s, n); return hipSuccess; }
+hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t) { memcpy(d, s, n); return hipSuccess; }
+hipError_t hipMemset(void *d, int v, size_t n) { memset(d, v, n); return hipSuccess; }
+hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t) { memset(d, v, n); return hipSuccess; }
+hipError_t hipStreamCreate(hipStream_t *s) { *s = (hipStream_t)malloc(1); return hipSuccess; }
+hipError_t hipStreamDestroy(hipStream_t s) { free(s); return hipSuccess; }
+hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipDeviceSynchronize() { return hipSuccess; }
+hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t, int) { *v = 2; return hipSuccess; }
+hipError_t hipFuncSetAttribute(const void *, hipFuncAttribute, int) { return hipSuccess; }
+hipError_t hipGetLastError() { return hipSuccess; }
+const char *hipGetErrorString(hipError_t e) { return e == hipSuccess ? "success" : "emulated error"; }
+struct emu_event { double t; };
+hipError_t hipEventCreate(hipEvent_t *e) { *e = new emu_event{0}; return hipSuccess; }
+hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { e->t = emu::now_s(); return hipSuccess; }
+hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) { *ms = (float)(1e3 * (b->t - a->t)); return hipSuccess; }
+
+// ---------------------------------------------------------------- rocBLAS
+#include <rocblas/rocblas.h>
+
+rocblas_status rocblas_create_handle(rocblas_handle *h) { *h = (rocblas_handle)malloc(1); return rocblas_status_success; }
+rocblas_status rocblas_destroy_handle(rocblas_handle h) { free(h); return rocblas_status_success; }
+rocblas_status rocblas_set_stream(rocblas_handle, hipStream_t) { return rocblas_status_success; }
+
+static void gemm_cm(rocblas_operation ta, rocblas_operation tb, int m, int n, int k, double alpha, const double *A,
+                    int lda, const double *B, int ldb, double beta, double *C, int ldc)
+{
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++) {
+            double s = 0;
+            for (int l = 0; l < k; l++) {
+                const double a = ta == rocblas_operation_none ? A[i + (size_t)l * lda] : A[l + (size_t)i * lda];
+                const double b = tb == rocblas_operation_none ? B[l + (size_t)j * ldb] : B[j + (size_t)l * ldb];
+                s += a * b;
+            }
+            double &c = C[i + (size_t)j * ldc];
+            c = beta == 0.0 ? alpha * s : alpha * s + beta * c;
+        }
+}
+
+rocblas_status rocblas_dgemm(rocblas_handle, rocblas_operation ta, rocblas_operation tb, rocblas_int m, rocblas_int n,
+                             rocblas_int k, const double *alpha, const double *A, rocblas_int lda, const double *B,
+                             rocblas_int ldb, const double *beta, double *C, rocblas_int ldc)
+{
+    if (m < 0 || n < 0 || k < 0) return rocblas_status_invalid_size;
+    if (lda < (ta == rocblas_operation_none ? m : k) || ldb < (tb == rocblas_operation_none ? k : n) || ldc < m)
+        return rocblas_status_invalid_size;
+    gemm_cm(ta, tb, m, n, k, *alpha, A, lda, B, ldb, *beta, C, ldc);
+    return rocblas_status_success;
+}
+
+rocblas_status rocblas_dgemm_strided_batched(rocblas_handle, rocblas_operation ta, rocblas_operation tb, rocblas_int m,
+                                             rocblas_int n, rocblas_int k, const double *alpha, const double *A,
+                                             rocblas_int lda, rocblas_stride sa, const double *B, rocblas_int ldb,
+                                             rocblas_stride sb, const double *beta, double *C, rocblas_int ldc,
+                                             rocblas_stride sc, rocblas_int batch)
+{
+    for (int b = 0; b < batch; b++)
+        gemm_cm(ta, tb, m, n, k, *alpha, A + b * sa, lda, B + b * sb, ldb, *beta, C + b * sc, ldc);
+    return rocblas_status_success;
+}

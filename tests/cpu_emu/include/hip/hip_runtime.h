@@ -1,0 +1,126 @@
+// CPU emulation of the subset of HIP the engine uses, for checking the exact
+// kernel source on the host (bounds checks with AddressSanitizer, poisoned
+// allocations, barrier/shuffle discipline).  Each block runs its threads as
+// OS threads with real barriers; blocks run one after another.  Test
+// infrastructure only: never used for the product build.
+#pragma once
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <cmath>
+#include <cstddef>
+#include <tuple>
+#include <utility>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+#define HIP_DYNAMIC_SHARED(type, var) type *var = (type *)emu::dyn_shared();
+
+using std::isfinite;
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+
+typedef enum hipError_t { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorInvalidValue = 1 } hipError_t;
+typedef enum hipMemcpyKind {
+    hipMemcpyHostToHost = 0, hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDevice = 3,
+    hipMemcpyDefault = 4
+} hipMemcpyKind;
+typedef enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 0 } hipDeviceAttribute_t;
+typedef enum hipFuncAttribute { hipFuncAttributeMaxDynamicSharedMemorySize = 0 } hipFuncAttribute;
+typedef struct emu_stream *hipStream_t;
+typedef struct emu_event *hipEvent_t;
+
+namespace emu {
+struct ThreadCtx {
+    dim3 tid, bid, bdim, gdim;
+    int lane = 0, wave = 0;
+    void *block = nullptr;
+};
+extern thread_local ThreadCtx tctx;
+void *dyn_shared();
+void syncthreads();
+void wave_exchange(uint64_t v, uint64_t *all);  // every lane of the wave deposits, all read
+int atomic_add_i(int *p, int v);
+double atomic_add_d(double *p, double v);
+int atomic_or_i(int *p, int v);
+void run_grid(const char *name, dim3 grid, dim3 block, size_t shmem, void (*thunk)(void *), void *ctx);
+
+template <typename T>
+inline T shfl_generic(T v, int src, int width)
+{
+    static_assert(sizeof(T) <= 8, "shuffle of at most 8 bytes");
+    uint64_t u = 0;
+    memcpy(&u, &v, sizeof(T));
+    uint64_t all[64];
+    wave_exchange(u, all);
+    const int base = (tctx.lane / width) * width;
+    T r;
+    memcpy(&r, &all[base + (src % width + width) % width], sizeof(T));
+    return r;
+}
+
+template <typename F, typename Tup>
+struct Thunk {
+    F f;
+    Tup args;
+    static void call(void *p)
+    {
+        auto *t = (Thunk *)p;
+        std::apply(t->f, t->args);
+    }
+};
+}  // namespace emu
+
+#define threadIdx (emu::tctx.tid)
+#define blockIdx (emu::tctx.bid)
+#define blockDim (emu::tctx.bdim)
+#define gridDim (emu::tctx.gdim)
+
+inline void __syncthreads() { emu::syncthreads(); }
+template <typename T>
+inline T __shfl(T v, int src, int width = 64) { return emu::shfl_generic(v, src, width); }
+template <typename T>
+inline T __shfl_xor(T v, int mask, int width = 64) { return emu::shfl_generic(v, (emu::tctx.lane % width) ^ mask, width); }
+inline int atomicAdd(int *p, int v) { return emu::atomic_add_i(p, v); }
+inline double atomicAdd(double *p, double v) { return emu::atomic_add_d(p, v); }
+inline int atomicOr(int *p, int v) { return emu::atomic_or_i(p, v); }
+
+hipError_t hipMalloc(void **p, size_t n);
+template <typename T>
+inline hipError_t hipMalloc(T **p, size_t n) { return hipMalloc((void **)p, n); }
+hipError_t hipFree(void *p);
+hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind k);
+hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind k, hipStream_t st = nullptr);
+hipError_t hipMemset(void *d, int v, size_t n);
+hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t st = nullptr);
+hipError_t hipStreamCreate(hipStream_t *s);
+hipError_t hipStreamDestroy(hipStream_t s);
+hipError_t hipStreamSynchronize(hipStream_t s);
+hipError_t hipDeviceSynchronize();
+hipError_t hipGetDevice(int *d);
+hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int dev);
+hipError_t hipFuncSetAttribute(const void *f, hipFuncAttribute a, int v);
+hipError_t hipGetLastError();
+const char *hipGetErrorString(hipError_t e);
+hipError_t hipEventCreate(hipEvent_t *e);
+hipError_t hipEventDestroy(hipEvent_t e);
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = nullptr);
+hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
+
+#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...)                                   \
+    do {                                                                                              \
+        auto emu_args_ = std::make_tuple(__VA_ARGS__);                                                \
+        emu::Thunk<decltype(&kernel), decltype(emu_args_)> emu_t_{&kernel, emu_args_};                 \
+        emu::run_grid(#kernel, dim3(grid), dim3(block), (size_t)(shmem),                              \
+                      &emu::Thunk<decltype(&kernel), decltype(emu_args_)>::call, &emu_t_);             \
+    } while (0)
