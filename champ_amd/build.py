@@ -1,7 +1,7 @@
 """Build the engine's shared library (libchamp_gpu.so) for gfx950.
 
-hipcc cross-compiles the device code without a GPU; the library links
-rocBLAS for the population-wide AO->MO GEMMs.  Run as
+hipcc cross-compiles the device code without a GPU; the library depends on
+the HIP runtime only (every contraction is an engine kernel).  Run as
 ``python -m champ_amd.build`` or through ``__graft_entry__.build()``.
 """
 from __future__ import annotations
@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libchamp_gpu.so")
 OBJDIR = os.path.join(HERE, "_build")
-SOURCES = ["engine_api.hip", "engine_kernels.hip", "engine_hpsi.hip"]
+SOURCES = ["engine_api.hip", "engine_kernels.hip", "engine_hpsi.hip", "engine_gemm.hip"]
 ARCH = os.environ.get("CHAMP_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -42,8 +42,7 @@ def build(verbose: bool = True) -> str:
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
-           "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")

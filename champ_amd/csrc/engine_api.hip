@@ -135,8 +135,6 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
     c->NQ = s->nloc ? s->nquad : 0; c->nloc = s->nloc; c->prec = precision; c->cdet0 = s->cdet[0];
     if (c->NM > 400) { set_error("init: more than 400 electrons of one spin"); delete c; *ctxp = nullptr; return CHAMP_ERR_UNSUPPORTED; }
     if (hipStreamCreate(&c->stream) != hipSuccess) { set_error("init: stream"); delete c; *ctxp = nullptr; return CHAMP_ERR_HIP; }
-    if (rocblas_create_handle(&c->blas) != rocblas_status_success) { set_error("init: rocblas"); champ_gpu_free(c); *ctxp = nullptr; return CHAMP_ERR_HIP; }
-    rocblas_set_stream(c->blas, c->stream);
 
     const int NE = c->NE, NC = c->NC, NB = c->NB, NO = c->NO, W = c->W, NM = c->NM, NCT = c->NCT;
     // ---- pecent (pot_nn, pot.f90:3-50)
@@ -325,7 +323,6 @@ extern "C" int champ_gpu_free(champ_ctx *c)
                     c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->nxquad};
     for (void *p : ptrs) if (p) hipFree(p);
     for (auto &t : c->ev_pool) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
-    if (c->blas) rocblas_destroy_handle(c->blas);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return CHAMP_OK;

@@ -691,7 +691,6 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
     hipLaunchKernelGGL(k_hpsi_jastrow, dim3(c->W), dim3(256), 0, c->stream, a);
     tend(c, slot);
     HIPCHECK(hipGetLastError());
-    const double one = 1.0, zero = 0.0;
     for (int w0 = 0; w0 < c->W; w0 += c->hpsi_chunk) {
         a.w0 = w0;
         a.nw = std::min(c->hpsi_chunk, c->W - w0);
@@ -701,11 +700,10 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
         tend(c, slot);
         HIPCHECK(hipGetLastError());
         tbegin(c, K_HPSI_GEMM, &slot);
-        rocblas_status st = rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->NO,
-                                          (int)(nwave * 5), c->NB, &one, c->coef, c->NB, c->aoall, c->NB, &zero,
-                                          c->orball, c->NO);
+        int rcg = launch_gemm(c, true, (int)(nwave * 5), c->NO, c->NB, c->aoall, c->NB, 0, c->coef, c->NB, 0,
+                              c->orball, c->NO, 0, 1);
         tend(c, slot);
-        if (st != rocblas_status_success) { set_error("rocblas_dgemm (orbitals) failed"); return CHAMP_ERR_HIP; }
+        if (rcg) return rcg;
         const int nmx = std::max(c->NUP, c->NDN);
         const size_t lds = (size_t)(nmx * (nmx + 1) + nmx) * sizeof(double);
         if (lds > 64 * 1024) HIPCHECK(hipFuncSetAttribute((const void *)k_hpsi_slater, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -725,11 +723,10 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
         for (int iab = 0; iab < 2; iab++) {
             const int nel = iab ? c->NDN : c->NUP, ish = iab ? c->NUP : 0;
             if (nel == 0) continue;
-            rocblas_status st = rocblas_dgemm_strided_batched(
-                c->blas, rocblas_operation_none, rocblas_operation_none, c->NB, nel, nel, &one, c->cocc[iab], c->NB, 0,
-                c->slmi + (size_t)iab * c->NM * c->NM, c->NM, (rocblas_stride)2 * c->NM * c->NM, &zero,
-                c->wmat + (size_t)ish * c->NB, c->NB, (rocblas_stride)c->NE * c->NB, c->W);
-            if (st != rocblas_status_success) { set_error("rocblas_dgemm_strided_batched (wmat) failed"); return CHAMP_ERR_HIP; }
+            int rcg = launch_gemm(c, false, nel, c->NB, nel, c->slmi + (size_t)iab * c->NM * c->NM, c->NM,
+                                  (long long)2 * c->NM * c->NM, c->cocc[iab], c->NB, 0, c->wmat + (size_t)ish * c->NB,
+                                  c->NB, (long long)c->NE * c->NB, c->W);
+            if (rcg) return rcg;
         }
         int dev = 0, ncu = 256;
         hipGetDevice(&dev);

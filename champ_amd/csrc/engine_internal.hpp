@@ -2,7 +2,6 @@
 // shared by the kernel translation units and the C-ABI.
 #pragma once
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <string>
 #include <vector>
@@ -39,7 +38,6 @@ struct champ_ctx {
     int nloc = 0, prec = 0, hpsi_chunk = 0, pair_cap = 0;
     double pecent = 0, cdet0 = 1;
     hipStream_t stream = nullptr;
-    rocblas_handle blas = nullptr;
 
     // ---- tables
     double *cent = nullptr;          // [NC][3]
@@ -119,6 +117,8 @@ int launch_drift_old(champ_ctx *c, int iel);
 int launch_hpsi(champ_ctx *c);
 int launch_rotqua(champ_ctx *c);
 int launch_basis_points(champ_ctx *c, int npt, const double *dpos, double *dout);
+int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int lda, long long sA, const double *B,
+                int ldb, long long sB, double *Cm, int ldc, long long sC, int batch);
 void set_error(const std::string &s);
 void tbegin(champ_ctx *c, int cls, int *slot);
 void tend(champ_ctx *c, int slot);

@@ -493,14 +493,11 @@ int launch_move_a(champ_ctx *c, int iel, int mode)
 int launch_move_gemm(champ_ctx *c)
 {
     // orbn[w][comp][o] = sum_m aomove[w][comp][m] * coef[o][m]
-    const double one = 1.0, zero = 0.0;
     int slot;
     tbegin(c, K_MOVE_GEMM, &slot);
-    rocblas_status st = rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->NO, 4 * c->W,
-                                      c->NB, &one, c->coef, c->NB, c->aomove, c->NB, &zero, c->orbn, c->NO);
+    int rc = launch_gemm(c, true, 4 * c->W, c->NO, c->NB, c->aomove, c->NB, 0, c->coef, c->NB, 0, c->orbn, c->NO, 0, 1);
     tend(c, slot);
-    if (st != rocblas_status_success) { set_error("rocblas_dgemm (move) failed"); return CHAMP_ERR_HIP; }
-    return CHAMP_OK;
+    return rc;
 }
 
 int launch_move_b(champ_ctx *c, int iel, int mode)

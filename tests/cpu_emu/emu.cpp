@@ -139,47 +139,3 @@ hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { e->t = emu::now_s(); return hipSuccess; }
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) { *ms = (float)(1e3 * (b->t - a->t)); return hipSuccess; }
 
-// ---------------------------------------------------------------- rocBLAS
-#include <rocblas/rocblas.h>
-
-rocblas_status rocblas_create_handle(rocblas_handle *h) { *h = (rocblas_handle)malloc(1); return rocblas_status_success; }
-rocblas_status rocblas_destroy_handle(rocblas_handle h) { free(h); return rocblas_status_success; }
-rocblas_status rocblas_set_stream(rocblas_handle, hipStream_t) { return rocblas_status_success; }
-
-static void gemm_cm(rocblas_operation ta, rocblas_operation tb, int m, int n, int k, double alpha, const double *A,
-                    int lda, const double *B, int ldb, double beta, double *C, int ldc)
-{
-    for (int j = 0; j < n; j++)
-        for (int i = 0; i < m; i++) {
-            double s = 0;
-            for (int l = 0; l < k; l++) {
-                const double a = ta == rocblas_operation_none ? A[i + (size_t)l * lda] : A[l + (size_t)i * lda];
-                const double b = tb == rocblas_operation_none ? B[l + (size_t)j * ldb] : B[j + (size_t)l * ldb];
-                s += a * b;
-            }
-            double &c = C[i + (size_t)j * ldc];
-            c = beta == 0.0 ? alpha * s : alpha * s + beta * c;
-        }
-}
-
-rocblas_status rocblas_dgemm(rocblas_handle, rocblas_operation ta, rocblas_operation tb, rocblas_int m, rocblas_int n,
-                             rocblas_int k, const double *alpha, const double *A, rocblas_int lda, const double *B,
-                             rocblas_int ldb, const double *beta, double *C, rocblas_int ldc)
-{
-    if (m < 0 || n < 0 || k < 0) return rocblas_status_invalid_size;
-    if (lda < (ta == rocblas_operation_none ? m : k) || ldb < (tb == rocblas_operation_none ? k : n) || ldc < m)
-        return rocblas_status_invalid_size;
-    gemm_cm(ta, tb, m, n, k, *alpha, A, lda, B, ldb, *beta, C, ldc);
-    return rocblas_status_success;
-}
-
-rocblas_status rocblas_dgemm_strided_batched(rocblas_handle, rocblas_operation ta, rocblas_operation tb, rocblas_int m,
-                                             rocblas_int n, rocblas_int k, const double *alpha, const double *A,
-                                             rocblas_int lda, rocblas_stride sa, const double *B, rocblas_int ldb,
-                                             rocblas_stride sb, const double *beta, double *C, rocblas_int ldc,
-                                             rocblas_stride sc, rocblas_int batch)
-{
-    for (int b = 0; b < batch; b++)
-        gemm_cm(ta, tb, m, n, k, *alpha, A + b * sa, lda, B + b * sb, ldb, *beta, C + b * sc, ldc);
-    return rocblas_status_success;
-}
