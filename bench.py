@@ -121,7 +121,7 @@ def main():
                          f"thread, {tc:.1f} s"}
 
     out = {
-        "metric": "walker-moves/sec at 160 e- on 1 MI355X (FP64 GPU path)",
+        "metric": "walker-moves/sec at 160 e\u207b on 1 MI355X, mixed vs FP64 GPU; \u0394E_L vs FP64",
         "value": round(value, 1),
         "unit": "walker-moves/s",
         "n_gpus": 1,
@@ -135,7 +135,8 @@ def main():
         "data": "synthetic (seeded generator: geometry, basis, MOs, Jastrow, ECP; walkers Gaussian around nuclei)",
         "config": {"workload": f"metrop1 sweep, {sy.nelec} e, {sy.ncent} nuclei, {sy.nbasis} AOs, ECP + 12-pt "
                                f"quadrature, {W} walkers", "walkers": W, "electrons": sy.nelec,
-                   "nuclei": sy.ncent, "aos": sy.nbasis, "precision": "fp64", "parallelism": "walkers on 1 GPU"},
+                   "nuclei": sy.ncent, "aos": sy.nbasis, "precision": "fp64", "path": "plain FP64 GPU path",
+                   "parallelism": "walkers on 1 GPU"},
         "walker_sweeps_per_s": round(W * args.steps / wall, 2),
         "acceptance": round(st["accepted"] / st["proposed"], 4),
         "roofline": roof,

@@ -52,9 +52,30 @@ def build(verbose: bool = True) -> str:
     return OUT
 
 
+FDIR = os.path.join(HERE, "fortran")
+FEXE = os.path.join(FDIR, "vmc_gpu")
+
+
+def build_fortran(verbose: bool = True) -> str:
+    """The Fortran binding module and population VMC driver (amdflang), linked
+    against libchamp_gpu.so next to it."""
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    moddir = os.path.join(OBJDIR, "fmod")
+    os.makedirs(moddir, exist_ok=True)
+    cmd = [fc, "-O2", "-J", moddir, os.path.join(FDIR, "champ_gpu_mod.f90"), os.path.join(FDIR, "vmc_gpu.f90"),
+           "-L", HERE, "-lchamp_gpu", "-Wl,-rpath,$ORIGIN/..", "-o", FEXE]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed:\n{r.stderr[-4000:]}")
+    if verbose:
+        print(f"built {FEXE}")
+    return FEXE
+
+
 if __name__ == "__main__":
     try:
         build()
+        build_fortran()
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

@@ -442,7 +442,7 @@ __global__ void __launch_bounds__(256) k_hpsi_vps(HArgs a)
 
 __global__ void k_pair_scan(int W, const int *count, int *off, int *total)
 {
-    // single block: exclusive scan over walkers, reset the work counter
+    // single block: exclusive scan of the per-walker pair counts
     __shared__ int part[1024];
     const int tid = threadIdx.x, bd = blockDim.x;
     const int per = (W + bd - 1) / bd;
@@ -450,7 +450,7 @@ __global__ void k_pair_scan(int W, const int *count, int *off, int *total)
     for (int i = 0; i < per; i++) { const int w = tid * per + i; if (w < W) s += count[w]; }
     part[tid] = s;
     __syncthreads();
-    if (tid == 0) { int acc = 0; for (int t = 0; t < bd; t++) { const int v = part[t]; part[t] = acc; acc += v; } total[0] = acc; total[1] = 0; }
+    if (tid == 0) { int acc = 0; for (int t = 0; t < bd; t++) { const int v = part[t]; part[t] = acc; acc += v; } total[0] = acc; }
     __syncthreads();
     int acc = part[tid];
     for (int i = 0; i < per; i++) { const int w = tid * per + i; if (w < W) { off[w] = acc; acc += count[w]; } }
@@ -497,12 +497,15 @@ __global__ void __launch_bounds__(256) k_hpsi_nonloc(HArgs a)
     const JasParams &jp = *a.jas;
     const EcpParams &ep = *a.ecp;
     const double kap = jp.kap;
+    // Grid-stride loop over the flattened pair list.  The pair index is a
+    // scalar (readfirstlane of the wave id) so the loop and its exit are
+    // uniform for the compiler: a lane-0 atomic broadcast with a shuffle would
+    // let the compiler split the lanes into different loop iterations.
     const int total = a.pair_total[0];
-    for (;;) {
-        int t = 0;
-        if (lane == 0) t = atomicAdd(&a.pair_total[1], 1);
-        t = __shfl(t, 0, WAVE);
-        if (t >= total) break;
+    const int nwv = blockDim.x / WAVE;
+    const int gwave = __builtin_amdgcn_readfirstlane(blockIdx.x * nwv + threadIdx.x / WAVE);
+    const int stride = gridDim.x * nwv;
+    for (int t = gwave; t < total; t += stride) {
         int lo = 0, hi = a.W - 1;
         while (lo < hi) {  // largest w with off[w] <= t
             const int mid = (lo + hi + 1) / 2;

@@ -91,6 +91,20 @@ template <typename T>
 inline T __shfl(T v, int src, int width = 64) { return emu::shfl_generic(v, src, width); }
 template <typename T>
 inline T __shfl_xor(T v, int mask, int width = 64) { return emu::shfl_generic(v, (emu::tctx.lane % width) ^ mask, width); }
+// Lane 0's value; the engine uses it only on values it holds wave-uniform, so
+// the emulation aborts if any lane of the wave disagrees.
+inline int __builtin_amdgcn_readfirstlane(int v)
+{
+    uint64_t all[64];
+    emu::wave_exchange((uint64_t)(uint32_t)v, all);
+    const int base = (emu::tctx.lane / 64) * 64;
+    for (int l = 0; l < 64; l++)
+        if (all[base + l] != all[base]) {
+            fprintf(stderr, "emu: readfirstlane of a non-uniform value\n");
+            abort();
+        }
+    return (int)(uint32_t)all[base];
+}
 inline int atomicAdd(int *p, int v) { return emu::atomic_add_i(p, v); }
 inline double atomicAdd(double *p, double v) { return emu::atomic_add_d(p, v); }
 inline int atomicOr(int *p, int v) { return emu::atomic_or_i(p, v); }

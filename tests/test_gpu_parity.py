@@ -20,12 +20,7 @@ def _hpsi_both(pr: Pair):
     return g, o
 
 
-HUNG = ("went silent twice on the GPU in the rocBLAS build of the engine (cause not shown on the CPU, "
-        "the CPU emulation of the same kernels passes); kept off the GPU for this round")
-
-
-@pytest.mark.parametrize("name,nwalk", [pytest.param("small", 6, marks=pytest.mark.skip(reason=HUNG)),
-                                        ("small_ae", 6), ("c22", 4)])
+@pytest.mark.parametrize("name,nwalk", [("small_ae", 6), ("small", 6), ("c22", 4)])
 def test_hpsi_parity(require_gpu, name, nwalk):
     pr = Pair(name, nwalk)
     pr.quad()
