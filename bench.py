@@ -46,7 +46,7 @@ def main():
     ap.add_argument("--system", default="c160")
     ap.add_argument("--tau", type=float, default=0.3)
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cpu-sweeps", type=int, default=1)
+    ap.add_argument("--cpu-sweeps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -284,6 +284,7 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
         TRY(dalloc(&c->pair_e, (size_t)W * c->pair_cap)); TRY(dalloc(&c->pair_c, (size_t)W * c->pair_cap));
         TRY(dalloc(&c->pair_vps, (size_t)W * c->pair_cap * CHAMP_MPS_L));
         TRY(dalloc(&c->pair_val, (size_t)W * c->pair_cap));
+        TRY(dalloc(&c->rrT, (size_t)W * NE * NC));
     }
     TRY(dalloc(&c->pair_count, W)); TRY(dalloc(&c->pair_off, (size_t)W + 1)); TRY(dalloc(&c->pair_total, 2));
     TRY(dalloc(&c->nxquad, W));
@@ -320,7 +321,7 @@ extern "C" int champ_gpu_free(champ_ctx *c)
                     c->rng, c->xq, c->yq, c->zq, c->xnew, c->dfus2o, c->rnew, c->rrnew, c->fsn_row, c->g1n, c->g2n,
                     c->vjnT, c->fsumn, c->aomove, c->orbn, c->ratio, c->psidn, c->psijn, c->vnew, c->vold, c->acc,
                     c->esum_dev, c->aoall, c->orball, c->wmat, c->pair_count, c->pair_off, c->pair_total,
-                    c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->nxquad};
+                    c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->rrT, c->nxquad};
     for (void *p : ptrs) if (p) hipFree(p);
     for (auto &t : c->ev_pool) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     if (c->stream) hipStreamDestroy(c->stream);

@@ -2,67 +2,12 @@
 // whole population.  distances + jastrow_factor4, AO values for all
 // electrons, the AO->MO GEMM, batched LU inverse of both Slater matrices,
 // nonloc_pot quadrature and the local-energy assembly.
-#include "engine_internal.hpp"
+#include "hpsi_args.hpp"
 
 #include <algorithm>
 #include <cmath>
 
 namespace champ {
-
-#define HB 0.5
-
-struct HArgs {
-    int W, NE, NUP, NM, NC, NB, NO, NQ, nloc, w0, nw, cap, accumulate, pad;
-    double pecent, cdet0;
-    const double *cent;
-    const int *iwctype;
-    SplineTab tab;
-    const double *rmax_ao;
-    const int *ao_cent, *ao_type, *ao_rad, *ao_slm, *iworbd;
-    const JasParams *jas;
-    const EcpParams *ecp;
-    double *xT, *r_en, *rr_en, *fso, *g1, *g2, *vjT, *fsumo, *d2o, *dorb, *slmi, *detiab, *ddx, *d2dx2;
-    double *psido, *psijo, *psi2o, *eold, *ekin, *pe_local, *vpsp;
-    double *aoall, *orball, *wmat;
-    int *pair_count, *pair_off, *pair_total, *pair_e, *pair_c, *nxquad;
-    double *pair_vps, *pair_val;
-    double *xq, *yq, *zq;
-    double *esum_dev;
-    int *status;
-};
-
-static HArgs hargs(champ_ctx *c)
-{
-    HArgs a;
-    a.W = c->W; a.NE = c->NE; a.NUP = c->NUP; a.NM = c->NM; a.NC = c->NC; a.NB = c->NB; a.NO = c->NO; a.NQ = c->NQ;
-    a.nloc = c->nloc; a.w0 = 0; a.nw = c->W; a.cap = c->pair_cap; a.accumulate = 0; a.pad = 0;
-    a.pecent = c->pecent; a.cdet0 = c->cdet0;
-    a.cent = c->cent; a.iwctype = c->iwctype; a.tab = c->tab; a.rmax_ao = c->rmax_ao;
-    a.ao_cent = c->ao_cent; a.ao_type = c->ao_type; a.ao_rad = c->ao_rad; a.ao_slm = c->ao_slm; a.iworbd = c->iworbd;
-    a.jas = c->jas; a.ecp = c->ecp;
-    a.xT = c->xT; a.r_en = c->r_en; a.rr_en = c->rr_en; a.fso = c->fso; a.g1 = c->g1; a.g2 = c->g2; a.vjT = c->vjT;
-    a.fsumo = c->fsumo; a.d2o = c->d2o; a.dorb = c->dorb; a.slmi = c->slmi; a.detiab = c->detiab; a.ddx = c->ddx;
-    a.d2dx2 = c->d2dx2; a.psido = c->psido; a.psijo = c->psijo; a.psi2o = c->psi2o; a.eold = c->eold; a.ekin = c->ekin;
-    a.pe_local = c->pe_local; a.vpsp = c->vpsp; a.aoall = c->aoall; a.orball = c->orball; a.wmat = c->wmat;
-    a.pair_count = c->pair_count; a.pair_off = c->pair_off; a.pair_total = c->pair_total; a.pair_e = c->pair_e;
-    a.pair_c = c->pair_c; a.nxquad = c->nxquad; a.pair_vps = c->pair_vps; a.pair_val = c->pair_val;
-    a.xq = c->xq; a.yq = c->yq; a.zq = c->zq; a.esum_dev = c->esum_dev; a.status = c->status;
-    return a;
-}
-
-template <typename T>
-__device__ T block_sum(T v, T *scratch)
-{
-    const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE, nwv = blockDim.x / WAVE;
-    v = wave_sum(v);
-    __syncthreads();
-    if (lane == 0) scratch[wv] = v;
-    __syncthreads();
-    T s = 0;
-    if (threadIdx.x == 0) for (int i = 0; i < nwv; i++) s += scratch[i];
-    __syncthreads();
-    return s;  // valid on thread 0
-}
 
 // ------------------------------------------------------------------------
 // distances(0) + jastrow_factor4 (jastrow4.f90:3-415, iforce_analy = 0),
@@ -436,6 +381,10 @@ __global__ void __launch_bounds__(256) k_hpsi_vps(HArgs a)
             off++;
         }
     }
+    // rr_en with the centre index innermost for the quadrature kernel
+    const double *rrw = a.rr_en + (size_t)w * NC * NE;
+    double *rrt = a.rrT + (size_t)w * NE * NC;
+    for (int t = tid; t < NE * NC; t += bd) rrt[t] = rrw[(size_t)(t % NC) * NE + t / NC];
     const double pes = block_sum(pe, red);
     if (tid == 0) a.pe_local[w] = pes;
 }
@@ -457,166 +406,7 @@ __global__ void k_pair_scan(int W, const int *count, int *off, int *total)
     if (tid == 0) off[W] = total[0];
 }
 
-// yl0 (nonloc.f90:317-340)
-__device__ __forceinline__ double yl0(int l, double c)
-{
-    if (l == 1) return 1.0;
-    if (l == 2) return 3.0 * c;
-    if (l == 3) return 2.5 * (3 * c * c - 1);
-    if (l == 4) return 3.5 * (5 * c * c * c - 3 * c);
-    return 0.0;
-}
 
-// psinl (nonlpsi.f90:4-86, ijas=4): e-e-n value with switched scalings
-__device__ __forceinline__ double psinl(const JasParams &jp, const double *c, double u, double rri, double rrj)
-{
-    const double kap = jp.kap;
-    if (rri == jp.asymp_r && rrj == jp.asymp_r) return 0.0;
-    double uu[CHAMP_NORDP + 1], ri[CHAMP_NORDP + 1], rj[CHAMP_NORDP + 1];
-    uu[0] = 1; ri[0] = 1; rj[0] = 1;
-    uu[1] = 1 - kap * u; ri[1] = 1 - kap * rri; rj[1] = 1 - kap * rrj;
-    for (int p = 2; p <= jp.nordc; p++) { uu[p] = uu[1] * uu[p - 1]; ri[p] = ri[1] * ri[p - 1]; rj[p] = rj[1] * rj[p - 1]; }
-    double psi = 0.0;
-    for (int ll = 0; ll < jp.nterms; ll++) {
-        const int k = jp.tk[ll], l = jp.tl[ll], m = jp.tm[ll];
-        const double ssl = (l == 0) ? 2.0 : ri[l] + rj[l];
-        const double ttm = (m == 0) ? 1.0 : ri[m] * rj[m];
-        psi = psi + c[ll] * uu[k] * ssl * ttm;
-    }
-    return psi;
-}
-
-// Non-local quadrature for the active (electron, centre) pairs: work queue
-// over the flattened pair list of the whole population.  Per pair and
-// point: orbitals_quad + nonlocd_quad (ratio = phi(x_q) . (coef^T slmi_i)),
-// nonlocj_quad4 (jastrow log ratio) and the vpsp_det term (nonloc.f90:248-274).
-__global__ void __launch_bounds__(256) k_hpsi_nonloc(HArgs a)
-{
-    const int lane = threadIdx.x % WAVE;
-    const int NE = a.NE, NC = a.NC, NQ = a.NQ;
-    const JasParams &jp = *a.jas;
-    const EcpParams &ep = *a.ecp;
-    const double kap = jp.kap;
-    // Grid-stride loop over the flattened pair list.  The pair index is a
-    // scalar (readfirstlane of the wave id) so the loop and its exit are
-    // uniform for the compiler: a lane-0 atomic broadcast with a shuffle would
-    // let the compiler split the lanes into different loop iterations.
-    const int total = a.pair_total[0];
-    const int nwv = blockDim.x / WAVE;
-    const int gwave = __builtin_amdgcn_readfirstlane(blockIdx.x * nwv + threadIdx.x / WAVE);
-    const int stride = gridDim.x * nwv;
-    for (int t = gwave; t < total; t += stride) {
-        int lo = 0, hi = a.W - 1;
-        while (lo < hi) {  // largest w with off[w] <= t
-            const int mid = (lo + hi + 1) / 2;
-            if (a.pair_off[mid] <= t) lo = mid; else hi = mid - 1;
-        }
-        const int w = lo, p = t - a.pair_off[w];
-        const size_t slot = (size_t)w * a.cap + p;
-        const int i = a.pair_e[slot], ic = a.pair_c[slot];
-        const int ict = a.iwctype[ic], lp = ep.lpot[ict];
-        const double *xw = a.xT + (size_t)w * 3 * NE;
-        const double *rw = a.r_en + (size_t)w * NC * NE, *rrw = a.rr_en + (size_t)w * NC * NE;
-        const double *F = a.fso + (size_t)w * NE * NE;
-        const double *Wm = a.wmat + ((size_t)w * NE + i) * a.NB;
-        const double ren = rw[(size_t)ic * NE + i], ri = 1.0 / ren;
-        const double rv0 = xw[i] - a.cent[3 * ic], rv1 = xw[NE + i] - a.cent[3 * ic + 1], rv2 = xw[2 * NE + i] - a.cent[3 * ic + 2];
-        const int iup = i < a.NUP;
-        double accum = 0.0;
-        for (int iq = 0; iq < NQ; iq++) {
-            const double xqv = a.xq[(size_t)w * NQ + iq], yqv = a.yq[(size_t)w * NQ + iq], zqv = a.zq[(size_t)w * NQ + iq];
-            double costh = rv0 * xqv + rv1 * yqv + rv2 * zqv;
-            costh = costh * ri;
-            const double q0 = ren * xqv + a.cent[3 * ic], q1 = ren * yqv + a.cent[3 * ic + 1], q2 = ren * zqv + a.cent[3 * ic + 2];
-            // distances of the point to every centre: lanes own centres c and c+64
-            double rq_r[2] = {0, 0}, rr_r[2] = {0, 0}, vx_r[2] = {0, 0}, vy_r[2] = {0, 0}, vz_r[2] = {0, 0};
-            for (int h = 0; h < 2; h++) {
-                const int c = lane + h * WAVE;
-                if (c < NC) {
-                    const double d0 = q0 - a.cent[3 * c], d1 = q1 - a.cent[3 * c + 1], d2 = q2 - a.cent[3 * c + 2];
-                    double r2 = 0;
-                    r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
-                    double r = sqrt(r2);
-                    if (c == ic) r = ren;
-                    rq_r[h] = r; rr_r[h] = scale_r(kap, r); vx_r[h] = d0; vy_r[h] = d1; vz_r[h] = d2;
-                }
-            }
-            // determinant ratio through the contracted inverse row
-            double dr = 0.0;
-            for (int mb = 0; mb < a.NB; mb += WAVE) {  // uniform trip count: shuffles need every lane
-                const int m = mb + lane < a.NB ? mb + lane : a.NB - 1;
-                const int c = a.ao_cent[m], h = c / WAVE, src = c % WAVE;
-                // both register halves are shuffled; the reader picks the half of its centre
-                const double rx0 = __shfl(vx_r[0], src, WAVE), rx1 = __shfl(vx_r[1], src, WAVE);
-                const double ry0 = __shfl(vy_r[0], src, WAVE), ry1 = __shfl(vy_r[1], src, WAVE);
-                const double rz0 = __shfl(vz_r[0], src, WAVE), rz1 = __shfl(vz_r[1], src, WAVE);
-                const double rq0 = __shfl(rq_r[0], src, WAVE), rq1 = __shfl(rq_r[1], src, WAVE);
-                const double rx = h ? rx1 : rx0, ry = h ? ry1 : ry0, rz = h ? rz1 : rz0, rr = h ? rq1 : rq0;
-                double o[5];
-                ao_eval<double>(a.tab, a.ao_type[m], a.ao_rad[m], a.ao_slm[m], rx, ry, rz, rr, a.rmax_ao[m], 0, o);
-                if (mb + lane < a.NB) dr += o[0] * Wm[m];
-            }
-            dr = wave_sum(dr);
-            // jastrow log ratio (nonlocj_quad4, nonloc.f90:732-806)
-            double fj = 0.0;
-            for (int jb = 0; jb < NE; jb += WAVE) {
-                // every lane runs the centre loop (uniform shuffles); lanes past NE discard
-                const int jj = jb + lane;
-                const bool on = jj < NE && jj != i;
-                const int jq = jj < NE ? jj : 0;
-                const int I = jq > i ? jq : i, J = jq > i ? i : jq;
-                double sspinn = 1; int isb = 0, ipar = 0;
-                if (I < a.NUP || J >= a.NUP) {
-                    if (jp.nspin2b == 2) isb = 1;
-                    else if (jp.nocuspb == 0) sspinn = 0.5;
-                    ipar = 1;
-                }
-                const double dx0 = xw[jq] - q0, dx1 = xw[NE + jq] - q1, dx2 = xw[2 * NE + jq] - q2;
-                double r2 = 0;
-                r2 = r2 + dx0 * dx0; r2 = r2 + dx1 * dx1; r2 = r2 + dx2 * dx2;
-                const double rij = sqrt(r2);
-                const double u = scale_r(kap, rij);
-                double fsn = 0.0;
-                if (u != jp.asymp_r) {
-                    const double *b = jp.b[isb];
-                    const double fee = b[0] * u / (1 + b[1] * u);
-                    fsn = sspinn * fee - jp.jb[ipar];
-                    double up = u;
-                    for (int o = 2; o <= jp.nordb; o++) { up = u * up; fsn = fsn + b[o] * up; }
-                }
-                if (jp.nordc > 1)
-                    for (int c = 0; c < NC; c++) {
-                        const double rrqc = c < WAVE ? __shfl(rr_r[0], c, WAVE) : __shfl(rr_r[1], c - WAVE, WAVE);
-                        fsn = fsn + psinl(jp, jp.c[a.iwctype[c]], u, rrqc, rrw[(size_t)c * NE + jq]);
-                    }
-                if (on) fj += fsn - F[(size_t)I * NE + J];
-            }
-            double fd = 0.0;
-            for (int h = 0; h < 2; h++) {
-                const int c = lane + h * WAVE;
-                if (c < NC) {
-                    const double rr = rr_r[h];
-                    if (rr != jp.asymp_r) {
-                        const int it = a.iwctype[c];
-                        double psi = jp.a4[it][0] * rr / (1.0 + jp.a4[it][1] * rr) - jp.ja[it];
-                        double rrip = rr;
-                        for (int o = 2; o <= jp.norda; o++) { rrip = rrip * rr; psi = psi + jp.a4[it][o] * rrip; }
-                        fd += psi;
-                    }
-                }
-            }
-            fj = wave_sum(fj);
-            fd = wave_sum(fd);
-            const double fsumn = fj + fd - F[(size_t)i * NE + i];
-            double term_radial = 0.0;
-            for (int l = 1; l <= lp - 1; l++) term_radial = term_radial + yl0(l, costh) * a.pair_vps[slot * CHAMP_MPS_L + l - 1];
-            const double trj = term_radial * ep.wq[iq] * exp(fsumn);
-            accum = accum + trj * dr;
-        }
-        if (lane == 0) a.pair_val[slot] = accum;
-        (void)iup;
-    }
-}
 
 // ------------------------------------------------------------------------
 // Local energy assembly, pot_local and psi (hpsi.f90:177-246,
@@ -731,12 +521,9 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
                                   c->NB, (long long)c->NE * c->NB, c->W);
             if (rcg) return rcg;
         }
-        int dev = 0, ncu = 256;
-        hipGetDevice(&dev);
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        hipLaunchKernelGGL(k_hpsi_nonloc, dim3(ncu * 8), dim3(256), 0, c->stream, a);
+        const int rcn = launch_nonloc(c, a);
         tend(c, slot);
-        HIPCHECK(hipGetLastError());
+        if (rcn) return rcn;
     }
     tbegin(c, K_HPSI_ELOC, &slot);
     hipLaunchKernelGGL(k_hpsi_eloc, dim3(c->W), dim3(256), 0, c->stream, a);

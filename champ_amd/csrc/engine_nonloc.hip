@@ -1,0 +1,364 @@
+// engine_nonloc.hip -- ECP quadrature of hpsi over the active (electron,
+// centre) pairs of the whole population: nonloc_pot.f90:3-108 and
+// nonloc.f90:5-315 with nonlocd_quad (:462-506) and nonlocj_quad4
+// (:656-899), single determinant.
+//
+// One 256-thread block per pair, quadrature points in blocks of QB.  Per
+// point q of a pair (i, ic) the kernel needs
+//   psi ratio   = sum_m phi_m(x_q) W[i][m],  W = slmi_i . coef (one GEMM per hpsi)
+//   log J ratio = sum_{j!=i} [b(u_qj) + sum_c P(u_qj, R_qc, R_jc)]
+//                 + sum_c a(R_qc) - sum_j F[i][j]
+// with P the e-e-n polynomial.  For a fixed (q, c) the polynomial is rewritten
+// as P = sum_k u^k sum_n A_kn(R_qc) R_jc^n (switched scaled distances), so a
+// lane that owns one centre c keeps A in registers for QB points and streams
+// over electrons j: about a dozen FMAs per (q, c, j), all indices fixed at
+// compile time by the e-e-n term table of the Jastrow order.
+//   phase A  lanes over (j, q): u_qj into LDS, e-e sums b(u_qj)
+//   phase B  lanes over (c, j-group): e-e-n and e-n sums
+//   phase C  lanes over AOs: the psi ratio
+// Reductions go lane -> wave (shuffles) -> per-wave slots in LDS -> thread 0,
+// in a fixed order, so results do not depend on timing.
+#include "hpsi_args.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace champ {
+
+// e-e-n terms (k, l, m) in the order of jastrow4.f90:203-213 and the non-zero
+// pattern of A_kn: term c u^k (Ri^l + Rj^l)(Ri Rj)^m adds c Ri^(l+m) to
+// A[k][m] and c Ri^m to A[k][l+m].
+template <int N>
+struct EenTable {
+    int n;
+    int k[40], l[40], m[40];
+    bool nz[N + 1][N + 1];
+    int nmax[N + 1];  // highest n with A[k][n] != 0, -1 if none
+    int kmax;
+    constexpr EenTable() : n(0), k{}, l{}, m{}, nz{}, nmax{}, kmax(-1)
+    {
+        for (int nn = 2; nn <= N; nn++)
+            for (int kk = nn - 1; kk >= 0; kk--) {
+                const int lhi = kk == 0 ? nn - kk - 2 : nn - kk;
+                for (int ll = lhi; ll >= 0; ll--) {
+                    const int mm = (nn - kk - ll) / 2;
+                    if (2 * mm == nn - kk - ll) {
+                        k[n] = kk; l[n] = ll; m[n] = mm; n++;
+                        nz[kk][mm] = true;
+                        nz[kk][ll + mm] = true;
+                    }
+                }
+            }
+        for (int kk = 0; kk <= N; kk++) {
+            nmax[kk] = -1;
+            for (int nn = 0; nn <= N; nn++)
+                if (nz[kk][nn]) nmax[kk] = nn;
+            if (nmax[kk] >= 0) kmax = kk;
+        }
+    }
+};
+
+// A_kn for one (point, centre): ri = switched scaled distance of the point to
+// the centre, c = e-e-n parameters of the centre's type.
+template <int N>
+__device__ __forceinline__ void een_coef(const double *c, double ri, double (&A)[N + 1][N + 1])
+{
+    constexpr EenTable<N> T{};
+    double rp[N + 1];
+    rp[0] = 1.0;
+#pragma unroll
+    for (int p = 1; p <= N; p++) rp[p] = p == 1 ? ri : rp[1] * rp[p - 1];
+#pragma unroll
+    for (int kk = 0; kk <= N; kk++)
+#pragma unroll
+        for (int nn = 0; nn <= N; nn++) A[kk][nn] = 0.0;
+#pragma unroll
+    for (int t = 0; t < T.n; t++) {
+        const double ct = c[t];
+        A[T.k[t]][T.m[t]] += ct * rp[T.l[t] + T.m[t]];
+        A[T.k[t]][T.l[t] + T.m[t]] += ct * rp[T.m[t]];
+    }
+}
+
+// P = sum_k u^k Q_k(rj), Q_k = sum_n A_kn rj^n, both by Horner's rule over the
+// non-zero pattern.
+template <int N>
+__device__ __forceinline__ double een_poly(const double (&A)[N + 1][N + 1], double u, double rj)
+{
+    constexpr EenTable<N> T{};
+    double P = 0.0;
+#pragma unroll
+    for (int kk = N; kk >= 0; kk--) {
+        if (T.kmax < 0 || kk > T.kmax) continue;
+        double Q = 0.0;
+        if (T.nmax[kk] >= 0) {
+            Q = A[kk][T.nmax[kk]];
+#pragma unroll
+            for (int nn = N - 1; nn >= 0; nn--) {
+                if (nn >= T.nmax[kk]) continue;
+                Q = T.nz[kk][nn] ? Q * rj + A[kk][nn] : Q * rj;
+            }
+        }
+        P = kk == T.kmax ? Q : P * u + Q;
+    }
+    return P;
+}
+
+// psianl (nonlpsi.f90:88-105): e-n term at scaled distance rr
+__device__ __forceinline__ double en_term(const JasParams &jp, int it, double rr)
+{
+    if (rr == jp.asymp_r) return 0.0;
+    double psi = jp.a4[it][0] * rr / (1.0 + jp.a4[it][1] * rr) - jp.ja[it];
+    double rrip = rr;
+    for (int o = 2; o <= jp.norda; o++) { rrip = rrip * rr; psi = psi + jp.a4[it][o] * rrip; }
+    return psi;
+}
+
+// psibnl (nonlpsi.f90:107-133): e-e term of electrons (i, j) at scaled u
+__device__ __forceinline__ double ee_term(const JasParams &jp, int nup, int i, int j, double u)
+{
+    if (u == jp.asymp_r) return 0.0;
+    const int I = j > i ? j : i, J = j > i ? i : j;
+    double sspinn = 1.0;
+    int isb = 0, ipar = 0;
+    if (I < nup || J >= nup) {
+        if (jp.nspin2b == 2) isb = 1;
+        else if (jp.nocuspb == 0) sspinn = 0.5;
+        ipar = 1;
+    }
+    const double *b = jp.b[isb];
+    const double fee = b[0] * u / (1 + b[1] * u);
+    double psi = sspinn * fee - jp.jb[ipar];
+    double up = u;
+    for (int o = 2; o <= jp.nordb; o++) { up = u * up; psi = psi + b[o] * up; }
+    return psi;
+}
+
+// lanes of a wave with equal (lane % QB) summed; the result is in lanes 0..QB-1
+template <int QB>
+__device__ __forceinline__ double strided_sum(double v)
+{
+#pragma unroll
+    for (int off = WAVE / 2; off >= QB; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+constexpr int NL_THREADS = 256;
+constexpr int NL_WAVES = NL_THREADS / WAVE;
+
+template <int NORDC, int QB>
+__global__ void __launch_bounds__(NL_THREADS) k_nonloc(HArgs a)
+{
+    HIP_DYNAMIC_SHARED(double, lds)
+    const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
+    const int NE = a.NE, NC = a.NC, NQ = a.NQ, NB = a.NB;
+    const JasParams &jp = *a.jas;
+    const double kap = jp.kap, asymp = jp.asymp_r;
+    double *U = lds;                        // [NE][QB]  scaled e-e distances u(q, j)
+    double *S = U + (size_t)NE * QB;        // [3][QB][NL_WAVES] per-wave sums: e-e, e-e-n + e-n, psi ratio
+    double *XQ = S + 3 * QB * NL_WAVES;     // [QB][3] quadrature points
+    double *FI = XQ + 3 * QB;               // [NL_WAVES] partial sums of F[i][.]
+
+    // XCD-aware contiguous ranges: workgroups are dealt round-robin to the 8
+    // XCDs, so logical block lb = (b % 8) * (G / 8) + b / 8 gives each XCD one
+    // contiguous eighth of the pair list (the same walkers share its L2).
+    const int G = gridDim.x, b = blockIdx.x;
+    const int lb = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+    const int total = a.pair_total[0];
+    const int per = (total + G - 1) / G;
+    const int t_lo = lb * per, t_hi = min(total, t_lo + per);
+
+    for (int t = t_lo; t < t_hi; t++) {
+        int lo = 0, hi = a.W - 1;
+        while (lo < hi) {  // walker: largest w with off[w] <= t
+            const int mid = (lo + hi + 1) / 2;
+            if (a.pair_off[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        const int w = lo;
+        const size_t slot = (size_t)w * a.cap + (t - a.pair_off[w]);
+        const int i = a.pair_e[slot], ic = a.pair_c[slot];
+        const int lp = a.ecp->lpot[a.iwctype[ic]];
+        const double *xw = a.xT + (size_t)w * 3 * NE;
+        const double *F = a.fso + ((size_t)w * NE + i) * NE;
+        const double *Wm = a.wmat + ((size_t)w * NE + i) * NB;
+        const double *rrt = a.rrT + (size_t)w * NE * NC;
+        const double cx = a.cent[3 * ic], cy = a.cent[3 * ic + 1], cz = a.cent[3 * ic + 2];
+        const double rv0 = xw[i] - cx, rv1 = xw[NE + i] - cy, rv2 = xw[2 * NE + i] - cz;
+        const double ren = a.r_en[((size_t)w * NC + ic) * NE + i], ri = 1.0 / ren;
+        const double *qx = a.xq + (size_t)w * NQ, *qy = a.yq + (size_t)w * NQ, *qz = a.zq + (size_t)w * NQ;
+
+        // sum_j F[i][j]: the old e-e and e-n values of electron i (F symmetric)
+        double fs = 0.0;
+        for (int j = tid; j < NE; j += NL_THREADS) fs += F[j];
+        fs = wave_sum(fs);
+        __syncthreads();  // LDS of the previous pair is free
+        if (lane == 0) FI[wv] = fs;
+
+        double pair_sum = 0.0;  // meaningful on thread 0
+        for (int q0 = 0; q0 < NQ; q0 += QB) {
+            if (tid < 3 * QB) {
+                const int qq = tid / 3, k = tid % 3, q = min(q0 + qq, NQ - 1);
+                const double *qv = k == 0 ? qx : (k == 1 ? qy : qz);
+                XQ[tid] = ren * qv[q] + a.cent[3 * ic + k];
+            }
+            __syncthreads();
+
+            // ---- phase A: u(q, j) and the e-e sums (lane keeps q = tid % QB)
+            {
+                const int qq = tid % QB;
+                const double px = XQ[3 * qq], py = XQ[3 * qq + 1], pz = XQ[3 * qq + 2];
+                double see = 0.0;
+                for (int j = tid / QB; j < NE; j += NL_THREADS / QB) {
+                    const double d0 = xw[j] - px, d1 = xw[NE + j] - py, d2 = xw[2 * NE + j] - pz;
+                    double r2 = 0;
+                    r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
+                    const double u = scale_r(kap, sqrt(r2));
+                    U[(size_t)j * QB + qq] = 1.0 - kap * u;
+                    if (j != i) see += ee_term(jp, a.NUP, i, j, u);
+                }
+                see = strided_sum<QB>(see);
+                if (lane < QB) S[(0 * QB + lane) * NL_WAVES + wv] = see;
+            }
+            __syncthreads();
+
+            // ---- phase B: e-e-n (lanes own a centre, stream over electrons) and e-n
+            double acc[QB];
+#pragma unroll
+            for (int qq = 0; qq < QB; qq++) acc[qq] = 0.0;
+            {
+                const int CT = NC < NL_THREADS ? NC : NL_THREADS, JG = NL_THREADS / CT;
+                for (int c0 = 0; c0 < NC; c0 += CT) {
+                    const int c = c0 + tid % CT, jg = tid / CT;
+                    if (jg < JG && c < NC) {
+                        const int it = a.iwctype[c];
+                        const double ecx = a.cent[3 * c], ecy = a.cent[3 * c + 1], ecz = a.cent[3 * c + 2];
+                        double A[QB][NORDC + 1][NORDC + 1];
+                        bool any_asym = false;
+                        bool asym[QB];
+#pragma unroll
+                        for (int qq = 0; qq < QB; qq++) {
+                            const double d0 = XQ[3 * qq] - ecx, d1 = XQ[3 * qq + 1] - ecy, d2 = XQ[3 * qq + 2] - ecz;
+                            double r2 = 0;
+                            r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
+                            const double rq = c == ic ? ren : sqrt(r2);
+                            const double rr = scale_r(kap, rq);
+                            asym[qq] = rr == asymp;
+                            any_asym = any_asym || asym[qq];
+                            if (jg == 0 && q0 + qq < NQ) acc[qq] += en_term(jp, it, rr);
+                            if constexpr (NORDC > 1) een_coef<NORDC>(jp.c[it], 1.0 - kap * rr, A[qq]);
+                        }
+                        if constexpr (NORDC > 1) {
+                            for (int j = jg; j < NE; j += JG) {
+                                if (j == i) continue;
+                                const double rrj = rrt[(size_t)j * NC + c];
+                                const double rj = 1.0 - kap * rrj;
+                                const double *uj = U + (size_t)j * QB;
+                                double P[QB];
+#pragma unroll
+                                for (int qq = 0; qq < QB; qq++) P[qq] = een_poly<NORDC>(A[qq], uj[qq], rj);
+                                if (any_asym && rrj == asymp) {  // psinl: both distances asymptotic -> 0
+#pragma unroll
+                                    for (int qq = 0; qq < QB; qq++)
+                                        if (asym[qq]) P[qq] = 0.0;
+                                }
+#pragma unroll
+                                for (int qq = 0; qq < QB; qq++) acc[qq] += P[qq];
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int qq = 0; qq < QB; qq++) {
+                const double v = wave_sum(acc[qq]);
+                if (lane == 0) S[(1 * QB + qq) * NL_WAVES + wv] = v;
+            }
+
+            // ---- phase C: psi ratio sum_m phi_m(x_q) W[i][m]
+            double dr[QB];
+#pragma unroll
+            for (int qq = 0; qq < QB; qq++) dr[qq] = 0.0;
+            for (int m = tid; m < NB; m += NL_THREADS) {
+                const int c = a.ao_cent[m];
+                const double wm = Wm[m];
+                const double ecx = a.cent[3 * c], ecy = a.cent[3 * c + 1], ecz = a.cent[3 * c + 2];
+#pragma unroll
+                for (int qq = 0; qq < QB; qq++) {
+                    const double d0 = XQ[3 * qq] - ecx, d1 = XQ[3 * qq + 1] - ecy, d2 = XQ[3 * qq + 2] - ecz;
+                    double r2 = 0;
+                    r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
+                    const double r = c == ic ? ren : sqrt(r2);
+                    double o[5];
+                    ao_eval<double>(a.tab, a.ao_type[m], a.ao_rad[m], a.ao_slm[m], d0, d1, d2, r, a.rmax_ao[m], 0, o);
+                    dr[qq] += o[0] * wm;
+                }
+            }
+#pragma unroll
+            for (int qq = 0; qq < QB; qq++) {
+                const double v = wave_sum(dr[qq]);
+                if (lane == 0) S[(2 * QB + qq) * NL_WAVES + wv] = v;
+            }
+            __syncthreads();
+
+            // ---- the vpsp_det terms of these points (nonloc.f90:248-274)
+            if (tid == 0) {
+                double fi = 0.0;
+                for (int k = 0; k < NL_WAVES; k++) fi += FI[k];
+                for (int qq = 0; qq < QB && q0 + qq < NQ; qq++) {
+                    const int q = q0 + qq;
+                    double see = 0.0, sen = 0.0, drq = 0.0;
+                    for (int k = 0; k < NL_WAVES; k++) {
+                        see += S[(0 * QB + qq) * NL_WAVES + k];
+                        sen += S[(1 * QB + qq) * NL_WAVES + k];
+                        drq += S[(2 * QB + qq) * NL_WAVES + k];
+                    }
+                    const double fsumn = see + sen - fi;
+                    double costh = rv0 * qx[q] + rv1 * qy[q] + rv2 * qz[q];
+                    costh = costh * ri;
+                    double term_radial = 0.0;
+                    for (int l = 1; l <= lp - 1; l++)
+                        term_radial = term_radial + yl0(l, costh) * a.pair_vps[slot * CHAMP_MPS_L + l - 1];
+                    const double trj = term_radial * a.ecp->wq[q] * exp(fsumn);
+                    pair_sum = pair_sum + trj * drq;
+                }
+            }
+            __syncthreads();  // S, XQ and U are rewritten by the next point block
+        }
+        if (tid == 0) a.pair_val[slot] = pair_sum;
+    }
+}
+
+template <int NORDC, int QB>
+static int launch_k(champ_ctx *c, const HArgs &a)
+{
+    const size_t lds = ((size_t)c->NE * QB + 3 * QB * NL_WAVES + 3 * QB + NL_WAVES) * sizeof(double);
+    if (lds > 160 * 1024) {
+        set_error("nonloc: too many electrons for the quadrature kernel's LDS table");
+        return CHAMP_ERR_UNSUPPORTED;
+    }
+    if (lds > 64 * 1024)
+        HIPCHECK(hipFuncSetAttribute((const void *)k_nonloc<NORDC, QB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int dev = 0, ncu = 256;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    hipLaunchKernelGGL((k_nonloc<NORDC, QB>), dim3(ncu * 8), dim3(NL_THREADS), lds, c->stream, a);
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
+int launch_nonloc(champ_ctx *c, const HArgs &a)
+{
+    switch (c->hjas.nordc) {
+    case 0: return launch_k<0, 4>(c, a);
+    case 1: return launch_k<1, 4>(c, a);
+    case 2: return launch_k<2, 4>(c, a);
+    case 3: return launch_k<3, 4>(c, a);
+    case 4: return launch_k<4, 4>(c, a);
+    case 5: return launch_k<5, 2>(c, a);
+    default:
+        set_error("nonloc: e-e-n order above 5 is not supported");
+        return CHAMP_ERR_UNSUPPORTED;
+    }
+}
+
+}  // namespace champ

@@ -1,0 +1,78 @@
+// hpsi_args.hpp -- argument block shared by the once-per-sweep (hpsi)
+// kernels and their launchers.
+#pragma once
+#include "engine_internal.hpp"
+
+namespace champ {
+
+#define HB 0.5
+
+struct HArgs {
+    int W, NE, NUP, NM, NC, NB, NO, NQ, nloc, w0, nw, cap, accumulate, pad;
+    double pecent, cdet0;
+    const double *cent;
+    const int *iwctype;
+    SplineTab tab;
+    const double *rmax_ao;
+    const int *ao_cent, *ao_type, *ao_rad, *ao_slm, *iworbd;
+    const JasParams *jas;
+    const EcpParams *ecp;
+    double *xT, *r_en, *rr_en, *fso, *g1, *g2, *vjT, *fsumo, *d2o, *dorb, *slmi, *detiab, *ddx, *d2dx2;
+    double *psido, *psijo, *psi2o, *eold, *ekin, *pe_local, *vpsp;
+    double *aoall, *orball, *wmat;
+    int *pair_count, *pair_off, *pair_total, *pair_e, *pair_c, *nxquad;
+    double *pair_vps, *pair_val;
+    double *rrT;  // [W][NE][NC] scaled e-n distances, centre index innermost
+    double *xq, *yq, *zq;
+    double *esum_dev;
+    int *status;
+};
+
+inline HArgs hargs(champ_ctx *c)
+{
+    HArgs a;
+    a.W = c->W; a.NE = c->NE; a.NUP = c->NUP; a.NM = c->NM; a.NC = c->NC; a.NB = c->NB; a.NO = c->NO; a.NQ = c->NQ;
+    a.nloc = c->nloc; a.w0 = 0; a.nw = c->W; a.cap = c->pair_cap; a.accumulate = 0; a.pad = 0;
+    a.pecent = c->pecent; a.cdet0 = c->cdet0;
+    a.cent = c->cent; a.iwctype = c->iwctype; a.tab = c->tab; a.rmax_ao = c->rmax_ao;
+    a.ao_cent = c->ao_cent; a.ao_type = c->ao_type; a.ao_rad = c->ao_rad; a.ao_slm = c->ao_slm; a.iworbd = c->iworbd;
+    a.jas = c->jas; a.ecp = c->ecp;
+    a.xT = c->xT; a.r_en = c->r_en; a.rr_en = c->rr_en; a.fso = c->fso; a.g1 = c->g1; a.g2 = c->g2; a.vjT = c->vjT;
+    a.fsumo = c->fsumo; a.d2o = c->d2o; a.dorb = c->dorb; a.slmi = c->slmi; a.detiab = c->detiab; a.ddx = c->ddx;
+    a.d2dx2 = c->d2dx2; a.psido = c->psido; a.psijo = c->psijo; a.psi2o = c->psi2o; a.eold = c->eold; a.ekin = c->ekin;
+    a.pe_local = c->pe_local; a.vpsp = c->vpsp; a.aoall = c->aoall; a.orball = c->orball; a.wmat = c->wmat;
+    a.pair_count = c->pair_count; a.pair_off = c->pair_off; a.pair_total = c->pair_total; a.pair_e = c->pair_e;
+    a.pair_c = c->pair_c; a.nxquad = c->nxquad; a.pair_vps = c->pair_vps; a.pair_val = c->pair_val;
+    a.rrT = c->rrT;
+    a.xq = c->xq; a.yq = c->yq; a.zq = c->zq; a.esum_dev = c->esum_dev; a.status = c->status;
+    return a;
+}
+
+template <typename T>
+__device__ T block_sum(T v, T *scratch)
+{
+    const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE, nwv = blockDim.x / WAVE;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) scratch[wv] = v;
+    __syncthreads();
+    T s = 0;
+    if (threadIdx.x == 0) for (int i = 0; i < nwv; i++) s += scratch[i];
+    __syncthreads();
+    return s;  // valid on thread 0
+}
+
+// yl0 (nonloc.f90:317-340)
+__device__ __forceinline__ double yl0(int l, double c)
+{
+    if (l == 1) return 1.0;
+    if (l == 2) return 3.0 * c;
+    if (l == 3) return 2.5 * (3 * c * c - 1);
+    if (l == 4) return 3.5 * (5 * c * c * c - 3 * c);
+    return 0.0;
+}
+
+// ECP quadrature over the active (electron, centre) pairs (engine_nonloc.hip)
+int launch_nonloc(champ_ctx *c, const HArgs &a);
+
+}  // namespace champ

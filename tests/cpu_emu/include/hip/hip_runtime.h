@@ -91,6 +91,10 @@ template <typename T>
 inline T __shfl(T v, int src, int width = 64) { return emu::shfl_generic(v, src, width); }
 template <typename T>
 inline T __shfl_xor(T v, int mask, int width = 64) { return emu::shfl_generic(v, (emu::tctx.lane % width) ^ mask, width); }
+// HIP's global integer min/max
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
+
 // Lane 0's value; the engine uses it only on values it holds wave-uniform, so
 // the emulation aborts if any lane of the wave disagrees.
 inline int __builtin_amdgcn_readfirstlane(int v)
