@@ -1,0 +1,143 @@
+// een_poly.hpp -- the electron-electron-nucleus part of the ijas=4 Jastrow
+// (jastrow4.f90:203-262, nonlpsi.f90:4-86) as a polynomial with its structure
+// fixed at compile time by the order nordc.
+//
+// With switched scaled distances u (electron-electron), x and y (the two
+// electrons to one centre) the term sum is
+//     P(u, x, y) = sum_t c_t u^k (x^l + y^l) (x y)^m
+//                = sum_{k,n} A_kn(x) u^k y^n,
+// where term t adds c_t x^(l+m) to A[k][m] and c_t x^m to A[k][l+m] (for l = 0
+// both land on A[k][m], giving the factor 2 of ss(0) = 2).  For one electron
+// and one centre the tables A, dA/dx and d2A/dx2 are built once; every partner
+// electron then costs a Horner evaluation over the non-zero pattern.  All
+// loop bounds and indices below are compile-time constants after unrolling.
+#pragma once
+#include "device_common.hpp"
+
+namespace champ {
+
+template <int N>
+struct EenTable {
+    int n;                    // number of terms (13 at N = 4)
+    int k[40], l[40], m[40];  // (k, l, m) in the order of jastrow4.f90:203-213
+    int idx[N + 1][N + 1];    // packed position of A[k][n], -1 where zero
+    int naz;                  // number of non-zero A entries (12 at N = 4)
+    int nmax[N + 1];          // highest non-zero power of y in row k, -1 if none
+    int kmax;                 // highest power of u, -1 if no terms
+    constexpr EenTable() : n(0), k{}, l{}, m{}, idx{}, naz(0), nmax{}, kmax(-1)
+    {
+        bool nz[N + 1][N + 1] = {};
+        for (int nn = 2; nn <= N; nn++)
+            for (int kk = nn - 1; kk >= 0; kk--) {
+                const int lhi = kk == 0 ? nn - kk - 2 : nn - kk;
+                for (int ll = lhi; ll >= 0; ll--) {
+                    const int mm = (nn - kk - ll) / 2;
+                    if (2 * mm == nn - kk - ll) {
+                        k[n] = kk; l[n] = ll; m[n] = mm; n++;
+                        nz[kk][mm] = true;
+                        nz[kk][ll + mm] = true;
+                    }
+                }
+            }
+        for (int kk = 0; kk <= N; kk++) {
+            nmax[kk] = -1;
+            for (int nn = 0; nn <= N; nn++) {
+                idx[kk][nn] = nz[kk][nn] ? naz++ : -1;
+                if (nz[kk][nn]) nmax[kk] = nn;
+            }
+            if (nmax[kk] >= 0) kmax = kk;
+        }
+    }
+};
+
+template <int N>
+constexpr int een_naz() { return EenTable<N>{}.naz > 0 ? EenTable<N>{}.naz : 1; }
+
+// Coefficient tables of one (electron, centre) at switched distance x:
+// tab[d * naz + idx] = d^d A / dx^d for d = 0..D.  c = parameters of the
+// centre's type in term order.
+template <int N, int D>
+__device__ __forceinline__ void een_tables(const double *c, double x, double *tab)
+{
+    constexpr EenTable<N> T{};
+    constexpr int NA = een_naz<N>();
+    double xp[N + 1];
+    xp[0] = 1.0;
+#pragma unroll
+    for (int p = 1; p <= N; p++) xp[p] = p == 1 ? x : xp[1] * xp[p - 1];
+    double acc[(D + 1) * NA];
+#pragma unroll
+    for (int e = 0; e < (D + 1) * NA; e++) acc[e] = 0.0;
+#pragma unroll
+    for (int t = 0; t < T.n; t++) {
+        const double ct = c[t];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int n = h == 0 ? T.m[t] : T.l[t] + T.m[t];
+            const int p = h == 0 ? T.l[t] + T.m[t] : T.m[t];
+            const int e = T.idx[T.k[t]][n];
+            acc[e] += ct * xp[p];
+            if (D >= 1 && p >= 1) acc[NA + e] += ct * (double)p * xp[p - 1];
+            if (D >= 2 && p >= 2) acc[2 * NA + e] += ct * (double)(p * (p - 1)) * xp[p - 2];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < (D + 1) * NA; e++) tab[e] = acc[e];
+}
+
+// P and its derivatives at (u, y) from the tables of x:
+//   D = 0: f;  D = 1: f, fu, fx, fy;  D = 2: also fuu, fxx, fyy, fux, fuy.
+struct EenOut {
+    double f, fu, fx, fy, fuu, fxx, fyy, fux, fuy;
+};
+
+template <int N, int D>
+__device__ __forceinline__ EenOut een_eval_tab(const double *tab, double u, double y)
+{
+    constexpr EenTable<N> T{};
+    constexpr int NA = een_naz<N>();
+    EenOut o{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // Horner in u over the row polynomials of y; the *2 names hold half the
+    // second derivative (p''/2) as the recurrence produces it.
+    double P = 0, Pu = 0, Puu2 = 0, Py = 0, Puy = 0, Pyy2 = 0, Px = 0, Pux = 0, Pxx = 0;
+#pragma unroll
+    for (int kk = N; kk >= 0; kk--) {
+        if (kk > T.kmax) continue;
+        double q = 0, q1 = 0, q2 = 0, r = 0, s = 0;  // row k: value, d/dy, (d2/dy2)/2; dA/dx row; d2A/dx2 row
+        const int nm = T.nmax[kk];
+        if (nm >= 0) {
+            q = tab[T.idx[kk][nm]];
+            if (D >= 1) r = tab[NA + T.idx[kk][nm]];
+            if (D >= 2) s = tab[2 * NA + T.idx[kk][nm]];
+#pragma unroll
+            for (int nn = N - 1; nn >= 0; nn--) {
+                if (nn >= nm) continue;
+                const int e = T.idx[kk][nn];
+                if (D >= 2) q2 = q2 * y + q1;
+                if (D >= 1) q1 = q1 * y + q;
+                q = e >= 0 ? q * y + tab[e] : q * y;
+                if (D >= 1) r = e >= 0 ? r * y + tab[NA + e] : r * y;
+                if (D >= 2) s = e >= 0 ? s * y + tab[2 * NA + e] : s * y;
+            }
+        }
+        if (kk == T.kmax) {
+            P = q; Py = q1; Pyy2 = q2; Px = r; Pxx = s;
+        } else {
+            if (D >= 2) Puu2 = Puu2 * u + Pu;
+            if (D >= 1) Pu = Pu * u + P;
+            P = P * u + q;
+            if (D >= 2) Puy = Puy * u + Py;
+            if (D >= 1) Py = Py * u + q1;
+            if (D >= 2) Pyy2 = Pyy2 * u + q2;
+            if (D >= 2) Pux = Pux * u + Px;
+            if (D >= 1) Px = Px * u + r;
+            if (D >= 2) Pxx = Pxx * u + s;
+        }
+    }
+    o.f = P;
+    if (D >= 1) { o.fu = Pu; o.fx = Px; o.fy = Py; }
+    if (D >= 2) { o.fuu = 2.0 * Puu2; o.fyy = 2.0 * Pyy2; o.fxx = Pxx; o.fux = Pux; o.fuy = Puy; }
+    return o;
+}
+
+}  // namespace champ

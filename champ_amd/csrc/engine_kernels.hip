@@ -7,6 +7,7 @@
 //   dorb[w][NE][3][NO].
 // One 64-lane wavefront serves one walker in the per-move kernels; the
 // AO->MO contractions are population-wide GEMMs.
+#include "een_poly.hpp"
 #include "engine_internal.hpp"
 
 #include <algorithm>
@@ -92,17 +93,30 @@ __global__ void __launch_bounds__(256) k_drift_old(MoveArgs a)
 //   then distances(iel) (distances.f90:3-167), jastrow4e with iflag=0
 //   (jastrow4e.f90:3-358) and basis_fns(iel, ider=1) (basis_fns.f90:8-214)
 //   writing the AO block that the MO GEMM contracts (orbitalse).
+// partner-electron slots per lane in the pair loop of k_move_a
+constexpr int MJB = 2;
+
+// LDS doubles per wave of k_move_a: r, rr, -kap (1 - kap rr) / r, rvec (3) per
+// centre, then the centre's e-e-n tables (value and d/dx)
+static size_t move_a_lds_per_wave(int NC, int nordc)
+{
+    const int naz[] = {0, 0, een_naz<2>(), een_naz<3>(), een_naz<4>(), een_naz<5>()};
+    return (size_t)6 * NC + (nordc > 1 && nordc <= 5 ? (size_t)2 * naz[nordc] * NC : 0);
+}
+
+template <int NORDC>
 __global__ void __launch_bounds__(256) k_move_a(MoveArgs a)
 {
     HIP_DYNAMIC_SHARED(double, lds)
+    constexpr int NA = een_naz<NORDC>();
     const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int w = blockIdx.x * (blockDim.x / WAVE) + wv;
     const bool active = w < a.W;
     const int NE = a.NE, NC = a.NC, iel = a.iel;
     const JasParams &jp = *a.jas;
     const double kap = jp.kap, asymp_r = jp.asymp_r;
-    double *Lr = lds + (size_t)wv * 7 * NC;
-    double *Lrr = Lr + NC, *Lsw = Lr + 2 * NC, *Ldd = Lr + 3 * NC, *Lrv = Lr + 4 * NC;
+    double *Lr = lds + (size_t)wv * (6 * NC + (NORDC > 1 ? 2 * NA * NC : 0));
+    double *Lrr = Lr + NC, *Lfm = Lr + 2 * NC, *Lrv = Lr + 3 * NC, *Ltab = Lr + 6 * NC;
 
     double xn0 = 0, xn1 = 0, xn2 = 0;
     if (active) {
@@ -135,7 +149,8 @@ __global__ void __launch_bounds__(256) k_move_a(MoveArgs a)
         }
     }
 
-    // ---- e-n distances of the moved electron + its e-n Jastrow term
+    // ---- e-n distances of the moved electron + its e-n Jastrow term; per
+    // centre the e-e-n coefficient tables at the moved electron (een_poly.hpp)
     double fs_en = 0, ge0 = 0, ge1 = 0, ge2 = 0;
     if (active) {
         for (int c = lane; c < NC; c += WAVE) {
@@ -145,10 +160,11 @@ __global__ void __launch_bounds__(256) k_move_a(MoveArgs a)
             const double r = sqrt(r2);
             const double rr = scale_r(kap, r);
             const double dd7 = 1.0 - kap * rr;
-            Lr[c] = r; Lrr[c] = rr; Lsw[c] = 1.0 - kap * rr; Ldd[c] = -kap * dd7;
+            Lr[c] = r; Lrr[c] = rr; Lfm[c] = -kap * dd7 / r;
             Lrv[3 * c] = rv0; Lrv[3 * c + 1] = rv1; Lrv[3 * c + 2] = rv2;
             a.rnew[(size_t)w * NC + c] = r;
             a.rrnew[(size_t)w * NC + c] = rr;
+            if constexpr (NORDC > 1) een_tables<NORDC, 1>(jp.c[a.iwctype[c]], 1.0 - kap * rr, Ltab + (size_t)c * 2 * NA);
             if (rr != asymp_r) {
                 const int it = a.iwctype[c];
                 const double *a4 = jp.a4[it];
@@ -183,99 +199,102 @@ __global__ void __launch_bounds__(256) k_move_a(MoveArgs a)
         double *g1n = a.g1n + (size_t)w * 3 * NE, *g2n = a.g2n + (size_t)w * 3 * NE;
         const double *rw = a.r_en + (size_t)w * NC * NE, *rrw = a.rr_en + (size_t)w * NC * NE;
         double dfs = 0, dg0 = 0, dg1 = 0, dg2 = 0;
-        const int nordc = jp.nordc;
-        for (int j = lane; j < NE; j += WAVE) {
-            if (j == iel) {
-                fsn[iel] = fs_en;
-                g1n[iel] = ge0; g1n[NE + iel] = ge1; g1n[2 * NE + iel] = ge2;
-                g2n[iel] = ge0; g2n[NE + iel] = ge1; g2n[2 * NE + iel] = ge2;
-                dfs += fs_en - fso_row[iel];
-                dg0 += ge0 - g1_row[iel]; dg1 += ge1 - g1_row[NE + iel]; dg2 += ge2 - g1_row[2 * NE + iel];
-                continue;
-            }
-            const bool inew = iel > j;  // moved electron is the first of the pair (i > j)
-            const int I = inew ? iel : j, J = inew ? j : iel;
-            const double xj0 = xw[j], xj1 = xw[NE + j], xj2 = xw[2 * NE + j];
-            double rv[3];
-            if (inew) { rv[0] = xn0 - xj0; rv[1] = xn1 - xj1; rv[2] = xn2 - xj2; }
-            else { rv[0] = xj0 - xn0; rv[1] = xj1 - xn1; rv[2] = xj2 - xn2; }
-            double r2 = 0;
-            r2 = r2 + rv[0] * rv[0]; r2 = r2 + rv[1] * rv[1]; r2 = r2 + rv[2] * rv[2];
-            const double rij = sqrt(r2);
-            double sspinn = 1; int isb = 0, ipar = 0;
-            if (I < a.NUP || J >= a.NUP) {
-                if (jp.nspin2b == 2) isb = 1;
-                else if (jp.nocuspb == 0) sspinn = 0.5;
-                ipar = 1;
-            }
-            const double u = scale_r(kap, rij);
-            const double dd1 = 1.0 - kap * u;
-            double fs = 0, fI[3] = {0, 0, 0}, fJ[3] = {0, 0, 0};
-            if (u != asymp_r) {
-                const double *b = jp.b[isb];
-                double top = sspinn * b[0] * u, topu = sspinn * b[0];
-                double bot = 1 + b[1] * u, botu = b[1], bot2 = bot * bot;
-                double fee = top / bot - jp.jb[ipar];
-                double feeu = topu / bot - botu * top / bot2;
-                double up = u;
-                for (int iord = 2; iord <= jp.nordb; iord++) {
-                    double un = u * up;
-                    fee = fee + b[iord] * un;
-                    feeu = feeu + b[iord] * iord * up;
-                    up = un;
+        if (lane == 0) {  // the moved electron's own entries: its e-n terms
+            fsn[iel] = fs_en;
+            g1n[iel] = ge0; g1n[NE + iel] = ge1; g1n[2 * NE + iel] = ge2;
+            g2n[iel] = ge0; g2n[NE + iel] = ge1; g2n[2 * NE + iel] = ge2;
+            dfs += fs_en - fso_row[iel];
+            dg0 += ge0 - g1_row[iel]; dg1 += ge1 - g1_row[NE + iel]; dg2 += ge2 - g1_row[2 * NE + iel];
+        }
+        // Partners in slots of WAVE electrons, MJB slots per lane; centres in
+        // the outer loop so each centre's table is read once per slot group.
+        // Per pair: gm/go = gradient on the moved / the partner electron.
+        for (int j0 = 0; j0 < NE; j0 += WAVE * MJB) {
+            double fs[MJB], gm[MJB][3], go[MJB][3], xj[MJB][3], uu1[MJB], fuf[MJB], sfu[MJB];
+            bool on[MJB];
+#pragma unroll
+            for (int s = 0; s < MJB; s++) {
+                const int j = j0 + s * WAVE + lane;
+                on[s] = j < NE && j != iel;
+                const int jq = j < NE ? j : 0;
+                fs[s] = 0; sfu[s] = 0;
+                for (int k = 0; k < 3; k++) { gm[s][k] = 0; go[s][k] = 0; }
+                xj[s][0] = xw[jq]; xj[s][1] = xw[NE + jq]; xj[s][2] = xw[2 * NE + jq];
+                const double rmo[3] = {xn0 - xj[s][0], xn1 - xj[s][1], xn2 - xj[s][2]};
+                double r2 = 0;
+                r2 = r2 + rmo[0] * rmo[0]; r2 = r2 + rmo[1] * rmo[1]; r2 = r2 + rmo[2] * rmo[2];
+                const double rij = sqrt(r2);
+                const double u = scale_r(kap, rij);
+                const double dd1 = 1.0 - kap * u;
+                uu1[s] = dd1;
+                fuf[s] = -kap * dd1 / rij;
+                // e-e term (jastrow4e.f90: psi_ee with its u-derivative)
+                const int I = j > iel ? j : iel, J = j > iel ? iel : j;
+                double sspinn = 1; int isb = 0, ipar = 0;
+                if (I < a.NUP || J >= a.NUP) {
+                    if (jp.nspin2b == 2) isb = 1;
+                    else if (jp.nocuspb == 0) sspinn = 0.5;
+                    ipar = 1;
                 }
-                feeu = feeu * dd1 / rij;
-                fs += fee;
-                for (int k = 0; k < 3; k++) { fI[k] += feeu * rv[k]; fJ[k] -= feeu * rv[k]; }
+                if (on[s] && u != asymp_r) {
+                    const double *b = jp.b[isb];
+                    double top = sspinn * b[0] * u, topu = sspinn * b[0];
+                    double bot = 1 + b[1] * u, botu = b[1], bot2 = bot * bot;
+                    double fee = top / bot - jp.jb[ipar];
+                    double feeu = topu / bot - botu * top / bot2;
+                    double up = u;
+                    for (int iord = 2; iord <= jp.nordb; iord++) {
+                        double un = u * up;
+                        fee = fee + b[iord] * un;
+                        feeu = feeu + b[iord] * iord * up;
+                        up = un;
+                    }
+                    feeu = feeu * dd1 / rij;
+                    fs[s] += fee;
+                    for (int k = 0; k < 3; k++) { gm[s][k] += feeu * rmo[k]; go[s][k] -= feeu * rmo[k]; }
+                }
             }
-            if (nordc > 1) {
-                double uu_[CHAMP_NORDP + 4], ri_[CHAMP_NORDP + 4], rj_[CHAMP_NORDP + 4], ss[CHAMP_NORDP + 1], tt[CHAMP_NORDP + 1];
-                double *uu = uu_ + 3, *rri = ri_ + 3, *rrj = rj_ + 3;
-                uu[-3] = uu[-2] = uu[-1] = 0; rri[-3] = rri[-2] = rri[-1] = 0; rrj[-3] = rrj[-2] = rrj[-1] = 0;
-                uu[0] = 1; rri[0] = 1; rrj[0] = 1; ss[0] = 2; tt[0] = 1;
-                uu[1] = 1 - kap * u;
-                const double dd1s = -kap * dd1;
-                for (int iord = 2; iord <= nordc; iord++) uu[iord] = uu[1] * uu[iord - 1];
+            if constexpr (NORDC > 1) {
                 for (int c = 0; c < NC; c++) {
-                    const double rjo = rw[(size_t)c * NE + j], rrjo = rrw[(size_t)c * NE + j];
-                    const double swjo = 1 - kap * rrjo, ddjo = -kap * (1.0 - kap * rrjo);
-                    double rvj[3] = {xj0 - a.cent[3 * c], xj1 - a.cent[3 * c + 1], xj2 - a.cent[3 * c + 2]};
-                    double rI, rrI, swI, ddI, rJ, rrJ, swJ, ddJ, rvI[3], rvJ[3];
-                    if (inew) {
-                        rI = Lr[c]; rrI = Lrr[c]; swI = Lsw[c]; ddI = Ldd[c];
-                        rvI[0] = Lrv[3 * c]; rvI[1] = Lrv[3 * c + 1]; rvI[2] = Lrv[3 * c + 2];
-                        rJ = rjo; rrJ = rrjo; swJ = swjo; ddJ = ddjo; rvJ[0] = rvj[0]; rvJ[1] = rvj[1]; rvJ[2] = rvj[2];
-                    } else {
-                        rJ = Lr[c]; rrJ = Lrr[c]; swJ = Lsw[c]; ddJ = Ldd[c];
-                        rvJ[0] = Lrv[3 * c]; rvJ[1] = Lrv[3 * c + 1]; rvJ[2] = Lrv[3 * c + 2];
-                        rI = rjo; rrI = rrjo; swI = swjo; ddI = ddjo; rvI[0] = rvj[0]; rvI[1] = rvj[1]; rvI[2] = rvj[2];
-                    }
-                    if (rrI == asymp_r && rrJ == asymp_r) continue;
-                    rri[1] = swI; rrj[1] = swJ;
-                    for (int iord = 1; iord <= nordc; iord++) {
-                        rri[iord] = rri[1] * rri[iord - 1];
-                        rrj[iord] = rrj[1] * rrj[iord - 1];
-                        ss[iord] = rri[iord] + rrj[iord];
-                        tt[iord] = rri[iord] * rrj[iord];
-                    }
-                    const Een<double, false> e = een_eval<double, false>(jp, jp.c[a.iwctype[c]], uu, rri, rrj, ss, tt);
-                    const double fu = e.fu * dd1s / rij, fi = e.fi * ddI / rI, fj = e.fj * ddJ / rJ;
-                    fs += e.fc;
-                    for (int k = 0; k < 3; k++) {
-                        fI[k] = fI[k] + fi * rvI[k] + fu * rv[k];
-                        fJ[k] = fJ[k] + fj * rvJ[k] - fu * rv[k];
+                    const double *tab = Ltab + (size_t)c * 2 * NA;
+                    const double rrm = Lrr[c], fmf = Lfm[c];
+                    const double rvm0 = Lrv[3 * c], rvm1 = Lrv[3 * c + 1], rvm2 = Lrv[3 * c + 2];
+                    const double cx = a.cent[3 * c], cy = a.cent[3 * c + 1], cz = a.cent[3 * c + 2];
+#pragma unroll
+                    for (int s = 0; s < MJB; s++) {
+                        if (j0 + s * WAVE >= NE) break;
+                        const int j = j0 + s * WAVE + lane;
+                        if (!on[s]) continue;
+                        const double rjo = rw[(size_t)c * NE + j], rrjo = rrw[(size_t)c * NE + j];
+                        if (rrm == asymp_r && rrjo == asymp_r) continue;
+                        const double swj = 1.0 - kap * rrjo;
+                        const EenOut e = een_eval_tab<NORDC, 1>(tab, uu1[s], swj);
+                        fs[s] += e.f;
+                        sfu[s] += e.fu;
+                        const double fmv = e.fx * fmf;
+                        const double fot = e.fy * (-kap * swj) / rjo;
+                        gm[s][0] += fmv * rvm0; gm[s][1] += fmv * rvm1; gm[s][2] += fmv * rvm2;
+                        go[s][0] += fot * (xj[s][0] - cx); go[s][1] += fot * (xj[s][1] - cy); go[s][2] += fot * (xj[s][2] - cz);
                     }
                 }
             }
-            const double *gi = inew ? fI : fJ, *gj = inew ? fJ : fI;
-            fsn[j] = fs;
-            dfs += fs - fso_row[j];
-            for (int k = 0; k < 3; k++) {
-                g1n[k * NE + j] = gi[k];
-                g2n[k * NE + j] = gj[k];
-                vjn[k * NE + j] = vj[k * NE + j] + gj[k] - g2_row[k * NE + j];
+#pragma unroll
+            for (int s = 0; s < MJB; s++) {
+                const int j = j0 + s * WAVE + lane;
+                if (!on[s]) continue;
+                const double fu = sfu[s] * fuf[s];
+                const double rmo[3] = {xn0 - xj[s][0], xn1 - xj[s][1], xn2 - xj[s][2]};
+                double gi[3], gj[3];
+                for (int k = 0; k < 3; k++) { gi[k] = gm[s][k] + fu * rmo[k]; gj[k] = go[s][k] - fu * rmo[k]; }
+                fsn[j] = fs[s];
+                dfs += fs[s] - fso_row[j];
+                for (int k = 0; k < 3; k++) {
+                    g1n[k * NE + j] = gi[k];
+                    g2n[k * NE + j] = gj[k];
+                    vjn[k * NE + j] = vj[k * NE + j] + gj[k] - g2_row[k * NE + j];
+                }
+                dg0 += gi[0] - g1_row[j]; dg1 += gi[1] - g1_row[NE + j]; dg2 += gi[2] - g1_row[2 * NE + j];
             }
-            dg0 += gi[0] - g1_row[j]; dg1 += gi[1] - g1_row[NE + j]; dg2 += gi[2] - g1_row[2 * NE + j];
         }
         dfs = wave_sum(dfs); dg0 = wave_sum(dg0); dg1 = wave_sum(dg1); dg2 = wave_sum(dg2);
         if (lane == 0) {
@@ -481,10 +500,26 @@ int launch_drift_old(champ_ctx *c, int iel)
 int launch_move_a(champ_ctx *c, int iel, int mode)
 {
     MoveArgs a = move_args(c, iel, mode, g_tau);
-    size_t lds = (size_t)4 * 7 * c->NC * sizeof(double);
+    const int nordc = c->hjas.nordc;
+    const size_t lds = (size_t)4 * move_a_lds_per_wave(c->NC, nordc) * sizeof(double);
+    if (nordc > 5) {
+        set_error("move: e-e-n order above 5 is not supported");
+        return CHAMP_ERR_UNSUPPORTED;
+    }
+    const void *fn = nordc == 5 ? (const void *)k_move_a<5> : nordc == 4 ? (const void *)k_move_a<4>
+                   : nordc == 3 ? (const void *)k_move_a<3> : nordc == 2 ? (const void *)k_move_a<2>
+                                                                        : (const void *)k_move_a<0>;
+    if (lds > 64 * 1024) HIPCHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int slot;
     tbegin(c, K_MOVE_A, &slot);
-    hipLaunchKernelGGL(k_move_a, dim3(nblocks(c->W, 4)), dim3(256), lds, c->stream, a);
+    const dim3 grid(nblocks(c->W, 4)), block(256);
+    switch (nordc) {
+    case 5: hipLaunchKernelGGL(k_move_a<5>, grid, block, lds, c->stream, a); break;
+    case 4: hipLaunchKernelGGL(k_move_a<4>, grid, block, lds, c->stream, a); break;
+    case 3: hipLaunchKernelGGL(k_move_a<3>, grid, block, lds, c->stream, a); break;
+    case 2: hipLaunchKernelGGL(k_move_a<2>, grid, block, lds, c->stream, a); break;
+    default: hipLaunchKernelGGL(k_move_a<0>, grid, block, lds, c->stream, a); break;
+    }
     tend(c, slot);
     HIPCHECK(hipGetLastError());
     return CHAMP_OK;

@@ -8,101 +8,22 @@
 //   psi ratio   = sum_m phi_m(x_q) W[i][m],  W = slmi_i . coef (one GEMM per hpsi)
 //   log J ratio = sum_{j!=i} [b(u_qj) + sum_c P(u_qj, R_qc, R_jc)]
 //                 + sum_c a(R_qc) - sum_j F[i][j]
-// with P the e-e-n polynomial.  For a fixed (q, c) the polynomial is rewritten
-// as P = sum_k u^k sum_n A_kn(R_qc) R_jc^n (switched scaled distances), so a
-// lane that owns one centre c keeps A in registers for QB points and streams
-// over electrons j: about a dozen FMAs per (q, c, j), all indices fixed at
-// compile time by the e-e-n term table of the Jastrow order.
+// with P the e-e-n polynomial.  For a fixed (q, c) the polynomial is written
+// as P = sum_k u^k sum_n A_kn(R_qc) R_jc^n (een_poly.hpp), so a lane that owns
+// one centre c keeps A in registers for QB points and streams over electrons
+// j: about a dozen FMAs per (q, c, j).
 //   phase A  lanes over (j, q): u_qj into LDS, e-e sums b(u_qj)
 //   phase B  lanes over (c, j-group): e-e-n and e-n sums
 //   phase C  lanes over AOs: the psi ratio
 // Reductions go lane -> wave (shuffles) -> per-wave slots in LDS -> thread 0,
 // in a fixed order, so results do not depend on timing.
+#include "een_poly.hpp"
 #include "hpsi_args.hpp"
 
 #include <algorithm>
 #include <cmath>
 
 namespace champ {
-
-// e-e-n terms (k, l, m) in the order of jastrow4.f90:203-213 and the non-zero
-// pattern of A_kn: term c u^k (Ri^l + Rj^l)(Ri Rj)^m adds c Ri^(l+m) to
-// A[k][m] and c Ri^m to A[k][l+m].
-template <int N>
-struct EenTable {
-    int n;
-    int k[40], l[40], m[40];
-    bool nz[N + 1][N + 1];
-    int nmax[N + 1];  // highest n with A[k][n] != 0, -1 if none
-    int kmax;
-    constexpr EenTable() : n(0), k{}, l{}, m{}, nz{}, nmax{}, kmax(-1)
-    {
-        for (int nn = 2; nn <= N; nn++)
-            for (int kk = nn - 1; kk >= 0; kk--) {
-                const int lhi = kk == 0 ? nn - kk - 2 : nn - kk;
-                for (int ll = lhi; ll >= 0; ll--) {
-                    const int mm = (nn - kk - ll) / 2;
-                    if (2 * mm == nn - kk - ll) {
-                        k[n] = kk; l[n] = ll; m[n] = mm; n++;
-                        nz[kk][mm] = true;
-                        nz[kk][ll + mm] = true;
-                    }
-                }
-            }
-        for (int kk = 0; kk <= N; kk++) {
-            nmax[kk] = -1;
-            for (int nn = 0; nn <= N; nn++)
-                if (nz[kk][nn]) nmax[kk] = nn;
-            if (nmax[kk] >= 0) kmax = kk;
-        }
-    }
-};
-
-// A_kn for one (point, centre): ri = switched scaled distance of the point to
-// the centre, c = e-e-n parameters of the centre's type.
-template <int N>
-__device__ __forceinline__ void een_coef(const double *c, double ri, double (&A)[N + 1][N + 1])
-{
-    constexpr EenTable<N> T{};
-    double rp[N + 1];
-    rp[0] = 1.0;
-#pragma unroll
-    for (int p = 1; p <= N; p++) rp[p] = p == 1 ? ri : rp[1] * rp[p - 1];
-#pragma unroll
-    for (int kk = 0; kk <= N; kk++)
-#pragma unroll
-        for (int nn = 0; nn <= N; nn++) A[kk][nn] = 0.0;
-#pragma unroll
-    for (int t = 0; t < T.n; t++) {
-        const double ct = c[t];
-        A[T.k[t]][T.m[t]] += ct * rp[T.l[t] + T.m[t]];
-        A[T.k[t]][T.l[t] + T.m[t]] += ct * rp[T.m[t]];
-    }
-}
-
-// P = sum_k u^k Q_k(rj), Q_k = sum_n A_kn rj^n, both by Horner's rule over the
-// non-zero pattern.
-template <int N>
-__device__ __forceinline__ double een_poly(const double (&A)[N + 1][N + 1], double u, double rj)
-{
-    constexpr EenTable<N> T{};
-    double P = 0.0;
-#pragma unroll
-    for (int kk = N; kk >= 0; kk--) {
-        if (T.kmax < 0 || kk > T.kmax) continue;
-        double Q = 0.0;
-        if (T.nmax[kk] >= 0) {
-            Q = A[kk][T.nmax[kk]];
-#pragma unroll
-            for (int nn = N - 1; nn >= 0; nn--) {
-                if (nn >= T.nmax[kk]) continue;
-                Q = T.nz[kk][nn] ? Q * rj + A[kk][nn] : Q * rj;
-            }
-        }
-        P = kk == T.kmax ? Q : P * u + Q;
-    }
-    return P;
-}
 
 // psianl (nonlpsi.f90:88-105): e-n term at scaled distance rr
 __device__ __forceinline__ double en_term(const JasParams &jp, int it, double rr)
@@ -232,7 +153,7 @@ __global__ void __launch_bounds__(NL_THREADS) k_nonloc(HArgs a)
                     if (jg < JG && c < NC) {
                         const int it = a.iwctype[c];
                         const double ecx = a.cent[3 * c], ecy = a.cent[3 * c + 1], ecz = a.cent[3 * c + 2];
-                        double A[QB][NORDC + 1][NORDC + 1];
+                        double A[QB][een_naz<NORDC>()];
                         bool any_asym = false;
                         bool asym[QB];
 #pragma unroll
@@ -245,7 +166,7 @@ __global__ void __launch_bounds__(NL_THREADS) k_nonloc(HArgs a)
                             asym[qq] = rr == asymp;
                             any_asym = any_asym || asym[qq];
                             if (jg == 0 && q0 + qq < NQ) acc[qq] += en_term(jp, it, rr);
-                            if constexpr (NORDC > 1) een_coef<NORDC>(jp.c[it], 1.0 - kap * rr, A[qq]);
+                            if constexpr (NORDC > 1) een_tables<NORDC, 0>(jp.c[it], 1.0 - kap * rr, A[qq]);
                         }
                         if constexpr (NORDC > 1) {
                             for (int j = jg; j < NE; j += JG) {
@@ -255,7 +176,7 @@ __global__ void __launch_bounds__(NL_THREADS) k_nonloc(HArgs a)
                                 const double *uj = U + (size_t)j * QB;
                                 double P[QB];
 #pragma unroll
-                                for (int qq = 0; qq < QB; qq++) P[qq] = een_poly<NORDC>(A[qq], uj[qq], rj);
+                                for (int qq = 0; qq < QB; qq++) P[qq] = een_eval_tab<NORDC, 0>(A[qq], uj[qq], rj).f;
                                 if (any_asym && rrj == asymp) {  // psinl: both distances asymptotic -> 0
 #pragma unroll
                                     for (int qq = 0; qq < QB; qq++)
