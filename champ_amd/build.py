@@ -28,28 +28,32 @@ FLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(OBJDIR, src.replace(".hip", ".o"))
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(src: str, objdir: str = OBJDIR, defines: tuple = ()) -> str:
+    obj = os.path.join(objdir, src.replace(".hip", ".o"))
+    cmd = [HIPCC, *FLAGS, *[f"-D{d}" for d in defines], "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
     return obj
 
 
-def build(verbose: bool = True) -> str:
-    os.makedirs(OBJDIR, exist_ok=True)
+def build(verbose: bool = True, out: str = OUT, defines: tuple = ()) -> str:
+    """Compile and link the library.  ``defines`` set the kernels' tuning
+    macros (CHAMP_MJB, CHAMP_MOVE_WPE, CHAMP_HJB, CHAMP_HJAS_WPE, CHAMP_NL_QB,
+    CHAMP_NL_WPE); ``out`` other than the default builds a variant elsewhere."""
+    objdir = OBJDIR if not defines and out == OUT else out + ".objs"
+    os.makedirs(objdir, exist_ok=True)
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = OUT + ".tmp"
+        objs = list(ex.map(lambda src: _compile(src, objdir, tuple(defines)), SOURCES))
+    tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
-    os.replace(tmp, OUT)
+    os.replace(tmp, out)
     if verbose:
-        print(f"built {OUT}")
-    return OUT
+        print(f"built {out}")
+    return out
 
 
 FDIR = os.path.join(HERE, "fortran")

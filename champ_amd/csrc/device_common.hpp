@@ -60,6 +60,12 @@ __device__ __forceinline__ T powi(T x, int m)
     return m < 0 ? T(1) / y : y;
 }
 
+// Occupancy hint for the compiler's register allocation: at least n waves of
+// a kernel per SIMD (n = 1 leaves the choice to the compiler).
+#ifndef CHAMP_WAVES_PER_EU
+#define CHAMP_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
+#endif
+
 // ------------------------------------------------------------------- tables
 struct SplineTab {
     const double *rwf, *d2rwf, *ce, *ae, *rmaxwf;

@@ -65,24 +65,24 @@ __device__ __forceinline__ void een_tables(const double *c, double x, double *ta
     xp[0] = 1.0;
 #pragma unroll
     for (int p = 1; p <= N; p++) xp[p] = p == 1 ? x : xp[1] * xp[p - 1];
-    double acc[(D + 1) * NA];
+    // one entry at a time: sum over the (term, half) pairs that land on it
 #pragma unroll
-    for (int e = 0; e < (D + 1) * NA; e++) acc[e] = 0.0;
+    for (int d = 0; d <= D; d++)
 #pragma unroll
-    for (int t = 0; t < T.n; t++) {
-        const double ct = c[t];
+        for (int e = 0; e < NA; e++) {
+            double v = 0.0;
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int n = h == 0 ? T.m[t] : T.l[t] + T.m[t];
-            const int p = h == 0 ? T.l[t] + T.m[t] : T.m[t];
-            const int e = T.idx[T.k[t]][n];
-            acc[e] += ct * xp[p];
-            if (D >= 1 && p >= 1) acc[NA + e] += ct * (double)p * xp[p - 1];
-            if (D >= 2 && p >= 2) acc[2 * NA + e] += ct * (double)(p * (p - 1)) * xp[p - 2];
+            for (int t = 0; t < T.n; t++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int n = h == 0 ? T.m[t] : T.l[t] + T.m[t];
+                    const int p = h == 0 ? T.l[t] + T.m[t] : T.m[t];
+                    if (T.idx[T.k[t]][n] != e || p < d) continue;
+                    const double f = d == 0 ? 1.0 : (d == 1 ? (double)p : (double)(p * (p - 1)));
+                    v += c[t] * f * xp[p - d];
+                }
+            tab[d * NA + e] = v;
         }
-    }
-#pragma unroll
-    for (int e = 0; e < (D + 1) * NA; e++) tab[e] = acc[e];
 }
 
 // P and its derivatives at (u, y) from the tables of x:
@@ -97,12 +97,15 @@ __device__ __forceinline__ EenOut een_eval_tab(const double *tab, double u, doub
     constexpr EenTable<N> T{};
     constexpr int NA = een_naz<N>();
     EenOut o{0, 0, 0, 0, 0, 0, 0, 0, 0};
-    // Horner in u over the row polynomials of y; the *2 names hold half the
-    // second derivative (p''/2) as the recurrence produces it.
+    // Horner in u over the row polynomials of y.  The derivative recurrences
+    // (d2 = d2 y + d1, d1 = d1 y + p, p = p y + a) start from zero, so their
+    // first updates are plain copies: the step counters are compile-time and
+    // no multiply by a zero accumulator is emitted.  *2 names hold p''/2.
     double P = 0, Pu = 0, Puu2 = 0, Py = 0, Puy = 0, Pyy2 = 0, Px = 0, Pux = 0, Pxx = 0;
 #pragma unroll
     for (int kk = N; kk >= 0; kk--) {
         if (kk > T.kmax) continue;
+        const int su = T.kmax - kk;  // Horner step in u
         double q = 0, q1 = 0, q2 = 0, r = 0, s = 0;  // row k: value, d/dy, (d2/dy2)/2; dA/dx row; d2A/dx2 row
         const int nm = T.nmax[kk];
         if (nm >= 0) {
@@ -112,24 +115,25 @@ __device__ __forceinline__ EenOut een_eval_tab(const double *tab, double u, doub
 #pragma unroll
             for (int nn = N - 1; nn >= 0; nn--) {
                 if (nn >= nm) continue;
+                const int st = nm - 1 - nn;  // Horner step in y
                 const int e = T.idx[kk][nn];
-                if (D >= 2) q2 = q2 * y + q1;
-                if (D >= 1) q1 = q1 * y + q;
+                if (D >= 2 && st >= 1) q2 = st == 1 ? q1 : q2 * y + q1;
+                if (D >= 1) q1 = st == 0 ? q : q1 * y + q;
                 q = e >= 0 ? q * y + tab[e] : q * y;
                 if (D >= 1) r = e >= 0 ? r * y + tab[NA + e] : r * y;
                 if (D >= 2) s = e >= 0 ? s * y + tab[2 * NA + e] : s * y;
             }
         }
-        if (kk == T.kmax) {
+        if (su == 0) {
             P = q; Py = q1; Pyy2 = q2; Px = r; Pxx = s;
         } else {
-            if (D >= 2) Puu2 = Puu2 * u + Pu;
-            if (D >= 1) Pu = Pu * u + P;
+            if (D >= 2 && su >= 2) Puu2 = su == 2 ? Pu : Puu2 * u + Pu;
+            if (D >= 1) Pu = su == 1 ? P : Pu * u + P;
             P = P * u + q;
-            if (D >= 2) Puy = Puy * u + Py;
+            if (D >= 2) Puy = su == 1 ? Py : Puy * u + Py;
             if (D >= 1) Py = Py * u + q1;
             if (D >= 2) Pyy2 = Pyy2 * u + q2;
-            if (D >= 2) Pux = Pux * u + Px;
+            if (D >= 2) Pux = su == 1 ? Px : Pux * u + Px;
             if (D >= 1) Px = Px * u + r;
             if (D >= 2) Pxx = Pxx * u + s;
         }

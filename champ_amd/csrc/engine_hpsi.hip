@@ -2,6 +2,7 @@
 // whole population.  distances + jastrow_factor4, AO values for all
 // electrons, the AO->MO GEMM, batched LU inverse of both Slater matrices,
 // nonloc_pot quadrature and the local-energy assembly.
+#include "een_poly.hpp"
 #include "hpsi_args.hpp"
 
 #include <algorithm>
@@ -12,8 +13,20 @@ namespace champ {
 // ------------------------------------------------------------------------
 // distances(0) + jastrow_factor4 (jastrow4.f90:3-415, iforce_analy = 0),
 // one 256-thread block per walker.
-__global__ void __launch_bounds__(256) k_hpsi_jastrow(HArgs a)
+// Tuning of k_hpsi_jastrow: partner slots per lane in the pair loop and the
+// minimum waves per SIMD asked of the register allocator.
+#ifndef CHAMP_HJB
+#define CHAMP_HJB 1
+#endif
+#ifndef CHAMP_HJAS_WPE
+#define CHAMP_HJAS_WPE 1
+#endif
+constexpr int HJB = CHAMP_HJB;
+
+template <int NORDC>
+__global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi_jastrow(HArgs a)
 {
+    HIP_DYNAMIC_SHARED(double, lds)
     __shared__ double red[16];
     const int w = blockIdx.x, tid = threadIdx.x, bd = blockDim.x;
     const int NE = a.NE, NC = a.NC;
@@ -72,95 +85,136 @@ __global__ void __launch_bounds__(256) k_hpsi_jastrow(HArgs a)
         fsp += fs; d2p += d2;
     }
 
-    // e-e and e-e-n pair terms (jastrow4.f90:79-296)
-    const int NP = NE * (NE - 1) / 2, nordc = jp.nordc;
-    for (int p = tid; p < NP; p += bd) {
-        int i = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
-        while (i * (i - 1) / 2 > p) i--;
-        while ((i + 1) * i / 2 <= p) i++;
-        const int j = p - i * (i - 1) / 2;
-        double rv[3] = {xw[i] - xw[j], xw[NE + i] - xw[NE + j], xw[2 * NE + i] - xw[2 * NE + j]};
-        double r2 = 0;
-        r2 = r2 + rv[0] * rv[0]; r2 = r2 + rv[1] * rv[1]; r2 = r2 + rv[2] * rv[2];
-        const double rij = sqrt(r2);
-        double sspinn = 1; int isb = 0, ipar = 0;
-        if (i < a.NUP || j >= a.NUP) {
-            if (jp.nspin2b == 2) isb = 1;
-            else if (jp.nocuspb == 0) sspinn = 0.5;
-            ipar = 1;
-        }
-        double fs = 0, d2 = 0, fI[3] = {0, 0, 0}, fJ[3] = {0, 0, 0};
-        const double u = scale_r(kap, rij);
-        double dd1 = 1.0 - kap * u, dd2 = -kap * dd1;
-        if (u != asymp_r) {
-            const double *b = jp.b[isb];
-            double top = sspinn * b[0] * u, topu = sspinn * b[0], topuu = 0;
-            double bot = 1 + b[1] * u, botu = b[1], botuu = 0, bot2 = bot * bot;
-            double fee = top / bot - jp.jb[ipar];
-            double feeu = topu / bot - botu * top / bot2;
-            double feeuu = topuu - (botuu * top + 2 * botu * topu) / bot + 2 * botu * botu * top / bot2;
-            feeuu = feeuu / bot;
-            double u1 = u, u2 = 1.0;  // uu(iord-1), uu(iord-2)
-            for (int iord = 2; iord <= jp.nordb; iord++) {
-                const double un = u * u1;
-                fee = fee + b[iord] * un;
-                feeu = feeu + b[iord] * iord * u1;
-                feeuu = feeuu + b[iord] * iord * (iord - 1) * u2;
-                u2 = u1; u1 = un;
+    // e-e and e-e-n pair terms (jastrow4.f90:79-296).  One wave per row i
+    // (rows dealt longest first), lanes over the partners j < i in slots of
+    // WAVE.  The row's per-centre e-e-n tables (value and two x-derivatives,
+    // een_poly.hpp) and scalars are built by the wave into its LDS region.
+    constexpr int NA = een_naz<NORDC>();
+    constexpr int RS = 3 * NA + 8;  // doubles per centre: tables, r, rr, dd7, dd9, rvec (3), 1/r
+    const int lane = tid % WAVE, wv = tid / WAVE, nwv = bd / WAVE;
+    double *rowt = lds + (size_t)wv * NC * RS;
+    for (int rr0 = wv; rr0 < NE - 1; rr0 += nwv) {
+        const int i = NE - 1 - rr0;
+        if constexpr (NORDC > 1) {
+            __syncwarp();  // the previous row's table is no longer read
+            for (int c = lane; c < NC; c += WAVE) {
+                double *rt = rowt + (size_t)c * RS;
+                const double ri = rw[(size_t)c * NE + i], rri = rrw[(size_t)c * NE + i];
+                const double sw = 1.0 - kap * rri;
+                een_tables<NORDC, 2>(jp.c[a.iwctype[c]], sw, rt);
+                rt[3 * NA] = ri;
+                rt[3 * NA + 1] = rri;
+                rt[3 * NA + 2] = -kap * sw;         // dd7
+                rt[3 * NA + 3] = -kap * (-kap * sw);  // dd9
+                for (int k = 0; k < 3; k++) rt[3 * NA + 4 + k] = xw[k * NE + i] - a.cent[3 * c + k];
+                rt[3 * NA + 7] = 1.0 / ri;
             }
-            feeuu = feeuu * dd1 * dd1 + feeu * dd2;
-            feeu = feeu * dd1 / rij;
-            fs += fee;
-            for (int k = 0; k < 3; k++) { fI[k] += feeu * rv[k]; fJ[k] -= feeu * rv[k]; }
-            d2 += 2 * (feeuu + 2 * feeu);
+            __syncwarp();
         }
-        if (nordc > 1) {
-            double uu_[CHAMP_NORDP + 4], ri_[CHAMP_NORDP + 4], rj_[CHAMP_NORDP + 4], ss[CHAMP_NORDP + 1], tt[CHAMP_NORDP + 1];
-            double *uu = uu_ + 3, *rri = ri_ + 3, *rrj = rj_ + 3;
-            uu[-3] = uu[-2] = uu[-1] = 0; rri[-3] = rri[-2] = rri[-1] = 0; rrj[-3] = rrj[-2] = rrj[-1] = 0;
-            uu[0] = 1; rri[0] = 1; rrj[0] = 1; ss[0] = 2; tt[0] = 1;
-            uu[1] = 1 - kap * u;
-            dd1 = -kap * dd1; dd2 = -kap * dd2;
-            for (int iord = 2; iord <= nordc; iord++) uu[iord] = uu[1] * uu[iord - 1];
-            for (int c = 0; c < NC; c++) {
-                const double ri = rw[(size_t)c * NE + i], rj = rw[(size_t)c * NE + j];
-                const double rri0 = rrw[(size_t)c * NE + i], rrj0 = rrw[(size_t)c * NE + j];
-                if (rri0 == asymp_r && rrj0 == asymp_r) continue;
-                const double dd7u = 1.0 - kap * rri0, dd8u = 1.0 - kap * rrj0;
-                const double dd7 = -kap * dd7u, dd9 = -kap * (-kap * dd7u), dd8 = -kap * dd8u, dd10 = -kap * (-kap * dd8u);
-                rri[1] = 1 - kap * rri0; rrj[1] = 1 - kap * rrj0;
-                const double s = ri + rj, t = ri - rj;
-                const double u2pst = rij * rij + s * t, u2mst = rij * rij - s * t;
-                for (int iord = 1; iord <= nordc; iord++) {
-                    rri[iord] = rri[1] * rri[iord - 1];
-                    rrj[iord] = rrj[1] * rrj[iord - 1];
-                    ss[iord] = rri[iord] + rrj[iord];
-                    tt[iord] = rri[iord] * rrj[iord];
-                }
-                const Een<double, true> e = een_eval<double, true>(jp, jp.c[a.iwctype[c]], uu, rri, rrj, ss, tt);
-                const double fuu = e.fuu * dd1 * dd1 + e.fu * dd2;
-                const double fu = e.fu * dd1 / rij;
-                const double fui = e.fui * dd1 * dd7, fuj = e.fuj * dd1 * dd8;
-                const double fii = e.fii * dd7 * dd7 + e.fi * dd9, fjj = e.fjj * dd8 * dd8 + e.fj * dd10;
-                const double fi = e.fi * dd7 / ri, fj = e.fj * dd8 / rj;
-                fs = fs + e.fc;
+        for (int j0 = 0; j0 < i; j0 += WAVE * HJB) {
+            double fs[HJB], d2[HJB], fI[HJB][3], fJ[HJB][3], rv[HJB][3], xj[HJB][3], sfu[HJB];
+            double uu1[HJB], dd1e[HJB], dd2e[HJB], irij[HJB], rij2[HJB];
+            bool on[HJB];
+#pragma unroll
+            for (int s = 0; s < HJB; s++) {
+                const int j = j0 + s * WAVE + lane;
+                on[s] = j < i;
+                const int jq = on[s] ? j : 0;
+                fs[s] = 0; d2[s] = 0; sfu[s] = 0;
                 for (int k = 0; k < 3; k++) {
-                    const double rvi = xw[k * NE + i] - a.cent[3 * c + k], rvj = xw[k * NE + j] - a.cent[3 * c + k];
-                    fI[k] = fI[k] + fi * rvi + fu * rv[k];
-                    fJ[k] = fJ[k] + fj * rvj - fu * rv[k];
+                    fI[s][k] = 0; fJ[s][k] = 0;
+                    xj[s][k] = xw[k * NE + jq];
+                    rv[s][k] = xw[k * NE + i] - xj[s][k];
                 }
-                d2 = d2 + 2 * (fuu + 2 * fu) + fui * u2pst / (ri * rij) + fuj * u2mst / (rj * rij) + fii + 2 * fi + fjj + 2 * fj;
+                double r2 = 0;
+                r2 = r2 + rv[s][0] * rv[s][0]; r2 = r2 + rv[s][1] * rv[s][1]; r2 = r2 + rv[s][2] * rv[s][2];
+                const double rij = sqrt(r2);
+                rij2[s] = rij * rij;
+                irij[s] = 1.0 / rij;
+                double sspinn = 1; int isb = 0, ipar = 0;
+                if (i < a.NUP || jq >= a.NUP) {
+                    if (jp.nspin2b == 2) isb = 1;
+                    else if (jp.nocuspb == 0) sspinn = 0.5;
+                    ipar = 1;
+                }
+                const double u = scale_r(kap, rij);
+                const double dd1 = 1.0 - kap * u, dd2 = -kap * dd1;
+                uu1[s] = dd1;
+                dd1e[s] = -kap * dd1;
+                dd2e[s] = -kap * dd2;
+                if (on[s] && u != asymp_r) {
+                    const double *b = jp.b[isb];
+                    double top = sspinn * b[0] * u, topu = sspinn * b[0], topuu = 0;
+                    double bot = 1 + b[1] * u, botu = b[1], botuu = 0, bot2 = bot * bot;
+                    double fee = top / bot - jp.jb[ipar];
+                    double feeu = topu / bot - botu * top / bot2;
+                    double feeuu = topuu - (botuu * top + 2 * botu * topu) / bot + 2 * botu * botu * top / bot2;
+                    feeuu = feeuu / bot;
+                    double u1 = u, u2 = 1.0;  // uu(iord-1), uu(iord-2)
+                    for (int iord = 2; iord <= jp.nordb; iord++) {
+                        const double un = u * u1;
+                        fee = fee + b[iord] * un;
+                        feeu = feeu + b[iord] * iord * u1;
+                        feeuu = feeuu + b[iord] * iord * (iord - 1) * u2;
+                        u2 = u1; u1 = un;
+                    }
+                    feeuu = feeuu * dd1 * dd1 + feeu * dd2;
+                    feeu = feeu * dd1 / rij;
+                    fs[s] += fee;
+                    for (int k = 0; k < 3; k++) { fI[s][k] += feeu * rv[s][k]; fJ[s][k] -= feeu * rv[s][k]; }
+                    d2[s] += 2 * (feeuu + 2 * feeu);
+                }
+            }
+            if constexpr (NORDC > 1) {
+                for (int c = 0; c < NC; c++) {
+                    const double *rt = rowt + (size_t)c * RS;
+                    const double ri = rt[3 * NA], rri = rt[3 * NA + 1], dd7 = rt[3 * NA + 2], dd9 = rt[3 * NA + 3];
+                    const double rvi0 = rt[3 * NA + 4], rvi1 = rt[3 * NA + 5], rvi2 = rt[3 * NA + 6], iri = rt[3 * NA + 7];
+                    const double cx = a.cent[3 * c], cy = a.cent[3 * c + 1], cz = a.cent[3 * c + 2];
+#pragma unroll
+                    for (int s = 0; s < HJB; s++) {
+                        if (j0 + s * WAVE >= i) break;
+                        const int j = j0 + s * WAVE + lane;
+                        if (!on[s]) continue;
+                        const double rj = rw[(size_t)c * NE + j], rrj = rrw[(size_t)c * NE + j];
+                        if (rri == asymp_r && rrj == asymp_r) continue;
+                        const double swj = 1.0 - kap * rrj;
+                        const double dd8 = -kap * swj, dd10 = -kap * (-kap * swj);
+                        const EenOut e = een_eval_tab<NORDC, 2>(rt, uu1[s], swj);
+                        const double d1 = dd1e[s];
+                        const double fuu = e.fuu * d1 * d1 + e.fu * dd2e[s];
+                        const double fu = e.fu * d1 * irij[s];
+                        const double fui = e.fux * d1 * dd7, fuj = e.fuy * d1 * dd8;
+                        const double fii = e.fxx * dd7 * dd7 + e.fx * dd9, fjj = e.fyy * dd8 * dd8 + e.fy * dd10;
+                        const double irj = 1.0 / rj;
+                        const double fi = e.fx * dd7 * iri, fj = e.fy * dd8 * irj;
+                        fs[s] += e.f;
+                        sfu[s] += fu;
+                        const double st = (ri + rj) * (ri - rj);
+                        const double u2pst = rij2[s] + st, u2mst = rij2[s] - st;
+                        fI[s][0] += fi * rvi0; fI[s][1] += fi * rvi1; fI[s][2] += fi * rvi2;
+                        fJ[s][0] += fj * (xj[s][0] - cx); fJ[s][1] += fj * (xj[s][1] - cy); fJ[s][2] += fj * (xj[s][2] - cz);
+                        d2[s] = d2[s] + 2 * (fuu + 2 * fu) + fui * u2pst * (iri * irij[s]) + fuj * u2mst * (irj * irij[s]) + fii +
+                                2 * fi + fjj + 2 * fj;
+                    }
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < HJB; s++) {
+                const int j = j0 + s * WAVE + lane;
+                if (!on[s]) continue;
+                for (int k = 0; k < 3; k++) { fI[s][k] += sfu[s] * rv[s][k]; fJ[s][k] -= sfu[s] * rv[s][k]; }
+                F[(size_t)i * NE + j] = fs[s];
+                F[(size_t)j * NE + i] = fs[s];
+                for (int k = 0; k < 3; k++) {
+                    G1[((size_t)i * 3 + k) * NE + j] = fI[s][k];
+                    G2[((size_t)i * 3 + k) * NE + j] = fJ[s][k];
+                    G1[((size_t)j * 3 + k) * NE + i] = fJ[s][k];
+                    G2[((size_t)j * 3 + k) * NE + i] = fI[s][k];
+                }
+                fsp += fs[s]; d2p += d2[s];
             }
         }
-        F[(size_t)i * NE + j] = fs;
-        F[(size_t)j * NE + i] = fs;
-        for (int k = 0; k < 3; k++) {
-            G1[((size_t)i * 3 + k) * NE + j] = fI[k];
-            G2[((size_t)i * 3 + k) * NE + j] = fJ[k];
-            G1[((size_t)j * 3 + k) * NE + i] = fJ[k];
-            G2[((size_t)j * 3 + k) * NE + i] = fI[k];
-        }
-        fsp += fs; d2p += d2;
     }
     __syncthreads();
     double *vj = a.vjT + (size_t)w * 3 * NE;
@@ -481,7 +535,26 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
     a.accumulate = accumulate;
     int slot;
     tbegin(c, K_HPSI_JAS, &slot);
-    hipLaunchKernelGGL(k_hpsi_jastrow, dim3(c->W), dim3(256), 0, c->stream, a);
+    {
+        const int nordc = c->hjas.nordc;
+        const int naz[] = {0, 0, een_naz<2>(), een_naz<3>(), een_naz<4>(), een_naz<5>()};
+        if (nordc > 5) {
+            set_error("hpsi: e-e-n order above 5 is not supported");
+            return CHAMP_ERR_UNSUPPORTED;
+        }
+        const size_t lds = nordc > 1 ? (size_t)4 * c->NC * (3 * naz[nordc] + 8) * sizeof(double) : 8;
+        const void *fn = nordc == 5 ? (const void *)k_hpsi_jastrow<5> : nordc == 4 ? (const void *)k_hpsi_jastrow<4>
+                       : nordc == 3 ? (const void *)k_hpsi_jastrow<3> : nordc == 2 ? (const void *)k_hpsi_jastrow<2>
+                                                                                : (const void *)k_hpsi_jastrow<0>;
+        if (lds > 64 * 1024) HIPCHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        switch (nordc) {
+        case 5: hipLaunchKernelGGL(k_hpsi_jastrow<5>, dim3(c->W), dim3(256), lds, c->stream, a); break;
+        case 4: hipLaunchKernelGGL(k_hpsi_jastrow<4>, dim3(c->W), dim3(256), lds, c->stream, a); break;
+        case 3: hipLaunchKernelGGL(k_hpsi_jastrow<3>, dim3(c->W), dim3(256), lds, c->stream, a); break;
+        case 2: hipLaunchKernelGGL(k_hpsi_jastrow<2>, dim3(c->W), dim3(256), lds, c->stream, a); break;
+        default: hipLaunchKernelGGL(k_hpsi_jastrow<0>, dim3(c->W), dim3(256), lds, c->stream, a); break;
+        }
+    }
     tend(c, slot);
     HIPCHECK(hipGetLastError());
     for (int w0 = 0; w0 < c->W; w0 += c->hpsi_chunk) {

@@ -93,8 +93,15 @@ __global__ void __launch_bounds__(256) k_drift_old(MoveArgs a)
 //   then distances(iel) (distances.f90:3-167), jastrow4e with iflag=0
 //   (jastrow4e.f90:3-358) and basis_fns(iel, ider=1) (basis_fns.f90:8-214)
 //   writing the AO block that the MO GEMM contracts (orbitalse).
-// partner-electron slots per lane in the pair loop of k_move_a
-constexpr int MJB = 2;
+// Tuning of k_move_a: partner-electron slots per lane in the pair loop and
+// the minimum waves per SIMD asked of the register allocator.
+#ifndef CHAMP_MJB
+#define CHAMP_MJB 2
+#endif
+#ifndef CHAMP_MOVE_WPE
+#define CHAMP_MOVE_WPE 1
+#endif
+constexpr int MJB = CHAMP_MJB;
 
 // LDS doubles per wave of k_move_a: r, rr, -kap (1 - kap rr) / r, rvec (3) per
 // centre, then the centre's e-e-n tables (value and d/dx)
@@ -105,7 +112,7 @@ static size_t move_a_lds_per_wave(int NC, int nordc)
 }
 
 template <int NORDC>
-__global__ void __launch_bounds__(256) k_move_a(MoveArgs a)
+__global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move_a(MoveArgs a)
 {
     HIP_DYNAMIC_SHARED(double, lds)
     constexpr int NA = een_naz<NORDC>();

@@ -64,11 +64,20 @@ __device__ __forceinline__ double strided_sum(double v)
     return v;
 }
 
+// Tuning of k_nonloc: quadrature points per block at e-e-n order <= 4 and the
+// minimum waves per SIMD asked of the register allocator.
+#ifndef CHAMP_NL_QB
+#define CHAMP_NL_QB 4
+#endif
+#ifndef CHAMP_NL_WPE
+#define CHAMP_NL_WPE 1
+#endif
+
 constexpr int NL_THREADS = 256;
 constexpr int NL_WAVES = NL_THREADS / WAVE;
 
 template <int NORDC, int QB>
-__global__ void __launch_bounds__(NL_THREADS) k_nonloc(HArgs a)
+__global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k_nonloc(HArgs a)
 {
     HIP_DYNAMIC_SHARED(double, lds)
     const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
@@ -270,11 +279,11 @@ static int launch_k(champ_ctx *c, const HArgs &a)
 int launch_nonloc(champ_ctx *c, const HArgs &a)
 {
     switch (c->hjas.nordc) {
-    case 0: return launch_k<0, 4>(c, a);
-    case 1: return launch_k<1, 4>(c, a);
-    case 2: return launch_k<2, 4>(c, a);
-    case 3: return launch_k<3, 4>(c, a);
-    case 4: return launch_k<4, 4>(c, a);
+    case 0: return launch_k<0, CHAMP_NL_QB>(c, a);
+    case 1: return launch_k<1, CHAMP_NL_QB>(c, a);
+    case 2: return launch_k<2, CHAMP_NL_QB>(c, a);
+    case 3: return launch_k<3, CHAMP_NL_QB>(c, a);
+    case 4: return launch_k<4, CHAMP_NL_QB>(c, a);
     case 5: return launch_k<5, 2>(c, a);
     default:
         set_error("nonloc: e-e-n order above 5 is not supported");

@@ -60,6 +60,8 @@ void *dyn_shared() { return ((Block *)tctx.block)->shmem.data(); }
 
 void syncthreads() { ((Block *)tctx.block)->bar.arrive_and_wait(); }
 
+void syncwarp() { ((Block *)tctx.block)->waves[tctx.wave]->bar.arrive_and_wait(); }
+
 void wave_exchange(uint64_t v, uint64_t *all)
 {
     Block *b = (Block *)tctx.block;

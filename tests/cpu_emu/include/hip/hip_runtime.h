@@ -50,6 +50,7 @@ extern thread_local ThreadCtx tctx;
 void *dyn_shared();
 void syncthreads();
 void wave_exchange(uint64_t v, uint64_t *all);  // every lane of the wave deposits, all read
+void syncwarp();                                 // barrier of the lanes of one wave
 int atomic_add_i(int *p, int v);
 double atomic_add_d(double *p, double v);
 int atomic_or_i(int *p, int v);
@@ -87,6 +88,7 @@ struct Thunk {
 #define gridDim (emu::tctx.gdim)
 
 inline void __syncthreads() { emu::syncthreads(); }
+inline void __syncwarp() { emu::syncwarp(); }
 template <typename T>
 inline T __shfl(T v, int src, int width = 64) { return emu::shfl_generic(v, src, width); }
 template <typename T>
