@@ -166,17 +166,36 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
                 }
             }
             if constexpr (NORDC > 1) {
+                // partner distances to centre c with one centre of lookahead
+                // (slots past the row read electron 0 and are not used)
+                double rjn[HJB], rrjn[HJB];
+                int jcl[HJB];
+#pragma unroll
+                for (int s = 0; s < HJB; s++) {
+                    const int j = j0 + s * WAVE + lane;
+                    jcl[s] = j < i ? j : 0;
+                    rjn[s] = rw[jcl[s]];
+                    rrjn[s] = rrw[jcl[s]];
+                }
                 for (int c = 0; c < NC; c++) {
                     const double *rt = rowt + (size_t)c * RS;
                     const double ri = rt[3 * NA], rri = rt[3 * NA + 1], dd7 = rt[3 * NA + 2], dd9 = rt[3 * NA + 3];
                     const double rvi0 = rt[3 * NA + 4], rvi1 = rt[3 * NA + 5], rvi2 = rt[3 * NA + 6], iri = rt[3 * NA + 7];
                     const double cx = a.cent[3 * c], cy = a.cent[3 * c + 1], cz = a.cent[3 * c + 2];
+                    double rjc[HJB], rrjc[HJB];
+#pragma unroll
+                    for (int s = 0; s < HJB; s++) {
+                        rjc[s] = rjn[s]; rrjc[s] = rrjn[s];
+                        if (c + 1 < NC) {
+                            rjn[s] = rw[(size_t)(c + 1) * NE + jcl[s]];
+                            rrjn[s] = rrw[(size_t)(c + 1) * NE + jcl[s]];
+                        }
+                    }
 #pragma unroll
                     for (int s = 0; s < HJB; s++) {
                         if (j0 + s * WAVE >= i) break;
-                        const int j = j0 + s * WAVE + lane;
                         if (!on[s]) continue;
-                        const double rj = rw[(size_t)c * NE + j], rrj = rrw[(size_t)c * NE + j];
+                        const double rj = rjc[s], rrj = rrjc[s];
                         if (rri == asymp_r && rrj == asymp_r) continue;
                         const double swj = 1.0 - kap * rrj;
                         const double dd8 = -kap * swj, dd10 = -kap * (-kap * swj);

@@ -262,17 +262,36 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                 }
             }
             if constexpr (NORDC > 1) {
+                // distances of the partners to centre c, one centre of lookahead
+                // (slots past NE read electron 0 and are not used)
+                double rjn[MJB], rrjn[MJB];
+                int jcl[MJB];
+#pragma unroll
+                for (int s = 0; s < MJB; s++) {
+                    const int j = j0 + s * WAVE + lane;
+                    jcl[s] = j < NE ? j : 0;
+                    rjn[s] = rw[jcl[s]];
+                    rrjn[s] = rrw[jcl[s]];
+                }
                 for (int c = 0; c < NC; c++) {
                     const double *tab = Ltab + (size_t)c * 2 * NA;
                     const double rrm = Lrr[c], fmf = Lfm[c];
                     const double rvm0 = Lrv[3 * c], rvm1 = Lrv[3 * c + 1], rvm2 = Lrv[3 * c + 2];
                     const double cx = a.cent[3 * c], cy = a.cent[3 * c + 1], cz = a.cent[3 * c + 2];
+                    double rjc[MJB], rrjc[MJB];
+#pragma unroll
+                    for (int s = 0; s < MJB; s++) {
+                        rjc[s] = rjn[s]; rrjc[s] = rrjn[s];
+                        if (c + 1 < NC) {
+                            rjn[s] = rw[(size_t)(c + 1) * NE + jcl[s]];
+                            rrjn[s] = rrw[(size_t)(c + 1) * NE + jcl[s]];
+                        }
+                    }
 #pragma unroll
                     for (int s = 0; s < MJB; s++) {
                         if (j0 + s * WAVE >= NE) break;
-                        const int j = j0 + s * WAVE + lane;
                         if (!on[s]) continue;
-                        const double rjo = rw[(size_t)c * NE + j], rrjo = rrw[(size_t)c * NE + j];
+                        const double rjo = rjc[s], rrjo = rrjc[s];
                         if (rrm == asymp_r && rrjo == asymp_r) continue;
                         const double swj = 1.0 - kap * rrjo;
                         const EenOut e = een_eval_tab<NORDC, 1>(tab, uu1[s], swj);

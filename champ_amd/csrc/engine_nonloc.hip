@@ -178,9 +178,12 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
                             if constexpr (NORDC > 1) een_tables<NORDC, 0>(jp.c[it], 1.0 - kap * rr, A[qq]);
                         }
                         if constexpr (NORDC > 1) {
+                            // one partner of lookahead on the scaled distance
+                            double rr_next = jg < NE ? rrt[(size_t)jg * NC + c] : 0.0;
                             for (int j = jg; j < NE; j += JG) {
+                                const double rrj = rr_next;
+                                if (j + JG < NE) rr_next = rrt[(size_t)(j + JG) * NC + c];
                                 if (j == i) continue;
-                                const double rrj = rrt[(size_t)j * NC + c];
                                 const double rj = 1.0 - kap * rrj;
                                 const double *uj = U + (size_t)j * QB;
                                 double P[QB];
