@@ -66,6 +66,11 @@ __device__ __forceinline__ T powi(T x, int m)
 #define CHAMP_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
 #endif
 
+// four doubles: the accumulator operand of the FP64 MFMA instruction
+#ifndef CHAMP_DBL4_DEFINED
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+#endif
+
 // ------------------------------------------------------------------- tables
 struct SplineTab {
     const double *rwf, *d2rwf, *ce, *ae, *rmaxwf;

@@ -4,64 +4,70 @@
 //   inverse rows contracted with the occupied MO coefficients for the
 //   quadrature ratios (nonlocd_quad, nonloc.f90:462-506, reassociated):
 //       wmat[b][i][m] = sum_j slmi[b][i][j] * cocc[j][m] ("NN", batched)
-// One LDS-tiled kernel: 64x64 output tile per 256-thread block, 4x4 per
-// thread, K staged through LDS in slabs of 16.
+// Matrix cores: v_mfma_f64_16x16x4f64, one 16x16 output tile per wavefront.
 #include "engine_internal.hpp"
 
 namespace champ {
 
-#define GB_M 64
-#define GB_N 64
-#define GB_K 16
-
-template <bool BT>
-__global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const double *__restrict__ A, int lda,
-                                              long long sA, const double *__restrict__ B, int ldb, long long sB,
-                                              double *__restrict__ Cm, int ldc, long long sC)
+// FP64 MFMA GEMM.  The instruction computes D(16x16) += A(16x4) B(4x16) with
+// lane l holding A[l % 16][l / 16], B[l / 16][l % 16] and D[4 (l / 16) + r][l % 16]
+// (r = 0..3).  A K-chunk of 16 is fed as four instructions in which lane l's
+// k-slot (l / 16) carries k = k0 + 4 (l / 16) + s at step s: the same mapping
+// for A and B, so the chunk's 16 products are all summed, and each lane reads
+// 4 consecutive k of its row (a contiguous 32 bytes of A, and of B^T).
+// One wavefront owns one 16 x (16 WN) tile of C; no LDS, no block barrier.
+template <bool BT, int WN>
+__global__ void __launch_bounds__(256) k_gemm_mfma(int M, int N, int K, const double *__restrict__ A, int lda,
+                                                   long long sA, const double *__restrict__ B, int ldb, long long sB,
+                                                   double *__restrict__ Cm, int ldc, long long sC)
 {
-    __shared__ double As[GB_K][GB_M + 1];
-    __shared__ double Bs[GB_K][GB_N + 1];
-    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
-    const int m0 = blockIdx.x * GB_M, n0 = blockIdx.y * GB_N;
-    A += (long long)blockIdx.z * sA;
-    B += (long long)blockIdx.z * sB;
-    Cm += (long long)blockIdx.z * sC;
-    double acc[4][4];
-    for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
-    for (int k0 = 0; k0 < K; k0 += GB_K) {
-        // A slab: 64 rows x 16 k, 4 elements per thread, k fastest in memory
-        for (int t = tid; t < GB_M * GB_K; t += 256) {
-            const int r = t / GB_K, kk = t % GB_K;
-            const int gm = m0 + r, gk = k0 + kk;
-            As[kk][r] = (gm < M && gk < K) ? A[(long long)gm * lda + gk] : 0.0;
-        }
-        for (int t = tid; t < GB_N * GB_K; t += 256) {
-            int nn, kk;
-            if (BT) { nn = t / GB_K; kk = t % GB_K; }
-            else { kk = t / GB_N; nn = t % GB_N; }
-            const int gn = n0 + nn, gk = k0 + kk;
-            double v = 0.0;
-            if (gn < N && gk < K) v = BT ? B[(long long)gn * ldb + gk] : B[(long long)gk * ldb + gn];
-            Bs[kk][nn] = v;
-        }
-        __syncthreads();
+    const int lane = threadIdx.x % WAVE;
+    const int nrt = (M + 15) / 16, ncg = (N + 16 * WN - 1) / (16 * WN);
+    const long long g = (long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (g >= (long long)nrt * ncg) return;  // whole wavefront
+    const int cg = (int)(g % ncg), rt = (int)(g / ncg);
+    A += (long long)blockIdx.y * sA;
+    B += (long long)blockIdx.y * sB;
+    Cm += (long long)blockIdx.y * sC;
+    const int row = rt * 16 + lane % 16, kq = 4 * (lane / 16);
+    const bool rok = row < M;
+    const double *Ar = A + (long long)(rok ? row : 0) * lda;
+    int col[WN];
+    bool cok[WN];
 #pragma unroll
-        for (int kk = 0; kk < GB_K; kk++) {
-            double a[4], b[4];
-            for (int i = 0; i < 4; i++) a[i] = As[kk][ty + 16 * i];
-            for (int j = 0; j < 4; j++) b[j] = Bs[kk][tx + 16 * j];
-            for (int i = 0; i < 4; i++)
-                for (int j = 0; j < 4; j++) acc[i][j] += a[i] * b[j];
-        }
-        __syncthreads();
+    for (int t = 0; t < WN; t++) {
+        col[t] = cg * 16 * WN + t * 16 + lane % 16;
+        cok[t] = col[t] < N;
     }
-    for (int i = 0; i < 4; i++) {
-        const int gm = m0 + ty + 16 * i;
-        if (gm >= M) continue;
-        for (int j = 0; j < 4; j++) {
-            const int gn = n0 + tx + 16 * j;
-            if (gn < N) Cm[(long long)gm * ldc + gn] = acc[i][j];
+    dbl4 acc[WN];
+#pragma unroll
+    for (int t = 0; t < WN; t++) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const int kb = k0 + kq;
+        double a[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) a[s] = (rok && kb + s < K) ? Ar[kb + s] : 0.0;
+#pragma unroll
+        for (int t = 0; t < WN; t++) {
+            double b[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const bool ok = cok[t] && kb + s < K;
+                const long long off = BT ? (long long)(cok[t] ? col[t] : 0) * ldb + (kb + s < K ? kb + s : 0)
+                                         : (long long)(kb + s < K ? kb + s : 0) * ldb + (cok[t] ? col[t] : 0);
+                b[s] = ok ? B[off] : 0.0;
+            }
+#pragma unroll
+            for (int s = 0; s < 4; s++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < WN; t++) {
+        if (!cok[t]) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int ro = rt * 16 + 4 * (lane / 16) + r;
+            if (ro < M) Cm[(long long)ro * ldc + col[t]] = acc[t][r];
         }
     }
 }
@@ -72,11 +78,17 @@ int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int
                 int ldb, long long sB, double *Cm, int ldc, long long sC, int batch)
 {
     if (M <= 0 || N <= 0 || batch <= 0) return CHAMP_OK;
-    dim3 grid((M + GB_M - 1) / GB_M, (N + GB_N - 1) / GB_N, batch);
+    const long long waves = (long long)((M + 15) / 16) * ((N + 15) / 16);
+    const long long blocks = (waves + 3) / 4;
+    if (blocks > 0x7fffffffLL || batch > 65535) {
+        set_error("gemm: problem too large for one launch");
+        return CHAMP_ERR_UNSUPPORTED;
+    }
+    dim3 grid((unsigned)blocks, batch);
     if (bt)
-        hipLaunchKernelGGL(k_gemm<true>, grid, dim3(256), 0, c->stream, M, N, K, A, lda, sA, B, ldb, sB, Cm, ldc, sC);
+        hipLaunchKernelGGL((k_gemm_mfma<true, 1>), grid, dim3(256), 0, c->stream, M, N, K, A, lda, sA, B, ldb, sB, Cm, ldc, sC);
     else
-        hipLaunchKernelGGL(k_gemm<false>, grid, dim3(256), 0, c->stream, M, N, K, A, lda, sA, B, ldb, sB, Cm, ldc, sC);
+        hipLaunchKernelGGL((k_gemm_mfma<false, 1>), grid, dim3(256), 0, c->stream, M, N, K, A, lda, sA, B, ldb, sB, Cm, ldc, sC);
     HIPCHECK(hipGetLastError());
     return CHAMP_OK;
 }

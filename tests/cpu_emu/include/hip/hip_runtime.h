@@ -111,6 +111,35 @@ inline int __builtin_amdgcn_readfirstlane(int v)
         }
     return (int)(uint32_t)all[base];
 }
+// FP64 MFMA 16x16x4: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]
+// and holds D[4 (l / 16) + r][l % 16], r = 0..3
+#define CHAMP_DBL4_DEFINED
+struct dbl4 {
+    double v[4];
+    double &operator[](int i) { return v[i]; }
+    const double &operator[](int i) const { return v[i]; }
+};
+inline dbl4 __builtin_amdgcn_mfma_f64_16x16x4f64(double a, double b, dbl4 c, int, int, int)
+{
+    uint64_t ua, ub, all_a[64], all_b[64];
+    memcpy(&ua, &a, 8);
+    memcpy(&ub, &b, 8);
+    emu::wave_exchange(ua, all_a);
+    emu::wave_exchange(ub, all_b);
+    const int l = emu::tctx.lane % 64, j = l % 16;
+    for (int r = 0; r < 4; r++) {
+        const int i = 4 * (l / 16) + r;
+        double s = c[r];
+        for (int k = 0; k < 4; k++) {
+            double x, y;
+            memcpy(&x, &all_a[k * 16 + i], 8);
+            memcpy(&y, &all_b[k * 16 + j], 8);
+            s += x * y;
+        }
+        c[r] = s;
+    }
+    return c;
+}
 inline int atomicAdd(int *p, int v) { return emu::atomic_add_i(p, v); }
 inline double atomicAdd(double *p, double v) { return emu::atomic_add_d(p, v); }
 inline int atomicOr(int *p, int v) { return emu::atomic_or_i(p, v); }
