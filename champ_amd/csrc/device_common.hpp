@@ -74,20 +74,34 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // ------------------------------------------------------------------- tables
 struct SplineTab {
     const double *rwf, *d2rwf, *ce, *ae, *rmaxwf;
+    const double *gref;  // [nr][2]: the grid-3 node radii (ref0, ref1) of interval jx
     int nr, nrbas_max;
     double r0, arg, dlogag;
 };
 
-// splfit.f90:3-94, grid type 3.  ider 0 -> wfv[0]; ider >= 1 -> wfv[0..2]
-// (R''' is only needed by forces and is not produced).
+// The node radii of interval jx exactly as splfit.f90 forms them; stored per
+// interval at set-up so the evaluation below needs no power loop.
+__device__ __forceinline__ void grid_refs(double r0, double arg, int jx, double &ref0, double &ref1)
+{
+    ref0 = r0 * powi(arg, jx - 1) - r0;
+    ref1 = (ref0 + r0) * arg - r0;
+}
+
+// Grid-3 interval of radius r (splfit.f90:22-24): 0 below the grid, nr past it.
+__device__ __forceinline__ int grid_jx(const SplineTab &tab, double r)
+{
+    const double xr = log((r + tab.r0) / tab.r0) / tab.dlogag + 1.0;
+    // compare before converting: a NaN or huge r must never become a table index
+    return (xr >= 1.0 && xr < (double)tab.nr) ? (int)xr : (xr >= (double)tab.nr ? tab.nr : 0);
+}
+
+// splfit.f90:3-94, grid type 3, with the interval jx of r already known.
+// ider 0 -> wfv[0]; ider >= 1 -> wfv[0..2] (the third derivative is only
+// needed by forces and is not produced).
 template <typename T>
-__device__ __forceinline__ void splfit(const SplineTab &tab, int it, int irb, T r, T wfv[3], int ider)
+__device__ __forceinline__ void splfit_jx(const SplineTab &tab, int it, int irb, T r, int jx, T wfv[3], int ider)
 {
     const size_t base = ((size_t)it * tab.nrbas_max + irb);
-    const double r0 = tab.r0, arg = tab.arg;
-    double xr = log(((double)r + r0) / r0) / tab.dlogag + 1.0;
-    // compare before converting: a NaN or huge r must never become a table index
-    int jx = (xr >= 1.0 && xr < (double)tab.nr) ? (int)xr : (xr >= (double)tab.nr ? tab.nr : 0);
     if (jx < 1) {
         const double *ce = tab.ce + base * 5;
         T v = (T)ce[0];
@@ -111,8 +125,7 @@ __device__ __forceinline__ void splfit(const SplineTab &tab, int it, int irb, T 
     } else {
         const double *rwf = tab.rwf + base * tab.nr;
         const double *d2rwf = tab.d2rwf + base * tab.nr;
-        double ref0 = r0 * powi(arg, jx - 1) - r0;
-        double ref1 = (ref0 + r0) * arg - r0;
+        const double ref0 = tab.gref[2 * jx], ref1 = tab.gref[2 * jx + 1];
         T delh = (T)(ref1 - ref0);
         T y0 = (T)rwf[jx - 1], y1 = (T)rwf[jx], s0 = (T)d2rwf[jx - 1], s1 = (T)d2rwf[jx];
         const T sixth = T(1) / T(6);
@@ -127,6 +140,12 @@ __device__ __forceinline__ void splfit(const SplineTab &tab, int it, int irb, T 
             wfv[2] = aa * s0 + bb * s1;
         }
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void splfit(const SplineTab &tab, int it, int irb, T r, T wfv[3], int ider)
+{
+    splfit_jx<T>(tab, it, irb, r, grid_jx(tab, (double)r), wfv, ider);
 }
 
 // slm.f90:7-200 (ider < 3): monomial y, gradient dy, Laplacian.

@@ -81,6 +81,17 @@ static double hpowi(double x, int m)
     return m < 0 ? 1.0 / y : y;
 }
 
+// node radii of every grid-3 interval (see grid_refs)
+__global__ void k_grid_refs(int nr, double r0, double arg, double *gref)
+{
+    const int jx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jx >= nr) return;
+    double ref0, ref1;
+    grid_refs(r0, arg, jx, ref0, ref1);
+    gref[2 * jx] = ref0;
+    gref[2 * jx + 1] = ref1;
+}
+
 template <typename T>
 static int dalloc(T **p, size_t n)
 {
@@ -160,6 +171,12 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
     c->tab.rwf = d_rwf; c->tab.d2rwf = d_d2; c->tab.ce = d_ce; c->tab.ae = d_ae; c->tab.rmaxwf = d_rmax;
     c->tab.nr = s->nr; c->tab.nrbas_max = s->nrbas_max; c->tab.r0 = s->grid_r0; c->tab.arg = s->grid_arg;
     c->tab.dlogag = std::log(s->grid_arg);
+    TRY(dalloc(&c->gref, (size_t)2 * s->nr));
+    hipLaunchKernelGGL(k_grid_refs, dim3((s->nr + 255) / 256), dim3(256), 0, c->stream, s->nr, s->grid_r0, s->grid_arg,
+                       c->gref);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    c->tab.gref = c->gref;
     // ---- AO table (basis_fns ordering: centres in order, AOs of the centre's type)
     std::vector<int> aoc, aot, aor, aol;
     std::vector<double> aorm;
@@ -176,6 +193,25 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
     TRY(upload(&c->ao_cent, aoc.data(), NB)); TRY(upload(&c->ao_type, aot.data(), NB));
     TRY(upload(&c->ao_rad, aor.data(), NB)); TRY(upload(&c->ao_slm, aol.data(), NB));
     TRY(upload(&c->rmax_ao, aorm.data(), NB));
+    // radial instances (centre, radial function) and each AO's instance
+    {
+        std::vector<int> ric, rit, rir, rfirst(NC);
+        std::vector<double> rirm;
+        for (int ic = 0; ic < NC; ic++) {
+            const int it = s->iwctype[ic];
+            rfirst[ic] = (int)ric.size();
+            for (int ir = 0; ir < s->nrbas[it]; ir++) {
+                ric.push_back(ic); rit.push_back(it); rir.push_back(ir);
+                rirm.push_back(s->rmaxwf[it * s->nrbas_max + ir]);
+            }
+        }
+        std::vector<int> aori(NB);
+        for (int m = 0; m < NB; m++) aori[m] = rfirst[aoc[m]] + aor[m];
+        c->NRI = (int)ric.size();
+        TRY(upload(&c->ri_cent, ric.data(), ric.size())); TRY(upload(&c->ri_type, rit.data(), rit.size()));
+        TRY(upload(&c->ri_rad, rir.data(), rir.size())); TRY(upload(&c->ri_rmax, rirm.data(), rirm.size()));
+        TRY(upload(&c->ao_ri, aori.data(), NB));
+    }
     TRY(upload(&c->coef, s->coef, (size_t)NO * NB));
     TRY(upload(&c->iworbd, s->iworbd, (size_t)NE));
     for (int iab = 0; iab < 2; iab++) {
@@ -321,7 +357,8 @@ extern "C" int champ_gpu_free(champ_ctx *c)
                     c->rng, c->xq, c->yq, c->zq, c->xnew, c->dfus2o, c->rnew, c->rrnew, c->fsn_row, c->g1n, c->g2n,
                     c->vjnT, c->fsumn, c->aomove, c->orbn, c->ratio, c->psidn, c->psijn, c->vnew, c->vold, c->acc,
                     c->esum_dev, c->aoall, c->orball, c->wmat, c->pair_count, c->pair_off, c->pair_total,
-                    c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->rrT, c->nxquad};
+                    c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->rrT, c->nxquad, c->ri_cent, c->ri_type,
+                    c->ri_rad, c->ri_rmax, c->ao_ri, c->gref};
     for (void *p : ptrs) if (p) hipFree(p);
     for (auto &t : c->ev_pool) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     if (c->stream) hipStreamDestroy(c->stream);
@@ -575,11 +612,14 @@ extern "C" int champ_gpu_basis_fns(champ_ctx *c, int32_t npt, const double *pos,
     double *dpos = nullptr, *dout = nullptr;
     HIPCHECK(hipMalloc(&dpos, (size_t)npt * 3 * 8));
     if (hipMalloc(&dout, (size_t)npt * 5 * c->NB * 8) != hipSuccess) { hipFree(dpos); set_error("basis_fns: alloc"); return CHAMP_ERR_NOMEM; }
-    hipMemcpy(dpos, pos, (size_t)npt * 3 * 8, hipMemcpyHostToDevice);
-    int rc = launch_basis_points(c, npt, dpos, dout);
-    if (!rc) {
-        hipStreamSynchronize(c->stream);
-        hipMemcpy(phin, dout, (size_t)npt * 5 * c->NB * 8, hipMemcpyDeviceToHost);
+    int rc = CHAMP_OK;
+    hipError_t e = hipMemcpy(dpos, pos, (size_t)npt * 3 * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) rc = launch_basis_points(c, npt, dpos, dout);
+    if (e == hipSuccess && !rc) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && !rc) e = hipMemcpy(phin, dout, (size_t)npt * 5 * c->NB * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        set_error(std::string("basis_fns: ") + hipGetErrorString(e));
+        rc = CHAMP_ERR_HIP;
     }
     hipFree(dpos);
     hipFree(dout);

@@ -44,6 +44,12 @@ struct champ_ctx {
     int *iwctype = nullptr;          // [NC]
     champ::SplineTab tab{};
     double *rmax_ao = nullptr;       // [NB] rmaxwf of each AO's shell
+    // radial instances: one per (centre, radial function of its type)
+    int NRI = 0;
+    int *ri_cent = nullptr, *ri_type = nullptr, *ri_rad = nullptr;  // [NRI]
+    double *ri_rmax = nullptr;                                      // [NRI]
+    int *ao_ri = nullptr;                                           // [NB] radial instance of each AO
+    double *gref = nullptr;                                         // [nr][2] grid node radii
     int *ao_cent = nullptr, *ao_type = nullptr, *ao_rad = nullptr, *ao_slm = nullptr;  // [NB]
     double *coef = nullptr;          // [NO][NB]
     double *cocc[2] = {nullptr, nullptr};  // [nel][NB] coef rows of occupied orbitals

@@ -88,6 +88,9 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
     double *S = U + (size_t)NE * QB;        // [3][QB][NL_WAVES] per-wave sums: e-e, e-e-n + e-n, psi ratio
     double *XQ = S + 3 * QB * NL_WAVES;     // [QB][3] quadrature points
     double *FI = XQ + 3 * QB;               // [NL_WAVES] partial sums of F[i][.]
+    double *GEO = FI + NL_WAVES;            // [QB][NC][5] point - centre: x, y, z, r, grid interval
+    double *RAD = GEO + (size_t)QB * NC * 5;  // [QB][NRI] radial functions at the points
+    const int NRI = a.NRI;
 
     // XCD-aware contiguous ranges: workgroups are dealt round-robin to the 8
     // XCDs, so logical block lb = (b % 8) * (G / 8) + b / 8 gives each XCD one
@@ -172,6 +175,11 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
                             r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
                             const double rq = c == ic ? ren : sqrt(r2);
                             const double rr = scale_r(kap, rq);
+                            if (jg == 0) {  // geometry of the point seen from centre c, for the AOs
+                                double *g = GEO + ((size_t)qq * NC + c) * 5;
+                                g[0] = d0; g[1] = d1; g[2] = d2; g[3] = rq;
+                                g[4] = (double)grid_jx(a.tab, rq);
+                            }
                             asym[qq] = rr == asymp;
                             any_asym = any_asym || asym[qq];
                             if (jg == 0 && q0 + qq < NQ) acc[qq] += en_term(jp, it, rr);
@@ -207,23 +215,29 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
                 if (lane == 0) S[(1 * QB + qq) * NL_WAVES + wv] = v;
             }
 
-            // ---- phase C: psi ratio sum_m phi_m(x_q) W[i][m]
+            // ---- phase C: psi ratio sum_m phi_m(x_q) W[i][m]; radial functions once
+            // per (point, centre, shell), the angular factor per AO
+            __syncthreads();  // GEO complete
+            for (int t = tid; t < QB * NRI; t += NL_THREADS) {
+                const int qq = t / NRI, ri = t % NRI;
+                const double *g = GEO + ((size_t)qq * NC + a.ri_cent[ri]) * 5;
+                double wfv[3] = {0.0, 0.0, 0.0};
+                if (g[3] <= a.ri_rmax[ri]) splfit_jx<double>(a.tab, a.ri_type[ri], a.ri_rad[ri], g[3], (int)g[4], wfv, 0);
+                RAD[(size_t)qq * NRI + ri] = wfv[0];
+            }
+            __syncthreads();
             double dr[QB];
 #pragma unroll
             for (int qq = 0; qq < QB; qq++) dr[qq] = 0.0;
             for (int m = tid; m < NB; m += NL_THREADS) {
-                const int c = a.ao_cent[m];
+                const int c = a.ao_cent[m], ri = a.ao_ri[m], l = a.ao_slm[m];
                 const double wm = Wm[m];
-                const double ecx = a.cent[3 * c], ecy = a.cent[3 * c + 1], ecz = a.cent[3 * c + 2];
 #pragma unroll
                 for (int qq = 0; qq < QB; qq++) {
-                    const double d0 = XQ[3 * qq] - ecx, d1 = XQ[3 * qq + 1] - ecy, d2 = XQ[3 * qq + 2] - ecz;
-                    double r2 = 0;
-                    r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
-                    const double r = c == ic ? ren : sqrt(r2);
-                    double o[5];
-                    ao_eval<double>(a.tab, a.ao_type[m], a.ao_rad[m], a.ao_slm[m], d0, d1, d2, r, a.rmax_ao[m], 0, o);
-                    dr[qq] += o[0] * wm;
+                    const double *g = GEO + ((size_t)qq * NC + c) * 5;
+                    double y, dy[3], lap;
+                    slm<double>(l, g[0], g[1], g[2], y, dy, lap);
+                    dr[qq] += (y * RAD[(size_t)qq * NRI + ri]) * wm;
                 }
             }
 #pragma unroll
@@ -264,7 +278,8 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
 template <int NORDC, int QB>
 static int launch_k(champ_ctx *c, const HArgs &a)
 {
-    const size_t lds = ((size_t)c->NE * QB + 3 * QB * NL_WAVES + 3 * QB + NL_WAVES) * sizeof(double);
+    const size_t lds = ((size_t)c->NE * QB + 3 * QB * NL_WAVES + 3 * QB + NL_WAVES + (size_t)QB * c->NC * 5 +
+                        (size_t)QB * c->NRI) * sizeof(double);
     if (lds > 160 * 1024) {
         set_error("nonloc: too many electrons for the quadrature kernel's LDS table");
         return CHAMP_ERR_UNSUPPORTED;

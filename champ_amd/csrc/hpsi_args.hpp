@@ -23,6 +23,9 @@ struct HArgs {
     int *pair_count, *pair_off, *pair_total, *pair_e, *pair_c, *nxquad;
     double *pair_vps, *pair_val;
     double *rrT;  // [W][NE][NC] scaled e-n distances, centre index innermost
+    int NRI;      // radial instances (centre, radial function)
+    const int *ri_cent, *ri_type, *ri_rad, *ao_ri;
+    const double *ri_rmax;
     double *xq, *yq, *zq;
     double *esum_dev;
     int *status;
@@ -44,6 +47,8 @@ inline HArgs hargs(champ_ctx *c)
     a.pair_count = c->pair_count; a.pair_off = c->pair_off; a.pair_total = c->pair_total; a.pair_e = c->pair_e;
     a.pair_c = c->pair_c; a.nxquad = c->nxquad; a.pair_vps = c->pair_vps; a.pair_val = c->pair_val;
     a.rrT = c->rrT;
+    a.NRI = c->NRI; a.ri_cent = c->ri_cent; a.ri_type = c->ri_type; a.ri_rad = c->ri_rad; a.ao_ri = c->ao_ri;
+    a.ri_rmax = c->ri_rmax;
     a.xq = c->xq; a.yq = c->yq; a.zq = c->zq; a.esum_dev = c->esum_dev; a.status = c->status;
     return a;
 }
