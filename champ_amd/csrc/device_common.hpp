@@ -182,14 +182,12 @@ __device__ __forceinline__ void slm(int l, T x, T yy, T z, T &y, T dy[3], T &lap
     }
 }
 
-// One AO at one point: basis_fns.f90:72-110 + phi_combine :217-296.
+// One AO from its radial function R, R', R'' (wfv) and its monomial l at the
+// point (rx, ry, rz) = x - centre, |.| = r: phi_combine (basis_fns.f90:217-296).
 // Returns value, gradient and (ider == 2) Laplacian.
 template <typename T>
-__device__ __forceinline__ void ao_eval(const SplineTab &tab, int it, int irad, int l, T rx, T ry, T rz, T r,
-                                        double rmax, int ider, T out[5])
+__device__ __forceinline__ void ao_combine(int l, T rx, T ry, T rz, T r, const T wfv[3], int ider, T out[5])
 {
-    T wfv[3] = {0, 0, 0};
-    if ((double)r <= rmax) splfit<T>(tab, it, irad, r, wfv, ider);
     T y, dy[3], lap;
     slm<T>(l, rx, ry, rz, y, dy, lap);
     out[0] = y * wfv[0];
@@ -206,6 +204,16 @@ __device__ __forceinline__ void ao_eval(const SplineTab &tab, int it, int irad, 
             out[4] = d2 + T(2) * dum * wfv[1];
         }
     }
+}
+
+// One AO at one point: basis_fns.f90:72-110 + phi_combine :217-296.
+template <typename T>
+__device__ __forceinline__ void ao_eval(const SplineTab &tab, int it, int irad, int l, T rx, T ry, T rz, T r,
+                                        double rmax, int ider, T out[5])
+{
+    T wfv[3] = {0, 0, 0};
+    if ((double)r <= rmax) splfit<T>(tab, it, irad, r, wfv, ider);
+    ao_combine<T>(l, rx, ry, rz, r, wfv, ider, out);
 }
 
 // ----------------------------------------------------------------- Jastrow

@@ -255,20 +255,30 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
 // (walker, electron), lanes over AOs.  Chunk of walkers [w0, w0+nw).
 __global__ void __launch_bounds__(256) k_hpsi_ao(HArgs a)
 {
-    const int lane = threadIdx.x % WAVE;
-    const long long gw = (long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    if (gw >= (long long)a.nw * a.NE) return;
+    HIP_DYNAMIC_SHARED(double, lds)
+    const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+    const long long gw = (long long)blockIdx.x * (blockDim.x / WAVE) + wv;
+    if (gw >= (long long)a.nw * a.NE) return;  // whole wavefront
     const int wc = (int)(gw / a.NE), e = (int)(gw % a.NE), w = a.w0 + wc;
-    const int NE = a.NE;
+    const int NE = a.NE, NRI = a.NRI;
     const double *xw = a.xT + (size_t)w * 3 * NE;
     const double *rw = a.r_en + (size_t)w * a.NC * NE;
     double *out = a.aoall + (size_t)gw * 5 * a.NB;
     const double x0 = xw[e], x1 = xw[NE + e], x2 = xw[2 * NE + e];
+    // radial functions (R, R', R'') once per (centre, shell), in the wave's LDS
+    double *R = lds + (size_t)wv * 3 * NRI;
+    for (int ri = lane; ri < NRI; ri += WAVE) {
+        const double r = rw[(size_t)a.ri_cent[ri] * NE + e];
+        double wfv[3] = {0.0, 0.0, 0.0};
+        if (r <= a.ri_rmax[ri]) splfit_jx<double>(a.tab, a.ri_type[ri], a.ri_rad[ri], r, grid_jx(a.tab, r), wfv, 2);
+        R[3 * ri] = wfv[0]; R[3 * ri + 1] = wfv[1]; R[3 * ri + 2] = wfv[2];
+    }
+    __syncwarp();
     for (int m = lane; m < a.NB; m += WAVE) {
         const int c = a.ao_cent[m];
         double o[5];
-        ao_eval<double>(a.tab, a.ao_type[m], a.ao_rad[m], a.ao_slm[m], x0 - a.cent[3 * c], x1 - a.cent[3 * c + 1],
-                        x2 - a.cent[3 * c + 2], rw[(size_t)c * NE + e], a.rmax_ao[m], 2, o);
+        ao_combine<double>(a.ao_slm[m], x0 - a.cent[3 * c], x1 - a.cent[3 * c + 1], x2 - a.cent[3 * c + 2],
+                           rw[(size_t)c * NE + e], R + 3 * a.ao_ri[m], 2, o);
         for (int k = 0; k < 5; k++) out[(size_t)k * a.NB + m] = o[k];
     }
 }
@@ -581,7 +591,8 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
         a.nw = std::min(c->hpsi_chunk, c->W - w0);
         const long long nwave = (long long)a.nw * c->NE;
         tbegin(c, K_HPSI_AO, &slot);
-        hipLaunchKernelGGL(k_hpsi_ao, dim3((unsigned)((nwave + 3) / 4)), dim3(256), 0, c->stream, a);
+        hipLaunchKernelGGL(k_hpsi_ao, dim3((unsigned)((nwave + 3) / 4)), dim3(256), (size_t)4 * 3 * c->NRI * sizeof(double),
+                           c->stream, a);
         tend(c, slot);
         HIPCHECK(hipGetLastError());
         tbegin(c, K_HPSI_GEMM, &slot);

@@ -23,6 +23,9 @@ struct MoveArgs {
     SplineTab tab;
     const double *rmax_ao;
     const int *ao_cent, *ao_type, *ao_rad, *ao_slm, *iworbd;
+    int NRI;
+    const int *ri_cent, *ri_type, *ri_rad, *ao_ri;
+    const double *ri_rmax;
     const JasParams *jas;
     double *xT, *r_en, *rr_en, *fso, *g1, *g2, *vjT, *fsumo, *dorb, *slmi, *detiab, *psido, *psijo, *psi2o;
     uint64_t *rng;
@@ -40,6 +43,8 @@ static MoveArgs move_args(champ_ctx *c, int iel, int mode, double tau)
     a.cent = c->cent; a.iwctype = c->iwctype; a.tab = c->tab; a.rmax_ao = c->rmax_ao;
     a.ao_cent = c->ao_cent; a.ao_type = c->ao_type; a.ao_rad = c->ao_rad; a.ao_slm = c->ao_slm;
     a.iworbd = c->iworbd; a.jas = c->jas;
+    a.NRI = c->NRI; a.ri_cent = c->ri_cent; a.ri_type = c->ri_type; a.ri_rad = c->ri_rad; a.ao_ri = c->ao_ri;
+    a.ri_rmax = c->ri_rmax;
     a.xT = c->xT; a.r_en = c->r_en; a.rr_en = c->rr_en; a.fso = c->fso; a.g1 = c->g1; a.g2 = c->g2;
     a.vjT = c->vjT; a.fsumo = c->fsumo; a.dorb = c->dorb; a.slmi = c->slmi; a.detiab = c->detiab;
     a.psido = c->psido; a.psijo = c->psijo; a.psi2o = c->psi2o; a.rng = c->rng;
@@ -104,11 +109,12 @@ __global__ void __launch_bounds__(256) k_drift_old(MoveArgs a)
 constexpr int MJB = CHAMP_MJB;
 
 // LDS doubles per wave of k_move_a: r, rr, -kap (1 - kap rr) / r, rvec (3) per
-// centre, then the centre's e-e-n tables (value and d/dx)
-static size_t move_a_lds_per_wave(int NC, int nordc)
+// centre, the centre's e-e-n tables (value and d/dx), then R and R' of each
+// radial instance at the moved electron
+static size_t move_a_lds_per_wave(int NC, int nordc, int NRI)
 {
     const int naz[] = {0, 0, een_naz<2>(), een_naz<3>(), een_naz<4>(), een_naz<5>()};
-    return (size_t)6 * NC + (nordc > 1 && nordc <= 5 ? (size_t)2 * naz[nordc] * NC : 0);
+    return (size_t)6 * NC + (nordc > 1 && nordc <= 5 ? (size_t)2 * naz[nordc] * NC : 0) + (size_t)2 * NRI;
 }
 
 template <int NORDC>
@@ -122,8 +128,9 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
     const int NE = a.NE, NC = a.NC, iel = a.iel;
     const JasParams &jp = *a.jas;
     const double kap = jp.kap, asymp_r = jp.asymp_r;
-    double *Lr = lds + (size_t)wv * (6 * NC + (NORDC > 1 ? 2 * NA * NC : 0));
+    double *Lr = lds + (size_t)wv * (6 * NC + (NORDC > 1 ? 2 * NA * NC : 0) + 2 * a.NRI);
     double *Lrr = Lr + NC, *Lfm = Lr + 2 * NC, *Lrv = Lr + 3 * NC, *Ltab = Lr + 6 * NC;
+    double *Lrad = Ltab + (NORDC > 1 ? 2 * NA * NC : 0);
 
     double xn0 = 0, xn1 = 0, xn2 = 0;
     if (active) {
@@ -331,12 +338,20 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
         }
 
         // ---- AOs of the moved electron (ider = 1) into the GEMM operand
+        // radial functions (R, R') once per (centre, shell), then each AO
+        for (int ri = lane; ri < a.NRI; ri += WAVE) {
+            const double r = Lr[a.ri_cent[ri]];
+            double wfv[3] = {0.0, 0.0, 0.0};
+            if (r <= a.ri_rmax[ri]) splfit_jx<double>(a.tab, a.ri_type[ri], a.ri_rad[ri], r, grid_jx(a.tab, r), wfv, 1);
+            Lrad[2 * ri] = wfv[0]; Lrad[2 * ri + 1] = wfv[1];
+        }
+        __syncwarp();
         double *ao = a.aomove + (size_t)w * 4 * a.NB;
         for (int m = lane; m < a.NB; m += WAVE) {
-            const int c = a.ao_cent[m];
+            const int c = a.ao_cent[m], ri = a.ao_ri[m];
+            const double wfv[3] = {Lrad[2 * ri], Lrad[2 * ri + 1], 0.0};
             double out[5];
-            ao_eval<double>(a.tab, a.ao_type[m], a.ao_rad[m], a.ao_slm[m], Lrv[3 * c], Lrv[3 * c + 1], Lrv[3 * c + 2], Lr[c],
-                            a.rmax_ao[m], 1, out);
+            ao_combine<double>(a.ao_slm[m], Lrv[3 * c], Lrv[3 * c + 1], Lrv[3 * c + 2], Lr[c], wfv, 1, out);
             ao[m] = out[0]; ao[a.NB + m] = out[1]; ao[2 * a.NB + m] = out[2]; ao[3 * a.NB + m] = out[3];
         }
     }
@@ -527,7 +542,7 @@ int launch_move_a(champ_ctx *c, int iel, int mode)
 {
     MoveArgs a = move_args(c, iel, mode, g_tau);
     const int nordc = c->hjas.nordc;
-    const size_t lds = (size_t)4 * move_a_lds_per_wave(c->NC, nordc) * sizeof(double);
+    const size_t lds = (size_t)4 * move_a_lds_per_wave(c->NC, nordc, c->NRI) * sizeof(double);
     if (nordc > 5) {
         set_error("move: e-e-n order above 5 is not supported");
         return CHAMP_ERR_UNSUPPORTED;
