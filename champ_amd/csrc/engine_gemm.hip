@@ -9,6 +9,71 @@
 
 namespace champ {
 
+// Which GEMM kernel launch_gemm uses: 1 = matrix cores (k_gemm_mfma),
+// 0 = the LDS-tiled VALU kernel (k_gemm_valu).
+#ifndef CHAMP_GEMM_MFMA
+#define CHAMP_GEMM_MFMA 1
+#endif
+
+// LDS-tiled VALU GEMM: 64x64 output tile per 256-thread block, 4x4 per
+// thread, K staged through LDS in slabs of 16.
+#define GB_M 64
+#define GB_N 64
+#define GB_K 16
+
+template <bool BT>
+__global__ void __launch_bounds__(256) k_gemm_valu(int M, int N, int K, const double *__restrict__ A, int lda,
+                                              long long sA, const double *__restrict__ B, int ldb, long long sB,
+                                              double *__restrict__ Cm, int ldc, long long sC)
+{
+    __shared__ double As[GB_K][GB_M + 1];
+    __shared__ double Bs[GB_K][GB_N + 1];
+    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    const int m0 = blockIdx.x * GB_M, n0 = blockIdx.y * GB_N;
+    A += (long long)blockIdx.z * sA;
+    B += (long long)blockIdx.z * sB;
+    Cm += (long long)blockIdx.z * sC;
+    double acc[4][4];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+    for (int k0 = 0; k0 < K; k0 += GB_K) {
+        // A slab: 64 rows x 16 k, 4 elements per thread, k fastest in memory
+        for (int t = tid; t < GB_M * GB_K; t += 256) {
+            const int r = t / GB_K, kk = t % GB_K;
+            const int gm = m0 + r, gk = k0 + kk;
+            As[kk][r] = (gm < M && gk < K) ? A[(long long)gm * lda + gk] : 0.0;
+        }
+        for (int t = tid; t < GB_N * GB_K; t += 256) {
+            int nn, kk;
+            if (BT) { nn = t / GB_K; kk = t % GB_K; }
+            else { kk = t / GB_N; nn = t % GB_N; }
+            const int gn = n0 + nn, gk = k0 + kk;
+            double v = 0.0;
+            if (gn < N && gk < K) v = BT ? B[(long long)gn * ldb + gk] : B[(long long)gk * ldb + gn];
+            Bs[kk][nn] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GB_K; kk++) {
+            double a[4], b[4];
+            for (int i = 0; i < 4; i++) a[i] = As[kk][ty + 16 * i];
+            for (int j = 0; j < 4; j++) b[j] = Bs[kk][tx + 16 * j];
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) acc[i][j] += a[i] * b[j];
+        }
+        __syncthreads();
+    }
+    for (int i = 0; i < 4; i++) {
+        const int gm = m0 + ty + 16 * i;
+        if (gm >= M) continue;
+        for (int j = 0; j < 4; j++) {
+            const int gn = n0 + tx + 16 * j;
+            if (gn < N) Cm[(long long)gm * ldc + gn] = acc[i][j];
+        }
+    }
+}
+
+
 // FP64 MFMA GEMM.  The instruction computes D(16x16) += A(16x4) B(4x16) with
 // lane l holding A[l % 16][l / 16], B[l / 16][l % 16] and D[4 (l / 16) + r][l % 16]
 // (r = 0..3).  A K-chunk of 16 is fed as four instructions in which lane l's
@@ -78,6 +143,19 @@ int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int
                 int ldb, long long sB, double *Cm, int ldc, long long sC, int batch)
 {
     if (M <= 0 || N <= 0 || batch <= 0) return CHAMP_OK;
+    if (!CHAMP_GEMM_MFMA) {
+        if (batch > 65535) {
+            set_error("gemm: batch too large for one launch");
+            return CHAMP_ERR_UNSUPPORTED;
+        }
+        dim3 grid((M + GB_M - 1) / GB_M, (N + GB_N - 1) / GB_N, batch);
+        if (bt)
+            hipLaunchKernelGGL(k_gemm_valu<true>, grid, dim3(256), 0, c->stream, M, N, K, A, lda, sA, B, ldb, sB, Cm, ldc, sC);
+        else
+            hipLaunchKernelGGL(k_gemm_valu<false>, grid, dim3(256), 0, c->stream, M, N, K, A, lda, sA, B, ldb, sB, Cm, ldc, sC);
+        HIPCHECK(hipGetLastError());
+        return CHAMP_OK;
+    }
     const long long waves = (long long)((M + 15) / 16) * ((N + 15) / 16);
     const long long blocks = (waves + 3) / 4;
     if (blocks > 0x7fffffffLL || batch > 65535) {

@@ -357,6 +357,39 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
     }
 }
 
+// Sherman-Morrison rows of one wave of a 4-wave block: rows wv + 4 r (r < MR,
+// i != row), columns lane and lane + WAVE (nel <= 2 WAVE).  Same arithmetic
+// as the row-by-row loop; all loads are issued before the reductions.
+template <int MR>
+__device__ __forceinline__ void sm_rows(double *S, int NM, int nel, int row, double rt, const double *s_row,
+                                        const double *s_orb, int wv, int lane)
+{
+    const bool j0 = lane < nel, j1 = lane + WAVE < nel;
+    double v0[MR], v1[MR];
+#pragma unroll
+    for (int r = 0; r < MR; r++) {
+        const int i = wv + 4 * r;
+        const bool ok = i < nel && i != row;
+        v0[r] = (ok && j0) ? S[(size_t)i * NM + lane] : 0.0;
+        v1[r] = (ok && j1) ? S[(size_t)i * NM + lane + WAVE] : 0.0;
+    }
+    const double o0 = j0 ? s_orb[lane] : 0.0, o1 = j1 ? s_orb[lane + WAVE] : 0.0;
+    const double w0 = j0 ? s_row[lane] : 0.0, w1 = j1 ? s_row[lane + WAVE] : 0.0;
+#pragma unroll
+    for (int r = 0; r < MR; r++) {
+        const int i = wv + 4 * r;
+        if (i >= nel) break;  // uniform in the wave
+        double sum = 0;
+        sum += v0[r] * o0;
+        sum += v1[r] * o1;
+        sum = wave_sum(sum);
+        sum = sum / rt;
+        if (i == row) continue;
+        if (j0) S[(size_t)i * NM + lane] = v0[r] - w0 * sum;
+        if (j1) S[(size_t)i * NM + lane + WAVE] = v1[r] - w1 * sum;
+    }
+}
+
 // ------------------------------------------------------------------------
 // Move kernel B, one 256-thread block per walker: determinante ratio
 // (determinante.f90:38-46), drift at the new position (iflag_move=0,
@@ -364,6 +397,9 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
 // acceptance jassav + detsav with the Sherman-Morrison update applied in
 // place (determinante.f90:47-62 + detsav.f90:42-93).
 //   mode 0: decide with the walker's stream; 1: compute only; 2: commit acc[w].
+//   MR: rows of the inverse per wave held in registers during the update
+//   (0: row-by-row loop for spin blocks above 2 WAVE electrons).
+template <int MR>
 __global__ void __launch_bounds__(256) k_move_b(MoveArgs a)
 {
     HIP_DYNAMIC_SHARED(double, lds)
@@ -439,16 +475,20 @@ __global__ void __launch_bounds__(256) k_move_b(MoveArgs a)
         s_orb[j] = ob[a.iworbd[ish + j]];
     }
     __syncthreads();
-    // Sherman-Morrison: rows i != iel
-    for (int i = wv; i < nel; i += nwv) {
-        if (i == row) continue;
-        double *Si = S + (size_t)i * a.NM;
-        double sum = 0;
-        for (int j = lane; j < nel; j += WAVE) sum += Si[j] * s_orb[j];
-        sum = wave_sum(sum);
-        sum = sum / rt;
-        for (int j = lane; j < nel; j += WAVE) Si[j] = Si[j] - s_row[j] * sum;
-    }
+    // Sherman-Morrison: rows i != iel (all rows of a wave loaded before any
+    // is reduced, so their loads are in flight together)
+    if constexpr (MR > 0)
+        sm_rows<MR>(S, a.NM, nel, row, rt, s_row, s_orb, wv, lane);
+    else
+        for (int i = wv; i < nel; i += nwv) {
+            if (i == row) continue;
+            double *Si = S + (size_t)i * a.NM;
+            double sum = 0;
+            for (int j = lane; j < nel; j += WAVE) sum += Si[j] * s_orb[j];
+            sum = wave_sum(sum);
+            sum = sum / rt;
+            for (int j = lane; j < nel; j += WAVE) Si[j] = Si[j] - s_row[j] * sum;
+        }
     for (int j = tid; j < nel; j += blockDim.x) S[(size_t)row * a.NM + j] = s_row[j] / rt;
     // detsav: orbital gradients of iel, spin determinant
     double *dor = a.dorb + ((size_t)w * NE + iel) * 3 * NO;
@@ -582,7 +622,17 @@ int launch_move_b(champ_ctx *c, int iel, int mode)
     size_t lds = (size_t)2 * c->NM * sizeof(double);
     int slot;
     tbegin(c, K_MOVE_B, &slot);
-    hipLaunchKernelGGL(k_move_b, dim3(c->W), dim3(256), lds, c->stream, a);
+    // rows per wave for the larger spin block (4 waves per block)
+    const int nmx = std::max(c->NUP, c->NDN), rpw = (nmx + 3) / 4;
+    const dim3 grid(c->W), block(256);
+    if (nmx > 2 * WAVE) hipLaunchKernelGGL(k_move_b<0>, grid, block, lds, c->stream, a);
+    else if (rpw <= 4) hipLaunchKernelGGL(k_move_b<4>, grid, block, lds, c->stream, a);
+    else if (rpw <= 8) hipLaunchKernelGGL(k_move_b<8>, grid, block, lds, c->stream, a);
+    else if (rpw <= 12) hipLaunchKernelGGL(k_move_b<12>, grid, block, lds, c->stream, a);
+    else if (rpw <= 16) hipLaunchKernelGGL(k_move_b<16>, grid, block, lds, c->stream, a);
+    else if (rpw <= 20) hipLaunchKernelGGL(k_move_b<20>, grid, block, lds, c->stream, a);
+    else if (rpw <= 24) hipLaunchKernelGGL(k_move_b<24>, grid, block, lds, c->stream, a);
+    else hipLaunchKernelGGL(k_move_b<32>, grid, block, lds, c->stream, a);
     tend(c, slot);
     HIPCHECK(hipGetLastError());
     return CHAMP_OK;
