@@ -56,13 +56,13 @@ constexpr int een_naz() { return EenTable<N>{}.naz > 0 ? EenTable<N>{}.naz : 1; 
 // Coefficient tables of one (electron, centre) at switched distance x:
 // tab[d * naz + idx] = d^d A / dx^d for d = 0..D.  c = parameters of the
 // centre's type in term order.
-template <int N, int D>
-__device__ __forceinline__ void een_tables(const double *c, double x, double *tab)
+template <int N, int D, typename T = double>
+__device__ __forceinline__ void een_tables(const double *c, T x, T *tab)
 {
-    constexpr EenTable<N> T{};
+    constexpr EenTable<N> ET{};
     constexpr int NA = een_naz<N>();
-    double xp[N + 1];
-    xp[0] = 1.0;
+    T xp[N + 1];
+    xp[0] = T(1);
 #pragma unroll
     for (int p = 1; p <= N; p++) xp[p] = p == 1 ? x : xp[1] * xp[p - 1];
     // one entry at a time: sum over the (term, half) pairs that land on it
@@ -70,16 +70,16 @@ __device__ __forceinline__ void een_tables(const double *c, double x, double *ta
     for (int d = 0; d <= D; d++)
 #pragma unroll
         for (int e = 0; e < NA; e++) {
-            double v = 0.0;
+            T v = T(0);
 #pragma unroll
-            for (int t = 0; t < T.n; t++)
+            for (int t = 0; t < ET.n; t++)
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
-                    const int n = h == 0 ? T.m[t] : T.l[t] + T.m[t];
-                    const int p = h == 0 ? T.l[t] + T.m[t] : T.m[t];
-                    if (T.idx[T.k[t]][n] != e || p < d) continue;
-                    const double f = d == 0 ? 1.0 : (d == 1 ? (double)p : (double)(p * (p - 1)));
-                    v += c[t] * f * xp[p - d];
+                    const int n = h == 0 ? ET.m[t] : ET.l[t] + ET.m[t];
+                    const int p = h == 0 ? ET.l[t] + ET.m[t] : ET.m[t];
+                    if (ET.idx[ET.k[t]][n] != e || p < d) continue;
+                    const T f = d == 0 ? T(1) : (d == 1 ? (T)p : (T)(p * (p - 1)));
+                    v += (T)c[t] * f * xp[p - d];
                 }
             tab[d * NA + e] = v;
         }
@@ -87,36 +87,37 @@ __device__ __forceinline__ void een_tables(const double *c, double x, double *ta
 
 // P and its derivatives at (u, y) from the tables of x:
 //   D = 0: f;  D = 1: f, fu, fx, fy;  D = 2: also fuu, fxx, fyy, fux, fuy.
+template <typename T = double>
 struct EenOut {
-    double f, fu, fx, fy, fuu, fxx, fyy, fux, fuy;
+    T f, fu, fx, fy, fuu, fxx, fyy, fux, fuy;
 };
 
-template <int N, int D>
-__device__ __forceinline__ EenOut een_eval_tab(const double *tab, double u, double y)
+template <int N, int D, typename T = double>
+__device__ __forceinline__ EenOut<T> een_eval_tab(const T *tab, T u, T y)
 {
-    constexpr EenTable<N> T{};
+    constexpr EenTable<N> ET{};
     constexpr int NA = een_naz<N>();
-    EenOut o{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    EenOut<T> o{0, 0, 0, 0, 0, 0, 0, 0, 0};
     // Horner in u over the row polynomials of y.  The derivative recurrences
     // (d2 = d2 y + d1, d1 = d1 y + p, p = p y + a) start from zero, so their
     // first updates are plain copies: the step counters are compile-time and
     // no multiply by a zero accumulator is emitted.  *2 names hold p''/2.
-    double P = 0, Pu = 0, Puu2 = 0, Py = 0, Puy = 0, Pyy2 = 0, Px = 0, Pux = 0, Pxx = 0;
+    T P = 0, Pu = 0, Puu2 = 0, Py = 0, Puy = 0, Pyy2 = 0, Px = 0, Pux = 0, Pxx = 0;
 #pragma unroll
     for (int kk = N; kk >= 0; kk--) {
-        if (kk > T.kmax) continue;
-        const int su = T.kmax - kk;  // Horner step in u
-        double q = 0, q1 = 0, q2 = 0, r = 0, s = 0;  // row k: value, d/dy, (d2/dy2)/2; dA/dx row; d2A/dx2 row
-        const int nm = T.nmax[kk];
+        if (kk > ET.kmax) continue;
+        const int su = ET.kmax - kk;  // Horner step in u
+        T q = 0, q1 = 0, q2 = 0, r = 0, s = 0;  // row k: value, d/dy, (d2/dy2)/2; dA/dx row; d2A/dx2 row
+        const int nm = ET.nmax[kk];
         if (nm >= 0) {
-            q = tab[T.idx[kk][nm]];
-            if (D >= 1) r = tab[NA + T.idx[kk][nm]];
-            if (D >= 2) s = tab[2 * NA + T.idx[kk][nm]];
+            q = tab[ET.idx[kk][nm]];
+            if (D >= 1) r = tab[NA + ET.idx[kk][nm]];
+            if (D >= 2) s = tab[2 * NA + ET.idx[kk][nm]];
 #pragma unroll
             for (int nn = N - 1; nn >= 0; nn--) {
                 if (nn >= nm) continue;
                 const int st = nm - 1 - nn;  // Horner step in y
-                const int e = T.idx[kk][nn];
+                const int e = ET.idx[kk][nn];
                 if (D >= 2 && st >= 1) q2 = st == 1 ? q1 : q2 * y + q1;
                 if (D >= 1) q1 = st == 0 ? q : q1 * y + q;
                 q = e >= 0 ? q * y + tab[e] : q * y;
@@ -140,7 +141,7 @@ __device__ __forceinline__ EenOut een_eval_tab(const double *tab, double u, doub
     }
     o.f = P;
     if (D >= 1) { o.fu = Pu; o.fx = Px; o.fy = Py; }
-    if (D >= 2) { o.fuu = 2.0 * Puu2; o.fyy = 2.0 * Pyy2; o.fxx = Pxx; o.fux = Pux; o.fuy = Puy; }
+    if (D >= 2) { o.fuu = T(2) * Puu2; o.fyy = T(2) * Pyy2; o.fxx = Pxx; o.fux = Pux; o.fuy = Puy; }
     return o;
 }
 

@@ -75,8 +75,8 @@ __global__ void __launch_bounds__(256) k_gemm_valu(int M, int N, int K, const do
 
 
 // FP64 MFMA GEMM.  The instruction computes D(16x16) += A(16x4) B(4x16) with
-// lane l holding A[l % 16][l / 16], B[l / 16][l % 16] and D[4 (l / 16) + r][l % 16]
-// (r = 0..3).  A K-chunk of 16 is fed as four instructions in which lane l's
+// lane l holding A[l % 16][l / 16], B[l / 16][l % 16] and D[4 r + l / 16][l % 16]
+// in accumulator element r = 0..3 (layout measured on gfx950).  A K-chunk of 16 is fed as four instructions in which lane l's
 // k-slot (l / 16) carries k = k0 + 4 (l / 16) + s at step s: the same mapping
 // for A and B, so the chunk's 16 products are all summed, and each lane reads
 // 4 consecutive k of its row (a contiguous 32 bytes of A, and of B^T).
@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int M, int N, int K, const do
         if (!cok[t]) continue;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int ro = rt * 16 + 4 * (lane / 16) + r;
+            const int ro = rt * 16 + 4 * r + lane / 16;
             if (ro < M) Cm[(long long)ro * ldc + col[t]] = acc[t][r];
         }
     }

@@ -23,7 +23,10 @@ namespace champ {
 #endif
 constexpr int HJB = CHAMP_HJB;
 
-template <int NORDC>
+// T: arithmetic type of the pair loop and its e-e-n tables (double on the
+// FP64 path, float on the mixed path); distances, e-n terms and all sums stay
+// double.
+template <int NORDC, typename T>
 __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi_jastrow(HArgs a)
 {
     HIP_DYNAMIC_SHARED(double, lds)
@@ -92,28 +95,29 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
     constexpr int NA = een_naz<NORDC>();
     constexpr int RS = 3 * NA + 8;  // doubles per centre: tables, r, rr, dd7, dd9, rvec (3), 1/r
     const int lane = tid % WAVE, wv = tid / WAVE, nwv = bd / WAVE;
-    double *rowt = lds + (size_t)wv * NC * RS;
+    T *rowt = (T *)lds + (size_t)wv * NC * RS;
+    const T tkap = (T)kap;
     for (int rr0 = wv; rr0 < NE - 1; rr0 += nwv) {
         const int i = NE - 1 - rr0;
         if constexpr (NORDC > 1) {
             __syncwarp();  // the previous row's table is no longer read
             for (int c = lane; c < NC; c += WAVE) {
-                double *rt = rowt + (size_t)c * RS;
+                T *rt = rowt + (size_t)c * RS;
                 const double ri = rw[(size_t)c * NE + i], rri = rrw[(size_t)c * NE + i];
-                const double sw = 1.0 - kap * rri;
-                een_tables<NORDC, 2>(jp.c[a.iwctype[c]], sw, rt);
-                rt[3 * NA] = ri;
-                rt[3 * NA + 1] = rri;
-                rt[3 * NA + 2] = -kap * sw;         // dd7
-                rt[3 * NA + 3] = -kap * (-kap * sw);  // dd9
-                for (int k = 0; k < 3; k++) rt[3 * NA + 4 + k] = xw[k * NE + i] - a.cent[3 * c + k];
-                rt[3 * NA + 7] = 1.0 / ri;
+                const T sw = T(1) - tkap * (T)rri;
+                een_tables<NORDC, 2, T>(jp.c[a.iwctype[c]], sw, rt);
+                rt[3 * NA] = (T)ri;
+                rt[3 * NA + 1] = (T)rri;
+                rt[3 * NA + 2] = -tkap * sw;           // dd7
+                rt[3 * NA + 3] = -tkap * (-tkap * sw);  // dd9
+                for (int k = 0; k < 3; k++) rt[3 * NA + 4 + k] = (T)(xw[k * NE + i] - a.cent[3 * c + k]);
+                rt[3 * NA + 7] = T(1) / (T)ri;
             }
             __syncwarp();
         }
         for (int j0 = 0; j0 < i; j0 += WAVE * HJB) {
-            double fs[HJB], d2[HJB], fI[HJB][3], fJ[HJB][3], rv[HJB][3], xj[HJB][3], sfu[HJB];
-            double uu1[HJB], dd1e[HJB], dd2e[HJB], irij[HJB], rij2[HJB];
+            T fs[HJB], d2[HJB], fI[HJB][3], fJ[HJB][3], rv[HJB][3], xj[HJB][3], sfu[HJB];
+            T uu1[HJB], dd1e[HJB], dd2e[HJB], irij[HJB], rij2[HJB];
             bool on[HJB];
 #pragma unroll
             for (int s = 0; s < HJB; s++) {
@@ -123,39 +127,40 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
                 fs[s] = 0; d2[s] = 0; sfu[s] = 0;
                 for (int k = 0; k < 3; k++) {
                     fI[s][k] = 0; fJ[s][k] = 0;
-                    xj[s][k] = xw[k * NE + jq];
-                    rv[s][k] = xw[k * NE + i] - xj[s][k];
+                    const double xjd = xw[k * NE + jq];
+                    xj[s][k] = (T)xjd;
+                    rv[s][k] = (T)(xw[k * NE + i] - xjd);
                 }
-                double r2 = 0;
+                T r2 = 0;
                 r2 = r2 + rv[s][0] * rv[s][0]; r2 = r2 + rv[s][1] * rv[s][1]; r2 = r2 + rv[s][2] * rv[s][2];
-                const double rij = sqrt(r2);
+                const T rij = sqrt(r2);
                 rij2[s] = rij * rij;
-                irij[s] = 1.0 / rij;
-                double sspinn = 1; int isb = 0, ipar = 0;
+                irij[s] = T(1) / rij;
+                T sspinn = 1; int isb = 0, ipar = 0;
                 if (i < a.NUP || jq >= a.NUP) {
                     if (jp.nspin2b == 2) isb = 1;
-                    else if (jp.nocuspb == 0) sspinn = 0.5;
+                    else if (jp.nocuspb == 0) sspinn = T(0.5);
                     ipar = 1;
                 }
-                const double u = scale_r(kap, rij);
-                const double dd1 = 1.0 - kap * u, dd2 = -kap * dd1;
+                const T u = scale_r<T>(tkap, rij);
+                const T dd1 = T(1) - tkap * u, dd2 = -tkap * dd1;
                 uu1[s] = dd1;
-                dd1e[s] = -kap * dd1;
-                dd2e[s] = -kap * dd2;
-                if (on[s] && u != asymp_r) {
+                dd1e[s] = -tkap * dd1;
+                dd2e[s] = -tkap * dd2;
+                if (on[s] && u != (T)asymp_r) {
                     const double *b = jp.b[isb];
-                    double top = sspinn * b[0] * u, topu = sspinn * b[0], topuu = 0;
-                    double bot = 1 + b[1] * u, botu = b[1], botuu = 0, bot2 = bot * bot;
-                    double fee = top / bot - jp.jb[ipar];
-                    double feeu = topu / bot - botu * top / bot2;
-                    double feeuu = topuu - (botuu * top + 2 * botu * topu) / bot + 2 * botu * botu * top / bot2;
+                    T top = sspinn * (T)b[0] * u, topu = sspinn * (T)b[0], topuu = 0;
+                    T bot = 1 + (T)b[1] * u, botu = (T)b[1], botuu = 0, bot2 = bot * bot;
+                    T fee = top / bot - (T)jp.jb[ipar];
+                    T feeu = topu / bot - botu * top / bot2;
+                    T feeuu = topuu - (botuu * top + 2 * botu * topu) / bot + 2 * botu * botu * top / bot2;
                     feeuu = feeuu / bot;
-                    double u1 = u, u2 = 1.0;  // uu(iord-1), uu(iord-2)
+                    T u1 = u, u2 = 1;  // uu(iord-1), uu(iord-2)
                     for (int iord = 2; iord <= jp.nordb; iord++) {
-                        const double un = u * u1;
-                        fee = fee + b[iord] * un;
-                        feeu = feeu + b[iord] * iord * u1;
-                        feeuu = feeuu + b[iord] * iord * (iord - 1) * u2;
+                        const T un = u * u1;
+                        fee = fee + (T)b[iord] * un;
+                        feeu = feeu + (T)b[iord] * (T)iord * u1;
+                        feeuu = feeuu + (T)b[iord] * (T)(iord * (iord - 1)) * u2;
                         u2 = u1; u1 = un;
                     }
                     feeuu = feeuu * dd1 * dd1 + feeu * dd2;
@@ -178,10 +183,10 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
                     rrjn[s] = rrw[jcl[s]];
                 }
                 for (int c = 0; c < NC; c++) {
-                    const double *rt = rowt + (size_t)c * RS;
-                    const double ri = rt[3 * NA], rri = rt[3 * NA + 1], dd7 = rt[3 * NA + 2], dd9 = rt[3 * NA + 3];
-                    const double rvi0 = rt[3 * NA + 4], rvi1 = rt[3 * NA + 5], rvi2 = rt[3 * NA + 6], iri = rt[3 * NA + 7];
-                    const double cx = a.cent[3 * c], cy = a.cent[3 * c + 1], cz = a.cent[3 * c + 2];
+                    const T *rt = rowt + (size_t)c * RS;
+                    const T ri = rt[3 * NA], rri = rt[3 * NA + 1], dd7 = rt[3 * NA + 2], dd9 = rt[3 * NA + 3];
+                    const T rvi0 = rt[3 * NA + 4], rvi1 = rt[3 * NA + 5], rvi2 = rt[3 * NA + 6], iri = rt[3 * NA + 7];
+                    const T cx = (T)a.cent[3 * c], cy = (T)a.cent[3 * c + 1], cz = (T)a.cent[3 * c + 2];
                     double rjc[HJB], rrjc[HJB];
 #pragma unroll
                     for (int s = 0; s < HJB; s++) {
@@ -195,22 +200,22 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
                     for (int s = 0; s < HJB; s++) {
                         if (j0 + s * WAVE >= i) break;
                         if (!on[s]) continue;
-                        const double rj = rjc[s], rrj = rrjc[s];
-                        if (rri == asymp_r && rrj == asymp_r) continue;
-                        const double swj = 1.0 - kap * rrj;
-                        const double dd8 = -kap * swj, dd10 = -kap * (-kap * swj);
-                        const EenOut e = een_eval_tab<NORDC, 2>(rt, uu1[s], swj);
-                        const double d1 = dd1e[s];
-                        const double fuu = e.fuu * d1 * d1 + e.fu * dd2e[s];
-                        const double fu = e.fu * d1 * irij[s];
-                        const double fui = e.fux * d1 * dd7, fuj = e.fuy * d1 * dd8;
-                        const double fii = e.fxx * dd7 * dd7 + e.fx * dd9, fjj = e.fyy * dd8 * dd8 + e.fy * dd10;
-                        const double irj = 1.0 / rj;
-                        const double fi = e.fx * dd7 * iri, fj = e.fy * dd8 * irj;
+                        const T rj = (T)rjc[s], rrj = (T)rrjc[s];
+                        if (rri == (T)asymp_r && rrj == (T)asymp_r) continue;
+                        const T swj = T(1) - tkap * rrj;
+                        const T dd8 = -tkap * swj, dd10 = -tkap * (-tkap * swj);
+                        const EenOut<T> e = een_eval_tab<NORDC, 2, T>(rt, uu1[s], swj);
+                        const T d1 = dd1e[s];
+                        const T fuu = e.fuu * d1 * d1 + e.fu * dd2e[s];
+                        const T fu = e.fu * d1 * irij[s];
+                        const T fui = e.fux * d1 * dd7, fuj = e.fuy * d1 * dd8;
+                        const T fii = e.fxx * dd7 * dd7 + e.fx * dd9, fjj = e.fyy * dd8 * dd8 + e.fy * dd10;
+                        const T irj = T(1) / rj;
+                        const T fi = e.fx * dd7 * iri, fj = e.fy * dd8 * irj;
                         fs[s] += e.f;
                         sfu[s] += fu;
-                        const double st = (ri + rj) * (ri - rj);
-                        const double u2pst = rij2[s] + st, u2mst = rij2[s] - st;
+                        const T st = (ri + rj) * (ri - rj);
+                        const T u2pst = rij2[s] + st, u2mst = rij2[s] - st;
                         fI[s][0] += fi * rvi0; fI[s][1] += fi * rvi1; fI[s][2] += fi * rvi2;
                         fJ[s][0] += fj * (xj[s][0] - cx); fJ[s][1] += fj * (xj[s][1] - cy); fJ[s][2] += fj * (xj[s][2] - cz);
                         d2[s] = d2[s] + 2 * (fuu + 2 * fu) + fui * u2pst * (iri * irij[s]) + fuj * u2mst * (irj * irij[s]) + fii +
@@ -223,15 +228,17 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
                 const int j = j0 + s * WAVE + lane;
                 if (!on[s]) continue;
                 for (int k = 0; k < 3; k++) { fI[s][k] += sfu[s] * rv[s][k]; fJ[s][k] -= sfu[s] * rv[s][k]; }
-                F[(size_t)i * NE + j] = fs[s];
-                F[(size_t)j * NE + i] = fs[s];
+                const double fsd = (double)fs[s];
+                F[(size_t)i * NE + j] = fsd;
+                F[(size_t)j * NE + i] = fsd;
                 for (int k = 0; k < 3; k++) {
-                    G1[((size_t)i * 3 + k) * NE + j] = fI[s][k];
-                    G2[((size_t)i * 3 + k) * NE + j] = fJ[s][k];
-                    G1[((size_t)j * 3 + k) * NE + i] = fJ[s][k];
-                    G2[((size_t)j * 3 + k) * NE + i] = fI[s][k];
+                    const double gI = (double)fI[s][k], gJ = (double)fJ[s][k];
+                    G1[((size_t)i * 3 + k) * NE + j] = gI;
+                    G2[((size_t)i * 3 + k) * NE + j] = gJ;
+                    G1[((size_t)j * 3 + k) * NE + i] = gJ;
+                    G2[((size_t)j * 3 + k) * NE + i] = gI;
                 }
-                fsp += fs[s]; d2p += d2[s];
+                fsp += fsd; d2p += (double)d2[s];
             }
         }
     }
@@ -253,6 +260,9 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
 // AO values, gradients and Laplacians of every electron (basis_fns with
 // ider=2 inside orbitals, orbitals_no_qmckl.f90:46): one wavefront per
 // (walker, electron), lanes over AOs.  Chunk of walkers [w0, w0+nw).
+// T: arithmetic type of the radial functions and AO values (double on the
+// FP64 path, float on the mixed path); the AO block is stored as double.
+template <typename T>
 __global__ void __launch_bounds__(256) k_hpsi_ao(HArgs a)
 {
     HIP_DYNAMIC_SHARED(double, lds)
@@ -266,20 +276,20 @@ __global__ void __launch_bounds__(256) k_hpsi_ao(HArgs a)
     double *out = a.aoall + (size_t)gw * 5 * a.NB;
     const double x0 = xw[e], x1 = xw[NE + e], x2 = xw[2 * NE + e];
     // radial functions (R, R', R'') once per (centre, shell), in the wave's LDS
-    double *R = lds + (size_t)wv * 3 * NRI;
+    T *R = (T *)lds + (size_t)wv * 3 * NRI;
     for (int ri = lane; ri < NRI; ri += WAVE) {
         const double r = rw[(size_t)a.ri_cent[ri] * NE + e];
-        double wfv[3] = {0.0, 0.0, 0.0};
-        if (r <= a.ri_rmax[ri]) splfit_jx<double>(a.tab, a.ri_type[ri], a.ri_rad[ri], r, grid_jx(a.tab, r), wfv, 2);
+        T wfv[3] = {T(0), T(0), T(0)};
+        if (r <= a.ri_rmax[ri]) splfit_jx<T>(a.tab, a.ri_type[ri], a.ri_rad[ri], (T)r, grid_jx(a.tab, r), wfv, 2);
         R[3 * ri] = wfv[0]; R[3 * ri + 1] = wfv[1]; R[3 * ri + 2] = wfv[2];
     }
     __syncwarp();
     for (int m = lane; m < a.NB; m += WAVE) {
         const int c = a.ao_cent[m];
-        double o[5];
-        ao_combine<double>(a.ao_slm[m], x0 - a.cent[3 * c], x1 - a.cent[3 * c + 1], x2 - a.cent[3 * c + 2],
-                           rw[(size_t)c * NE + e], R + 3 * a.ao_ri[m], 2, o);
-        for (int k = 0; k < 5; k++) out[(size_t)k * a.NB + m] = o[k];
+        T o[5];
+        ao_combine<T>(a.ao_slm[m], (T)(x0 - a.cent[3 * c]), (T)(x1 - a.cent[3 * c + 1]), (T)(x2 - a.cent[3 * c + 2]),
+                      (T)rw[(size_t)c * NE + e], R + 3 * a.ao_ri[m], 2, o);
+        for (int k = 0; k < 5; k++) out[(size_t)k * a.NB + m] = (double)o[k];
     }
 }
 
@@ -557,6 +567,29 @@ __global__ void __launch_bounds__(256) k_hpsi_eloc(HArgs a)
     }
 }
 
+template <typename T>
+static int launch_hpsi_jastrow_t(champ_ctx *c, const HArgs &a, int nordc, int naz)
+{
+    const size_t lds = nordc > 1 ? (size_t)4 * c->NC * (3 * naz + 8) * sizeof(T) : 8;
+    const void *fn = nordc == 5 ? (const void *)k_hpsi_jastrow<5, T> : nordc == 4 ? (const void *)k_hpsi_jastrow<4, T>
+                   : nordc == 3 ? (const void *)k_hpsi_jastrow<3, T> : nordc == 2 ? (const void *)k_hpsi_jastrow<2, T>
+                                                                            : (const void *)k_hpsi_jastrow<0, T>;
+    if (lds > 160 * 1024) {
+        set_error("hpsi: the Jastrow kernel's LDS tables exceed 160 KB for this system");
+        return CHAMP_ERR_UNSUPPORTED;
+    }
+    if (lds > 64 * 1024) HIPCHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    switch (nordc) {
+    case 5: hipLaunchKernelGGL((k_hpsi_jastrow<5, T>), dim3(c->W), dim3(256), lds, c->stream, a); break;
+    case 4: hipLaunchKernelGGL((k_hpsi_jastrow<4, T>), dim3(c->W), dim3(256), lds, c->stream, a); break;
+    case 3: hipLaunchKernelGGL((k_hpsi_jastrow<3, T>), dim3(c->W), dim3(256), lds, c->stream, a); break;
+    case 2: hipLaunchKernelGGL((k_hpsi_jastrow<2, T>), dim3(c->W), dim3(256), lds, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_hpsi_jastrow<0, T>), dim3(c->W), dim3(256), lds, c->stream, a); break;
+    }
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
 // ------------------------------------------------------------------------
 int launch_hpsi_impl(champ_ctx *c, int accumulate)
 {
@@ -571,18 +604,9 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
             set_error("hpsi: e-e-n order above 5 is not supported");
             return CHAMP_ERR_UNSUPPORTED;
         }
-        const size_t lds = nordc > 1 ? (size_t)4 * c->NC * (3 * naz[nordc] + 8) * sizeof(double) : 8;
-        const void *fn = nordc == 5 ? (const void *)k_hpsi_jastrow<5> : nordc == 4 ? (const void *)k_hpsi_jastrow<4>
-                       : nordc == 3 ? (const void *)k_hpsi_jastrow<3> : nordc == 2 ? (const void *)k_hpsi_jastrow<2>
-                                                                                : (const void *)k_hpsi_jastrow<0>;
-        if (lds > 64 * 1024) HIPCHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        switch (nordc) {
-        case 5: hipLaunchKernelGGL(k_hpsi_jastrow<5>, dim3(c->W), dim3(256), lds, c->stream, a); break;
-        case 4: hipLaunchKernelGGL(k_hpsi_jastrow<4>, dim3(c->W), dim3(256), lds, c->stream, a); break;
-        case 3: hipLaunchKernelGGL(k_hpsi_jastrow<3>, dim3(c->W), dim3(256), lds, c->stream, a); break;
-        case 2: hipLaunchKernelGGL(k_hpsi_jastrow<2>, dim3(c->W), dim3(256), lds, c->stream, a); break;
-        default: hipLaunchKernelGGL(k_hpsi_jastrow<0>, dim3(c->W), dim3(256), lds, c->stream, a); break;
-        }
+        const int rcj = c->prec == CHAMP_PREC_MIXED ? launch_hpsi_jastrow_t<float>(c, a, nordc, naz[nordc])
+                                                     : launch_hpsi_jastrow_t<double>(c, a, nordc, naz[nordc]);
+        if (rcj) return rcj;
     }
     tend(c, slot);
     HIPCHECK(hipGetLastError());
@@ -591,8 +615,12 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
         a.nw = std::min(c->hpsi_chunk, c->W - w0);
         const long long nwave = (long long)a.nw * c->NE;
         tbegin(c, K_HPSI_AO, &slot);
-        hipLaunchKernelGGL(k_hpsi_ao, dim3((unsigned)((nwave + 3) / 4)), dim3(256), (size_t)4 * 3 * c->NRI * sizeof(double),
-                           c->stream, a);
+        if (c->prec == CHAMP_PREC_MIXED)
+            hipLaunchKernelGGL(k_hpsi_ao<float>, dim3((unsigned)((nwave + 3) / 4)), dim3(256),
+                               (size_t)4 * 3 * c->NRI * sizeof(float), c->stream, a);
+        else
+            hipLaunchKernelGGL(k_hpsi_ao<double>, dim3((unsigned)((nwave + 3) / 4)), dim3(256),
+                               (size_t)4 * 3 * c->NRI * sizeof(double), c->stream, a);
         tend(c, slot);
         HIPCHECK(hipGetLastError());
         tbegin(c, K_HPSI_GEMM, &slot);

@@ -101,23 +101,26 @@ __global__ void __launch_bounds__(256) k_drift_old(MoveArgs a)
 // Tuning of k_move_a: partner-electron slots per lane in the pair loop and
 // the minimum waves per SIMD asked of the register allocator.
 #ifndef CHAMP_MJB
-#define CHAMP_MJB 2
+#define CHAMP_MJB 1
 #endif
 #ifndef CHAMP_MOVE_WPE
 #define CHAMP_MOVE_WPE 1
 #endif
 constexpr int MJB = CHAMP_MJB;
 
-// LDS doubles per wave of k_move_a: r, rr, -kap (1 - kap rr) / r, rvec (3) per
-// centre, the centre's e-e-n tables (value and d/dx), then R and R' of each
+// LDS of one wave of k_move_a, in doubles: r, rr, -kap (1 - kap rr) / r,
+// rvec (3) per centre (double), then in the arithmetic type (tsize bytes) the
+// centre's e-e-n tables (value and d/dx, na entries each) and R, R' of each
 // radial instance at the moved electron
-static size_t move_a_lds_per_wave(int NC, int nordc, int NRI)
+__host__ __device__ __forceinline__ int move_a_lds_doubles(int NC, int na, int NRI, int tsize)
 {
-    const int naz[] = {0, 0, een_naz<2>(), een_naz<3>(), een_naz<4>(), een_naz<5>()};
-    return (size_t)6 * NC + (nordc > 1 && nordc <= 5 ? (size_t)2 * naz[nordc] * NC : 0) + (size_t)2 * NRI;
+    return 6 * NC + ((2 * na * NC + 2 * NRI) * tsize + 7) / 8;
 }
 
-template <int NORDC>
+// T: arithmetic type of the pair loop, the e-e-n tables and the AO values
+// (double on the FP64 path, float on the mixed path); distances of the moved
+// electron, the state and every sum across lanes stay double.
+template <int NORDC, typename T>
 __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move_a(MoveArgs a)
 {
     HIP_DYNAMIC_SHARED(double, lds)
@@ -128,9 +131,10 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
     const int NE = a.NE, NC = a.NC, iel = a.iel;
     const JasParams &jp = *a.jas;
     const double kap = jp.kap, asymp_r = jp.asymp_r;
-    double *Lr = lds + (size_t)wv * (6 * NC + (NORDC > 1 ? 2 * NA * NC : 0) + 2 * a.NRI);
-    double *Lrr = Lr + NC, *Lfm = Lr + 2 * NC, *Lrv = Lr + 3 * NC, *Ltab = Lr + 6 * NC;
-    double *Lrad = Ltab + (NORDC > 1 ? 2 * NA * NC : 0);
+    double *Lr = lds + (size_t)wv * move_a_lds_doubles(NC, NORDC > 1 ? NA : 0, a.NRI, (int)sizeof(T));
+    double *Lrr = Lr + NC, *Lfm = Lr + 2 * NC, *Lrv = Lr + 3 * NC;
+    T *Ltab = (T *)(Lr + 6 * NC);
+    T *Lrad = Ltab + (NORDC > 1 ? 2 * NA * NC : 0);
 
     double xn0 = 0, xn1 = 0, xn2 = 0;
     if (active) {
@@ -178,7 +182,7 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
             Lrv[3 * c] = rv0; Lrv[3 * c + 1] = rv1; Lrv[3 * c + 2] = rv2;
             a.rnew[(size_t)w * NC + c] = r;
             a.rrnew[(size_t)w * NC + c] = rr;
-            if constexpr (NORDC > 1) een_tables<NORDC, 1>(jp.c[a.iwctype[c]], 1.0 - kap * rr, Ltab + (size_t)c * 2 * NA);
+            if constexpr (NORDC > 1) een_tables<NORDC, 1, T>(jp.c[a.iwctype[c]], (T)(1.0 - kap * rr), Ltab + (size_t)c * 2 * NA);
             if (rr != asymp_r) {
                 const int it = a.iwctype[c];
                 const double *a4 = jp.a4[it];
@@ -223,8 +227,9 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
         // Partners in slots of WAVE electrons, MJB slots per lane; centres in
         // the outer loop so each centre's table is read once per slot group.
         // Per pair: gm/go = gradient on the moved / the partner electron.
+        const T tkap = (T)kap, tasymp = (T)asymp_r;
         for (int j0 = 0; j0 < NE; j0 += WAVE * MJB) {
-            double fs[MJB], gm[MJB][3], go[MJB][3], xj[MJB][3], uu1[MJB], fuf[MJB], sfu[MJB];
+            T fs[MJB], gm[MJB][3], go[MJB][3], xj[MJB][3], uu1[MJB], fuf[MJB], sfu[MJB];
             bool on[MJB];
 #pragma unroll
             for (int s = 0; s < MJB; s++) {
@@ -233,34 +238,35 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                 const int jq = j < NE ? j : 0;
                 fs[s] = 0; sfu[s] = 0;
                 for (int k = 0; k < 3; k++) { gm[s][k] = 0; go[s][k] = 0; }
-                xj[s][0] = xw[jq]; xj[s][1] = xw[NE + jq]; xj[s][2] = xw[2 * NE + jq];
-                const double rmo[3] = {xn0 - xj[s][0], xn1 - xj[s][1], xn2 - xj[s][2]};
-                double r2 = 0;
+                const double xd0 = xw[jq], xd1 = xw[NE + jq], xd2 = xw[2 * NE + jq];
+                xj[s][0] = (T)xd0; xj[s][1] = (T)xd1; xj[s][2] = (T)xd2;
+                const T rmo[3] = {(T)(xn0 - xd0), (T)(xn1 - xd1), (T)(xn2 - xd2)};
+                T r2 = 0;
                 r2 = r2 + rmo[0] * rmo[0]; r2 = r2 + rmo[1] * rmo[1]; r2 = r2 + rmo[2] * rmo[2];
-                const double rij = sqrt(r2);
-                const double u = scale_r(kap, rij);
-                const double dd1 = 1.0 - kap * u;
+                const T rij = sqrt(r2);
+                const T u = scale_r<T>(tkap, rij);
+                const T dd1 = T(1) - tkap * u;
                 uu1[s] = dd1;
-                fuf[s] = -kap * dd1 / rij;
+                fuf[s] = -tkap * dd1 / rij;
                 // e-e term (jastrow4e.f90: psi_ee with its u-derivative)
                 const int I = j > iel ? j : iel, J = j > iel ? iel : j;
-                double sspinn = 1; int isb = 0, ipar = 0;
+                T sspinn = 1; int isb = 0, ipar = 0;
                 if (I < a.NUP || J >= a.NUP) {
                     if (jp.nspin2b == 2) isb = 1;
-                    else if (jp.nocuspb == 0) sspinn = 0.5;
+                    else if (jp.nocuspb == 0) sspinn = T(0.5);
                     ipar = 1;
                 }
-                if (on[s] && u != asymp_r) {
+                if (on[s] && u != tasymp) {
                     const double *b = jp.b[isb];
-                    double top = sspinn * b[0] * u, topu = sspinn * b[0];
-                    double bot = 1 + b[1] * u, botu = b[1], bot2 = bot * bot;
-                    double fee = top / bot - jp.jb[ipar];
-                    double feeu = topu / bot - botu * top / bot2;
-                    double up = u;
+                    T top = sspinn * (T)b[0] * u, topu = sspinn * (T)b[0];
+                    T bot = 1 + (T)b[1] * u, botu = (T)b[1], bot2 = bot * bot;
+                    T fee = top / bot - (T)jp.jb[ipar];
+                    T feeu = topu / bot - botu * top / bot2;
+                    T up = u;
                     for (int iord = 2; iord <= jp.nordb; iord++) {
-                        double un = u * up;
-                        fee = fee + b[iord] * un;
-                        feeu = feeu + b[iord] * iord * up;
+                        T un = u * up;
+                        fee = fee + (T)b[iord] * un;
+                        feeu = feeu + (T)b[iord] * (T)iord * up;
                         up = un;
                     }
                     feeu = feeu * dd1 / rij;
@@ -281,10 +287,11 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                     rrjn[s] = rrw[jcl[s]];
                 }
                 for (int c = 0; c < NC; c++) {
-                    const double *tab = Ltab + (size_t)c * 2 * NA;
-                    const double rrm = Lrr[c], fmf = Lfm[c];
-                    const double rvm0 = Lrv[3 * c], rvm1 = Lrv[3 * c + 1], rvm2 = Lrv[3 * c + 2];
-                    const double cx = a.cent[3 * c], cy = a.cent[3 * c + 1], cz = a.cent[3 * c + 2];
+                    const T *tab = Ltab + (size_t)c * 2 * NA;
+                    const double rrm = Lrr[c];
+                    const T fmf = (T)Lfm[c];
+                    const T rvm0 = (T)Lrv[3 * c], rvm1 = (T)Lrv[3 * c + 1], rvm2 = (T)Lrv[3 * c + 2];
+                    const T cx = (T)a.cent[3 * c], cy = (T)a.cent[3 * c + 1], cz = (T)a.cent[3 * c + 2];
                     double rjc[MJB], rrjc[MJB];
 #pragma unroll
                     for (int s = 0; s < MJB; s++) {
@@ -298,14 +305,14 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                     for (int s = 0; s < MJB; s++) {
                         if (j0 + s * WAVE >= NE) break;
                         if (!on[s]) continue;
-                        const double rjo = rjc[s], rrjo = rrjc[s];
-                        if (rrm == asymp_r && rrjo == asymp_r) continue;
-                        const double swj = 1.0 - kap * rrjo;
-                        const EenOut e = een_eval_tab<NORDC, 1>(tab, uu1[s], swj);
+                        if (rrm == asymp_r && rrjc[s] == asymp_r) continue;
+                        const T rjo = (T)rjc[s];
+                        const T swj = T(1) - tkap * (T)rrjc[s];
+                        const EenOut<T> e = een_eval_tab<NORDC, 1, T>(tab, uu1[s], swj);
                         fs[s] += e.f;
                         sfu[s] += e.fu;
-                        const double fmv = e.fx * fmf;
-                        const double fot = e.fy * (-kap * swj) / rjo;
+                        const T fmv = e.fx * fmf;
+                        const T fot = e.fy * (-tkap * swj) / rjo;
                         gm[s][0] += fmv * rvm0; gm[s][1] += fmv * rvm1; gm[s][2] += fmv * rvm2;
                         go[s][0] += fot * (xj[s][0] - cx); go[s][1] += fot * (xj[s][1] - cy); go[s][2] += fot * (xj[s][2] - cz);
                     }
@@ -315,12 +322,12 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
             for (int s = 0; s < MJB; s++) {
                 const int j = j0 + s * WAVE + lane;
                 if (!on[s]) continue;
-                const double fu = sfu[s] * fuf[s];
-                const double rmo[3] = {xn0 - xj[s][0], xn1 - xj[s][1], xn2 - xj[s][2]};
+                const T fu = sfu[s] * fuf[s];
+                const T rmo[3] = {(T)(xn0 - (double)xj[s][0]), (T)(xn1 - (double)xj[s][1]), (T)(xn2 - (double)xj[s][2])};
                 double gi[3], gj[3];
-                for (int k = 0; k < 3; k++) { gi[k] = gm[s][k] + fu * rmo[k]; gj[k] = go[s][k] - fu * rmo[k]; }
-                fsn[j] = fs[s];
-                dfs += fs[s] - fso_row[j];
+                for (int k = 0; k < 3; k++) { gi[k] = (double)(gm[s][k] + fu * rmo[k]); gj[k] = (double)(go[s][k] - fu * rmo[k]); }
+                fsn[j] = (double)fs[s];
+                dfs += (double)fs[s] - fso_row[j];
                 for (int k = 0; k < 3; k++) {
                     g1n[k * NE + j] = gi[k];
                     g2n[k * NE + j] = gj[k];
@@ -341,17 +348,17 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
         // radial functions (R, R') once per (centre, shell), then each AO
         for (int ri = lane; ri < a.NRI; ri += WAVE) {
             const double r = Lr[a.ri_cent[ri]];
-            double wfv[3] = {0.0, 0.0, 0.0};
-            if (r <= a.ri_rmax[ri]) splfit_jx<double>(a.tab, a.ri_type[ri], a.ri_rad[ri], r, grid_jx(a.tab, r), wfv, 1);
+            T wfv[3] = {T(0), T(0), T(0)};
+            if (r <= a.ri_rmax[ri]) splfit_jx<T>(a.tab, a.ri_type[ri], a.ri_rad[ri], (T)r, grid_jx(a.tab, r), wfv, 1);
             Lrad[2 * ri] = wfv[0]; Lrad[2 * ri + 1] = wfv[1];
         }
         __syncwarp();
         double *ao = a.aomove + (size_t)w * 4 * a.NB;
         for (int m = lane; m < a.NB; m += WAVE) {
             const int c = a.ao_cent[m], ri = a.ao_ri[m];
-            const double wfv[3] = {Lrad[2 * ri], Lrad[2 * ri + 1], 0.0};
-            double out[5];
-            ao_combine<double>(a.ao_slm[m], Lrv[3 * c], Lrv[3 * c + 1], Lrv[3 * c + 2], Lr[c], wfv, 1, out);
+            const T wfv[3] = {Lrad[2 * ri], Lrad[2 * ri + 1], T(0)};
+            T out[5];
+            ao_combine<T>(a.ao_slm[m], (T)Lrv[3 * c], (T)Lrv[3 * c + 1], (T)Lrv[3 * c + 2], (T)Lr[c], wfv, 1, out);
             ao[m] = out[0]; ao[a.NB + m] = out[1]; ao[2 * a.NB + m] = out[2]; ao[3 * a.NB + m] = out[3];
         }
     }
@@ -578,32 +585,45 @@ int launch_drift_old(champ_ctx *c, int iel)
     return CHAMP_OK;
 }
 
+template <typename T>
+static int launch_move_a_t(champ_ctx *c, const MoveArgs &a, int nordc)
+{
+    const int naz[] = {0, 0, een_naz<2>(), een_naz<3>(), een_naz<4>(), een_naz<5>()};
+    const size_t lds = (size_t)4 * move_a_lds_doubles(c->NC, nordc > 1 ? naz[nordc] : 0, c->NRI, (int)sizeof(T)) *
+                       sizeof(double);
+    const void *fn = nordc == 5 ? (const void *)k_move_a<5, T> : nordc == 4 ? (const void *)k_move_a<4, T>
+                   : nordc == 3 ? (const void *)k_move_a<3, T> : nordc == 2 ? (const void *)k_move_a<2, T>
+                                                                        : (const void *)k_move_a<0, T>;
+    if (lds > 160 * 1024) {
+        set_error("move: the move kernel's LDS tables exceed 160 KB for this system");
+        return CHAMP_ERR_UNSUPPORTED;
+    }
+    if (lds > 64 * 1024) HIPCHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const dim3 grid(nblocks(c->W, 4)), block(256);
+    switch (nordc) {
+    case 5: hipLaunchKernelGGL((k_move_a<5, T>), grid, block, lds, c->stream, a); break;
+    case 4: hipLaunchKernelGGL((k_move_a<4, T>), grid, block, lds, c->stream, a); break;
+    case 3: hipLaunchKernelGGL((k_move_a<3, T>), grid, block, lds, c->stream, a); break;
+    case 2: hipLaunchKernelGGL((k_move_a<2, T>), grid, block, lds, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_move_a<0, T>), grid, block, lds, c->stream, a); break;
+    }
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
 int launch_move_a(champ_ctx *c, int iel, int mode)
 {
     MoveArgs a = move_args(c, iel, mode, g_tau);
     const int nordc = c->hjas.nordc;
-    const size_t lds = (size_t)4 * move_a_lds_per_wave(c->NC, nordc, c->NRI) * sizeof(double);
     if (nordc > 5) {
         set_error("move: e-e-n order above 5 is not supported");
         return CHAMP_ERR_UNSUPPORTED;
     }
-    const void *fn = nordc == 5 ? (const void *)k_move_a<5> : nordc == 4 ? (const void *)k_move_a<4>
-                   : nordc == 3 ? (const void *)k_move_a<3> : nordc == 2 ? (const void *)k_move_a<2>
-                                                                        : (const void *)k_move_a<0>;
-    if (lds > 64 * 1024) HIPCHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int slot;
     tbegin(c, K_MOVE_A, &slot);
-    const dim3 grid(nblocks(c->W, 4)), block(256);
-    switch (nordc) {
-    case 5: hipLaunchKernelGGL(k_move_a<5>, grid, block, lds, c->stream, a); break;
-    case 4: hipLaunchKernelGGL(k_move_a<4>, grid, block, lds, c->stream, a); break;
-    case 3: hipLaunchKernelGGL(k_move_a<3>, grid, block, lds, c->stream, a); break;
-    case 2: hipLaunchKernelGGL(k_move_a<2>, grid, block, lds, c->stream, a); break;
-    default: hipLaunchKernelGGL(k_move_a<0>, grid, block, lds, c->stream, a); break;
-    }
+    const int rc = c->prec == CHAMP_PREC_MIXED ? launch_move_a_t<float>(c, a, nordc) : launch_move_a_t<double>(c, a, nordc);
     tend(c, slot);
-    HIPCHECK(hipGetLastError());
-    return CHAMP_OK;
+    return rc;
 }
 
 int launch_move_gemm(champ_ctx *c)

@@ -26,32 +26,34 @@
 namespace champ {
 
 // psianl (nonlpsi.f90:88-105): e-n term at scaled distance rr
-__device__ __forceinline__ double en_term(const JasParams &jp, int it, double rr)
+template <typename T>
+__device__ __forceinline__ T en_term(const JasParams &jp, int it, T rr)
 {
-    if (rr == jp.asymp_r) return 0.0;
-    double psi = jp.a4[it][0] * rr / (1.0 + jp.a4[it][1] * rr) - jp.ja[it];
-    double rrip = rr;
-    for (int o = 2; o <= jp.norda; o++) { rrip = rrip * rr; psi = psi + jp.a4[it][o] * rrip; }
+    if (rr == (T)jp.asymp_r) return T(0);
+    T psi = (T)jp.a4[it][0] * rr / (T(1) + (T)jp.a4[it][1] * rr) - (T)jp.ja[it];
+    T rrip = rr;
+    for (int o = 2; o <= jp.norda; o++) { rrip = rrip * rr; psi = psi + (T)jp.a4[it][o] * rrip; }
     return psi;
 }
 
 // psibnl (nonlpsi.f90:107-133): e-e term of electrons (i, j) at scaled u
-__device__ __forceinline__ double ee_term(const JasParams &jp, int nup, int i, int j, double u)
+template <typename T>
+__device__ __forceinline__ T ee_term(const JasParams &jp, int nup, int i, int j, T u)
 {
-    if (u == jp.asymp_r) return 0.0;
+    if (u == (T)jp.asymp_r) return T(0);
     const int I = j > i ? j : i, J = j > i ? i : j;
-    double sspinn = 1.0;
+    T sspinn = T(1);
     int isb = 0, ipar = 0;
     if (I < nup || J >= nup) {
         if (jp.nspin2b == 2) isb = 1;
-        else if (jp.nocuspb == 0) sspinn = 0.5;
+        else if (jp.nocuspb == 0) sspinn = T(0.5);
         ipar = 1;
     }
     const double *b = jp.b[isb];
-    const double fee = b[0] * u / (1 + b[1] * u);
-    double psi = sspinn * fee - jp.jb[ipar];
-    double up = u;
-    for (int o = 2; o <= jp.nordb; o++) { up = u * up; psi = psi + b[o] * up; }
+    const T fee = (T)b[0] * u / (T(1) + (T)b[1] * u);
+    T psi = sspinn * fee - (T)jp.jb[ipar];
+    T up = u;
+    for (int o = 2; o <= jp.nordb; o++) { up = u * up; psi = psi + (T)b[o] * up; }
     return psi;
 }
 
@@ -76,21 +78,23 @@ __device__ __forceinline__ double strided_sum(double v)
 constexpr int NL_THREADS = 256;
 constexpr int NL_WAVES = NL_THREADS / WAVE;
 
-template <int NORDC, int QB>
+// T: arithmetic type of the distances, Jastrow terms and AO values (double
+// on the FP64 path, float on the mixed path).  Sums across lanes, the per-
+// point assembly and every table in memory stay double.
+template <int NORDC, int QB, typename T>
 __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k_nonloc(HArgs a)
 {
     HIP_DYNAMIC_SHARED(double, lds)
     const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
-    const int NE = a.NE, NC = a.NC, NQ = a.NQ, NB = a.NB;
+    const int NE = a.NE, NC = a.NC, NQ = a.NQ, NB = a.NB, NRI = a.NRI;
     const JasParams &jp = *a.jas;
-    const double kap = jp.kap, asymp = jp.asymp_r;
-    double *U = lds;                        // [NE][QB]  scaled e-e distances u(q, j)
-    double *S = U + (size_t)NE * QB;        // [3][QB][NL_WAVES] per-wave sums: e-e, e-e-n + e-n, psi ratio
+    const T kap = (T)jp.kap, asymp = (T)jp.asymp_r;
+    double *S = lds;                        // [3][QB][NL_WAVES] per-wave sums: e-e, e-e-n + e-n, psi ratio
     double *XQ = S + 3 * QB * NL_WAVES;     // [QB][3] quadrature points
     double *FI = XQ + 3 * QB;               // [NL_WAVES] partial sums of F[i][.]
-    double *GEO = FI + NL_WAVES;            // [QB][NC][5] point - centre: x, y, z, r, grid interval
-    double *RAD = GEO + (size_t)QB * NC * 5;  // [QB][NRI] radial functions at the points
-    const int NRI = a.NRI;
+    T *U = (T *)(FI + NL_WAVES);            // [NE][QB] switched scaled e-e distances u(q, j)
+    T *GEO = U + (size_t)NE * QB;           // [QB][NC][5] point - centre: x, y, z, r, grid interval
+    T *RAD = GEO + (size_t)QB * NC * 5;     // [QB][NRI] radial functions at the points
 
     // XCD-aware contiguous ranges: workgroups are dealt round-robin to the 8
     // XCDs, so logical block lb = (b % 8) * (G / 8) + b / 8 gives each XCD one
@@ -140,24 +144,24 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
             {
                 const int qq = tid % QB;
                 const double px = XQ[3 * qq], py = XQ[3 * qq + 1], pz = XQ[3 * qq + 2];
-                double see = 0.0;
+                T see = T(0);
                 for (int j = tid / QB; j < NE; j += NL_THREADS / QB) {
-                    const double d0 = xw[j] - px, d1 = xw[NE + j] - py, d2 = xw[2 * NE + j] - pz;
-                    double r2 = 0;
+                    const T d0 = (T)(xw[j] - px), d1 = (T)(xw[NE + j] - py), d2 = (T)(xw[2 * NE + j] - pz);
+                    T r2 = 0;
                     r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
-                    const double u = scale_r(kap, sqrt(r2));
-                    U[(size_t)j * QB + qq] = 1.0 - kap * u;
-                    if (j != i) see += ee_term(jp, a.NUP, i, j, u);
+                    const T u = scale_r<T>(kap, sqrt(r2));
+                    U[(size_t)j * QB + qq] = T(1) - kap * u;
+                    if (j != i) see += ee_term<T>(jp, a.NUP, i, j, u);
                 }
-                see = strided_sum<QB>(see);
-                if (lane < QB) S[(0 * QB + lane) * NL_WAVES + wv] = see;
+                const double sees = strided_sum<QB>((double)see);
+                if (lane < QB) S[(0 * QB + lane) * NL_WAVES + wv] = sees;
             }
             __syncthreads();
 
             // ---- phase B: e-e-n (lanes own a centre, stream over electrons) and e-n
-            double acc[QB];
+            T acc[QB];
 #pragma unroll
-            for (int qq = 0; qq < QB; qq++) acc[qq] = 0.0;
+            for (int qq = 0; qq < QB; qq++) acc[qq] = T(0);
             {
                 const int CT = NC < NL_THREADS ? NC : NL_THREADS, JG = NL_THREADS / CT;
                 for (int c0 = 0; c0 < NC; c0 += CT) {
@@ -165,42 +169,42 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
                     if (jg < JG && c < NC) {
                         const int it = a.iwctype[c];
                         const double ecx = a.cent[3 * c], ecy = a.cent[3 * c + 1], ecz = a.cent[3 * c + 2];
-                        double A[QB][een_naz<NORDC>()];
+                        T A[QB][een_naz<NORDC>()];
                         bool any_asym = false;
                         bool asym[QB];
 #pragma unroll
                         for (int qq = 0; qq < QB; qq++) {
-                            const double d0 = XQ[3 * qq] - ecx, d1 = XQ[3 * qq + 1] - ecy, d2 = XQ[3 * qq + 2] - ecz;
-                            double r2 = 0;
+                            const T d0 = (T)(XQ[3 * qq] - ecx), d1 = (T)(XQ[3 * qq + 1] - ecy), d2 = (T)(XQ[3 * qq + 2] - ecz);
+                            T r2 = 0;
                             r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
-                            const double rq = c == ic ? ren : sqrt(r2);
-                            const double rr = scale_r(kap, rq);
+                            const T rq = c == ic ? (T)ren : sqrt(r2);
+                            const T rr = scale_r<T>(kap, rq);
                             if (jg == 0) {  // geometry of the point seen from centre c, for the AOs
-                                double *g = GEO + ((size_t)qq * NC + c) * 5;
+                                T *g = GEO + ((size_t)qq * NC + c) * 5;
                                 g[0] = d0; g[1] = d1; g[2] = d2; g[3] = rq;
-                                g[4] = (double)grid_jx(a.tab, rq);
+                                g[4] = (T)grid_jx(a.tab, (double)rq);
                             }
                             asym[qq] = rr == asymp;
                             any_asym = any_asym || asym[qq];
-                            if (jg == 0 && q0 + qq < NQ) acc[qq] += en_term(jp, it, rr);
-                            if constexpr (NORDC > 1) een_tables<NORDC, 0>(jp.c[it], 1.0 - kap * rr, A[qq]);
+                            if (jg == 0 && q0 + qq < NQ) acc[qq] += en_term<T>(jp, it, rr);
+                            if constexpr (NORDC > 1) een_tables<NORDC, 0, T>(jp.c[it], T(1) - kap * rr, A[qq]);
                         }
                         if constexpr (NORDC > 1) {
                             // one partner of lookahead on the scaled distance
                             double rr_next = jg < NE ? rrt[(size_t)jg * NC + c] : 0.0;
                             for (int j = jg; j < NE; j += JG) {
-                                const double rrj = rr_next;
+                                const T rrj = (T)rr_next;
                                 if (j + JG < NE) rr_next = rrt[(size_t)(j + JG) * NC + c];
                                 if (j == i) continue;
-                                const double rj = 1.0 - kap * rrj;
-                                const double *uj = U + (size_t)j * QB;
-                                double P[QB];
+                                const T rj = T(1) - kap * rrj;
+                                const T *uj = U + (size_t)j * QB;
+                                T P[QB];
 #pragma unroll
-                                for (int qq = 0; qq < QB; qq++) P[qq] = een_eval_tab<NORDC, 0>(A[qq], uj[qq], rj).f;
+                                for (int qq = 0; qq < QB; qq++) P[qq] = een_eval_tab<NORDC, 0, T>(A[qq], uj[qq], rj).f;
                                 if (any_asym && rrj == asymp) {  // psinl: both distances asymptotic -> 0
 #pragma unroll
                                     for (int qq = 0; qq < QB; qq++)
-                                        if (asym[qq]) P[qq] = 0.0;
+                                        if (asym[qq]) P[qq] = T(0);
                                 }
 #pragma unroll
                                 for (int qq = 0; qq < QB; qq++) acc[qq] += P[qq];
@@ -211,38 +215,38 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
             }
 #pragma unroll
             for (int qq = 0; qq < QB; qq++) {
-                const double v = wave_sum(acc[qq]);
+                const double v = wave_sum((double)acc[qq]);
                 if (lane == 0) S[(1 * QB + qq) * NL_WAVES + wv] = v;
             }
 
             // ---- phase C: psi ratio sum_m phi_m(x_q) W[i][m]; radial functions once
             // per (point, centre, shell), the angular factor per AO
             __syncthreads();  // GEO complete
-            for (int t = tid; t < QB * NRI; t += NL_THREADS) {
-                const int qq = t / NRI, ri = t % NRI;
-                const double *g = GEO + ((size_t)qq * NC + a.ri_cent[ri]) * 5;
-                double wfv[3] = {0.0, 0.0, 0.0};
-                if (g[3] <= a.ri_rmax[ri]) splfit_jx<double>(a.tab, a.ri_type[ri], a.ri_rad[ri], g[3], (int)g[4], wfv, 0);
+            for (int tt = tid; tt < QB * NRI; tt += NL_THREADS) {
+                const int qq = tt / NRI, ri = tt % NRI;
+                const T *g = GEO + ((size_t)qq * NC + a.ri_cent[ri]) * 5;
+                T wfv[3] = {T(0), T(0), T(0)};
+                if ((double)g[3] <= a.ri_rmax[ri]) splfit_jx<T>(a.tab, a.ri_type[ri], a.ri_rad[ri], g[3], (int)g[4], wfv, 0);
                 RAD[(size_t)qq * NRI + ri] = wfv[0];
             }
             __syncthreads();
-            double dr[QB];
+            T dr[QB];
 #pragma unroll
-            for (int qq = 0; qq < QB; qq++) dr[qq] = 0.0;
+            for (int qq = 0; qq < QB; qq++) dr[qq] = T(0);
             for (int m = tid; m < NB; m += NL_THREADS) {
                 const int c = a.ao_cent[m], ri = a.ao_ri[m], l = a.ao_slm[m];
-                const double wm = Wm[m];
+                const T wm = (T)Wm[m];
 #pragma unroll
                 for (int qq = 0; qq < QB; qq++) {
-                    const double *g = GEO + ((size_t)qq * NC + c) * 5;
-                    double y, dy[3], lap;
-                    slm<double>(l, g[0], g[1], g[2], y, dy, lap);
+                    const T *g = GEO + ((size_t)qq * NC + c) * 5;
+                    T y, dy[3], lap;
+                    slm<T>(l, g[0], g[1], g[2], y, dy, lap);
                     dr[qq] += (y * RAD[(size_t)qq * NRI + ri]) * wm;
                 }
             }
 #pragma unroll
             for (int qq = 0; qq < QB; qq++) {
-                const double v = wave_sum(dr[qq]);
+                const double v = wave_sum((double)dr[qq]);
                 if (lane == 0) S[(2 * QB + qq) * NL_WAVES + wv] = v;
             }
             __syncthreads();
@@ -269,44 +273,50 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
                     pair_sum = pair_sum + trj * drq;
                 }
             }
-            __syncthreads();  // S, XQ and U are rewritten by the next point block
+            __syncthreads();  // S, XQ, U, GEO and RAD are rewritten by the next point block
         }
         if (tid == 0) a.pair_val[slot] = pair_sum;
     }
 }
 
-template <int NORDC, int QB>
+template <int NORDC, int QB, typename T>
 static int launch_k(champ_ctx *c, const HArgs &a)
 {
-    const size_t lds = ((size_t)c->NE * QB + 3 * QB * NL_WAVES + 3 * QB + NL_WAVES + (size_t)QB * c->NC * 5 +
-                        (size_t)QB * c->NRI) * sizeof(double);
+    const size_t lds = (3 * QB * NL_WAVES + 3 * QB + NL_WAVES) * sizeof(double) +
+                       ((size_t)c->NE * QB + (size_t)QB * c->NC * 5 + (size_t)QB * c->NRI) * sizeof(T);
     if (lds > 160 * 1024) {
-        set_error("nonloc: too many electrons for the quadrature kernel's LDS table");
+        set_error("nonloc: the quadrature kernel's LDS tables exceed 160 KB for this system");
         return CHAMP_ERR_UNSUPPORTED;
     }
     if (lds > 64 * 1024)
-        HIPCHECK(hipFuncSetAttribute((const void *)k_nonloc<NORDC, QB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        HIPCHECK(hipFuncSetAttribute((const void *)k_nonloc<NORDC, QB, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int dev = 0, ncu = 256;
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    hipLaunchKernelGGL((k_nonloc<NORDC, QB>), dim3(ncu * 8), dim3(NL_THREADS), lds, c->stream, a);
+    hipLaunchKernelGGL((k_nonloc<NORDC, QB, T>), dim3(ncu * 8), dim3(NL_THREADS), lds, c->stream, a);
     HIPCHECK(hipGetLastError());
     return CHAMP_OK;
 }
 
-int launch_nonloc(champ_ctx *c, const HArgs &a)
+template <typename T>
+static int launch_t(champ_ctx *c, const HArgs &a)
 {
     switch (c->hjas.nordc) {
-    case 0: return launch_k<0, CHAMP_NL_QB>(c, a);
-    case 1: return launch_k<1, CHAMP_NL_QB>(c, a);
-    case 2: return launch_k<2, CHAMP_NL_QB>(c, a);
-    case 3: return launch_k<3, CHAMP_NL_QB>(c, a);
-    case 4: return launch_k<4, CHAMP_NL_QB>(c, a);
-    case 5: return launch_k<5, 2>(c, a);
+    case 0: return launch_k<0, CHAMP_NL_QB, T>(c, a);
+    case 1: return launch_k<1, CHAMP_NL_QB, T>(c, a);
+    case 2: return launch_k<2, CHAMP_NL_QB, T>(c, a);
+    case 3: return launch_k<3, CHAMP_NL_QB, T>(c, a);
+    case 4: return launch_k<4, CHAMP_NL_QB, T>(c, a);
+    case 5: return launch_k<5, 2, T>(c, a);
     default:
         set_error("nonloc: e-e-n order above 5 is not supported");
         return CHAMP_ERR_UNSUPPORTED;
     }
+}
+
+int launch_nonloc(champ_ctx *c, const HArgs &a)
+{
+    return c->prec == CHAMP_PREC_MIXED ? launch_t<float>(c, a) : launch_t<double>(c, a);
 }
 
 }  // namespace champ

@@ -52,9 +52,9 @@ static double check(std::mt19937_64 &g)
         const double u = P(g), x = P(g), y = P(g);
         double tab[3 * NA];
         een_tables<N, 2>(c, x, tab);
-        const EenOut o = een_eval_tab<N, 2>(tab, u, y);
-        const EenOut o1 = een_eval_tab<N, 1>(tab, u, y);
-        const EenOut o0 = een_eval_tab<N, 0>(tab, u, y);
+        const EenOut<double> o = een_eval_tab<N, 2>(tab, u, y);
+        const EenOut<double> o1 = een_eval_tab<N, 1>(tab, u, y);
+        const EenOut<double> o0 = een_eval_tab<N, 0>(tab, u, y);
         const Ref r = direct<N>(c, u, x, y);
         const double e[] = {relerr(o.f, r.f), relerr(o.fu, r.fu), relerr(o.fx, r.fx), relerr(o.fy, r.fy),
                             relerr(o.fuu, r.fuu), relerr(o.fxx, r.fxx), relerr(o.fyy, r.fyy), relerr(o.fux, r.fux),

@@ -112,7 +112,8 @@ inline int __builtin_amdgcn_readfirstlane(int v)
     return (int)(uint32_t)all[base];
 }
 // FP64 MFMA 16x16x4: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]
-// and holds D[4 (l / 16) + r][l % 16], r = 0..3
+// and holds D[4 r + l / 16][l % 16] in accumulator element r = 0..3 (the
+// layout measured on gfx950 with a one-hot probe of A and B)
 #define CHAMP_DBL4_DEFINED
 struct dbl4 {
     double v[4];
@@ -128,7 +129,7 @@ inline dbl4 __builtin_amdgcn_mfma_f64_16x16x4f64(double a, double b, dbl4 c, int
     emu::wave_exchange(ub, all_b);
     const int l = emu::tctx.lane % 64, j = l % 16;
     for (int r = 0; r < 4; r++) {
-        const int i = 4 * (l / 16) + r;
+        const int i = 4 * r + l / 16;
         double s = c[r];
         for (int k = 0; k < 4; k++) {
             double x, y;
