@@ -6,7 +6,14 @@ electron a drift-diffusion proposal, psie (distances, jastrow4e, orbitalse,
 Sherman-Morrison ratio), the new drift, accept/reject and commit; then one
 hpsi (full Jastrow, orbitals, LU inverse, ECP quadrature, local energy) per
 walker.  Workload: 40 nuclei (20 Z_eff=4 with s/p non-local ECP + 20 H),
-160 electrons (80/80), 600 AOs, 12-point quadrature, 4096 walkers, FP64.
+160 electrons (80/80), 600 AOs, 12-point quadrature, 4096 walkers.
+
+The same walkers and RNG streams are run on the plain FP64 path and on the
+mixed-precision path (FP32 Jastrow, per-move AOs and quadrature Jastrow terms;
+FP64 determinants, inverse refresh every 80 moves and local energy).  `value`
+is the mixed path's rate; the FP64 rate, their ratio and the per-walker
+local-energy and log-psi differences on identical configurations are in the
+same line.
 
 Prints one JSON line (walker-moves/s = walkers * electrons * sweeps / s).
 """
@@ -23,9 +30,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic FP64 flop counts of the kernels' loop structure (per walker):
+# Algorithmic FP64 flop counts (per walker) of the reference algorithm's loop
+# structure, the per-unit figures the roofline is priced with:
 #   move (jastrow4e + AO):  (Ne-1)*Nc*(25T+60) + AO
-#   hpsi_nonloc (per quadrature point): (Ne-1)*(Nc*(4T+40) + 60) + Nc*40 + Nao*(40 + 2)
+#   hpsi_nonloc (per quadrature point): (Ne-1)*(Nc*(4T+40)+60) + Nc*40 + Nao*42
+#   hpsi_jastrow: Ne(Ne-1)/2 * Nc * (54T+80)
 # T = e-e-n terms (13), Nc = centres, Nao = AOs.
 
 
@@ -35,6 +44,57 @@ def flops_move_jastrow(ne, nc, nt, nao, nrad_total):
 
 def flops_nonloc_point(ne, nc, nt, nao):
     return (ne - 1) * (nc * (4 * nt + 40) + 60) + nc * 40 + nao * 42
+
+
+def run_path(sy, W, precision, args, label):
+    from champ_amd import system as S
+    from champ_amd.engine import Engine
+
+    eng = Engine(sy, W, precision=precision)
+    eng.set_walkers(sy.initial_walkers(W, seed=1))
+    eng.seed_rng(S.seed_from_string())
+    g0 = eng.hpsi()
+    print(f"[bench] {label}: {sy.name}: Ne={sy.nelec} Nc={sy.ncent} Nao={sy.nbasis} W={W}; warmup {args.warmup}",
+          flush=True)
+    if args.warmup:
+        eng.metrop1(args.warmup, args.tau)
+    eng.enable_timing(True)
+    t0 = time.perf_counter()
+    st = eng.metrop1(args.steps, args.tau)
+    t1 = time.perf_counter()
+    kt = eng.kernel_times()
+    eng.enable_timing(False)
+    nxq = eng.array("nxquad")
+    eng.close()
+    wall = t1 - t0
+    print(f"[bench] {label}: {args.steps} sweeps in {wall:.3f}s; acceptance {st['accepted'] / st['proposed']:.3f}; "
+          f"<E_L> {st['esum'] / (W * args.steps):.6f}; flagged {st['nflagged']}", flush=True)
+    for k, (ms, n) in kt.items():
+        if n:
+            print(f"[bench]   {label} {k:20s} {ms:10.2f} ms  {n:6d} launches  {ms / n:9.4f} ms/launch", flush=True)
+    return {"wall": wall, "value": W * sy.nelec * args.steps / wall, "kt": kt, "st": st, "nxq": nxq, "hpsi0": g0}
+
+
+def roofline(sy, W, kt, nxq):
+    from champ_amd import system as S
+    nt = len(S.een_terms(sy.nordc))
+    nrad = int(sum(sy.nrbas[t] for t in sy.iwctype))
+    dname, (dms, dn) = max(kt.items(), key=lambda kv: kv[1][0])
+    if dname == "hpsi_nonloc":
+        fl = float(np.sum(nxq)) * flops_nonloc_point(sy.nelec, sy.ncent, nt, sy.nbasis)
+    elif dname == "move_jastrow_ao":
+        fl = W * flops_move_jastrow(sy.nelec, sy.ncent, nt, sy.nbasis, nrad)
+    elif dname == "hpsi_jastrow":
+        fl = W * 0.5 * sy.nelec * (sy.nelec - 1) * sy.ncent * (54 * nt + 80)
+    else:
+        return None
+    if not dn:
+        return None
+    ach = fl / (dms / dn / 1e3) / 1e12
+    return {"bound": "mfma", "pipe": "fp64-valu", "kernel": dname, "achieved": round(ach, 3), "peak": 78.6,
+            "unit": "TFLOP/s", "frac": round(ach / 78.6, 4), "traffic": None, "avg_launch_ms": round(dms / dn, 4),
+            "flops_per_launch": fl,
+            "flops_basis": "reference algorithm's per-term operation count, not the kernel's executed instructions"}
 
 
 def main():
@@ -48,6 +108,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-sweeps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true", help="time the plain FP64 path only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs the walker population on one GPU; "
@@ -55,53 +116,32 @@ def main():
         sys.exit(2)
 
     from champ_amd import system as S
-    from champ_amd.engine import Engine
+    from champ_amd.engine import PREC_FP64, PREC_MIXED
 
     sy = S.config(args.system)
     W = args.walkers
-    eng = Engine(sy, W)
-    eng.set_walkers(sy.initial_walkers(W, seed=1))
-    eng.seed_rng(S.seed_from_string())
-    eng.hpsi()
-    print(f"[bench] {sy.name}: Ne={sy.nelec} Nc={sy.ncent} Nao={sy.nbasis} W={W}; warmup {args.warmup}", flush=True)
-    if args.warmup:
-        eng.metrop1(args.warmup, args.tau)
-    eng.enable_timing(True)
-    t0 = time.perf_counter()
-    st = eng.metrop1(args.steps, args.tau)
-    t1 = time.perf_counter()
-    kt = eng.kernel_times()
-    eng.enable_timing(False)
-    wall = t1 - t0
-    moves = W * sy.nelec * args.steps
-    value = moves / wall
-    nxq = eng.array("nxquad")
-    print(f"[bench] {args.steps} sweeps in {wall:.3f}s; acceptance {st['accepted'] / st['proposed']:.3f}; "
-          f"<E_L> {st['esum'] / (W * args.steps):.4f}; flagged {st['nflagged']}", flush=True)
-    for k, (ms, n) in kt.items():
-        if n:
-            print(f"[bench]   {k:20s} {ms:10.2f} ms  {n:6d} launches  {ms / n:9.4f} ms/launch", flush=True)
+    r64 = run_path(sy, W, PREC_FP64, args, "fp64")
+    rmx = None if args.fp64_only else run_path(sy, W, PREC_MIXED, args, "mixed")
+    head = rmx if rmx is not None else r64
 
-    # dominant kernel class by time + its algorithmic rate
-    nt = len(S.een_terms(sy.nordc))
-    nrad = int(sum(sy.nrbas[t] for t in sy.iwctype))
-    dom = max(kt.items(), key=lambda kv: kv[1][0])
-    dname, (dms, dn) = dom[0], dom[1]
-    if dname == "hpsi_nonloc":
-        fl = float(np.sum(nxq)) * flops_nonloc_point(sy.nelec, sy.ncent, nt, sy.nbasis)
-    elif dname == "move_jastrow_ao":
-        fl = W * flops_move_jastrow(sy.nelec, sy.ncent, nt, sy.nbasis, nrad)
-    elif dname == "hpsi_jastrow":
-        fl = W * 0.5 * sy.nelec * (sy.nelec - 1) * sy.ncent * (54 * nt + 80)
-    else:
-        fl = None
-    roof = None
-    if fl is not None and dn:
-        avg_s = dms / dn / 1e3
-        ach = fl / avg_s / 1e12
-        roof = {"bound": "mfma", "pipe": "fp64-valu", "kernel": dname, "achieved": round(ach, 3), "peak": 78.6,
-                "unit": "TFLOP/s", "frac": round(ach / 78.6, 4), "traffic": None,
-                "avg_launch_ms": round(dms / dn, 4), "flops_per_launch": fl}
+    acc = None
+    if rmx is not None:
+        g64, gmx = r64["hpsi0"], rmx["hpsi0"]
+        de = np.abs(gmx["energy"] - g64["energy"])
+        lp64 = np.log(np.abs(g64["psid"])) + g64["psij"]
+        lpmx = np.log(np.abs(gmx["psid"])) + gmx["psij"]
+        dl = np.abs(lpmx - lp64)
+        e64 = r64["st"]["esum"] / (W * args.steps)
+        emx = rmx["st"]["esum"] / (W * args.steps)
+        acc = {"sample": f"initial hpsi of all {W} walkers, identical configurations",
+               "dE_L_max_Ha": float(de.max()), "dE_L_p99_Ha": float(np.percentile(de, 99)),
+               "dE_L_mean_Ha": float(de.mean()),
+               "dE_L_rel_max": float((de / np.maximum(1.0, np.abs(g64["energy"]))).max()),
+               "dlnpsi_max": float(dl.max()), "dlnpsi_mean": float(dl.mean()),
+               "vmc_mean_E_fp64": e64, "vmc_mean_E_mixed": emx,
+               "vmc_mean_note": f"mean E_L over the {args.steps} timed sweeps of each run (same seeds)"}
+        print(f"[bench] accuracy: |dE_L| max {acc['dE_L_max_Ha']:.3e} Ha p99 {acc['dE_L_p99_Ha']:.3e} "
+              f"mean {acc['dE_L_mean_Ha']:.3e}; |dln psi| max {acc['dlnpsi_max']:.3e}", flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -111,35 +151,40 @@ def main():
         nwc = nth
         x0 = sy.initial_walkers(nwc, seed=1)
         streams = ffi.seeded_streams(S.seed_from_string(), nwc)
-        es, acc = np.zeros(1), np.zeros(1)
+        es, ac = np.zeros(1), np.zeros(1)
         cs = sy.as_ctypes()
         tc = ffi.lib().orc_vmc_run(C.byref(cs), nwc, ffi.P(x0), ffi.U(streams), args.cpu_sweeps, args.tau, nth,
-                                    ffi.P(es), ffi.P(acc))
+                                    ffi.P(es), ffi.P(ac))
         cpu = {"value": round(nwc * sy.nelec * args.cpu_sweeps / tc, 2), "unit": "walker-moves/s", "cores": nth,
                "kind": "port",
                "sample": f"{nwc} walkers x {args.cpu_sweeps} metrop1 sweep(s) of the same system, one walker per "
                          f"thread, {tc:.1f} s"}
 
     out = {
-        "metric": "walker-moves/sec at 160 e\u207b on 1 MI355X, mixed vs FP64 GPU; \u0394E_L vs FP64",
-        "value": round(value, 1),
+        "metric": "walker-moves/sec at 160 e⁻ on 1 MI355X, mixed vs FP64 GPU; ΔE_L vs FP64",
+        "value": round(head["value"], 1),
         "unit": "walker-moves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * wall / args.steps, 2),
+        "ms_per_step": round(1e3 * head["wall"] / args.steps, 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if rmx is None else "f32+f64",
         "data": "synthetic (seeded generator: geometry, basis, MOs, Jastrow, ECP; walkers Gaussian around nuclei)",
         "config": {"workload": f"metrop1 sweep, {sy.nelec} e, {sy.ncent} nuclei, {sy.nbasis} AOs, ECP + 12-pt "
                                f"quadrature, {W} walkers", "walkers": W, "electrons": sy.nelec,
-                   "nuclei": sy.ncent, "aos": sy.nbasis, "precision": "fp64", "path": "plain FP64 GPU path",
+                   "nuclei": sy.ncent, "aos": sy.nbasis, "precision": "fp64" if rmx is None else "mixed",
+                   "path": "plain FP64 GPU path" if rmx is None else "mixed-precision GPU path",
                    "parallelism": "walkers on 1 GPU"},
-        "walker_sweeps_per_s": round(W * args.steps / wall, 2),
-        "acceptance": round(st["accepted"] / st["proposed"], 4),
-        "roofline": roof,
+        "fp64_value": round(r64["value"], 1),
+        "mixed_over_fp64": None if rmx is None else round(rmx["value"] / r64["value"], 3),
+        "walker_sweeps_per_s": round(W * args.steps / head["wall"], 2),
+        "acceptance": round(head["st"]["accepted"] / head["st"]["proposed"], 4),
+        "accuracy_mixed_vs_fp64": acc,
+        "roofline": roofline(sy, W, head["kt"], head["nxq"]),
+        "roofline_fp64_path": roofline(sy, W, r64["kt"], r64["nxq"]),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -120,7 +120,7 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
 {
     if (!ctxp || !s || nwalk <= 0) { set_error("init: bad arguments"); return CHAMP_ERR_ARG; }
     *ctxp = nullptr;
-    if (precision != CHAMP_PREC_FP64) { set_error("init: only CHAMP_PREC_FP64 is available in this build"); return CHAMP_ERR_UNSUPPORTED; }
+    if (precision != CHAMP_PREC_FP64 && precision != CHAMP_PREC_MIXED) { set_error("init: unknown precision"); return CHAMP_ERR_ARG; }
     if (s->nctype > CHAMP_MAXCT || s->ncent > 128 || s->nelec > 1024 || s->nelec < 2 || s->ndet != 1 ||
         s->norda >= CHAMP_NORDP || s->nordb >= CHAMP_NORDP || s->nordc >= CHAMP_NORDP || s->nspin2b < 1 ||
         s->nspin2b > 2 || (s->nloc != 0 && s->nloc != 4) || s->nquad > CHAMP_MPS_QUAD) {
@@ -144,6 +144,8 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
     c->W = nwalk; c->NE = s->nelec; c->NUP = s->nup; c->NDN = s->nelec - s->nup;
     c->NM = std::max(c->NUP, c->NDN); c->NC = s->ncent; c->NCT = s->nctype; c->NB = s->nbasis; c->NO = s->norb;
     c->NQ = s->nloc ? s->nquad : 0; c->nloc = s->nloc; c->prec = precision; c->cdet0 = s->cdet[0];
+    // mixed path: the FP32-fed Sherman-Morrison inverse is recomputed in FP64 every 80 moves
+    c->lu_every = precision == CHAMP_PREC_MIXED ? 80 : 0;
     if (c->NM > 400) { set_error("init: more than 400 electrons of one spin"); delete c; *ctxp = nullptr; return CHAMP_ERR_UNSUPPORTED; }
     if (hipStreamCreate(&c->stream) != hipSuccess) { set_error("init: stream"); delete c; *ctxp = nullptr; return CHAMP_ERR_HIP; }
 
@@ -509,6 +511,8 @@ extern "C" int champ_gpu_metrop1(champ_ctx *c, int32_t nsweep, double tau, champ
             if ((rc = launch_move_a(c, iel, 0))) return rc;
             if ((rc = launch_move_gemm(c))) return rc;
             if ((rc = launch_move_b(c, iel, 0))) return rc;
+            if (c->lu_every > 0 && (iel + 1) % c->lu_every == 0 && iel + 1 < c->NE && (rc = launch_slater_refresh(c)))
+                return rc;
         }
         if ((rc = launch_hpsi_impl(c, 1))) return rc;
         if (c->timing && (isw % 4) == 3) tflush(c);

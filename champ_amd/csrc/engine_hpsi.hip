@@ -567,6 +567,62 @@ __global__ void __launch_bounds__(256) k_hpsi_eloc(HArgs a)
     }
 }
 
+// AO values, orbitals and the LU inverse of both Slater matrices of every
+// walker, in chunks of walkers (orbitals of all electrons are large): the
+// determinant part of hpsi, also used on its own for the periodic FP64
+// refresh of the inverse on the mixed path.
+static int launch_orbitals_slater(champ_ctx *c, HArgs a)
+{
+    int slot;
+    for (int w0 = 0; w0 < c->W; w0 += c->hpsi_chunk) {
+        a.w0 = w0;
+        a.nw = std::min(c->hpsi_chunk, c->W - w0);
+        const long long nwave = (long long)a.nw * c->NE;
+        tbegin(c, K_HPSI_AO, &slot);
+        // FP64 on both paths: these AOs feed the LU inverse that hpsi and the
+        // periodic refresh recompute from scratch, and the Slater matrices of a
+        // trial function can be ill-conditioned enough to turn FP32 AO rounding
+        // into 1e-3 relative errors of the determinant
+        hipLaunchKernelGGL(k_hpsi_ao<double>, dim3((unsigned)((nwave + 3) / 4)), dim3(256),
+                           (size_t)4 * 3 * c->NRI * sizeof(double), c->stream, a);
+        tend(c, slot);
+        HIPCHECK(hipGetLastError());
+        tbegin(c, K_HPSI_GEMM, &slot);
+        int rcg = launch_gemm(c, true, (int)(nwave * 5), c->NO, c->NB, c->aoall, c->NB, 0, c->coef, c->NB, 0,
+                              c->orball, c->NO, 0, 1);
+        tend(c, slot);
+        if (rcg) return rcg;
+        const int nmx = std::max(c->NUP, c->NDN);
+        const size_t lds = (size_t)(nmx * (nmx + 1) + nmx) * sizeof(double);
+        if (lds > 64 * 1024) HIPCHECK(hipFuncSetAttribute((const void *)k_hpsi_slater, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        tbegin(c, K_HPSI_SLATER, &slot);
+        hipLaunchKernelGGL(k_hpsi_slater, dim3(2 * a.nw), dim3(256), lds, c->stream, a);
+        tend(c, slot);
+        HIPCHECK(hipGetLastError());
+    }
+    return CHAMP_OK;
+}
+
+// psido / psi2o from refreshed determinants (the Jastrow part is unchanged)
+__global__ void k_refresh_psi(HArgs a)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= a.W) return;
+    const double psid = 0.0 + a.detiab[2 * w] * a.detiab[2 * w + 1] * a.cdet0;
+    a.psido[w] = psid;
+    a.psi2o[w] = 2 * (log(fabs(psid)) + a.psijo[w]);
+}
+
+int launch_slater_refresh(champ_ctx *c)
+{
+    HArgs a = hargs(c);
+    const int rc = launch_orbitals_slater(c, a);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_refresh_psi, dim3((c->W + 255) / 256), dim3(256), 0, c->stream, a);
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
 template <typename T>
 static int launch_hpsi_jastrow_t(champ_ctx *c, const HArgs &a, int nordc, int naz)
 {
@@ -610,32 +666,8 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
     }
     tend(c, slot);
     HIPCHECK(hipGetLastError());
-    for (int w0 = 0; w0 < c->W; w0 += c->hpsi_chunk) {
-        a.w0 = w0;
-        a.nw = std::min(c->hpsi_chunk, c->W - w0);
-        const long long nwave = (long long)a.nw * c->NE;
-        tbegin(c, K_HPSI_AO, &slot);
-        if (c->prec == CHAMP_PREC_MIXED)
-            hipLaunchKernelGGL(k_hpsi_ao<float>, dim3((unsigned)((nwave + 3) / 4)), dim3(256),
-                               (size_t)4 * 3 * c->NRI * sizeof(float), c->stream, a);
-        else
-            hipLaunchKernelGGL(k_hpsi_ao<double>, dim3((unsigned)((nwave + 3) / 4)), dim3(256),
-                               (size_t)4 * 3 * c->NRI * sizeof(double), c->stream, a);
-        tend(c, slot);
-        HIPCHECK(hipGetLastError());
-        tbegin(c, K_HPSI_GEMM, &slot);
-        int rcg = launch_gemm(c, true, (int)(nwave * 5), c->NO, c->NB, c->aoall, c->NB, 0, c->coef, c->NB, 0,
-                              c->orball, c->NO, 0, 1);
-        tend(c, slot);
-        if (rcg) return rcg;
-        const int nmx = std::max(c->NUP, c->NDN);
-        const size_t lds = (size_t)(nmx * (nmx + 1) + nmx) * sizeof(double);
-        if (lds > 64 * 1024) HIPCHECK(hipFuncSetAttribute((const void *)k_hpsi_slater, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        tbegin(c, K_HPSI_SLATER, &slot);
-        hipLaunchKernelGGL(k_hpsi_slater, dim3(2 * a.nw), dim3(256), lds, c->stream, a);
-        tend(c, slot);
-        HIPCHECK(hipGetLastError());
-    }
+    const int rco = launch_orbitals_slater(c, a);
+    if (rco) return rco;
     a.w0 = 0; a.nw = c->W;
     if (c->nloc > 0) {
         tbegin(c, K_HPSI_NONLOC, &slot);

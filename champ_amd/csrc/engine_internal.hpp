@@ -50,6 +50,7 @@ struct champ_ctx {
     double *ri_rmax = nullptr;                                      // [NRI]
     int *ao_ri = nullptr;                                           // [NB] radial instance of each AO
     double *gref = nullptr;                                         // [nr][2] grid node radii
+    int lu_every = 0;  // moves between FP64 recomputes of the Slater inverse inside a sweep (0: only in hpsi)
     int *ao_cent = nullptr, *ao_type = nullptr, *ao_rad = nullptr, *ao_slm = nullptr;  // [NB]
     double *coef = nullptr;          // [NO][NB]
     double *cocc[2] = {nullptr, nullptr};  // [nel][NB] coef rows of occupied orbitals
@@ -122,6 +123,7 @@ int launch_move_gemm(champ_ctx *c);
 int launch_move_b(champ_ctx *c, int iel, int mode);
 int launch_drift_old(champ_ctx *c, int iel);
 int launch_hpsi(champ_ctx *c);
+int launch_slater_refresh(champ_ctx *c);
 int launch_rotqua(champ_ctx *c);
 int launch_basis_points(champ_ctx *c, int npt, const double *dpos, double *dout);
 int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int lda, long long sA, const double *B,

@@ -78,9 +78,11 @@ __device__ __forceinline__ double strided_sum(double v)
 constexpr int NL_THREADS = 256;
 constexpr int NL_WAVES = NL_THREADS / WAVE;
 
-// T: arithmetic type of the distances, Jastrow terms and AO values (double
-// on the FP64 path, float on the mixed path).  Sums across lanes, the per-
-// point assembly and every table in memory stay double.
+// T: arithmetic type of the Jastrow part (double on the FP64 path, float on
+// the mixed path).  The determinant ratio at the points stays double on both
+// paths: sum_m phi_m W[i][m] cancels strongly when the inverse has large
+// entries, so FP32 AO values would lose its leading digits.  Sums across
+// lanes, the per-point assembly and every table in memory are double.
 template <int NORDC, int QB, typename T>
 __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k_nonloc(HArgs a)
 {
@@ -92,9 +94,9 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
     double *S = lds;                        // [3][QB][NL_WAVES] per-wave sums: e-e, e-e-n + e-n, psi ratio
     double *XQ = S + 3 * QB * NL_WAVES;     // [QB][3] quadrature points
     double *FI = XQ + 3 * QB;               // [NL_WAVES] partial sums of F[i][.]
-    T *U = (T *)(FI + NL_WAVES);            // [NE][QB] switched scaled e-e distances u(q, j)
-    T *GEO = U + (size_t)NE * QB;           // [QB][NC][5] point - centre: x, y, z, r, grid interval
-    T *RAD = GEO + (size_t)QB * NC * 5;     // [QB][NRI] radial functions at the points
+    double *GEO = FI + NL_WAVES;            // [QB][NC][5] point - centre: x, y, z, r, grid interval
+    double *RAD = GEO + (size_t)QB * NC * 5;  // [QB][NRI] radial functions at the points
+    T *U = (T *)(RAD + (size_t)QB * NRI);   // [NE][QB] switched scaled e-e distances u(q, j)
 
     // XCD-aware contiguous ranges: workgroups are dealt round-robin to the 8
     // XCDs, so logical block lb = (b % 8) * (G / 8) + b / 8 gives each XCD one
@@ -174,16 +176,20 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
                         bool asym[QB];
 #pragma unroll
                         for (int qq = 0; qq < QB; qq++) {
-                            const T d0 = (T)(XQ[3 * qq] - ecx), d1 = (T)(XQ[3 * qq + 1] - ecy), d2 = (T)(XQ[3 * qq + 2] - ecz);
+                            const double e0 = XQ[3 * qq] - ecx, e1 = XQ[3 * qq + 1] - ecy, e2 = XQ[3 * qq + 2] - ecz;
+                            if (jg == 0) {  // geometry of the point seen from centre c, for the AOs (double)
+                                double r2d = 0;
+                                r2d = r2d + e0 * e0; r2d = r2d + e1 * e1; r2d = r2d + e2 * e2;
+                                const double rqd = c == ic ? ren : sqrt(r2d);
+                                double *g = GEO + ((size_t)qq * NC + c) * 5;
+                                g[0] = e0; g[1] = e1; g[2] = e2; g[3] = rqd;
+                                g[4] = (double)grid_jx(a.tab, rqd);
+                            }
+                            const T d0 = (T)e0, d1 = (T)e1, d2 = (T)e2;
                             T r2 = 0;
                             r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
                             const T rq = c == ic ? (T)ren : sqrt(r2);
                             const T rr = scale_r<T>(kap, rq);
-                            if (jg == 0) {  // geometry of the point seen from centre c, for the AOs
-                                T *g = GEO + ((size_t)qq * NC + c) * 5;
-                                g[0] = d0; g[1] = d1; g[2] = d2; g[3] = rq;
-                                g[4] = (T)grid_jx(a.tab, (double)rq);
-                            }
                             asym[qq] = rr == asymp;
                             any_asym = any_asym || asym[qq];
                             if (jg == 0 && q0 + qq < NQ) acc[qq] += en_term<T>(jp, it, rr);
@@ -224,29 +230,29 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
             __syncthreads();  // GEO complete
             for (int tt = tid; tt < QB * NRI; tt += NL_THREADS) {
                 const int qq = tt / NRI, ri = tt % NRI;
-                const T *g = GEO + ((size_t)qq * NC + a.ri_cent[ri]) * 5;
-                T wfv[3] = {T(0), T(0), T(0)};
-                if ((double)g[3] <= a.ri_rmax[ri]) splfit_jx<T>(a.tab, a.ri_type[ri], a.ri_rad[ri], g[3], (int)g[4], wfv, 0);
+                const double *g = GEO + ((size_t)qq * NC + a.ri_cent[ri]) * 5;
+                double wfv[3] = {0.0, 0.0, 0.0};
+                if (g[3] <= a.ri_rmax[ri]) splfit_jx<double>(a.tab, a.ri_type[ri], a.ri_rad[ri], g[3], (int)g[4], wfv, 0);
                 RAD[(size_t)qq * NRI + ri] = wfv[0];
             }
             __syncthreads();
-            T dr[QB];
+            double dr[QB];
 #pragma unroll
-            for (int qq = 0; qq < QB; qq++) dr[qq] = T(0);
+            for (int qq = 0; qq < QB; qq++) dr[qq] = 0.0;
             for (int m = tid; m < NB; m += NL_THREADS) {
                 const int c = a.ao_cent[m], ri = a.ao_ri[m], l = a.ao_slm[m];
-                const T wm = (T)Wm[m];
+                const double wm = Wm[m];
 #pragma unroll
                 for (int qq = 0; qq < QB; qq++) {
-                    const T *g = GEO + ((size_t)qq * NC + c) * 5;
-                    T y, dy[3], lap;
-                    slm<T>(l, g[0], g[1], g[2], y, dy, lap);
+                    const double *g = GEO + ((size_t)qq * NC + c) * 5;
+                    double y, dy[3], lap;
+                    slm<double>(l, g[0], g[1], g[2], y, dy, lap);
                     dr[qq] += (y * RAD[(size_t)qq * NRI + ri]) * wm;
                 }
             }
 #pragma unroll
             for (int qq = 0; qq < QB; qq++) {
-                const double v = wave_sum((double)dr[qq]);
+                const double v = wave_sum(dr[qq]);
                 if (lane == 0) S[(2 * QB + qq) * NL_WAVES + wv] = v;
             }
             __syncthreads();
@@ -282,8 +288,8 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
 template <int NORDC, int QB, typename T>
 static int launch_k(champ_ctx *c, const HArgs &a)
 {
-    const size_t lds = (3 * QB * NL_WAVES + 3 * QB + NL_WAVES) * sizeof(double) +
-                       ((size_t)c->NE * QB + (size_t)QB * c->NC * 5 + (size_t)QB * c->NRI) * sizeof(T);
+    const size_t lds = (3 * QB * NL_WAVES + 3 * QB + NL_WAVES + (size_t)QB * c->NC * 5 + (size_t)QB * c->NRI) * sizeof(double) +
+                       (size_t)c->NE * QB * sizeof(T);
     if (lds > 160 * 1024) {
         set_error("nonloc: the quadrature kernel's LDS tables exceed 160 KB for this system");
         return CHAMP_ERR_UNSUPPORTED;

@@ -48,7 +48,11 @@ extern "C" {
 
 /* ---- precision modes ----------------------------------------------------- */
 #define CHAMP_PREC_FP64  0     /* every stage in IEEE double                      */
-#define CHAMP_PREC_MIXED 1     /* FP32 AO/Jastrow/MO, FP64 inverse updates and E_L */
+/* FP32 arithmetic for AO values, the Jastrow (moves and hpsi) and the ECP
+ * quadrature terms; FP64 for the AO->MO contraction (matrix cores), the
+ * Sherman-Morrison updates, a full recompute of the Slater inverse every 80
+ * moves, every sum across lanes and the local energy. */
+#define CHAMP_PREC_MIXED 1
 
 /*
  * System description (read-only tables shared by all walkers).  Mirrors the
