@@ -50,6 +50,17 @@ struct EenTable {
     }
 };
 
+// Switched scaled distance 1 - kap R(r) = exp(-kap r) of isc = 2
+// (switch_scale2, scale_dist.f90:314-331).  In double the reference's form
+// 1 - kap R; in float that form cancels catastrophically far from a centre
+// (R -> 1/kap), so the float path evaluates exp(-kap r) directly.
+template <typename T>
+__device__ __forceinline__ T switched_from(T kap, T r, T R)
+{
+    if constexpr (sizeof(T) == sizeof(float)) return exp(-kap * r);
+    else return T(1) - kap * R;
+}
+
 template <int N>
 constexpr int een_naz() { return EenTable<N>{}.naz > 0 ? EenTable<N>{}.naz : 1; }
 

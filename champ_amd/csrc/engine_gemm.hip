@@ -137,6 +137,68 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int M, int N, int K, const do
     }
 }
 
+// Strip variant: each wavefront owns 16 rows x (16 WN) columns, so every A
+// value it loads feeds WN instructions; the block's four wavefronts share the
+// B chunk of 16 k x (16 WN) columns through LDS (double-buffered, one barrier
+// per chunk).  Same k re-pairing and accumulator layout as k_gemm_mfma.
+template <bool BT, int WN>
+__global__ void __launch_bounds__(256) k_gemm_mfma_strip(int M, int N, int K, const double *__restrict__ A, int lda,
+                                                         long long sA, const double *__restrict__ B, int ldb,
+                                                         long long sB, double *__restrict__ Cm, int ldc, long long sC)
+{
+    constexpr int BN = 16 * WN;
+    __shared__ double Bs[2][16][BN + 1];
+    const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
+    const int m0 = blockIdx.x * 64 + wv * 16, n0 = blockIdx.y * BN;
+    A += (long long)blockIdx.z * sA;
+    B += (long long)blockIdx.z * sB;
+    Cm += (long long)blockIdx.z * sC;
+    const int row = m0 + lane % 16, kq = 4 * (lane / 16);
+    const bool rok = row < M;
+    const double *Ar = A + (long long)(rok ? row : 0) * lda;
+    dbl4 acc[WN];
+#pragma unroll
+    for (int t = 0; t < WN; t++) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    auto load_b = [&](int buf, int k0) {
+        for (int e = tid; e < 16 * BN; e += 256) {
+            int kk, nn;
+            if (BT) { nn = e / 16; kk = e % 16; }
+            else { kk = e / BN; nn = e % BN; }
+            const int gk = k0 + kk, gn = n0 + nn;
+            double v = 0.0;
+            if (gk < K && gn < N) v = BT ? B[(long long)gn * ldb + gk] : B[(long long)gk * ldb + gn];
+            Bs[buf][kk][nn] = v;
+        }
+    };
+    const int nk = (K + 15) / 16;
+    load_b(0, 0);
+    __syncthreads();
+    for (int kc = 0; kc < nk; kc++) {
+        const int buf = kc & 1, k0 = kc * 16;
+        if (kc + 1 < nk) load_b(buf ^ 1, k0 + 16);  // the other buffer was last read before the previous barrier
+        const int kb = k0 + kq;
+        double a[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) a[s] = (rok && kb + s < K) ? Ar[kb + s] : 0.0;
+#pragma unroll
+        for (int t = 0; t < WN; t++)
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], Bs[buf][kq + s][t * 16 + lane % 16], acc[t], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < WN; t++) {
+        const int col = n0 + t * 16 + lane % 16;
+        if (col >= N) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int ro = m0 + 4 * r + lane / 16;
+            if (ro < M) Cm[(long long)ro * ldc + col] = acc[t][r];
+        }
+    }
+}
+
 // C[b] (M x N, row-major, ldc) = A[b] (M x K, lda) * op(B[b]); op = B^T when
 // bt (B stored N x K) else B (K x N).
 int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int lda, long long sA, const double *B,
@@ -155,6 +217,22 @@ int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int
             hipLaunchKernelGGL(k_gemm_valu<false>, grid, dim3(256), 0, c->stream, M, N, K, A, lda, sA, B, ldb, sB, Cm, ldc, sC);
         HIPCHECK(hipGetLastError());
         return CHAMP_OK;
+    }
+    // strips of 16 x 80 columns per wavefront, 64 rows per block
+    {
+        constexpr int WN = 5, BN = 16 * WN;
+        const long long gx = (M + 63) / 64, gy = (N + BN - 1) / BN;
+        if (gx <= 0x7fffffffLL && gy <= 65535 && batch <= 65535) {
+            dim3 grid((unsigned)gx, (unsigned)gy, batch);
+            if (bt)
+                hipLaunchKernelGGL((k_gemm_mfma_strip<true, WN>), grid, dim3(256), 0, c->stream, M, N, K, A, lda, sA, B, ldb,
+                                   sB, Cm, ldc, sC);
+            else
+                hipLaunchKernelGGL((k_gemm_mfma_strip<false, WN>), grid, dim3(256), 0, c->stream, M, N, K, A, lda, sA, B,
+                                   ldb, sB, Cm, ldc, sC);
+            HIPCHECK(hipGetLastError());
+            return CHAMP_OK;
+        }
     }
     const long long waves = (long long)((M + 15) / 16) * ((N + 15) / 16);
     const long long blocks = (waves + 3) / 4;

@@ -104,10 +104,10 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
             for (int c = lane; c < NC; c += WAVE) {
                 T *rt = rowt + (size_t)c * RS;
                 const double ri = rw[(size_t)c * NE + i], rri = rrw[(size_t)c * NE + i];
-                const T sw = T(1) - tkap * (T)rri;
+                const T sw = (T)(1.0 - kap * rri);
                 een_tables<NORDC, 2, T>(jp.c[a.iwctype[c]], sw, rt);
                 rt[3 * NA] = (T)ri;
-                rt[3 * NA + 1] = (T)rri;
+                rt[3 * NA + 1] = rri == asymp_r ? T(1) : T(0);  // electron i asymptotic for centre c
                 rt[3 * NA + 2] = -tkap * sw;           // dd7
                 rt[3 * NA + 3] = -tkap * (-tkap * sw);  // dd9
                 for (int k = 0; k < 3; k++) rt[3 * NA + 4 + k] = (T)(xw[k * NE + i] - a.cent[3 * c + k]);
@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
                     ipar = 1;
                 }
                 const T u = scale_r<T>(tkap, rij);
-                const T dd1 = T(1) - tkap * u, dd2 = -tkap * dd1;
+                const T dd1 = switched_from<T>(tkap, rij, u), dd2 = -tkap * dd1;
                 uu1[s] = dd1;
                 dd1e[s] = -tkap * dd1;
                 dd2e[s] = -tkap * dd2;
@@ -184,7 +184,8 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
                 }
                 for (int c = 0; c < NC; c++) {
                     const T *rt = rowt + (size_t)c * RS;
-                    const T ri = rt[3 * NA], rri = rt[3 * NA + 1], dd7 = rt[3 * NA + 2], dd9 = rt[3 * NA + 3];
+                    const T ri = rt[3 * NA], dd7 = rt[3 * NA + 2], dd9 = rt[3 * NA + 3];
+                    const bool iasym = rt[3 * NA + 1] != T(0);
                     const T rvi0 = rt[3 * NA + 4], rvi1 = rt[3 * NA + 5], rvi2 = rt[3 * NA + 6], iri = rt[3 * NA + 7];
                     const T cx = (T)a.cent[3 * c], cy = (T)a.cent[3 * c + 1], cz = (T)a.cent[3 * c + 2];
                     double rjc[HJB], rrjc[HJB];
@@ -200,9 +201,9 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
                     for (int s = 0; s < HJB; s++) {
                         if (j0 + s * WAVE >= i) break;
                         if (!on[s]) continue;
-                        const T rj = (T)rjc[s], rrj = (T)rrjc[s];
-                        if (rri == (T)asymp_r && rrj == (T)asymp_r) continue;
-                        const T swj = T(1) - tkap * rrj;
+                        const T rj = (T)rjc[s];
+                        if (iasym && rrjc[s] == asymp_r) continue;
+                        const T swj = (T)(1.0 - kap * rrjc[s]);
                         const T dd8 = -tkap * swj, dd10 = -tkap * (-tkap * swj);
                         const EenOut<T> e = een_eval_tab<NORDC, 2, T>(rt, uu1[s], swj);
                         const T d1 = dd1e[s];

@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                 r2 = r2 + rmo[0] * rmo[0]; r2 = r2 + rmo[1] * rmo[1]; r2 = r2 + rmo[2] * rmo[2];
                 const T rij = sqrt(r2);
                 const T u = scale_r<T>(tkap, rij);
-                const T dd1 = T(1) - tkap * u;
+                const T dd1 = switched_from<T>(tkap, rij, u);
                 uu1[s] = dd1;
                 fuf[s] = -tkap * dd1 / rij;
                 // e-e term (jastrow4e.f90: psi_ee with its u-derivative)
@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                         if (!on[s]) continue;
                         if (rrm == asymp_r && rrjc[s] == asymp_r) continue;
                         const T rjo = (T)rjc[s];
-                        const T swj = T(1) - tkap * (T)rrjc[s];
+                        const T swj = (T)(1.0 - kap * rrjc[s]);
                         const EenOut<T> e = een_eval_tab<NORDC, 1, T>(tab, uu1[s], swj);
                         fs[s] += e.f;
                         sfu[s] += e.fu;

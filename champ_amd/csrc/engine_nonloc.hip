@@ -151,8 +151,9 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
                     const T d0 = (T)(xw[j] - px), d1 = (T)(xw[NE + j] - py), d2 = (T)(xw[2 * NE + j] - pz);
                     T r2 = 0;
                     r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
-                    const T u = scale_r<T>(kap, sqrt(r2));
-                    U[(size_t)j * QB + qq] = T(1) - kap * u;
+                    const T rjq = sqrt(r2);
+                    const T u = scale_r<T>(kap, rjq);
+                    U[(size_t)j * QB + qq] = switched_from<T>(kap, rjq, u);
                     if (j != i) see += ee_term<T>(jp, a.NUP, i, j, u);
                 }
                 const double sees = strided_sum<QB>((double)see);
@@ -193,21 +194,21 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
                             asym[qq] = rr == asymp;
                             any_asym = any_asym || asym[qq];
                             if (jg == 0 && q0 + qq < NQ) acc[qq] += en_term<T>(jp, it, rr);
-                            if constexpr (NORDC > 1) een_tables<NORDC, 0, T>(jp.c[it], T(1) - kap * rr, A[qq]);
+                            if constexpr (NORDC > 1) een_tables<NORDC, 0, T>(jp.c[it], switched_from<T>(kap, rq, rr), A[qq]);
                         }
                         if constexpr (NORDC > 1) {
                             // one partner of lookahead on the scaled distance
                             double rr_next = jg < NE ? rrt[(size_t)jg * NC + c] : 0.0;
                             for (int j = jg; j < NE; j += JG) {
-                                const T rrj = (T)rr_next;
+                                const double rrj = rr_next;
                                 if (j + JG < NE) rr_next = rrt[(size_t)(j + JG) * NC + c];
                                 if (j == i) continue;
-                                const T rj = T(1) - kap * rrj;
+                                const T rj = (T)(1.0 - jp.kap * rrj);
                                 const T *uj = U + (size_t)j * QB;
                                 T P[QB];
 #pragma unroll
                                 for (int qq = 0; qq < QB; qq++) P[qq] = een_eval_tab<NORDC, 0, T>(A[qq], uj[qq], rj).f;
-                                if (any_asym && rrj == asymp) {  // psinl: both distances asymptotic -> 0
+                                if (any_asym && rrj == jp.asymp_r) {  // psinl: both distances asymptotic -> 0
 #pragma unroll
                                     for (int qq = 0; qq < QB; qq++)
                                         if (asym[qq]) P[qq] = T(0);
