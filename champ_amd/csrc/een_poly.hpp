@@ -54,11 +54,40 @@ struct EenTable {
 // (switch_scale2, scale_dist.f90:314-331).  In double the reference's form
 // 1 - kap R; in float that form cancels catastrophically far from a centre
 // (R -> 1/kap), so the float path evaluates exp(-kap r) directly.
+// kap is taken in double: rounding it to float would shift every exponent the
+// same way and bias sums over many pairs.
 template <typename T>
-__device__ __forceinline__ T switched_from(T kap, T r, T R)
+__device__ __forceinline__ T switched_from(double kap, T r, T R)
 {
-    if constexpr (sizeof(T) == sizeof(float)) return exp(-kap * r);
-    else return T(1) - kap * R;
+    if constexpr (sizeof(T) == sizeof(float)) return exp((float)(-kap * (double)r));
+    else return T(1) - (T)kap * R;
+}
+
+// ds = s - 1/kap = -x / kap for the float form of pade_minus_asym, with 1/kap
+// applied in double (see switched_from)
+template <typename T>
+__device__ __forceinline__ T asym_offset(double asymp_r, T x)
+{
+    return (T)(-(double)x * asymp_r);
+}
+
+// Pade + polynomial of a scaled distance s minus its asymptote (s -> U = 1/kap):
+//   scale0 p0 s/(1 + p1 s) + sum_{o>=2} p_o s^o  -  (same at s = U),
+// with ds = s - U passed in exactly (-exp(-kap r)/kap).  Written as
+// ds [scale0 p0/((1 + p1 s)(1 + p1 U)) + sum_o p_o g_o], g_o = s g_(o-1) + U^(o-1),
+// so nothing cancels when s is near the asymptote: the form the float path
+// uses for psibnl / psianl (nonlpsi.f90:88-133) minus asymp_jasb / asymp_jasa.
+template <typename T>
+__device__ __forceinline__ T pade_minus_asym(const double *p, int nord, T scale0, T s, T ds, T U)
+{
+    T v = scale0 * (T)p[0] / ((T(1) + (T)p[1] * s) * (T(1) + (T)p[1] * U));
+    T g = T(1), Up = T(1);
+    for (int o = 2; o <= nord; o++) {
+        Up = Up * U;
+        g = s * g + Up;
+        v = v + (T)p[o] * g;
+    }
+    return ds * v;
 }
 
 template <int N>

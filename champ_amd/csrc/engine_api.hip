@@ -566,6 +566,7 @@ extern "C" int64_t champ_gpu_array_size(champ_ctx *c, int32_t which)
     case CHAMP_ARR_DETIAB: case CHAMP_ARR_VPSP: return W * 2;
     case CHAMP_ARR_PELOC: case CHAMP_ARR_D2J: case CHAMP_ARR_NXQUAD: return W;
     case CHAMP_ARR_ORBN: return W * 4 * NO;
+    case CHAMP_ARR_FSO: return W * NE * NE;
     default: return -1;
     }
 }
@@ -598,6 +599,7 @@ extern "C" int champ_gpu_get_array(champ_ctx *c, int32_t which, double *out)
     case CHAMP_ARR_VPSP: rc = get(c->vpsp, n, h); if (!rc) std::memcpy(out, h.data(), n * 8); break;
     case CHAMP_ARR_PELOC: rc = get(c->pe_local, n, h); if (!rc) std::memcpy(out, h.data(), n * 8); break;
     case CHAMP_ARR_D2J: rc = get(c->d2o, n, h); if (!rc) std::memcpy(out, h.data(), n * 8); break;
+    case CHAMP_ARR_FSO: rc = get(c->fso, n, h); if (!rc) std::memcpy(out, h.data(), n * 8); break;
     case CHAMP_ARR_ORBN: rc = get(c->orbn, n, h); if (!rc) std::memcpy(out, h.data(), n * 8); break;
     case CHAMP_ARR_NXQUAD: {
         std::vector<int> hi(W);

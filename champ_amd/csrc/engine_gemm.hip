@@ -218,11 +218,13 @@ int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int
         HIPCHECK(hipGetLastError());
         return CHAMP_OK;
     }
-    // strips of 16 x 80 columns per wavefront, 64 rows per block
+    // strips of 16 x 80 columns per wavefront, 64 rows per block, when that
+    // still gives at least two blocks per CU (tall problems such as the
+    // orbitals of all electrons); otherwise one 16x16 tile per wavefront
     {
         constexpr int WN = 5, BN = 16 * WN;
         const long long gx = (M + 63) / 64, gy = (N + BN - 1) / BN;
-        if (gx <= 0x7fffffffLL && gy <= 65535 && batch <= 65535) {
+        if (gx * gy * batch >= 512 && gx <= 0x7fffffffLL && gy <= 65535 && batch <= 65535) {
             dim3 grid((unsigned)gx, (unsigned)gy, batch);
             if (bt)
                 hipLaunchKernelGGL((k_gemm_mfma_strip<true, WN>), grid, dim3(256), 0, c->stream, M, N, K, A, lda, sA, B, ldb,

@@ -143,11 +143,11 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
                     ipar = 1;
                 }
                 const T u = scale_r<T>(tkap, rij);
-                const T dd1 = switched_from<T>(tkap, rij, u), dd2 = -tkap * dd1;
+                const T dd1 = switched_from<T>(kap, rij, u), dd2 = -tkap * dd1;
                 uu1[s] = dd1;
                 dd1e[s] = -tkap * dd1;
                 dd2e[s] = -tkap * dd2;
-                if (on[s] && u != (T)asymp_r) {
+                if (on[s] && (sizeof(T) == sizeof(float) || u != (T)asymp_r)) {
                     const double *b = jp.b[isb];
                     T top = sspinn * (T)b[0] * u, topu = sspinn * (T)b[0], topuu = 0;
                     T bot = 1 + (T)b[1] * u, botu = (T)b[1], botuu = 0, bot2 = bot * bot;
@@ -165,6 +165,8 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
                     }
                     feeuu = feeuu * dd1 * dd1 + feeu * dd2;
                     feeu = feeu * dd1 / rij;
+                    if constexpr (sizeof(T) == sizeof(float))  // value minus asymptote without cancellation
+                        fee = pade_minus_asym<T>(b, jp.nordb, sspinn, u, asym_offset<T>(asymp_r, dd1), (T)asymp_r);
                     fs[s] += fee;
                     for (int k = 0; k < 3; k++) { fI[s][k] += feeu * rv[s][k]; fJ[s][k] -= feeu * rv[s][k]; }
                     d2[s] += 2 * (feeuu + 2 * feeu);

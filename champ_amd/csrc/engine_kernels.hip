@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                 r2 = r2 + rmo[0] * rmo[0]; r2 = r2 + rmo[1] * rmo[1]; r2 = r2 + rmo[2] * rmo[2];
                 const T rij = sqrt(r2);
                 const T u = scale_r<T>(tkap, rij);
-                const T dd1 = switched_from<T>(tkap, rij, u);
+                const T dd1 = switched_from<T>(kap, rij, u);
                 uu1[s] = dd1;
                 fuf[s] = -tkap * dd1 / rij;
                 // e-e term (jastrow4e.f90: psi_ee with its u-derivative)
@@ -256,7 +256,7 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                     else if (jp.nocuspb == 0) sspinn = T(0.5);
                     ipar = 1;
                 }
-                if (on[s] && u != tasymp) {
+                if (on[s] && (sizeof(T) == sizeof(float) || u != tasymp)) {
                     const double *b = jp.b[isb];
                     T top = sspinn * (T)b[0] * u, topu = sspinn * (T)b[0];
                     T bot = 1 + (T)b[1] * u, botu = (T)b[1], bot2 = bot * bot;
@@ -270,6 +270,8 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                         up = un;
                     }
                     feeu = feeu * dd1 / rij;
+                    if constexpr (sizeof(T) == sizeof(float))  // value minus asymptote without cancellation
+                        fee = pade_minus_asym<T>(b, jp.nordb, sspinn, u, asym_offset<T>(asymp_r, dd1), tasymp);
                     fs[s] += fee;
                     for (int k = 0; k < 3; k++) { gm[s][k] += feeu * rmo[k]; go[s][k] -= feeu * rmo[k]; }
                 }

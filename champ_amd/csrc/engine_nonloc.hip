@@ -25,10 +25,13 @@
 
 namespace champ {
 
-// psianl (nonlpsi.f90:88-105): e-n term at scaled distance rr
+// psianl (nonlpsi.f90:88-105): e-n term at scaled distance rr; x = 1 - kap rr
+// (exp(-kap r)) for the float form, which subtracts the asymptote exactly
 template <typename T>
-__device__ __forceinline__ T en_term(const JasParams &jp, int it, T rr)
+__device__ __forceinline__ T en_term(const JasParams &jp, int it, T rr, T x)
 {
+    if constexpr (sizeof(T) == sizeof(float))
+        return pade_minus_asym<T>(jp.a4[it], jp.norda, T(1), rr, asym_offset<T>(jp.asymp_r, x), (T)jp.asymp_r);
     if (rr == (T)jp.asymp_r) return T(0);
     T psi = (T)jp.a4[it][0] * rr / (T(1) + (T)jp.a4[it][1] * rr) - (T)jp.ja[it];
     T rrip = rr;
@@ -36,11 +39,12 @@ __device__ __forceinline__ T en_term(const JasParams &jp, int it, T rr)
     return psi;
 }
 
-// psibnl (nonlpsi.f90:107-133): e-e term of electrons (i, j) at scaled u
+// psibnl (nonlpsi.f90:107-133): e-e term of electrons (i, j) at scaled u;
+// x = 1 - kap u (exp(-kap r)) for the float form
 template <typename T>
-__device__ __forceinline__ T ee_term(const JasParams &jp, int nup, int i, int j, T u)
+__device__ __forceinline__ T ee_term(const JasParams &jp, int nup, int i, int j, T u, T x)
 {
-    if (u == (T)jp.asymp_r) return T(0);
+    if (sizeof(T) != sizeof(float) && u == (T)jp.asymp_r) return T(0);
     const int I = j > i ? j : i, J = j > i ? i : j;
     T sspinn = T(1);
     int isb = 0, ipar = 0;
@@ -50,6 +54,8 @@ __device__ __forceinline__ T ee_term(const JasParams &jp, int nup, int i, int j,
         ipar = 1;
     }
     const double *b = jp.b[isb];
+    if constexpr (sizeof(T) == sizeof(float))
+        return pade_minus_asym<T>(b, jp.nordb, sspinn, u, asym_offset<T>(jp.asymp_r, x), (T)jp.asymp_r);
     const T fee = (T)b[0] * u / (T(1) + (T)b[1] * u);
     T psi = sspinn * fee - (T)jp.jb[ipar];
     T up = u;
@@ -91,6 +97,7 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
     const int NE = a.NE, NC = a.NC, NQ = a.NQ, NB = a.NB, NRI = a.NRI;
     const JasParams &jp = *a.jas;
     const T kap = (T)jp.kap, asymp = (T)jp.asymp_r;
+    const double kapd = jp.kap, asympd = jp.asymp_r;
     double *S = lds;                        // [3][QB][NL_WAVES] per-wave sums: e-e, e-e-n + e-n, psi ratio
     double *XQ = S + 3 * QB * NL_WAVES;     // [QB][3] quadrature points
     double *FI = XQ + 3 * QB;               // [NL_WAVES] partial sums of F[i][.]
@@ -153,8 +160,9 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
                     r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
                     const T rjq = sqrt(r2);
                     const T u = scale_r<T>(kap, rjq);
-                    U[(size_t)j * QB + qq] = switched_from<T>(kap, rjq, u);
-                    if (j != i) see += ee_term<T>(jp, a.NUP, i, j, u);
+                    const T xs = switched_from<T>(kapd, rjq, u);
+                    U[(size_t)j * QB + qq] = xs;
+                    if (j != i) see += ee_term<T>(jp, a.NUP, i, j, u, xs);
                 }
                 const double sees = strided_sum<QB>((double)see);
                 if (lane < QB) S[(0 * QB + lane) * NL_WAVES + wv] = sees;
@@ -191,10 +199,11 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
                             r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
                             const T rq = c == ic ? (T)ren : sqrt(r2);
                             const T rr = scale_r<T>(kap, rq);
+                            const T xq = switched_from<T>(kapd, rq, rr);
                             asym[qq] = rr == asymp;
                             any_asym = any_asym || asym[qq];
-                            if (jg == 0 && q0 + qq < NQ) acc[qq] += en_term<T>(jp, it, rr);
-                            if constexpr (NORDC > 1) een_tables<NORDC, 0, T>(jp.c[it], switched_from<T>(kap, rq, rr), A[qq]);
+                            if (jg == 0 && q0 + qq < NQ) acc[qq] += en_term<T>(jp, it, rr, xq);
+                            if constexpr (NORDC > 1) een_tables<NORDC, 0, T>(jp.c[it], xq, A[qq]);
                         }
                         if constexpr (NORDC > 1) {
                             // one partner of lookahead on the scaled distance
@@ -203,12 +212,12 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
                                 const double rrj = rr_next;
                                 if (j + JG < NE) rr_next = rrt[(size_t)(j + JG) * NC + c];
                                 if (j == i) continue;
-                                const T rj = (T)(1.0 - jp.kap * rrj);
+                                const T rj = (T)(1.0 - kapd * rrj);
                                 const T *uj = U + (size_t)j * QB;
                                 T P[QB];
 #pragma unroll
                                 for (int qq = 0; qq < QB; qq++) P[qq] = een_eval_tab<NORDC, 0, T>(A[qq], uj[qq], rj).f;
-                                if (any_asym && rrj == jp.asymp_r) {  // psinl: both distances asymptotic -> 0
+                                if (any_asym && rrj == asympd) {  // psinl: both distances asymptotic -> 0
 #pragma unroll
                                     for (int qq = 0; qq < QB; qq++)
                                         if (asym[qq]) P[qq] = T(0);

@@ -22,7 +22,7 @@ LIBPATH = os.path.join(HERE, "libchamp_gpu.so")
 PREC_FP64 = 0
 PREC_MIXED = 1
 
-ARR = dict(slmi=1, vj=2, ddx=3, d2dx2=4, detiab=5, vpsp=6, peloc=7, d2j=8, orbn=9, vjn=10, nxquad=11)
+ARR = dict(slmi=1, vj=2, ddx=3, d2dx2=4, detiab=5, vpsp=6, peloc=7, d2j=8, orbn=9, vjn=10, nxquad=11, fso=12)
 
 EXPORTS = [
     "champ_gpu_init", "champ_gpu_free", "champ_gpu_last_error", "champ_gpu_set_walkers", "champ_gpu_get_walkers",
@@ -184,7 +184,8 @@ class Engine:
         self._check(self.L.champ_gpu_get_array(self.ptr, which, _p(out)), f"get_array({name})")
         W, NE, NM, NO = self.nwalk, self.nelec, self.nmax, self.norb
         shapes = dict(slmi=(W, 2, NM, NM), vj=(W, NE, 3), vjn=(W, NE, 3), ddx=(W, NE, 3), d2dx2=(W, NE),
-                      detiab=(W, 2), vpsp=(W, 2), peloc=(W,), d2j=(W,), orbn=(W, 4, NO), nxquad=(W,))
+                      detiab=(W, 2), vpsp=(W, 2), peloc=(W,), d2j=(W,), orbn=(W, 4, NO), nxquad=(W,),
+                      fso=(W, self.nelec, self.nelec))
         return out.reshape(shapes[name])
 
     def basis_fns(self, pos: np.ndarray) -> np.ndarray:

@@ -201,6 +201,7 @@ int champ_gpu_get_status(champ_ctx *ctx, int32_t *status);
 #define CHAMP_ARR_ORBN   9   /* [nwalk][4][norb]   orbn,dorbn of last psie     */
 #define CHAMP_ARR_VJN   10   /* [nwalk][nelec][3]  vjn of last psie            */
 #define CHAMP_ARR_NXQUAD 11  /* [nwalk]            quadrature points used      */
+#define CHAMP_ARR_FSO    12  /* [nwalk][nelec][nelec] Jastrow pair values fso (diagonal: e-n) */
 int64_t champ_gpu_array_size(champ_ctx *ctx, int32_t which);
 int champ_gpu_get_array(champ_ctx *ctx, int32_t which, double *out);
 
