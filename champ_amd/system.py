@@ -417,7 +417,8 @@ def _zigzag_chain(nheavy: int, nh: int, bond: float = 1.4, hbond: float = 2.0):
 
 def make_system(name: str, nheavy: int, nh: int, nelec: int, heavy_shells: str, h_shells: str,
                 zheavy: float = 4.0, nloc: int = 0, seed: int = 1234, norb: int | None = None,
-                kappa: float = 0.6, norda: int = 5, nordb: int = 5, nordc: int = 4) -> System:
+                kappa: float = 0.6, norda: int = 5, nordb: int = 5, nordc: int = 4,
+                ndet: int = 1, nvirt: int = 0, max_rank: int = 4) -> System:
     """A synthetic molecule in CHAMP's table form.
 
     heavy_shells / h_shells: shell recipe such as "ssssppp d" or "sssspppdd"
@@ -459,12 +460,15 @@ def make_system(name: str, nheavy: int, nh: int, nelec: int, heavy_shells: str, 
     nbasis = int(sum(nbastyp[t] for t in iwctype))
 
     nocc = max(nup, nelec - nup)
-    norb = norb or nocc
+    norb = norb or (nocc + nvirt)
     coef = rng.normal(0.0, 0.3, size=(norb, nbasis))
-    iworbd = np.zeros((1, nelec), dtype=np.int32)
-    iworbd[0, :nup] = np.arange(nup)
-    iworbd[0, nup:] = np.arange(nelec - nup)
-    cdet = np.ones(1)
+    if ndet > 1:
+        iworbd, cdet = excited_determinants(nelec, nup, norb, ndet, max_rank, seed + 7)
+    else:
+        iworbd = np.zeros((1, nelec), dtype=np.int32)
+        iworbd[0, :nup] = np.arange(nup)
+        iworbd[0, nup:] = np.arange(nelec - nup)
+        cdet = np.ones(1)
 
     a4 = rng.uniform(-0.1, 0.1, size=(2, norda + 1))
     b = rng.uniform(-0.1, 0.1, size=(1, nordb + 1))
@@ -503,6 +507,47 @@ def make_system(name: str, nheavy: int, nh: int, nelec: int, heavy_shells: str, 
                   ecp_exponent=ecp_exponent, xq0=xq0, yq0=yq0, zq0=zq0, wq=wq, shells=shells)
 
 
+def excited_determinants(nelec: int, nup: int, norb: int, ndet: int, max_rank: int, seed: int):
+    """Determinant list for a multideterminant trial function: det 0 is the
+    reference (orbitals 0..n-1 of each spin, CHAMP's kref = 1); dets 1.. are
+    distinct random 1..max_rank-fold excitations (holes among the reference
+    orbitals, particles among the others), the rank split at random between
+    the spins, each spin's orbitals listed in a random slot order (so that the
+    phase of the reordering matters), CI coefficients ~N(0,1) * 0.5**rank."""
+    rng = np.random.default_rng(seed)
+    ndn = nelec - nup
+    iworbd = np.zeros((ndet, nelec), dtype=np.int32)
+    iworbd[0, :nup] = np.arange(nup)
+    iworbd[0, nup:] = np.arange(ndn)
+    cdet = np.zeros(ndet)
+    cdet[0] = 1.0
+    seen = {(tuple(range(nup)), tuple(range(ndn)))}
+    k = 1
+    while k < ndet:
+        rank = int(rng.integers(1, max_rank + 1))
+        rup = int(rng.integers(0, rank + 1))
+        rup, rdn = min(rup, nup, norb - nup), min(rank - rup, ndn, norb - ndn)
+        if rup + rdn == 0:
+            continue
+        spins = []
+        for n, r in ((nup, rup), (ndn, rdn)):
+            occ = list(range(n))
+            holes = rng.choice(n, size=r, replace=False) if r else []
+            parts = rng.choice(np.arange(n, norb), size=r, replace=False) if r else []
+            for h, q in zip(holes, parts):
+                occ[int(h)] = int(q)
+            spins.append(occ)
+        key = (tuple(sorted(spins[0])), tuple(sorted(spins[1])))
+        if key in seen:
+            continue
+        seen.add(key)
+        iworbd[k, :nup] = rng.permutation(spins[0]) if rng.random() < 0.5 else spins[0]
+        iworbd[k, nup:] = rng.permutation(spins[1]) if rng.random() < 0.5 else spins[1]
+        cdet[k] = rng.normal() * 0.5 ** (rup + rdn)
+        k += 1
+    return iworbd, cdet
+
+
 def config(name: str, seed: int = 1234) -> System:
     """Named workloads.
 
@@ -510,7 +555,20 @@ def config(name: str, seed: int = 1234) -> System:
     c160 : 20 heavy (Z_eff=4, s/p non-local ECP) + 20 H, 160 e, Nao = 600
            (heavy 4s3p2d, H 2s1p); 12-point icosahedral quadrature
     small: 2 heavy + 2 H, 10 e, ECP - fast checks
+    small_md / small_ae_md: the small systems with 40 determinants (1-4-fold
+           excitations into 6 virtual orbitals)
+    c22_md: c22 with 60 determinants into 8 virtuals
+    md256: 64 heavy (Z_eff=4, ECP), 256 e, Nao = 960 (sssppd), 5000
+           determinants from 1-4-fold excitations within 40 virtuals
     """
+    if name == "small_md":
+        return make_system("small_md", 2, 2, 10, "ssppd", "sp", nloc=4, seed=seed, ndet=40, nvirt=6)
+    if name == "small_ae_md":
+        return make_system("small_ae_md", 2, 2, 10, "ssppd", "sp", nloc=0, seed=seed, ndet=40, nvirt=6)
+    if name == "c22_md":
+        return make_system("c22_md", 4, 6, 22, "sssspppd", "sp", nloc=0, seed=seed, ndet=60, nvirt=8)
+    if name == "md256":
+        return make_system("md256", 64, 0, 256, "sssppd", "sp", nloc=4, seed=seed, ndet=5000, nvirt=40)
     if name == "c22":
         return make_system("c22", 4, 6, 22, "sssspppd", "sp", nloc=0, seed=seed)
     if name == "c160":

@@ -36,6 +36,13 @@
  *   hpsi E_L                      -- kinetic part vs finite-difference Laplacian
  *                                    of ln psi built from the oracle's own psi;
  *                                    drift vs finite-difference gradient.
+ *   multideterminant psi_D        -- every determinant built and inverted on its
+ *                                    own (no excitation tables); drift, E_L and
+ *                                    the ECP term vs finite differences and full
+ *                                    psi at the quadrature points; per-move psi
+ *                                    and drift vs a fresh evaluation; a reordered
+ *                                    copy of the reference determinant gives
+ *                                    (c0 +- c1) D_ref by the parity of the order.
  * The end-to-end per-walker values of the Fortran reference itself are not
  * available (the Fortran build needs an MPI module this image lacks), so the
  * composition of the pinned pieces is what fixes hpsi/psie.
@@ -70,6 +77,11 @@ struct oracle_walker {
     double psido, psijo, psi2o, eold;
     double ratio_kref;
     uint64_t rng[4];
+    /* multideterminant state (ndet > 1): every determinant's own LU inverse
+     * and value per spin, the proposal's, and the orbital-space coefficients
+     * vmd[i][o] = d ln psi_D / d phi_o(r_i) */
+    int ndet;
+    double *md_inv, *md_det, *md_invn, *md_detn, *vmd, md_psidn;
 };
 
 /* ------------------------------------------------------------------------ */
@@ -499,6 +511,12 @@ oracle_walker_t *orc_walker_new(const champ_system_t *s)
     w->phin = xcalloc(nb, 8); w->dphin = xcalloc(3 * nb, 8); w->d2phin = xcalloc(nb, 8);
     w->vps = xcalloc((size_t)ne * nc * CHAMP_MPS_L, 8);
     w->asymp_jasa = xcalloc(s->nctype, 8);
+    w->ndet = s->ndet;
+    if (s->ndet > 1) {
+        w->md_inv = xcalloc((size_t)s->ndet * 2 * nm * nm, 8); w->md_det = xcalloc((size_t)s->ndet * 2, 8);
+        w->md_invn = xcalloc((size_t)s->ndet * nm * nm, 8); w->md_detn = xcalloc(s->ndet, 8);
+        w->vmd = xcalloc((size_t)ne * no, 8);
+    }
     orc_asymp(s, w->asymp_jasa, w->asymp_jasb);
     w->pecent = orc_pot_nn(s);
     for (int iq = 0; iq < s->nquad; iq++) { w->xq[iq] = s->xq0[iq]; w->yq[iq] = s->yq0[iq]; w->zq[iq] = s->zq0[iq]; }
@@ -512,7 +530,7 @@ void orc_walker_free(oracle_walker_t *w)
                     w->r_ee_sav, w->rvec_ee_sav, w->fjo, w->fjn, w->fso, w->fsn, w->fijo, w->fijn,
                     w->d2ijo, w->d2ijn, w->vj, w->vjn, w->orb, w->dorb, w->ddorb, w->orbn, w->dorbn,
                     w->ddorbn, w->slmi, w->slmin, w->fp, w->fpp, w->ddx, w->d2dx2, w->phin, w->dphin,
-                    w->d2phin, w->vps, w->asymp_jasa};
+                    w->d2phin, w->vps, w->asymp_jasa, w->md_inv, w->md_det, w->md_invn, w->md_detn, w->vmd};
     for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
     free(w);
 }
@@ -951,13 +969,21 @@ static void determinante(const champ_system_t *s, oracle_walker_t *w, int iel, i
     for (int j = 0; j < nel; j++) w->slmin[j + ikel] = slmi[j + ikel] / ratio;
 }
 
-/* compute_determinante_grad (single det), determinante.f90:69-283, 285-328 */
+static void md_grad(const champ_system_t *s, const oracle_walker_t *w, int iel, int iflag_move, double g[3]);
+
+/* compute_determinante_grad, determinante.f90:69-283, 285-328 */
 void orc_determinante_grad(const champ_system_t *s, oracle_walker_t *w, int iel, int iflag_move, double vd[3])
 {
     const int nup = w->nup, no = s->norb;
     const int iab = iel < nup ? 0 : 1, nel = iab ? w->ndn : nup, ish = iab ? nup : 0;
     const size_t ik = (size_t)(iel - ish) * nel;
     double ref[3] = {0, 0, 0};
+    if (s->ndet > 1) {
+        md_grad(s, w, iel, iflag_move, ref);
+        const double *vj = iflag_move == 1 ? w->vj : w->vjn;
+        for (int k = 0; k < 3; k++) vd[k] = 0.0 + ref[k] + vj[3 * iel + k];
+        return;
+    }
     if (iflag_move == 1) {
         const double *slmi = w->slmi + (size_t)iab * w->nmax * w->nmax;
         for (int j = 0; j < nel; j++) {
@@ -985,6 +1011,112 @@ static void detsav(const champ_system_t *s, oracle_walker_t *w, int iel)
     for (int iorb = 0; iorb < no; iorb++) {
         w->orb[(size_t)iel * no + iorb] = w->orbn[iorb];
         for (int k = 0; k < 3; k++) w->dorb[((size_t)iel * 3 + k) * no + iorb] = w->dorbn[k * no + iorb];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multideterminant psi_D = sum_k cdet_k D_up,k D_dn,k (determinant_psit.f90:
+ * 3-26) by brute force: every determinant k is built in its own orbital order
+ * iworbd(:,k) and inverted on its own, so neither the excitation tables nor
+ * their phase convention (set_input_data.f90:46-414) enter.  The reference
+ * reaches the same psi, drift and local energy through aa = S^-1 Phi and the
+ * small excitation determinants (multideterminant.f90:3-255,
+ * multideterminante.f90:3-222).  Row i of a determinant is linear in
+ * phi(r_i), so every one-electron quantity is d ln psi_D / d phi_o(r_i):
+ *   vmd[i][o] = sum_k wt_k sum_{j: iworbd(j,k) = o} inv_k[j][i],
+ *   wt_k = cdet_k D_up,k D_dn,k / psi_D,
+ * and grad_i psi_D / psi_D = sum_o vmd[i][o] grad phi_o(r_i), likewise the
+ * Laplacian and the ECP quadrature ratio sum_o vmd[i][o] phi_o(x_q).        */
+static double md_build_spin(const champ_system_t *s, const oracle_walker_t *w, int k, int iab, int iel,
+                            const double *orbrow, double *inv)
+{
+    const int no = s->norb, nup = w->nup, nel = iab ? w->ndn : nup, ish = iab ? nup : 0;
+    const int32_t *iw = s->iworbd + (size_t)k * w->nelec + ish;
+    for (int j = 0; j < nel; j++)
+        for (int i = 0; i < nel; i++)
+            inv[(size_t)j * nel + i] = (i + ish == iel) ? orbrow[iw[j]] : w->orb[(size_t)(ish + i) * no + iw[j]];
+    double det = 1.0;
+    if (nel > 0 && orc_matinv(inv, nel, &det) != 0) det = 0.0;
+    return det;
+}
+
+static double md_weights_sum(const champ_system_t *s, const double *dup, int sup, const double *ddn, int sdn)
+{
+    double psid = 0.0;
+    for (int k = 0; k < s->ndet; k++) psid = psid + dup[(size_t)k * sup] * ddn[(size_t)k * sdn] * s->cdet[k];
+    return psid;
+}
+
+/* all determinants at the current configuration: psi_D, ddx, d2dx2, vmd */
+static double md_hpsi(const champ_system_t *s, oracle_walker_t *w)
+{
+    const int ne = w->nelec, no = s->norb, nup = w->nup, nm = w->nmax;
+    for (int k = 0; k < s->ndet; k++)
+        for (int iab = 0; iab < 2; iab++)
+            w->md_det[2 * k + iab] = md_build_spin(s, w, k, iab, -1, NULL, w->md_inv + ((size_t)k * 2 + iab) * nm * nm);
+    const double psid = md_weights_sum(s, w->md_det, 2, w->md_det + 1, 2);
+    memset(w->vmd, 0, sizeof(double) * ne * no);
+    for (int k = 0; k < s->ndet; k++) {
+        const double wt = s->cdet[k] * w->md_det[2 * k] * w->md_det[2 * k + 1] / psid;
+        for (int iab = 0; iab < 2; iab++) {
+            const int nel = iab ? w->ndn : nup, ish = iab ? nup : 0;
+            const double *inv = w->md_inv + ((size_t)k * 2 + iab) * nm * nm;
+            const int32_t *iw = s->iworbd + (size_t)k * ne + ish;
+            for (int i = 0; i < nel; i++)
+                for (int j = 0; j < nel; j++) w->vmd[(size_t)(ish + i) * no + iw[j]] += wt * inv[(size_t)i * nel + j];
+        }
+    }
+    for (int i = 0; i < ne; i++) {
+        double d[3] = {0, 0, 0}, dd = 0;
+        for (int o = 0; o < no; o++) {
+            const double v = w->vmd[(size_t)i * no + o];
+            for (int k = 0; k < 3; k++) d[k] = d[k] + v * w->dorb[((size_t)i * 3 + k) * no + o];
+            dd = dd + v * w->ddorb[(size_t)i * no + o];
+        }
+        for (int k = 0; k < 3; k++) w->ddx[3 * i + k] = d[k];
+        w->d2dx2[i] = dd;
+    }
+    return psid;
+}
+
+/* electron iel moved (orbitals orbn/dorbn at the new position): every
+ * determinant of its spin rebuilt; returns the new psi_D */
+static double md_psie(const champ_system_t *s, oracle_walker_t *w, int iel)
+{
+    const int nm = w->nmax, iab = iel < w->nup ? 0 : 1;
+    for (int k = 0; k < s->ndet; k++)
+        w->md_detn[k] = md_build_spin(s, w, k, iab, iel, w->orbn, w->md_invn + (size_t)k * nm * nm);
+    w->md_psidn = iab == 0 ? md_weights_sum(s, w->md_detn, 1, w->md_det + 1, 2)
+                           : md_weights_sum(s, w->md_det, 2, w->md_detn, 1);
+    return w->md_psidn;
+}
+
+/* grad ln psi_D of electron iel: current (iflag_move=1) or proposed (0) */
+static void md_grad(const champ_system_t *s, const oracle_walker_t *w, int iel, int iflag_move, double g[3])
+{
+    const int ne = w->nelec, no = s->norb, nm = w->nmax, nup = w->nup;
+    const int iab = iel < nup ? 0 : 1, nel = iab ? w->ndn : nup, ish = iab ? nup : 0, row = iel - ish;
+    const double psid = iflag_move ? md_weights_sum(s, w->md_det, 2, w->md_det + 1, 2) : w->md_psidn;
+    g[0] = g[1] = g[2] = 0.0;
+    for (int k = 0; k < s->ndet; k++) {
+        const double dme = iflag_move ? w->md_det[2 * k + iab] : w->md_detn[k];
+        const double wt = s->cdet[k] * dme * w->md_det[2 * k + 1 - iab] / psid;
+        const double *inv = iflag_move ? w->md_inv + ((size_t)k * 2 + iab) * nm * nm : w->md_invn + (size_t)k * nm * nm;
+        const int32_t *iw = s->iworbd + (size_t)k * ne + ish;
+        for (int j = 0; j < nel; j++) {
+            const double v = wt * inv[(size_t)row * nel + j];
+            for (int c = 0; c < 3; c++)
+                g[c] = g[c] + v * (iflag_move ? w->dorb[((size_t)iel * 3 + c) * no + iw[j]] : w->dorbn[c * no + iw[j]]);
+        }
+    }
+}
+
+static void md_accept(const champ_system_t *s, oracle_walker_t *w, int iel)
+{
+    const int nm = w->nmax, iab = iel < w->nup ? 0 : 1;
+    for (int k = 0; k < s->ndet; k++) {
+        memcpy(w->md_inv + ((size_t)k * 2 + iab) * nm * nm, w->md_invn + (size_t)k * nm * nm, sizeof(double) * nm * nm);
+        w->md_det[2 * k + iab] = w->md_detn[k];
     }
 }
 
@@ -1127,7 +1259,10 @@ static void nonloc_pot(const champ_system_t *s, oracle_walker_t *w, double *pe)
                 }
                 const size_t ikel = (size_t)nel * (i - ish);
                 double det_ratio = 0.0;
-                for (int j = 0; j < nel; j++) det_ratio = det_ratio + slmi[j + ikel] * orbq[s->iworbd[j + ish]];
+                if (s->ndet > 1)
+                    for (int o = 0; o < no; o++) det_ratio = det_ratio + w->vmd[(size_t)i * no + o] * orbq[o];
+                else
+                    for (int j = 0; j < nel; j++) det_ratio = det_ratio + slmi[j + ikel] * orbq[s->iworbd[j + ish]];
                 /* nonlocj_quad4 */
                 for (int jc = 0; jc < nc; jc++) rrq[jc] = (1.0 - exp(-kap * rq[jc])) / kap;
                 double fsumn = 0;
@@ -1179,7 +1314,7 @@ static double pot_local(const champ_system_t *s, const oracle_walker_t *w)
     return pe;
 }
 
-/* hpsi, hpsi.f90:3-273 (single determinant, one state) */
+/* hpsi, hpsi.f90:3-273 (one state) */
 void orc_hpsi(const champ_system_t *s, oracle_walker_t *w, double *psid, double *psij, double *ekin, double *energy)
 {
     const int ne = w->nelec, nup = w->nup;
@@ -1189,6 +1324,7 @@ void orc_hpsi(const champ_system_t *s, oracle_walker_t *w, double *psid, double 
     for (int k = 0; k < 3 * ne; k++) w->vj[k] = w->fjo[k];
     const double d2j = w->d2o, pj = w->fsumo;
     int bad = determinant(s, w);
+    const double psid_md = s->ndet > 1 ? md_hpsi(s, w) : 0.0;
     if (s->nloc > 0) nonloc_pot(s, w, &pe_local);
     else { w->vpsp_det[0] = w->vpsp_det[1] = 0.0; w->nxquad = 0; }
     double ekin_det[2], eloc_det[2];
@@ -1208,7 +1344,7 @@ void orc_hpsi(const champ_system_t *s, oracle_walker_t *w, double *psid, double 
         ekin_other = ekin_other + (-HB * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
     }
     double e_other = ekin_other + pe_local;
-    double pd = 0.0 + w->detiab[0] * w->detiab[1] * s->cdet[0];
+    double pd = s->ndet > 1 ? psid_md : 0.0 + w->detiab[0] * w->detiab[1] * s->cdet[0];
     double en = 0.0 + eloc_det[0] + eloc_det[1] + e_other;
     double ek = 0.0 + ekin_det[0] + ekin_det[1] + ekin_other;
     w->psid = pd; w->psij = pj; w->ekin = ek; w->energy = en; w->pe_local = pe_local; w->d2j = d2j;
@@ -1236,6 +1372,7 @@ void orc_psie(const champ_system_t *s, oracle_walker_t *w, int iel, const double
     for (int k = 0; k < 3 * ne; k++) w->vjn[k] = w->fjn[k];
     *psijn = w->fsumn;
     determinante(s, w, iel, iflag);
+    if (s->ndet > 1) { *psidn = md_psie(s, w, iel); return; }
     if (w->detn == 0.0) { *psidn = 0.0; return; }
     const int iab = iel < w->nup ? 0 : 1;
     *psidn = 0.0 + w->detn * w->detiab[1 - iab] * s->cdet[0];
@@ -1245,6 +1382,7 @@ void orc_accept(const champ_system_t *s, oracle_walker_t *w, int iel)
 {
     for (int k = 0; k < 3; k++) w->x[3 * iel + k] = w->xnew_e[k];
     jassav(w, iel, 0);
+    if (s->ndet > 1) md_accept(s, w, iel);
     detsav(s, w, iel);
 }
 void orc_reject(const champ_system_t *s, oracle_walker_t *w, int iel) { (void)s; distancese_restore(w, iel); }

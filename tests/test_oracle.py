@@ -346,6 +346,48 @@ def test_sherman_morrison_equals_fresh_lu():
         assert np.max(np.abs(w.slmi() - w2.slmi())) < 1e-10 * np.max(np.abs(w2.slmi()))
 
 
+def test_multidet_moves_equal_fresh_evaluation():
+    """Per-move multideterminant psi and drift (every determinant rebuilt with
+    the moved row) against a fresh hpsi at the moved configuration."""
+    sy = S.config("c22_md")
+    x = sy.initial_walkers(1, seed=8)[0]
+    w = ffi.walker(sy); w.set_x(x); w.hpsi()
+    rng = np.random.default_rng(3)
+    for iel in (0, 5, 11, 21):
+        xn = x[iel] + rng.normal(0, 0.3, 3)
+        pdn, pjn = w.psie(iel, xn)
+        vn = w.grad(iel, 0)
+        w.accept(iel)
+        x[iel] = xn
+        w2 = ffi.walker(sy); w2.set_x(x); h2 = w2.hpsi()
+        assert abs(pdn / h2["psid"] - 1) < 1e-11
+        assert np.max(np.abs(vn - w2.grad(iel, 1))) < 1e-10 * max(1.0, np.max(np.abs(vn)))
+        assert np.max(np.abs(w.grad(iel, 1) - vn)) < 1e-12 * max(1.0, np.max(np.abs(vn)))
+
+
+def test_multidet_phase_of_reordered_reference():
+    """Two determinants with the reference's orbitals, the second with two
+    slots of each spin exchanged: psi_D = (c0 + c1) D_ref (two exchanges) and
+    every derivative of ln psi_D equals the single determinant's."""
+    sy1 = S.config("small")
+    sy = S.config("small")
+    iw = np.repeat(sy.iworbd, 2, axis=0)
+    iw[1, [0, 1]] = iw[1, [1, 0]]
+    iw[1, [5, 7]] = iw[1, [7, 5]]
+    sy.iworbd = iw
+    sy.cdet = np.array([0.7, 0.2])
+    x = sy.initial_walkers(1, seed=4)[0]
+    a = ffi.walker(sy1); a.set_x(x); ha = a.hpsi()
+    b = ffi.walker(sy); b.set_x(x); hb = b.hpsi()
+    assert abs(hb["psid"] / (0.9 * ha["psid"]) - 1) < 1e-12
+    assert abs(hb["energy"] - ha["energy"]) < 1e-10 * abs(ha["energy"])
+    assert np.max(np.abs(b.ddx()[0] - a.ddx()[0])) < 1e-10 * np.max(np.abs(a.ddx()[0]))
+    iw[1, [0, 1]] = iw[1, [1, 0]]   # one exchange left: odd permutation
+    sy.iworbd = iw
+    c = ffi.walker(sy); c.set_x(x); hc = c.hpsi()
+    assert abs(hc["psid"] / (0.5 * ha["psid"]) - 1) < 1e-12
+
+
 # ------------------------------------------------------ drift and E_L
 def _lnpsi(w, xx):
     w.set_x(xx)
@@ -353,7 +395,7 @@ def _lnpsi(w, xx):
     return math.log(abs(h["psid"])) + h["psij"], h
 
 
-@pytest.mark.parametrize("name", ["small_ae", "small"])
+@pytest.mark.parametrize("name", ["small_ae", "small", "small_ae_md", "small_md"])
 def test_drift_and_kinetic_vs_finite_differences(name):
     sy = S.config(name)
     x = sy.initial_walkers(1, seed=12)[0]
@@ -376,9 +418,10 @@ def test_drift_and_kinetic_vs_finite_differences(name):
     assert abs(ekin_fd - h0["ekin"]) < 1e-4 * max(1, abs(h0["ekin"]))
 
 
-def test_nonlocal_energy_by_brute_force():
+@pytest.mark.parametrize("name", ["small", "small_md"])
+def test_nonlocal_energy_by_brute_force(name):
     """vpsp_det re-evaluated from full psi values at the quadrature points."""
-    sy = S.config("small")
+    sy = S.config(name)
     x = sy.initial_walkers(1, seed=21)[0]
     w = ffi.walker(sy)
     f0, h0 = _lnpsi(w, x)
