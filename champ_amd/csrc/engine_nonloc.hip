@@ -105,16 +105,18 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
     double *RAD = GEO + (size_t)QB * NC * 5;  // [QB][NRI] radial functions at the points
     T *U = (T *)(RAD + (size_t)QB * NRI);   // [NE][QB] switched scaled e-e distances u(q, j)
 
-    // XCD-aware contiguous ranges: workgroups are dealt round-robin to the 8
-    // XCDs, so logical block lb = (b % 8) * (G / 8) + b / 8 gives each XCD one
-    // contiguous eighth of the pair list (the same walkers share its L2).
+    // XCD-aware assignment: workgroups are dealt round-robin to the 8 XCDs,
+    // so the blocks b = x, x + 8, ... run on XCD x, which takes one contiguous
+    // eighth of the pair list; inside it the blocks stride by their count, so
+    // the blocks resident at one time work on neighbouring pairs (the same
+    // one or two walkers, whose tables stay in that XCD's L2).
     const int G = gridDim.x, b = blockIdx.x;
-    const int lb = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+    const int nx = (G % 8 == 0) ? 8 : 1, gx = G / nx, xcd = b % nx, j0 = b / nx;
     const int total = a.pair_total[0];
-    const int per = (total + G - 1) / G;
-    const int t_lo = lb * per, t_hi = min(total, t_lo + per);
+    const int per_x = (total + nx - 1) / nx;
+    const int x_lo = xcd * per_x, x_hi = min(total, x_lo + per_x);
 
-    for (int t = t_lo; t < t_hi; t++) {
+    for (int t = x_lo + j0; t < x_hi; t += gx) {
         int lo = 0, hi = a.W - 1;
         while (lo < hi) {  // walker: largest w with off[w] <= t
             const int mid = (lo + hi + 1) / 2;
@@ -306,10 +308,13 @@ static int launch_k(champ_ctx *c, const HArgs &a)
     }
     if (lds > 64 * 1024)
         HIPCHECK(hipFuncSetAttribute((const void *)k_nonloc<NORDC, QB, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int dev = 0, ncu = 256;
+    // one wave of blocks: as many as can be resident at once
+    int dev = 0, ncu = 256, per_cu = 1;
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    hipLaunchKernelGGL((k_nonloc<NORDC, QB, T>), dim3(ncu * 8), dim3(NL_THREADS), lds, c->stream, a);
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonloc<NORDC, QB, T>, NL_THREADS, lds));
+    per_cu = std::max(1, per_cu);
+    hipLaunchKernelGGL((k_nonloc<NORDC, QB, T>), dim3(ncu * per_cu), dim3(NL_THREADS), lds, c->stream, a);
     HIPCHECK(hipGetLastError());
     return CHAMP_OK;
 }

@@ -160,6 +160,8 @@ hipError_t hipDeviceSynchronize();
 hipError_t hipGetDevice(int *d);
 hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int dev);
 hipError_t hipFuncSetAttribute(const void *f, hipFuncAttribute a, int v);
+template <typename F>
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, F, int, size_t) { *n = 4; return hipSuccess; }
 hipError_t hipGetLastError();
 const char *hipGetErrorString(hipError_t e);
 hipError_t hipEventCreate(hipEvent_t *e);
