@@ -121,10 +121,10 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
     if (!ctxp || !s || nwalk <= 0) { set_error("init: bad arguments"); return CHAMP_ERR_ARG; }
     *ctxp = nullptr;
     if (precision != CHAMP_PREC_FP64 && precision != CHAMP_PREC_MIXED) { set_error("init: unknown precision"); return CHAMP_ERR_ARG; }
-    if (s->nctype > CHAMP_MAXCT || s->ncent > 128 || s->nelec > 1024 || s->nelec < 2 || s->ndet != 1 ||
+    if (s->nctype > CHAMP_MAXCT || s->ncent > 128 || s->nelec > 1024 || s->nelec < 2 || s->ndet < 1 ||
         s->norda >= CHAMP_NORDP || s->nordb >= CHAMP_NORDP || s->nordc >= CHAMP_NORDP || s->nspin2b < 1 ||
         s->nspin2b > 2 || (s->nloc != 0 && s->nloc != 4) || s->nquad > CHAMP_MPS_QUAD) {
-        set_error("init: system outside the supported envelope (nctype<=4, ncent<=128, 2<=nelec<=1024, ndet=1, orders<8, nloc 0|4)");
+        set_error("init: system outside the supported envelope (nctype<=4, ncent<=128, 2<=nelec<=1024, ndet>=1, orders<8, nloc 0|4)");
         return CHAMP_ERR_UNSUPPORTED;
     }
     for (int e = 0; e < s->nelec; e++)
@@ -344,6 +344,8 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
         const size_t lds = (size_t)(NM * (NM + 1) + NM) * sizeof(double);
         if (lds > 160 * 1024) { set_error("init: Slater matrix does not fit in LDS"); champ_gpu_free(c); *ctxp = nullptr; return CHAMP_ERR_UNSUPPORTED; }
     }
+    // excitation tables and state of a multideterminant trial function
+    TRY(md_init(c, s));
     return CHAMP_OK;
 }
 
@@ -362,6 +364,7 @@ extern "C" int champ_gpu_free(champ_ctx *c)
                     c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->rrT, c->nxquad, c->ri_cent, c->ri_type,
                     c->ri_rad, c->ri_rmax, c->ao_ri, c->gref};
     for (void *p : ptrs) if (p) hipFree(p);
+    md_free(c);
     for (auto &t : c->ev_pool) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -462,7 +465,9 @@ extern "C" int champ_gpu_compute_determinante_grad(champ_ctx *c, int32_t iel, in
 {
     if (!c || !vd || iel < 0 || iel >= c->NE) return CHAMP_ERR_ARG;
     if (iflag_move == 1) {
-        int rc = launch_drift_old(c, iel);
+        int rc = launch_md_pre(c, iel);
+        if (rc) return rc;
+        rc = launch_drift_old(c, iel);
         if (rc) return rc;
         HIPCHECK(hipMemcpyAsync(vd, c->vold, (size_t)3 * c->W * 8, hipMemcpyDeviceToHost, c->stream));
     } else if (iflag_move == 0) {
@@ -474,6 +479,26 @@ extern "C" int champ_gpu_compute_determinante_grad(champ_ctx *c, int32_t iel, in
     return CHAMP_OK;
 }
 
+// multideterminant steps of a move, timed as one kernel class
+static int launch_md_pre_timed(champ_ctx *c, int iel)
+{
+    if (!c->md) return CHAMP_OK;
+    int slot;
+    tbegin(c, K_MD, &slot);
+    const int rc = launch_md_pre(c, iel);
+    tend(c, slot);
+    return rc;
+}
+static int launch_md_post_timed(champ_ctx *c, int iel)
+{
+    if (!c->md) return CHAMP_OK;
+    int slot;
+    tbegin(c, K_MD, &slot);
+    const int rc = launch_md_post(c, iel);
+    tend(c, slot);
+    return rc;
+}
+
 extern "C" int champ_gpu_psie(champ_ctx *c, int32_t iel, const double *xnew, double *psidn, double *psijn, int32_t iflag)
 {
     if (!c || !xnew || iel < 0 || iel >= c->NE) return CHAMP_ERR_ARG;
@@ -482,6 +507,7 @@ extern "C" int champ_gpu_psie(champ_ctx *c, int32_t iel, const double *xnew, dou
     int rc;
     if ((rc = launch_move_a(c, iel, 1))) return rc;
     if ((rc = launch_move_gemm(c))) return rc;
+    if ((rc = launch_md_post_timed(c, iel))) return rc;
     if ((rc = launch_move_b(c, iel, 1))) return rc;
     if ((rc = copy_out(c, psidn, c->psidn, c->W))) return rc;
     if ((rc = copy_out(c, psijn, c->psijn, c->W))) return rc;
@@ -508,8 +534,10 @@ extern "C" int champ_gpu_metrop1(champ_ctx *c, int32_t nsweep, double tau, champ
     for (int isw = 0; isw < nsweep; isw++) {
         if (c->nloc > 0 && (rc = launch_rotqua(c))) return rc;
         for (int iel = 0; iel < c->NE; iel++) {
+            if ((rc = launch_md_pre_timed(c, iel))) return rc;
             if ((rc = launch_move_a(c, iel, 0))) return rc;
             if ((rc = launch_move_gemm(c))) return rc;
+            if ((rc = launch_md_post_timed(c, iel))) return rc;
             if ((rc = launch_move_b(c, iel, 0))) return rc;
             if (c->lu_every > 0 && (iel + 1) % c->lu_every == 0 && iel + 1 < c->NE && (rc = launch_slater_refresh(c)))
                 return rc;
@@ -634,7 +662,7 @@ extern "C" int champ_gpu_basis_fns(champ_ctx *c, int32_t npt, const double *pos,
 
 static const char *kclass_name[K_NCLASS] = {"move_jastrow_ao", "move_mo_gemm", "move_ratio_commit", "hpsi_jastrow",
                                             "hpsi_ao", "hpsi_mo_gemm", "hpsi_slater_lu", "hpsi_nonloc",
-                                            "hpsi_eloc", "other"};
+                                            "hpsi_eloc", "multidet", "other"};
 
 extern "C" int champ_gpu_kernel_times(champ_ctx *c, int32_t *n, char *names, double *ms, int64_t *launches)
 {

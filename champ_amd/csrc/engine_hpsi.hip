@@ -556,7 +556,7 @@ __global__ void __launch_bounds__(256) k_hpsi_eloc(HArgs a)
         const double eloc0 = k0 + vp0, eloc1 = k1 + vp1;
         const double energy = 0.0 + eloc0 + eloc1 + e_other;
         const double ek = 0.0 + k0 + k1 + ekin_other;
-        const double psid = 0.0 + a.detiab[2 * w] * a.detiab[2 * w + 1] * a.cdet0;
+        const double psid = 0.0 + a.detiab[2 * w] * a.detiab[2 * w + 1] * (a.md ? a.md_S[w] : a.cdet0);
         const double psij = a.fsumo[w];
         a.psido[w] = psid; a.psijo[w] = psij; a.eold[w] = energy; a.ekin[w] = ek;
         a.psi2o[w] = 2 * (log(fabs(psid)) + psij);
@@ -602,6 +602,12 @@ static int launch_orbitals_slater(champ_ctx *c, HArgs a)
         hipLaunchKernelGGL(k_hpsi_slater, dim3(2 * a.nw), dim3(256), lds, c->stream, a);
         tend(c, slot);
         HIPCHECK(hipGetLastError());
+        if (c->md) {
+            tbegin(c, K_MD, &slot);
+            const int rcm = launch_md_hpsi(c, w0, a.nw);
+            tend(c, slot);
+            if (rcm) return rcm;
+        }
     }
     return CHAMP_OK;
 }
@@ -611,7 +617,7 @@ __global__ void k_refresh_psi(HArgs a)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= a.W) return;
-    const double psid = 0.0 + a.detiab[2 * w] * a.detiab[2 * w + 1] * a.cdet0;
+    const double psid = 0.0 + a.detiab[2 * w] * a.detiab[2 * w + 1] * (a.md ? a.md_S[w] : a.cdet0);
     a.psido[w] = psid;
     a.psi2o[w] = 2 * (log(fabs(psid)) + a.psijo[w]);
 }
@@ -678,8 +684,14 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
         HIPCHECK(hipGetLastError());
         hipLaunchKernelGGL(k_pair_scan, dim3(1), dim3(1024), 0, c->stream, c->W, c->pair_count, c->pair_off, c->pair_total);
         HIPCHECK(hipGetLastError());
-        // wmat[w][i][m] = sum_j slmi[w][spin][i][j] coef[iworbd(j)][m], per spin
-        for (int iab = 0; iab < 2; iab++) {
+        // wmat[w][i][m] = sum_j slmi[w][spin][i][j] coef[iworbd(j)][m], per spin;
+        // multideterminant: wmat[w][i][m] = sum_o V[w][i][o] coef[o][m]
+        if (c->md) {
+            int rcg = launch_gemm(c, false, c->NE, c->NB, c->NO, c->md_V, c->NO, (long long)c->NE * c->NO, c->coef,
+                                  c->NB, 0, c->wmat, c->NB, (long long)c->NE * c->NB, c->W);
+            if (rcg) return rcg;
+        }
+        for (int iab = 0; iab < 2 && !c->md; iab++) {
             const int nel = iab ? c->NDN : c->NUP, ish = iab ? c->NUP : 0;
             if (nel == 0) continue;
             int rcg = launch_gemm(c, false, nel, c->NB, nel, c->slmi + (size_t)iab * c->NM * c->NM, c->NM,

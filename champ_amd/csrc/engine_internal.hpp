@@ -28,7 +28,7 @@ struct Timer {
 };
 
 enum KClass { K_MOVE_A = 0, K_MOVE_GEMM, K_MOVE_B, K_HPSI_JAS, K_HPSI_AO, K_HPSI_GEMM, K_HPSI_SLATER,
-              K_HPSI_NONLOC, K_HPSI_ELOC, K_OTHER, K_NCLASS };
+              K_HPSI_NONLOC, K_HPSI_ELOC, K_MD, K_OTHER, K_NCLASS };
 
 }  // namespace champ
 
@@ -108,6 +108,26 @@ struct champ_ctx {
     double *rrT = nullptr;                     // [W][NE][NC] rr_en with the centre innermost
     int *nxquad = nullptr;                     // [W]
 
+    // ---- multideterminant (ndet > 1): excitation tables per spin and state
+    int md = 0, md_ndet = 0, md_NV = 0, md_NU = 0, md_nu[2] = {0, 0}, md_nv[2] = {0, 0};
+    int *md_du = nullptr, *md_dd = nullptr;   // [ndet] unique spin determinant of each det
+    double *md_c = nullptr;                   // [ndet] cdet with the reordering phase
+    int *md_rank[2] = {nullptr, nullptr}, *md_hole[2] = {nullptr, nullptr}, *md_part[2] = {nullptr, nullptr};  // [nu][4]
+    int *md_virt[2] = {nullptr, nullptr};     // [nv] orbital of each particle index
+    int *md_woff[2] = {nullptr, nullptr}, *md_wdet[2] = {nullptr, nullptr};  // dets of each spin determinant
+    int *md_yoff[2] = {nullptr, nullptr}, *md_yent[2] = {nullptr, nullptr};  // cofactors gathered per Y cell
+    int *md_kslot[2] = {nullptr, nullptr}, *md_vidx[2] = {nullptr, nullptr};  // [NO] slot / particle index or -1
+    double *md_aa = nullptr;    // [W][2][2 buffers][NM][NV]
+    double *md_D = nullptr;     // [W][2][2][NU]
+    double *md_cof = nullptr;   // [W][2][2][NU][16]
+    double *md_S = nullptr, *md_Sn = nullptr;  // [W] psi_D / (D_up,ref D_dn,ref), current and proposed
+    int *md_cur = nullptr;      // [W][2] current buffer per spin
+    double *md_vcol = nullptr, *md_vcoln = nullptr;  // [W][NO] column iel of V, current and proposed
+    double *md_V = nullptr;     // [W][NE][NO]
+    double *md_Y = nullptr, *md_Z = nullptr;  // [W][NV][NM] scratch
+    double *md_Wt = nullptr;    // [W][NU] scratch
+    double *md_g = nullptr;     // [W][NV] scratch
+
     // ---- timing
     bool timing = false;
     std::vector<champ::Timer> ev_pool;
@@ -128,6 +148,11 @@ int launch_rotqua(champ_ctx *c);
 int launch_basis_points(champ_ctx *c, int npt, const double *dpos, double *dout);
 int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int lda, long long sA, const double *B,
                 int ldb, long long sB, double *Cm, int ldc, long long sC, int batch);
+int md_init(champ_ctx *c, const champ_system_t *s);
+void md_free(champ_ctx *c);
+int launch_md_hpsi(champ_ctx *c, int w0, int nw);
+int launch_md_pre(champ_ctx *c, int iel);
+int launch_md_post(champ_ctx *c, int iel);
 void set_error(const std::string &s);
 void tbegin(champ_ctx *c, int cls, int *slot);
 void tend(champ_ctx *c, int slot);

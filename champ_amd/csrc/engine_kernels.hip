@@ -33,6 +33,10 @@ struct MoveArgs {
     double *ratio, *psidn, *psijn, *vnew, *vold, *esum_dev;
     unsigned char *acc;
     int *status;
+    int md;                            // multideterminant trial function
+    const double *md_vcol, *md_vcoln;  // [W][NO] d ln psi_D / d phi_o(r_iel), current / proposed
+    double *md_S, *md_Sn;              // [W] psi_D / (D_up,ref D_dn,ref), current / proposed
+    int *md_cur;                       // [W][2] current buffer of the multideterminant state
 };
 
 static MoveArgs move_args(champ_ctx *c, int iel, int mode, double tau)
@@ -52,6 +56,8 @@ static MoveArgs move_args(champ_ctx *c, int iel, int mode, double tau)
     a.g1n = c->g1n; a.g2n = c->g2n; a.vjnT = c->vjnT; a.fsumn = c->fsumn; a.aomove = c->aomove; a.orbn = c->orbn;
     a.ratio = c->ratio; a.psidn = c->psidn; a.psijn = c->psijn; a.vnew = c->vnew; a.vold = c->vold;
     a.esum_dev = c->esum_dev; a.acc = c->acc; a.status = c->status;
+    a.md = c->md; a.md_vcol = c->md_vcol; a.md_vcoln = c->md_vcoln; a.md_S = c->md_S; a.md_Sn = c->md_Sn;
+    a.md_cur = c->md_cur;
     return a;
 }
 
@@ -67,12 +73,20 @@ __device__ __forceinline__ void drift_old(const MoveArgs &a, int w, int lane, do
     const double *srow = a.slmi + ((size_t)w * 2 + iab) * a.NM * a.NM + (size_t)(iel - ish) * a.NM;
     const double *dor = a.dorb + ((size_t)w * a.NE + iel) * 3 * a.NO;
     double v0 = 0, v1 = 0, v2 = 0;
-    for (int j = lane; j < nel; j += WAVE) {
-        const double s = srow[j];
-        const int jo = a.iworbd[ish + j];
-        v0 += s * dor[jo];
-        v1 += s * dor[a.NO + jo];
-        v2 += s * dor[2 * a.NO + jo];
+    if (a.md) {  // sum over every orbital of d ln psi_D / d phi_o grad phi_o
+        const double *vc = a.md_vcol + (size_t)w * a.NO;
+        for (int o = lane; o < a.NO; o += WAVE) {
+            const double s = vc[o];
+            v0 += s * dor[o]; v1 += s * dor[a.NO + o]; v2 += s * dor[2 * a.NO + o];
+        }
+    } else {
+        for (int j = lane; j < nel; j += WAVE) {
+            const double s = srow[j];
+            const int jo = a.iworbd[ish + j];
+            v0 += s * dor[jo];
+            v1 += s * dor[a.NO + jo];
+            v2 += s * dor[2 * a.NO + jo];
+        }
     }
     v0 = wave_sum(v0); v1 = wave_sum(v1); v2 = wave_sum(v2);
     const double *vj = a.vjT + (size_t)w * 3 * a.NE;
@@ -427,17 +441,26 @@ __global__ void __launch_bounds__(256) k_move_b(MoveArgs a)
         for (int j = lane; j < nel; j += WAVE) rt += srow[j] * ob[a.iworbd[ish + j]];
         rt = wave_sum(rt);
         double v0 = 0, v1 = 0, v2 = 0;
-        for (int j = lane; j < nel; j += WAVE) {
-            const double sn = srow[j] / rt;
-            const int jo = a.iworbd[ish + j];
-            v0 += sn * ob[NO + jo]; v1 += sn * ob[2 * NO + jo]; v2 += sn * ob[3 * NO + jo];
+        if (a.md) {
+            const double *vc = a.md_vcoln + (size_t)w * NO;
+            for (int o = lane; o < NO; o += WAVE) {
+                const double sn = vc[o];
+                v0 += sn * ob[NO + o]; v1 += sn * ob[2 * NO + o]; v2 += sn * ob[3 * NO + o];
+            }
+        } else {
+            for (int j = lane; j < nel; j += WAVE) {
+                const double sn = srow[j] / rt;
+                const int jo = a.iworbd[ish + j];
+                v0 += sn * ob[NO + jo]; v1 += sn * ob[2 * NO + jo]; v2 += sn * ob[3 * NO + jo];
+            }
         }
         v0 = wave_sum(v0); v1 = wave_sum(v1); v2 = wave_sum(v2);
         if (lane == 0) {
             const double *vjn = a.vjnT + (size_t)w * 3 * NE;
             double vn[3] = {0.0 + v0 + vjn[iel], 0.0 + v1 + vjn[NE + iel], 0.0 + v2 + vjn[2 * NE + iel]};
             const double detn = a.detiab[2 * w + iab] * rt;
-            const double psidn = (rt == 0.0) ? 0.0 : 0.0 + detn * a.detiab[2 * w + 1 - iab] * a.cdet0;
+            const double psidn = (rt == 0.0) ? 0.0
+                                             : 0.0 + detn * a.detiab[2 * w + 1 - iab] * (a.md ? a.md_Sn[w] : a.cdet0);
             const double psijn = a.psijn[w];
             int accept = 0;
             if (a.mode == 0) {
@@ -472,7 +495,10 @@ __global__ void __launch_bounds__(256) k_move_b(MoveArgs a)
             }
             a.ratio[w] = rt; a.psidn[w] = psidn;
             for (int k = 0; k < 3; k++) a.vnew[3 * w + k] = vn[k];
-            if (accept) { a.psido[w] = psidn; a.psijo[w] = psijn; }
+            if (accept) {
+                a.psido[w] = psidn; a.psijo[w] = psijn;
+                if (a.md) { a.md_S[w] = a.md_Sn[w]; a.md_cur[2 * w + iab] ^= 1; }
+            }
             s_ratio = rt; s_detn = detn; s_acc = accept;
         }
     }

@@ -29,6 +29,8 @@ struct HArgs {
     double *xq, *yq, *zq;
     double *esum_dev;
     int *status;
+    int md;                 // multideterminant: psi_D = D_up,ref D_dn,ref md_S
+    const double *md_S;     // [W]
 };
 
 inline HArgs hargs(champ_ctx *c)
@@ -50,6 +52,7 @@ inline HArgs hargs(champ_ctx *c)
     a.NRI = c->NRI; a.ri_cent = c->ri_cent; a.ri_type = c->ri_type; a.ri_rad = c->ri_rad; a.ao_ri = c->ao_ri;
     a.ri_rmax = c->ri_rmax;
     a.xq = c->xq; a.yq = c->yq; a.zq = c->zq; a.esum_dev = c->esum_dev; a.status = c->status;
+    a.md = c->md; a.md_S = c->md_S;
     return a;
 }
 

@@ -88,7 +88,10 @@ typedef struct champ_system {
     int32_t norb;                   /* MOs referenced by any determinant          */
     const double  *coef;            /* [norb][nbasis]  coef(m,iorb) of CHAMP      */
     int32_t ndet;                   /* determinants (1 = single Slater det)       */
-    const int32_t *iworbd;          /* [ndet][nelec] orbital of electron slot (0-based) */
+    const int32_t *iworbd;          /* [ndet][nelec] orbital of electron slot (0-based);
+                                       determinant 0 is the reference (kref = 1); every
+                                       other one may differ from it by at most 4
+                                       orbitals per spin, in any slot order          */
     const double  *cdet;            /* [ndet] CI coefficients                     */
 
     /* Jastrow ijas=4, isc=2                                                     */
@@ -158,7 +161,14 @@ int champ_gpu_rotqua(champ_ctx *ctx);
 int champ_gpu_hpsi(champ_ctx *ctx, double *psid, double *psij, double *ekin,
                    double *energy);
 
-/* compute_determinante_grad (determinante.f90:69-283), single determinant:
+/* With ndet > 1 the multideterminant parts run inside these same calls:
+ * init builds the excitation tables (multideterminants_define,
+ * set_input_data.f90:46-414), hpsi adds multideterminant_hpsi
+ * (multideterminant.f90:3-255) and psie / compute_determinante_grad / accept
+ * add multideterminante(_grad) and its saves (multideterminante.f90:3-222,
+ * detsav.f90).  psid and psidn are then sum_k cdet_k D_up,k D_dn,k.        */
+
+/* compute_determinante_grad (determinante.f90:69-283):
  * iflag_move=1 -> drift of electron iel at its current position,
  * iflag_move=0 -> drift at the position last proposed by psie.
  * vd: [nwalk][3].                                                             */
@@ -192,10 +202,10 @@ int champ_gpu_get_status(champ_ctx *ctx, int32_t *status);
 /* Array readback for checking (host buffer sized by champ_gpu_array_size).   */
 #define CHAMP_ARR_SLMI   1   /* [nwalk][2][nmax][nmax] transpose inverse rows  */
 #define CHAMP_ARR_VJ     2   /* [nwalk][nelec][3]  Jastrow gradient (vj)       */
-#define CHAMP_ARR_DDX    3   /* [nwalk][nelec][3]  grad ln D                   */
-#define CHAMP_ARR_D2DX2  4   /* [nwalk][nelec]     lap D / D                   */
-#define CHAMP_ARR_DETIAB 5   /* [nwalk][2]         spin determinants           */
-#define CHAMP_ARR_VPSP   6   /* [nwalk][2]         nonlocal energy per spin    */
+#define CHAMP_ARR_DDX    3   /* [nwalk][nelec][3]  grad_i psi_D / psi_D (all determinants) */
+#define CHAMP_ARR_D2DX2  4   /* [nwalk][nelec]     lap_i psi_D / psi_D (all determinants)  */
+#define CHAMP_ARR_DETIAB 5   /* [nwalk][2]         reference spin determinants */
+#define CHAMP_ARR_VPSP   6   /* [nwalk][2]         nonlocal energy per spin (psi_D with all determinants) */
 #define CHAMP_ARR_PELOC  7   /* [nwalk]            local potential             */
 #define CHAMP_ARR_D2J    8   /* [nwalk]            Jastrow Laplacian sum       */
 #define CHAMP_ARR_ORBN   9   /* [nwalk][4][norb]   orbn,dorbn of last psie     */

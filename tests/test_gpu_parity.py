@@ -2,7 +2,9 @@
 
 Bar: 1e-10 relative for psi ratios, drift vectors and local energies (the
 plain FP64 path), checked walker by walker.  Every call goes through the
-C-ABI (libchamp_gpu.so).
+C-ABI (libchamp_gpu.so).  The *_md systems are multideterminant trial
+functions (40-60 determinants, 1-4-fold excitations, reordered orbital slots);
+the oracle evaluates them determinant by determinant.
 """
 import numpy as np
 import pytest
@@ -20,7 +22,8 @@ def _hpsi_both(pr: Pair):
     return g, o
 
 
-@pytest.mark.parametrize("name,nwalk", [("small_ae", 6), ("small", 6), ("c22", 4)])
+@pytest.mark.parametrize("name,nwalk", [("small_ae", 6), ("small", 6), ("c22", 4), ("small_ae_md", 4),
+                                        ("small_md", 4), ("c22_md", 3)])
 def test_hpsi_parity(require_gpu, name, nwalk):
     pr = Pair(name, nwalk)
     pr.quad()
@@ -42,7 +45,7 @@ def test_hpsi_parity(require_gpu, name, nwalk):
     assert not np.any(pr.eng.status())
 
 
-@pytest.mark.parametrize("name,nwalk", [("small", 5), ("c22", 4)])
+@pytest.mark.parametrize("name,nwalk", [("small", 5), ("c22", 4), ("small_md", 4), ("c22_md", 3)])
 def test_psie_drift_accept_parity(require_gpu, name, nwalk):
     """psie + compute_determinante_grad + accept/reject over every electron."""
     pr = Pair(name, nwalk, seed=5)
@@ -96,12 +99,13 @@ def test_basis_fns_parity(require_gpu):
         assert relnorm(g[p, 4], d2) < 1e-12
 
 
-def test_metrop1_trajectory(require_gpu):
+@pytest.mark.parametrize("name", ["small", "small_md"])
+def test_metrop1_trajectory(require_gpu, name):
     """Two metrop1 sweeps from the same seeds: the GPU population follows the
     oracle chains (same uniforms; gaussians agree to the last bits)."""
     from champ_amd import system as S
     from oracle import ffi
-    pr = Pair("small", 6, seed=9)
+    pr = Pair(name, 6, seed=9)
     streams = ffi.seeded_streams(S.seed_from_string(), pr.nwalk)
     pr.eng.set_rng(streams)
     for w in range(pr.nwalk):
