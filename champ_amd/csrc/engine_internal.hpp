@@ -117,9 +117,10 @@ struct champ_ctx {
     int *md_woff[2] = {nullptr, nullptr}, *md_wdet[2] = {nullptr, nullptr};  // dets of each spin determinant
     int *md_yoff[2] = {nullptr, nullptr}, *md_yent[2] = {nullptr, nullptr};  // cofactors gathered per Y cell
     int *md_kslot[2] = {nullptr, nullptr}, *md_vidx[2] = {nullptr, nullptr};  // [NO] slot / particle index or -1
+    int *md_zoff[2] = {nullptr, nullptr}, *md_zent[2] = {nullptr, nullptr};  // (u, b) of each particle (CSR)
     double *md_aa = nullptr;    // [W][2][2 buffers][NM][NV]
     double *md_D = nullptr;     // [W][2][2][NU]
-    double *md_cof = nullptr;   // [W][2][2][NU][16]
+    double *md_tb = nullptr;    // [W][NU][4] scratch: C_u^T times a column of the inverse
     double *md_S = nullptr, *md_Sn = nullptr;  // [W] psi_D / (D_up,ref D_dn,ref), current and proposed
     int *md_cur = nullptr;      // [W][2] current buffer per spin
     double *md_vcol = nullptr, *md_vcoln = nullptr;  // [W][NO] column iel of V, current and proposed

@@ -27,9 +27,10 @@
 // A move of electron iel (reference ratio R0) changes aa by a rank-one term,
 //   aa'[j][v] = aa[j][v] + Ainv[j][iel] / R0 * (phi'_v - sum_j' phi'_{o_j'} aa[j'][v]),
 // after which the spin's determinants, S, Y and the new electron's column of
-// V follow (multideterminante.f90:3-222).  aa, the spin determinants and
-// their cofactors are double-buffered per walker and spin; an accepted move
-// flips the buffer.
+// V follow (multideterminante.f90:3-222).  aa and the spin determinants are
+// double-buffered per walker and spin; an accepted move flips the buffer.
+// Cofactors are never stored: the moves need only C_u^T (Ainv column) per
+// unique determinant, hpsi recomputes each Y entry's cofactor from aa.
 #include "engine_internal.hpp"
 
 #include <algorithm>
@@ -46,10 +47,11 @@ struct MdArgs {
     const int *du, *dd;
     const double *cdet;
     const int *rank[2], *hole[2], *part[2], *virt[2], *woff[2], *wdet[2], *yoff[2], *yent[2], *kslot[2], *vidx[2];
+    const int *zoff[2], *zent[2];
     const int *iworbd;
     const double *slmi, *orball, *orbn;
     double *ddx, *d2dx2;
-    double *aa, *D, *cof, *S, *Sn, *vcol, *vcoln, *V, *Y, *Z, *Wt, *g;
+    double *aa, *D, *tb, *S, *Sn, *vcol, *vcoln, *V, *Y, *Z, *Wt, *g;
     int *cur;
 };
 
@@ -63,10 +65,11 @@ static MdArgs md_args(champ_ctx *c, int iel)
         m.rank[s] = c->md_rank[s]; m.hole[s] = c->md_hole[s]; m.part[s] = c->md_part[s]; m.virt[s] = c->md_virt[s];
         m.woff[s] = c->md_woff[s]; m.wdet[s] = c->md_wdet[s]; m.yoff[s] = c->md_yoff[s]; m.yent[s] = c->md_yent[s];
         m.kslot[s] = c->md_kslot[s]; m.vidx[s] = c->md_vidx[s];
+        m.zoff[s] = c->md_zoff[s]; m.zent[s] = c->md_zent[s];
     }
     m.du = c->md_du; m.dd = c->md_dd; m.cdet = c->md_c; m.iworbd = c->iworbd;
     m.slmi = c->slmi; m.orball = c->orball; m.orbn = c->orbn; m.ddx = c->ddx; m.d2dx2 = c->d2dx2;
-    m.aa = c->md_aa; m.D = c->md_D; m.cof = c->md_cof; m.S = c->md_S; m.Sn = c->md_Sn; m.vcol = c->md_vcol;
+    m.aa = c->md_aa; m.D = c->md_D; m.tb = c->md_tb; m.S = c->md_S; m.Sn = c->md_Sn; m.vcol = c->md_vcol;
     m.vcoln = c->md_vcoln; m.V = c->md_V; m.Y = c->md_Y; m.Z = c->md_Z; m.Wt = c->md_Wt; m.g = c->md_g;
     m.cur = c->md_cur;
     return m;
@@ -80,10 +83,6 @@ __device__ __forceinline__ double *md_aa(const MdArgs &m, int w, int s, int b)
 __device__ __forceinline__ double *md_D(const MdArgs &m, int w, int s, int b)
 {
     return m.D + (((size_t)w * 2 + s) * 2 + b) * (size_t)m.NU;
-}
-__device__ __forceinline__ double *md_cof(const MdArgs &m, int w, int s, int b)
-{
-    return m.cof + (((size_t)w * 2 + s) * 2 + b) * (size_t)m.NU * 16;
 }
 
 // det(alpha) and its cofactors C[a][b] (at a * 4 + b) for rank r <= 4;
@@ -119,17 +118,41 @@ __device__ double small_det_cof(int r, const double *al /*[4][4]*/, double *C /*
 #undef M
 }
 
+__device__ __forceinline__ void md_alpha(const MdArgs &m, int s, int u, int r, const double *aa, double *al)
+{
+    for (int a = 0; a < r; a++)
+        for (int b = 0; b < r; b++) al[a * 4 + b] = aa[(size_t)m.hole[s][4 * u + a] * m.NV + m.part[s][4 * u + b]];
+}
+
 // spin determinants of spin s from aa (one thread per unique determinant)
-__device__ void md_dets(const MdArgs &m, int s, const double *aa, double *D, double *cof)
+__device__ void md_dets(const MdArgs &m, int s, const double *aa, double *D)
 {
     for (int u = threadIdx.x; u < m.nu[s]; u += blockDim.x) {
         const int r = m.rank[s][u];
         double al[16], C[16];
-        for (int a = 0; a < r; a++)
-            for (int b = 0; b < r; b++) al[a * 4 + b] = aa[(size_t)m.hole[s][4 * u + a] * m.NV + m.part[s][4 * u + b]];
+        md_alpha(m, s, u, r, aa, al);
         D[u] = small_det_cof(r, al, C);
-        for (int t = 0; t < 16; t++) cof[(size_t)u * 16 + t] = C[t];
     }
+}
+
+// cofactor C_u[a][b] of alpha_u (a (r-1) x (r-1) minor of aa)
+__device__ double md_cof_entry(const MdArgs &m, int s, int u, int a, int b, const double *aa)
+{
+    const int r = m.rank[s][u];
+    if (r == 1) return 1.0;
+    int rows[3], cols[3], n = 0;
+    for (int t = 0; t < r; t++) if (t != a) rows[n++] = m.hole[s][4 * u + t];
+    n = 0;
+    for (int t = 0; t < r; t++) if (t != b) cols[n++] = m.part[s][4 * u + t];
+    const int NV = m.NV;
+#define E(i, j) aa[(size_t)rows[i] * NV + cols[j]]
+    double mn;
+    if (r == 2) mn = E(0, 0);
+    else if (r == 3) mn = E(0, 0) * E(1, 1) - E(0, 1) * E(1, 0);
+    else mn = E(0, 0) * (E(1, 1) * E(2, 2) - E(1, 2) * E(2, 1)) - E(0, 1) * (E(1, 0) * E(2, 2) - E(1, 2) * E(2, 0)) +
+              E(0, 2) * (E(1, 0) * E(2, 1) - E(1, 1) * E(2, 0));
+#undef E
+    return ((a + b) & 1) ? -mn : mn;
 }
 
 __device__ double md_block_sum(double v, double *red)
@@ -153,11 +176,11 @@ __device__ double md_sum(const MdArgs &m, const double *Dup, const double *Ddn, 
     return md_block_sum(s, red);
 }
 
-// weights W_u of spin s (the other spin's determinants Do), then
-// Y[v][j] = sum W_u C_u[a][b]; Y, Wt: per-walker scratch
-__device__ void md_weights_y(const MdArgs &m, int w, int s, const double *Do, double S, const double *cof, int nel)
+// weights W_u = sum_{k using u} c_k D_other,k / S of spin s (Do: the other
+// spin's determinants) into the per-walker scratch Wt
+__device__ void md_weights(const MdArgs &m, int w, int s, const double *Do, double S)
 {
-    double *Wt = m.Wt + (size_t)w * m.NU, *Y = m.Y + (size_t)w * m.NV * m.NM;
+    double *Wt = m.Wt + (size_t)w * m.NU;
     const int *oth = s == 0 ? m.dd : m.du;
     const double sinv = 1.0 / S;
     for (int u = threadIdx.x; u < m.nu[s]; u += blockDim.x) {
@@ -169,14 +192,54 @@ __device__ void md_weights_y(const MdArgs &m, int w, int s, const double *Do, do
         Wt[u] = t * sinv;
     }
     __syncthreads();
+}
+
+// Y[v][j] = sum W_u C_u[a][b] over (h_a, p_b) = (j, v), cofactors from aa
+__device__ void md_y(const MdArgs &m, int w, int s, const double *aa, int nel)
+{
+    const double *Wt = m.Wt + (size_t)w * m.NU;
+    double *Y = m.Y + (size_t)w * m.NV * m.NM;
     for (int t = threadIdx.x; t < m.nv[s] * nel; t += blockDim.x) {
         const int v = t / nel, j = t % nel, cell = v * m.NM + j;
         double y = 0;
         for (int e = m.yoff[s][cell]; e < m.yoff[s][cell + 1]; e++) {
-            const int ent = m.yent[s][e];
-            y += Wt[ent >> 4] * cof[ent];
+            const int ent = m.yent[s][e], u = ent >> 4;
+            y += Wt[u] * md_cof_entry(m, s, u, (ent >> 2) & 3, ent & 3, aa);
         }
         Y[cell] = y;
+    }
+    __syncthreads();
+}
+
+// Column i of Z = Y Ainv without forming Y:
+//   z[v] = sum_u W_u sum_b [p_b = v] t_u[b],  t_u[b] = sum_a C_u[a][b] Ainv[h_a][i],
+// C_u from aa in registers; t_u into the per-walker scratch tb, then one
+// gather per particle v.  A: the spin's slmi (A[i][j] = Ainv[j][i]).
+__device__ void md_zcol(const MdArgs &m, int w, int s, const double *aa, const double *A, int i, double scale,
+                        double *z)
+{
+    double *tb = m.tb + (size_t)w * m.NU * 4;
+    for (int u = threadIdx.x; u < m.nu[s]; u += blockDim.x) {
+        const int r = m.rank[s][u];
+        double al[16], C[16], x[4];
+        md_alpha(m, s, u, r, aa, al);
+        small_det_cof(r, al, C);
+        for (int a = 0; a < r; a++) x[a] = A[(size_t)i * m.NM + m.hole[s][4 * u + a]];
+        for (int b = 0; b < r; b++) {
+            double t = 0;
+            for (int a = 0; a < r; a++) t += C[a * 4 + b] * x[a];
+            tb[(size_t)u * 4 + b] = t;
+        }
+    }
+    __syncthreads();
+    const double *Wt = m.Wt + (size_t)w * m.NU;
+    for (int v = threadIdx.x; v < m.nv[s]; v += blockDim.x) {
+        double x = 0;
+        for (int e = m.zoff[s][v]; e < m.zoff[s][v + 1]; e++) {
+            const int ent = m.zent[s][e];
+            x += Wt[ent >> 2] * tb[ent];
+        }
+        z[v] = x * scale;
     }
     __syncthreads();
 }
@@ -204,7 +267,7 @@ __global__ void __launch_bounds__(256) k_md_hpsi(MdArgs m)
     __syncthreads();
     for (int s = 0; s < 2; s++) {
         const int b = m.cur[2 * w + s];
-        md_dets(m, s, md_aa(m, w, s, b), md_D(m, w, s, b), md_cof(m, w, s, b));
+        md_dets(m, s, md_aa(m, w, s, b), md_D(m, w, s, b));
     }
     __syncthreads();
     const double *D0 = md_D(m, w, 0, m.cur[2 * w]), *D1 = md_D(m, w, 1, m.cur[2 * w + 1]);
@@ -216,7 +279,8 @@ __global__ void __launch_bounds__(256) k_md_hpsi(MdArgs m)
         const int nel = s ? NE - m.NUP : m.NUP, ish = s ? m.NUP : 0, b = m.cur[2 * w + s], nv = m.nv[s];
         const double *A = m.slmi + ((size_t)w * 2 + s) * NM * NM;
         const double *aa = md_aa(m, w, s, b);
-        md_weights_y(m, w, s, s ? D0 : D1, S, md_cof(m, w, s, b), nel);
+        md_weights(m, w, s, s ? D0 : D1, S);
+        md_y(m, w, s, aa, nel);
         for (int t = tid; t < nv * nel; t += bd) {  // Z[v][i] = sum_j Y[v][j] Ainv[j][i]
             const int v = t / nel, i = t % nel;
             double z = 0;
@@ -261,17 +325,12 @@ __global__ void __launch_bounds__(256) k_md_pre(MdArgs m)
     const int NE = m.NE, NO = m.NO, NV = m.NV, NM = m.NM, iel = m.iel;
     const int s = iel < m.NUP ? 0 : 1, nel = s ? NE - m.NUP : m.NUP, ish = s ? m.NUP : 0, i = iel - ish;
     const int b = m.cur[2 * w + s], bo = m.cur[2 * w + 1 - s], nv = m.nv[s];
+    (void)nel;
     const double *A = m.slmi + ((size_t)w * 2 + s) * NM * NM;
     const double *aa = md_aa(m, w, s, b);
-    md_weights_y(m, w, s, md_D(m, w, 1 - s, bo), m.S[w], md_cof(m, w, s, b), nel);
+    md_weights(m, w, s, md_D(m, w, 1 - s, bo), m.S[w]);
     double *z = m.Z + (size_t)w * NV * NM;
-    const double *Y = m.Y + (size_t)w * NV * NM;
-    for (int v = tid; v < nv; v += bd) {
-        double x = 0;
-        for (int j = 0; j < nel; j++) x += Y[(size_t)v * NM + j] * A[(size_t)i * NM + j];
-        z[v] = x;
-    }
-    __syncthreads();
+    md_zcol(m, w, s, aa, A, i, 1.0, z);
     double *vc = m.vcol + (size_t)w * NO;
     for (int o = tid; o < NO; o += bd) {
         const int j = m.kslot[s][o], v = m.vidx[s][o];
@@ -326,7 +385,7 @@ __global__ void __launch_bounds__(256) k_md_post(MdArgs m)
         aan[(size_t)j * NV + v] = aa[(size_t)j * NV + v] + A[(size_t)i * NM + j] * rinv * gw[v];
     }
     __syncthreads();
-    md_dets(m, s, aan, md_D(m, w, s, bn), md_cof(m, w, s, bn));
+    md_dets(m, s, aan, md_D(m, w, s, bn));
     __syncthreads();
     const double *Dn = md_D(m, w, s, bn), *Do = md_D(m, w, 1 - s, bo);
     const double Sn = s == 0 ? md_sum(m, Dn, Do, red) : md_sum(m, Do, Dn, red);
@@ -335,15 +394,9 @@ __global__ void __launch_bounds__(256) k_md_post(MdArgs m)
         for (int o = tid; o < NO; o += bd) vcn[o] = 0.0;
         return;
     }
-    md_weights_y(m, w, s, Do, Sn, md_cof(m, w, s, bn), nel);
+    md_weights(m, w, s, Do, Sn);
     double *z = m.Z + (size_t)w * NV * NM;
-    const double *Y = m.Y + (size_t)w * NV * NM;
-    for (int v = tid; v < nv; v += bd) {
-        double x = 0;
-        for (int j = 0; j < nel; j++) x += Y[(size_t)v * NM + j] * A[(size_t)i * NM + j];
-        z[v] = x * rinv;
-    }
-    __syncthreads();
+    md_zcol(m, w, s, aan, A, i, rinv, z);
     for (int o = tid; o < NO; o += bd) {
         const int j = m.kslot[s][o], v = m.vidx[s][o];
         double x = 0;
@@ -520,6 +573,15 @@ int md_init(champ_ctx *c, const champ_system_t *s)
             yoff[t + 1] = yoff[t] + (int)cell[t].size();
             yent.insert(yent.end(), cell[t].begin(), cell[t].end());
         }
+        // (u, b) with particle p_b = v, per v (CSR): the column gathers of the moves
+        std::vector<std::vector<int>> pv(NV);
+        for (int u = 1; u < nu; u++)
+            for (int b = 0; b < T[sp].rank[u]; b++) pv[part[4 * u + b]].push_back(u * 4 + b);
+        std::vector<int> zoff(NV + 1, 0), zent;
+        for (int v = 0; v < NV; v++) {
+            zoff[v + 1] = zoff[v] + (int)pv[v].size();
+            zent.insert(zent.end(), pv[v].begin(), pv[v].end());
+        }
         int rc;
         if ((rc = md_upload(&c->md_rank[sp], T[sp].rank))) return rc;
         if ((rc = md_upload(&c->md_hole[sp], T[sp].hole))) return rc;
@@ -531,6 +593,8 @@ int md_init(champ_ctx *c, const champ_system_t *s)
         if ((rc = md_upload(&c->md_yent[sp], yent))) return rc;
         if ((rc = md_upload(&c->md_kslot[sp], T[sp].kslot))) return rc;
         if ((rc = md_upload(&c->md_vidx[sp], T[sp].vidx))) return rc;
+        if ((rc = md_upload(&c->md_zoff[sp], zoff))) return rc;
+        if ((rc = md_upload(&c->md_zent[sp], zent))) return rc;
     }
     int rc;
     if ((rc = md_upload(&c->md_du, uidx[0]))) return rc;
@@ -539,7 +603,7 @@ int md_init(champ_ctx *c, const champ_system_t *s)
     const size_t NU = c->md_NU;
     if ((rc = md_zeros(&c->md_aa, (size_t)W * 4 * NM * NV))) return rc;
     if ((rc = md_zeros(&c->md_D, (size_t)W * 4 * NU))) return rc;
-    if ((rc = md_zeros(&c->md_cof, (size_t)W * 4 * NU * 16))) return rc;
+    if ((rc = md_zeros(&c->md_tb, (size_t)W * NU * 4))) return rc;
     if ((rc = md_zeros(&c->md_S, (size_t)W))) return rc;
     if ((rc = md_zeros(&c->md_Sn, (size_t)W))) return rc;
     if ((rc = md_zeros(&c->md_cur, (size_t)W * 2))) return rc;
@@ -555,12 +619,13 @@ int md_init(champ_ctx *c, const champ_system_t *s)
 
 void md_free(champ_ctx *c)
 {
-    void *ps[] = {c->md_du, c->md_dd, c->md_c, c->md_aa, c->md_D, c->md_cof, c->md_S, c->md_Sn, c->md_cur,
+    void *ps[] = {c->md_du, c->md_dd, c->md_c, c->md_aa, c->md_D, c->md_tb, c->md_S, c->md_Sn, c->md_cur,
                   c->md_vcol, c->md_vcoln, c->md_V, c->md_Y, c->md_Z, c->md_Wt, c->md_g};
     for (void *p : ps) if (p) hipFree(p);
     for (int sp = 0; sp < 2; sp++) {
         void *qs[] = {c->md_rank[sp], c->md_hole[sp], c->md_part[sp], c->md_virt[sp], c->md_woff[sp], c->md_wdet[sp],
-                      c->md_yoff[sp], c->md_yent[sp], c->md_kslot[sp], c->md_vidx[sp]};
+                      c->md_yoff[sp], c->md_yent[sp], c->md_kslot[sp], c->md_vidx[sp], c->md_zoff[sp],
+                      c->md_zent[sp]};
         for (void *q : qs) if (q) hipFree(q);
     }
 }
