@@ -102,6 +102,7 @@ struct champ_ctx {
     double *orball = nullptr;  // [chunk][NE][5][NO]
     double *wmat = nullptr;    // [W][NE][NB]
     int *pair_count = nullptr, *pair_off = nullptr, *pair_total = nullptr;  // [W], [W+1], [2]
+    int *grp_off = nullptr;                    // [W+1] prefix sums of the per-walker pair groups
     int *pair_e = nullptr, *pair_c = nullptr;  // [W*pair_cap]
     double *pair_vps = nullptr;                // [W*pair_cap][MPS_L]
     double *pair_val = nullptr;                // [W*pair_cap]
@@ -121,6 +122,7 @@ struct champ_ctx {
     double *md_aa = nullptr;    // [W][2][2 buffers][NM][NV]
     double *md_D = nullptr;     // [W][2][2][NU]
     double *md_tb = nullptr;    // [W][NU][4] scratch: C_u^T times a column of the inverse
+    double *md_G = nullptr;     // [W][NU][16] hpsi scratch: weighted cofactors
     double *md_S = nullptr, *md_Sn = nullptr;  // [W] psi_D / (D_up,ref D_dn,ref), current and proposed
     int *md_cur = nullptr;      // [W][2] current buffer per spin
     double *md_vcol = nullptr, *md_vcoln = nullptr;  // [W][NO] column iel of V, current and proposed

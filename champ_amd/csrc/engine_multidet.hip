@@ -29,8 +29,8 @@
 // after which the spin's determinants, S, Y and the new electron's column of
 // V follow (multideterminante.f90:3-222).  aa and the spin determinants are
 // double-buffered per walker and spin; an accepted move flips the buffer.
-// Cofactors are never stored: the moves need only C_u^T (Ainv column) per
-// unique determinant, hpsi recomputes each Y entry's cofactor from aa.
+// The moves never store cofactors: they need only C_u^T (Ainv column) per
+// unique determinant; hpsi writes the weighted cofactors once into scratch.
 #include "engine_internal.hpp"
 
 #include <algorithm>
@@ -51,7 +51,7 @@ struct MdArgs {
     const int *iworbd;
     const double *slmi, *orball, *orbn;
     double *ddx, *d2dx2;
-    double *aa, *D, *tb, *S, *Sn, *vcol, *vcoln, *V, *Y, *Z, *Wt, *g;
+    double *aa, *D, *tb, *G, *S, *Sn, *vcol, *vcoln, *V, *Y, *Z, *Wt, *g;
     int *cur;
 };
 
@@ -69,7 +69,7 @@ static MdArgs md_args(champ_ctx *c, int iel)
     }
     m.du = c->md_du; m.dd = c->md_dd; m.cdet = c->md_c; m.iworbd = c->iworbd;
     m.slmi = c->slmi; m.orball = c->orball; m.orbn = c->orbn; m.ddx = c->ddx; m.d2dx2 = c->d2dx2;
-    m.aa = c->md_aa; m.D = c->md_D; m.tb = c->md_tb; m.S = c->md_S; m.Sn = c->md_Sn; m.vcol = c->md_vcol;
+    m.aa = c->md_aa; m.D = c->md_D; m.tb = c->md_tb; m.G = c->md_G; m.S = c->md_S; m.Sn = c->md_Sn; m.vcol = c->md_vcol;
     m.vcoln = c->md_vcoln; m.V = c->md_V; m.Y = c->md_Y; m.Z = c->md_Z; m.Wt = c->md_Wt; m.g = c->md_g;
     m.cur = c->md_cur;
     return m;
@@ -85,74 +85,61 @@ __device__ __forceinline__ double *md_D(const MdArgs &m, int w, int s, int b)
     return m.D + (((size_t)w * 2 + s) * 2 + b) * (size_t)m.NU;
 }
 
-// det(alpha) and its cofactors C[a][b] (at a * 4 + b) for rank r <= 4;
-// rank 0 is the reference itself (ratio 1)
-__device__ double small_det_cof(int r, const double *al /*[4][4]*/, double *C /*[16]*/)
+// det(alpha) and all cofactors C[a][b] (at a * 4 + b) of a 4 x 4 matrix.
+// alpha_u of rank r < 4 is padded with an identity block, which leaves its
+// determinant and the cofactors of its r x r part unchanged; every index is
+// a compile-time constant, so the matrices stay in registers.
+__device__ __forceinline__ double det_cof4(const double *al /*[16]*/, double *C /*[16]*/)
 {
 #define M(i, j) al[(i) * 4 + (j)]
-    if (r == 0) return 1.0;
-    if (r == 1) { C[0] = 1.0; return M(0, 0); }
-    if (r == 2) {
-        C[0] = M(1, 1); C[1] = -M(1, 0); C[4] = -M(0, 1); C[5] = M(0, 0);
-        return M(0, 0) * M(1, 1) - M(0, 1) * M(1, 0);
-    }
-    if (r == 3) {
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) {
-                const int a0 = a == 0 ? 1 : 0, a1 = a == 2 ? 1 : 2, b0 = b == 0 ? 1 : 0, b1 = b == 2 ? 1 : 2;
-                const double mn = M(a0, b0) * M(a1, b1) - M(a0, b1) * M(a1, b0);
-                C[a * 4 + b] = ((a + b) & 1) ? -mn : mn;
-            }
-        return M(0, 0) * C[0] + M(0, 1) * C[1] + M(0, 2) * C[2];
-    }
+#pragma unroll
     for (int a = 0; a < 4; a++)
+#pragma unroll
         for (int b = 0; b < 4; b++) {
-            int ra[3], cb[3], na = 0, nb = 0;
-            for (int t = 0; t < 4; t++) { if (t != a) ra[na++] = t; if (t != b) cb[nb++] = t; }
-            const double mn = M(ra[0], cb[0]) * (M(ra[1], cb[1]) * M(ra[2], cb[2]) - M(ra[1], cb[2]) * M(ra[2], cb[1]))
-                            - M(ra[0], cb[1]) * (M(ra[1], cb[0]) * M(ra[2], cb[2]) - M(ra[1], cb[2]) * M(ra[2], cb[0]))
-                            + M(ra[0], cb[2]) * (M(ra[1], cb[0]) * M(ra[2], cb[1]) - M(ra[1], cb[1]) * M(ra[2], cb[0]));
+            const int r0 = a == 0 ? 1 : 0, r1 = a <= 1 ? 2 : 1, r2 = a <= 2 ? 3 : 2;
+            const int c0 = b == 0 ? 1 : 0, c1 = b <= 1 ? 2 : 1, c2 = b <= 2 ? 3 : 2;
+            const double mn = M(r0, c0) * (M(r1, c1) * M(r2, c2) - M(r1, c2) * M(r2, c1))
+                            - M(r0, c1) * (M(r1, c0) * M(r2, c2) - M(r1, c2) * M(r2, c0))
+                            + M(r0, c2) * (M(r1, c0) * M(r2, c1) - M(r1, c1) * M(r2, c0));
             C[a * 4 + b] = ((a + b) & 1) ? -mn : mn;
         }
     return M(0, 0) * C[0] + M(0, 1) * C[1] + M(0, 2) * C[2] + M(0, 3) * C[3];
 #undef M
 }
 
-__device__ __forceinline__ void md_alpha(const MdArgs &m, int s, int u, int r, const double *aa, double *al)
+__device__ __forceinline__ double det4(const double *al)
 {
-    for (int a = 0; a < r; a++)
-        for (int b = 0; b < r; b++) al[a * 4 + b] = aa[(size_t)m.hole[s][4 * u + a] * m.NV + m.part[s][4 * u + b]];
+#define M(i, j) al[(i) * 4 + (j)]
+    const double s0 = M(0, 0) * M(1, 1) - M(1, 0) * M(0, 1), s1 = M(0, 0) * M(1, 2) - M(1, 0) * M(0, 2);
+    const double s2 = M(0, 0) * M(1, 3) - M(1, 0) * M(0, 3), s3 = M(0, 1) * M(1, 2) - M(1, 1) * M(0, 2);
+    const double s4 = M(0, 1) * M(1, 3) - M(1, 1) * M(0, 3), s5 = M(0, 2) * M(1, 3) - M(1, 2) * M(0, 3);
+    const double c5 = M(2, 2) * M(3, 3) - M(3, 2) * M(2, 3), c4 = M(2, 1) * M(3, 3) - M(3, 1) * M(2, 3);
+    const double c3 = M(2, 1) * M(3, 2) - M(3, 1) * M(2, 2), c2 = M(2, 0) * M(3, 3) - M(3, 0) * M(2, 3);
+    const double c1 = M(2, 0) * M(3, 2) - M(3, 0) * M(2, 2), c0 = M(2, 0) * M(3, 1) - M(3, 0) * M(2, 1);
+    return s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+#undef M
+}
+
+// alpha_u[a][b] = aa[h_a][p_b], padded to 4 x 4 with the identity
+__device__ __forceinline__ void md_alpha(const MdArgs &m, int s, int u, const double *aa, double *al)
+{
+    const int r = m.rank[s][u];
+    const int *h = m.hole[s] + 4 * u, *p = m.part[s] + 4 * u;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            al[a * 4 + b] = (a < r && b < r) ? aa[(size_t)h[a] * m.NV + p[b]] : (a == b ? 1.0 : 0.0);
 }
 
 // spin determinants of spin s from aa (one thread per unique determinant)
 __device__ void md_dets(const MdArgs &m, int s, const double *aa, double *D)
 {
     for (int u = threadIdx.x; u < m.nu[s]; u += blockDim.x) {
-        const int r = m.rank[s][u];
-        double al[16], C[16];
-        md_alpha(m, s, u, r, aa, al);
-        D[u] = small_det_cof(r, al, C);
+        double al[16];
+        md_alpha(m, s, u, aa, al);
+        D[u] = det4(al);
     }
-}
-
-// cofactor C_u[a][b] of alpha_u (a (r-1) x (r-1) minor of aa)
-__device__ double md_cof_entry(const MdArgs &m, int s, int u, int a, int b, const double *aa)
-{
-    const int r = m.rank[s][u];
-    if (r == 1) return 1.0;
-    int rows[3], cols[3], n = 0;
-    for (int t = 0; t < r; t++) if (t != a) rows[n++] = m.hole[s][4 * u + t];
-    n = 0;
-    for (int t = 0; t < r; t++) if (t != b) cols[n++] = m.part[s][4 * u + t];
-    const int NV = m.NV;
-#define E(i, j) aa[(size_t)rows[i] * NV + cols[j]]
-    double mn;
-    if (r == 2) mn = E(0, 0);
-    else if (r == 3) mn = E(0, 0) * E(1, 1) - E(0, 1) * E(1, 0);
-    else mn = E(0, 0) * (E(1, 1) * E(2, 2) - E(1, 2) * E(2, 1)) - E(0, 1) * (E(1, 0) * E(2, 2) - E(1, 2) * E(2, 0)) +
-              E(0, 2) * (E(1, 0) * E(2, 1) - E(1, 1) * E(2, 0));
-#undef E
-    return ((a + b) & 1) ? -mn : mn;
 }
 
 __device__ double md_block_sum(double v, double *red)
@@ -194,18 +181,27 @@ __device__ void md_weights(const MdArgs &m, int w, int s, const double *Do, doub
     __syncthreads();
 }
 
-// Y[v][j] = sum W_u C_u[a][b] over (h_a, p_b) = (j, v), cofactors from aa
+// Y[v][j] = sum W_u C_u[a][b] over (h_a, p_b) = (j, v): the weighted
+// cofactors of each unique determinant into the scratch G (one thread per
+// determinant), then one gather per cell
 __device__ void md_y(const MdArgs &m, int w, int s, const double *aa, int nel)
 {
     const double *Wt = m.Wt + (size_t)w * m.NU;
+    double *G = m.G + (size_t)w * m.NU * 16;
+    for (int u = threadIdx.x; u < m.nu[s]; u += blockDim.x) {
+        double al[16], C[16];
+        md_alpha(m, s, u, aa, al);
+        det_cof4(al, C);
+        const double wu = Wt[u];
+#pragma unroll
+        for (int t = 0; t < 16; t++) G[(size_t)u * 16 + t] = wu * C[t];
+    }
+    __syncthreads();
     double *Y = m.Y + (size_t)w * m.NV * m.NM;
     for (int t = threadIdx.x; t < m.nv[s] * nel; t += blockDim.x) {
         const int v = t / nel, j = t % nel, cell = v * m.NM + j;
         double y = 0;
-        for (int e = m.yoff[s][cell]; e < m.yoff[s][cell + 1]; e++) {
-            const int ent = m.yent[s][e], u = ent >> 4;
-            y += Wt[u] * md_cof_entry(m, s, u, (ent >> 2) & 3, ent & 3, aa);
-        }
+        for (int e = m.yoff[s][cell]; e < m.yoff[s][cell + 1]; e++) y += G[m.yent[s][e]];
         Y[cell] = y;
     }
     __syncthreads();
@@ -221,14 +217,18 @@ __device__ void md_zcol(const MdArgs &m, int w, int s, const double *aa, const d
     double *tb = m.tb + (size_t)w * m.NU * 4;
     for (int u = threadIdx.x; u < m.nu[s]; u += blockDim.x) {
         const int r = m.rank[s][u];
+        const int *h = m.hole[s] + 4 * u;
         double al[16], C[16], x[4];
-        md_alpha(m, s, u, r, aa, al);
-        small_det_cof(r, al, C);
-        for (int a = 0; a < r; a++) x[a] = A[(size_t)i * m.NM + m.hole[s][4 * u + a]];
-        for (int b = 0; b < r; b++) {
+        md_alpha(m, s, u, aa, al);
+        det_cof4(al, C);
+#pragma unroll
+        for (int a = 0; a < 4; a++) x[a] = a < r ? A[(size_t)i * m.NM + h[a]] : 0.0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
             double t = 0;
-            for (int a = 0; a < r; a++) t += C[a * 4 + b] * x[a];
-            tb[(size_t)u * 4 + b] = t;
+#pragma unroll
+            for (int a = 0; a < 4; a++) t += C[a * 4 + b] * x[a];
+            if (b < r) tb[(size_t)u * 4 + b] = t;
         }
     }
     __syncthreads();
@@ -604,6 +604,7 @@ int md_init(champ_ctx *c, const champ_system_t *s)
     if ((rc = md_zeros(&c->md_aa, (size_t)W * 4 * NM * NV))) return rc;
     if ((rc = md_zeros(&c->md_D, (size_t)W * 4 * NU))) return rc;
     if ((rc = md_zeros(&c->md_tb, (size_t)W * NU * 4))) return rc;
+    if ((rc = md_zeros(&c->md_G, (size_t)W * NU * 16))) return rc;
     if ((rc = md_zeros(&c->md_S, (size_t)W))) return rc;
     if ((rc = md_zeros(&c->md_Sn, (size_t)W))) return rc;
     if ((rc = md_zeros(&c->md_cur, (size_t)W * 2))) return rc;
@@ -619,7 +620,7 @@ int md_init(champ_ctx *c, const champ_system_t *s)
 
 void md_free(champ_ctx *c)
 {
-    void *ps[] = {c->md_du, c->md_dd, c->md_c, c->md_aa, c->md_D, c->md_tb, c->md_S, c->md_Sn, c->md_cur,
+    void *ps[] = {c->md_du, c->md_dd, c->md_c, c->md_aa, c->md_D, c->md_tb, c->md_G, c->md_S, c->md_Sn, c->md_cur,
                   c->md_vcol, c->md_vcoln, c->md_V, c->md_Y, c->md_Z, c->md_Wt, c->md_g};
     for (void *p : ps) if (p) hipFree(p);
     for (int sp = 0; sp < 2; sp++) {
