@@ -325,6 +325,7 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
         TRY(dalloc(&c->rrT, (size_t)W * NE * NC));
     }
     TRY(dalloc(&c->pair_count, W)); TRY(dalloc(&c->pair_off, (size_t)W + 1)); TRY(dalloc(&c->pair_total, 2));
+    TRY(dalloc(&c->grp_off, (size_t)W + 1));
     TRY(dalloc(&c->nxquad, W));
     HIPCHECK(hipMemset(c->status, 0, sizeof(int) * W));
     HIPCHECK(hipMemset(c->nxquad, 0, sizeof(int) * W));
@@ -360,7 +361,7 @@ extern "C" int champ_gpu_free(champ_ctx *c)
                     c->d2dx2, c->psido, c->psijo, c->psi2o, c->eold, c->ekin, c->pe_local, c->vpsp, c->status,
                     c->rng, c->xq, c->yq, c->zq, c->xnew, c->dfus2o, c->rnew, c->rrnew, c->fsn_row, c->g1n, c->g2n,
                     c->vjnT, c->fsumn, c->aomove, c->orbn, c->ratio, c->psidn, c->psijn, c->vnew, c->vold, c->acc,
-                    c->esum_dev, c->aoall, c->orball, c->wmat, c->pair_count, c->pair_off, c->pair_total,
+                    c->esum_dev, c->aoall, c->orball, c->wmat, c->pair_count, c->pair_off, c->pair_total, c->grp_off,
                     c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->rrT, c->nxquad, c->ri_cent, c->ri_type,
                     c->ri_rad, c->ri_rmax, c->ao_ri, c->gref};
     for (void *p : ptrs) if (p) hipFree(p);

@@ -485,20 +485,24 @@ __global__ void __launch_bounds__(256) k_hpsi_vps(HArgs a)
     if (tid == 0) a.pe_local[w] = pes;
 }
 
-__global__ void k_pair_scan(int W, const int *count, int *off, int *total)
+__global__ void k_pair_scan(int W, const int *count, int *off, int *total, int div)
 {
-    // single block: exclusive scan of the per-walker pair counts
+    // single block: exclusive scan of the per-walker counts of pairs
+    // (div = 1) or of groups of div pairs
     __shared__ int part[1024];
     const int tid = threadIdx.x, bd = blockDim.x;
     const int per = (W + bd - 1) / bd;
     int s = 0;
-    for (int i = 0; i < per; i++) { const int w = tid * per + i; if (w < W) s += count[w]; }
+    for (int i = 0; i < per; i++) { const int w = tid * per + i; if (w < W) s += (count[w] + div - 1) / div; }
     part[tid] = s;
     __syncthreads();
     if (tid == 0) { int acc = 0; for (int t = 0; t < bd; t++) { const int v = part[t]; part[t] = acc; acc += v; } total[0] = acc; }
     __syncthreads();
     int acc = part[tid];
-    for (int i = 0; i < per; i++) { const int w = tid * per + i; if (w < W) { off[w] = acc; acc += count[w]; } }
+    for (int i = 0; i < per; i++) {
+        const int w = tid * per + i;
+        if (w < W) { off[w] = acc; acc += (count[w] + div - 1) / div; }
+    }
     if (tid == 0) off[W] = total[0];
 }
 
@@ -682,7 +686,10 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
         tbegin(c, K_HPSI_NONLOC, &slot);
         hipLaunchKernelGGL(k_hpsi_vps, dim3(c->W), dim3(256), 0, c->stream, a);
         HIPCHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_pair_scan, dim3(1), dim3(1024), 0, c->stream, c->W, c->pair_count, c->pair_off, c->pair_total);
+        hipLaunchKernelGGL(k_pair_scan, dim3(1), dim3(1024), 0, c->stream, c->W, c->pair_count, c->pair_off, c->pair_total, 1);
+        HIPCHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_pair_scan, dim3(1), dim3(1024), 0, c->stream, c->W, c->pair_count, c->grp_off,
+                           c->pair_total + 1, NL_GROUP);
         HIPCHECK(hipGetLastError());
         // wmat[w][i][m] = sum_j slmi[w][spin][i][j] coef[iworbd(j)][m], per spin;
         // multideterminant: wmat[w][i][m] = sum_o V[w][i][o] coef[o][m]

@@ -297,6 +297,380 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
     }
 }
 
+// ------------------------------------------------------------------------
+// Matrix-core form of the quadrature's e-e-n sum.  By the x <-> y symmetry of
+// the polynomial, P(u, x, y) = sum_k u^k sum_n A_kn(y) x^n with the tables
+// A_kn of the partner's distance y (een_poly.hpp).  For the points q of a
+// group of pairs and the partners j of one walker,
+//   M_k[q][j] = sum_{(b, n)} x_qb^n A_kn(y_jb)          (K = centres x powers)
+// is a product of a (points x K) matrix of powers by a (K x partners) matrix
+// of partner tables, and the e-e-n sum of point q is
+//   sum_{j != i} sum_k u_qj^k M_k[q][j].
+// A block takes MMA_GP active pairs of one walker (MMA_GP * NQ <= 48 points,
+// three 16-row tiles), 8 wavefronts.  The items (row k of the tables, 16
+// partners) are dealt to the waves; each item accumulates its three row tiles
+// on v_mfma_f64_16x16x4f64 (lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; accumulator element r holds D[4 r + l / 16][l % 16]) and
+// adds u^k M_k to per-lane point sums.  The e-e and e-n sums and the psi
+// ratio are computed as in k_nonloc.
+constexpr int MMA_GP = NL_GROUP, MMA_MT = 3, MMA_MP = 16 * MMA_MT, MMA_THREADS = 512, MMA_WAVES = MMA_THREADS / WAVE;
+constexpr int MMA_TG = MMA_THREADS / MMA_MP;  // thread groups over (point, partner) in the point phases
+
+// row k of the tables: the non-zero powers n and their packed entries
+template <int N, int K>
+struct EenRow {
+    int ng;
+    int n[N + 1], e[N + 1];
+    constexpr EenRow() : ng(0), n{}, e{}
+    {
+        constexpr EenTable<N> ET{};
+        for (int nn = 0; nn <= N; nn++)
+            if (ET.idx[K][nn] >= 0) { n[ng] = nn; e[ng] = ET.idx[K][nn]; ng++; }
+    }
+};
+
+// polynomial coefficients in y of every table entry: A_e(y) = sum_p poly[e][p] y^p
+template <int N>
+__device__ void een_poly_coef(const double *c, double *poly)
+{
+    constexpr EenTable<N> ET{};
+    constexpr int NA = een_naz<N>();
+    for (int t = 0; t < NA * (N + 1); t++) poly[t] = 0.0;
+    for (int t = 0; t < ET.n; t++)
+        for (int h = 0; h < 2; h++) {
+            const int n = h == 0 ? ET.m[t] : ET.l[t] + ET.m[t];
+            const int pw = h == 0 ? ET.l[t] + ET.m[t] : ET.m[t];
+            poly[ET.idx[ET.k[t]][n] * (N + 1) + pw] += c[t];
+        }
+}
+
+template <int N, int K>
+__device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int npts, const double *POLY, const double *XQ,
+                                         const double *UQ, const int *IPT, const double *rrt, double kapd,
+                                         double part[MMA_MT][4])
+{
+    constexpr EenRow<N, K> R{};
+    constexpr int NA = een_naz<N>();
+    const int lane = threadIdx.x % WAVE, sl = lane / 16, ln = lane % 16;
+    const int NC = a.NC, NE = a.NE, KG = NC * R.ng;
+    const int j = j0 + ln;
+    dbl4 acc[MMA_MT];
+#pragma unroll
+    for (int mt = 0; mt < MMA_MT; mt++) acc[mt] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int k4 = 0; k4 < KG; k4 += 4) {
+        const int kap_i = k4 + sl;
+        const bool kv = kap_i < KG;
+        const int b = kv ? kap_i / R.ng : 0, t = kv ? kap_i % R.ng : 0;
+        int n = 0, e = 0;
+#pragma unroll
+        for (int q = 0; q < R.ng; q++) if (q == t) { n = R.n[q]; e = R.e[q]; }
+        double bv = 0.0;
+        if (kv && j < NE) {
+            const double y = 1.0 - kapd * rrt[(size_t)j * NC + b];
+            const double *pc = POLY + ((size_t)a.iwctype[b] * NA + e) * (N + 1);
+            double v = pc[N];
+#pragma unroll
+            for (int pw = N - 1; pw >= 0; pw--) v = v * y + pc[pw];
+            bv = v;
+        }
+#pragma unroll
+        for (int mt = 0; mt < MMA_MT; mt++) {
+            const int pt = mt * 16 + ln;
+            double av = 0.0;
+            if (kv && pt < npts) {
+                const double x = XQ[(size_t)b * MMA_MP + pt], x2 = x * x;
+                av = n == 0 ? 1.0 : n == 1 ? x : n == 2 ? x2 : n == 3 ? x2 * x : x2 * x2;
+            }
+            acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[mt], 0, 0, 0);
+        }
+    }
+    if (j >= NE) return;
+#pragma unroll
+    for (int mt = 0; mt < MMA_MT; mt++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int pt = mt * 16 + 4 * r + sl;
+            if (pt >= npts || j == IPT[pt]) continue;
+            const double u = UQ[(size_t)j * MMA_MP + pt];
+            double up = 1.0;
+#pragma unroll
+            for (int q = 0; q < K; q++) up *= u;
+            part[mt][r] += up * acc[mt][r];
+        }
+}
+
+template <int N>
+__device__ __forceinline__ void mma_dispatch(int k, const HArgs &a, int w, int j0, int npts, const double *POLY,
+                                             const double *XQ, const double *UQ, const int *IPT, const double *rrt,
+                                             double kapd, double part[MMA_MT][4])
+{
+    constexpr EenTable<N> ET{};
+    switch (k) {
+    case 0: if constexpr (ET.kmax >= 0) mma_item<N, 0>(a, w, j0, npts, POLY, XQ, UQ, IPT, rrt, kapd, part); break;
+    case 1: if constexpr (ET.kmax >= 1) mma_item<N, 1>(a, w, j0, npts, POLY, XQ, UQ, IPT, rrt, kapd, part); break;
+    case 2: if constexpr (ET.kmax >= 2) mma_item<N, 2>(a, w, j0, npts, POLY, XQ, UQ, IPT, rrt, kapd, part); break;
+    case 3: if constexpr (ET.kmax >= 3) mma_item<N, 3>(a, w, j0, npts, POLY, XQ, UQ, IPT, rrt, kapd, part); break;
+    case 4: if constexpr (ET.kmax >= 4) mma_item<N, 4>(a, w, j0, npts, POLY, XQ, UQ, IPT, rrt, kapd, part); break;
+    default: break;
+    }
+}
+
+// LDS of k_nonloc_mma in doubles: POLY, XP, SP, TG partials, SW, FI, IPT/PI (as
+// doubles), XQ, then UQ (phase C aliases GEO + RAD onto UQ)
+__host__ __device__ __forceinline__ size_t mma_lds_doubles(int NE, int NC, int NQ, int NRI, int N)
+{
+    const size_t uq = (size_t)NE * MMA_MP, geo = (size_t)NQ * NC * 5 + (size_t)NQ * NRI;
+    return (size_t)CHAMP_MAXCT * 20 * (N + 1) + 3 * MMA_MP + 3 * MMA_MP + 2 * MMA_TG * MMA_MP + MMA_WAVES * MMA_MP +
+           MMA_GP + MMA_MP + 4 * MMA_GP + (size_t)NC * MMA_MP + (uq > geo ? uq : geo);
+}
+
+template <int NORDC>
+__global__ void __launch_bounds__(MMA_THREADS) k_nonloc_mma(HArgs a)
+{
+    HIP_DYNAMIC_SHARED(double, lds)
+    constexpr EenTable<NORDC> ET{};
+    constexpr int NA = een_naz<NORDC>();
+    static_assert(NA <= 20 && MMA_GP * 12 <= MMA_MP, "table or group sizes");
+    const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
+    const int NE = a.NE, NC = a.NC, NQ = a.NQ, NB = a.NB, NRI = a.NRI;
+    const JasParams &jp = *a.jas;
+    const double kap = jp.kap;
+    double *POLY = lds;                                        // [MAXCT][NA][NORDC+1]
+    double *XP = POLY + (size_t)CHAMP_MAXCT * 20 * (NORDC + 1);  // [MP][3]
+    double *SP = XP + 3 * MMA_MP;                              // [3][MP]: e-e + e-n, e-e-n, psi ratio
+    double *TP = SP + 3 * MMA_MP;                              // [2][TG][MP] partial sums
+    double *SW = TP + 2 * MMA_TG * MMA_MP;                     // [WAVES][MP]
+    double *FI = SW + MMA_WAVES * MMA_MP;                      // [GP]
+    int *IPT = (int *)(FI + MMA_GP);                           // [MP] electron of each point (as ints)
+    int *PI = IPT + 2 * MMA_MP;                                // [GP][4]: i, ic, slot lo, slot hi
+    double *XQ = (double *)(PI + 8 * MMA_GP);                  // [NC][MP] switched point-centre distances
+    double *UQ = XQ + (size_t)NC * MMA_MP;                     // [NE][MP] switched point-partner distances
+    double *GEO = UQ;                                          // phase C: [NQ][NC][5]
+    double *RAD = GEO + (size_t)NQ * NC * 5;                   //          [NQ][NRI]
+
+    if constexpr (NORDC > 1)
+        for (int it = tid; it < a.jas->nctype; it += MMA_THREADS)
+            een_poly_coef<NORDC>(jp.c[it], POLY + (size_t)it * NA * (NORDC + 1));
+
+    const int G = gridDim.x, bk = blockIdx.x;
+    const int nx = (G % 8 == 0) ? 8 : 1, gx = G / nx, xcd = bk % nx, g0 = bk / nx;
+    const int total = a.pair_total[1];
+    const int per_x = (total + nx - 1) / nx;
+    const int x_lo = xcd * per_x, x_hi = min(total, x_lo + per_x);
+
+    for (int gi = x_lo + g0; gi < x_hi; gi += gx) {
+        int lo = 0, hi = a.W - 1;
+        while (lo < hi) {  // walker: largest w with grp_off[w] <= gi
+            const int mid = (lo + hi + 1) / 2;
+            if (a.grp_off[mid] <= gi) lo = mid; else hi = mid - 1;
+        }
+        const int w = lo;
+        const int p0 = MMA_GP * (gi - a.grp_off[w]);
+        const int np = min(MMA_GP, a.pair_count[w] - p0), npts = np * NQ;
+        const double *xw = a.xT + (size_t)w * 3 * NE;
+        const double *rrt = a.rrT + (size_t)w * NE * NC;
+        __syncthreads();  // LDS of the previous group is free
+        if (tid < np) {
+            const size_t slot = (size_t)w * a.cap + p0 + tid;
+            PI[4 * tid] = a.pair_e[slot];
+            PI[4 * tid + 1] = a.pair_c[slot];
+        }
+        __syncthreads();
+        for (int t = tid; t < MMA_MP; t += MMA_THREADS) IPT[t] = t < npts ? PI[4 * (t / NQ)] : -1;
+        for (int t = tid; t < 3 * npts; t += MMA_THREADS) {
+            const int pt = t / 3, k = t % 3, p = pt / NQ, q = pt % NQ;
+            const int i = PI[4 * p], ic = PI[4 * p + 1];
+            const double ren = a.r_en[((size_t)w * NC + ic) * NE + i];
+            const double *qv = (k == 0 ? a.xq : (k == 1 ? a.yq : a.zq)) + (size_t)w * NQ;
+            XP[t] = ren * qv[q] + a.cent[3 * ic + k];
+        }
+        if (wv < np) {  // sum_j F[i][j] of each pair
+            const double *F = a.fso + ((size_t)w * NE + PI[4 * wv]) * NE;
+            double fs = 0.0;
+            for (int j = lane; j < NE; j += WAVE) fs += F[j];
+            fs = wave_sum(fs);
+            if (lane == 0) FI[wv] = fs;
+        }
+        __syncthreads();
+
+        // ---- points: x_qb and the e-n sums; u_qj and the e-e sums
+        {
+            const int pt = tid % MMA_MP, tg = tid / MMA_MP;
+            double sen = 0.0, see = 0.0;
+            if (tg < MMA_TG) {
+                const bool live = pt < npts;
+                const int p = live ? pt / NQ : 0;
+                const int i = live ? PI[4 * p] : 0, ic = live ? PI[4 * p + 1] : 0;
+                const double px = XP[3 * pt], py = XP[3 * pt + 1], pz = XP[3 * pt + 2];
+                for (int b = tg; b < NC; b += MMA_TG) {
+                    double xq = 0.0;
+                    if (live) {
+                        const double e0 = px - a.cent[3 * b], e1 = py - a.cent[3 * b + 1], e2 = pz - a.cent[3 * b + 2];
+                        double r2 = 0;
+                        r2 = r2 + e0 * e0; r2 = r2 + e1 * e1; r2 = r2 + e2 * e2;
+                        const double rq = b == ic ? a.r_en[((size_t)w * NC + ic) * NE + i] : sqrt(r2);
+                        const double rr = scale_r<double>(kap, rq);
+                        xq = 1.0 - kap * rr;
+                        sen += en_term<double>(jp, a.iwctype[b], rr, xq);
+                    }
+                    XQ[(size_t)b * MMA_MP + pt] = xq;
+                }
+                for (int j = tg; j < NE; j += MMA_TG) {
+                    double xs = 0.0;
+                    if (live) {
+                        const double d0 = xw[j] - px, d1 = xw[NE + j] - py, d2 = xw[2 * NE + j] - pz;
+                        double r2 = 0;
+                        r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
+                        const double u = scale_r<double>(kap, sqrt(r2));
+                        xs = 1.0 - kap * u;
+                        if (j != i) see += ee_term<double>(jp, a.NUP, i, j, u, xs);
+                    }
+                    UQ[(size_t)j * MMA_MP + pt] = xs;
+                }
+                TP[tg * MMA_MP + pt] = see;
+                TP[(MMA_TG + tg) * MMA_MP + pt] = sen;
+            }
+        }
+        __syncthreads();
+        for (int pt = tid; pt < MMA_MP; pt += MMA_THREADS) {
+            double v = 0.0;
+            for (int tg = 0; tg < MMA_TG; tg++) v += TP[tg * MMA_MP + pt];
+            for (int tg = 0; tg < MMA_TG; tg++) v += TP[(MMA_TG + tg) * MMA_MP + pt];
+            SP[pt] = v;
+        }
+
+        // ---- e-e-n on the matrix cores: items (row k, 16 partners) over the waves
+        {
+            double part[MMA_MT][4];
+#pragma unroll
+            for (int mt = 0; mt < MMA_MT; mt++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) part[mt][r] = 0.0;
+            if constexpr (NORDC > 1) {
+                const int NT = (NE + 15) / 16, nitem = (ET.kmax + 1) * NT;
+                for (int it = wv; it < nitem; it += MMA_WAVES)
+                    mma_dispatch<NORDC>(it / NT, a, w, (it % NT) * 16, npts, POLY, XQ, UQ, IPT, rrt, kap, part);
+            }
+            // sum over the 16 lanes holding the same rows
+#pragma unroll
+            for (int mt = 0; mt < MMA_MT; mt++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    double v = part[mt][r];
+                    v += __shfl_xor(v, 1, WAVE); v += __shfl_xor(v, 2, WAVE);
+                    v += __shfl_xor(v, 4, WAVE); v += __shfl_xor(v, 8, WAVE);
+                    if (lane % 16 == 0) SW[wv * MMA_MP + mt * 16 + 4 * r + lane / 16] = v;
+                }
+        }
+        __syncthreads();
+        for (int pt = tid; pt < MMA_MP; pt += MMA_THREADS) {
+            double v = 0.0;
+            for (int k = 0; k < MMA_WAVES; k++) v += SW[k * MMA_MP + pt];
+            SP[MMA_MP + pt] = v;
+        }
+
+        // ---- psi ratio of every point, one pair at a time (GEO and RAD over UQ)
+        for (int p = 0; p < np; p++) {
+            const int i = PI[4 * p], ic = PI[4 * p + 1];
+            __syncthreads();  // UQ / the previous pair's GEO and RAD are free
+            for (int t = tid; t < NQ * NC; t += MMA_THREADS) {
+                const int q = t / NC, c = t % NC;
+                const double *xp = XP + 3 * (p * NQ + q);
+                const double e0 = xp[0] - a.cent[3 * c], e1 = xp[1] - a.cent[3 * c + 1], e2 = xp[2] - a.cent[3 * c + 2];
+                double r2 = 0;
+                r2 = r2 + e0 * e0; r2 = r2 + e1 * e1; r2 = r2 + e2 * e2;
+                const double rqd = c == ic ? a.r_en[((size_t)w * NC + ic) * NE + i] : sqrt(r2);
+                double *g = GEO + ((size_t)q * NC + c) * 5;
+                g[0] = e0; g[1] = e1; g[2] = e2; g[3] = rqd;
+                g[4] = (double)grid_jx(a.tab, rqd);
+            }
+            __syncthreads();
+            for (int t = tid; t < NQ * NRI; t += MMA_THREADS) {
+                const int q = t / NRI, ri = t % NRI;
+                const double *g = GEO + ((size_t)q * NC + a.ri_cent[ri]) * 5;
+                double wfv[3] = {0.0, 0.0, 0.0};
+                if (g[3] <= a.ri_rmax[ri]) splfit_jx<double>(a.tab, a.ri_type[ri], a.ri_rad[ri], g[3], (int)g[4], wfv, 0);
+                RAD[(size_t)q * NRI + ri] = wfv[0];
+            }
+            __syncthreads();
+            const double *Wm = a.wmat + ((size_t)w * NE + i) * NB;
+            constexpr int QM = 12;
+            double dr[QM];
+#pragma unroll
+            for (int q = 0; q < QM; q++) dr[q] = 0.0;
+            for (int m = tid; m < NB; m += MMA_THREADS) {
+                const int c = a.ao_cent[m], ri = a.ao_ri[m], l = a.ao_slm[m];
+                const double wm = Wm[m];
+#pragma unroll
+                for (int q = 0; q < QM; q++) {
+                    if (q >= NQ) break;
+                    const double *g = GEO + ((size_t)q * NC + c) * 5;
+                    double y, dy[3], lap;
+                    slm<double>(l, g[0], g[1], g[2], y, dy, lap);
+                    dr[q] += (y * RAD[(size_t)q * NRI + ri]) * wm;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < QM; q++) {
+                if (q >= NQ) break;
+                const double v = wave_sum(dr[q]);
+                if (lane == 0) SW[wv * MMA_MP + q] = v;
+            }
+            __syncthreads();
+            for (int q = tid; q < NQ; q += MMA_THREADS) {
+                double v = 0.0;
+                for (int k = 0; k < MMA_WAVES; k++) v += SW[k * MMA_MP + q];
+                SP[2 * MMA_MP + p * NQ + q] = v;
+            }
+        }
+        __syncthreads();
+
+        // ---- the vpsp_det terms (nonloc.f90:248-274), one thread per pair
+        if (tid < np) {
+            const int p = tid, i = PI[4 * p], ic = PI[4 * p + 1];
+            const size_t slot = (size_t)w * a.cap + p0 + p;
+            const int lp = a.ecp->lpot[a.iwctype[ic]];
+            const double cx = a.cent[3 * ic], cy = a.cent[3 * ic + 1], cz = a.cent[3 * ic + 2];
+            const double rv0 = xw[i] - cx, rv1 = xw[NE + i] - cy, rv2 = xw[2 * NE + i] - cz;
+            const double ri = 1.0 / a.r_en[((size_t)w * NC + ic) * NE + i];
+            const double *qx = a.xq + (size_t)w * NQ, *qy = a.yq + (size_t)w * NQ, *qz = a.zq + (size_t)w * NQ;
+            double pair_sum = 0.0;
+            for (int q = 0; q < NQ; q++) {
+                const int pt = p * NQ + q;
+                const double fsumn = SP[pt] + SP[MMA_MP + pt] - FI[p];
+                double costh = rv0 * qx[q] + rv1 * qy[q] + rv2 * qz[q];
+                costh = costh * ri;
+                double term_radial = 0.0;
+                for (int l = 1; l <= lp - 1; l++)
+                    term_radial = term_radial + yl0(l, costh) * a.pair_vps[slot * CHAMP_MPS_L + l - 1];
+                const double trj = term_radial * a.ecp->wq[q] * exp(fsumn);
+                pair_sum = pair_sum + trj * SP[2 * MMA_MP + pt];
+            }
+            a.pair_val[slot] = pair_sum;
+        }
+    }
+}
+
+template <int NORDC>
+static int launch_mma(champ_ctx *c, const HArgs &a)
+{
+    const size_t lds = mma_lds_doubles(c->NE, c->NC, c->NQ, c->NRI, NORDC) * sizeof(double);
+    if (lds > 160 * 1024) {
+        set_error("nonloc: the quadrature kernel's LDS tables exceed 160 KB for this system");
+        return CHAMP_ERR_UNSUPPORTED;
+    }
+    if (lds > 64 * 1024)
+        HIPCHECK(hipFuncSetAttribute((const void *)k_nonloc_mma<NORDC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int dev = 0, ncu = 256, per_cu = 1;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonloc_mma<NORDC>, MMA_THREADS, lds));
+    per_cu = std::max(1, per_cu);
+    hipLaunchKernelGGL((k_nonloc_mma<NORDC>), dim3(ncu * per_cu), dim3(MMA_THREADS), lds, c->stream, a);
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
 template <int NORDC, int QB, typename T>
 static int launch_k(champ_ctx *c, const HArgs &a)
 {
@@ -335,8 +709,24 @@ static int launch_t(champ_ctx *c, const HArgs &a)
     }
 }
 
+#ifndef CHAMP_NL_MMA
+#define CHAMP_NL_MMA 1
+#endif
+
 int launch_nonloc(champ_ctx *c, const HArgs &a)
 {
+    // the matrix-core form on the FP64 path: up to 12 points per pair (48
+    // points per group) and its LDS within 160 KB
+    const bool mma = CHAMP_NL_MMA && c->prec == CHAMP_PREC_FP64 && c->NQ <= 12 && c->hjas.nordc >= 2 &&
+                     c->hjas.nordc <= 5 && mma_lds_doubles(c->NE, c->NC, c->NQ, c->NRI, c->hjas.nordc) * 8 <= 160 * 1024;
+    if (mma) {
+        switch (c->hjas.nordc) {
+        case 2: return launch_mma<2>(c, a);
+        case 3: return launch_mma<3>(c, a);
+        case 4: return launch_mma<4>(c, a);
+        default: return launch_mma<5>(c, a);
+        }
+    }
     return c->prec == CHAMP_PREC_MIXED ? launch_t<float>(c, a) : launch_t<double>(c, a);
 }
 

@@ -21,6 +21,7 @@ struct HArgs {
     double *psido, *psijo, *psi2o, *eold, *ekin, *pe_local, *vpsp;
     double *aoall, *orball, *wmat;
     int *pair_count, *pair_off, *pair_total, *pair_e, *pair_c, *nxquad;
+    int *grp_off;  // [W+1] groups of NL_GROUP pairs per walker (prefix sums); pair_total[1] = groups
     double *pair_vps, *pair_val;
     double *rrT;  // [W][NE][NC] scaled e-n distances, centre index innermost
     int NRI;      // radial instances (centre, radial function)
@@ -47,7 +48,7 @@ inline HArgs hargs(champ_ctx *c)
     a.d2dx2 = c->d2dx2; a.psido = c->psido; a.psijo = c->psijo; a.psi2o = c->psi2o; a.eold = c->eold; a.ekin = c->ekin;
     a.pe_local = c->pe_local; a.vpsp = c->vpsp; a.aoall = c->aoall; a.orball = c->orball; a.wmat = c->wmat;
     a.pair_count = c->pair_count; a.pair_off = c->pair_off; a.pair_total = c->pair_total; a.pair_e = c->pair_e;
-    a.pair_c = c->pair_c; a.nxquad = c->nxquad; a.pair_vps = c->pair_vps; a.pair_val = c->pair_val;
+    a.pair_c = c->pair_c; a.nxquad = c->nxquad; a.grp_off = c->grp_off; a.pair_vps = c->pair_vps; a.pair_val = c->pair_val;
     a.rrT = c->rrT;
     a.NRI = c->NRI; a.ri_cent = c->ri_cent; a.ri_type = c->ri_type; a.ri_rad = c->ri_rad; a.ao_ri = c->ao_ri;
     a.ri_rmax = c->ri_rmax;
@@ -82,5 +83,6 @@ __device__ __forceinline__ double yl0(int l, double c)
 
 // ECP quadrature over the active (electron, centre) pairs (engine_nonloc.hip)
 int launch_nonloc(champ_ctx *c, const HArgs &a);
+constexpr int NL_GROUP = 4;  // pairs of one walker per block of the matrix-core quadrature kernel
 
 }  // namespace champ

@@ -82,6 +82,8 @@ void run_grid(const char *name, dim3 grid, dim3 block, size_t shmem, void (*thun
     const int nt = block.x * block.y * block.z;
     const long long id = g_launch_id++;
     g_watch_name = name;
+    static const bool trace = getenv("EMU_TRACE") != nullptr;
+    if (trace) fprintf(stderr, "EMU launch %s grid %u block %d lds %zu\n", name, grid.x, nt, shmem);
     g_watch_t0 = now_s();
     g_watch_id = id;
     for (unsigned bz = 0; bz < grid.z; bz++)
