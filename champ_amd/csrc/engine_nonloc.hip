@@ -306,151 +306,114 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
 // is a product of a (points x K) matrix of powers by a (K x partners) matrix
 // of partner tables, and the e-e-n sum of point q is
 //   sum_{j != i} sum_k u_qj^k M_k[q][j].
-// A block takes MMA_GP active pairs of one walker (MMA_GP * NQ <= 48 points,
-// three 16-row tiles), 8 wavefronts.  The items (row k of the tables, 16
-// partners) are dealt to the waves; each item accumulates its three row tiles
-// on v_mfma_f64_16x16x4f64 (lane l supplies A[l % 16][l / 16] and
-// B[l / 16][l % 16]; accumulator element r holds D[4 r + l / 16][l % 16]) and
-// adds u^k M_k to per-lane point sums.  The e-e and e-n sums and the psi
+// A block takes MMA_GP active pairs of one walker (MMA_GP * NQ <= 48 points:
+// three 16-row tiles).  Items (16 partners, one row tile) are dealt to the
+// four waves.  An item walks the centres four at a time: the four K-slots of
+// an MFMA are four centres at one table entry (k, n), so a lane builds all
+// table entries of its (centre, partner) once and feeds one
+// v_mfma_f64_16x16x4f64 per entry into the accumulator of row k (lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16]; element r of the
+// accumulator holds D[4 r + l / 16][l % 16]).  The partner distances are read
+// from rr_en, partners innermost, so the 16 lanes of a slot load one line.
+// The epilogue recomputes u_qj for its 4 x 1 (point, partner) entries and adds
+// sum_k u^k M_k to per-lane point sums.  The e-e and e-n sums and the psi
 // ratio are computed as in k_nonloc.
-constexpr int MMA_GP = NL_GROUP, MMA_MT = 3, MMA_MP = 16 * MMA_MT, MMA_THREADS = 512, MMA_WAVES = MMA_THREADS / WAVE;
-constexpr int MMA_TG = MMA_THREADS / MMA_MP;  // thread groups over (point, partner) in the point phases
+constexpr int MMA_GP = NL_GROUP, MMA_MT = 3, MMA_MP = 16 * MMA_MT, MMA_THREADS = 256, MMA_WAVES = MMA_THREADS / WAVE;
+constexpr int MMA_TG = MMA_THREADS / MMA_MP;  // thread groups over (point, partner) in the point phase
 
-// row k of the tables: the non-zero powers n and their packed entries
-template <int N, int K>
-struct EenRow {
-    int ng;
-    int n[N + 1], e[N + 1];
-    constexpr EenRow() : ng(0), n{}, e{}
-    {
-        constexpr EenTable<N> ET{};
-        for (int nn = 0; nn <= N; nn++)
-            if (ET.idx[K][nn] >= 0) { n[ng] = nn; e[ng] = ET.idx[K][nn]; ng++; }
-    }
-};
-
-// polynomial coefficients in y of every table entry: A_e(y) = sum_p poly[e][p] y^p
 template <int N>
-__device__ void een_poly_coef(const double *c, double *poly)
+__device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int mt, int npts, const double *XP,
+                                         const double *XQ, const int *IPT, double *swv)
 {
     constexpr EenTable<N> ET{};
-    constexpr int NA = een_naz<N>();
-    for (int t = 0; t < NA * (N + 1); t++) poly[t] = 0.0;
-    for (int t = 0; t < ET.n; t++)
-        for (int h = 0; h < 2; h++) {
-            const int n = h == 0 ? ET.m[t] : ET.l[t] + ET.m[t];
-            const int pw = h == 0 ? ET.l[t] + ET.m[t] : ET.m[t];
-            poly[ET.idx[ET.k[t]][n] * (N + 1) + pw] += c[t];
-        }
-}
-
-template <int N, int K>
-__device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int npts, const double *POLY, const double *XQ,
-                                         const double *UQ, const int *IPT, const double *rrt, double kapd,
-                                         double part[MMA_MT][4])
-{
-    constexpr EenRow<N, K> R{};
-    constexpr int NA = een_naz<N>();
+    constexpr int NA = een_naz<N>(), KR = ET.kmax + 1;
     const int lane = threadIdx.x % WAVE, sl = lane / 16, ln = lane % 16;
-    const int NC = a.NC, NE = a.NE, KG = NC * R.ng;
-    const int j = j0 + ln;
-    dbl4 acc[MMA_MT];
+    const int NC = a.NC, NE = a.NE;
+    const JasParams &jp = *a.jas;
+    const double kap = jp.kap;
+    const int j = j0 + ln, pta = mt * 16 + ln;
+    const bool jv = j < NE, pv = pta < npts;
+    const double *rrw = a.rr_en + (size_t)w * NC * NE;
+    dbl4 acc[KR];
 #pragma unroll
-    for (int mt = 0; mt < MMA_MT; mt++) acc[mt] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for (int k4 = 0; k4 < KG; k4 += 4) {
-        const int kap_i = k4 + sl;
-        const bool kv = kap_i < KG;
-        const int b = kv ? kap_i / R.ng : 0, t = kv ? kap_i % R.ng : 0;
-        int n = 0, e = 0;
+    for (int k = 0; k < KR; k++) acc[k] = dbl4{0.0, 0.0, 0.0, 0.0};
+    double ynext = (jv && sl < NC) ? rrw[(size_t)sl * NE + j] : 0.0;
+    for (int b0 = 0; b0 < NC; b0 += 4) {
+        const int b = b0 + sl;
+        const bool bv = b < NC;
+        const double rrb = ynext;
+        if (b + 4 < NC && jv) ynext = rrw[(size_t)(b + 4) * NE + j];  // next centre's distance in flight
+        double tab[NA];
+        const double y = 1.0 - kap * rrb;
+        een_tables<N, 0, double>(jp.c[bv ? a.iwctype[b] : 0], y, tab);
+        const bool live_b = bv && jv, live_a = bv && pv;
+        double xp[N + 1];
+        const double x = live_a ? XQ[(size_t)b * MMA_MP + pta] : 0.0;
+        xp[0] = live_a ? 1.0 : 0.0;
 #pragma unroll
-        for (int q = 0; q < R.ng; q++) if (q == t) { n = R.n[q]; e = R.e[q]; }
-        double bv = 0.0;
-        if (kv && j < NE) {
-            const double y = 1.0 - kapd * rrt[(size_t)j * NC + b];
-            const double *pc = POLY + ((size_t)a.iwctype[b] * NA + e) * (N + 1);
-            double v = pc[N];
+        for (int p = 1; p <= N; p++) xp[p] = xp[p - 1] * x;
 #pragma unroll
-            for (int pw = N - 1; pw >= 0; pw--) v = v * y + pc[pw];
-            bv = v;
-        }
+        for (int k = 0; k < KR; k++)
 #pragma unroll
-        for (int mt = 0; mt < MMA_MT; mt++) {
-            const int pt = mt * 16 + ln;
-            double av = 0.0;
-            if (kv && pt < npts) {
-                const double x = XQ[(size_t)b * MMA_MP + pt], x2 = x * x;
-                av = n == 0 ? 1.0 : n == 1 ? x : n == 2 ? x2 : n == 3 ? x2 * x : x2 * x2;
+            for (int n = 0; n <= N; n++) {
+                const int e = ET.idx[k][n];
+                if (e < 0) continue;
+                acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(xp[n], live_b ? tab[e] : 0.0, acc[k], 0, 0, 0);
             }
-            acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[mt], 0, 0, 0);
-        }
     }
-    if (j >= NE) return;
+    const double *xw = a.xT + (size_t)w * 3 * NE;
+    const double xj = jv ? xw[j] : 0.0, yj = jv ? xw[NE + j] : 0.0, zj = jv ? xw[2 * NE + j] : 0.0;
 #pragma unroll
-    for (int mt = 0; mt < MMA_MT; mt++)
+    for (int r = 0; r < 4; r++) {
+        const int pt = mt * 16 + 4 * r + sl;
+        double v = 0.0;
+        if (jv && pt < npts && j != IPT[pt]) {
+            const double d0 = xj - XP[3 * pt], d1 = yj - XP[3 * pt + 1], d2 = zj - XP[3 * pt + 2];
+            double r2 = 0;
+            r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
+            const double u = 1.0 - kap * scale_r<double>(kap, sqrt(r2));
+            v = acc[KR - 1][r];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int pt = mt * 16 + 4 * r + sl;
-            if (pt >= npts || j == IPT[pt]) continue;
-            const double u = UQ[(size_t)j * MMA_MP + pt];
-            double up = 1.0;
-#pragma unroll
-            for (int q = 0; q < K; q++) up *= u;
-            part[mt][r] += up * acc[mt][r];
+            for (int k = KR - 2; k >= 0; k--) v = v * u + acc[k][r];
         }
-}
-
-template <int N>
-__device__ __forceinline__ void mma_dispatch(int k, const HArgs &a, int w, int j0, int npts, const double *POLY,
-                                             const double *XQ, const double *UQ, const int *IPT, const double *rrt,
-                                             double kapd, double part[MMA_MT][4])
-{
-    constexpr EenTable<N> ET{};
-    switch (k) {
-    case 0: if constexpr (ET.kmax >= 0) mma_item<N, 0>(a, w, j0, npts, POLY, XQ, UQ, IPT, rrt, kapd, part); break;
-    case 1: if constexpr (ET.kmax >= 1) mma_item<N, 1>(a, w, j0, npts, POLY, XQ, UQ, IPT, rrt, kapd, part); break;
-    case 2: if constexpr (ET.kmax >= 2) mma_item<N, 2>(a, w, j0, npts, POLY, XQ, UQ, IPT, rrt, kapd, part); break;
-    case 3: if constexpr (ET.kmax >= 3) mma_item<N, 3>(a, w, j0, npts, POLY, XQ, UQ, IPT, rrt, kapd, part); break;
-    case 4: if constexpr (ET.kmax >= 4) mma_item<N, 4>(a, w, j0, npts, POLY, XQ, UQ, IPT, rrt, kapd, part); break;
-    default: break;
+        // sum over the 16 partners (lanes of equal l / 16) into the wave's point slot
+        v += __shfl_xor(v, 1, WAVE); v += __shfl_xor(v, 2, WAVE);
+        v += __shfl_xor(v, 4, WAVE); v += __shfl_xor(v, 8, WAVE);
+        if (ln == 0) swv[pt] += v;
     }
 }
 
-// LDS of k_nonloc_mma in doubles: POLY, XP, SP, TG partials, SW, FI, IPT/PI (as
-// doubles), XQ, then UQ (phase C aliases GEO + RAD onto UQ)
-__host__ __device__ __forceinline__ size_t mma_lds_doubles(int NE, int NC, int NQ, int NRI, int N)
+// LDS of k_nonloc_mma in doubles: XP, SP, TG partials, SW, FI, IPT/PI (as
+// doubles), XQ, then GEO + RAD of phase C
+__host__ __device__ __forceinline__ size_t mma_lds_doubles(int NC, int NQ, int NRI)
 {
-    const size_t uq = (size_t)NE * MMA_MP, geo = (size_t)NQ * NC * 5 + (size_t)NQ * NRI;
-    return (size_t)CHAMP_MAXCT * 20 * (N + 1) + 3 * MMA_MP + 3 * MMA_MP + 2 * MMA_TG * MMA_MP + MMA_WAVES * MMA_MP +
-           MMA_GP + MMA_MP + 4 * MMA_GP + (size_t)NC * MMA_MP + (uq > geo ? uq : geo);
+    return 3 * MMA_MP + 3 * MMA_MP + 2 * MMA_TG * MMA_MP + MMA_WAVES * MMA_MP + MMA_GP + MMA_MP + 4 * MMA_GP +
+           (size_t)NC * MMA_MP + (size_t)NQ * NC * 5 + (size_t)NQ * NRI;
 }
 
+#ifndef CHAMP_MMA_WPE
+#define CHAMP_MMA_WPE 2
+#endif
 template <int NORDC>
-__global__ void __launch_bounds__(MMA_THREADS) k_nonloc_mma(HArgs a)
+__global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE) k_nonloc_mma(HArgs a)
 {
     HIP_DYNAMIC_SHARED(double, lds)
     constexpr EenTable<NORDC> ET{};
-    constexpr int NA = een_naz<NORDC>();
-    static_assert(NA <= 20 && MMA_GP * 12 <= MMA_MP, "table or group sizes");
+    static_assert(MMA_GP * 12 <= MMA_MP, "group size");
     const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
     const int NE = a.NE, NC = a.NC, NQ = a.NQ, NB = a.NB, NRI = a.NRI;
     const JasParams &jp = *a.jas;
     const double kap = jp.kap;
-    double *POLY = lds;                                        // [MAXCT][NA][NORDC+1]
-    double *XP = POLY + (size_t)CHAMP_MAXCT * 20 * (NORDC + 1);  // [MP][3]
+    double *XP = lds;                                          // [MP][3]
     double *SP = XP + 3 * MMA_MP;                              // [3][MP]: e-e + e-n, e-e-n, psi ratio
     double *TP = SP + 3 * MMA_MP;                              // [2][TG][MP] partial sums
     double *SW = TP + 2 * MMA_TG * MMA_MP;                     // [WAVES][MP]
     double *FI = SW + MMA_WAVES * MMA_MP;                      // [GP]
-    int *IPT = (int *)(FI + MMA_GP);                           // [MP] electron of each point (as ints)
-    int *PI = IPT + 2 * MMA_MP;                                // [GP][4]: i, ic, slot lo, slot hi
+    int *IPT = (int *)(FI + MMA_GP);                           // [MP] electron of each point
+    int *PI = IPT + 2 * MMA_MP;                                // [GP][4]: i, ic
     double *XQ = (double *)(PI + 8 * MMA_GP);                  // [NC][MP] switched point-centre distances
-    double *UQ = XQ + (size_t)NC * MMA_MP;                     // [NE][MP] switched point-partner distances
-    double *GEO = UQ;                                          // phase C: [NQ][NC][5]
+    double *GEO = XQ + (size_t)NC * MMA_MP;                    // phase C: [NQ][NC][5]
     double *RAD = GEO + (size_t)NQ * NC * 5;                   //          [NQ][NRI]
-
-    if constexpr (NORDC > 1)
-        for (int it = tid; it < a.jas->nctype; it += MMA_THREADS)
-            een_poly_coef<NORDC>(jp.c[it], POLY + (size_t)it * NA * (NORDC + 1));
 
     const int G = gridDim.x, bk = blockIdx.x;
     const int nx = (G % 8 == 0) ? 8 : 1, gx = G / nx, xcd = bk % nx, g0 = bk / nx;
@@ -468,7 +431,6 @@ __global__ void __launch_bounds__(MMA_THREADS) k_nonloc_mma(HArgs a)
         const int p0 = MMA_GP * (gi - a.grp_off[w]);
         const int np = min(MMA_GP, a.pair_count[w] - p0), npts = np * NQ;
         const double *xw = a.xT + (size_t)w * 3 * NE;
-        const double *rrt = a.rrT + (size_t)w * NE * NC;
         __syncthreads();  // LDS of the previous group is free
         if (tid < np) {
             const size_t slot = (size_t)w * a.cap + p0 + tid;
@@ -493,7 +455,7 @@ __global__ void __launch_bounds__(MMA_THREADS) k_nonloc_mma(HArgs a)
         }
         __syncthreads();
 
-        // ---- points: x_qb and the e-n sums; u_qj and the e-e sums
+        // ---- points: x_qb and the e-n sums, the e-e sums
         {
             const int pt = tid % MMA_MP, tg = tid / MMA_MP;
             double sen = 0.0, see = 0.0;
@@ -501,7 +463,8 @@ __global__ void __launch_bounds__(MMA_THREADS) k_nonloc_mma(HArgs a)
                 const bool live = pt < npts;
                 const int p = live ? pt / NQ : 0;
                 const int i = live ? PI[4 * p] : 0, ic = live ? PI[4 * p + 1] : 0;
-                const double px = XP[3 * pt], py = XP[3 * pt + 1], pz = XP[3 * pt + 2];
+                const double px = live ? XP[3 * pt] : 0.0, py = live ? XP[3 * pt + 1] : 0.0;
+                const double pz = live ? XP[3 * pt + 2] : 0.0;
                 for (int b = tg; b < NC; b += MMA_TG) {
                     double xq = 0.0;
                     if (live) {
@@ -515,18 +478,15 @@ __global__ void __launch_bounds__(MMA_THREADS) k_nonloc_mma(HArgs a)
                     }
                     XQ[(size_t)b * MMA_MP + pt] = xq;
                 }
-                for (int j = tg; j < NE; j += MMA_TG) {
-                    double xs = 0.0;
-                    if (live) {
+                if (live)
+                    for (int j = tg; j < NE; j += MMA_TG) {
+                        if (j == i) continue;
                         const double d0 = xw[j] - px, d1 = xw[NE + j] - py, d2 = xw[2 * NE + j] - pz;
                         double r2 = 0;
                         r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
                         const double u = scale_r<double>(kap, sqrt(r2));
-                        xs = 1.0 - kap * u;
-                        if (j != i) see += ee_term<double>(jp, a.NUP, i, j, u, xs);
+                        see += ee_term<double>(jp, a.NUP, i, j, u, 1.0 - kap * u);
                     }
-                    UQ[(size_t)j * MMA_MP + pt] = xs;
-                }
                 TP[tg * MMA_MP + pt] = see;
                 TP[(MMA_TG + tg) * MMA_MP + pt] = sen;
             }
@@ -539,28 +499,20 @@ __global__ void __launch_bounds__(MMA_THREADS) k_nonloc_mma(HArgs a)
             SP[pt] = v;
         }
 
-        // ---- e-e-n on the matrix cores: items (row k, 16 partners) over the waves
+        // ---- e-e-n on the matrix cores: items (16 partners, one row tile) over the
+        // waves; each wave sums its items into its own row of SW
         {
-            double part[MMA_MT][4];
-#pragma unroll
-            for (int mt = 0; mt < MMA_MT; mt++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) part[mt][r] = 0.0;
+            double *swv = SW + wv * MMA_MP;
+            for (int pt = lane; pt < MMA_MP; pt += WAVE) swv[pt] = 0.0;
+            __syncwarp();
             if constexpr (NORDC > 1) {
-                const int NT = (NE + 15) / 16, nitem = (ET.kmax + 1) * NT;
-                for (int it = wv; it < nitem; it += MMA_WAVES)
-                    mma_dispatch<NORDC>(it / NT, a, w, (it % NT) * 16, npts, POLY, XQ, UQ, IPT, rrt, kap, part);
-            }
-            // sum over the 16 lanes holding the same rows
-#pragma unroll
-            for (int mt = 0; mt < MMA_MT; mt++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    double v = part[mt][r];
-                    v += __shfl_xor(v, 1, WAVE); v += __shfl_xor(v, 2, WAVE);
-                    v += __shfl_xor(v, 4, WAVE); v += __shfl_xor(v, 8, WAVE);
-                    if (lane % 16 == 0) SW[wv * MMA_MP + mt * 16 + 4 * r + lane / 16] = v;
+                const int NT = (NE + 15) / 16, MTU = (npts + 15) / 16;
+                for (int it = wv; it < NT * MTU; it += MMA_WAVES) {
+                    const int mt = it % MTU;
+                    mma_item<NORDC>(a, w, (it / MTU) * 16, mt, npts, XP, XQ, IPT, swv);
+                    __syncwarp();
                 }
+            }
         }
         __syncthreads();
         for (int pt = tid; pt < MMA_MP; pt += MMA_THREADS) {
@@ -569,10 +521,10 @@ __global__ void __launch_bounds__(MMA_THREADS) k_nonloc_mma(HArgs a)
             SP[MMA_MP + pt] = v;
         }
 
-        // ---- psi ratio of every point, one pair at a time (GEO and RAD over UQ)
+        // ---- psi ratio of every point, one pair at a time
         for (int p = 0; p < np; p++) {
             const int i = PI[4 * p], ic = PI[4 * p + 1];
-            __syncthreads();  // UQ / the previous pair's GEO and RAD are free
+            __syncthreads();  // the previous pair's GEO, RAD and SW are free
             for (int t = tid; t < NQ * NC; t += MMA_THREADS) {
                 const int q = t / NC, c = t % NC;
                 const double *xp = XP + 3 * (p * NQ + q);
@@ -654,7 +606,7 @@ __global__ void __launch_bounds__(MMA_THREADS) k_nonloc_mma(HArgs a)
 template <int NORDC>
 static int launch_mma(champ_ctx *c, const HArgs &a)
 {
-    const size_t lds = mma_lds_doubles(c->NE, c->NC, c->NQ, c->NRI, NORDC) * sizeof(double);
+    const size_t lds = mma_lds_doubles(c->NC, c->NQ, c->NRI) * sizeof(double);
     if (lds > 160 * 1024) {
         set_error("nonloc: the quadrature kernel's LDS tables exceed 160 KB for this system");
         return CHAMP_ERR_UNSUPPORTED;
@@ -718,7 +670,7 @@ int launch_nonloc(champ_ctx *c, const HArgs &a)
     // the matrix-core form on the FP64 path: up to 12 points per pair (48
     // points per group) and its LDS within 160 KB
     const bool mma = CHAMP_NL_MMA && c->prec == CHAMP_PREC_FP64 && c->NQ <= 12 && c->hjas.nordc >= 2 &&
-                     c->hjas.nordc <= 5 && mma_lds_doubles(c->NE, c->NC, c->NQ, c->NRI, c->hjas.nordc) * 8 <= 160 * 1024;
+                     c->hjas.nordc <= 5 && mma_lds_doubles(c->NC, c->NQ, c->NRI) * 8 <= 160 * 1024;
     if (mma) {
         switch (c->hjas.nordc) {
         case 2: return launch_mma<2>(c, a);
