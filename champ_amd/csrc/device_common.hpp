@@ -66,9 +66,10 @@ __device__ __forceinline__ T powi(T x, int m)
 #define CHAMP_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
 #endif
 
-// four doubles: the accumulator operand of the FP64 MFMA instruction
+// four doubles / floats: the accumulator operands of the FP64 / FP32 MFMA instructions
 #ifndef CHAMP_DBL4_DEFINED
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 #endif
 
 // ------------------------------------------------------------------- tables

@@ -321,12 +321,30 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
 constexpr int MMA_GP = NL_GROUP, MMA_MT = 3, MMA_MP = 16 * MMA_MT, MMA_THREADS = 256, MMA_WAVES = MMA_THREADS / WAVE;
 constexpr int MMA_TG = MMA_THREADS / MMA_MP;  // thread groups over (point, partner) in the point phase
 
-template <int N>
+// accumulator type and instruction per arithmetic type; row of accumulator
+// element r in lane slot sl = l / 16 (measured on gfx950 with one-hot
+// operands: f64 D[4 r + l / 16][l % 16], f32 D[4 (l / 16) + r][l % 16])
+template <typename T> struct Mma;
+template <> struct Mma<double> {
+    typedef dbl4 acc;
+    static __device__ __forceinline__ acc zero() { return dbl4{0.0, 0.0, 0.0, 0.0}; }
+    static __device__ __forceinline__ acc op(double a, double b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int r, int sl) { return 4 * r + sl; }
+};
+template <> struct Mma<float> {
+    typedef flt4 acc;
+    static __device__ __forceinline__ acc zero() { return flt4{0.0f, 0.0f, 0.0f, 0.0f}; }
+    static __device__ __forceinline__ acc op(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int r, int sl) { return 4 * sl + r; }
+};
+
+template <int N, typename T>
 __device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int mt, int npts, const double *XP,
-                                         const double *XQ, const int *IPT, double *swv)
+                                         const T *XQ, const int *IPT, double *swv)
 {
     constexpr EenTable<N> ET{};
     constexpr int NA = een_naz<N>(), KR = ET.kmax + 1;
+    typedef typename Mma<T>::acc Acc;
     const int lane = threadIdx.x % WAVE, sl = lane / 16, ln = lane % 16;
     const int NC = a.NC, NE = a.NE;
     const JasParams &jp = *a.jas;
@@ -334,22 +352,21 @@ __device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int mt, 
     const int j = j0 + ln, pta = mt * 16 + ln;
     const bool jv = j < NE, pv = pta < npts;
     const double *rrw = a.rr_en + (size_t)w * NC * NE;
-    dbl4 acc[KR];
+    Acc acc[KR];
 #pragma unroll
-    for (int k = 0; k < KR; k++) acc[k] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < KR; k++) acc[k] = Mma<T>::zero();
     double ynext = (jv && sl < NC) ? rrw[(size_t)sl * NE + j] : 0.0;
     for (int b0 = 0; b0 < NC; b0 += 4) {
         const int b = b0 + sl;
         const bool bv = b < NC;
         const double rrb = ynext;
         if (b + 4 < NC && jv) ynext = rrw[(size_t)(b + 4) * NE + j];  // next centre's distance in flight
-        double tab[NA];
-        const double y = 1.0 - kap * rrb;
-        een_tables<N, 0, double>(jp.c[bv ? a.iwctype[b] : 0], y, tab);
+        T tab[NA];
+        een_tables<N, 0, T>(jp.c[bv ? a.iwctype[b] : 0], (T)(1.0 - kap * rrb), tab);
         const bool live_b = bv && jv, live_a = bv && pv;
-        double xp[N + 1];
-        const double x = live_a ? XQ[(size_t)b * MMA_MP + pta] : 0.0;
-        xp[0] = live_a ? 1.0 : 0.0;
+        T xp[N + 1];
+        const T x = live_a ? XQ[(size_t)b * MMA_MP + pta] : T(0);
+        xp[0] = live_a ? T(1) : T(0);
 #pragma unroll
         for (int p = 1; p <= N; p++) xp[p] = xp[p - 1] * x;
 #pragma unroll
@@ -358,23 +375,26 @@ __device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int mt, 
             for (int n = 0; n <= N; n++) {
                 const int e = ET.idx[k][n];
                 if (e < 0) continue;
-                acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(xp[n], live_b ? tab[e] : 0.0, acc[k], 0, 0, 0);
+                acc[k] = Mma<T>::op(xp[n], live_b ? tab[e] : T(0), acc[k]);
             }
     }
     const double *xw = a.xT + (size_t)w * 3 * NE;
     const double xj = jv ? xw[j] : 0.0, yj = jv ? xw[NE + j] : 0.0, zj = jv ? xw[2 * NE + j] : 0.0;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        const int pt = mt * 16 + 4 * r + sl;
+        const int pt = mt * 16 + Mma<T>::row(r, sl);
         double v = 0.0;
         if (jv && pt < npts && j != IPT[pt]) {
-            const double d0 = xj - XP[3 * pt], d1 = yj - XP[3 * pt + 1], d2 = zj - XP[3 * pt + 2];
-            double r2 = 0;
+            // the switched e-e distance as the point phase forms it
+            const T d0 = (T)(xj - XP[3 * pt]), d1 = (T)(yj - XP[3 * pt + 1]), d2 = (T)(zj - XP[3 * pt + 2]);
+            T r2 = 0;
             r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
-            const double u = 1.0 - kap * scale_r<double>(kap, sqrt(r2));
-            v = acc[KR - 1][r];
+            const T rjq = sqrt(r2);
+            const T u = switched_from<T>(kap, rjq, scale_r<T>((T)kap, rjq));
+            T pv2 = acc[KR - 1][r];
 #pragma unroll
-            for (int k = KR - 2; k >= 0; k--) v = v * u + acc[k][r];
+            for (int k = KR - 2; k >= 0; k--) pv2 = pv2 * u + acc[k][r];
+            v = (double)pv2;
         }
         // sum over the 16 partners (lanes of equal l / 16) into the wave's point slot
         v += __shfl_xor(v, 1, WAVE); v += __shfl_xor(v, 2, WAVE);
@@ -384,17 +404,19 @@ __device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int mt, 
 }
 
 // LDS of k_nonloc_mma in doubles: XP, SP, TG partials, SW, FI, IPT/PI (as
-// doubles), XQ, then GEO + RAD of phase C
-__host__ __device__ __forceinline__ size_t mma_lds_doubles(int NC, int NQ, int NRI)
+// doubles), XQ (tsize bytes per entry), then GEO + RAD of phase C
+__host__ __device__ __forceinline__ size_t mma_lds_doubles(int NC, int NQ, int NRI, int tsize)
 {
     return 3 * MMA_MP + 3 * MMA_MP + 2 * MMA_TG * MMA_MP + MMA_WAVES * MMA_MP + MMA_GP + MMA_MP + 4 * MMA_GP +
-           (size_t)NC * MMA_MP + (size_t)NQ * NC * 5 + (size_t)NQ * NRI;
+           ((size_t)NC * MMA_MP * tsize + 7) / 8 + (size_t)NQ * NC * 5 + (size_t)NQ * NRI;
 }
 
 #ifndef CHAMP_MMA_WPE
 #define CHAMP_MMA_WPE 2
 #endif
-template <int NORDC>
+// T: arithmetic type of the Jastrow terms and of the matrix cores (double on
+// the FP64 path, float on the mixed path); sums and the psi ratio are double.
+template <int NORDC, typename T>
 __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE) k_nonloc_mma(HArgs a)
 {
     HIP_DYNAMIC_SHARED(double, lds)
@@ -403,7 +425,8 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
     const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
     const int NE = a.NE, NC = a.NC, NQ = a.NQ, NB = a.NB, NRI = a.NRI;
     const JasParams &jp = *a.jas;
-    const double kap = jp.kap;
+    const double kapd = jp.kap;
+    const T kap = (T)jp.kap;
     double *XP = lds;                                          // [MP][3]
     double *SP = XP + 3 * MMA_MP;                              // [3][MP]: e-e + e-n, e-e-n, psi ratio
     double *TP = SP + 3 * MMA_MP;                              // [2][TG][MP] partial sums
@@ -411,8 +434,8 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
     double *FI = SW + MMA_WAVES * MMA_MP;                      // [GP]
     int *IPT = (int *)(FI + MMA_GP);                           // [MP] electron of each point
     int *PI = IPT + 2 * MMA_MP;                                // [GP][4]: i, ic
-    double *XQ = (double *)(PI + 8 * MMA_GP);                  // [NC][MP] switched point-centre distances
-    double *GEO = XQ + (size_t)NC * MMA_MP;                    // phase C: [NQ][NC][5]
+    T *XQ = (T *)(PI + 8 * MMA_GP);                            // [NC][MP] switched point-centre distances
+    double *GEO = (double *)PI + 4 * MMA_GP + ((size_t)NC * MMA_MP * sizeof(T) + 7) / 8;  // phase C: [NQ][NC][5]
     double *RAD = GEO + (size_t)NQ * NC * 5;                   //          [NQ][NRI]
 
     const int G = gridDim.x, bk = blockIdx.x;
@@ -466,26 +489,27 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
                 const double px = live ? XP[3 * pt] : 0.0, py = live ? XP[3 * pt + 1] : 0.0;
                 const double pz = live ? XP[3 * pt + 2] : 0.0;
                 for (int b = tg; b < NC; b += MMA_TG) {
-                    double xq = 0.0;
+                    T xq = T(0);
                     if (live) {
-                        const double e0 = px - a.cent[3 * b], e1 = py - a.cent[3 * b + 1], e2 = pz - a.cent[3 * b + 2];
-                        double r2 = 0;
+                        const T e0 = (T)(px - a.cent[3 * b]), e1 = (T)(py - a.cent[3 * b + 1]), e2 = (T)(pz - a.cent[3 * b + 2]);
+                        T r2 = 0;
                         r2 = r2 + e0 * e0; r2 = r2 + e1 * e1; r2 = r2 + e2 * e2;
-                        const double rq = b == ic ? a.r_en[((size_t)w * NC + ic) * NE + i] : sqrt(r2);
-                        const double rr = scale_r<double>(kap, rq);
-                        xq = 1.0 - kap * rr;
-                        sen += en_term<double>(jp, a.iwctype[b], rr, xq);
+                        const T rq = b == ic ? (T)a.r_en[((size_t)w * NC + ic) * NE + i] : sqrt(r2);
+                        const T rr = scale_r<T>(kap, rq);
+                        xq = switched_from<T>(kapd, rq, rr);
+                        sen += (double)en_term<T>(jp, a.iwctype[b], rr, xq);
                     }
                     XQ[(size_t)b * MMA_MP + pt] = xq;
                 }
                 if (live)
                     for (int j = tg; j < NE; j += MMA_TG) {
                         if (j == i) continue;
-                        const double d0 = xw[j] - px, d1 = xw[NE + j] - py, d2 = xw[2 * NE + j] - pz;
-                        double r2 = 0;
+                        const T d0 = (T)(xw[j] - px), d1 = (T)(xw[NE + j] - py), d2 = (T)(xw[2 * NE + j] - pz);
+                        T r2 = 0;
                         r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
-                        const double u = scale_r<double>(kap, sqrt(r2));
-                        see += ee_term<double>(jp, a.NUP, i, j, u, 1.0 - kap * u);
+                        const T rjq = sqrt(r2);
+                        const T u = scale_r<T>(kap, rjq);
+                        see += (double)ee_term<T>(jp, a.NUP, i, j, u, switched_from<T>(kapd, rjq, u));
                     }
                 TP[tg * MMA_MP + pt] = see;
                 TP[(MMA_TG + tg) * MMA_MP + pt] = sen;
@@ -509,7 +533,7 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
                 const int NT = (NE + 15) / 16, MTU = (npts + 15) / 16;
                 for (int it = wv; it < NT * MTU; it += MMA_WAVES) {
                     const int mt = it % MTU;
-                    mma_item<NORDC>(a, w, (it / MTU) * 16, mt, npts, XP, XQ, IPT, swv);
+                    mma_item<NORDC, T>(a, w, (it / MTU) * 16, mt, npts, XP, XQ, IPT, swv);
                     __syncwarp();
                 }
             }
@@ -603,22 +627,22 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
     }
 }
 
-template <int NORDC>
+template <int NORDC, typename T>
 static int launch_mma(champ_ctx *c, const HArgs &a)
 {
-    const size_t lds = mma_lds_doubles(c->NC, c->NQ, c->NRI) * sizeof(double);
+    const size_t lds = mma_lds_doubles(c->NC, c->NQ, c->NRI, (int)sizeof(T)) * sizeof(double);
     if (lds > 160 * 1024) {
         set_error("nonloc: the quadrature kernel's LDS tables exceed 160 KB for this system");
         return CHAMP_ERR_UNSUPPORTED;
     }
     if (lds > 64 * 1024)
-        HIPCHECK(hipFuncSetAttribute((const void *)k_nonloc_mma<NORDC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        HIPCHECK(hipFuncSetAttribute((const void *)k_nonloc_mma<NORDC, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int dev = 0, ncu = 256, per_cu = 1;
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonloc_mma<NORDC>, MMA_THREADS, lds));
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonloc_mma<NORDC, T>, MMA_THREADS, lds));
     per_cu = std::max(1, per_cu);
-    hipLaunchKernelGGL((k_nonloc_mma<NORDC>), dim3(ncu * per_cu), dim3(MMA_THREADS), lds, c->stream, a);
+    hipLaunchKernelGGL((k_nonloc_mma<NORDC, T>), dim3(ncu * per_cu), dim3(MMA_THREADS), lds, c->stream, a);
     HIPCHECK(hipGetLastError());
     return CHAMP_OK;
 }
@@ -667,16 +691,25 @@ static int launch_t(champ_ctx *c, const HArgs &a)
 
 int launch_nonloc(champ_ctx *c, const HArgs &a)
 {
-    // the matrix-core form on the FP64 path: up to 12 points per pair (48
-    // points per group) and its LDS within 160 KB
-    const bool mma = CHAMP_NL_MMA && c->prec == CHAMP_PREC_FP64 && c->NQ <= 12 && c->hjas.nordc >= 2 &&
-                     c->hjas.nordc <= 5 && mma_lds_doubles(c->NC, c->NQ, c->NRI) * 8 <= 160 * 1024;
+    // the matrix-core form: up to 12 points per pair (48 points per group) and
+    // its LDS within 160 KB
+    const bool mixed = c->prec == CHAMP_PREC_MIXED;
+    const bool mma = CHAMP_NL_MMA && c->NQ <= 12 && c->hjas.nordc >= 2 && c->hjas.nordc <= 5 &&
+                     mma_lds_doubles(c->NC, c->NQ, c->NRI, mixed ? 4 : 8) * 8 <= 160 * 1024;
+    if (mma && mixed) {
+        switch (c->hjas.nordc) {
+        case 2: return launch_mma<2, float>(c, a);
+        case 3: return launch_mma<3, float>(c, a);
+        case 4: return launch_mma<4, float>(c, a);
+        default: return launch_mma<5, float>(c, a);
+        }
+    }
     if (mma) {
         switch (c->hjas.nordc) {
-        case 2: return launch_mma<2>(c, a);
-        case 3: return launch_mma<3>(c, a);
-        case 4: return launch_mma<4>(c, a);
-        default: return launch_mma<5>(c, a);
+        case 2: return launch_mma<2, double>(c, a);
+        case 3: return launch_mma<3, double>(c, a);
+        case 4: return launch_mma<4, double>(c, a);
+        default: return launch_mma<5, double>(c, a);
         }
     }
     return c->prec == CHAMP_PREC_MIXED ? launch_t<float>(c, a) : launch_t<double>(c, a);

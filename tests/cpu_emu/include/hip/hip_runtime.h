@@ -141,6 +141,34 @@ inline dbl4 __builtin_amdgcn_mfma_f64_16x16x4f64(double a, double b, dbl4 c, int
     }
     return c;
 }
+// FP32 MFMA 16x16x4: A and B as above; accumulator element r of lane l holds
+// D[4 (l / 16) + r][l % 16] (measured on gfx950 likewise)
+struct flt4 {
+    float v[4];
+    float &operator[](int i) { return v[i]; }
+    const float &operator[](int i) const { return v[i]; }
+};
+inline flt4 __builtin_amdgcn_mfma_f32_16x16x4f32(float a, float b, flt4 c, int, int, int)
+{
+    uint64_t ua = 0, ub = 0, all_a[64], all_b[64];
+    memcpy(&ua, &a, 4);
+    memcpy(&ub, &b, 4);
+    emu::wave_exchange(ua, all_a);
+    emu::wave_exchange(ub, all_b);
+    const int l = emu::tctx.lane % 64, j = l % 16;
+    for (int r = 0; r < 4; r++) {
+        const int i = 4 * (l / 16) + r;
+        float s = c[r];
+        for (int k = 0; k < 4; k++) {
+            float x, y;
+            memcpy(&x, &all_a[k * 16 + i], 4);
+            memcpy(&y, &all_b[k * 16 + j], 4);
+            s += x * y;
+        }
+        c[r] = s;
+    }
+    return c;
+}
 inline int atomicAdd(int *p, int v) { return emu::atomic_add_i(p, v); }
 inline double atomicAdd(double *p, double v) { return emu::atomic_add_d(p, v); }
 inline int atomicOr(int *p, int v) { return emu::atomic_or_i(p, v); }
