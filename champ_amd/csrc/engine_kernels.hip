@@ -106,14 +106,13 @@ __global__ void __launch_bounds__(256) k_drift_old(MoveArgs a)
 }
 
 // ------------------------------------------------------------------------
-// Move kernel A, one block of four wavefronts per walker (the population
-// alone gives one wave per SIMD; four per walker hide the pair loop's latency):
+// Move kernel A, one wavefront per walker:
 //   mode 0: metrop1 proposal (drift at old position, 3 gauss draws;
 //           metrop_mov1_driftdif.f90:105-117); mode 1: xnew given.
 //   then distances(iel) (distances.f90:3-167), jastrow4e with iflag=0
 //   (jastrow4e.f90:3-358) and basis_fns(iel, ider=1) (basis_fns.f90:8-214)
 //   writing the AO block that the MO GEMM contracts (orbitalse).
-// Tuning of k_move_a: partner-electron slots per thread in the pair loop and
+// Tuning of k_move_a: partner-electron slots per lane in the pair loop and
 // the minimum waves per SIMD asked of the register allocator.
 #ifndef CHAMP_MJB
 #define CHAMP_MJB 1
@@ -122,12 +121,11 @@ __global__ void __launch_bounds__(256) k_drift_old(MoveArgs a)
 #define CHAMP_MOVE_WPE 1
 #endif
 constexpr int MJB = CHAMP_MJB;
-constexpr int MOVE_WAVES = 4;  // waves of k_move_a per walker
 
-// LDS of k_move_a (one walker per block), in doubles: r, rr,
-// -kap (1 - kap rr) / r, rvec (3) per centre (double), then in the arithmetic
-// type (tsize bytes) the centre's e-e-n tables (value and d/dx, na entries
-// each) and R, R' of each radial instance at the moved electron
+// LDS of one wave of k_move_a, in doubles: r, rr, -kap (1 - kap rr) / r,
+// rvec (3) per centre (double), then in the arithmetic type (tsize bytes) the
+// centre's e-e-n tables (value and d/dx, na entries each) and R, R' of each
+// radial instance at the moved electron
 __host__ __device__ __forceinline__ int move_a_lds_doubles(int NC, int na, int NRI, int tsize)
 {
     return 6 * NC + ((2 * na * NC + 2 * NRI) * tsize + 7) / 8;
@@ -140,23 +138,22 @@ template <int NORDC, typename T>
 __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move_a(MoveArgs a)
 {
     HIP_DYNAMIC_SHARED(double, lds)
-    __shared__ double s_xn[3];
-    __shared__ double s_red[4][MOVE_WAVES];
     constexpr int NA = een_naz<NORDC>();
-    const int tid = threadIdx.x, wv = tid / WAVE, lane = tid % WAVE;
-    const int w = blockIdx.x;
+    const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int w = blockIdx.x * (blockDim.x / WAVE) + wv;
+    const bool active = w < a.W;
     const int NE = a.NE, NC = a.NC, iel = a.iel;
     const JasParams &jp = *a.jas;
     const double kap = jp.kap, asymp_r = jp.asymp_r;
-    double *Lr = lds;
+    double *Lr = lds + (size_t)wv * move_a_lds_doubles(NC, NORDC > 1 ? NA : 0, a.NRI, (int)sizeof(T));
     double *Lrr = Lr + NC, *Lfm = Lr + 2 * NC, *Lrv = Lr + 3 * NC;
     T *Ltab = (T *)(Lr + 6 * NC);
     T *Lrad = Ltab + (NORDC > 1 ? 2 * NA * NC : 0);
 
     double xn0 = 0, xn1 = 0, xn2 = 0;
-    {
+    if (active) {
         const double *xw = a.xT + (size_t)w * 3 * NE;
-        if (a.mode == 0 && wv == 0) {
+        if (a.mode == 0) {
             double vo[3];
             drift_old(a, w, lane, vo);
             if (lane == 0) {
@@ -174,21 +171,21 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                     xn[k] = xw[k * NE + iel] + dx;
                 }
                 for (int k = 0; k < 4; k++) a.rng[4 * (size_t)w + k] = s[k];
-                for (int k = 0; k < 3; k++) { a.xnew[3 * w + k] = xn[k]; a.vold[3 * w + k] = vo[k]; s_xn[k] = xn[k]; }
+                for (int k = 0; k < 3; k++) { a.xnew[3 * w + k] = xn[k]; a.vold[3 * w + k] = vo[k]; }
                 a.dfus2o[w] = dfus2o;
+                xn0 = xn[0]; xn1 = xn[1]; xn2 = xn[2];
             }
-        } else if (a.mode != 0 && tid < 3) {
-            s_xn[tid] = a.xnew[3 * w + tid];
+            xn0 = __shfl(xn0, 0, WAVE); xn1 = __shfl(xn1, 0, WAVE); xn2 = __shfl(xn2, 0, WAVE);
+        } else {
+            xn0 = a.xnew[3 * w]; xn1 = a.xnew[3 * w + 1]; xn2 = a.xnew[3 * w + 2];
         }
-        __syncthreads();
-        xn0 = s_xn[0]; xn1 = s_xn[1]; xn2 = s_xn[2];
     }
 
     // ---- e-n distances of the moved electron + its e-n Jastrow term; per
     // centre the e-e-n coefficient tables at the moved electron (een_poly.hpp)
     double fs_en = 0, ge0 = 0, ge1 = 0, ge2 = 0;
-    {
-        for (int c = tid; c < NC; c += blockDim.x) {
+    if (active) {
+        for (int c = lane; c < NC; c += WAVE) {
             const double rv0 = xn0 - a.cent[3 * c], rv1 = xn1 - a.cent[3 * c + 1], rv2 = xn2 - a.cent[3 * c + 2];
             double r2 = 0;
             r2 = r2 + rv0 * rv0; r2 = r2 + rv1 * rv1; r2 = r2 + rv2 * rv2;
@@ -220,14 +217,10 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
             }
         }
         fs_en = wave_sum(fs_en); ge0 = wave_sum(ge0); ge1 = wave_sum(ge1); ge2 = wave_sum(ge2);
-        if (lane == 0) { s_red[0][wv] = fs_en; s_red[1][wv] = ge0; s_red[2][wv] = ge1; s_red[3][wv] = ge2; }
-        __syncthreads();
-        fs_en = 0; ge0 = 0; ge1 = 0; ge2 = 0;
-        for (int k = 0; k < MOVE_WAVES; k++) { fs_en += s_red[0][k]; ge0 += s_red[1][k]; ge1 += s_red[2][k]; ge2 += s_red[3][k]; }
     }
     __syncthreads();
 
-    {
+    if (active) {
         const double *xw = a.xT + (size_t)w * 3 * NE;
         const double *fso_row = a.fso + ((size_t)w * NE + iel) * NE;
         const double *g1_row = a.g1 + ((size_t)w * NE + iel) * 3 * NE;
@@ -238,25 +231,23 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
         double *g1n = a.g1n + (size_t)w * 3 * NE, *g2n = a.g2n + (size_t)w * 3 * NE;
         const double *rw = a.r_en + (size_t)w * NC * NE, *rrw = a.rr_en + (size_t)w * NC * NE;
         double dfs = 0, dg0 = 0, dg1 = 0, dg2 = 0;
-        if (tid == 0) {  // the moved electron's own entries: its e-n terms
+        if (lane == 0) {  // the moved electron's own entries: its e-n terms
             fsn[iel] = fs_en;
             g1n[iel] = ge0; g1n[NE + iel] = ge1; g1n[2 * NE + iel] = ge2;
             g2n[iel] = ge0; g2n[NE + iel] = ge1; g2n[2 * NE + iel] = ge2;
             dfs += fs_en - fso_row[iel];
             dg0 += ge0 - g1_row[iel]; dg1 += ge1 - g1_row[NE + iel]; dg2 += ge2 - g1_row[2 * NE + iel];
         }
-        // Partners dealt round-robin to the block's threads (wave wv takes
-        // electrons wv, wv + 4, ...), MJB slots of 256 per thread; centres in
+        // Partners in slots of WAVE electrons, MJB slots per lane; centres in
         // the outer loop so each centre's table is read once per slot group.
         // Per pair: gm/go = gradient on the moved / the partner electron.
         const T tkap = (T)kap, tasymp = (T)asymp_r;
-        const int tslot = MOVE_WAVES * lane + wv;
-        for (int j0 = 0; j0 < NE; j0 += 256 * MJB) {
+        for (int j0 = 0; j0 < NE; j0 += WAVE * MJB) {
             T fs[MJB], gm[MJB][3], go[MJB][3], xj[MJB][3], uu1[MJB], fuf[MJB], sfu[MJB];
             bool on[MJB];
 #pragma unroll
             for (int s = 0; s < MJB; s++) {
-                const int j = j0 + s * 256 + tslot;
+                const int j = j0 + s * WAVE + lane;
                 on[s] = j < NE && j != iel;
                 const int jq = j < NE ? j : 0;
                 fs[s] = 0; sfu[s] = 0;
@@ -306,7 +297,7 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                 int jcl[MJB];
 #pragma unroll
                 for (int s = 0; s < MJB; s++) {
-                    const int j = j0 + s * 256 + tslot;
+                    const int j = j0 + s * WAVE + lane;
                     jcl[s] = j < NE ? j : 0;
                     rjn[s] = rw[jcl[s]];
                     rrjn[s] = rrw[jcl[s]];
@@ -328,7 +319,7 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                     }
 #pragma unroll
                     for (int s = 0; s < MJB; s++) {
-                        if (j0 + s * 256 >= NE) break;
+                        if (j0 + s * WAVE >= NE) break;
                         if (!on[s]) continue;
                         if (rrm == asymp_r && rrjc[s] == asymp_r) continue;
                         const T rjo = (T)rjc[s];
@@ -345,7 +336,7 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
             }
 #pragma unroll
             for (int s = 0; s < MJB; s++) {
-                const int j = j0 + s * 256 + tslot;
+                const int j = j0 + s * WAVE + lane;
                 if (!on[s]) continue;
                 const T fu = sfu[s] * fuf[s];
                 const T rmo[3] = {(T)(xn0 - (double)xj[s][0]), (T)(xn1 - (double)xj[s][1]), (T)(xn2 - (double)xj[s][2])};
@@ -362,11 +353,7 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
             }
         }
         dfs = wave_sum(dfs); dg0 = wave_sum(dg0); dg1 = wave_sum(dg1); dg2 = wave_sum(dg2);
-        if (lane == 0) { s_red[0][wv] = dfs; s_red[1][wv] = dg0; s_red[2][wv] = dg1; s_red[3][wv] = dg2; }
-        __syncthreads();
-        if (tid == 0) {
-            dfs = 0; dg0 = 0; dg1 = 0; dg2 = 0;
-            for (int k = 0; k < MOVE_WAVES; k++) { dfs += s_red[0][k]; dg0 += s_red[1][k]; dg1 += s_red[2][k]; dg2 += s_red[3][k]; }
+        if (lane == 0) {
             const double fsn_tot = a.fsumo[w] + dfs;
             a.fsumn[w] = fsn_tot;
             a.psijn[w] = fsn_tot;
@@ -375,15 +362,15 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
 
         // ---- AOs of the moved electron (ider = 1) into the GEMM operand
         // radial functions (R, R') once per (centre, shell), then each AO
-        for (int ri = tid; ri < a.NRI; ri += blockDim.x) {
+        for (int ri = lane; ri < a.NRI; ri += WAVE) {
             const double r = Lr[a.ri_cent[ri]];
             T wfv[3] = {T(0), T(0), T(0)};
             if (r <= a.ri_rmax[ri]) splfit_jx<T>(a.tab, a.ri_type[ri], a.ri_rad[ri], (T)r, grid_jx(a.tab, r), wfv, 1);
             Lrad[2 * ri] = wfv[0]; Lrad[2 * ri + 1] = wfv[1];
         }
-        __syncthreads();
+        __syncwarp();
         double *ao = a.aomove + (size_t)w * 4 * a.NB;
-        for (int m = tid; m < a.NB; m += blockDim.x) {
+        for (int m = lane; m < a.NB; m += WAVE) {
             const int c = a.ao_cent[m], ri = a.ao_ri[m];
             const T wfv[3] = {Lrad[2 * ri], Lrad[2 * ri + 1], T(0)};
             T out[5];
@@ -630,7 +617,7 @@ template <typename T>
 static int launch_move_a_t(champ_ctx *c, const MoveArgs &a, int nordc)
 {
     const int naz[] = {0, 0, een_naz<2>(), een_naz<3>(), een_naz<4>(), een_naz<5>()};
-    const size_t lds = (size_t)move_a_lds_doubles(c->NC, nordc > 1 ? naz[nordc] : 0, c->NRI, (int)sizeof(T)) *
+    const size_t lds = (size_t)4 * move_a_lds_doubles(c->NC, nordc > 1 ? naz[nordc] : 0, c->NRI, (int)sizeof(T)) *
                        sizeof(double);
     const void *fn = nordc == 5 ? (const void *)k_move_a<5, T> : nordc == 4 ? (const void *)k_move_a<4, T>
                    : nordc == 3 ? (const void *)k_move_a<3, T> : nordc == 2 ? (const void *)k_move_a<2, T>
@@ -640,7 +627,7 @@ static int launch_move_a_t(champ_ctx *c, const MoveArgs &a, int nordc)
         return CHAMP_ERR_UNSUPPORTED;
     }
     if (lds > 64 * 1024) HIPCHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const dim3 grid(c->W), block(64 * MOVE_WAVES);
+    const dim3 grid(nblocks(c->W, 4)), block(256);
     switch (nordc) {
     case 5: hipLaunchKernelGGL((k_move_a<5, T>), grid, block, lds, c->stream, a); break;
     case 4: hipLaunchKernelGGL((k_move_a<4, T>), grid, block, lds, c->stream, a); break;
