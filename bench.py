@@ -75,13 +75,26 @@ def run_path(sy, W, precision, args, label):
     return {"wall": wall, "value": W * sy.nelec * args.steps / wall, "kt": kt, "st": st, "nxq": nxq, "hpsi0": g0}
 
 
-def roofline(sy, W, kt, nxq):
+# HBM bytes per launch of the quadrature kernel at the default workload (c160,
+# 4096 walkers): FETCH_SIZE x 2 + WRITE_SIZE x 1 from two rocprofv3 --pmc
+# passes, scaled by the same counters on a device-to-device copy of known size
+# (profiles/k_nonloc_c160_traffic.md)
+NONLOC_TRAFFIC = {("c160", 4096, "fp64"): None, ("c160", 4096, "mixed"): None}
+
+
+def roofline(sy, W, kt, nxq, path):
     from champ_amd import system as S
     nt = len(S.een_terms(sy.nordc))
     nrad = int(sum(sy.nrbas[t] for t in sy.iwctype))
     dname, (dms, dn) = max(kt.items(), key=lambda kv: kv[1][0])
+    # the e-e-n sums of the quadrature run on the matrix cores (FP64 on the FP64
+    # path, FP32 on the mixed path); the per-move and hpsi Jastrow on the VALU
+    pipe, peak = ("fp64-valu", 78.6)
+    traffic = None
     if dname == "hpsi_nonloc":
         fl = float(np.sum(nxq)) * flops_nonloc_point(sy.nelec, sy.ncent, nt, sy.nbasis)
+        pipe, peak = ("fp32-mfma", 157.3) if path == "mixed" else ("fp64-mfma", 78.6)
+        traffic = NONLOC_TRAFFIC.get((sy.name, W, path))
     elif dname == "move_jastrow_ao":
         fl = W * flops_move_jastrow(sy.nelec, sy.ncent, nt, sy.nbasis, nrad)
     elif dname == "hpsi_jastrow":
@@ -91,8 +104,8 @@ def roofline(sy, W, kt, nxq):
     if not dn:
         return None
     ach = fl / (dms / dn / 1e3) / 1e12
-    return {"bound": "mfma", "pipe": "fp64-valu", "kernel": dname, "achieved": round(ach, 3), "peak": 78.6,
-            "unit": "TFLOP/s", "frac": round(ach / 78.6, 4), "traffic": None, "avg_launch_ms": round(dms / dn, 4),
+    return {"bound": "mfma", "pipe": pipe, "kernel": dname, "achieved": round(ach, 3), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": traffic, "avg_launch_ms": round(dms / dn, 4),
             "flops_per_launch": fl,
             "flops_basis": "reference algorithm's per-term operation count, not the kernel's executed instructions"}
 
@@ -183,8 +196,8 @@ def main():
         "walker_sweeps_per_s": round(W * args.steps / head["wall"], 2),
         "acceptance": round(head["st"]["accepted"] / head["st"]["proposed"], 4),
         "accuracy_mixed_vs_fp64": acc,
-        "roofline": roofline(sy, W, head["kt"], head["nxq"]),
-        "roofline_fp64_path": roofline(sy, W, r64["kt"], r64["nxq"]),
+        "roofline": roofline(sy, W, head["kt"], head["nxq"], "fp64" if rmx is None else "mixed"),
+        "roofline_fp64_path": roofline(sy, W, r64["kt"], r64["nxq"], "fp64"),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
