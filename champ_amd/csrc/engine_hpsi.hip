@@ -289,9 +289,11 @@ __global__ void __launch_bounds__(256) k_hpsi_ao(HArgs a)
     __syncwarp();
     for (int m = lane; m < a.NB; m += WAVE) {
         const int c = a.ao_cent[m];
-        T o[5];
-        ao_combine<T>(a.ao_slm[m], (T)(x0 - a.cent[3 * c]), (T)(x1 - a.cent[3 * c + 1]), (T)(x2 - a.cent[3 * c + 2]),
-                      (T)rw[(size_t)c * NE + e], R + 3 * a.ao_ri[m], 2, o);
+        const T *Rm = R + 3 * a.ao_ri[m];
+        T o[5] = {T(0), T(0), T(0), T(0), T(0)};
+        if (Rm[0] != T(0) || Rm[1] != T(0) || Rm[2] != T(0))  // else beyond the cut radius: exactly zero
+            ao_combine<T>(a.ao_slm[m], (T)(x0 - a.cent[3 * c]), (T)(x1 - a.cent[3 * c + 1]),
+                          (T)(x2 - a.cent[3 * c + 2]), (T)rw[(size_t)c * NE + e], Rm, 2, o);
         for (int k = 0; k < 5; k++) out[(size_t)k * a.NB + m] = (double)o[k];
     }
 }

@@ -373,8 +373,9 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
         for (int m = lane; m < a.NB; m += WAVE) {
             const int c = a.ao_cent[m], ri = a.ao_ri[m];
             const T wfv[3] = {Lrad[2 * ri], Lrad[2 * ri + 1], T(0)};
-            T out[5];
-            ao_combine<T>(a.ao_slm[m], (T)Lrv[3 * c], (T)Lrv[3 * c + 1], (T)Lrv[3 * c + 2], (T)Lr[c], wfv, 1, out);
+            T out[5] = {T(0), T(0), T(0), T(0), T(0)};
+            if (wfv[0] != T(0) || wfv[1] != T(0))  // else beyond the cut radius: exactly zero
+                ao_combine<T>(a.ao_slm[m], (T)Lrv[3 * c], (T)Lrv[3 * c + 1], (T)Lrv[3 * c + 2], (T)Lr[c], wfv, 1, out);
             ao[m] = out[0]; ao[a.NB + m] = out[1]; ao[2 * a.NB + m] = out[2]; ao[3 * a.NB + m] = out[3];
         }
     }

@@ -307,10 +307,10 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
 // of partner tables, and the e-e-n sum of point q is
 //   sum_{j != i} sum_k u_qj^k M_k[q][j].
 // A block takes MMA_GP active pairs of one walker (MMA_GP * NQ <= 48 points:
-// three 16-row tiles).  Items (16 partners, one row tile) are dealt to the
-// four waves.  An item walks the centres four at a time: the four K-slots of
-// an MFMA are four centres at one table entry (k, n), so a lane builds all
-// table entries of its (centre, partner) once and feeds one
+// three 16-row tiles).  Items (16 partners) are dealt to the four waves.  An
+// item walks the centres four at a time: the four K-slots of an MFMA are four
+// centres at one table entry (k, n), so a lane builds all table entries of
+// its (centre, partner) once and feeds, for each row tile, one
 // v_mfma_f64_16x16x4f64 per entry into the accumulator of row k (lane l
 // supplies A[l % 16][l / 16] and B[l / 16][l % 16]; element r of the
 // accumulator holds D[4 r + l / 16][l % 16]).  The partner distances are read
@@ -339,8 +339,8 @@ template <> struct Mma<float> {
 };
 
 template <int N, typename T>
-__device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int mt, int npts, const double *XP,
-                                         const T *XQ, const int *IPT, double *swv)
+__device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int npts, const double *XP, const T *XQ,
+                                         const int *IPT, double *swv)
 {
     constexpr EenTable<N> ET{};
     constexpr int NA = een_naz<N>(), KR = ET.kmax + 1;
@@ -349,58 +349,71 @@ __device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int mt, 
     const int NC = a.NC, NE = a.NE;
     const JasParams &jp = *a.jas;
     const double kap = jp.kap;
-    const int j = j0 + ln, pta = mt * 16 + ln;
-    const bool jv = j < NE, pv = pta < npts;
+    const int j = j0 + ln;
+    const bool jv = j < NE;
     const double *rrw = a.rr_en + (size_t)w * NC * NE;
-    Acc acc[KR];
+    Acc acc[MMA_MT][KR];
 #pragma unroll
-    for (int k = 0; k < KR; k++) acc[k] = Mma<T>::zero();
+    for (int mt = 0; mt < MMA_MT; mt++)
+#pragma unroll
+        for (int k = 0; k < KR; k++) acc[mt][k] = Mma<T>::zero();
     double ynext = (jv && sl < NC) ? rrw[(size_t)sl * NE + j] : 0.0;
     for (int b0 = 0; b0 < NC; b0 += 4) {
         const int b = b0 + sl;
         const bool bv = b < NC;
         const double rrb = ynext;
         if (b + 4 < NC && jv) ynext = rrw[(size_t)(b + 4) * NE + j];  // next centre's distance in flight
+        // the partner's table entries at centre b: built once, used by every row tile
         T tab[NA];
         een_tables<N, 0, T>(jp.c[bv ? a.iwctype[b] : 0], (T)(1.0 - kap * rrb), tab);
-        const bool live_b = bv && jv, live_a = bv && pv;
-        T xp[N + 1];
-        const T x = live_a ? XQ[(size_t)b * MMA_MP + pta] : T(0);
-        xp[0] = live_a ? T(1) : T(0);
+        if (!(bv && jv)) {
 #pragma unroll
-        for (int p = 1; p <= N; p++) xp[p] = xp[p - 1] * x;
+            for (int e = 0; e < NA; e++) tab[e] = T(0);
+        }
 #pragma unroll
-        for (int k = 0; k < KR; k++)
+        for (int mt = 0; mt < MMA_MT; mt++) {
+            const int pta = mt * 16 + ln;
+            const bool live_a = bv && pta < npts;
+            T xp[N + 1];
+            const T x = live_a ? XQ[(size_t)b * MMA_MP + pta] : T(0);
+            xp[0] = live_a ? T(1) : T(0);
 #pragma unroll
-            for (int n = 0; n <= N; n++) {
-                const int e = ET.idx[k][n];
-                if (e < 0) continue;
-                acc[k] = Mma<T>::op(xp[n], live_b ? tab[e] : T(0), acc[k]);
-            }
+            for (int p = 1; p <= N; p++) xp[p] = xp[p - 1] * x;
+#pragma unroll
+            for (int k = 0; k < KR; k++)
+#pragma unroll
+                for (int n = 0; n <= N; n++) {
+                    const int e = ET.idx[k][n];
+                    if (e < 0) continue;
+                    acc[mt][k] = Mma<T>::op(xp[n], tab[e], acc[mt][k]);
+                }
+        }
     }
     const double *xw = a.xT + (size_t)w * 3 * NE;
     const double xj = jv ? xw[j] : 0.0, yj = jv ? xw[NE + j] : 0.0, zj = jv ? xw[2 * NE + j] : 0.0;
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int pt = mt * 16 + Mma<T>::row(r, sl);
-        double v = 0.0;
-        if (jv && pt < npts && j != IPT[pt]) {
-            // the switched e-e distance as the point phase forms it
-            const T d0 = (T)(xj - XP[3 * pt]), d1 = (T)(yj - XP[3 * pt + 1]), d2 = (T)(zj - XP[3 * pt + 2]);
-            T r2 = 0;
-            r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
-            const T rjq = sqrt(r2);
-            const T u = switched_from<T>(kap, rjq, scale_r<T>((T)kap, rjq));
-            T pv2 = acc[KR - 1][r];
+    for (int mt = 0; mt < MMA_MT; mt++)
 #pragma unroll
-            for (int k = KR - 2; k >= 0; k--) pv2 = pv2 * u + acc[k][r];
-            v = (double)pv2;
+        for (int r = 0; r < 4; r++) {
+            const int pt = mt * 16 + Mma<T>::row(r, sl);
+            double v = 0.0;
+            if (jv && pt < npts && j != IPT[pt]) {
+                // the switched e-e distance as the point phase forms it
+                const T d0 = (T)(xj - XP[3 * pt]), d1 = (T)(yj - XP[3 * pt + 1]), d2 = (T)(zj - XP[3 * pt + 2]);
+                T r2 = 0;
+                r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
+                const T rjq = sqrt(r2);
+                const T u = switched_from<T>(kap, rjq, scale_r<T>((T)kap, rjq));
+                T pv2 = acc[mt][KR - 1][r];
+#pragma unroll
+                for (int k = KR - 2; k >= 0; k--) pv2 = pv2 * u + acc[mt][k][r];
+                v = (double)pv2;
+            }
+            // sum over the 16 partners (lanes of equal l / 16) into the wave's point slot
+            v += __shfl_xor(v, 1, WAVE); v += __shfl_xor(v, 2, WAVE);
+            v += __shfl_xor(v, 4, WAVE); v += __shfl_xor(v, 8, WAVE);
+            if (ln == 0) swv[pt] += v;
         }
-        // sum over the 16 partners (lanes of equal l / 16) into the wave's point slot
-        v += __shfl_xor(v, 1, WAVE); v += __shfl_xor(v, 2, WAVE);
-        v += __shfl_xor(v, 4, WAVE); v += __shfl_xor(v, 8, WAVE);
-        if (ln == 0) swv[pt] += v;
-    }
 }
 
 // LDS of k_nonloc_mma in doubles: XP, SP, TG partials, SW, FI, IPT/PI (as
@@ -523,17 +536,16 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
             SP[pt] = v;
         }
 
-        // ---- e-e-n on the matrix cores: items (16 partners, one row tile) over the
-        // waves; each wave sums its items into its own row of SW
+        // ---- e-e-n on the matrix cores: items (16 partners, all row tiles) over
+        // the waves; each wave sums its items into its own row of SW
         {
             double *swv = SW + wv * MMA_MP;
             for (int pt = lane; pt < MMA_MP; pt += WAVE) swv[pt] = 0.0;
             __syncwarp();
             if constexpr (NORDC > 1) {
-                const int NT = (NE + 15) / 16, MTU = (npts + 15) / 16;
-                for (int it = wv; it < NT * MTU; it += MMA_WAVES) {
-                    const int mt = it % MTU;
-                    mma_item<NORDC, T>(a, w, (it / MTU) * 16, mt, npts, XP, XQ, IPT, swv);
+                const int NT = (NE + 15) / 16;
+                for (int it = wv; it < NT; it += MMA_WAVES) {
+                    mma_item<NORDC, T>(a, w, it * 16, npts, XP, XQ, IPT, swv);
                     __syncwarp();
                 }
             }
@@ -580,10 +592,12 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
 #pragma unroll
                 for (int q = 0; q < QM; q++) {
                     if (q >= NQ) break;
+                    const double rad = RAD[(size_t)q * NRI + ri];
+                    if (rad == 0.0) continue;  // beyond the shell's cut radius: the AO is exactly zero
                     const double *g = GEO + ((size_t)q * NC + c) * 5;
                     double y, dy[3], lap;
                     slm<double>(l, g[0], g[1], g[2], y, dy, lap);
-                    dr[q] += (y * RAD[(size_t)q * NRI + ri]) * wm;
+                    dr[q] += (y * rad) * wm;
                 }
             }
 #pragma unroll
