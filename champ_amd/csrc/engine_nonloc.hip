@@ -339,8 +339,8 @@ template <> struct Mma<float> {
 };
 
 template <int N, typename T>
-__device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int npts, const double *XP, const T *XQ,
-                                         const int *IPT, double *swv)
+__device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int c_lo, int c_hi, int npts,
+                                         const double *XP, const T *XQ, const int *IPT, double *swv)
 {
     constexpr EenTable<N> ET{};
     constexpr int NA = een_naz<N>(), KR = ET.kmax + 1;
@@ -357,12 +357,12 @@ __device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int npts
     for (int mt = 0; mt < MMA_MT; mt++)
 #pragma unroll
         for (int k = 0; k < KR; k++) acc[mt][k] = Mma<T>::zero();
-    double ynext = (jv && sl < NC) ? rrw[(size_t)sl * NE + j] : 0.0;
-    for (int b0 = 0; b0 < NC; b0 += 4) {
+    double ynext = (jv && c_lo + sl < c_hi) ? rrw[(size_t)(c_lo + sl) * NE + j] : 0.0;
+    for (int b0 = c_lo; b0 < c_hi; b0 += 4) {
         const int b = b0 + sl;
-        const bool bv = b < NC;
+        const bool bv = b < c_hi;
         const double rrb = ynext;
-        if (b + 4 < NC && jv) ynext = rrw[(size_t)(b + 4) * NE + j];  // next centre's distance in flight
+        if (b + 4 < c_hi && jv) ynext = rrw[(size_t)(b + 4) * NE + j];  // next centre's distance in flight
         // the partner's table entries at centre b: built once, used by every row tile
         T tab[NA];
         een_tables<N, 0, T>(jp.c[bv ? a.iwctype[b] : 0], (T)(1.0 - kap * rrb), tab);
@@ -543,9 +543,21 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
             for (int pt = lane; pt < MMA_MP; pt += WAVE) swv[pt] = 0.0;
             __syncwarp();
             if constexpr (NORDC > 1) {
-                const int NT = (NE + 15) / 16;
-                for (int it = wv; it < NT; it += MMA_WAVES) {
-                    mma_item<NORDC, T>(a, w, it * 16, npts, XP, XQ, IPT, swv);
+                // half-tiles (16 partners x half the centres) dealt in contiguous
+                // runs of equal length to the waves; two halves of one tile in a
+                // run make one item over all centres
+                const int NT = (NE + 15) / 16, nh = 2 * NT, per = (nh + MMA_WAVES - 1) / MMA_WAVES;
+                const int ncm = min(NC, ((NC / 2 + 3) / 4) * 4);  // centres of the first half (a multiple of 4)
+                const int h_hi = min(nh, (wv + 1) * per);
+                for (int h = wv * per; h < h_hi;) {
+                    const int tile = h / 2;
+                    if (h % 2 == 0 && h + 1 < h_hi) {
+                        mma_item<NORDC, T>(a, w, tile * 16, 0, NC, npts, XP, XQ, IPT, swv);
+                        h += 2;
+                    } else {
+                        mma_item<NORDC, T>(a, w, tile * 16, h % 2 ? ncm : 0, h % 2 ? NC : ncm, npts, XP, XQ, IPT, swv);
+                        h += 1;
+                    }
                     __syncwarp();
                 }
             }
