@@ -366,6 +366,7 @@ extern "C" int champ_gpu_free(champ_ctx *c)
                     c->ri_rad, c->ri_rmax, c->ao_ri, c->gref};
     for (void *p : ptrs) if (p) hipFree(p);
     md_free(c);
+    if (c->sweep_exec) hipGraphExecDestroy(c->sweep_exec);
     for (auto &t : c->ev_pool) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -526,25 +527,64 @@ extern "C" int champ_gpu_accept(champ_ctx *c, int32_t iel, const uint8_t *accept
     return CHAMP_OK;
 }
 
+// One metrop1 sweep: rotqua, Ne single-electron moves, hpsi accumulating the
+// local energy.  Every launch argument depends only on the context and tau.
+static int launch_sweep(champ_ctx *c)
+{
+    int rc;
+    if (c->nloc > 0 && (rc = launch_rotqua(c))) return rc;
+    for (int iel = 0; iel < c->NE; iel++) {
+        if ((rc = launch_md_pre_timed(c, iel))) return rc;
+        if ((rc = launch_move_a(c, iel, 0))) return rc;
+        if ((rc = launch_move_gemm(c))) return rc;
+        if ((rc = launch_md_post_timed(c, iel))) return rc;
+        if ((rc = launch_move_b(c, iel, 0))) return rc;
+        if (c->lu_every > 0 && (iel + 1) % c->lu_every == 0 && iel + 1 < c->NE && (rc = launch_slater_refresh(c)))
+            return rc;
+    }
+    return launch_hpsi_impl(c, 1);
+}
+
+// The sweep captured once per tau as a HIP graph and replayed; it holds
+// kernel nodes only (the estimator sums are cleared before the first replay).
+static int sweep_graph(champ_ctx *c, double tau)
+{
+    if (c->sweep_exec && c->sweep_tau == tau) return CHAMP_OK;
+    if (c->sweep_exec) { HIPCHECK(hipGraphExecDestroy(c->sweep_exec)); c->sweep_exec = nullptr; }
+    HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    const int rc = launch_sweep(c);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) { set_error(std::string("sweep capture: ") + hipGetErrorString(e)); return CHAMP_ERR_HIP; }
+    const hipError_t ei = hipGraphInstantiate(&c->sweep_exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) { c->sweep_exec = nullptr; set_error(std::string("sweep graph: ") + hipGetErrorString(ei)); return CHAMP_ERR_HIP; }
+    c->sweep_tau = tau;
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_set_graphs(champ_ctx *c, int32_t on)
+{
+    if (!c) return CHAMP_ERR_ARG;
+    c->graphs = on != 0;
+    return CHAMP_OK;
+}
+
 extern "C" int champ_gpu_metrop1(champ_ctx *c, int32_t nsweep, double tau, champ_sweep_stats *stats)
 {
     if (!c || nsweep < 0 || !(tau > 0)) return CHAMP_ERR_ARG;
     set_tau(tau);
     HIPCHECK(hipMemsetAsync(c->esum_dev, 0, sizeof(double) * 4, c->stream));
     int rc;
-    for (int isw = 0; isw < nsweep; isw++) {
-        if (c->nloc > 0 && (rc = launch_rotqua(c))) return rc;
-        for (int iel = 0; iel < c->NE; iel++) {
-            if ((rc = launch_md_pre_timed(c, iel))) return rc;
-            if ((rc = launch_move_a(c, iel, 0))) return rc;
-            if ((rc = launch_move_gemm(c))) return rc;
-            if ((rc = launch_md_post_timed(c, iel))) return rc;
-            if ((rc = launch_move_b(c, iel, 0))) return rc;
-            if (c->lu_every > 0 && (iel + 1) % c->lu_every == 0 && iel + 1 < c->NE && (rc = launch_slater_refresh(c)))
-                return rc;
+    if (c->graphs && !c->timing && nsweep > 0) {
+        if ((rc = sweep_graph(c, tau))) return rc;
+        for (int isw = 0; isw < nsweep; isw++) HIPCHECK(hipGraphLaunch(c->sweep_exec, c->stream));
+    } else {
+        for (int isw = 0; isw < nsweep; isw++) {
+            if ((rc = launch_sweep(c))) return rc;
+            if (c->timing && (isw % 4) == 3) tflush(c);
         }
-        if ((rc = launch_hpsi_impl(c, 1))) return rc;
-        if (c->timing && (isw % 4) == 3) tflush(c);
     }
     double h[4];
     HIPCHECK(hipMemcpyAsync(h, c->esum_dev, sizeof(h), hipMemcpyDeviceToHost, c->stream));

@@ -131,6 +131,11 @@ struct champ_ctx {
     double *md_Wt = nullptr;    // [W][NU] scratch
     double *md_g = nullptr;     // [W][NV] scratch
 
+    // ---- metrop1 sweep as a HIP graph (captured per tau)
+    int graphs = 1;
+    hipGraphExec_t sweep_exec = nullptr;
+    double sweep_tau = 0;
+
     // ---- timing
     bool timing = false;
     std::vector<champ::Timer> ev_pool;

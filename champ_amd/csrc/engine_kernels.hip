@@ -121,6 +121,10 @@ __global__ void __launch_bounds__(256) k_drift_old(MoveArgs a)
 #define CHAMP_MOVE_WPE 1
 #endif
 constexpr int MJB = CHAMP_MJB;
+#ifndef CHAMP_MOVE_PF
+#define CHAMP_MOVE_PF 4
+#endif
+constexpr int MOVE_PF = CHAMP_MOVE_PF;  // centres whose partner distances are loaded one group ahead
 
 // LDS of one wave of k_move_a, in doubles: r, rr, -kap (1 - kap rr) / r,
 // rvec (3) per centre (double), then in the arithmetic type (tsize bytes) the
@@ -291,46 +295,61 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                 }
             }
             if constexpr (NORDC > 1) {
-                // distances of the partners to centre c, one centre of lookahead
-                // (slots past NE read electron 0 and are not used)
-                double rjn[MJB], rrjn[MJB];
+                // distances of the partners to the centres, MOVE_PF centres at a
+                // time with the next group's loads in flight (slots past NE read
+                // electron 0 and are not used)
                 int jcl[MJB];
 #pragma unroll
                 for (int s = 0; s < MJB; s++) {
                     const int j = j0 + s * WAVE + lane;
                     jcl[s] = j < NE ? j : 0;
-                    rjn[s] = rw[jcl[s]];
-                    rrjn[s] = rrw[jcl[s]];
                 }
-                for (int c = 0; c < NC; c++) {
-                    const T *tab = Ltab + (size_t)c * 2 * NA;
-                    const double rrm = Lrr[c];
-                    const T fmf = (T)Lfm[c];
-                    const T rvm0 = (T)Lrv[3 * c], rvm1 = (T)Lrv[3 * c + 1], rvm2 = (T)Lrv[3 * c + 2];
-                    const T cx = (T)a.cent[3 * c], cy = (T)a.cent[3 * c + 1], cz = (T)a.cent[3 * c + 2];
-                    double rjc[MJB], rrjc[MJB];
+                double rjc[MOVE_PF][MJB], rrjc[MOVE_PF][MJB], rjn[MOVE_PF][MJB], rrjn[MOVE_PF][MJB];
+#pragma unroll
+                for (int p = 0; p < MOVE_PF; p++)
 #pragma unroll
                     for (int s = 0; s < MJB; s++) {
-                        rjc[s] = rjn[s]; rrjc[s] = rrjn[s];
-                        if (c + 1 < NC) {
-                            rjn[s] = rw[(size_t)(c + 1) * NE + jcl[s]];
-                            rrjn[s] = rrw[(size_t)(c + 1) * NE + jcl[s]];
-                        }
+                        const int c = p < NC ? p : 0;
+                        rjn[p][s] = rw[(size_t)c * NE + jcl[s]];
+                        rrjn[p][s] = rrw[(size_t)c * NE + jcl[s]];
                     }
+                for (int c0 = 0; c0 < NC; c0 += MOVE_PF) {
 #pragma unroll
-                    for (int s = 0; s < MJB; s++) {
-                        if (j0 + s * WAVE >= NE) break;
-                        if (!on[s]) continue;
-                        if (rrm == asymp_r && rrjc[s] == asymp_r) continue;
-                        const T rjo = (T)rjc[s];
-                        const T swj = (T)(1.0 - kap * rrjc[s]);
-                        const EenOut<T> e = een_eval_tab<NORDC, 1, T>(tab, uu1[s], swj);
-                        fs[s] += e.f;
-                        sfu[s] += e.fu;
-                        const T fmv = e.fx * fmf;
-                        const T fot = e.fy * (-tkap * swj) / rjo;
-                        gm[s][0] += fmv * rvm0; gm[s][1] += fmv * rvm1; gm[s][2] += fmv * rvm2;
-                        go[s][0] += fot * (xj[s][0] - cx); go[s][1] += fot * (xj[s][1] - cy); go[s][2] += fot * (xj[s][2] - cz);
+                    for (int p = 0; p < MOVE_PF; p++)
+#pragma unroll
+                        for (int s = 0; s < MJB; s++) {
+                            rjc[p][s] = rjn[p][s]; rrjc[p][s] = rrjn[p][s];
+                            const int cn = c0 + MOVE_PF + p;
+                            if (cn < NC) {
+                                rjn[p][s] = rw[(size_t)cn * NE + jcl[s]];
+                                rrjn[p][s] = rrw[(size_t)cn * NE + jcl[s]];
+                            }
+                        }
+#pragma unroll
+                    for (int p = 0; p < MOVE_PF; p++) {
+                        const int c = c0 + p;
+                        if (c >= NC) break;
+                        const T *tab = Ltab + (size_t)c * 2 * NA;
+                        const double rrm = Lrr[c];
+                        const T fmf = (T)Lfm[c];
+                        const T rvm0 = (T)Lrv[3 * c], rvm1 = (T)Lrv[3 * c + 1], rvm2 = (T)Lrv[3 * c + 2];
+                        const T cx = (T)a.cent[3 * c], cy = (T)a.cent[3 * c + 1], cz = (T)a.cent[3 * c + 2];
+#pragma unroll
+                        for (int s = 0; s < MJB; s++) {
+                            if (j0 + s * WAVE >= NE) break;
+                            if (!on[s]) continue;
+                            if (rrm == asymp_r && rrjc[p][s] == asymp_r) continue;
+                            const T rjo = (T)rjc[p][s];
+                            const T swj = (T)(1.0 - kap * rrjc[p][s]);
+                            const EenOut<T> e = een_eval_tab<NORDC, 1, T>(tab, uu1[s], swj);
+                            fs[s] += e.f;
+                            sfu[s] += e.fu;
+                            const T fmv = e.fx * fmf;
+                            const T fot = e.fy * (-tkap * swj) / rjo;
+                            gm[s][0] += fmv * rvm0; gm[s][1] += fmv * rvm1; gm[s][2] += fmv * rvm2;
+                            go[s][0] += fot * (xj[s][0] - cx); go[s][1] += fot * (xj[s][1] - cy);
+                            go[s][2] += fot * (xj[s][2] - cz);
+                        }
                     }
                 }
             }

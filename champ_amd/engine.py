@@ -29,7 +29,7 @@ EXPORTS = [
     "champ_gpu_set_rng", "champ_gpu_get_rng", "champ_gpu_seed_rng", "champ_gpu_set_quadrature", "champ_gpu_rotqua",
     "champ_gpu_hpsi", "champ_gpu_compute_determinante_grad", "champ_gpu_psie", "champ_gpu_accept",
     "champ_gpu_metrop1", "champ_gpu_get_walker_scalars", "champ_gpu_get_status", "champ_gpu_array_size",
-    "champ_gpu_get_array", "champ_gpu_basis_fns", "champ_gpu_kernel_times",
+    "champ_gpu_get_array", "champ_gpu_basis_fns", "champ_gpu_kernel_times", "champ_gpu_set_graphs",
 ]
 
 
@@ -72,6 +72,7 @@ def load_library(path: str = LIBPATH):
     L.champ_gpu_get_array.argtypes = [vp, i32, dp]
     L.champ_gpu_basis_fns.argtypes = [vp, i32, dp, dp]
     L.champ_gpu_kernel_times.argtypes = [vp, C.POINTER(i32), C.c_char_p, dp, C.POINTER(C.c_int64)]
+    L.champ_gpu_set_graphs.argtypes = [vp, i32]
     _libs[path] = L
     return L
 
@@ -195,6 +196,10 @@ class Engine:
         return out
 
     # ---------------------------------------------------------------- timing
+    def set_graphs(self, on: bool = True):
+        """metrop1 replays a captured HIP graph per sweep (default) or launches eagerly."""
+        self._check(self.L.champ_gpu_set_graphs(self.ptr, 1 if on else 0), "set_graphs")
+
     def enable_timing(self, on: bool = True):
         n = C.c_int32(1 if on else 0)
         self._check(self.L.champ_gpu_kernel_times(self.ptr, C.byref(n), None, None, None), "kernel_times")

@@ -193,6 +193,12 @@ int champ_gpu_accept(champ_ctx *ctx, int32_t iel, const uint8_t *accept);
 int champ_gpu_metrop1(champ_ctx *ctx, int32_t nsweep, double tau,
                       champ_sweep_stats *stats);
 
+/* metrop1 replays each sweep as a HIP graph captured on first use (per tau);
+ * on = 0 launches every kernel from the host instead.  No counterpart in the
+ * reference.  Kernel-class timing (champ_gpu_kernel_times) always runs
+ * without the graph.                                                          */
+int champ_gpu_set_graphs(champ_ctx *ctx, int32_t on);
+
 /* Per-walker state readback for checking: psido, psijo, eold, psi2o [nwalk]
  * and the status words.                                                       */
 int champ_gpu_get_walker_scalars(champ_ctx *ctx, double *psido, double *psijo,

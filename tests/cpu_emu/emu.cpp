@@ -143,3 +143,38 @@ hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { e->t = emu::now_s(); return hipSuccess; }
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) { *ms = (float)(1e3 * (b->t - a->t)); return hipSuccess; }
 
+
+// ---- stream capture and graphs
+struct emu_graph {
+    std::vector<std::function<void()>> ops;
+};
+static thread_local emu_graph *g_capture = nullptr;
+namespace emu {
+bool capturing() { return g_capture != nullptr; }
+void record(std::function<void()> f) { g_capture->ops.push_back(std::move(f)); }
+}  // namespace emu
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode)
+{
+    if (g_capture) return hipErrorInvalidValue;
+    g_capture = new emu_graph();
+    return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g)
+{
+    if (!g_capture) return hipErrorInvalidValue;
+    *g = g_capture;
+    g_capture = nullptr;
+    return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, hipGraphNode_t *, char *, size_t)
+{
+    *e = new emu_graph(*g);
+    return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t)
+{
+    for (auto &f : e->ops) f();
+    return hipSuccess;
+}
+hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }

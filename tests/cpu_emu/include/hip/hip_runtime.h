@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstddef>
+#include <functional>
 #include <tuple>
 #include <utility>
 
@@ -197,10 +198,37 @@ hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = nullptr);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
 
+// Stream capture: launches between hipStreamBeginCapture and
+// hipStreamEndCapture are recorded (closures over copies of their arguments)
+// and run in order by hipGraphLaunch.
+typedef enum hipStreamCaptureMode {
+    hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1, hipStreamCaptureModeRelaxed = 2
+} hipStreamCaptureMode;
+typedef struct emu_graph *hipGraph_t;
+typedef struct emu_graph *hipGraphExec_t;
+typedef struct emu_graph_node *hipGraphNode_t;
+namespace emu {
+bool capturing();
+void record(std::function<void()> f);
+}  // namespace emu
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode m);
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t *g);
+hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, hipGraphNode_t *, char *, size_t);
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t s);
+hipError_t hipGraphDestroy(hipGraph_t g);
+hipError_t hipGraphExecDestroy(hipGraphExec_t e);
+
 #define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...)                                   \
     do {                                                                                              \
         auto emu_args_ = std::make_tuple(__VA_ARGS__);                                                \
-        emu::Thunk<decltype(&kernel), decltype(emu_args_)> emu_t_{&kernel, emu_args_};                 \
-        emu::run_grid(#kernel, dim3(grid), dim3(block), (size_t)(shmem),                              \
-                      &emu::Thunk<decltype(&kernel), decltype(emu_args_)>::call, &emu_t_);             \
+        typedef emu::Thunk<decltype(&kernel), decltype(emu_args_)> EmuTT_;                             \
+        EmuTT_ emu_t_{&kernel, emu_args_};                                                            \
+        const dim3 emu_g_ = dim3(grid), emu_b_ = dim3(block);                                         \
+        const size_t emu_sh_ = (size_t)(shmem);                                                       \
+        if (emu::capturing())                                                                         \
+            emu::record([emu_t_, emu_g_, emu_b_, emu_sh_]() mutable {                                  \
+                emu::run_grid(#kernel, emu_g_, emu_b_, emu_sh_, &EmuTT_::call, &emu_t_);              \
+            });                                                                                       \
+        else                                                                                          \
+            emu::run_grid(#kernel, emu_g_, emu_b_, emu_sh_, &EmuTT_::call, &emu_t_);                  \
     } while (0)

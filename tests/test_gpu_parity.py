@@ -9,7 +9,8 @@ the oracle evaluates them determinant by determinant.
 import numpy as np
 import pytest
 
-from tests.gpu_helpers import Pair, rel, relnorm
+from champ_amd.engine import Engine
+from tests.gpu_helpers import ENGINE_LIB, Pair, rel, relnorm
 
 pytestmark = pytest.mark.gpu
 
@@ -125,6 +126,36 @@ def test_metrop1_trajectory(require_gpu, name):
     rg = pr.eng.get_rng()
     for w in range(pr.nwalk):
         assert np.array_equal(rg[w], pr.orc[w].get_rng())
+
+
+@pytest.mark.parametrize("name", ["small", "small_md"])
+def test_metrop1_graph_replays_match_eager(require_gpu, name):
+    """metrop1 through the captured sweep graph (replayed three times, with an
+    eager hpsi and readbacks between replays) against the same sweeps launched
+    eagerly from identical walkers and streams: bit-identical walkers."""
+    from champ_amd import system as S
+    sy = S.config(name)
+    x0 = sy.initial_walkers(4, seed=3)
+    engs = []
+    for graphs in (True, False):
+        e = Engine(sy, 4, lib_path=ENGINE_LIB)
+        e.set_graphs(graphs)
+        e.set_walkers(x0)
+        e.seed_rng(S.seed_from_string())
+        e.hpsi()
+        engs.append(e)
+    out = []
+    for e in engs:
+        s1 = e.metrop1(2, tau=0.3)
+        h = e.hpsi()
+        s2 = e.metrop1(1, tau=0.3)
+        out.append((s1, s2, h, e.get_walkers(), e.scalars()))
+    (a1, a2, ha, xa, sa), (b1, b2, hb, xb, sb) = out
+    print(f"[graph {name}] accepted {a1['accepted']} {b1['accepted']} esum {a1['esum']} {b1['esum']}", flush=True)
+    assert a1 == b1 and a2 == b2
+    assert np.array_equal(xa, xb)
+    assert np.array_equal(ha["energy"], hb["energy"])
+    assert np.array_equal(sa["eold"], sb["eold"])
 
 
 @pytest.mark.parametrize("nwalk", [2])
