@@ -8,12 +8,15 @@ hpsi (full Jastrow, orbitals, LU inverse, ECP quadrature, local energy) per
 walker.  Workload: 40 nuclei (20 Z_eff=4 with s/p non-local ECP + 20 H),
 160 electrons (80/80), 600 AOs, 12-point quadrature, 4096 walkers.
 
-The same walkers and RNG streams are run on the plain FP64 path and on the
-mixed-precision path (FP32 Jastrow, per-move AOs and quadrature Jastrow terms;
+Each path first runs the autotuner (champ_gpu_autotune: GEMM kinds and the
+quadrature form timed on the live walkers).  The same walkers and RNG streams
+are run on the plain FP64 path and on the mixed-precision path (FP32 Jastrow, per-move AOs and quadrature Jastrow terms;
 FP64 determinants, inverse refresh every 80 moves and local energy).  `value`
 is the mixed path's rate; the FP64 rate, their ratio and the per-walker
 local-energy and log-psi differences on identical configurations are in the
-same line.
+same line.  The timed region launches every kernel with HIP events around
+each kernel class (the per-kernel times and the roofline come from it); the
+same sweeps replayed from the captured HIP graph are timed after it.
 
 Prints one JSON line (walker-moves/s = walkers * electrons * sweeps / s).
 """
@@ -54,8 +57,9 @@ def run_path(sy, W, precision, args, label):
     eng.set_walkers(sy.initial_walkers(W, seed=1))
     eng.seed_rng(S.seed_from_string())
     g0 = eng.hpsi()
-    print(f"[bench] {label}: {sy.name}: Ne={sy.nelec} Nc={sy.ncent} Nao={sy.nbasis} W={W}; warmup {args.warmup}",
-          flush=True)
+    tuned = eng.autotune() if args.autotune else None
+    print(f"[bench] {label}: {sy.name}: Ne={sy.nelec} Nc={sy.ncent} Nao={sy.nbasis} W={W}; warmup {args.warmup}; "
+          f"autotuned {tuned}", flush=True)
     if args.warmup:
         eng.metrop1(args.warmup, args.tau)
     eng.enable_timing(True)
@@ -65,6 +69,14 @@ def run_path(sy, W, precision, args, label):
     kt = eng.kernel_times()
     eng.enable_timing(False)
     nxq = eng.array("nxquad")
+    # the same number of sweeps replayed from the captured sweep graph, no events
+    gwall = None
+    if args.graph_check:
+        eng.set_graphs(True)
+        eng.metrop1(1, args.tau)  # captures the graph
+        tg0 = time.perf_counter()
+        eng.metrop1(args.steps, args.tau)
+        gwall = time.perf_counter() - tg0
     eng.close()
     wall = t1 - t0
     print(f"[bench] {label}: {args.steps} sweeps in {wall:.3f}s; acceptance {st['accepted'] / st['proposed']:.3f}; "
@@ -72,7 +84,10 @@ def run_path(sy, W, precision, args, label):
     for k, (ms, n) in kt.items():
         if n:
             print(f"[bench]   {label} {k:20s} {ms:10.2f} ms  {n:6d} launches  {ms / n:9.4f} ms/launch", flush=True)
-    return {"wall": wall, "value": W * sy.nelec * args.steps / wall, "kt": kt, "st": st, "nxq": nxq, "hpsi0": g0}
+    if gwall is not None:
+        print(f"[bench] {label}: graph replay {args.steps} sweeps in {gwall:.3f}s", flush=True)
+    return {"wall": wall, "value": W * sy.nelec * args.steps / wall, "kt": kt, "st": st, "nxq": nxq, "hpsi0": g0,
+            "graph_value": None if gwall is None else W * sy.nelec * args.steps / gwall, "tuned": tuned}
 
 
 # HBM bytes per launch of the quadrature kernel at the default workload (c160,
@@ -122,6 +137,10 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="time the plain FP64 path only")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the shape rules instead of timing the candidate kernel forms first")
+    ap.add_argument("--no-graph-check", dest="graph_check", action="store_false",
+                    help="skip the second timing of the sweeps replayed from the captured HIP graph")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs the walker population on one GPU; "
@@ -194,6 +213,11 @@ def main():
         "fp64_value": round(r64["value"], 1),
         "mixed_over_fp64": None if rmx is None else round(rmx["value"] / r64["value"], 3),
         "walker_sweeps_per_s": round(W * args.steps / head["wall"], 2),
+        "autotuned": head["tuned"],
+        "graph_replay": None if head["graph_value"] is None else {
+            "value": round(head["graph_value"], 1), "unit": "walker-moves/s",
+            "note": "the same number of sweeps replayed from the captured sweep graph, without per-kernel events "
+                    "(value and roofline come from the event-timed eager launches)"},
         "acceptance": round(head["st"]["accepted"] / head["st"]["proposed"], 4),
         "accuracy_mixed_vs_fp64": acc,
         "roofline": roofline(sy, W, head["kt"], head["nxq"], "fp64" if rmx is None else "mixed"),

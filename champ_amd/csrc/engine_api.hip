@@ -2,6 +2,7 @@
 // Context set-up uploads the CHAMP tables once; every other entry point acts
 // on the whole walker population on the context's stream.
 #include "engine_internal.hpp"
+#include "hpsi_args.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -562,6 +563,81 @@ static int sweep_graph(champ_ctx *c, double tau)
     if (ei != hipSuccess) { c->sweep_exec = nullptr; set_error(std::string("sweep graph: ") + hipGetErrorString(ei)); return CHAMP_ERR_HIP; }
     c->sweep_tau = tau;
     return CHAMP_OK;
+}
+
+// ------------------------------------------------------------------------
+// Autotuner: each candidate kernel form of a tunable launch is timed on the
+// context's own walkers (HIP events around repeated launches on its stream)
+// and the fastest is kept.  Every candidate computes the same quantities.
+static int time_launch(champ_ctx *c, int (*fn)(champ_ctx *), int reps, float *ms)
+{
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    int rc = fn(c);  // warm-up (code load, caches)
+    if (!rc) {
+        hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps && !rc; r++) rc = fn(c);
+        hipEventRecord(e1, c->stream);
+        hipEventSynchronize(e1);
+        hipEventElapsedTime(ms, e0, e1);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+static int tune_move_gemm(champ_ctx *c) { return launch_move_gemm(c); }
+static int tune_orb_gemm(champ_ctx *c)
+{
+    const int nw = std::min(c->hpsi_chunk, c->W);
+    return launch_gemm(c, true, nw * c->NE * 5, c->NO, c->NB, c->aoall, c->NB, 0, c->coef, c->NB, 0, c->orball, c->NO,
+                       0, 1, c->tune_gemm[1]);
+}
+static int tune_wmat(champ_ctx *c) { return launch_wmat(c); }
+static int tune_nonloc(champ_ctx *c)
+{
+    HArgs a = hargs(c);
+    return launch_nonloc(c, a);
+}
+
+extern "C" int champ_gpu_autotune(champ_ctx *c, int32_t *choices)
+{
+    if (!c) return CHAMP_ERR_ARG;
+    const bool timing = c->timing;
+    c->timing = false;
+    int rc = launch_hpsi_impl(c, 0);  // every input of the tuned launches in place
+    int (*site[3])(champ_ctx *) = {tune_move_gemm, tune_orb_gemm, tune_wmat};
+    for (int s = 0; s < 3 && !rc; s++) {
+        float best = 1e30f;
+        int pick = GEMM_AUTO;
+        for (int kind : {GEMM_TILE, GEMM_STRIP, GEMM_VALU}) {
+            c->tune_gemm[s] = kind;
+            float ms = 0;
+            if ((rc = time_launch(c, site[s], 3, &ms))) break;
+            if (ms < best) { best = ms; pick = kind; }
+        }
+        c->tune_gemm[s] = pick;
+    }
+    if (!rc && c->nloc > 0) {
+        float best = 1e30f;
+        int pick = 0;
+        for (int kind : {0, 1}) {
+            c->tune_nonloc = kind;
+            float ms = 0;
+            if ((rc = time_launch(c, tune_nonloc, 2, &ms))) break;
+            if (ms < best) { best = ms; pick = kind; }
+        }
+        c->tune_nonloc = pick;
+    }
+    if (!rc) rc = launch_hpsi_impl(c, 0);  // state as before the trials
+    if (!rc) HIPCHECK(hipStreamSynchronize(c->stream));
+    c->timing = timing;
+    if (c->sweep_exec) { hipGraphExecDestroy(c->sweep_exec); c->sweep_exec = nullptr; }
+    if (choices) {
+        for (int s = 0; s < 3; s++) choices[s] = c->tune_gemm[s];
+        choices[3] = c->tune_nonloc;
+    }
+    return rc;
 }
 
 extern "C" int champ_gpu_set_graphs(champ_ctx *c, int32_t on)

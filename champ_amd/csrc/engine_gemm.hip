@@ -202,10 +202,10 @@ __global__ void __launch_bounds__(256) k_gemm_mfma_strip(int M, int N, int K, co
 // C[b] (M x N, row-major, ldc) = A[b] (M x K, lda) * op(B[b]); op = B^T when
 // bt (B stored N x K) else B (K x N).
 int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int lda, long long sA, const double *B,
-                int ldb, long long sB, double *Cm, int ldc, long long sC, int batch)
+                int ldb, long long sB, double *Cm, int ldc, long long sC, int batch, int kind)
 {
     if (M <= 0 || N <= 0 || batch <= 0) return CHAMP_OK;
-    if (!CHAMP_GEMM_MFMA) {
+    if (!CHAMP_GEMM_MFMA || kind == GEMM_VALU) {
         if (batch > 65535) {
             set_error("gemm: batch too large for one launch");
             return CHAMP_ERR_UNSUPPORTED;
@@ -224,7 +224,8 @@ int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int
     {
         constexpr int WN = 5, BN = 16 * WN;
         const long long gx = (M + 63) / 64, gy = (N + BN - 1) / BN;
-        if (gx * gy * batch >= 512 && gx <= 0x7fffffffLL && gy <= 65535 && batch <= 65535) {
+        const bool fits = gx <= 0x7fffffffLL && gy <= 65535 && batch <= 65535;
+        if (fits && (kind == GEMM_STRIP || (kind == GEMM_AUTO && gx * gy * batch >= 512))) {
             dim3 grid((unsigned)gx, (unsigned)gy, batch);
             if (bt)
                 hipLaunchKernelGGL((k_gemm_mfma_strip<true, WN>), grid, dim3(256), 0, c->stream, M, N, K, A, lda, sA, B, ldb,

@@ -598,7 +598,7 @@ static int launch_orbitals_slater(champ_ctx *c, HArgs a)
         HIPCHECK(hipGetLastError());
         tbegin(c, K_HPSI_GEMM, &slot);
         int rcg = launch_gemm(c, true, (int)(nwave * 5), c->NO, c->NB, c->aoall, c->NB, 0, c->coef, c->NB, 0,
-                              c->orball, c->NO, 0, 1);
+                              c->orball, c->NO, 0, 1, c->tune_gemm[1]);
         tend(c, slot);
         if (rcg) return rcg;
         const int nmx = std::max(c->NUP, c->NDN);
@@ -661,6 +661,24 @@ static int launch_hpsi_jastrow_t(champ_ctx *c, const HArgs &a, int nordc, int na
     return CHAMP_OK;
 }
 
+// wmat[w][i][m] = sum_j slmi[w][spin][i][j] coef[iworbd(j)][m], per spin;
+// multideterminant: wmat[w][i][m] = sum_o V[w][i][o] coef[o][m]
+int launch_wmat(champ_ctx *c)
+{
+    if (c->md)
+        return launch_gemm(c, false, c->NE, c->NB, c->NO, c->md_V, c->NO, (long long)c->NE * c->NO, c->coef, c->NB, 0,
+                           c->wmat, c->NB, (long long)c->NE * c->NB, c->W, c->tune_gemm[2]);
+    for (int iab = 0; iab < 2; iab++) {
+        const int nel = iab ? c->NDN : c->NUP, ish = iab ? c->NUP : 0;
+        if (nel == 0) continue;
+        int rcg = launch_gemm(c, false, nel, c->NB, nel, c->slmi + (size_t)iab * c->NM * c->NM, c->NM,
+                              (long long)2 * c->NM * c->NM, c->cocc[iab], c->NB, 0, c->wmat + (size_t)ish * c->NB,
+                              c->NB, (long long)c->NE * c->NB, c->W, c->tune_gemm[2]);
+        if (rcg) return rcg;
+    }
+    return CHAMP_OK;
+}
+
 // ------------------------------------------------------------------------
 int launch_hpsi_impl(champ_ctx *c, int accumulate)
 {
@@ -693,21 +711,8 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
         hipLaunchKernelGGL(k_pair_scan, dim3(1), dim3(1024), 0, c->stream, c->W, c->pair_count, c->grp_off,
                            c->pair_total + 1, NL_GROUP);
         HIPCHECK(hipGetLastError());
-        // wmat[w][i][m] = sum_j slmi[w][spin][i][j] coef[iworbd(j)][m], per spin;
-        // multideterminant: wmat[w][i][m] = sum_o V[w][i][o] coef[o][m]
-        if (c->md) {
-            int rcg = launch_gemm(c, false, c->NE, c->NB, c->NO, c->md_V, c->NO, (long long)c->NE * c->NO, c->coef,
-                                  c->NB, 0, c->wmat, c->NB, (long long)c->NE * c->NB, c->W);
-            if (rcg) return rcg;
-        }
-        for (int iab = 0; iab < 2 && !c->md; iab++) {
-            const int nel = iab ? c->NDN : c->NUP, ish = iab ? c->NUP : 0;
-            if (nel == 0) continue;
-            int rcg = launch_gemm(c, false, nel, c->NB, nel, c->slmi + (size_t)iab * c->NM * c->NM, c->NM,
-                                  (long long)2 * c->NM * c->NM, c->cocc[iab], c->NB, 0, c->wmat + (size_t)ish * c->NB,
-                                  c->NB, (long long)c->NE * c->NB, c->W);
-            if (rcg) return rcg;
-        }
+        const int rcw = launch_wmat(c);
+        if (rcw) return rcw;
         const int rcn = launch_nonloc(c, a);
         tend(c, slot);
         if (rcn) return rcn;

@@ -131,8 +131,14 @@ struct champ_ctx {
     double *md_Wt = nullptr;    // [W][NU] scratch
     double *md_g = nullptr;     // [W][NV] scratch
 
-    // ---- metrop1 sweep as a HIP graph (captured per tau)
-    int graphs = 1;
+    // ---- choices of the autotuner (champ_gpu_autotune): GEMM kind per call site
+    // (move orbitals, hpsi orbitals, ECP coefficient rows) and quadrature form
+    int tune_gemm[3] = {-1, -1, -1};
+    int tune_nonloc = -1;  // -1: shape rule, 0: matrix cores, 1: VALU form
+
+    // ---- metrop1 sweep as a HIP graph (captured per tau); off by default:
+    // replaying the ~500-node sweep graph measured 5% slower than launching
+    int graphs = 0;
     hipGraphExec_t sweep_exec = nullptr;
     double sweep_tau = 0;
 
@@ -154,8 +160,12 @@ int launch_hpsi(champ_ctx *c);
 int launch_slater_refresh(champ_ctx *c);
 int launch_rotqua(champ_ctx *c);
 int launch_basis_points(champ_ctx *c, int npt, const double *dpos, double *dout);
+// kind: GEMM_AUTO (shape rule), GEMM_TILE (16 x 16 per wavefront), GEMM_STRIP
+// (16 x 80 per wavefront, B chunks in LDS) or GEMM_VALU
+enum { GEMM_AUTO = -1, GEMM_TILE = 0, GEMM_STRIP = 1, GEMM_VALU = 2 };
 int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int lda, long long sA, const double *B,
-                int ldb, long long sB, double *Cm, int ldc, long long sC, int batch);
+                int ldb, long long sB, double *Cm, int ldc, long long sC, int batch, int kind = GEMM_AUTO);
+int launch_wmat(champ_ctx *c);
 int md_init(champ_ctx *c, const champ_system_t *s);
 void md_free(champ_ctx *c);
 int launch_md_hpsi(champ_ctx *c, int w0, int nw);

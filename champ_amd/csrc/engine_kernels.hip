@@ -121,10 +121,13 @@ __global__ void __launch_bounds__(256) k_drift_old(MoveArgs a)
 #define CHAMP_MOVE_WPE 1
 #endif
 constexpr int MJB = CHAMP_MJB;
+// centres whose partner distances are loaded one group ahead: 4 on the FP64
+// path (0.30 against 0.335 ms per move at 160 e x 4096 walkers), 1 on the FP32
+// path (its register budget; 4 ran at 0.32 against 0.24 ms)
 #ifndef CHAMP_MOVE_PF
 #define CHAMP_MOVE_PF 4
 #endif
-constexpr int MOVE_PF = CHAMP_MOVE_PF;  // centres whose partner distances are loaded one group ahead
+template <typename T> constexpr int move_pf() { return sizeof(T) == sizeof(double) ? CHAMP_MOVE_PF : 1; }
 
 // LDS of one wave of k_move_a, in doubles: r, rr, -kap (1 - kap rr) / r,
 // rvec (3) per centre (double), then in the arithmetic type (tsize bytes) the
@@ -304,6 +307,7 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                     const int j = j0 + s * WAVE + lane;
                     jcl[s] = j < NE ? j : 0;
                 }
+                constexpr int MOVE_PF = move_pf<T>();
                 double rjc[MOVE_PF][MJB], rrjc[MOVE_PF][MJB], rjn[MOVE_PF][MJB], rrjn[MOVE_PF][MJB];
 #pragma unroll
                 for (int p = 0; p < MOVE_PF; p++)
@@ -679,7 +683,8 @@ int launch_move_gemm(champ_ctx *c)
     // orbn[w][comp][o] = sum_m aomove[w][comp][m] * coef[o][m]
     int slot;
     tbegin(c, K_MOVE_GEMM, &slot);
-    int rc = launch_gemm(c, true, 4 * c->W, c->NO, c->NB, c->aomove, c->NB, 0, c->coef, c->NB, 0, c->orbn, c->NO, 0, 1);
+    int rc = launch_gemm(c, true, 4 * c->W, c->NO, c->NB, c->aomove, c->NB, 0, c->coef, c->NB, 0, c->orbn, c->NO, 0, 1,
+                         c->tune_gemm[0]);
     tend(c, slot);
     return rc;
 }

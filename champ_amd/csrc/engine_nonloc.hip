@@ -720,7 +720,7 @@ int launch_nonloc(champ_ctx *c, const HArgs &a)
     // the matrix-core form: up to 12 points per pair (48 points per group) and
     // its LDS within 160 KB
     const bool mixed = c->prec == CHAMP_PREC_MIXED;
-    const bool mma = CHAMP_NL_MMA && c->NQ <= 12 && c->hjas.nordc >= 2 && c->hjas.nordc <= 5 &&
+    const bool mma = CHAMP_NL_MMA && c->tune_nonloc != 1 && c->NQ <= 12 && c->hjas.nordc >= 2 && c->hjas.nordc <= 5 &&
                      mma_lds_doubles(c->NC, c->NQ, c->NRI, mixed ? 4 : 8) * 8 <= 160 * 1024;
     if (mma && mixed) {
         switch (c->hjas.nordc) {

@@ -30,6 +30,7 @@ EXPORTS = [
     "champ_gpu_hpsi", "champ_gpu_compute_determinante_grad", "champ_gpu_psie", "champ_gpu_accept",
     "champ_gpu_metrop1", "champ_gpu_get_walker_scalars", "champ_gpu_get_status", "champ_gpu_array_size",
     "champ_gpu_get_array", "champ_gpu_basis_fns", "champ_gpu_kernel_times", "champ_gpu_set_graphs",
+    "champ_gpu_autotune",
 ]
 
 
@@ -73,6 +74,7 @@ def load_library(path: str = LIBPATH):
     L.champ_gpu_basis_fns.argtypes = [vp, i32, dp, dp]
     L.champ_gpu_kernel_times.argtypes = [vp, C.POINTER(i32), C.c_char_p, dp, C.POINTER(C.c_int64)]
     L.champ_gpu_set_graphs.argtypes = [vp, i32]
+    L.champ_gpu_autotune.argtypes = [vp, C.POINTER(i32)]
     _libs[path] = L
     return L
 
@@ -196,8 +198,14 @@ class Engine:
         return out
 
     # ---------------------------------------------------------------- timing
+    def autotune(self) -> Dict[str, int]:
+        """Time the candidate kernel forms on the current walkers and keep the fastest."""
+        ch = (C.c_int32 * 4)()
+        self._check(self.L.champ_gpu_autotune(self.ptr, ch), "autotune")
+        return dict(move_gemm=ch[0], orbital_gemm=ch[1], ecp_rows_gemm=ch[2], quadrature=ch[3])
+
     def set_graphs(self, on: bool = True):
-        """metrop1 replays a captured HIP graph per sweep (default) or launches eagerly."""
+        """metrop1 replays a captured HIP graph per sweep, or launches eagerly (default)."""
         self._check(self.L.champ_gpu_set_graphs(self.ptr, 1 if on else 0), "set_graphs")
 
     def enable_timing(self, on: bool = True):

@@ -193,10 +193,19 @@ int champ_gpu_accept(champ_ctx *ctx, int32_t iel, const uint8_t *accept);
 int champ_gpu_metrop1(champ_ctx *ctx, int32_t nsweep, double tau,
                       champ_sweep_stats *stats);
 
-/* metrop1 replays each sweep as a HIP graph captured on first use (per tau);
- * on = 0 launches every kernel from the host instead.  No counterpart in the
- * reference.  Kernel-class timing (champ_gpu_kernel_times) always runs
- * without the graph.                                                          */
+/* Autotuner: times each candidate kernel form of the tunable launches on the
+ * context's current walkers and keeps the fastest for the context's later
+ * calls.  choices (may be NULL) [4]: the kernel kind of the AO->MO GEMM of the
+ * moves, of the hpsi orbitals, of the ECP coefficient rows (0: 16 x 16 tiles
+ * per wavefront, 1: 16 x 80 strips with LDS-shared B chunks, 2: VALU), and the
+ * ECP quadrature form (0: matrix cores, 1: VALU).  Leaves the walker state as
+ * it was (hpsi is recomputed).  No counterpart in the reference.              */
+int champ_gpu_autotune(champ_ctx *ctx, int32_t *choices);
+
+/* on = 1: metrop1 replays each sweep as a HIP graph captured on first use
+ * (per tau); on = 0 (default): every kernel is launched from the host.  No
+ * counterpart in the reference.  Kernel-class timing (champ_gpu_kernel_times)
+ * always runs without the graph.                                              */
 int champ_gpu_set_graphs(champ_ctx *ctx, int32_t on);
 
 /* Per-walker state readback for checking: psido, psijo, eold, psi2o [nwalk]

@@ -178,3 +178,5 @@ hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t)
 }
 hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
 hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }
+
+hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }

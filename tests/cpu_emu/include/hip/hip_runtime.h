@@ -197,6 +197,7 @@ hipError_t hipEventCreate(hipEvent_t *e);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = nullptr);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
+hipError_t hipEventSynchronize(hipEvent_t e);
 
 // Stream capture: launches between hipStreamBeginCapture and
 // hipStreamEndCapture are recorded (closures over copies of their arguments)

@@ -128,6 +128,20 @@ def test_metrop1_trajectory(require_gpu, name):
         assert np.array_equal(rg[w], pr.orc[w].get_rng())
 
 
+@pytest.mark.parametrize("name,nwalk", [("small", 4), ("c22_md", 3)])
+def test_autotune_keeps_parity(require_gpu, name, nwalk):
+    """The autotuner's kernel choices (GEMM kinds, quadrature form) leave hpsi
+    at the oracle's values."""
+    pr = Pair(name, nwalk, seed=6)
+    pr.quad()
+    ch = pr.eng.autotune()
+    print(f"[autotune {name}] {ch}", flush=True)
+    assert ch["move_gemm"] in (0, 1, 2) and ch["orbital_gemm"] in (0, 1, 2) and ch["ecp_rows_gemm"] in (0, 1, 2)
+    g, o = _hpsi_both(pr)
+    for key in ("psid", "psij", "ekin", "energy"):
+        assert rel(g[key], [d[key] for d in o]) < TOL, key
+
+
 @pytest.mark.parametrize("name", ["small", "small_md"])
 def test_metrop1_graph_replays_match_eager(require_gpu, name):
     """metrop1 through the captured sweep graph (replayed three times, with an
