@@ -94,7 +94,7 @@ def run_path(sy, W, precision, args, label):
 # 4096 walkers): FETCH_SIZE x 2 + WRITE_SIZE x 1 from two rocprofv3 --pmc
 # passes, scaled by the same counters on a device-to-device copy of known size
 # (profiles/k_nonloc_c160_traffic.md)
-NONLOC_TRAFFIC = {("c160", 4096, "fp64"): None, ("c160", 4096, "mixed"): None}
+NONLOC_TRAFFIC = {("c160", 4096, "fp64"): 2.82e9, ("c160", 4096, "mixed"): 2.80e9}
 
 
 def roofline(sy, W, kt, nxq, path):

@@ -607,7 +607,9 @@ extern "C" int champ_gpu_autotune(champ_ctx *c, int32_t *choices)
     c->timing = false;
     int rc = launch_hpsi_impl(c, 0);  // every input of the tuned launches in place
     int (*site[3])(champ_ctx *) = {tune_move_gemm, tune_orb_gemm, tune_wmat};
-    for (int s = 0; s < 3 && !rc; s++) {
+    // the ECP coefficient rows (wmat) exist only with a non-local pseudopotential
+    const int nsite = c->nloc > 0 ? 3 : 2;
+    for (int s = 0; s < nsite && !rc; s++) {
         float best = 1e30f;
         int pick = GEMM_AUTO;
         for (int kind : {GEMM_TILE, GEMM_STRIP, GEMM_VALU}) {

@@ -665,6 +665,10 @@ static int launch_hpsi_jastrow_t(champ_ctx *c, const HArgs &a, int nordc, int na
 // multideterminant: wmat[w][i][m] = sum_o V[w][i][o] coef[o][m]
 int launch_wmat(champ_ctx *c)
 {
+    if (!c->wmat) {
+        set_error("wmat: no ECP coefficient rows (all-electron system)");
+        return CHAMP_ERR_STATE;
+    }
     if (c->md)
         return launch_gemm(c, false, c->NE, c->NB, c->NO, c->md_V, c->NO, (long long)c->NE * c->NO, c->coef, c->NB, 0,
                            c->wmat, c->NB, (long long)c->NE * c->NB, c->W, c->tune_gemm[2]);
