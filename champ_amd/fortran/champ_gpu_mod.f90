@@ -106,6 +106,16 @@ module champ_gpu_mod
       real(c_double), value :: tau
       type(champ_sweep_stats), intent(out) :: stats
     end function
+    integer(c_int) function champ_gpu_autotune(ctx, choices) bind(c, name="champ_gpu_autotune")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: choices(4)
+    end function
+    integer(c_int) function champ_gpu_set_graphs(ctx, on) bind(c, name="champ_gpu_set_graphs")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function
     integer(c_int) function champ_gpu_get_walker_scalars(ctx, psido, psijo, eold, psi2o) &
         bind(c, name="champ_gpu_get_walker_scalars")
       import :: c_int, c_ptr, c_double
@@ -116,7 +126,8 @@ module champ_gpu_mod
 
   public :: champ_gpu_init, champ_gpu_free, champ_gpu_set_walkers, champ_gpu_get_walkers, champ_gpu_seed_rng
   public :: champ_gpu_rotqua, champ_gpu_hpsi, champ_gpu_compute_determinante_grad, champ_gpu_psie
-  public :: champ_gpu_accept, champ_gpu_metrop1, champ_gpu_get_walker_scalars
+  public :: champ_gpu_accept, champ_gpu_metrop1, champ_gpu_get_walker_scalars, champ_gpu_autotune
+  public :: champ_gpu_set_graphs
   public :: gpu_check, gpu_hpsi, gpu_psie, gpu_compute_determinante_grad, gpu_metrop1
 
 contains
