@@ -170,7 +170,13 @@ def test_metrop1_graph_replays_match_eager(require_gpu, name):
         out.append((s1, s2, h, e.get_walkers(), e.scalars()))
     (a1, a2, ha, xa, sa), (b1, b2, hb, xb, sb) = out
     print(f"[graph {name}] accepted {a1['accepted']} {b1['accepted']} esum {a1['esum']} {b1['esum']}", flush=True)
-    assert a1 == b1 and a2 == b2
+    # walkers, per-walker energies and acceptance counts are bit-identical; the
+    # estimator sums are accumulated across walkers by atomic adds, whose order
+    # is not fixed, so they agree to rounding
+    for sa_, sb_ in ((a1, b1), (a2, b2)):
+        assert sa_["accepted"] == sb_["accepted"] and sa_["nflagged"] == sb_["nflagged"]
+        for k in ("esum", "esum2"):
+            assert abs(sa_[k] - sb_[k]) <= 1e-12 * max(1.0, abs(sb_[k])), k
     assert np.array_equal(xa, xb)
     assert np.array_equal(ha["energy"], hb["energy"])
     assert np.array_equal(sa["eold"], sb["eold"])
