@@ -18,6 +18,11 @@ same line.  The timed region launches every kernel with HIP events around
 each kernel class (the per-kernel times and the roofline come from it); the
 same sweeps replayed from the captured HIP graph are timed after it.
 
+A second workload, reported beside the headline (`heterogeneous_batch`):
+eight molecules of 20-300 electrons with 4096 down to 512 walkers swept
+together (champ_gpu_metrop1_batch, FP64 and mixed) against the same
+populations swept one molecule after the other on the FP64 path.
+
 Prints one JSON line (walker-moves/s = walkers * electrons * sweeps / s).
 """
 from __future__ import annotations
@@ -90,6 +95,57 @@ def run_path(sy, W, precision, args, label):
             "graph_value": None if gwall is None else W * sy.nelec * args.steps / gwall, "tuned": tuned}
 
 
+def run_hetero(args):
+    """Eight molecules (20-300 electrons, Nao ~ 3.75 Ne, ECP, 4096 down to 512
+    walkers) swept together with champ_gpu_metrop1_batch, each population on
+    its own stream, beside the same populations swept one molecule after the
+    other on the FP64 path.  Shape rules, no autotuning."""
+    from champ_amd import system as S
+    from champ_amd.engine import PREC_FP64, PREC_MIXED, Engine, metrop1_batch
+
+    mols = S.hetero_batch(scale=args.hetero_scale)
+    moves = sum(W * sy.nelec for sy, W in mols)
+    steps = args.hetero_steps
+
+    def make(prec):
+        engs = []
+        for sy, W in mols:
+            e = Engine(sy, W, precision=prec)
+            e.set_walkers(sy.initial_walkers(W, seed=1))
+            e.seed_rng(S.seed_from_string())
+            e.hpsi()
+            engs.append(e)
+        return engs
+
+    out = {"molecules": [{"electrons": sy.nelec, "aos": sy.nbasis, "walkers": W} for sy, W in mols],
+           "walker_moves_per_sweep": moves, "steps": steps, "unit": "walker-moves/s"}
+    engs = make(PREC_FP64)
+    for e in engs:
+        e.metrop1(1, args.tau)
+    t0 = time.perf_counter()
+    for e in engs:
+        e.metrop1(steps, args.tau)
+    t_seq = time.perf_counter() - t0
+    for e in engs:
+        e.close()
+    out["sequential_fp64"] = round(moves * steps / t_seq, 1)
+    print(f"[bench] hetero: {len(mols)} molecules, {moves} walker-moves per sweep; sequential fp64 {steps} sweeps in "
+          f"{t_seq:.3f}s", flush=True)
+    for prec, key in ((PREC_FP64, "batched_fp64"), (PREC_MIXED, "batched_mixed")):
+        engs = make(prec)
+        metrop1_batch(engs, 1, args.tau)
+        t0 = time.perf_counter()
+        metrop1_batch(engs, steps, args.tau)
+        tb = time.perf_counter() - t0
+        for e in engs:
+            e.close()
+        out[key] = round(moves * steps / tb, 1)
+        print(f"[bench] hetero: {key} {steps} sweeps in {tb:.3f}s", flush=True)
+    out["batched_mixed_over_sequential_fp64"] = round(out["batched_mixed"] / out["sequential_fp64"], 3)
+    out["batched_fp64_over_sequential_fp64"] = round(out["batched_fp64"] / out["sequential_fp64"], 3)
+    return out
+
+
 # HBM bytes per launch of the quadrature kernel at the default workload (c160,
 # 4096 walkers): FETCH_SIZE x 2 + WRITE_SIZE x 1 from two rocprofv3 --pmc
 # passes, scaled by the same counters on a device-to-device copy of known size
@@ -141,6 +197,10 @@ def main():
                     help="keep the shape rules instead of timing the candidate kernel forms first")
     ap.add_argument("--no-graph-check", dest="graph_check", action="store_false",
                     help="skip the second timing of the sweeps replayed from the captured HIP graph")
+    ap.add_argument("--no-hetero", dest="hetero", action="store_false",
+                    help="skip the heterogeneous batch (eight molecules of 20-300 electrons swept together)")
+    ap.add_argument("--hetero-steps", type=int, default=2)
+    ap.add_argument("--hetero-scale", type=float, default=1.0, help="scale of the batch's walker counts")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs the walker population on one GPU; "
@@ -174,6 +234,8 @@ def main():
                "vmc_mean_note": f"mean E_L over the {args.steps} timed sweeps of each run (same seeds)"}
         print(f"[bench] accuracy: |dE_L| max {acc['dE_L_max_Ha']:.3e} Ha p99 {acc['dE_L_p99_Ha']:.3e} "
               f"mean {acc['dE_L_mean_Ha']:.3e}; |dln psi| max {acc['dlnpsi_max']:.3e}", flush=True)
+
+    het = run_hetero(args) if args.hetero else None
 
     cpu = None
     if not args.no_cpu:
@@ -222,6 +284,7 @@ def main():
         "accuracy_mixed_vs_fp64": acc,
         "roofline": roofline(sy, W, head["kt"], head["nxq"], "fp64" if rmx is None else "mixed"),
         "roofline_fp64_path": roofline(sy, W, r64["kt"], r64["nxq"], "fp64"),
+        "heterogeneous_batch": het,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

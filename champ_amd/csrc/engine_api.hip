@@ -216,6 +216,11 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
         TRY(upload(&c->ao_ri, aori.data(), NB));
     }
     TRY(upload(&c->coef, s->coef, (size_t)NO * NB));
+    if (precision == CHAMP_PREC_MIXED) {
+        std::vector<float> cf((size_t)NO * NB);
+        for (size_t t = 0; t < cf.size(); t++) cf[t] = (float)s->coef[t];
+        TRY(upload(&c->coef32, cf.data(), cf.size()));
+    }
     TRY(upload(&c->iworbd, s->iworbd, (size_t)NE));
     for (int iab = 0; iab < 2; iab++) {
         const int nel = iab ? c->NDN : c->NUP, ish = iab ? c->NUP : 0;
@@ -357,7 +362,7 @@ extern "C" int champ_gpu_free(champ_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->cent, c->iwctype, (void *)c->tab.rwf, (void *)c->tab.d2rwf, (void *)c->tab.ce, (void *)c->tab.ae,
                     (void *)c->tab.rmaxwf, c->rmax_ao, c->ao_cent, c->ao_type, c->ao_rad, c->ao_slm, c->coef,
-                    c->cocc[0], c->cocc[1], c->iworbd, c->jas, c->ecp, c->xq0, c->yq0, c->zq0, c->xT, c->r_en,
+                    c->cocc[0], c->cocc[1], c->coef32, c->iworbd, c->jas, c->ecp, c->xq0, c->yq0, c->zq0, c->xT, c->r_en,
                     c->rr_en, c->fso, c->g1, c->g2, c->vjT, c->fsumo, c->d2o, c->dorb, c->slmi, c->detiab, c->ddx,
                     c->d2dx2, c->psido, c->psijo, c->psi2o, c->eold, c->ekin, c->pe_local, c->vpsp, c->status,
                     c->rng, c->xq, c->yq, c->zq, c->xnew, c->dfus2o, c->rnew, c->rrnew, c->fsn_row, c->g1n, c->g2n,
@@ -528,22 +533,51 @@ extern "C" int champ_gpu_accept(champ_ctx *c, int32_t iel, const uint8_t *accept
     return CHAMP_OK;
 }
 
+// One single-electron move of every walker (drift, proposal, ratio, decision,
+// commit) and the periodic refresh of the Slater inverses.
+static int launch_move_step(champ_ctx *c, int iel)
+{
+    int rc;
+    if ((rc = launch_md_pre_timed(c, iel))) return rc;
+    if ((rc = launch_move_a(c, iel, 0))) return rc;
+    if ((rc = launch_move_gemm(c))) return rc;
+    if ((rc = launch_md_post_timed(c, iel))) return rc;
+    if ((rc = launch_move_b(c, iel, 0))) return rc;
+    if (c->lu_every > 0 && (iel + 1) % c->lu_every == 0 && iel + 1 < c->NE && (rc = launch_slater_refresh(c)))
+        return rc;
+    return CHAMP_OK;
+}
+
 // One metrop1 sweep: rotqua, Ne single-electron moves, hpsi accumulating the
 // local energy.  Every launch argument depends only on the context and tau.
 static int launch_sweep(champ_ctx *c)
 {
     int rc;
     if (c->nloc > 0 && (rc = launch_rotqua(c))) return rc;
-    for (int iel = 0; iel < c->NE; iel++) {
-        if ((rc = launch_md_pre_timed(c, iel))) return rc;
-        if ((rc = launch_move_a(c, iel, 0))) return rc;
-        if ((rc = launch_move_gemm(c))) return rc;
-        if ((rc = launch_md_post_timed(c, iel))) return rc;
-        if ((rc = launch_move_b(c, iel, 0))) return rc;
-        if (c->lu_every > 0 && (iel + 1) % c->lu_every == 0 && iel + 1 < c->NE && (rc = launch_slater_refresh(c)))
-            return rc;
-    }
+    for (int iel = 0; iel < c->NE; iel++)
+        if ((rc = launch_move_step(c, iel))) return rc;
     return launch_hpsi_impl(c, 1);
+}
+
+// Estimator sums of the sweeps since the last clear, and the flagged walkers.
+static int sweep_stats(champ_ctx *c, int32_t nsweep, champ_sweep_stats *stats)
+{
+    double h[4];
+    HIPCHECK(hipMemcpyAsync(h, c->esum_dev, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    tflush(c);
+    if (stats) {
+        std::vector<int> st(c->W);
+        HIPCHECK(hipMemcpy(st.data(), c->status, sizeof(int) * c->W, hipMemcpyDeviceToHost));
+        int64_t nf = 0;
+        for (int v : st) nf += v != 0;
+        stats->accepted = h[0];
+        stats->proposed = (double)nsweep * c->W * c->NE;
+        stats->esum = h[1];
+        stats->esum2 = h[2];
+        stats->nflagged = nf;
+    }
+    return CHAMP_OK;
 }
 
 // The sweep captured once per tau as a HIP graph and replayed; it holds
@@ -586,7 +620,17 @@ static int time_launch(champ_ctx *c, int (*fn)(champ_ctx *), int reps, float *ms
     hipEventDestroy(e1);
     return rc;
 }
-static int tune_move_gemm(champ_ctx *c) { return launch_move_gemm(c); }
+// the per-move contraction timed in its setting: a compute-only move of
+// electron 0 to its own position (move kernel A, the contraction, the
+// multideterminant column, the ratio); the walkers' state is left unchanged
+static int tune_move_gemm(champ_ctx *c)
+{
+    int rc = launch_move_a(c, 0, 1);
+    if (!rc) rc = launch_move_gemm(c);
+    if (!rc && c->md) rc = launch_md_post(c, 0);
+    if (!rc) rc = launch_move_b(c, 0, 1);
+    return rc;
+}
 static int tune_orb_gemm(champ_ctx *c)
 {
     const int nw = std::min(c->hpsi_chunk, c->W);
@@ -606,17 +650,21 @@ extern "C" int champ_gpu_autotune(champ_ctx *c, int32_t *choices)
     const bool timing = c->timing;
     c->timing = false;
     int rc = launch_hpsi_impl(c, 0);  // every input of the tuned launches in place
+    if (!rc) rc = launch_stay(c, 0);
     int (*site[3])(champ_ctx *) = {tune_move_gemm, tune_orb_gemm, tune_wmat};
     // the ECP coefficient rows (wmat) exist only with a non-local pseudopotential
     const int nsite = c->nloc > 0 ? 3 : 2;
     for (int s = 0; s < nsite && !rc; s++) {
         float best = 1e30f;
         int pick = GEMM_AUTO;
-        for (int kind : {GEMM_TILE, GEMM_STRIP, GEMM_VALU}) {
-            c->tune_gemm[s] = kind;
+        // the mixed path's per-move contraction may also run on FP32 operands
+        const int nk = (s == 0 && c->prec == CHAMP_PREC_MIXED) ? 4 : 3;
+        const int kinds[4] = {GEMM_TILE, GEMM_STRIP, GEMM_VALU, GEMM_F32};
+        for (int ik = 0; ik < nk; ik++) {
+            c->tune_gemm[s] = kinds[ik];
             float ms = 0;
-            if ((rc = time_launch(c, site[s], 3, &ms))) break;
-            if (ms < best) { best = ms; pick = kind; }
+            if ((rc = time_launch(c, site[s], 10, &ms))) break;
+            if (ms < best) { best = ms; pick = kinds[ik]; }
         }
         c->tune_gemm[s] = pick;
     }
@@ -626,7 +674,7 @@ extern "C" int champ_gpu_autotune(champ_ctx *c, int32_t *choices)
         for (int kind : {0, 1}) {
             c->tune_nonloc = kind;
             float ms = 0;
-            if ((rc = time_launch(c, tune_nonloc, 2, &ms))) break;
+            if ((rc = time_launch(c, tune_nonloc, 3, &ms))) break;
             if (ms < best) { best = ms; pick = kind; }
         }
         c->tune_nonloc = pick;
@@ -664,21 +712,42 @@ extern "C" int champ_gpu_metrop1(champ_ctx *c, int32_t nsweep, double tau, champ
             if (c->timing && (isw % 4) == 3) tflush(c);
         }
     }
-    double h[4];
-    HIPCHECK(hipMemcpyAsync(h, c->esum_dev, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
-    tflush(c);
-    if (stats) {
-        std::vector<int> st(c->W);
-        HIPCHECK(hipMemcpy(st.data(), c->status, sizeof(int) * c->W, hipMemcpyDeviceToHost));
-        int64_t nf = 0;
-        for (int v : st) nf += v != 0;
-        stats->accepted = h[0];
-        stats->proposed = (double)nsweep * c->W * c->NE;
-        stats->esum = h[1];
-        stats->esum2 = h[2];
-        stats->nflagged = nf;
+    return sweep_stats(c, nsweep, stats);
+}
+
+extern "C" int champ_gpu_metrop1_batch(champ_ctx *const *ctxs, int32_t n, int32_t nsweep, double tau,
+                                       champ_sweep_stats *stats)
+{
+    if (!ctxs || n < 1 || nsweep < 0 || !(tau > 0)) return CHAMP_ERR_ARG;
+    int nemax = 0;
+    for (int m = 0; m < n; m++) {
+        if (!ctxs[m]) return CHAMP_ERR_ARG;
+        for (int k = 0; k < m; k++)
+            if (ctxs[k] == ctxs[m]) { set_error("metrop1_batch: a context appears twice"); return CHAMP_ERR_ARG; }
+        nemax = std::max(nemax, ctxs[m]->NE);
     }
+    set_tau(tau);
+    for (int m = 0; m < n; m++) HIPCHECK(hipMemsetAsync(ctxs[m]->esum_dev, 0, sizeof(double) * 4, ctxs[m]->stream));
+    // Each population's kernels go to its own stream in the order of its
+    // own metrop1 sweep; the host deals the move steps round-robin over the
+    // populations so that all of them have work queued at once, and queues
+    // a population's hpsi right after its last electron.
+    int rc;
+    for (int isw = 0; isw < nsweep; isw++) {
+        for (int m = 0; m < n; m++)
+            if (ctxs[m]->nloc > 0 && (rc = launch_rotqua(ctxs[m]))) return rc;
+        for (int iel = 0; iel < nemax; iel++)
+            for (int m = 0; m < n; m++) {
+                champ_ctx *c = ctxs[m];
+                if (iel >= c->NE) continue;
+                if ((rc = launch_move_step(c, iel))) return rc;
+                if (iel == c->NE - 1 && (rc = launch_hpsi_impl(c, 1))) return rc;
+            }
+        for (int m = 0; m < n; m++)
+            if (ctxs[m]->timing && (isw % 4) == 3) tflush(ctxs[m]);
+    }
+    for (int m = 0; m < n; m++)
+        if ((rc = sweep_stats(ctxs[m], nsweep, stats ? stats + m : nullptr))) return rc;
     return CHAMP_OK;
 }
 

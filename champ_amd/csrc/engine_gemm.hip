@@ -137,6 +137,80 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int M, int N, int K, const do
     }
 }
 
+// FP32 operands, FP32 accumulation, FP64 result: the per-move AO->MO
+// contraction of the mixed path on v_mfma_f32_16x16x4f32 (A[l % 16][l / 16],
+// B[l / 16][l % 16] as above; accumulator element r of lane l holds
+// D[4 (l / 16) + r][l % 16], measured on gfx950).  Same k-slot pairing as
+// k_gemm_mfma: each lane reads 4 consecutive k of its row (16 bytes).
+template <bool BT, int WN>
+__global__ void __launch_bounds__(256) k_gemm_mfma_f32(int M, int N, int K, const float *__restrict__ A, int lda,
+                                                       const float *__restrict__ B, int ldb, double *__restrict__ Cm,
+                                                       int ldc)
+{
+    const int lane = threadIdx.x % WAVE;
+    const int nrt = (M + 15) / 16, ncg = (N + 16 * WN - 1) / (16 * WN);
+    const long long g = (long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (g >= (long long)nrt * ncg) return;  // whole wavefront
+    const int cg = (int)(g % ncg), rt = (int)(g / ncg);
+    const int row = rt * 16 + lane % 16, kq = 4 * (lane / 16);
+    const bool rok = row < M;
+    const float *Ar = A + (long long)(rok ? row : 0) * lda;
+    int col[WN];
+    bool cok[WN];
+#pragma unroll
+    for (int t = 0; t < WN; t++) {
+        col[t] = cg * 16 * WN + t * 16 + lane % 16;
+        cok[t] = col[t] < N;
+    }
+    flt4 acc[WN];
+#pragma unroll
+    for (int t = 0; t < WN; t++) acc[t] = flt4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const int kb = k0 + kq;
+        float a[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) a[s] = (rok && kb + s < K) ? Ar[kb + s] : 0.0f;
+#pragma unroll
+        for (int t = 0; t < WN; t++) {
+            float b[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const bool ok = cok[t] && kb + s < K;
+                const long long off = BT ? (long long)(cok[t] ? col[t] : 0) * ldb + (kb + s < K ? kb + s : 0)
+                                         : (long long)(kb + s < K ? kb + s : 0) * ldb + (cok[t] ? col[t] : 0);
+                b[s] = ok ? B[off] : 0.0f;
+            }
+#pragma unroll
+            for (int s = 0; s < 4; s++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], b[s], acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < WN; t++) {
+        if (!cok[t]) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int ro = rt * 16 + 4 * (lane / 16) + r;
+            if (ro < M) Cm[(long long)ro * ldc + col[t]] = (double)acc[t][r];
+        }
+    }
+}
+
+int launch_gemm_f32(champ_ctx *c, int M, int N, int K, const float *A, int lda, const float *B, int ldb, double *Cm,
+                    int ldc)
+{
+    if (M <= 0 || N <= 0) return CHAMP_OK;
+    const long long waves = (long long)((M + 15) / 16) * ((N + 15) / 16);
+    const long long blocks = (waves + 3) / 4;
+    if (blocks > 0x7fffffffLL) {
+        set_error("gemm: problem too large for one launch");
+        return CHAMP_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL((k_gemm_mfma_f32<true, 1>), dim3((unsigned)blocks), dim3(256), 0, c->stream, M, N, K, A, lda, B,
+                       ldb, Cm, ldc);
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
 // Strip variant: each wavefront owns 16 rows x (16 WN) columns, so every A
 // value it loads feeds WN instructions; the block's four wavefronts share the
 // B chunk of 16 k x (16 WN) columns through LDS (double-buffered, one barrier

@@ -27,7 +27,7 @@ constexpr int HJB = CHAMP_HJB;
 // FP64 path, float on the mixed path); distances, e-n terms and all sums stay
 // double.
 template <int NORDC, typename T>
-__global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi_jastrow(HArgs a)
+__global__ void __launch_bounds__(512) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi_jastrow(HArgs a)
 {
     HIP_DYNAMIC_SHARED(double, lds)
     __shared__ double red[16];
@@ -641,7 +641,11 @@ int launch_slater_refresh(champ_ctx *c)
 template <typename T>
 static int launch_hpsi_jastrow_t(champ_ctx *c, const HArgs &a, int nordc, int naz)
 {
-    const size_t lds = nordc > 1 ? (size_t)4 * c->NC * (3 * naz + 8) * sizeof(T) : 8;
+    // up to 8 wavefronts per walker (one LDS row-table region each), so that
+    // few walkers are in flight per XCD and their distance tables stay in L2
+    const size_t per_wave = nordc > 1 ? (size_t)c->NC * (3 * naz + 8) * sizeof(T) : 8;
+    const int nwv = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024 - 1024) / per_wave));
+    const size_t lds = nwv * per_wave;
     const void *fn = nordc == 5 ? (const void *)k_hpsi_jastrow<5, T> : nordc == 4 ? (const void *)k_hpsi_jastrow<4, T>
                    : nordc == 3 ? (const void *)k_hpsi_jastrow<3, T> : nordc == 2 ? (const void *)k_hpsi_jastrow<2, T>
                                                                             : (const void *)k_hpsi_jastrow<0, T>;
@@ -650,12 +654,13 @@ static int launch_hpsi_jastrow_t(champ_ctx *c, const HArgs &a, int nordc, int na
         return CHAMP_ERR_UNSUPPORTED;
     }
     if (lds > 64 * 1024) HIPCHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const dim3 block(WAVE * nwv);
     switch (nordc) {
-    case 5: hipLaunchKernelGGL((k_hpsi_jastrow<5, T>), dim3(c->W), dim3(256), lds, c->stream, a); break;
-    case 4: hipLaunchKernelGGL((k_hpsi_jastrow<4, T>), dim3(c->W), dim3(256), lds, c->stream, a); break;
-    case 3: hipLaunchKernelGGL((k_hpsi_jastrow<3, T>), dim3(c->W), dim3(256), lds, c->stream, a); break;
-    case 2: hipLaunchKernelGGL((k_hpsi_jastrow<2, T>), dim3(c->W), dim3(256), lds, c->stream, a); break;
-    default: hipLaunchKernelGGL((k_hpsi_jastrow<0, T>), dim3(c->W), dim3(256), lds, c->stream, a); break;
+    case 5: hipLaunchKernelGGL((k_hpsi_jastrow<5, T>), dim3(c->W), block, lds, c->stream, a); break;
+    case 4: hipLaunchKernelGGL((k_hpsi_jastrow<4, T>), dim3(c->W), block, lds, c->stream, a); break;
+    case 3: hipLaunchKernelGGL((k_hpsi_jastrow<3, T>), dim3(c->W), block, lds, c->stream, a); break;
+    case 2: hipLaunchKernelGGL((k_hpsi_jastrow<2, T>), dim3(c->W), block, lds, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_hpsi_jastrow<0, T>), dim3(c->W), block, lds, c->stream, a); break;
     }
     HIPCHECK(hipGetLastError());
     return CHAMP_OK;

@@ -53,6 +53,7 @@ struct champ_ctx {
     int lu_every = 0;  // moves between FP64 recomputes of the Slater inverse inside a sweep (0: only in hpsi)
     int *ao_cent = nullptr, *ao_type = nullptr, *ao_rad = nullptr, *ao_slm = nullptr;  // [NB]
     double *coef = nullptr;          // [NO][NB]
+    float *coef32 = nullptr;         // [NO][NB] FP32 copy for the mixed path's per-move contraction
     double *cocc[2] = {nullptr, nullptr};  // [nel][NB] coef rows of occupied orbitals
     int *iworbd = nullptr;           // [NE]
     champ::JasParams hjas{};
@@ -91,7 +92,7 @@ struct champ_ctx {
     double *g1n = nullptr, *g2n = nullptr;     // [W][3][NE]
     double *vjnT = nullptr;    // [W][3][NE]
     double *fsumn = nullptr;   // [W]
-    double *aomove = nullptr;  // [W][4][NB]
+    double *aomove = nullptr;  // [W][4][NB] (the mixed path stores FP32 values in its first half)
     double *orbn = nullptr;    // [W][4][NO]
     double *ratio = nullptr, *psidn = nullptr, *psijn = nullptr, *vnew = nullptr /*[W][3]*/, *vold = nullptr;
     unsigned char *acc = nullptr;
@@ -156,16 +157,20 @@ int launch_move_a(champ_ctx *c, int iel, int mode);
 int launch_move_gemm(champ_ctx *c);
 int launch_move_b(champ_ctx *c, int iel, int mode);
 int launch_drift_old(champ_ctx *c, int iel);
+int launch_stay(champ_ctx *c, int iel);
 int launch_hpsi(champ_ctx *c);
 int launch_slater_refresh(champ_ctx *c);
 int launch_rotqua(champ_ctx *c);
 int launch_basis_points(champ_ctx *c, int npt, const double *dpos, double *dout);
 // kind: GEMM_AUTO (shape rule), GEMM_TILE (16 x 16 per wavefront), GEMM_STRIP
 // (16 x 80 per wavefront, B chunks in LDS) or GEMM_VALU
-enum { GEMM_AUTO = -1, GEMM_TILE = 0, GEMM_STRIP = 1, GEMM_VALU = 2 };
+enum { GEMM_AUTO = -1, GEMM_TILE = 0, GEMM_STRIP = 1, GEMM_VALU = 2, GEMM_F32 = 3 };
 int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int lda, long long sA, const double *B,
                 int ldb, long long sB, double *Cm, int ldc, long long sC, int batch, int kind = GEMM_AUTO);
 int launch_wmat(champ_ctx *c);
+// FP32 A [M][K] times FP32 B^T (B stored [N][K]) into FP64 C (matrix cores)
+int launch_gemm_f32(champ_ctx *c, int M, int N, int K, const float *A, int lda, const float *B, int ldb, double *Cm,
+                    int ldc);
 int md_init(champ_ctx *c, const champ_system_t *s);
 void md_free(champ_ctx *c);
 int launch_md_hpsi(champ_ctx *c, int w0, int nw);

@@ -37,6 +37,7 @@ struct MoveArgs {
     const double *md_vcol, *md_vcoln;  // [W][NO] d ln psi_D / d phi_o(r_iel), current / proposed
     double *md_S, *md_Sn;              // [W] psi_D / (D_up,ref D_dn,ref), current / proposed
     int *md_cur;                       // [W][2] current buffer of the multideterminant state
+    int ao32;                          // AO block stored as FP32 for the FP32 contraction
 };
 
 static MoveArgs move_args(champ_ctx *c, int iel, int mode, double tau)
@@ -58,6 +59,7 @@ static MoveArgs move_args(champ_ctx *c, int iel, int mode, double tau)
     a.esum_dev = c->esum_dev; a.acc = c->acc; a.status = c->status;
     a.md = c->md; a.md_vcol = c->md_vcol; a.md_vcoln = c->md_vcoln; a.md_S = c->md_S; a.md_Sn = c->md_Sn;
     a.md_cur = c->md_cur;
+    a.ao32 = c->prec == CHAMP_PREC_MIXED && c->tune_gemm[0] == GEMM_F32;
     return a;
 }
 
@@ -392,14 +394,20 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
             Lrad[2 * ri] = wfv[0]; Lrad[2 * ri + 1] = wfv[1];
         }
         __syncwarp();
+        // the AO block as FP64, or as FP32 for the FP32 contraction (a.ao32)
         double *ao = a.aomove + (size_t)w * 4 * a.NB;
+        float *ao32 = (float *)a.aomove + (size_t)w * 4 * a.NB;
         for (int m = lane; m < a.NB; m += WAVE) {
             const int c = a.ao_cent[m], ri = a.ao_ri[m];
             const T wfv[3] = {Lrad[2 * ri], Lrad[2 * ri + 1], T(0)};
             T out[5] = {T(0), T(0), T(0), T(0), T(0)};
             if (wfv[0] != T(0) || wfv[1] != T(0))  // else beyond the cut radius: exactly zero
                 ao_combine<T>(a.ao_slm[m], (T)Lrv[3 * c], (T)Lrv[3 * c + 1], (T)Lrv[3 * c + 2], (T)Lr[c], wfv, 1, out);
-            ao[m] = out[0]; ao[a.NB + m] = out[1]; ao[2 * a.NB + m] = out[2]; ao[3 * a.NB + m] = out[3];
+            if (a.ao32) {
+                for (int k = 0; k < 4; k++) ao32[(size_t)k * a.NB + m] = (float)out[k];
+            } else {
+                for (int k = 0; k < 4; k++) ao[(size_t)k * a.NB + m] = (double)out[k];
+            }
         }
     }
 }
@@ -629,6 +637,22 @@ __global__ void k_rotqua(int W, int nq, uint64_t *rng, const double *xq0, const 
 // launchers
 static inline int nblocks(int n, int per) { return (n + per - 1) / per; }
 
+// xnew of every walker = the current position of electron iel (a proposal of
+// zero length; the autotuner times move steps on it)
+__global__ void k_stay(int W, int NE, int iel, const double *xT, double *xnew)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    for (int k = 0; k < 3; k++) xnew[3 * w + k] = xT[((size_t)w * 3 + k) * NE + iel];
+}
+
+int launch_stay(champ_ctx *c, int iel)
+{
+    hipLaunchKernelGGL(k_stay, dim3(nblocks(c->W, 256)), dim3(256), 0, c->stream, c->W, c->NE, iel, c->xT, c->xnew);
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
 int launch_drift_old(champ_ctx *c, int iel)
 {
     MoveArgs a = move_args(c, iel, 1, g_tau);
@@ -680,11 +704,15 @@ int launch_move_a(champ_ctx *c, int iel, int mode)
 
 int launch_move_gemm(champ_ctx *c)
 {
-    // orbn[w][comp][o] = sum_m aomove[w][comp][m] * coef[o][m]
+    // orbn[w][comp][o] = sum_m aomove[w][comp][m] * coef[o][m]: FP64 matrix cores,
+    // or (mixed path, when the autotuner finds it faster) FP32 operands and accumulation
     int slot;
     tbegin(c, K_MOVE_GEMM, &slot);
-    int rc = launch_gemm(c, true, 4 * c->W, c->NO, c->NB, c->aomove, c->NB, 0, c->coef, c->NB, 0, c->orbn, c->NO, 0, 1,
-                         c->tune_gemm[0]);
+    int rc = (c->prec == CHAMP_PREC_MIXED && c->tune_gemm[0] == GEMM_F32)
+                 ? launch_gemm_f32(c, 4 * c->W, c->NO, c->NB, (const float *)c->aomove, c->NB, c->coef32, c->NB,
+                                   c->orbn, c->NO)
+                 : launch_gemm(c, true, 4 * c->W, c->NO, c->NB, c->aomove, c->NB, 0, c->coef, c->NB, 0, c->orbn, c->NO,
+                               0, 1, c->tune_gemm[0]);
     tend(c, slot);
     return rc;
 }

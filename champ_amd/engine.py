@@ -30,7 +30,7 @@ EXPORTS = [
     "champ_gpu_hpsi", "champ_gpu_compute_determinante_grad", "champ_gpu_psie", "champ_gpu_accept",
     "champ_gpu_metrop1", "champ_gpu_get_walker_scalars", "champ_gpu_get_status", "champ_gpu_array_size",
     "champ_gpu_get_array", "champ_gpu_basis_fns", "champ_gpu_kernel_times", "champ_gpu_set_graphs",
-    "champ_gpu_autotune",
+    "champ_gpu_autotune", "champ_gpu_metrop1_batch",
 ]
 
 
@@ -75,6 +75,7 @@ def load_library(path: str = LIBPATH):
     L.champ_gpu_kernel_times.argtypes = [vp, C.POINTER(i32), C.c_char_p, dp, C.POINTER(C.c_int64)]
     L.champ_gpu_set_graphs.argtypes = [vp, i32]
     L.champ_gpu_autotune.argtypes = [vp, C.POINTER(i32)]
+    L.champ_gpu_metrop1_batch.argtypes = [C.POINTER(vp), i32, i32, C.c_double, C.POINTER(SweepStats)]
     _libs[path] = L
     return L
 
@@ -224,3 +225,22 @@ class Engine:
             nm = names.raw[32 * k:32 * k + 32].split(b"\0", 1)[0].decode()
             out[nm] = (float(ms[k]), int(cnt[k]))
         return out
+
+
+def metrop1_batch(engines, nsweep: int, tau: float = 0.5):
+    """metrop1 over several populations at once (champ_gpu_metrop1_batch):
+    each engine evolves as under its own metrop1 while the GPU overlaps them.
+    Returns one stats dict per engine."""
+    engines = list(engines)
+    if not engines:
+        raise ValueError("metrop1_batch: no engines")
+    L = engines[0].L
+    if any(e.L is not L for e in engines):
+        raise ValueError("metrop1_batch: engines from different library builds")
+    ptrs = (C.c_void_p * len(engines))(*[e.ptr.value for e in engines])
+    st = (SweepStats * len(engines))()
+    rc = L.champ_gpu_metrop1_batch(ptrs, len(engines), int(nsweep), float(tau), st)
+    if rc != 0:
+        raise RuntimeError(f"metrop1_batch failed ({rc}): {L.champ_gpu_last_error().decode()}")
+    return [dict(accepted=s.accepted, proposed=s.proposed, esum=s.esum, esum2=s.esum2, nflagged=s.nflagged)
+            for s in st]

@@ -106,6 +106,14 @@ module champ_gpu_mod
       real(c_double), value :: tau
       type(champ_sweep_stats), intent(out) :: stats
     end function
+    integer(c_int) function champ_gpu_metrop1_batch(ctxs, n, nsweep, tau, stats) &
+        bind(c, name="champ_gpu_metrop1_batch")
+      import :: c_int, c_ptr, c_int32_t, c_double, champ_sweep_stats
+      type(c_ptr), intent(in) :: ctxs(*)
+      integer(c_int32_t), value :: n, nsweep
+      real(c_double), value :: tau
+      type(champ_sweep_stats), intent(out) :: stats(*)
+    end function
     integer(c_int) function champ_gpu_autotune(ctx, choices) bind(c, name="champ_gpu_autotune")
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx
@@ -127,7 +135,7 @@ module champ_gpu_mod
   public :: champ_gpu_init, champ_gpu_free, champ_gpu_set_walkers, champ_gpu_get_walkers, champ_gpu_seed_rng
   public :: champ_gpu_rotqua, champ_gpu_hpsi, champ_gpu_compute_determinante_grad, champ_gpu_psie
   public :: champ_gpu_accept, champ_gpu_metrop1, champ_gpu_get_walker_scalars, champ_gpu_autotune
-  public :: champ_gpu_set_graphs
+  public :: champ_gpu_set_graphs, champ_gpu_metrop1_batch
   public :: gpu_check, gpu_hpsi, gpu_psie, gpu_compute_determinante_grad, gpu_metrop1
 
 contains

@@ -582,6 +582,23 @@ def config(name: str, seed: int = 1234) -> System:
     raise KeyError(name)
 
 
+# Heterogeneous batch: (electrons, heavy atoms, H atoms, walkers).  Heavy
+# atoms carry 4s3p2d (25 AOs) and H 2s1p (5 AOs), the c160 recipe, so
+# Nao ~ 3.75 Ne; walker counts fall from 4096 to 512 as the molecules grow.
+HETERO = [(20, 3, 2, 4096), (40, 5, 5, 3584), (60, 8, 7, 3072), (96, 12, 12, 2560),
+          (128, 16, 16, 2048), (160, 20, 20, 1536), (220, 28, 27, 1024), (300, 38, 37, 512)]
+
+
+def hetero_batch(seed: int = 1234, scale: float = 1.0):
+    """The eight molecules of the heterogeneous batch with their walker counts
+    (scaled by `scale`, at least 64): a list of (System, nwalk)."""
+    out = []
+    for k, (ne, nh_, nhyd, nw) in enumerate(HETERO):
+        sysm = make_system(f"het{ne}", nh_, nhyd, ne, "sssspppdd", "ssp", nloc=4, seed=seed + k)
+        out.append((sysm, max(64, int(nw * scale))))
+    return out
+
+
 def seed_from_string(cseed: str = "18123437465549275475255231234865") -> np.ndarray:
     """The 32-character CHAMP seed read as 8 x i4 (parser.f90:315,654)."""
     if len(cseed) < 32:

@@ -193,12 +193,25 @@ int champ_gpu_accept(champ_ctx *ctx, int32_t iel, const uint8_t *accept);
 int champ_gpu_metrop1(champ_ctx *ctx, int32_t nsweep, double tau,
                       champ_sweep_stats *stats);
 
+/* metrop1 over n populations at once, typically different molecules (each
+ * context its own system, walker count and precision): every context's
+ * kernels run on its own stream in the order of its own champ_gpu_metrop1,
+ * so each population evolves exactly as under champ_gpu_metrop1 with the
+ * same tau, while the GPU overlaps the populations.  stats (may be NULL)
+ * [n].  The contexts must be distinct; sweeps are launched, not replayed
+ * from a graph.  No counterpart in the reference, which runs one molecule
+ * per process.                                                                */
+int champ_gpu_metrop1_batch(champ_ctx *const *ctxs, int32_t n, int32_t nsweep,
+                            double tau, champ_sweep_stats *stats);
+
 /* Autotuner: times each candidate kernel form of the tunable launches on the
  * context's current walkers and keeps the fastest for the context's later
  * calls.  choices (may be NULL) [4]: the kernel kind of the AO->MO GEMM of the
  * moves, of the hpsi orbitals, of the ECP coefficient rows (0: 16 x 16 tiles
- * per wavefront, 1: 16 x 80 strips with LDS-shared B chunks, 2: VALU), and the
- * ECP quadrature form (0: matrix cores, 1: VALU).  Leaves the walker state as
+ * per wavefront, 1: 16 x 80 strips with LDS-shared B chunks, 2: VALU, 3: FP32
+ * operands and accumulation, a candidate for the moves of the mixed path),
+ * and the ECP quadrature form (0: matrix cores, 1: VALU); -1 where nothing was
+ * tuned.  Leaves the walker state as
  * it was (hpsi is recomputed).  No counterpart in the reference.              */
 int champ_gpu_autotune(champ_ctx *ctx, int32_t *choices);
 

@@ -30,7 +30,7 @@ def _pair(name, nwalk, seed):
     return sy, out
 
 
-@pytest.mark.parametrize("name,nwalk", [("small", 8), ("c22", 8)])
+@pytest.mark.parametrize("name,nwalk", [("small", 8), ("c22", 8), ("small_md", 6), ("c22_md", 4)])
 def test_mixed_hpsi_close_to_fp64(require_gpu, name, nwalk):
     sy, (e64, e32) = _pair(name, nwalk, seed=11)
     g64, g32 = e64.hpsi(), e32.hpsi()
@@ -46,12 +46,13 @@ def test_mixed_hpsi_close_to_fp64(require_gpu, name, nwalk):
     assert not np.any(e32.status())
 
 
-def test_mixed_metrop1_runs(require_gpu):
-    sy, (e64, e32) = _pair("small", 16, seed=13)
+@pytest.mark.parametrize("name", ["small", "small_md"])
+def test_mixed_metrop1_runs(require_gpu, name):
+    sy, (e64, e32) = _pair(name, 16, seed=13)
     e64.hpsi(); e32.hpsi()
     s64 = e64.metrop1(3, tau=0.3)
     s32 = e32.metrop1(3, tau=0.3)
-    print(f"[mixed metrop1] acc fp64 {s64['accepted'] / s64['proposed']:.3f} mixed {s32['accepted'] / s32['proposed']:.3f}",
+    print(f"[mixed metrop1 {name}] acc fp64 {s64['accepted'] / s64['proposed']:.3f} mixed {s32['accepted'] / s32['proposed']:.3f}",
           flush=True)
     assert s32["nflagged"] == 0
     assert np.isfinite(s32["esum"])
