@@ -113,6 +113,11 @@ static int upload(T **p, const T *h, size_t n)
 
 using namespace champ;
 
+// LDS bytes up to which the Slater inverse works in LDS (variant builds lower
+// it to exercise the global work-matrix form on small systems)
+#ifndef CHAMP_SLATER_LDS_MAX
+#define CHAMP_SLATER_LDS_MAX (160 * 1024)
+#endif
 #define TRY(x) do { int rc_ = (x); if (rc_) { champ_gpu_free(c); *ctxp = nullptr; return rc_; } } while (0)
 
 extern "C" const char *champ_gpu_last_error(void) { return g_err.c_str(); }
@@ -346,10 +351,11 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
         HIPCHECK(hipMemcpy(c->yq, hy.data(), hy.size() * 8, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(c->zq, hz.data(), hz.size() * 8, hipMemcpyHostToDevice));
     }
-    // large LDS for the Slater inverse
+    // the Slater inverse works in LDS up to ~140 electrons of one spin, above
+    // that in a global work matrix per (walker, spin) of an hpsi chunk
     {
         const size_t lds = (size_t)(NM * (NM + 1) + NM) * sizeof(double);
-        if (lds > 160 * 1024) { set_error("init: Slater matrix does not fit in LDS"); champ_gpu_free(c); *ctxp = nullptr; return CHAMP_ERR_UNSUPPORTED; }
+        if (lds > CHAMP_SLATER_LDS_MAX) TRY(dalloc(&c->lu_ws, (size_t)2 * c->hpsi_chunk * NM * (NM + 1)));
     }
     // excitation tables and state of a multideterminant trial function
     TRY(md_init(c, s));
@@ -369,7 +375,7 @@ extern "C" int champ_gpu_free(champ_ctx *c)
                     c->vjnT, c->fsumn, c->aomove, c->orbn, c->ratio, c->psidn, c->psijn, c->vnew, c->vold, c->acc,
                     c->esum_dev, c->aoall, c->orball, c->wmat, c->pair_count, c->pair_off, c->pair_total, c->grp_off,
                     c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->rrT, c->nxquad, c->ri_cent, c->ri_type,
-                    c->ri_rad, c->ri_rmax, c->ao_ri, c->gref};
+                    c->ri_rad, c->ri_rmax, c->ao_ri, c->gref, c->lu_ws};
     for (void *p : ptrs) if (p) hipFree(p);
     md_free(c);
     if (c->sweep_exec) hipGraphExecDestroy(c->sweep_exec);

@@ -303,9 +303,13 @@ __global__ void __launch_bounds__(256) k_hpsi_ao(HArgs a)
 // matrix in LDS, in-place Gauss-Jordan inverse with partial (row) pivoting,
 // determinant from the pivots with matinv's mantissa/exponent loop
 // (matinv.f90:65-84), then ddx / d2dx2 and the dorb rows for the moves.
+// GLB: the work matrix of more than ~140 electrons of one spin does not fit
+// in LDS; it then lives in the block's slice of a global work buffer (L2
+// resident during the elimination), with the same arithmetic in the same order.
+template <bool GLB>
 __global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
 {
-    HIP_DYNAMIC_SHARED(double, L)
+    HIP_DYNAMIC_SHARED(double, lds)
     __shared__ int perm[512];
     __shared__ double piv[512];
     __shared__ int s_p;
@@ -315,7 +319,8 @@ __global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
     const int NE = a.NE, NO = a.NO;
     const int nel = iab ? NE - a.NUP : a.NUP, ish = iab ? a.NUP : 0;
     const int ld = nel + 1;
-    double *col = L + (size_t)nel * ld;
+    double *L = GLB ? a.lu_ws + (size_t)blockIdx.x * a.NM * (a.NM + 1) : lds;
+    double *col = GLB ? lds : L + (size_t)nel * ld;
     const double *ob = a.orball + (size_t)wc * NE * 5 * NO;
     if (tid == 0) s_bad = 0;
     for (int t = tid; t < nel * nel; t += bd) {
@@ -602,10 +607,17 @@ static int launch_orbitals_slater(champ_ctx *c, HArgs a)
         tend(c, slot);
         if (rcg) return rcg;
         const int nmx = std::max(c->NUP, c->NDN);
-        const size_t lds = (size_t)(nmx * (nmx + 1) + nmx) * sizeof(double);
-        if (lds > 64 * 1024) HIPCHECK(hipFuncSetAttribute((const void *)k_hpsi_slater, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         tbegin(c, K_HPSI_SLATER, &slot);
-        hipLaunchKernelGGL(k_hpsi_slater, dim3(2 * a.nw), dim3(256), lds, c->stream, a);
+        if (a.lu_ws) {
+            hipLaunchKernelGGL(k_hpsi_slater<true>, dim3(2 * a.nw), dim3(256), (size_t)nmx * sizeof(double),
+                               c->stream, a);
+        } else {
+            const size_t lds = (size_t)(nmx * (nmx + 1) + nmx) * sizeof(double);
+            if (lds > 64 * 1024)
+                HIPCHECK(hipFuncSetAttribute((const void *)k_hpsi_slater<false>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(k_hpsi_slater<false>, dim3(2 * a.nw), dim3(256), lds, c->stream, a);
+        }
         tend(c, slot);
         HIPCHECK(hipGetLastError());
         if (c->md) {

@@ -75,6 +75,7 @@ struct champ_ctx {
     double *d2o = nullptr;     // [W]
     double *dorb = nullptr;    // [W][NE][3][NO]
     double *slmi = nullptr;    // [W][2][NM][NM]
+    double *lu_ws = nullptr;   // [2 hpsi_chunk][NM (NM+1)]: Slater work matrices too large for LDS
     double *detiab = nullptr;  // [W][2]
     double *ddx = nullptr;     // [W][NE][3]
     double *d2dx2 = nullptr;   // [W][NE]

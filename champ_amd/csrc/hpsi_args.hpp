@@ -32,6 +32,7 @@ struct HArgs {
     int *status;
     int md;                 // multideterminant: psi_D = D_up,ref D_dn,ref md_S
     const double *md_S;     // [W]
+    double *lu_ws;          // [2 hpsi_chunk][NM (NM+1)] Slater work matrices when they exceed LDS, else null
 };
 
 inline HArgs hargs(champ_ctx *c)
@@ -54,6 +55,7 @@ inline HArgs hargs(champ_ctx *c)
     a.ri_rmax = c->ri_rmax;
     a.xq = c->xq; a.yq = c->yq; a.zq = c->zq; a.esum_dev = c->esum_dev; a.status = c->status;
     a.md = c->md; a.md_S = c->md_S;
+    a.lu_ws = c->lu_ws;
     return a;
 }
 

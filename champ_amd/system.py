@@ -560,6 +560,7 @@ def config(name: str, seed: int = 1234) -> System:
     c22_md: c22 with 60 determinants into 8 virtuals
     md256: 64 heavy (Z_eff=4, ECP), 256 e, Nao = 960 (sssppd), 5000
            determinants from 1-4-fold excitations within 40 virtuals
+    het20 ... het300: the molecules of the heterogeneous batch (HETERO)
     """
     if name == "small_md":
         return make_system("small_md", 2, 2, 10, "ssppd", "sp", nloc=4, seed=seed, ndet=40, nvirt=6)
@@ -579,6 +580,9 @@ def config(name: str, seed: int = 1234) -> System:
         return make_system("small_ae", 2, 2, 10, "ssppd", "sp", nloc=0, seed=seed)
     if name == "c60":
         return make_system("c60", 8, 6, 60, "sssspppdd", "ssp", nloc=4, seed=seed)
+    for k, (ne, nheavy, nhyd, _) in enumerate(HETERO):
+        if name == f"het{ne}":
+            return make_system(name, nheavy, nhyd, ne, "sssspppdd", "ssp", nloc=4, seed=seed + k)
     raise KeyError(name)
 
 
@@ -592,11 +596,7 @@ HETERO = [(20, 3, 2, 4096), (40, 5, 5, 3584), (60, 8, 7, 3072), (96, 12, 12, 256
 def hetero_batch(seed: int = 1234, scale: float = 1.0):
     """The eight molecules of the heterogeneous batch with their walker counts
     (scaled by `scale`, at least 64): a list of (System, nwalk)."""
-    out = []
-    for k, (ne, nh_, nhyd, nw) in enumerate(HETERO):
-        sysm = make_system(f"het{ne}", nh_, nhyd, ne, "sssspppdd", "ssp", nloc=4, seed=seed + k)
-        out.append((sysm, max(64, int(nw * scale))))
-    return out
+    return [(config(f"het{ne}", seed), max(64, int(nw * scale))) for ne, _, _, nw in HETERO]
 
 
 def seed_from_string(cseed: str = "18123437465549275475255231234865") -> np.ndarray:

@@ -46,9 +46,11 @@ def test_hpsi_parity(require_gpu, name, nwalk):
     assert not np.any(pr.eng.status())
 
 
-@pytest.mark.parametrize("name,nwalk", [("small", 5), ("c22", 4), ("small_md", 4), ("c22_md", 3)])
+@pytest.mark.parametrize("name,nwalk", [("small", 5), ("c22", 4), ("small_md", 4), ("c22_md", 3), ("het300", 1)])
 def test_psie_drift_accept_parity(require_gpu, name, nwalk):
-    """psie + compute_determinante_grad + accept/reject over every electron."""
+    """psie + compute_determinante_grad + accept/reject over every electron
+    (het300: 150 electrons per spin, the row-by-row Sherman-Morrison form and
+    the inverse in a global work matrix)."""
     pr = Pair(name, nwalk, seed=5)
     pr.quad()
     _hpsi_both(pr)
@@ -180,6 +182,29 @@ def test_metrop1_graph_replays_match_eager(require_gpu, name):
     assert np.array_equal(xa, xb)
     assert np.array_equal(ha["energy"], hb["energy"])
     assert np.array_equal(sa["eold"], sb["eold"])
+
+
+def test_large_spin_block_parity(require_gpu):
+    """300 electrons (150 per spin, the largest molecule of the heterogeneous
+    batch): the Slater matrices exceed LDS, so the inverse works in a global
+    work matrix.  hpsi, the inverse itself, and one down-spin electron's psie
+    and drift against the oracle."""
+    pr = Pair("het300", 1, seed=5)
+    pr.quad()
+    g, o = _hpsi_both(pr)
+    print(f"[het300] E gpu={g['energy']} orc={[d['energy'] for d in o]}", flush=True)
+    for key in ("psid", "psij", "energy"):
+        assert rel(g[key], [d[key] for d in o]) < TOL, key
+    assert relnorm(pr.eng.array("slmi")[0], pr.orc[0].slmi()) < 1e-9
+    iel = 231
+    x = pr.eng.get_walkers()
+    xnew = x[:, iel, :] + 0.3
+    pd, pj = pr.eng.psie(iel, xnew)
+    vn = pr.eng.compute_determinante_grad(iel, 0)
+    opd, opj = pr.orc[0].psie(iel, xnew[0])
+    assert rel(pd[0], opd) < TOL
+    assert abs(pj[0] - opj) <= TOL * abs(opj)
+    assert relnorm(vn[0], pr.orc[0].grad(iel, 0)) < TOL
 
 
 @pytest.mark.parametrize("nwalk", [2])
