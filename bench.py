@@ -185,7 +185,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=10,
+                    help="untimed sweeps first: the walkers spread from their initial Gaussian clouds and the ECP pair count settles")
     ap.add_argument("--walkers", type=int, default=4096)
     ap.add_argument("--system", default="c160")
     ap.add_argument("--tau", type=float, default=0.3)

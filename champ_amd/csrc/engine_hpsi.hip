@@ -724,7 +724,8 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
     if (rco) return rco;
     a.w0 = 0; a.nw = c->W;
     if (c->nloc > 0) {
-        tbegin(c, K_HPSI_NONLOC, &slot);
+        // vps and the active-pair list, their offsets, the coefficient rows
+        tbegin(c, K_HPSI_ECP, &slot);
         hipLaunchKernelGGL(k_hpsi_vps, dim3(c->W), dim3(256), 0, c->stream, a);
         HIPCHECK(hipGetLastError());
         hipLaunchKernelGGL(k_pair_scan, dim3(1), dim3(1024), 0, c->stream, c->W, c->pair_count, c->pair_off, c->pair_total, 1);
@@ -733,7 +734,9 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
                            c->pair_total + 1, NL_GROUP);
         HIPCHECK(hipGetLastError());
         const int rcw = launch_wmat(c);
+        tend(c, slot);
         if (rcw) return rcw;
+        tbegin(c, K_HPSI_NONLOC, &slot);  // the quadrature kernel alone
         const int rcn = launch_nonloc(c, a);
         tend(c, slot);
         if (rcn) return rcn;

@@ -28,7 +28,7 @@ struct Timer {
 };
 
 enum KClass { K_MOVE_A = 0, K_MOVE_GEMM, K_MOVE_B, K_HPSI_JAS, K_HPSI_AO, K_HPSI_GEMM, K_HPSI_SLATER,
-              K_HPSI_NONLOC, K_HPSI_ELOC, K_MD, K_OTHER, K_NCLASS };
+              K_HPSI_ECP, K_HPSI_NONLOC, K_HPSI_ELOC, K_MD, K_OTHER, K_NCLASS };
 
 }  // namespace champ
 
@@ -117,7 +117,8 @@ struct champ_ctx {
     double *md_c = nullptr;                   // [ndet] cdet with the reordering phase
     int *md_rank[2] = {nullptr, nullptr}, *md_hole[2] = {nullptr, nullptr}, *md_part[2] = {nullptr, nullptr};  // [nu][4]
     int *md_virt[2] = {nullptr, nullptr};     // [nv] orbital of each particle index
-    int *md_woff[2] = {nullptr, nullptr}, *md_wdet[2] = {nullptr, nullptr};  // dets of each spin determinant
+    int *md_woff[2] = {nullptr, nullptr}, *md_woth[2] = {nullptr, nullptr};  // dets of each spin determinant (CSR):
+    double *md_wcf[2] = {nullptr, nullptr};                                  // other spin's determinant, c_k
     int *md_yoff[2] = {nullptr, nullptr}, *md_yent[2] = {nullptr, nullptr};  // cofactors gathered per Y cell
     int *md_kslot[2] = {nullptr, nullptr}, *md_vidx[2] = {nullptr, nullptr};  // [NO] slot / particle index or -1
     int *md_zoff[2] = {nullptr, nullptr}, *md_zent[2] = {nullptr, nullptr};  // (u, b) of each particle (CSR)

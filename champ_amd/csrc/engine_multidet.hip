@@ -46,7 +46,8 @@ struct MdArgs {
     int nu[2], nv[2];
     const int *du, *dd;
     const double *cdet;
-    const int *rank[2], *hole[2], *part[2], *virt[2], *woff[2], *wdet[2], *yoff[2], *yent[2], *kslot[2], *vidx[2];
+    const int *rank[2], *hole[2], *part[2], *virt[2], *woff[2], *woth[2], *yoff[2], *yent[2], *kslot[2], *vidx[2];
+    const double *wcf[2];
     const int *zoff[2], *zent[2];
     const int *iworbd;
     const double *slmi, *orball, *orbn;
@@ -63,7 +64,7 @@ static MdArgs md_args(champ_ctx *c, int iel)
     for (int s = 0; s < 2; s++) {
         m.nu[s] = c->md_nu[s]; m.nv[s] = c->md_nv[s];
         m.rank[s] = c->md_rank[s]; m.hole[s] = c->md_hole[s]; m.part[s] = c->md_part[s]; m.virt[s] = c->md_virt[s];
-        m.woff[s] = c->md_woff[s]; m.wdet[s] = c->md_wdet[s]; m.yoff[s] = c->md_yoff[s]; m.yent[s] = c->md_yent[s];
+        m.woff[s] = c->md_woff[s]; m.woth[s] = c->md_woth[s]; m.wcf[s] = c->md_wcf[s]; m.yoff[s] = c->md_yoff[s]; m.yent[s] = c->md_yent[s];
         m.kslot[s] = c->md_kslot[s]; m.vidx[s] = c->md_vidx[s];
         m.zoff[s] = c->md_zoff[s]; m.zent[s] = c->md_zent[s];
     }
@@ -120,6 +121,82 @@ __device__ __forceinline__ double det4(const double *al)
 #undef M
 }
 
+// det(alpha) of the leading R x R block of alpha (row stride R) and, with
+// COF, its cofactors C[a][b] at a * R + b.  Every index is a compile-time
+// constant, so the matrices stay in registers.
+template <int R, bool COF>
+__device__ __forceinline__ double det_cof_r(const double *al, double *C)
+{
+#define M(i, j) al[(i) * R + (j)]
+    if constexpr (R == 1) {
+        if (COF) C[0] = 1.0;
+        return M(0, 0);
+    } else if constexpr (R == 2) {
+        if (COF) { C[0] = M(1, 1); C[1] = -M(1, 0); C[2] = -M(0, 1); C[3] = M(0, 0); }
+        return M(0, 0) * M(1, 1) - M(0, 1) * M(1, 0);
+    } else if constexpr (R == 3) {
+        double c[9];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                const int r0 = a == 0 ? 1 : 0, r1 = a == 2 ? 1 : 2, c0 = b == 0 ? 1 : 0, c1 = b == 2 ? 1 : 2;
+                const double mn = M(r0, c0) * M(r1, c1) - M(r0, c1) * M(r1, c0);
+                c[a * 3 + b] = ((a + b) & 1) ? -mn : mn;
+            }
+        if (COF)
+#pragma unroll
+            for (int t = 0; t < 9; t++) C[t] = c[t];
+        return M(0, 0) * c[0] + M(0, 1) * c[1] + M(0, 2) * c[2];
+    } else {
+        if (COF) return det_cof4(al, C);
+        return det4(al);
+    }
+#undef M
+}
+
+// alpha_u (R x R, R = rank of u) gathered from aa
+template <int R>
+__device__ __forceinline__ void md_alpha_r(const MdArgs &m, int s, int u, const double *aa, double *al)
+{
+    const int *h = m.hole[s] + 4 * u, *p = m.part[s] + 4 * u;
+#pragma unroll
+    for (int a = 0; a < R; a++)
+#pragma unroll
+        for (int b = 0; b < R; b++) al[a * R + b] = aa[(size_t)h[a] * m.NV + p[b]];
+}
+
+// determinant of unique determinant u of rank R
+template <int R>
+__device__ __forceinline__ double md_det_u(const MdArgs &m, int s, int u, const double *aa)
+{
+    double al[R * R];
+    md_alpha_r<R>(m, s, u, aa, al);
+    return det_cof_r<R, false>(al, nullptr);
+}
+
+// t_u[b] = sum_a C_u[a][b] Ainv[h_a][i] (b < R) into tbu; Ai: row i of the
+// spin's slmi (Ai[j] = Ainv[j][i]).  With D, also det(alpha_u) into *D.
+template <int R>
+__device__ __forceinline__ void md_tb_u(const MdArgs &m, int s, int u, const double *aa, const double *Ai, double *tbu,
+                                        double *D)
+{
+    const int *h = m.hole[s] + 4 * u;
+    double al[R * R], C[R * R], x[R];
+    md_alpha_r<R>(m, s, u, aa, al);
+    const double d = det_cof_r<R, true>(al, C);
+    if (D) *D = d;
+#pragma unroll
+    for (int a = 0; a < R; a++) x[a] = Ai[h[a]];
+#pragma unroll
+    for (int b = 0; b < R; b++) {
+        double t = 0;
+#pragma unroll
+        for (int a = 0; a < R; a++) t += C[a * R + b] * x[a];
+        tbu[b] = t;
+    }
+}
+
 // alpha_u[a][b] = aa[h_a][p_b], padded to 4 x 4 with the identity
 __device__ __forceinline__ void md_alpha(const MdArgs &m, int s, int u, const double *aa, double *al)
 {
@@ -136,9 +213,15 @@ __device__ __forceinline__ void md_alpha(const MdArgs &m, int s, int u, const do
 __device__ void md_dets(const MdArgs &m, int s, const double *aa, double *D)
 {
     for (int u = threadIdx.x; u < m.nu[s]; u += blockDim.x) {
-        double al[16];
-        md_alpha(m, s, u, aa, al);
-        D[u] = det4(al);
+        double d = 1.0;  // rank 0: the reference itself
+        switch (m.rank[s][u]) {
+        case 1: d = md_det_u<1>(m, s, u, aa); break;
+        case 2: d = md_det_u<2>(m, s, u, aa); break;
+        case 3: d = md_det_u<3>(m, s, u, aa); break;
+        case 4: d = md_det_u<4>(m, s, u, aa); break;
+        default: break;
+        }
+        D[u] = d;
     }
 }
 
@@ -164,18 +247,18 @@ __device__ double md_sum(const MdArgs &m, const double *Dup, const double *Ddn, 
 }
 
 // weights W_u = sum_{k using u} c_k D_other,k / S of spin s (Do: the other
-// spin's determinants) into the per-walker scratch Wt
+// spin's determinants) into the per-walker scratch Wt.  The table holds c_k
+// and the other spin's determinant index of each (u, k) entry, so that one
+// gather stands between an entry and its product.
 __device__ void md_weights(const MdArgs &m, int w, int s, const double *Do, double S)
 {
     double *Wt = m.Wt + (size_t)w * m.NU;
-    const int *oth = s == 0 ? m.dd : m.du;
     const double sinv = 1.0 / S;
+    const double *cf = m.wcf[s];
+    const int *oth = m.woth[s];
     for (int u = threadIdx.x; u < m.nu[s]; u += blockDim.x) {
         double t = 0;
-        for (int e = m.woff[s][u]; e < m.woff[s][u + 1]; e++) {
-            const int k = m.wdet[s][e];
-            t += m.cdet[k] * Do[oth[k]];
-        }
+        for (int e = m.woff[s][u]; e < m.woff[s][u + 1]; e++) t += cf[e] * Do[oth[e]];
         Wt[u] = t * sinv;
     }
     __syncthreads();
@@ -209,37 +292,42 @@ __device__ void md_y(const MdArgs &m, int w, int s, const double *aa, int nel)
 
 // Column i of Z = Y Ainv without forming Y:
 //   z[v] = sum_u W_u sum_b [p_b = v] t_u[b],  t_u[b] = sum_a C_u[a][b] Ainv[h_a][i],
-// C_u from aa in registers; t_u into the per-walker scratch tb, then one
-// gather per particle v.  A: the spin's slmi (A[i][j] = Ainv[j][i]).
-__device__ void md_zcol(const MdArgs &m, int w, int s, const double *aa, const double *A, int i, double scale,
-                        double *z)
+// in two passes.  md_tb: C_u from aa in registers, t_u into the per-walker
+// scratch tb (and, with D, the determinants of aa as they come out of the
+// same cofactors).  md_zgather: one gather per particle v.  A: the spin's
+// slmi (A[i][j] = Ainv[j][i]).
+__device__ void md_tb(const MdArgs &m, int w, int s, const double *aa, const double *A, int i, double *D)
 {
     double *tb = m.tb + (size_t)w * m.NU * 4;
+    const double *Ai = A + (size_t)i * m.NM;
     for (int u = threadIdx.x; u < m.nu[s]; u += blockDim.x) {
-        const int r = m.rank[s][u];
-        const int *h = m.hole[s] + 4 * u;
-        double al[16], C[16], x[4];
-        md_alpha(m, s, u, aa, al);
-        det_cof4(al, C);
-#pragma unroll
-        for (int a = 0; a < 4; a++) x[a] = a < r ? A[(size_t)i * m.NM + h[a]] : 0.0;
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-            double t = 0;
-#pragma unroll
-            for (int a = 0; a < 4; a++) t += C[a * 4 + b] * x[a];
-            if (b < r) tb[(size_t)u * 4 + b] = t;
+        double *tbu = tb + (size_t)u * 4, *Du = D ? D + u : nullptr;
+        switch (m.rank[s][u]) {
+        case 1: md_tb_u<1>(m, s, u, aa, Ai, tbu, Du); break;
+        case 2: md_tb_u<2>(m, s, u, aa, Ai, tbu, Du); break;
+        case 3: md_tb_u<3>(m, s, u, aa, Ai, tbu, Du); break;
+        case 4: md_tb_u<4>(m, s, u, aa, Ai, tbu, Du); break;
+        default: if (Du) *Du = 1.0; break;  // the reference: no particles
         }
     }
     __syncthreads();
+}
+
+__device__ void md_zgather(const MdArgs &m, int w, int s, double scale, double *z)
+{
+    // one wavefront per particle v, lanes over its (u, b) entries (about
+    // 1.5 ndet / nv of them), then a fixed-order wave reduction
+    const double *tb = m.tb + (size_t)w * m.NU * 4;
     const double *Wt = m.Wt + (size_t)w * m.NU;
-    for (int v = threadIdx.x; v < m.nv[s]; v += blockDim.x) {
+    const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE, nwv = blockDim.x / WAVE;
+    for (int v = wv; v < m.nv[s]; v += nwv) {
         double x = 0;
-        for (int e = m.zoff[s][v]; e < m.zoff[s][v + 1]; e++) {
+        for (int e = m.zoff[s][v] + lane; e < m.zoff[s][v + 1]; e += WAVE) {
             const int ent = m.zent[s][e];
             x += Wt[ent >> 2] * tb[ent];
         }
-        z[v] = x * scale;
+        x = wave_sum(x);
+        if (lane == 0) z[v] = x * scale;
     }
     __syncthreads();
 }
@@ -330,7 +418,8 @@ __global__ void __launch_bounds__(256) k_md_pre(MdArgs m)
     const double *aa = md_aa(m, w, s, b);
     md_weights(m, w, s, md_D(m, w, 1 - s, bo), m.S[w]);
     double *z = m.Z + (size_t)w * NV * NM;
-    md_zcol(m, w, s, aa, A, i, 1.0, z);
+    md_tb(m, w, s, aa, A, i, nullptr);
+    md_zgather(m, w, s, 1.0, z);
     double *vc = m.vcol + (size_t)w * NO;
     for (int o = tid; o < NO; o += bd) {
         const int j = m.kslot[s][o], v = m.vidx[s][o];
@@ -385,8 +474,8 @@ __global__ void __launch_bounds__(256) k_md_post(MdArgs m)
         aan[(size_t)j * NV + v] = aa[(size_t)j * NV + v] + A[(size_t)i * NM + j] * rinv * gw[v];
     }
     __syncthreads();
-    md_dets(m, s, aan, md_D(m, w, s, bn));
-    __syncthreads();
+    // the spin's determinants and t_u from the same cofactors of aa'
+    md_tb(m, w, s, aan, A, i, md_D(m, w, s, bn));
     const double *Dn = md_D(m, w, s, bn), *Do = md_D(m, w, 1 - s, bo);
     const double Sn = s == 0 ? md_sum(m, Dn, Do, red) : md_sum(m, Do, Dn, red);
     if (tid == 0) m.Sn[w] = Sn;
@@ -396,7 +485,7 @@ __global__ void __launch_bounds__(256) k_md_post(MdArgs m)
     }
     md_weights(m, w, s, Do, Sn);
     double *z = m.Z + (size_t)w * NV * NM;
-    md_zcol(m, w, s, aan, A, i, rinv, z);
+    md_zgather(m, w, s, rinv, z);
     for (int o = tid; o < NO; o += bd) {
         const int j = m.kslot[s][o], v = m.vidx[s][o];
         double x = 0;
@@ -536,6 +625,27 @@ int md_init(champ_ctx *c, const champ_system_t *s)
             }
             uidx[sp][k] = u;
         }
+        // unique determinants in order of rank (stable, the reference stays
+        // first): the lanes of a wavefront then mostly share one rank, and the
+        // per-rank cofactor code of the moves runs without divergence
+        {
+            const int nu = (int)T[sp].rank.size();
+            std::vector<int> order(nu), pos(nu);
+            for (int u = 0; u < nu; u++) order[u] = u;
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return T[sp].rank[x] < T[sp].rank[y]; });
+            std::vector<int> rk(nu), ho((size_t)nu * MD_MAXR), po((size_t)nu * MD_MAXR);
+            for (int n = 0; n < nu; n++) {
+                const int u = order[n];
+                pos[u] = n;
+                rk[n] = T[sp].rank[u];
+                for (int t = 0; t < MD_MAXR; t++) {
+                    ho[(size_t)n * MD_MAXR + t] = T[sp].hole[(size_t)u * MD_MAXR + t];
+                    po[(size_t)n * MD_MAXR + t] = T[sp].part_orb[(size_t)u * MD_MAXR + t];
+                }
+            }
+            T[sp].rank.swap(rk); T[sp].hole.swap(ho); T[sp].part_orb.swap(po);
+            for (int k = 0; k < ndet; k++) uidx[sp][k] = pos[uidx[sp][k]];
+        }
         // virtual orbitals: every particle of this spin
         std::vector<int> isv(NO, 0);
         for (size_t u = 1; u < T[sp].rank.size(); u++)
@@ -562,6 +672,10 @@ int md_init(champ_ctx *c, const champ_system_t *s)
         wdet.resize(ndet);
         std::vector<int> fill(woff.begin(), woff.end() - 1);
         for (int k = 0; k < ndet; k++) wdet[fill[uidx[sp][k]]++] = k;
+        // per entry: c_k and the other spin's unique determinant of det k
+        std::vector<double> wcf(ndet);
+        std::vector<int> woth(ndet);
+        for (int e = 0; e < ndet; e++) { wcf[e] = cd[wdet[e]]; woth[e] = uidx[1 - sp][wdet[e]]; }
         // cofactor entries gathered into each cell (v, j) of Y (CSR over v * NM + j)
         std::vector<std::vector<int>> cell((size_t)NV * NM);
         for (int u = 1; u < nu; u++)
@@ -588,7 +702,8 @@ int md_init(champ_ctx *c, const champ_system_t *s)
         if ((rc = md_upload(&c->md_part[sp], part))) return rc;
         if ((rc = md_upload(&c->md_virt[sp], T[sp].virt))) return rc;
         if ((rc = md_upload(&c->md_woff[sp], woff))) return rc;
-        if ((rc = md_upload(&c->md_wdet[sp], wdet))) return rc;
+        if ((rc = md_upload(&c->md_woth[sp], woth))) return rc;
+        if ((rc = md_upload(&c->md_wcf[sp], wcf))) return rc;
         if ((rc = md_upload(&c->md_yoff[sp], yoff))) return rc;
         if ((rc = md_upload(&c->md_yent[sp], yent))) return rc;
         if ((rc = md_upload(&c->md_kslot[sp], T[sp].kslot))) return rc;
@@ -624,7 +739,8 @@ void md_free(champ_ctx *c)
                   c->md_vcol, c->md_vcoln, c->md_V, c->md_Y, c->md_Z, c->md_Wt, c->md_g};
     for (void *p : ps) if (p) hipFree(p);
     for (int sp = 0; sp < 2; sp++) {
-        void *qs[] = {c->md_rank[sp], c->md_hole[sp], c->md_part[sp], c->md_virt[sp], c->md_woff[sp], c->md_wdet[sp],
+        void *qs[] = {c->md_rank[sp], c->md_hole[sp], c->md_part[sp], c->md_virt[sp], c->md_woff[sp], c->md_woth[sp],
+                      c->md_wcf[sp],
                       c->md_yoff[sp], c->md_yent[sp], c->md_kslot[sp], c->md_vidx[sp], c->md_zoff[sp],
                       c->md_zent[sp]};
         for (void *q : qs) if (q) hipFree(q);
