@@ -147,10 +147,11 @@ def run_hetero(args):
 
 
 # HBM bytes per launch of the quadrature kernel at the default workload (c160,
-# 4096 walkers): FETCH_SIZE x 2 + WRITE_SIZE x 1 from two rocprofv3 --pmc
-# passes, scaled by the same counters on a device-to-device copy of known size
+# 4096 walkers, the six timed sweeps after ten warm-up sweeps): FETCH_SIZE x 2
+# + WRITE_SIZE x 1 from two rocprofv3 --pmc passes over the default bench,
+# scaled by the same counters on a device-to-device copy of known size
 # (profiles/k_nonloc_c160_traffic.md)
-NONLOC_TRAFFIC = {("c160", 4096, "fp64"): 2.82e9, ("c160", 4096, "mixed"): 2.80e9}
+NONLOC_TRAFFIC = {("c160", 4096, "fp64"): 3.46e9, ("c160", 4096, "mixed"): 3.45e9}
 
 
 def roofline(sy, W, kt, nxq, path):

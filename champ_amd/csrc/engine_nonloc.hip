@@ -416,12 +416,24 @@ __device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int c_lo
         }
 }
 
+// Quadrature points per pass of the psi-ratio phase (a whole pair's 12 by
+// default).  Smaller passes shrink that phase's LDS tables so that more blocks
+// stay resident (6 points: 3 blocks per CU instead of 2 at c160), but the
+// kernel is bound by instruction issue, not by occupancy: 6 and 4 points per
+// pass measured 3% and 7% slower.
+#ifndef CHAMP_MMA_QH
+#define CHAMP_MMA_QH 12
+#endif
+constexpr int MMA_QH = CHAMP_MMA_QH;
+static_assert(MMA_QH >= 1 && MMA_QH <= 12, "psi-ratio pass size");
+
 // LDS of k_nonloc_mma in doubles: XP, SP, TG partials, SW, FI, IPT/PI (as
-// doubles), XQ (tsize bytes per entry), then GEO + RAD of phase C
+// doubles), XQ (tsize bytes per entry), then GEO + RAD of one psi-ratio pass
 __host__ __device__ __forceinline__ size_t mma_lds_doubles(int NC, int NQ, int NRI, int tsize)
 {
+    const size_t qh = NQ < MMA_QH ? NQ : MMA_QH;
     return 3 * MMA_MP + 3 * MMA_MP + 2 * MMA_TG * MMA_MP + MMA_WAVES * MMA_MP + MMA_GP + MMA_MP + 4 * MMA_GP +
-           ((size_t)NC * MMA_MP * tsize + 7) / 8 + (size_t)NQ * NC * 5 + (size_t)NQ * NRI;
+           ((size_t)NC * MMA_MP * tsize + 7) / 8 + qh * NC * 5 + qh * NRI;
 }
 
 #ifndef CHAMP_MMA_WPE
@@ -448,8 +460,9 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
     int *IPT = (int *)(FI + MMA_GP);                           // [MP] electron of each point
     int *PI = IPT + 2 * MMA_MP;                                // [GP][4]: i, ic
     T *XQ = (T *)(PI + 8 * MMA_GP);                            // [NC][MP] switched point-centre distances
-    double *GEO = (double *)PI + 4 * MMA_GP + ((size_t)NC * MMA_MP * sizeof(T) + 7) / 8;  // phase C: [NQ][NC][5]
-    double *RAD = GEO + (size_t)NQ * NC * 5;                   //          [NQ][NRI]
+    const int QH = NQ < MMA_QH ? NQ : MMA_QH;                  // points per psi-ratio pass
+    double *GEO = (double *)PI + 4 * MMA_GP + ((size_t)NC * MMA_MP * sizeof(T) + 7) / 8;  // phase C: [QH][NC][5]
+    double *RAD = GEO + (size_t)QH * NC * 5;                   //          [QH][NRI]
 
     const int G = gridDim.x, bk = blockIdx.x;
     const int nx = (G % 8 == 0) ? 8 : 1, gx = G / nx, xcd = bk % nx, g0 = bk / nx;
@@ -569,13 +582,14 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
             SP[MMA_MP + pt] = v;
         }
 
-        // ---- psi ratio of every point, one pair at a time
-        for (int p = 0; p < np; p++) {
+        // ---- psi ratio of every point, QH points of one pair per pass
+        for (int pq = 0; pq < np * ((NQ + QH - 1) / QH); pq++) {
+            const int npass = (NQ + QH - 1) / QH, p = pq / npass, q0 = (pq % npass) * QH, nq = min(QH, NQ - q0);
             const int i = PI[4 * p], ic = PI[4 * p + 1];
-            __syncthreads();  // the previous pair's GEO, RAD and SW are free
-            for (int t = tid; t < NQ * NC; t += MMA_THREADS) {
+            __syncthreads();  // the previous pass's GEO, RAD and SW are free
+            for (int t = tid; t < nq * NC; t += MMA_THREADS) {
                 const int q = t / NC, c = t % NC;
-                const double *xp = XP + 3 * (p * NQ + q);
+                const double *xp = XP + 3 * (p * NQ + q0 + q);
                 const double e0 = xp[0] - a.cent[3 * c], e1 = xp[1] - a.cent[3 * c + 1], e2 = xp[2] - a.cent[3 * c + 2];
                 double r2 = 0;
                 r2 = r2 + e0 * e0; r2 = r2 + e1 * e1; r2 = r2 + e2 * e2;
@@ -585,7 +599,7 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
                 g[4] = (double)grid_jx(a.tab, rqd);
             }
             __syncthreads();
-            for (int t = tid; t < NQ * NRI; t += MMA_THREADS) {
+            for (int t = tid; t < nq * NRI; t += MMA_THREADS) {
                 const int q = t / NRI, ri = t % NRI;
                 const double *g = GEO + ((size_t)q * NC + a.ri_cent[ri]) * 5;
                 double wfv[3] = {0.0, 0.0, 0.0};
@@ -594,7 +608,7 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
             }
             __syncthreads();
             const double *Wm = a.wmat + ((size_t)w * NE + i) * NB;
-            constexpr int QM = 12;
+            constexpr int QM = MMA_QH;
             double dr[QM];
 #pragma unroll
             for (int q = 0; q < QM; q++) dr[q] = 0.0;
@@ -603,7 +617,7 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
                 const double wm = Wm[m];
 #pragma unroll
                 for (int q = 0; q < QM; q++) {
-                    if (q >= NQ) break;
+                    if (q >= nq) break;
                     const double rad = RAD[(size_t)q * NRI + ri];
                     if (rad == 0.0) continue;  // beyond the shell's cut radius: the AO is exactly zero
                     const double *g = GEO + ((size_t)q * NC + c) * 5;
@@ -614,15 +628,15 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
             }
 #pragma unroll
             for (int q = 0; q < QM; q++) {
-                if (q >= NQ) break;
+                if (q >= nq) break;
                 const double v = wave_sum(dr[q]);
                 if (lane == 0) SW[wv * MMA_MP + q] = v;
             }
             __syncthreads();
-            for (int q = tid; q < NQ; q += MMA_THREADS) {
+            for (int q = tid; q < nq; q += MMA_THREADS) {
                 double v = 0.0;
                 for (int k = 0; k < MMA_WAVES; k++) v += SW[k * MMA_MP + q];
-                SP[2 * MMA_MP + p * NQ + q] = v;
+                SP[2 * MMA_MP + p * NQ + q0 + q] = v;
             }
         }
         __syncthreads();
