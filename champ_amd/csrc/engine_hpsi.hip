@@ -354,15 +354,11 @@ __global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
         for (int j = tid; j < nel; j += bd) L[k * ld + j] = (j == k) ? pinv : L[k * ld + j] * pinv;
         for (int i = tid; i < nel; i += bd) col[i] = L[i * ld + k];
         __syncthreads();
-        // rank-one elimination: rows over the waves, columns over the lanes
-        // (no integer division per element)
-        for (int i = wv; i < nel; i += bd / WAVE) {
+        for (int t = tid; t < nel * nel; t += bd) {
+            const int i = t / nel, j = t % nel;
             if (i == k) continue;
-            const double ci = col[i];
-            for (int j = lane; j < nel; j += WAVE) {
-                const double base = (j == k) ? 0.0 : L[i * ld + j];
-                L[i * ld + j] = base - L[k * ld + j] * ci;
-            }
+            const double base = (j == k) ? 0.0 : L[i * ld + j];
+            L[i * ld + j] = base - L[k * ld + j] * col[i];
         }
         __syncthreads();
     }
