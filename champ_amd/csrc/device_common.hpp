@@ -183,6 +183,45 @@ __device__ __forceinline__ void slm(int l, T x, T yy, T z, T &y, T dy[3], T &lap
     }
 }
 
+// The monomial of slm as x^ex y^ey z^ez times a constant, decoded from l
+// without branches (exponents packed 2 bits per l, the constant's class 3
+// bits per l, in slm's order), so that lanes holding different l stay in step.
+struct SlmMono {
+    int ex, ey, ez;
+    double c;
+};
+
+__device__ __forceinline__ SlmMono slm_decode(int l)
+{
+    const bool ok = l >= 1 && l <= 20;
+    const int k = ok ? l - 1 : 0;
+    SlmMono m;
+    m.ex = (int)((0x56b01604ull >> (2 * k)) & 3);
+    m.ey = (int)((0x1b18418410ull >> (2 * k)) & 3);
+    m.ez = (int)((0xe491091040ull >> (2 * k)) & 3);
+    const int cls = (int)((0x96cbadb134da248ull >> (3 * k)) & 7);
+    const double cs = 0.28209479177387814, cp = 0.48860251190291992;
+    const double cd1 = 0.630783130505040, cd2 = 1.092548430592079;
+    const double cf = 0.74635266518023078, cf2 = 1.66889529453113635, cf3 = 2.89061144264055405;
+    double c = cls == 0 ? cs : cls == 1 ? cp : cls == 2 ? cd1 : cls == 3 ? cd2 : cls == 4 ? cf : cls == 5 ? cf2 : cf3;
+    m.c = ok ? c : 0.0;
+    return m;
+}
+
+template <typename T>
+__device__ __forceinline__ T ipow3(T t, int e)
+{
+    const T t2 = t * t;
+    return e == 0 ? T(1) : e == 1 ? t : e == 2 ? t2 : t2 * t;
+}
+
+// value of slm's monomial at (x, y, z)
+template <typename T>
+__device__ __forceinline__ T slm_value(const SlmMono &m, T x, T y, T z)
+{
+    return (T)m.c * ipow3(x, m.ex) * ipow3(y, m.ey) * ipow3(z, m.ez);
+}
+
 // One AO from its radial function R, R', R'' (wfv) and its monomial l at the
 // point (rx, ry, rz) = x - centre, |.| = r: phi_combine (basis_fns.f90:217-296).
 // Returns value, gradient and (ider == 2) Laplacian.

@@ -252,13 +252,13 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
 #pragma unroll
             for (int qq = 0; qq < QB; qq++) dr[qq] = 0.0;
             for (int m = tid; m < NB; m += NL_THREADS) {
-                const int c = a.ao_cent[m], ri = a.ao_ri[m], l = a.ao_slm[m];
+                const int c = a.ao_cent[m], ri = a.ao_ri[m];
+                const SlmMono mono = slm_decode(a.ao_slm[m]);  // once per AO, branch-free
                 const double wm = Wm[m];
 #pragma unroll
                 for (int qq = 0; qq < QB; qq++) {
                     const double *g = GEO + ((size_t)qq * NC + c) * 5;
-                    double y, dy[3], lap;
-                    slm<double>(l, g[0], g[1], g[2], y, dy, lap);
+                    const double y = slm_value<double>(mono, g[0], g[1], g[2]);
                     dr[qq] += (y * RAD[(size_t)qq * NRI + ri]) * wm;
                 }
             }
@@ -613,7 +613,8 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
 #pragma unroll
             for (int q = 0; q < QM; q++) dr[q] = 0.0;
             for (int m = tid; m < NB; m += MMA_THREADS) {
-                const int c = a.ao_cent[m], ri = a.ao_ri[m], l = a.ao_slm[m];
+                const int c = a.ao_cent[m], ri = a.ao_ri[m];
+                const SlmMono mono = slm_decode(a.ao_slm[m]);  // once per AO, branch-free
                 const double wm = Wm[m];
 #pragma unroll
                 for (int q = 0; q < QM; q++) {
@@ -621,8 +622,7 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
                     const double rad = RAD[(size_t)q * NRI + ri];
                     if (rad == 0.0) continue;  // beyond the shell's cut radius: the AO is exactly zero
                     const double *g = GEO + ((size_t)q * NC + c) * 5;
-                    double y, dy[3], lap;
-                    slm<double>(l, g[0], g[1], g[2], y, dy, lap);
+                    const double y = slm_value<double>(mono, g[0], g[1], g[2]);
                     dr[q] += (y * rad) * wm;
                 }
             }
