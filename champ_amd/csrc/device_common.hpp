@@ -149,7 +149,9 @@ __device__ __forceinline__ void splfit(const SplineTab &tab, int it, int irb, T 
     splfit_jx<T>(tab, it, irb, r, grid_jx(tab, (double)r), wfv, ider);
 }
 
-// slm.f90:7-200 (ider < 3): monomial y, gradient dy, Laplacian.
+// slm.f90:7-200 (ider < 3): monomial y, gradient dy, Laplacian, case by
+// case.  The kernels use the branch-free slm_decode / slm_all below, which
+// tests/cpu_emu/slm_check.cpp checks against this form for every l.
 template <typename T>
 __device__ __forceinline__ void slm(int l, T x, T yy, T z, T &y, T dy[3], T &lap)
 {
@@ -222,6 +224,32 @@ __device__ __forceinline__ T slm_value(const SlmMono &m, T x, T y, T z)
     return (T)m.c * ipow3(x, m.ex) * ipow3(y, m.ey) * ipow3(z, m.ez);
 }
 
+// t^e, e t^(e-1) and e (e-1) t^(e-2) for e = 0..3 (zero where the power is negative)
+template <typename T>
+__device__ __forceinline__ void ipow3_d(T t, int e, T &p0, T &p1, T &p2)
+{
+    const T t2 = t * t;
+    p0 = e == 0 ? T(1) : e == 1 ? t : e == 2 ? t2 : t2 * t;
+    p1 = e == 0 ? T(0) : e == 1 ? T(1) : e == 2 ? T(2) * t : T(3) * t2;
+    p2 = e < 2 ? T(0) : e == 2 ? T(2) : T(6) * t;
+}
+
+// slm's monomial y, gradient dy and Laplacian at (x, yy, z), branch-free
+template <typename T>
+__device__ __forceinline__ void slm_all(const SlmMono &m, T x, T yy, T z, T &y, T dy[3], T &lap)
+{
+    T x0, x1, x2, y0, y1, y2, z0, z1, z2;
+    ipow3_d(x, m.ex, x0, x1, x2);
+    ipow3_d(yy, m.ey, y0, y1, y2);
+    ipow3_d(z, m.ez, z0, z1, z2);
+    const T c = (T)m.c;
+    y = c * x0 * y0 * z0;
+    dy[0] = c * x1 * y0 * z0;
+    dy[1] = c * x0 * y1 * z0;
+    dy[2] = c * x0 * y0 * z1;
+    lap = c * (x2 * y0 * z0 + x0 * y2 * z0 + x0 * y0 * z2);
+}
+
 // One AO from its radial function R, R', R'' (wfv) and its monomial l at the
 // point (rx, ry, rz) = x - centre, |.| = r: phi_combine (basis_fns.f90:217-296).
 // Returns value, gradient and (ider == 2) Laplacian.
@@ -229,7 +257,7 @@ template <typename T>
 __device__ __forceinline__ void ao_combine(int l, T rx, T ry, T rz, T r, const T wfv[3], int ider, T out[5])
 {
     T y, dy[3], lap;
-    slm<T>(l, rx, ry, rz, y, dy, lap);
+    slm_all<T>(slm_decode(l), rx, ry, rz, y, dy, lap);  // lanes of one wave hold different l
     out[0] = y * wfv[0];
     if (ider >= 1) {
         T ri = T(1) / r;
