@@ -314,6 +314,7 @@ __global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
     __shared__ double piv[512];
     __shared__ int s_p;
     __shared__ int s_bad;
+    __shared__ double s_akk;
     const int wc = blockIdx.x / 2, iab = blockIdx.x % 2, w = a.w0 + wc;
     const int tid = threadIdx.x, bd = blockDim.x, lane = tid % WAVE, wv = tid / WAVE;
     const int NE = a.NE, NO = a.NO;
@@ -340,19 +341,24 @@ __global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
                 const int oi = __shfl_xor(bi, off, WAVE);
                 if (ob2 > best || (ob2 == best && oi < bi)) { best = ob2; bi = oi; }
             }
-            if (lane == 0) { s_p = bi; perm[k] = bi; }
+            if (lane == 0) {
+                s_p = bi; perm[k] = bi;
+                // pivot and old diagonal, read before any thread writes this step
+                piv[k] = L[bi * ld + k]; s_akk = L[k * ld + k];
+                if (piv[k] == 0.0) s_bad = 1;
+            }
         }
         __syncthreads();
+        // row swap, scaling of the pivot row and the copy of column k in one pass
         const int p = s_p;
-        if (p != k)
-            for (int j = tid; j < nel; j += bd) { const double t = L[k * ld + j]; L[k * ld + j] = L[p * ld + j]; L[p * ld + j] = t; }
-        __syncthreads();
-        const double pv = L[k * ld + k];
-        if (tid == 0) { piv[k] = pv; if (pv == 0.0) s_bad = 1; }
-        const double pinv = 1.0 / pv;
-        __syncthreads();
-        for (int j = tid; j < nel; j += bd) L[k * ld + j] = (j == k) ? pinv : L[k * ld + j] * pinv;
-        for (int i = tid; i < nel; i += bd) col[i] = L[i * ld + k];
+        const double pinv = 1.0 / piv[k], akk = s_akk;
+        for (int j = tid; j < nel; j += bd) {
+            const double rk = L[k * ld + j], rp = L[p * ld + j];
+            L[k * ld + j] = (j == k) ? pinv : rp * pinv;
+            if (p != k) L[p * ld + j] = rk;
+        }
+        // column k after the swap (row k is skipped by the elimination)
+        for (int i = tid; i < nel; i += bd) col[i] = (i == p) ? akk : (i == k ? 0.0 : L[i * ld + k]);
         __syncthreads();
         for (int t = tid; t < nel * nel; t += bd) {
             const int i = t / nel, j = t % nel;
