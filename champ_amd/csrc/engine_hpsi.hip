@@ -329,6 +329,7 @@ __global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
         L[i * ld + j] = ob[((size_t)(ish + i) * 5) * NO + a.iworbd[ish + j]];
     }
     __syncthreads();
+    const int i0 = tid / nel, j0 = tid % nel, si = bd / nel, sj = bd % nel;
     for (int k = 0; k < nel; k++) {
         if (wv == 0) {
             double best = -1.0; int bi = k;
@@ -360,11 +361,15 @@ __global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
         // column k after the swap (row k is skipped by the elimination)
         for (int i = tid; i < nel; i += bd) col[i] = (i == p) ? akk : (i == k ? 0.0 : L[i * ld + k]);
         __syncthreads();
-        for (int t = tid; t < nel * nel; t += bd) {
-            const int i = t / nel, j = t % nel;
-            if (i == k) continue;
-            const double base = (j == k) ? 0.0 : L[i * ld + j];
-            L[i * ld + j] = base - L[k * ld + j] * col[i];
+        // elements t = tid, tid + bd, ... of the nel x nel block, (i, j) stepped
+        // incrementally instead of divided out per element
+        for (int i = i0, j = j0; i < nel;) {
+            if (i != k) {
+                const double base = (j == k) ? 0.0 : L[i * ld + j];
+                L[i * ld + j] = base - L[k * ld + j] * col[i];
+            }
+            i += si; j += sj;
+            if (j >= nel) { j -= nel; i++; }
         }
         __syncthreads();
     }
