@@ -329,7 +329,8 @@ __global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
         L[i * ld + j] = ob[((size_t)(ish + i) * 5) * NO + a.iworbd[ish + j]];
     }
     __syncthreads();
-    const int i0 = tid / nel, j0 = tid % nel, si = bd / nel, sj = bd % nel;
+    const int nd = nel > 0 ? nel : 1;  // element stepping of the elimination (no elimination when nel = 0)
+    const int i0 = tid / nd, j0 = tid % nd, si = bd / nd, sj = bd % nd;
     for (int k = 0; k < nel; k++) {
         if (wv == 0) {
             double best = -1.0; int bi = k;

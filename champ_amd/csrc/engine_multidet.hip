@@ -469,9 +469,14 @@ __global__ void __launch_bounds__(256) k_md_post(MdArgs m)
     }
     __syncthreads();
     const double rinv = 1.0 / R0;
-    for (int t = tid; t < nel * nv; t += bd) {
-        const int j = t / nv, v = t % nv;
-        aan[(size_t)j * NV + v] = aa[(size_t)j * NV + v] + A[(size_t)i * NM + j] * rinv * gw[v];
+    // elements t = tid, tid + bd, ... of the nel x nv block, (j, v) stepped
+    // incrementally (a spin without particle orbitals has no block)
+    if (nv > 0) {
+        for (int j = tid / nv, v = tid % nv; j < nel;) {
+            aan[(size_t)j * NV + v] = aa[(size_t)j * NV + v] + A[(size_t)i * NM + j] * rinv * gw[v];
+            j += bd / nv; v += bd % nv;
+            if (v >= nv) { v -= nv; j++; }
+        }
     }
     __syncthreads();
     // the spin's determinants and t_u from the same cofactors of aa'

@@ -599,12 +599,16 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
                 g[4] = (double)grid_jx(a.tab, rqd);
             }
             __syncthreads();
-            for (int t = tid; t < nq * NRI; t += MMA_THREADS) {
-                const int q = t / NRI, ri = t % NRI;
-                const double *g = GEO + ((size_t)q * NC + a.ri_cent[ri]) * 5;
-                double wfv[3] = {0.0, 0.0, 0.0};
-                if (g[3] <= a.ri_rmax[ri]) splfit_jx<double>(a.tab, a.ri_type[ri], a.ri_rad[ri], g[3], (int)g[4], wfv, 0);
-                RAD[(size_t)q * NRI + ri] = wfv[0];
+            // one radial instance per thread, its table indices read once for all points
+            for (int ri = tid; ri < NRI; ri += MMA_THREADS) {
+                const int cen = a.ri_cent[ri], ty = a.ri_type[ri], rd = a.ri_rad[ri];
+                const double rmax = a.ri_rmax[ri];
+                for (int q = 0; q < nq; q++) {
+                    const double *g = GEO + ((size_t)q * NC + cen) * 5;
+                    double wfv[3] = {0.0, 0.0, 0.0};
+                    if (g[3] <= rmax) splfit_jx<double>(a.tab, ty, rd, g[3], (int)g[4], wfv, 0);
+                    RAD[(size_t)q * NRI + ri] = wfv[0];
+                }
             }
             __syncthreads();
             const double *Wm = a.wmat + ((size_t)w * NE + i) * NB;
