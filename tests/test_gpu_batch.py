@@ -56,3 +56,32 @@ def test_metrop1_batch_rejects_repeated_context(require_gpu):
     e = _engines()[0]
     with pytest.raises(RuntimeError, match="twice"):
         metrop1_batch([e, e], 1, tau=0.3)
+
+
+def test_metrop1_batch_heterogeneous_molecules(require_gpu):
+    """Three molecules of the heterogeneous batch (20, 96 and 300 electrons;
+    the last with its Slater inverse in a global work matrix), FP64 and mixed:
+    batched sweeps equal each molecule's own sweeps bit for bit."""
+    from champ_amd import system as S
+    cases = [("het20", 8, PREC_MIXED), ("het96", 4, PREC_FP64), ("het300", 2, PREC_FP64)]
+
+    def make():
+        out = []
+        for k, (name, nw, prec) in enumerate(cases):
+            sy = S.config(name)
+            e = Engine(sy, nw, precision=prec, lib_path=ENGINE_LIB)
+            e.set_walkers(sy.initial_walkers(nw, seed=21 + k))
+            e.seed_rng(S.seed_from_string())
+            e.hpsi()
+            out.append(e)
+        return out
+
+    batch, single = make(), make()
+    sb = metrop1_batch(batch, 1, tau=0.3)
+    ss = [e.metrop1(1, tau=0.3) for e in single]
+    for m, (name, nw, prec) in enumerate(cases):
+        print(f"[batch {name} W={nw} prec={prec}] accepted {sb[m]['accepted']} {ss[m]['accepted']}", flush=True)
+        assert sb[m]["accepted"] == ss[m]["accepted"] and sb[m]["nflagged"] == ss[m]["nflagged"]
+        assert abs(sb[m]["esum"] - ss[m]["esum"]) <= 1e-12 * max(1.0, abs(ss[m]["esum"])), name
+        assert np.array_equal(batch[m].get_walkers(), single[m].get_walkers()), name
+        assert np.array_equal(batch[m].scalars()["eold"], single[m].scalars()["eold"]), name
