@@ -359,6 +359,12 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
     }
     // excitation tables and state of a multideterminant trial function
     TRY(md_init(c, s));
+    // walker streams from CHAMP's default seed "18123437465549275475255231234865"
+    // read as 8 x i4 (parser.f90:315,654), until seed_rng / set_rng replace them
+    {
+        const int32_t seed0[8] = {1812, 3437, 4655, 4927, 5475, 2552, 3123, 4865};
+        TRY(champ_gpu_seed_rng(c, seed0));
+    }
     return CHAMP_OK;
 }
 

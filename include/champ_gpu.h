@@ -142,7 +142,9 @@ int champ_gpu_get_walkers(champ_ctx *ctx, double *x);
 
 /* Per-walker xoshiro256** state, [nwalk][4] uint64.  seed_rng reproduces
  * parser.f90:651-661: the 32-character seed read as 8 x i4 (setrn), then
- * walker w takes the stream jump()ed w times (one walker per MPI rank).     */
+ * walker w takes the stream jump()ed w times (one walker per MPI rank).
+ * init seeds the streams with CHAMP's default seed string
+ * "18123437465549275475255231234865" (parser.f90:315).                      */
 int champ_gpu_set_rng(champ_ctx *ctx, const uint64_t *state);
 int champ_gpu_get_rng(champ_ctx *ctx, uint64_t *state);
 int champ_gpu_seed_rng(champ_ctx *ctx, const int32_t seed[8]);

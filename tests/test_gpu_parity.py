@@ -240,3 +240,14 @@ def test_c160_parity(require_gpu, nwalk):
         assert rel(pd[w], opd) < TOL
         assert abs(pj[w] - opj) <= TOL * abs(opj)
         assert relnorm(vn[w], pr.orc[w].grad(iel, 0)) < TOL
+
+
+def test_init_seeds_default_streams(require_gpu):
+    """A new population's walker streams are those of CHAMP's default seed
+    string, as seed_rng(default) gives them (no all-zero xoshiro state)."""
+    from champ_amd import system as S
+    e = Engine(S.config("small"), 3, lib_path=ENGINE_LIB)
+    r0 = e.get_rng()
+    assert np.all(r0.reshape(3, 4).any(axis=1))
+    e.seed_rng(S.seed_from_string())
+    assert np.array_equal(r0, e.get_rng())
