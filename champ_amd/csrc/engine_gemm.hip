@@ -207,14 +207,15 @@ int launch_gemm_f32(champ_ctx *c, int M, int N, int K, const float *A, int lda, 
                     int ldc)
 {
     if (M <= 0 || N <= 0) return CHAMP_OK;
-    const long long waves = (long long)((M + 15) / 16) * ((N + 15) / 16);
+    constexpr int TWN = CHAMP_GEMM_TILE_WN;  // tiles along N per wavefront, as the FP64 tile form
+    const long long waves = (long long)((M + 15) / 16) * ((N + 16 * TWN - 1) / (16 * TWN));
     const long long blocks = (waves + 3) / 4;
     if (blocks > 0x7fffffffLL) {
         set_error("gemm: problem too large for one launch");
         return CHAMP_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL((k_gemm_mfma_f32<true, 1>), dim3((unsigned)blocks), dim3(256), 0, c->stream, M, N, K, A, lda, B,
-                       ldb, Cm, ldc);
+    hipLaunchKernelGGL((k_gemm_mfma_f32<true, TWN>), dim3((unsigned)blocks), dim3(256), 0, c->stream, M, N, K, A, lda,
+                       B, ldb, Cm, ldc);
     HIPCHECK(hipGetLastError());
     return CHAMP_OK;
 }
