@@ -16,7 +16,6 @@ namespace champ {
 static thread_local std::string g_err;
 void set_error(const std::string &s) { g_err = s; }
 int launch_hpsi_impl(champ_ctx *c, int accumulate);
-void set_tau(double tau);
 
 void tbegin(champ_ctx *c, int cls, int *slot)
 {
@@ -143,6 +142,7 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
             if (ir < 0 || ir >= s->nrbas[it] || l < 1 || l > CHAMP_MAX_SLM) { set_error("init: AO table entry out of range"); return CHAMP_ERR_ARG; }
         }
     if (s->nr < 2 || !s->cdet) { set_error("init: radial grid or cdet missing"); return CHAMP_ERR_ARG; }
+    if (s->nup < 0 || s->nup > s->nelec) { set_error("init: nup outside 0..nelec"); return CHAMP_ERR_ARG; }
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) { set_error("init: no HIP device"); return CHAMP_ERR_HIP; }
     champ_ctx *c = new champ_ctx();
@@ -353,8 +353,10 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
     }
     // the Slater inverse works in LDS up to ~140 electrons of one spin, above
     // that in a global work matrix per (walker, spin) of an hpsi chunk
+    // (the kernel's own pivot arrays take SLATER_STATIC_LDS bytes beside the
+    // dynamic work matrix; both count against one block's LDS)
     {
-        const size_t lds = (size_t)(NM * (NM + 1) + NM) * sizeof(double);
+        const size_t lds = (size_t)(NM * (NM + 1) + NM) * sizeof(double) + SLATER_STATIC_LDS;
         if (lds > CHAMP_SLATER_LDS_MAX) TRY(dalloc(&c->lu_ws, (size_t)2 * c->hpsi_chunk * NM * (NM + 1)));
     }
     // excitation tables and state of a multideterminant trial function
@@ -418,6 +420,12 @@ extern "C" int champ_gpu_get_walkers(champ_ctx *c, double *x)
 extern "C" int champ_gpu_set_rng(champ_ctx *c, const uint64_t *st)
 {
     if (!c || !st) return CHAMP_ERR_ARG;
+    // an all-zero xoshiro256** state stays zero forever (u = 0 on every draw)
+    for (int w = 0; w < c->W; w++)
+        if ((st[4 * (size_t)w] | st[4 * (size_t)w + 1] | st[4 * (size_t)w + 2] | st[4 * (size_t)w + 3]) == 0) {
+            set_error("set_rng: walker " + std::to_string(w) + " has an all-zero xoshiro256** state");
+            return CHAMP_ERR_ARG;
+        }
     HIPCHECK(hipMemcpy(c->rng, st, sizeof(uint64_t) * 4 * c->W, hipMemcpyHostToDevice));
     return CHAMP_OK;
 }
@@ -477,6 +485,16 @@ extern "C" int champ_gpu_hpsi(champ_ctx *c, double *psid, double *psij, double *
     if ((rc = copy_out(c, psij, c->psijo, c->W))) return rc;
     if ((rc = copy_out(c, ekin, c->ekin, c->W))) return rc;
     if ((rc = copy_out(c, energy, c->eold, c->W))) return rc;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_determinant(champ_ctx *c, double *psid)
+{
+    if (!c) return CHAMP_ERR_ARG;
+    int rc = launch_slater_refresh(c);
+    if (rc) return rc;
+    if ((rc = copy_out(c, psid, c->psido, c->W))) return rc;
     HIPCHECK(hipStreamSynchronize(c->stream));
     return CHAMP_OK;
 }
@@ -702,6 +720,36 @@ extern "C" int champ_gpu_autotune(champ_ctx *c, int32_t *choices)
     return rc;
 }
 
+extern "C" int champ_gpu_set_tuning(champ_ctx *c, const int32_t *choices)
+{
+    if (!c || !choices) return CHAMP_ERR_ARG;
+    for (int s = 0; s < 3; s++) {
+        const int k = choices[s];
+        const bool f32_ok = s == 0 && c->prec == CHAMP_PREC_MIXED;
+        if (k < GEMM_AUTO || k > GEMM_F32 || (k == GEMM_F32 && !f32_ok)) {
+            set_error("set_tuning: GEMM kind " + std::to_string(k) + " not available at site " + std::to_string(s));
+            return CHAMP_ERR_ARG;
+        }
+    }
+    if (choices[3] < -1 || choices[3] > 1) { set_error("set_tuning: quadrature form must be -1, 0 or 1"); return CHAMP_ERR_ARG; }
+    if (c->nloc == 0 && (choices[2] != GEMM_AUTO || choices[3] != -1)) {
+        set_error("set_tuning: no ECP sites on an all-electron system");
+        return CHAMP_ERR_ARG;
+    }
+    for (int s = 0; s < 3; s++) c->tune_gemm[s] = choices[s];
+    c->tune_nonloc = choices[3];
+    if (c->sweep_exec) { HIPCHECK(hipGraphExecDestroy(c->sweep_exec)); c->sweep_exec = nullptr; }
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_set_lu_every(champ_ctx *c, int32_t nmoves)
+{
+    if (!c || nmoves < 0) return CHAMP_ERR_ARG;
+    c->lu_every = nmoves;
+    if (c->sweep_exec) { HIPCHECK(hipGraphExecDestroy(c->sweep_exec)); c->sweep_exec = nullptr; }
+    return CHAMP_OK;
+}
+
 extern "C" int champ_gpu_set_graphs(champ_ctx *c, int32_t on)
 {
     if (!c) return CHAMP_ERR_ARG;
@@ -712,7 +760,7 @@ extern "C" int champ_gpu_set_graphs(champ_ctx *c, int32_t on)
 extern "C" int champ_gpu_metrop1(champ_ctx *c, int32_t nsweep, double tau, champ_sweep_stats *stats)
 {
     if (!c || nsweep < 0 || !(tau > 0)) return CHAMP_ERR_ARG;
-    set_tau(tau);
+    c->tau = tau;
     HIPCHECK(hipMemsetAsync(c->esum_dev, 0, sizeof(double) * 4, c->stream));
     int rc;
     if (c->graphs && !c->timing && nsweep > 0) {
@@ -738,7 +786,7 @@ extern "C" int champ_gpu_metrop1_batch(champ_ctx *const *ctxs, int32_t n, int32_
             if (ctxs[k] == ctxs[m]) { set_error("metrop1_batch: a context appears twice"); return CHAMP_ERR_ARG; }
         nemax = std::max(nemax, ctxs[m]->NE);
     }
-    set_tau(tau);
+    for (int m = 0; m < n; m++) ctxs[m]->tau = tau;
     for (int m = 0; m < n; m++) HIPCHECK(hipMemsetAsync(ctxs[m]->esum_dev, 0, sizeof(double) * 4, ctxs[m]->stream));
     // Each population's kernels go to its own stream in the order of its
     // own metrop1 sweep; the host deals the move steps round-robin over the

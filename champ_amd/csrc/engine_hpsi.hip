@@ -309,6 +309,7 @@ __global__ void __launch_bounds__(256) k_hpsi_ao(HArgs a)
 template <bool GLB>
 __global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
 {
+    // static arrays: SLATER_STATIC_LDS bytes (hpsi_args.hpp)
     HIP_DYNAMIC_SHARED(double, lds)
     __shared__ int perm[512];
     __shared__ double piv[512];
@@ -624,8 +625,10 @@ static int launch_orbitals_slater(champ_ctx *c, HArgs a)
             hipLaunchKernelGGL(k_hpsi_slater<true>, dim3(2 * a.nw), dim3(256), (size_t)nmx * sizeof(double),
                                c->stream, a);
         } else {
+            // dynamic work matrix; with SLATER_STATIC_LDS it fits one block's LDS
+            // (otherwise init allocated lu_ws and the branch above is taken)
             const size_t lds = (size_t)(nmx * (nmx + 1) + nmx) * sizeof(double);
-            if (lds > 64 * 1024)
+            if (lds + SLATER_STATIC_LDS > 64 * 1024)
                 HIPCHECK(hipFuncSetAttribute((const void *)k_hpsi_slater<false>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             hipLaunchKernelGGL(k_hpsi_slater<false>, dim3(2 * a.nw), dim3(256), lds, c->stream, a);

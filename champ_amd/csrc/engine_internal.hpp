@@ -37,6 +37,7 @@ struct champ_ctx {
     int W = 0, NE = 0, NUP = 0, NDN = 0, NM = 0, NC = 0, NCT = 0, NB = 0, NO = 0, NQ = 0;
     int nloc = 0, prec = 0, hpsi_chunk = 0, pair_cap = 0;
     double pecent = 0, cdet0 = 1;
+    double tau = 0.5;  // time step of the metropolis kernels, set by each metrop1 / metrop1_batch call
     hipStream_t stream = nullptr;
 
     // ---- tables

@@ -63,9 +63,6 @@ static MoveArgs move_args(champ_ctx *c, int iel, int mode, double tau)
     return a;
 }
 
-// run-time tau for the metropolis kernels
-static double g_tau = 0.5;
-
 // ------------------------------------------------------------------------
 // Drift at the current position of iel: compute_determinante_grad with
 // iflag_move=1 (determinante.f90:127-157, determinante_ref_grad :285-328).
@@ -655,7 +652,7 @@ int launch_stay(champ_ctx *c, int iel)
 
 int launch_drift_old(champ_ctx *c, int iel)
 {
-    MoveArgs a = move_args(c, iel, 1, g_tau);
+    MoveArgs a = move_args(c, iel, 1, c->tau);
     hipLaunchKernelGGL(k_drift_old, dim3(nblocks(c->W, 4)), dim3(256), 0, c->stream, a);
     HIPCHECK(hipGetLastError());
     return CHAMP_OK;
@@ -689,7 +686,7 @@ static int launch_move_a_t(champ_ctx *c, const MoveArgs &a, int nordc)
 
 int launch_move_a(champ_ctx *c, int iel, int mode)
 {
-    MoveArgs a = move_args(c, iel, mode, g_tau);
+    MoveArgs a = move_args(c, iel, mode, c->tau);
     const int nordc = c->hjas.nordc;
     if (nordc > 5) {
         set_error("move: e-e-n order above 5 is not supported");
@@ -719,7 +716,7 @@ int launch_move_gemm(champ_ctx *c)
 
 int launch_move_b(champ_ctx *c, int iel, int mode)
 {
-    MoveArgs a = move_args(c, iel, mode, g_tau);
+    MoveArgs a = move_args(c, iel, mode, c->tau);
     size_t lds = (size_t)2 * c->NM * sizeof(double);
     int slot;
     tbegin(c, K_MOVE_B, &slot);
@@ -747,6 +744,5 @@ int launch_rotqua(champ_ctx *c)
     return CHAMP_OK;
 }
 
-void set_tau(double tau) { g_tau = tau; }
 
 }  // namespace champ

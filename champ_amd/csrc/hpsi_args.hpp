@@ -83,6 +83,11 @@ __device__ __forceinline__ double yl0(int l, double c)
     return 0.0;
 }
 
+// static LDS of the Slater-inverse kernel k_hpsi_slater (perm[512], piv[512],
+// s_p, s_bad, s_akk), counted with its dynamic work matrix when choosing
+// between the LDS and the global-work-matrix forms
+constexpr size_t SLATER_STATIC_LDS = 512 * sizeof(int) + 512 * sizeof(double) + 2 * sizeof(int) + sizeof(double) + 8;
+
 // ECP quadrature over the active (electron, centre) pairs (engine_nonloc.hip)
 int launch_nonloc(champ_ctx *c, const HArgs &a);
 constexpr int NL_GROUP = 4;  // pairs of one walker per block of the matrix-core quadrature kernel

@@ -30,7 +30,8 @@ EXPORTS = [
     "champ_gpu_hpsi", "champ_gpu_compute_determinante_grad", "champ_gpu_psie", "champ_gpu_accept",
     "champ_gpu_metrop1", "champ_gpu_get_walker_scalars", "champ_gpu_get_status", "champ_gpu_array_size",
     "champ_gpu_get_array", "champ_gpu_basis_fns", "champ_gpu_kernel_times", "champ_gpu_set_graphs",
-    "champ_gpu_autotune", "champ_gpu_metrop1_batch",
+    "champ_gpu_autotune", "champ_gpu_metrop1_batch", "champ_gpu_determinant", "champ_gpu_set_lu_every",
+    "champ_gpu_set_tuning",
 ]
 
 
@@ -76,6 +77,9 @@ def load_library(path: str = LIBPATH):
     L.champ_gpu_set_graphs.argtypes = [vp, i32]
     L.champ_gpu_autotune.argtypes = [vp, C.POINTER(i32)]
     L.champ_gpu_metrop1_batch.argtypes = [C.POINTER(vp), i32, i32, C.c_double, C.POINTER(SweepStats)]
+    L.champ_gpu_determinant.argtypes = [vp, dp]
+    L.champ_gpu_set_lu_every.argtypes = [vp, i32]
+    L.champ_gpu_set_tuning.argtypes = [vp, C.POINTER(i32)]
     _libs[path] = L
     return L
 
@@ -150,6 +154,17 @@ class Engine:
         self._check(self.L.champ_gpu_hpsi(self.ptr, *map(_p, o)), "hpsi")
         return dict(psid=o[0], psij=o[1], ekin=o[2], energy=o[3])
 
+    def determinant(self) -> np.ndarray:
+        """FP64 recompute of orbitals, Slater inverse and determinants at the
+        current positions (determinant.f90:9-140); returns psid per walker."""
+        pd = np.zeros(self.nwalk)
+        self._check(self.L.champ_gpu_determinant(self.ptr, _p(pd)), "determinant")
+        return pd
+
+    def set_lu_every(self, nmoves: int):
+        """Moves between the FP64 recomputes of the inverse inside a metrop1 sweep (0: only in hpsi)."""
+        self._check(self.L.champ_gpu_set_lu_every(self.ptr, int(nmoves)), "set_lu_every")
+
     def compute_determinante_grad(self, iel: int, iflag_move: int) -> np.ndarray:
         v = np.zeros((self.nwalk, 3))
         self._check(self.L.champ_gpu_compute_determinante_grad(self.ptr, int(iel), int(iflag_move), _p(v)),
@@ -204,6 +219,11 @@ class Engine:
         ch = (C.c_int32 * 4)()
         self._check(self.L.champ_gpu_autotune(self.ptr, ch), "autotune")
         return dict(move_gemm=ch[0], orbital_gemm=ch[1], ecp_rows_gemm=ch[2], quadrature=ch[3])
+
+    def set_tuning(self, move_gemm=-1, orbital_gemm=-1, ecp_rows_gemm=-1, quadrature=-1):
+        """Apply kernel choices without timing them (the layout autotune returns; -1: shape rule)."""
+        ch = (C.c_int32 * 4)(move_gemm, orbital_gemm, ecp_rows_gemm, quadrature)
+        self._check(self.L.champ_gpu_set_tuning(self.ptr, ch), "set_tuning")
 
     def set_graphs(self, on: bool = True):
         """metrop1 replays a captured HIP graph per sweep, or launches eagerly (default)."""

@@ -561,6 +561,7 @@ def config(name: str, seed: int = 1234) -> System:
     md256: 64 heavy (Z_eff=4, ECP), 256 e, Nao = 960 (sssppd), 5000
            determinants from 1-4-fold excitations within 40 virtuals
     het20 ... het300: the molecules of the heterogeneous batch (HETERO)
+    ae276 / ae280: all-electron, 138 / 140 electrons per spin (35 heavy + 34 / 35 H)
     """
     if name == "small_md":
         return make_system("small_md", 2, 2, 10, "ssppd", "sp", nloc=4, seed=seed, ndet=40, nvirt=6)
@@ -580,6 +581,13 @@ def config(name: str, seed: int = 1234) -> System:
         return make_system("small_ae", 2, 2, 10, "ssppd", "sp", nloc=0, seed=seed)
     if name == "c60":
         return make_system("c60", 8, 6, 60, "sssspppdd", "ssp", nloc=4, seed=seed)
+    # all-electron molecules with 138 and 140 electrons of one spin: the Slater
+    # work matrix plus the inverse kernel's pivot arrays just fit one block's
+    # LDS at 138 and just exceed it at 140 (then the global work matrix)
+    if name == "ae276":
+        return make_system("ae276", 35, 34, 276, "sssspppdd", "ssp", nloc=0, seed=seed)
+    if name == "ae280":
+        return make_system("ae280", 35, 35, 280, "sssspppdd", "ssp", nloc=0, seed=seed)
     for k, (ne, nheavy, nhyd, _) in enumerate(HETERO):
         if name == f"het{ne}":
             return make_system(name, nheavy, nhyd, ne, "sssspppdd", "ssp", nloc=4, seed=seed + k)

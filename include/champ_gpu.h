@@ -163,6 +163,21 @@ int champ_gpu_rotqua(champ_ctx *ctx);
 int champ_gpu_hpsi(champ_ctx *ctx, double *psid, double *psij, double *ekin,
                    double *energy);
 
+/* determinant (determinant.f90:9-140) on its own: orbitals of all electrons
+ * at the current positions, both Slater matrices, their LU inverse (matinv,
+ * matinv.f90:22-92), the spin determinants, ddx/d2dx2 and the orbital
+ * gradients of every electron (plus the multideterminant state when ndet > 1),
+ * all in FP64; then psid and psi2o refreshed with the unchanged Jastrow.
+ * The mixed path runs this same recompute every lu_every moves inside
+ * metrop1.  psid (may be NULL) [nwalk].                                      */
+int champ_gpu_determinant(champ_ctx *ctx, double *psid);
+
+/* Moves between the FP64 recomputes of the Slater inverse inside a metrop1
+ * sweep (0: only the one in hpsi).  Defaults: 80 on the mixed path, 0 on the
+ * FP64 path.  No counterpart in the reference, which rebuilds the inverse
+ * once per sweep in hpsi.                                                     */
+int champ_gpu_set_lu_every(champ_ctx *ctx, int32_t nmoves);
+
 /* With ndet > 1 the multideterminant parts run inside these same calls:
  * init builds the excitation tables (multideterminants_define,
  * set_input_data.f90:46-414), hpsi adds multideterminant_hpsi
@@ -216,6 +231,12 @@ int champ_gpu_metrop1_batch(champ_ctx *const *ctxs, int32_t n, int32_t nsweep,
  * tuned.  Leaves the walker state as
  * it was (hpsi is recomputed).  No counterpart in the reference.              */
 int champ_gpu_autotune(champ_ctx *ctx, int32_t *choices);
+
+/* Applies kernel choices without timing them, in the layout autotune returns
+ * (choices [4]; -1 restores the shape rule of that site).  GEMM kind 3 (FP32)
+ * is accepted for the moves of the mixed path only.  No counterpart in the
+ * reference.                                                                  */
+int champ_gpu_set_tuning(champ_ctx *ctx, const int32_t *choices);
 
 /* on = 1: metrop1 replays each sweep as a HIP graph captured on first use
  * (per tau); on = 0 (default): every kernel is launched from the host.  No
