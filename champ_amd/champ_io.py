@@ -1,0 +1,443 @@
+"""CHAMP's text input files -> :class:`champ_amd.system.System`.
+
+Reads what a CHAMP VMC run loads for the per-move path and builds the same
+tables CHAMP builds from them at input time:
+
+* the input file itself, a subset of its fdf syntax (``%module`` blocks of
+  ``key value`` lines and ``load <kind> <file>`` lines; src/vmc/parser.f90),
+  with CHAMP's defaults for what it leaves out (nloc 4, nquad 6,
+  parser.f90:577-578; vmc_tau 0.5, parser.f90:405);
+* the molecule (.xyz; read_data_parser.f90:133-298): atom types in order of
+  first appearance, Z (all-electron) or Z - core (pseudopotential);
+* the numerical basis pointers (.bfinfo; read_data_parser.f90:2035-2188) and
+  the radial grids <pool><basis>.basis.<atom> (read_bas_num.f90:11-247):
+  the small-r polynomial, the cut radius, the exponential tail and the spline;
+* Gaussian pseudopotentials <pool><pseudopot>.gauss_ecp.dat.<atom>
+  (readps_gauss, read_bas_num.f90:338-477): local component stored at index
+  lpot, the non-local l = 0, 1, ... at 1, 2, ...;
+* orbitals (.lcao; read_data_parser.f90:925-1134), determinants with optional
+  CSFs and CSF map (.det; :305-446, :1138-1441; the determinant coefficients
+  are rebuilt from the CSF coefficients when a map is present), and the
+  ijas = 4 Jastrow parameters (:532-920).
+
+Orbital files in CHAMP's older AO order (components of a shell block listed
+component by component over the block's radial functions) are converted to
+the order of the pointer file (radial function by radial function) by
+:func:`convert_lcao_order`, the reordering tools/convert_lcao_to_trexio.py
+applies before the reference's CI runs such a case.
+
+Files may be gzipped (``name.gz`` is read when ``name`` is absent).
+"""
+from __future__ import annotations
+
+import gzip
+import os
+import re
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import system as S
+
+# element symbols H..Xe; Z is the index + 1
+_ELEMENTS = ("H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca Sc Ti V Cr Mn Fe Co Ni Cu Zn Ga Ge As Se Br Kr "
+             "Rb Sr Y Zr Nb Mo Tc Ru Rh Pd Ag Cd In Sn Sb Te I Xe").split()
+
+
+def element_z_core(symbol: str) -> Tuple[int, int]:
+    """(Z, core electrons a pseudopotential removes): the core of the preceding
+    noble gas, as in CHAMP's periodic table module."""
+    sym = symbol.strip().capitalize()
+    if sym not in _ELEMENTS:
+        raise KeyError(f"element {symbol!r} not in H..Xe")
+    z = _ELEMENTS.index(sym) + 1
+    core = 0 if z <= 2 else 2 if z <= 10 else 10 if z <= 18 else 18 if z <= 36 else 36
+    return z, core
+
+
+# --------------------------------------------------------------------------
+def _open_text(path: str):
+    if os.path.exists(path):
+        return open(path, "rt")
+    if os.path.exists(path + ".gz"):
+        return gzip.open(path + ".gz", "rt")
+    raise FileNotFoundError(path)
+
+
+class _Records:
+    """Fortran list-directed reading over the lines of a file: each read
+    takes n values starting at the current line, continuing on the next lines
+    when the current one has fewer, and then skips the rest of the last line
+    it used (a read of zero values consumes one line)."""
+
+    _num = re.compile(r"^[+-]?(\d+\.?\d*|\.\d+)([eEdD][+-]?\d+)?$")
+
+    def __init__(self, lines: Sequence[str]):
+        self.lines = list(lines)
+        self.i = 0
+
+    def eof(self) -> bool:
+        return self.i >= len(self.lines)
+
+    def line(self) -> str:
+        s = self.lines[self.i]
+        self.i += 1
+        return s
+
+    def skip_to(self, word: str) -> bool:
+        while self.i < len(self.lines):
+            w = self.lines[self.i].split()
+            if w and w[0] == word:
+                return True
+            self.i += 1
+        return False
+
+    def keyword(self, n: int) -> list:
+        """A 'keyword v1 .. vn' record whose values may continue on the next lines."""
+        toks = self.line().split()[1:]
+        vals = [int(float(t)) for t in toks[:n] if self._num.match(t)]
+        if len(vals) < n:
+            vals += self.read(n - len(vals), int)
+        return vals
+
+    def read(self, n: int, conv=float) -> list:
+        if n == 0:
+            self.line()
+            return []
+        out: list = []
+        while len(out) < n:
+            if self.eof():
+                raise ValueError(f"end of file while reading {n} values")
+            for tok in self.line().split():
+                if len(out) == n:
+                    break
+                if not self._num.match(tok):
+                    break  # text after the values (a comment field)
+                v = float(tok.replace("d", "e").replace("D", "e"))
+                out.append(int(v) if conv is int else v)
+        return out
+
+
+def _records(path: str) -> _Records:
+    with _open_text(path) as fh:
+        return _Records(fh.read().splitlines())
+
+
+# --------------------------------------------------------------------------
+def read_input(path: str) -> Dict:
+    """The subset of a CHAMP input file this path needs: every ``key value``
+    of every %module block (flattened, as fdf labels are global) and the
+    ``load`` lines."""
+    keys: Dict[str, str] = {}
+    loads: Dict[str, str] = {}
+    with open(path) as fh:
+        for raw in fh:
+            line = raw.split("#", 1)[0].strip()
+            if not line or line.startswith("%"):
+                continue
+            w = line.split(None, 2)
+            if w[0] == "load" and len(w) >= 3:
+                loads[w[1]] = w[2].strip()
+            elif len(w) >= 2:
+                keys[w[0]] = line.split(None, 1)[1].strip().strip("'\"")
+    return {"keys": keys, "loads": loads, "dir": os.path.dirname(os.path.abspath(path))}
+
+
+def read_molecule(path: str, nloc: int):
+    """.xyz: count, comment, then symbol x y z [valence charge] (bohr)."""
+    with _open_text(path) as fh:
+        lines = [l for l in fh.read().splitlines()]
+    n = int(lines[0].split()[0])
+    sym, xyz, val = [], [], []
+    for l in lines[2:2 + n]:
+        w = l.split()
+        sym.append(w[0])
+        xyz.append([float(v) for v in w[1:4]])
+        val.append(float(w[4]) if len(w) > 4 else None)
+    types: List[str] = []
+    for s in sym:
+        if s not in types:
+            types.append(s)
+    iwctype = np.array([types.index(s) for s in sym], dtype=np.int32)
+    znuc = np.zeros(len(types))
+    for it, t in enumerate(types):
+        given = [val[i] for i in range(n) if sym[i] == t and val[i] is not None]
+        if given:
+            znuc[it] = given[-1]
+        else:
+            z, core = element_z_core(t)
+            znuc[it] = z if nloc == 0 else z - core
+    return types, np.array(xyz, dtype=np.float64), iwctype, znuc
+
+
+def read_bfinfo(path: str, nctype: int):
+    """Per atom type: (AO count, slm index of each AO, radial function of each AO), 1-based as in the file."""
+    r = _records(path)
+    if not r.skip_to("qmc_bf_info") and not r.skip_to("basis"):
+        raise ValueError(f"{path}: no qmc_bf_info line")
+    r.line()
+    out = []
+    for _ in range(nctype):
+        counts = r.read(6, int)
+        nb = counts[0]
+        out.append((nb, r.read(nb, int), r.read(nb, int)))
+    return out
+
+
+def read_basis_grid(path: str, nloc: int):
+    """A numerical radial basis file: header nrbas igrid nr arg r0 icusp, the
+    angular momenta of the radial functions (all-electron files only), then
+    nr rows of r and the nrbas radial values."""
+    r = _records(path)
+    hdr = r.lines[0].split()
+    r.i = 1
+    nrbas, igrid, nr = int(hdr[0]), int(hdr[1]), int(hdr[2])
+    arg, r0, icusp = float(hdr[3]), float(hdr[4]), int(hdr[5])
+    lval = r.read(nrbas, int) if nloc == 0 else [0] * nrbas
+    data = np.array([r.read(nrbas + 1) for _ in range(nr)], dtype=np.float64)
+    return dict(nrbas=nrbas, igrid=igrid, nr=nr, arg=arg, r0=r0, icusp=icusp, l=lval, x=data[:, 0],
+                rwf=data[:, 1:].T.copy())
+
+
+def read_gauss_ecp(path: str):
+    """A Gaussian pseudopotential file: label, lpot, then lpot components
+    (local first) of nterm lines 'coefficient power exponent'.  Returns lpot
+    and, per 0-based storage index (local at lpot - 1, non-local l at l),
+    the (coef, power, exponent) terms."""
+    r = _records(path)
+    r.line()  # label
+    lpot = r.read(1, int)[0]
+    comps: Dict[int, list] = {}
+    for l in range(1, lpot + 1):
+        idx = lpot - 1 if l == 1 else l - 2
+        nt = r.read(1, int)[0]
+        terms = []
+        for _ in range(nt):
+            c, p, e = r.read(3)
+            terms.append((c, int(p), e))
+        comps[idx] = terms
+    return lpot, comps
+
+
+def read_lcao(path: str) -> np.ndarray:
+    """coef[iorb][ibasis] of every orbital in the file."""
+    r = _records(path)
+    if not r.skip_to("lcao"):
+        raise ValueError(f"{path}: no lcao line")
+    w = r.line().split()
+    norb, nbasis = int(w[1]), int(w[2])
+    return np.array([r.read(nbasis) for _ in range(norb)], dtype=np.float64)
+
+
+def read_determinants(path: str, nelec: int):
+    """cdet[ndet] and iworbd[ndet][nelec] (0-based orbitals); with a csf and
+    csfmap section cdet is rebuilt as sum_csf ccsf * (map coefficient)."""
+    r = _records(path)
+    if not r.skip_to("determinants"):
+        raise ValueError(f"{path}: no determinants line")
+    ndet = r.keyword(1)[0]
+    cdet = np.array(r.read(ndet))
+    iworbd = np.array([r.read(nelec, int) for _ in range(ndet)], dtype=np.int32) - 1
+    if r.skip_to("csf"):
+        ncsf = r.keyword(1)[0]
+        ccsf = np.array(r.read(ncsf))
+        if r.skip_to("csfmap"):
+            w = r.keyword(3)
+            if w[0] != ncsf or w[1] != ndet:
+                raise ValueError(f"{path}: csfmap sizes {w[:2]} against ncsf {ncsf}, ndet {ndet}")
+            cdet = np.zeros(ndet)
+            for icsf in range(ncsf):
+                nterm = r.read(1, int)[0]
+                for _ in range(nterm):
+                    idt, cx = r.read(2)
+                    cdet[int(idt) - 1] += ccsf[icsf] * cx
+    return cdet, iworbd
+
+
+def read_jastrow(path: str, nctype: int, nspin2b: int = 1):
+    """ijas = 4 parameters: (norda, nordb, nordc, scalek, a4[nctype][norda+1],
+    b[nspin2b][nordb+1], c[nctype][nterms4(nordc)])."""
+    r = _records(path)
+    if not r.skip_to("jastrow_parameter"):
+        raise ValueError(f"{path}: no jastrow_parameter line")
+    r.line()
+    norda, nordb, nordc = r.read(3, int)
+    scalek = r.read(1)[0]
+    na, nb, nc = 2 + max(0, norda - 1), 2 + max(0, nordb - 1), S.nterms4(nordc)
+    a4 = np.array([r.read(na) for _ in range(nctype)])
+    b = np.array([r.read(nb) for _ in range(nspin2b)])
+    c = np.array([r.read(nc) for _ in range(nctype)]).reshape(nctype, nc)
+    return norda, nordb, nordc, scalek, a4, b, c
+
+
+# --------------------------------------------------------------------------
+_SLM_LK = {}
+for _s in range(1, 36):
+    _SLM_LK[_s] = (0, 0) if _s == 1 else (1, _s - 2) if _s <= 4 else (2, _s - 5) if _s <= 10 else \
+        (3, _s - 11) if _s <= 20 else (4, _s - 21)
+_COMP_L = [0] + [1] * 3 + [2] * 6 + [3] * 10 + [4] * 15  # component index (older format) -> l
+_COMP_K = [0, 0, 1, 2] + list(range(6)) + list(range(10)) + list(range(15))
+
+
+def convert_lcao_order(coef_old: np.ndarray, bfinfo_old: str, bfinfo_new: str, symbols: Sequence[str],
+                       types: Sequence[str]) -> np.ndarray:
+    """Orbital coefficients in CHAMP's older AO order (per atom: for each of
+    the 35 Cartesian components s, x, y, z, xx ... the radial functions of that
+    component's shells) put into the order of the newer pointer file (per
+    atom: radial function by radial function, its components in slm order).
+    An AO is identified by (l, component within l, radial function) on both
+    sides; the coefficients are moved, not scaled."""
+    r = _records(bfinfo_old)
+    if not r.skip_to("qmc_bf_info"):
+        raise ValueError(f"{bfinfo_old}: no qmc_bf_info line")
+    r.line()
+    old_types = []
+    for _ in range(len(types)):
+        counts = r.read(35, int)
+        rad = r.read(sum(counts), int)
+        keys, q = [], 0
+        for ci, n in enumerate(counts):
+            for _ in range(n):
+                keys.append((_COMP_L[ci], _COMP_K[ci], rad[q]))
+                q += 1
+        old_types.append(keys)
+    new_info = read_bfinfo(bfinfo_new, len(types))
+    perm_per_type = []
+    for it in range(len(types)):
+        nb, slm, rad = new_info[it]
+        pos = {k: q for q, k in enumerate(old_types[it])}
+        if len(pos) != len(old_types[it]) or len(old_types[it]) != nb:
+            raise ValueError(f"type {types[it]}: the two basis descriptions do not list the same AOs")
+        perm_per_type.append([pos[_SLM_LK[slm[p]] + (rad[p],)] for p in range(nb)])
+    cols = []
+    off = 0
+    for s in symbols:
+        perm = perm_per_type[types.index(s)]
+        cols += [off + q for q in perm]
+        off += len(perm)
+    if off != coef_old.shape[1]:
+        raise ValueError(f"orbital file has {coef_old.shape[1]} AOs, basis descriptions {off}")
+    return np.ascontiguousarray(coef_old[:, cols])
+
+
+# --------------------------------------------------------------------------
+def _tables_from_grids(grids: List[Dict], nloc: int):
+    nct = len(grids)
+    nrmax = max(g["nrbas"] for g in grids)
+    nr = grids[0]["nr"]
+    for g in grids:
+        if g["igrid"] != 3:
+            raise NotImplementedError("radial grid type other than 3 (shifted exponential)")
+        if (g["nr"], g["arg"], g["r0"]) != (nr, grids[0]["arg"], grids[0]["r0"]):
+            raise NotImplementedError("atom types with different radial grids")
+        if g["icusp"] == 1 and nloc == 0 and any(l == 0 for l in g["l"]):
+            raise NotImplementedError("cusp-constrained small-r fit (icusp = 1)")
+    rwf = np.zeros((nct, nrmax, nr)); d2 = np.zeros_like(rwf)
+    ce = np.zeros((nct, nrmax, S.NCOEF)); ae = np.zeros((nct, nrmax, 2)); rmax = np.zeros((nct, nrmax))
+    for it, g in enumerate(grids):
+        for irb in range(g["nrbas"]):
+            rwf[it, irb], d2[it, irb], ce[it, irb], ae[it, irb], rmax[it, irb] = S.fit_radial_tables(g["x"], g["rwf"][irb])
+    # r0 of grid type 3 divided by (arg^(nr-1) - 1), read_bas_num.f90:146
+    g0 = grids[0]
+    r0 = g0["r0"] / (g0["arg"] ** (g0["nr"] - 1) - 1.0)
+    return nr, r0, g0["arg"], np.array([g["nrbas"] for g in grids], dtype=np.int32), rwf, d2, ce, ae, rmax
+
+
+def load_input(inp: str, orbitals: Optional[np.ndarray] = None, nquad: Optional[int] = None,
+               name: Optional[str] = None) -> S.System:
+    """A CHAMP VMC input file -> System.  ``orbitals`` replaces the file the
+    input names (e.g. after :func:`convert_lcao_order`)."""
+    cfg = read_input(inp)
+    keys, loads, base = cfg["keys"], cfg["loads"], cfg["dir"]
+    pool = os.path.join(base, keys.get("pool", "./"))
+
+    def resolve(p: str) -> str:
+        p = p.strip().strip("'\"")
+        if p[:6].lower() == "$pool/":
+            return os.path.join(pool, p[6:])
+        return os.path.join(base, p)
+
+    nloc = int(keys.get("nloc", 4))
+    nelec, nup = int(keys["nelec"]), int(keys["nup"])
+    types, cent, iwctype, znuc = read_molecule(resolve(loads["molecule"]), nloc)
+    info = read_bfinfo(resolve(loads["basis_num_info"]), len(types))
+    bas_id = keys["basis"]
+    grids = [read_basis_grid(os.path.join(pool, f"{bas_id}.basis.{t}"), nloc) for t in types]
+    nr, r0, arg, nrbas, rwf, d2, ce, ae, rmax = _tables_from_grids(grids, nloc)
+    nbastyp = np.array([i[0] for i in info], dtype=np.int32)
+    nbmax = int(nbastyp.max())
+    iwrwf = np.zeros((len(types), nbmax), dtype=np.int32)
+    iwlbas = np.ones((len(types), nbmax), dtype=np.int32)
+    for it, (nb, slm, rad) in enumerate(info):
+        if max(slm) > S.SLM_OF_L[3][-1]:
+            raise NotImplementedError("g functions (slm index above 20)")
+        iwlbas[it, :nb] = slm
+        iwrwf[it, :nb] = np.array(rad) - 1
+
+    cdet, iworbd = read_determinants(resolve(loads["determinants"]), nelec)
+    coef = orbitals if orbitals is not None else read_lcao(resolve(loads["orbitals"]))
+    norb = int(iworbd.max()) + 1
+    if coef.shape[0] < norb:
+        raise ValueError(f"determinants use {norb} orbitals, the orbital file has {coef.shape[0]}")
+    coef = np.ascontiguousarray(coef[:norb])
+    nbasis = int(sum(nbastyp[t] for t in iwctype))
+    if coef.shape[1] != nbasis:
+        raise ValueError(f"orbital file has {coef.shape[1]} AOs, basis pointers {nbasis}")
+    nspin2 = int(keys.get("nspin2", 1))
+    nspin2b = abs(nspin2)
+    nocuspb = 1 if nspin2 == -1 else 0
+    norda, nordb, nordc, scalek, a4, b, c = read_jastrow(resolve(loads["jastrow"]), len(types), nspin2b)
+    if keys.get("ijas", "4") != "4" or keys.get("isc", "2") != "2":
+        raise NotImplementedError("Jastrow other than ijas = 4, isc = 2")
+
+    nct = len(types)
+    lpot = np.ones(nct, dtype=np.int32)
+    nterm = np.zeros((nct, S.MPS_L), dtype=np.int32)
+    npow = np.zeros((nct, S.MPS_L, S.MGAUSS), dtype=np.int32)
+    ecoef = np.zeros((nct, S.MPS_L, S.MGAUSS)); eexp = np.zeros_like(ecoef)
+    nq = 0
+    if nloc:
+        if nloc != 4:
+            raise NotImplementedError(f"nloc = {nloc} (only Gaussian pseudopotentials, nloc = 4)")
+        pp = keys["pseudopot"]
+        for it, t in enumerate(types):
+            lp, comps = read_gauss_ecp(os.path.join(pool, f"{pp}.gauss_ecp.dat.{t}"))
+            if lp > S.MPS_L:
+                raise ValueError(f"{t}: {lp} pseudopotential components (at most {S.MPS_L})")
+            lpot[it] = lp
+            for idx, terms in comps.items():
+                if len(terms) > S.MGAUSS:
+                    raise ValueError(f"{t}: {len(terms)} Gaussian terms in one component (at most {S.MGAUSS})")
+                nterm[it, idx] = len(terms)
+                for g, (cf, pw, ex) in enumerate(terms):
+                    ecoef[it, idx, g], npow[it, idx, g], eexp[it, idx, g] = cf, pw, ex
+        nq = nquad if nquad is not None else int(keys.get("nquad", 6))
+    xq0, yq0, zq0, wq = gesqua(nq) if nq else (np.zeros(1), np.zeros(1), np.zeros(1), np.zeros(1))
+    return S.System(name=name or os.path.splitext(os.path.basename(inp))[0], nelec=nelec, nup=nup, cent=cent,
+                    iwctype=iwctype, znuc=znuc, grid_r0=r0, grid_arg=arg, nr=nr, nrbas=nrbas, rwf=rwf, d2rwf=d2,
+                    ce=ce, ae=ae, rmaxwf=rmax, nbastyp=nbastyp, iwrwf=iwrwf, iwlbas=iwlbas, coef=coef,
+                    iworbd=iworbd, cdet=cdet, norda=norda, nordb=nordb, nordc=nordc, scalek=scalek, a4=a4,
+                    b=b, c=c, nloc=nloc, lpot=lpot, necp_term=nterm, necp_power=npow, ecp_coef=ecoef,
+                    ecp_exponent=eexp, xq0=xq0, yq0=yq0, zq0=zq0, wq=wq, nspin2b=nspin2b, nocuspb=nocuspb)
+
+
+def gesqua(nq: int):
+    """The quadrature rules gesqua provides for this path: the octahedral
+    6-point and the icosahedral 12-point rules (rotqua.f90:146-253)."""
+    if nq == 12:
+        return S.icosa12()
+    if nq == 6:
+        xq, yq, zq = np.zeros(6), np.zeros(6), np.zeros(6)
+        xq[0], xq[1], yq[2], yq[3], zq[4], zq[5] = 1.0, -1.0, 1.0, -1.0, 1.0, -1.0
+        return xq, yq, zq, np.full(6, 1.0 / 6.0)
+    raise NotImplementedError(f"{nq}-point quadrature (6 and 12 are provided)")
+
+
+def vmc_settings(inp: str) -> Dict[str, float]:
+    """Blocking and move settings of the input with CHAMP's defaults
+    (parser.f90:398-410)."""
+    k = read_input(inp)["keys"]
+    return dict(vmc_nstep=int(k.get("vmc_nstep", 1)), vmc_nblk=int(k.get("vmc_nblk", 1)),
+                vmc_nblkeq=int(k.get("vmc_nblkeq", 2)), vmc_tau=float(k.get("vmc_tau", 0.5)),
+                imetro=int(k.get("imetro", 6)), deltar=float(k.get("deltar", 5.0)),
+                deltat=float(k.get("deltat", 1.0)), fbias=float(k.get("fbias", 1.0)))

@@ -775,6 +775,43 @@ extern "C" int champ_gpu_metrop1(champ_ctx *c, int32_t nsweep, double tau, champ
     return sweep_stats(c, nsweep, stats);
 }
 
+extern "C" int champ_gpu_vmc(champ_ctx *c, int32_t nblk, int32_t nstep, int32_t nblkeq, double tau, int32_t imetro,
+                             double *enow, champ_vmc_result *res)
+{
+    if (!c || nblk < 1 || nstep < 1 || nblkeq < 0 || !(tau > 0) || (imetro != 1 && imetro != 6)) {
+        set_error("vmc: nblk, nstep >= 1, nblkeq >= 0, tau > 0 and imetro 1 or 6 required");
+        return CHAMP_ERR_ARG;
+    }
+    if (imetro == 6) { set_error("vmc: metrop6 moves are not built"); return CHAMP_ERR_UNSUPPORTED; }
+    int rc;
+    champ_sweep_stats st;
+    // equilibration blocks, not accumulated (vmc.f90: vmc_nblkeq before zerest)
+    for (int b = 0; b < nblkeq; b++)
+        if ((rc = champ_gpu_metrop1(c, nstep, tau, nullptr))) return rc;
+    // acuest: ecum += esum, ecm2 += esum * enow, wcum += wsum per block
+    double wcum = 0, ecum = 0, ecm2 = 0, e2cum = 0, acc = 0, prop = 0;
+    const double wsum = (double)c->W * nstep;
+    for (int b = 0; b < nblk; b++) {
+        if ((rc = champ_gpu_metrop1(c, nstep, tau, &st))) return rc;
+        const double en = st.esum / wsum;
+        if (enow) enow[b] = en;
+        wcum += wsum; ecum += st.esum; ecm2 += st.esum * en; e2cum += st.esum2;
+        acc += st.accepted; prop += st.proposed;
+    }
+    if (res) {
+        const double efin = ecum / wcum;
+        res->efin = efin;
+        res->eerr = std::sqrt(std::fabs(ecm2 / wcum - efin * efin) / nblk);
+        res->sigma = std::sqrt(std::fabs(e2cum / wcum - efin * efin));
+        const double err1 = res->sigma / std::sqrt(wcum);
+        res->tcorr = err1 > 0 ? (res->eerr / err1) * (res->eerr / err1) : 0.0;
+        res->accept = prop > 0 ? acc / prop : 0.0;
+        res->nsamples = (int64_t)wcum;
+        res->nflagged = st.nflagged;
+    }
+    return CHAMP_OK;
+}
+
 extern "C" int champ_gpu_metrop1_batch(champ_ctx *const *ctxs, int32_t n, int32_t nsweep, double tau,
                                        champ_sweep_stats *stats)
 {

@@ -31,8 +31,13 @@ EXPORTS = [
     "champ_gpu_metrop1", "champ_gpu_get_walker_scalars", "champ_gpu_get_status", "champ_gpu_array_size",
     "champ_gpu_get_array", "champ_gpu_basis_fns", "champ_gpu_kernel_times", "champ_gpu_set_graphs",
     "champ_gpu_autotune", "champ_gpu_metrop1_batch", "champ_gpu_determinant", "champ_gpu_set_lu_every",
-    "champ_gpu_set_tuning",
+    "champ_gpu_set_tuning", "champ_gpu_vmc",
 ]
+
+
+class VmcResult(C.Structure):
+    _fields_ = [("efin", C.c_double), ("eerr", C.c_double), ("sigma", C.c_double), ("tcorr", C.c_double),
+                ("accept", C.c_double), ("nsamples", C.c_int64), ("nflagged", C.c_int64)]
 
 
 class SweepStats(C.Structure):
@@ -80,6 +85,7 @@ def load_library(path: str = LIBPATH):
     L.champ_gpu_determinant.argtypes = [vp, dp]
     L.champ_gpu_set_lu_every.argtypes = [vp, i32]
     L.champ_gpu_set_tuning.argtypes = [vp, C.POINTER(i32)]
+    L.champ_gpu_vmc.argtypes = [vp, i32, i32, i32, C.c_double, i32, dp, C.POINTER(VmcResult)]
     _libs[path] = L
     return L
 
@@ -185,6 +191,16 @@ class Engine:
         st = SweepStats()
         self._check(self.L.champ_gpu_metrop1(self.ptr, int(nsweep), float(tau), C.byref(st)), "metrop1")
         return dict(accepted=st.accepted, proposed=st.proposed, esum=st.esum, esum2=st.esum2, nflagged=st.nflagged)
+
+    def vmc(self, nblk: int, nstep: int, nblkeq: int = 0, tau: float = 0.5, imetro: int = 1):
+        """Blocked VMC (vmc.f90 with acuest/finwrt estimators): returns the
+        result fields and the block averages."""
+        enow = np.zeros(int(nblk))
+        r = VmcResult()
+        self._check(self.L.champ_gpu_vmc(self.ptr, int(nblk), int(nstep), int(nblkeq), float(tau), int(imetro),
+                                         _p(enow), C.byref(r)), "vmc")
+        return dict(efin=r.efin, eerr=r.eerr, sigma=r.sigma, tcorr=r.tcorr, accept=r.accept, nsamples=r.nsamples,
+                    nflagged=r.nflagged, enow=enow)
 
     def scalars(self) -> Dict[str, np.ndarray]:
         o = [np.zeros(self.nwalk) for _ in range(4)]

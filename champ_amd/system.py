@@ -118,6 +118,15 @@ def tabulate_shell(x: np.ndarray, alphas: Sequence[float], coefs: Sequence[float
     rwf = np.zeros_like(x)
     for a, cf in zip(alphas, coefs):
         rwf = rwf + gnorm(a, l) * cf * np.exp(-a * r2)
+    return fit_radial_tables(x, rwf, cutoff_rmax)
+
+
+def fit_radial_tables(x: np.ndarray, rwf: np.ndarray, cutoff_rmax: float = 1.0e-12):
+    """The fits CHAMP attaches to a radial function tabulated on the grid
+    (read_data_trexio.f90:1090-1179, read_bas_num.f90:162-247 with icusp=0):
+    the small-r polynomial ce, the cut radius, the exponential tail ae when the
+    function has not decayed at the grid end, and the spline second
+    derivatives.  Returns rwf, d2rwf, ce, ae, rmaxwf."""
     # small-r polynomial through the first NCOEF nodes (dgesv on a Vandermonde)
     V = np.array([[x[i] ** j for j in range(NCOEF)] for i in range(NCOEF)])
     ce = np.linalg.solve(V, rwf[:NCOEF].copy())

@@ -210,6 +210,29 @@ int champ_gpu_accept(champ_ctx *ctx, int32_t iel, const uint8_t *accept);
 int champ_gpu_metrop1(champ_ctx *ctx, int32_t nsweep, double tau,
                       champ_sweep_stats *stats);
 
+/* Result of a blocked VMC run (champ_gpu_vmc).                            */
+typedef struct champ_vmc_result {
+    double efin;            /* total E: sum of E_L over the accumulated blocks / samples   */
+    double eerr;            /* its error from the block averages (finwrt.f90:182-187):
+                               sqrt(|sum_b w_b e_b^2 / W - efin^2| / nblk)                  */
+    double sigma;           /* sqrt(<E_L^2> - <E_L>^2) over every sample                   */
+    double tcorr;           /* (eerr / (sigma / sqrt(samples)))^2: the correlation time in
+                               sweeps, 1 for uncorrelated samples (CHAMP's Tcor column)      */
+    double accept;          /* accepted / proposed moves of the accumulated blocks          */
+    int64_t nsamples;       /* walker-sweeps accumulated: nwalk * nstep * nblk             */
+    int64_t nflagged;       /* walkers with a non-zero status word at the end              */
+} champ_vmc_result;
+
+/* vmc (vmc.f90:3-159) over the population: nblkeq equilibration blocks, then
+ * nblk blocks of nstep sweeps whose energies are accumulated per block as
+ * acuest does (acuest.f90; the block weight is nwalk * nstep, as the sum over
+ * CHAMP's one-walker ranks in acuest_reduce).  imetro = 1: metrop1 sweeps
+ * (drift-diffusion, time step tau); imetro = 6 (metrop6) is refused with
+ * CHAMP_ERR_UNSUPPORTED.  enow (may be NULL) [nblk]: the block averages.
+ * res (may be NULL).                                                        */
+int champ_gpu_vmc(champ_ctx *ctx, int32_t nblk, int32_t nstep, int32_t nblkeq, double tau, int32_t imetro,
+                  double *enow, champ_vmc_result *res);
+
 /* metrop1 over n populations at once, typically different molecules (each
  * context its own system, walker count and precision): every context's
  * kernels run on its own stream in the order of its own champ_gpu_metrop1,

@@ -1,7 +1,8 @@
 """The mixed-precision path (CHAMP_PREC_MIXED) against the FP64 path on
 identical walker configurations: per-walker local energy and log psi
-differences, and a short metrop1 run including the mid-sweep FP64 refresh of
-the Slater inverse.
+differences, and a short metrop1 run with the FP64 refresh of the Slater
+inverse every 3 moves (set_lu_every; the default 80 only acts above 80
+electrons, which tests/test_gpu_configs.py covers at 160 electrons).
 
 The bound is relative and loose: FP32 carries about 7 digits, and the
 synthetic systems' local energies (random wavefunctions) reach hundreds of
@@ -50,6 +51,7 @@ def test_mixed_hpsi_close_to_fp64(require_gpu, name, nwalk):
 def test_mixed_metrop1_runs(require_gpu, name):
     sy, (e64, e32) = _pair(name, 16, seed=13)
     e64.hpsi(); e32.hpsi()
+    e32.set_lu_every(3)
     s64 = e64.metrop1(3, tau=0.3)
     s32 = e32.metrop1(3, tau=0.3)
     print(f"[mixed metrop1 {name}] acc fp64 {s64['accepted'] / s64['proposed']:.3f} mixed {s32['accepted'] / s32['proposed']:.3f}",

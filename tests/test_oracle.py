@@ -469,3 +469,163 @@ def test_pe_local_all_electron():
         for j in range(i):
             pe += sy.znuc[sy.iwctype[i]] * sy.znuc[sy.iwctype[j]] / np.linalg.norm(sy.cent[i] - sy.cent[j])
     assert abs(pe - w.parts()["pe_local"]) < 1e-12 * abs(pe)
+
+
+# ------------------------------------------------ pins added for gauss_pot,
+# the ECP local channel, rotqua and the e-e-n coefficient order
+def _bfd_system():
+    """A system carrying the BFD carbon and hydrogen pseudopotentials read from
+    the reference's CI files (tests/golden/champ_inputs/C4H6_ci44/pool)."""
+    from champ_amd import champ_io as io
+    sy = S.config("small")
+    pool = os.path.join(HERE, "golden", "champ_inputs", "C4H6_ci44", "pool")
+    for it, atom in enumerate(("C", "H")):
+        lp, comps = io.read_gauss_ecp(os.path.join(pool, f"BFD.gauss_ecp.dat.{atom}"))
+        sy.lpot[it] = lp
+        sy.necp_term[it] = 0
+        for idx, terms in comps.items():
+            sy.necp_term[it, idx] = len(terms)
+            for g, (cf, pw, ex) in enumerate(terms):
+                sy.ecp_coef[it, idx, g], sy.necp_power[it, idx, g], sy.ecp_exponent[it, idx, g] = cf, pw, ex
+    sy.znuc[:] = [4.0, 1.0]
+    return sy
+
+
+def _vps(L, cs, sy, r, l, it):
+    v, dv = np.zeros(1), np.zeros(1)
+    L.orc_gauss_pot(C.byref(cs), float(r), l, it, ffi.P(v), ffi.P(dv))
+    return v[0], dv[0]
+
+
+def test_gauss_pot_bfd_closed_forms():
+    """BFD pseudopotentials are finite at the nucleus by construction: the
+    local channel's c r^-1 e^(-a r^2) term (power 1) cancels -Z/r, so
+    V_loc(0) = the coefficient of its power-2 term (C: -19.17537323,
+    H: -8.228005709676), and a non-local channel of one power-2 term equals
+    its coefficient at r = 0 (C s: 22.55164191).  Beyond a r^2 > 700 every
+    Gaussian is dropped and V_loc = -Z/r exactly.  dV/dr against central
+    differences.  A wrong power offset (n - 2), a missing -Z/r or a local
+    component stored at the wrong index fails one of these."""
+    sy = _bfd_system()
+    cs = sy.as_ctypes()
+    L = ffi.lib()
+    for it, z, v0 in ((0, 4.0, -19.17537323), (1, 1.0, -8.228005709676)):
+        lp = int(sy.lpot[it])
+        r = 1e-7
+        v, _ = _vps(L, cs, sy, r, lp, it)
+        assert abs((v - z / r) - v0) < 1e-5, (it, v - z / r)
+        r = 14.0  # smallest exponent 3.94: 3.94 * 196 > 700
+        v, dv = _vps(L, cs, sy, r, lp, it)
+        assert v == 0.0 and dv == 0.0  # gauss_pot alone; getvps adds -Z/r
+        for r in (0.3, 0.9, 2.2):
+            h = 1e-5
+            vp, _ = _vps(L, cs, sy, r + h, lp, it)
+            vm, _ = _vps(L, cs, sy, r - h, lp, it)
+            _, dv = _vps(L, cs, sy, r, lp, it)
+            assert abs((vp - vm) / (2 * h) - dv) < 1e-7 * max(1.0, abs(dv))
+    v, _ = _vps(L, cs, sy, 1e-9, 1, 0)
+    assert abs(v - 22.55164191) < 1e-6
+
+
+def test_pe_local_ecp_closed_form():
+    """pe_local on a pseudopotential system: nuclear repulsion, electron
+    repulsion and, per (electron, centre), the local channel
+    sum_k c_k r^(n_k - 2) exp(-a_k r^2) - Z/r, evaluated here from the BFD
+    files' own numbers (nonloc_pot.f90:44-63, readps_gauss.f90:6-92)."""
+    from champ_amd import champ_io as io
+    sy = _bfd_system()
+    x = sy.initial_walkers(1, seed=2)[0]
+    w = ffi.walker(sy)
+    w.set_x(x)
+    w.set_quad(sy.xq0, sy.yq0, sy.zq0)
+    w.hpsi()
+    pool = os.path.join(HERE, "golden", "champ_inputs", "C4H6_ci44", "pool")
+    local = {}
+    for it, atom in enumerate(("C", "H")):
+        lp, comps = io.read_gauss_ecp(os.path.join(pool, f"BFD.gauss_ecp.dat.{atom}"))
+        local[it] = comps[lp - 1]
+    pe = 0.0
+    for i in range(1, sy.nelec):
+        for j in range(i):
+            pe += 1 / np.linalg.norm(x[i] - x[j])
+    for i in range(1, sy.ncent):
+        for j in range(i):
+            pe += sy.znuc[sy.iwctype[i]] * sy.znuc[sy.iwctype[j]] / np.linalg.norm(sy.cent[i] - sy.cent[j])
+    for i in range(sy.nelec):
+        for ic in range(sy.ncent):
+            it = int(sy.iwctype[ic])
+            r = float(np.linalg.norm(x[i] - sy.cent[ic]))
+            pe += sum(cf * r ** (pw - 2) * math.exp(-ex * r * r) for cf, pw, ex in local[it] if ex * r * r <= 700)
+            pe -= sy.znuc[it] / r
+    assert abs(pe - w.parts()["pe_local"]) < 1e-12 * abs(pe)
+
+
+def test_rotqua_known_answer():
+    """rotqua from the walker's stream: the first accepted pair (x1, x2) of
+    uniforms in the unit disk gives Marsaglia's point X on the sphere, the
+    second accepted pair (u1, u2) an angle in the plane orthogonal to X:
+    Y = u e_theta + v e_phi, Z = v e_theta - u e_phi with e_theta, e_phi the
+    spherical unit vectors at X (rotqua.f90:363-419).  The rule's points q map
+    to q_x X + q_y Y + q_z Z.  (Y, Z, X) is left-handed: det = -1, a
+    reflection that the rule's inversion symmetry leaves harmless.  Checked
+    on the 12-point rule for 20 draws."""
+    sy = S.config("small")
+    cs = sy.as_ctypes()
+    L = ffi.lib()
+    st = ffi.seeded_streams(S.seed_from_string(), 1)[0].copy()
+    for _ in range(20):
+        s_py = st.copy()
+        u = lambda: L.orc_random_dp(ffi.U(s_py))  # noqa: E731
+        while True:
+            a, b = 1 - 2 * u(), 1 - 2 * u()
+            if a * a + b * b < 1:
+                break
+        s2 = a * a + b * b
+        X = np.array([2 * a * math.sqrt(1 - s2), 2 * b * math.sqrt(1 - s2), 1 - 2 * s2])
+        th, ph = math.acos(X[2]), math.atan2(X[1], X[0])
+        e_th = np.array([math.cos(th) * math.cos(ph), math.cos(th) * math.sin(ph), -math.sin(th)])
+        e_ph = np.array([-math.sin(ph), math.cos(ph), 0.0])
+        while True:
+            p, q = 2 * u() - 1, 2 * u() - 1
+            if p * p + q * q < 1:
+                break
+        n = math.hypot(p, q)
+        p, q = p / n, q / n
+        Y = p * e_th + q * e_ph
+        Z = q * e_th - p * e_ph
+        xq, yq, zq = np.zeros(12), np.zeros(12), np.zeros(12)
+        L.orc_rotqua(C.byref(cs), ffi.U(st), ffi.P(xq), ffi.P(yq), ffi.P(zq))
+        expect = np.outer(sy.xq0, X) + np.outer(sy.yq0, Y) + np.outer(sy.zq0, Z)
+        assert np.max(np.abs(np.stack([xq, yq, zq], 1) - expect)) < 1e-12
+        assert np.array_equal(st, s_py)  # the same number of draws
+        assert abs(np.linalg.det(np.stack([X, Y, Z])) + 1) < 1e-12
+
+
+# (k, l, m) of each c parameter for nordc = 4, from CHAMP's published form of
+# f_een (docs/users/preparation/jastrow.md:60-66: p = 2..nordc, k = p-1..0,
+# l = lmax..0, m = (p-k-l)/2 when integer; lmax = p-k, or p-k-2 when k = 0
+# so that no pure e-n term u^0 (x^p + y^p) appears)
+EEN_NORDC4 = [(1, 1, 0), (0, 0, 1),
+              (2, 1, 0), (1, 2, 0), (1, 0, 1), (0, 1, 1),
+              (3, 1, 0), (2, 2, 0), (2, 0, 1), (1, 3, 0), (1, 1, 1), (0, 2, 1), (0, 0, 2)]
+
+
+def test_een_coefficient_order_one_hot():
+    """Two electrons, one centre, a = b = 0 and a one-hot c: the oracle's log
+    Jastrow is exactly c u^k (x^l + y^l) (x y)^m with u, x, y the switched
+    distances exp(-kappa r) of the pair and of each electron to the centre."""
+    base = S.make_system("een2", 1, 0, 2, "sp", "s", nloc=0, nordc=4, seed=5)
+    assert S.een_terms(4) == EEN_NORDC4 and S.nterms4(4) == 13
+    x = np.array([[0.3, -0.2, 0.5], [-0.4, 0.6, 0.1]])
+    kap = base.scalek
+    u = math.exp(-kap * np.linalg.norm(x[0] - x[1]))
+    xa = math.exp(-kap * np.linalg.norm(x[0] - base.cent[0]))
+    ya = math.exp(-kap * np.linalg.norm(x[1] - base.cent[0]))
+    for t, (k, l, m) in enumerate(EEN_NORDC4):
+        sy = S.System(**{**base.__dict__, "a4": np.zeros_like(base.a4), "b": np.zeros_like(base.b),
+                         "c": np.zeros_like(base.c)})
+        sy.c[0, t] = 0.7
+        w = ffi.walker(sy)
+        w.set_x(x)
+        psij = w.hpsi()["psij"]
+        assert abs(psij - 0.7 * u ** k * (xa ** l + ya ** l) * (xa * ya) ** m) < 1e-14, (t, (k, l, m))
