@@ -38,12 +38,18 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic FP64 flop counts (per walker) of the reference algorithm's loop
-# structure, the per-unit figures the roofline is priced with:
+# Work per launch of the timed kernel classes.
+#
+# Reference count (the per-term operation count of CHAMP's loops, the basis of
+# the work-equivalent rates):
 #   move (jastrow4e + AO):  (Ne-1)*Nc*(25T+60) + AO
 #   hpsi_nonloc (per quadrature point): (Ne-1)*(Nc*(4T+40)+60) + Nc*40 + Nao*42
 #   hpsi_jastrow: Ne(Ne-1)/2 * Nc * (54T+80)
-# T = e-e-n terms (13), Nc = centres, Nao = AOs.
+# Executed matrix-core work of the quadrature kernel (k_nonloc_mma): every
+# group of up to four active pairs of a walker runs, for each 16-partner tile
+# and each step of four centres, 3 row tiles x naz table entries of one
+# 16x16x4 MFMA (2048 flop).  T = e-e-n terms (13 at nordc = 4), naz = the
+# non-zero A_kn table entries (12 at nordc = 4), Nc centres, Nao AOs.
 
 
 def flops_move_jastrow(ne, nc, nt, nao, nrad_total):
@@ -52,6 +58,23 @@ def flops_move_jastrow(ne, nc, nt, nao, nrad_total):
 
 def flops_nonloc_point(ne, nc, nt, nao):
     return (ne - 1) * (nc * (4 * nt + 40) + 60) + nc * 40 + nao * 42
+
+
+def een_naz(nordc):
+    """Non-zero entries of the e-e-n tables A_kn (een_poly.hpp): term (k, l, m)
+    fills A[k][m] and A[k][l + m]."""
+    from champ_amd import system as S
+    return len({(k, m) for k, l, m in S.een_terms(nordc)} | {(k, l + m) for k, l, m in S.een_terms(nordc)})
+
+
+def mfma_flops_nonloc(sy, nxq):
+    """Executed MFMA flop of one k_nonloc_mma launch for the walkers' active
+    pair counts (nxquad / nquad per walker)."""
+    nq = max(1, len(sy.wq))
+    pairs = np.asarray(nxq, dtype=np.int64) // nq
+    groups = int(np.sum((pairs + 3) // 4))
+    per_group = -(-sy.nelec // 16) * -(-sy.ncent // 4) * 3 * een_naz(sy.nordc) * 2048
+    return float(groups) * per_group
 
 
 def run_path(sy, W, precision, args, label):
@@ -69,7 +92,8 @@ def run_path(sy, W, precision, args, label):
         eng.metrop1(args.warmup, args.tau)
     eng.enable_timing(True)
     t0 = time.perf_counter()
-    st = eng.metrop1(args.steps, args.tau)
+    # blocks of one sweep (champ_gpu_vmc): the blocked mean and error of E_L
+    vr = eng.vmc(args.steps, 1, nblkeq=0, tau=args.tau)
     t1 = time.perf_counter()
     kt = eng.kernel_times()
     eng.enable_timing(False)
@@ -82,17 +106,40 @@ def run_path(sy, W, precision, args, label):
         tg0 = time.perf_counter()
         eng.metrop1(args.steps, args.tau)
         gwall = time.perf_counter() - tg0
-    eng.close()
+        eng.set_graphs(False)
     wall = t1 - t0
-    print(f"[bench] {label}: {args.steps} sweeps in {wall:.3f}s; acceptance {st['accepted'] / st['proposed']:.3f}; "
-          f"<E_L> {st['esum'] / (W * args.steps):.6f}; flagged {st['nflagged']}", flush=True)
+    print(f"[bench] {label}: {args.steps} sweeps in {wall:.3f}s; acceptance {vr['accept']:.3f}; "
+          f"<E_L> {vr['efin']:.6f} +- {vr['eerr']:.6f}; flagged {vr['nflagged']}", flush=True)
     for k, (ms, n) in kt.items():
         if n:
             print(f"[bench]   {label} {k:20s} {ms:10.2f} ms  {n:6d} launches  {ms / n:9.4f} ms/launch", flush=True)
     if gwall is not None:
         print(f"[bench] {label}: graph replay {args.steps} sweeps in {gwall:.3f}s", flush=True)
-    return {"wall": wall, "value": W * sy.nelec * args.steps / wall, "kt": kt, "st": st, "nxq": nxq, "hpsi0": g0,
-            "graph_value": None if gwall is None else W * sy.nelec * args.steps / gwall, "tuned": tuned}
+    return {"wall": wall, "value": W * sy.nelec * args.steps / wall, "kt": kt, "vmc": vr, "nxq": nxq, "hpsi0": g0,
+            "graph_value": None if gwall is None else W * sy.nelec * args.steps / gwall, "tuned": tuned, "eng": eng}
+
+
+def accuracy_at_sampled(e64, e32, W):
+    """The mixed path's per-walker errors at the configurations its timed
+    sweeps reached: both paths take the mixed population's positions and
+    streams, draw the same quadrature rotation and evaluate hpsi."""
+    x, st = e32.get_walkers(), e32.get_rng()
+    out = []
+    for e in (e64, e32):
+        e.set_walkers(x)
+        e.set_rng(st)
+        e.rotqua()
+        out.append(e.hpsi())
+    g64, gmx = out
+    de = np.abs(gmx["energy"] - g64["energy"])
+    dl = np.abs(np.log(np.abs(gmx["psid"])) + gmx["psij"] - np.log(np.abs(g64["psid"])) - g64["psij"])
+    return {"sample": f"all {W} walkers at the configurations after the mixed path's timed sweeps, identical "
+                      f"positions and quadrature on both paths",
+            "dE_L_max_Ha": float(de.max()), "dE_L_p99_Ha": float(np.percentile(de, 99)),
+            "dE_L_mean_Ha": float(de.mean()),
+            "dE_L_rel_max": float((de / np.maximum(1.0, np.abs(g64["energy"]))).max()),
+            "dlnpsi_max": float(dl.max()), "dlnpsi_mean": float(dl.mean()),
+            "E_L_abs_median_Ha": float(np.median(np.abs(g64["energy"])))}
 
 
 def run_hetero(args):
@@ -147,39 +194,106 @@ def run_hetero(args):
 
 
 # HBM bytes per launch of the quadrature kernel at the default workload (c160,
-# 4096 walkers, the six timed sweeps after ten warm-up sweeps): FETCH_SIZE x 2
-# + WRITE_SIZE x 1 from two rocprofv3 --pmc passes over the default bench,
+# 4096 walkers, the timed sweeps after ten warm-up sweeps): FETCH_SIZE x 2 +
+# WRITE_SIZE x 1 from two rocprofv3 --pmc passes over the default bench,
 # scaled by the same counters on a device-to-device copy of known size
 # (profiles/k_nonloc_c160_traffic.md)
 NONLOC_TRAFFIC = {("c160", 4096, "fp64"): 3.46e9, ("c160", 4096, "mixed"): 3.45e9}
 
+PEAK = {"fp64-mfma": 78.6, "fp32-mfma": 157.3, "fp64-valu": 78.6,
+        # FP32 without packed math (this build never emits v_pk_* FP32): one
+        # FMA per lane per cycle, the FP64 rate
+        "fp32-valu-unpacked": 78.6, "hbm": 8000.0}
 
-def roofline(sy, W, kt, nxq, path):
+
+def per_kernel(sy, W, kt, nxq, path, acc):
+    """Each timed kernel class against its own roof: work per launch (stated
+    per class) over its mean launch time."""
     from champ_amd import system as S
     nt = len(S.een_terms(sy.nordc))
     nrad = int(sum(sy.nrbas[t] for t in sy.iwctype))
-    dname, (dms, dn) = max(kt.items(), key=lambda kv: kv[1][0])
-    # the e-e-n sums of the quadrature run on the matrix cores (FP64 on the FP64
-    # path, FP32 on the mixed path); the per-move and hpsi Jastrow on the VALU
-    pipe, peak = ("fp64-valu", 78.6)
-    traffic = None
+    ns, ne, nc, nb, no = max(sy.nup, sy.ndn), sy.nelec, sy.ncent, sy.nbasis, sy.norb
+    chunk = int(max(1, min(W, 3.0e9 / (ne * 5 * nb * 8))))
+    wchunk = W / -(-W // chunk)  # walkers per hpsi chunk launch, on average
+    valu = "fp32-valu-unpacked" if path == "mixed" else "fp64-valu"
+    mfma = "fp32-mfma" if path == "mixed" else "fp64-mfma"
+    work = {
+        "move_jastrow_ao": (W * flops_move_jastrow(ne, nc, nt, nb, nrad), valu, "TFLOP/s",
+                            "reference count (Ne-1) Nc (25T+60) + AO per walker", False),
+        "move_mo_gemm": (2.0 * 4 * W * no * nb, "fp64-mfma", "TFLOP/s", "2 (4W) norb Nao", True),
+        "move_ratio_commit": (W * 8.0 * (ns + 4 * no + 3 * ne) + W * acc * (16.0 * ns * ns + 8.0 * (3 * no + 20 * ne + 2 * nc)),
+                              "hbm", "GB/s", "inverse row + orbitals per walker; inverse read+write and the "
+                              "Jastrow/orbital commits per accepted walker", True),
+        "hpsi_jastrow": (W * 0.5 * ne * (ne - 1) * nc * (54 * nt + 80), valu, "TFLOP/s",
+                         "reference count Ne(Ne-1)/2 Nc (54T+80) per walker", False),
+        "hpsi_mo_gemm": (2.0 * wchunk * ne * 5 * no * nb, "fp64-mfma", "TFLOP/s", "2 (5 Ne W_chunk) norb Nao", True),
+        "hpsi_slater_lu": (wchunk * 2 * 2.0 * ns ** 3, "fp64-valu", "TFLOP/s", "2 Nsigma^3 per spin matrix", True),
+        "hpsi_nonloc": (mfma_flops_nonloc(sy, nxq), mfma, "TFLOP/s", "executed 16x16x4 MFMA flop (groups x "
+                        "partner tiles x centre quads x 3 row tiles x naz)", True),
+    }
+    out = {}
+    for k, (ms, n) in kt.items():
+        if not n or k not in work:
+            continue
+        w, pipe, unit, basis, util = work[k]
+        t = ms / n / 1e3
+        ach = w / t / (1e9 if unit == "GB/s" else 1e12)
+        out[k] = {"pipe": pipe, "achieved": round(ach, 3), "peak": PEAK[pipe], "unit": unit,
+                  "frac": round(ach / PEAK[pipe], 4), "avg_launch_ms": round(ms / n, 4), "work_per_launch": w,
+                  "work_basis": basis,
+                  # False: a reference per-term count the kernel does not execute
+                  # (its factorised polynomial does less), a work-equivalent rate
+                  "utilisation": util}
+    return out
+
+
+def roofline(sy, W, kt, nxq, path, acc):
+    """The dominant kernel class (most time in the timed sweeps) against its roof."""
+    from champ_amd import system as S
+    pk = per_kernel(sy, W, kt, nxq, path, acc)
+    dname = max((k for k in pk), key=lambda k: kt[k][0])
+    d = pk[dname]
+    bound = "hbm" if d["pipe"] == "hbm" else "mfma"
+    r = {"bound": bound, "kernel": dname, "pipe": d["pipe"], "achieved": d["achieved"], "peak": d["peak"],
+         "unit": d["unit"], "frac": d["frac"], "traffic": None, "avg_launch_ms": d["avg_launch_ms"],
+         "work_per_launch": d["work_per_launch"], "work_basis": d["work_basis"]}
     if dname == "hpsi_nonloc":
-        fl = float(np.sum(nxq)) * flops_nonloc_point(sy.nelec, sy.ncent, nt, sy.nbasis)
-        pipe, peak = ("fp32-mfma", 157.3) if path == "mixed" else ("fp64-mfma", 78.6)
-        traffic = NONLOC_TRAFFIC.get((sy.name, W, path))
-    elif dname == "move_jastrow_ao":
-        fl = W * flops_move_jastrow(sy.nelec, sy.ncent, nt, sy.nbasis, nrad)
-    elif dname == "hpsi_jastrow":
-        fl = W * 0.5 * sy.nelec * (sy.nelec - 1) * sy.ncent * (54 * nt + 80)
-    else:
-        return None
-    if not dn:
-        return None
-    ach = fl / (dms / dn / 1e3) / 1e12
-    return {"bound": "mfma", "pipe": pipe, "kernel": dname, "achieved": round(ach, 3), "peak": peak,
-            "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": traffic, "avg_launch_ms": round(dms / dn, 4),
-            "flops_per_launch": fl,
-            "flops_basis": "reference algorithm's per-term operation count, not the kernel's executed instructions"}
+        npts = float(np.sum(nxq))
+        nt = len(S.een_terms(sy.nordc))
+        ref = npts * flops_nonloc_point(sy.nelec, sy.ncent, nt, sy.nbasis)
+        t = d["avg_launch_ms"] / 1e3
+        r["traffic"] = NONLOC_TRAFFIC.get((sy.name, W, path))
+        r["traffic_source"] = "profiles/k_nonloc_c160_traffic.md" if r["traffic"] else None
+        r["quadrature_points_per_launch"] = int(npts)
+        r["work_equivalent"] = {
+            "achieved": round(ref / t / 1e12, 3), "unit": "TFLOP/s",
+            "basis": "reference per-term count per point x points; the factorised kernel executes ~7x fewer "
+                     "operations, so this rate is not a utilisation"}
+    return r
+
+
+def cpu_baseline(sy, args):
+    """The oracle (one walker chain per thread, -O2 at the host's x86-64
+    level) on the host cores this process may use, at most 16 (the GPU
+    host's share per GPU); two walkers per thread for a fixed number of sweeps."""
+    import ctypes as C
+    from champ_amd import system as S
+    from oracle import ffi
+    model, ncpu, navail, _ = ffi.host_cpu()
+    nth = args.cpu_threads or min(16, navail)
+    L, level = ffi.baseline_lib()
+    nwc = 2 * nth
+    x0 = sy.initial_walkers(nwc, seed=1)
+    streams = ffi.seeded_streams(S.seed_from_string(), nwc)
+    es, ac = np.zeros(1), np.zeros(1)
+    cs = sy.as_ctypes()
+    tc = L.orc_vmc_run(C.byref(cs), nwc, ffi.P(x0), ffi.U(streams), args.cpu_sweeps, args.tau, nth, ffi.P(es),
+                       ffi.P(ac))
+    return {"value": round(nwc * sy.nelec * args.cpu_sweeps / tc, 2), "unit": "walker-moves/s", "cores": nth,
+            "kind": "port", "cpu_model": model, "nproc": ncpu, "cpus_available": navail,
+            "build": f"gcc -O2 -march={level} (the highest level this host supports)",
+            "sample": f"{nwc} walkers x {args.cpu_sweeps} metrop1 sweeps of the same system, two walker chains per "
+                      f"thread on {nth} threads, {tc:.1f} s"}
 
 
 def main():
@@ -191,8 +305,8 @@ def main():
     ap.add_argument("--walkers", type=int, default=4096)
     ap.add_argument("--system", default="c160")
     ap.add_argument("--tau", type=float, default=0.3)
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cpu-sweeps", type=int, default=30)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: the usable cores, at most 16)")
+    ap.add_argument("--cpu-sweeps", type=int, default=15)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="time the plain FP64 path only")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
@@ -220,44 +334,27 @@ def main():
 
     acc = None
     if rmx is not None:
-        g64, gmx = r64["hpsi0"], rmx["hpsi0"]
-        de = np.abs(gmx["energy"] - g64["energy"])
-        lp64 = np.log(np.abs(g64["psid"])) + g64["psij"]
-        lpmx = np.log(np.abs(gmx["psid"])) + gmx["psij"]
-        dl = np.abs(lpmx - lp64)
-        e64 = r64["st"]["esum"] / (W * args.steps)
-        emx = rmx["st"]["esum"] / (W * args.steps)
-        acc = {"sample": f"initial hpsi of all {W} walkers, identical configurations",
-               "dE_L_max_Ha": float(de.max()), "dE_L_p99_Ha": float(np.percentile(de, 99)),
-               "dE_L_mean_Ha": float(de.mean()),
-               "dE_L_rel_max": float((de / np.maximum(1.0, np.abs(g64["energy"]))).max()),
-               "dlnpsi_max": float(dl.max()), "dlnpsi_mean": float(dl.mean()),
-               "vmc_mean_E_fp64": e64, "vmc_mean_E_mixed": emx,
-               "vmc_mean_note": f"mean E_L over the {args.steps} timed sweeps of each run (same seeds)"}
-        print(f"[bench] accuracy: |dE_L| max {acc['dE_L_max_Ha']:.3e} Ha p99 {acc['dE_L_p99_Ha']:.3e} "
-              f"mean {acc['dE_L_mean_Ha']:.3e}; |dln psi| max {acc['dlnpsi_max']:.3e}", flush=True)
+        acc = accuracy_at_sampled(r64["eng"], rmx["eng"], W)
+        e64, emx = r64["vmc"], rmx["vmc"]
+        acc["vmc_mean_E_fp64"] = {"E": e64["efin"], "err": e64["eerr"]}
+        acc["vmc_mean_E_mixed"] = {"E": emx["efin"], "err": emx["eerr"]}
+        acc["vmc_mean_diff_in_fp64_errors"] = round(abs(emx["efin"] - e64["efin"]) / max(e64["eerr"], 1e-300), 3)
+        acc["vmc_mean_note"] = (f"blocked mean of E_L over the {args.steps} timed sweeps of each run (blocks of one "
+                                f"sweep of all walkers, same seeds)")
+        print(f"[bench] accuracy at sampled configurations: |dE_L| max {acc['dE_L_max_Ha']:.3e} Ha p99 "
+              f"{acc['dE_L_p99_Ha']:.3e} mean {acc['dE_L_mean_Ha']:.3e}; |dln psi| max {acc['dlnpsi_max']:.3e}; "
+              f"VMC means {e64['efin']:.4f} +- {e64['eerr']:.4f} (fp64) {emx['efin']:.4f} +- {emx['eerr']:.4f} "
+              f"(mixed)", flush=True)
+    for r in (r64, rmx):
+        if r is not None:
+            r["eng"].close()
 
     het = run_hetero(args) if args.hetero else None
+    cpu = None if args.no_cpu else cpu_baseline(sy, args)
 
-    cpu = None
-    if not args.no_cpu:
-        from oracle import ffi
-        import ctypes as C
-        nth = args.cpu_threads
-        nwc = nth
-        x0 = sy.initial_walkers(nwc, seed=1)
-        streams = ffi.seeded_streams(S.seed_from_string(), nwc)
-        es, ac = np.zeros(1), np.zeros(1)
-        cs = sy.as_ctypes()
-        tc = ffi.lib().orc_vmc_run(C.byref(cs), nwc, ffi.P(x0), ffi.U(streams), args.cpu_sweeps, args.tau, nth,
-                                    ffi.P(es), ffi.P(ac))
-        cpu = {"value": round(nwc * sy.nelec * args.cpu_sweeps / tc, 2), "unit": "walker-moves/s", "cores": nth,
-               "kind": "port",
-               "sample": f"{nwc} walkers x {args.cpu_sweeps} metrop1 sweep(s) of the same system, one walker per "
-                         f"thread, {tc:.1f} s"}
-
+    acc_rate = head["vmc"]["accept"]
     out = {
-        "metric": "walker-moves/sec at 160 e⁻ on 1 MI355X, mixed vs FP64 GPU; ΔE_L vs FP64",
+        "metric": "walker-moves/sec at 160 e\u207b on 1 MI355X, mixed vs FP64 GPU; \u0394E_L vs FP64",
         "value": round(head["value"], 1),
         "unit": "walker-moves/s",
         "n_gpus": 1,
@@ -282,10 +379,12 @@ def main():
             "value": round(head["graph_value"], 1), "unit": "walker-moves/s",
             "note": "the same number of sweeps replayed from the captured sweep graph, without per-kernel events "
                     "(value and roofline come from the event-timed eager launches)"},
-        "acceptance": round(head["st"]["accepted"] / head["st"]["proposed"], 4),
+        "acceptance": round(acc_rate, 4),
         "accuracy_mixed_vs_fp64": acc,
-        "roofline": roofline(sy, W, head["kt"], head["nxq"], "fp64" if rmx is None else "mixed"),
-        "roofline_fp64_path": roofline(sy, W, r64["kt"], r64["nxq"], "fp64"),
+        "roofline": roofline(sy, W, head["kt"], head["nxq"], "fp64" if rmx is None else "mixed", acc_rate),
+        "roofline_fp64_path": roofline(sy, W, r64["kt"], r64["nxq"], "fp64", r64["vmc"]["accept"]),
+        "per_kernel": per_kernel(sy, W, head["kt"], head["nxq"], "fp64" if rmx is None else "mixed", acc_rate),
+        "per_kernel_fp64_path": per_kernel(sy, W, r64["kt"], r64["nxq"], "fp64", r64["vmc"]["accept"]),
         "heterogeneous_batch": het,
         "cpu_baseline": cpu,
     }

@@ -83,6 +83,38 @@ def lib():
     return _lib
 
 
+def host_cpu():
+    """(model name, logical CPUs, CPUs this process may run on, x86-64 level)."""
+    model, flags = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    v4 = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+    v3 = {"avx", "avx2", "fma", "bmi1", "bmi2", "f16c", "movbe"}
+    level = "x86-64-v4" if v4 <= flags and v3 <= flags else "x86-64-v3" if v3 <= flags else "x86-64-v2"
+    return model, os.cpu_count() or 1, len(os.sched_getaffinity(0)), level
+
+
+def baseline_lib():
+    """The oracle built for the host's highest x86-64 level (the CPU baseline
+    leg of the bench), with orc_vmc_run bound; returns (library, level)."""
+    level = host_cpu()[3]
+    name = {"x86-64-v4": "liboracle_v4.so", "x86-64-v3": "liboracle_v3.so"}.get(level, "liboracle.so")
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        path, level = LIB, "x86-64-v2"
+    L = C.CDLL(path)
+    L.orc_vmc_run.argtypes = [C.c_void_p, C.c_int, dp, u64p, C.c_int, C.c_double, C.c_int, dp, dp]
+    L.orc_vmc_run.restype = C.c_double
+    return L, level
+
+
 def P(a: np.ndarray):
     return a.ctypes.data_as(dp)
 
