@@ -61,20 +61,22 @@ FDIR = os.path.join(HERE, "fortran")
 FEXE = os.path.join(FDIR, "vmc_gpu")
 
 
-def build_fortran(verbose: bool = True) -> str:
+def build_fortran(verbose: bool = True, out: str = FEXE) -> str:
     """The Fortran binding module and population VMC driver (amdflang), linked
-    against libchamp_gpu.so next to it."""
+    against libchamp_gpu.so in this package (found at run time through an
+    rpath relative to the executable when it is built in place)."""
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
-    moddir = os.path.join(OBJDIR, "fmod")
+    moddir = os.path.join(OBJDIR, "fmod") if out == FEXE else out + ".mod"
     os.makedirs(moddir, exist_ok=True)
+    rpath = "$ORIGIN/.." if out == FEXE else HERE
     cmd = [fc, "-O2", "-J", moddir, os.path.join(FDIR, "champ_gpu_mod.f90"), os.path.join(FDIR, "vmc_gpu.f90"),
-           "-L", HERE, "-lchamp_gpu", "-Wl,-rpath,$ORIGIN/..", "-o", FEXE]
+           "-L", HERE, "-lchamp_gpu", f"-Wl,-rpath,{rpath}", "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"amdflang failed:\n{r.stderr[-4000:]}")
     if verbose:
-        print(f"built {FEXE}")
-    return FEXE
+        print(f"built {out}")
+    return out
 
 
 if __name__ == "__main__":

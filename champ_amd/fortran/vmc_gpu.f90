@@ -88,7 +88,7 @@ program vmc_gpu
     esum = esum + eblk
     esum2 = esum2 + eblk * eblk
     acc = acc + st%accepted / st%proposed
-    write(*,'(a,i6,a,f22.12,a,f8.5,a,i0)') ' block', iblk, '  E_L', eblk, '  acc', st%accepted / st%proposed, &
+    write(*,'(a,i6,a,es25.17,a,f8.5,a,i0)') ' block', iblk, '  E_L', eblk, '  acc', st%accepted / st%proposed, &
          '  flagged ', st%nflagged
   end do
   call system_clock(c1)
