@@ -3,6 +3,7 @@
 // on the whole walker population on the context's stream.
 #include "engine_internal.hpp"
 #include "hpsi_args.hpp"
+#include "metrop6.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -322,6 +323,7 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
     TRY(dalloc(&c->ratio, W)); TRY(dalloc(&c->psidn, W)); TRY(dalloc(&c->psijn, W));
     TRY(dalloc(&c->vnew, (size_t)3 * W)); TRY(dalloc(&c->vold, (size_t)3 * W));
     TRY(dalloc(&c->acc, W)); TRY(dalloc(&c->esum_dev, 4));
+    TRY(dalloc(&c->m6rec, (size_t)W * M6_REC));
     // hpsi scratch
     const double per_w = (double)NE * 5 * NB * 8;
     c->hpsi_chunk = (int)std::max(1.0, std::min((double)W, 3.0e9 / per_w));
@@ -383,7 +385,7 @@ extern "C" int champ_gpu_free(champ_ctx *c)
                     c->vjnT, c->fsumn, c->aomove, c->orbn, c->ratio, c->psidn, c->psijn, c->vnew, c->vold, c->acc,
                     c->esum_dev, c->aoall, c->orball, c->wmat, c->pair_count, c->pair_off, c->pair_total, c->grp_off,
                     c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->rrT, c->nxquad, c->ri_cent, c->ri_type,
-                    c->ri_rad, c->ri_rmax, c->ao_ri, c->gref, c->lu_ws};
+                    c->ri_rad, c->ri_rmax, c->ao_ri, c->gref, c->lu_ws, c->m6rec};
     for (void *p : ptrs) if (p) hipFree(p);
     md_free(c);
     if (c->sweep_exec) hipGraphExecDestroy(c->sweep_exec);
@@ -610,11 +612,13 @@ static int sweep_stats(champ_ctx *c, int32_t nsweep, champ_sweep_stats *stats)
     return CHAMP_OK;
 }
 
-// The sweep captured once per tau as a HIP graph and replayed; it holds
-// kernel nodes only (the estimator sums are cleared before the first replay).
+// The sweep captured once per (tau, mover, metrop6 parameters) as a HIP graph
+// and replayed; it holds kernel nodes only (the estimator sums are cleared
+// before the first replay).
 static int sweep_graph(champ_ctx *c, double tau)
 {
-    if (c->sweep_exec && c->sweep_tau == tau) return CHAMP_OK;
+    const double key[5] = {tau, (double)c->imetro, c->m6_deltar, c->m6_deltat, c->m6_fbias};
+    if (c->sweep_exec && std::equal(key, key + 5, c->sweep_key)) return CHAMP_OK;
     if (c->sweep_exec) { HIPCHECK(hipGraphExecDestroy(c->sweep_exec)); c->sweep_exec = nullptr; }
     HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
     const int rc = launch_sweep(c);
@@ -625,7 +629,7 @@ static int sweep_graph(champ_ctx *c, double tau)
     const hipError_t ei = hipGraphInstantiate(&c->sweep_exec, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     if (ei != hipSuccess) { c->sweep_exec = nullptr; set_error(std::string("sweep graph: ") + hipGetErrorString(ei)); return CHAMP_ERR_HIP; }
-    c->sweep_tau = tau;
+    std::copy(key, key + 5, c->sweep_key);
     return CHAMP_OK;
 }
 
@@ -757,9 +761,9 @@ extern "C" int champ_gpu_set_graphs(champ_ctx *c, int32_t on)
     return CHAMP_OK;
 }
 
-extern "C" int champ_gpu_metrop1(champ_ctx *c, int32_t nsweep, double tau, champ_sweep_stats *stats)
+// nsweep sweeps with the context's current mover (c->imetro)
+static int run_sweeps(champ_ctx *c, int32_t nsweep, double tau, champ_sweep_stats *stats)
 {
-    if (!c || nsweep < 0 || !(tau > 0)) return CHAMP_ERR_ARG;
     c->tau = tau;
     HIPCHECK(hipMemsetAsync(c->esum_dev, 0, sizeof(double) * 4, c->stream));
     int rc;
@@ -775,6 +779,32 @@ extern "C" int champ_gpu_metrop1(champ_ctx *c, int32_t nsweep, double tau, champ
     return sweep_stats(c, nsweep, stats);
 }
 
+extern "C" int champ_gpu_metrop1(champ_ctx *c, int32_t nsweep, double tau, champ_sweep_stats *stats)
+{
+    if (!c || nsweep < 0 || !(tau > 0)) return CHAMP_ERR_ARG;
+    c->imetro = 1;
+    return run_sweeps(c, nsweep, tau, stats);
+}
+
+extern "C" int champ_gpu_set_metrop6(champ_ctx *c, double deltar, double deltat, double fbias)
+{
+    if (!c || !std::isfinite(deltar) || !std::isfinite(deltat) || !std::isfinite(fbias)) return CHAMP_ERR_ARG;
+    // the input checks of parser.f90:774-787: out-of-range steps are reset, fbias truncated to [0, 2]
+    c->m6_deltar = deltar < 1.0 ? 2.0 : deltar;
+    c->m6_deltat = (deltat < 0.0 || deltat > 2.0) ? 2.0 : deltat;
+    c->m6_fbias = std::min(2.0, std::max(0.0, fbias));
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_metrop6(champ_ctx *c, int32_t nsweep, champ_sweep_stats *stats)
+{
+    if (!c || nsweep < 0) return CHAMP_ERR_ARG;
+    c->imetro = 6;
+    const int rc = run_sweeps(c, nsweep, c->tau, stats);
+    c->imetro = 1;
+    return rc;
+}
+
 extern "C" int champ_gpu_vmc(champ_ctx *c, int32_t nblk, int32_t nstep, int32_t nblkeq, double tau, int32_t imetro,
                              double *enow, champ_vmc_result *res)
 {
@@ -782,17 +812,21 @@ extern "C" int champ_gpu_vmc(champ_ctx *c, int32_t nblk, int32_t nstep, int32_t 
         set_error("vmc: nblk, nstep >= 1, nblkeq >= 0, tau > 0 and imetro 1 or 6 required");
         return CHAMP_ERR_ARG;
     }
-    if (imetro == 6) { set_error("vmc: metrop6 moves are not built"); return CHAMP_ERR_UNSUPPORTED; }
     int rc;
     champ_sweep_stats st;
+    // the mover of vmc.f90:120-127: metrop6 for imetro 6 (its parameters from
+    // champ_gpu_set_metrop6; tau unused), metrop1 otherwise
+    auto sweeps = [&](champ_sweep_stats *s) {
+        return imetro == 6 ? champ_gpu_metrop6(c, nstep, s) : champ_gpu_metrop1(c, nstep, tau, s);
+    };
     // equilibration blocks, not accumulated (vmc.f90: vmc_nblkeq before zerest)
     for (int b = 0; b < nblkeq; b++)
-        if ((rc = champ_gpu_metrop1(c, nstep, tau, nullptr))) return rc;
+        if ((rc = sweeps(nullptr))) return rc;
     // acuest: ecum += esum, ecm2 += esum * enow, wcum += wsum per block
     double wcum = 0, ecum = 0, ecm2 = 0, e2cum = 0, acc = 0, prop = 0;
     const double wsum = (double)c->W * nstep;
     for (int b = 0; b < nblk; b++) {
-        if ((rc = champ_gpu_metrop1(c, nstep, tau, &st))) return rc;
+        if ((rc = sweeps(&st))) return rc;
         const double en = st.esum / wsum;
         if (enow) enow[b] = en;
         wcum += wsum; ecum += st.esum; ecm2 += st.esum * en; e2cum += st.esum2;
@@ -823,7 +857,7 @@ extern "C" int champ_gpu_metrop1_batch(champ_ctx *const *ctxs, int32_t n, int32_
             if (ctxs[k] == ctxs[m]) { set_error("metrop1_batch: a context appears twice"); return CHAMP_ERR_ARG; }
         nemax = std::max(nemax, ctxs[m]->NE);
     }
-    for (int m = 0; m < n; m++) ctxs[m]->tau = tau;
+    for (int m = 0; m < n; m++) { ctxs[m]->tau = tau; ctxs[m]->imetro = 1; }
     for (int m = 0; m < n; m++) HIPCHECK(hipMemsetAsync(ctxs[m]->esum_dev, 0, sizeof(double) * 4, ctxs[m]->stream));
     // Each population's kernels go to its own stream in the order of its
     // own metrop1 sweep; the host deals the move steps round-robin over the

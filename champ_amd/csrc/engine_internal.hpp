@@ -38,6 +38,8 @@ struct champ_ctx {
     int nloc = 0, prec = 0, hpsi_chunk = 0, pair_cap = 0;
     double pecent = 0, cdet0 = 1;
     double tau = 0.5;  // time step of the metropolis kernels, set by each metrop1 / metrop1_batch call
+    int imetro = 1;    // mover of the sweep kernels: 1 metrop1 (drift-diffusion), 6 metrop6 (accelerated)
+    double m6_deltar = 5.0, m6_deltat = 1.0, m6_fbias = 1.0;  // metrop6 parameters (parser.f90:402-404)
     hipStream_t stream = nullptr;
 
     // ---- tables
@@ -97,6 +99,7 @@ struct champ_ctx {
     double *aomove = nullptr;  // [W][4][NB] (the mixed path stores FP32 values in its first half)
     double *orbn = nullptr;    // [W][4][NO]
     double *ratio = nullptr, *psidn = nullptr, *psijn = nullptr, *vnew = nullptr /*[W][3]*/, *vold = nullptr;
+    double *m6rec = nullptr;   // [W][M6_REC] metrop6 proposal record
     unsigned char *acc = nullptr;
     double *esum_dev = nullptr;  // [4] accepted, esum, esum2, nflag
 
@@ -144,7 +147,7 @@ struct champ_ctx {
     // replaying the ~500-node sweep graph measured 5% slower than launching
     int graphs = 0;
     hipGraphExec_t sweep_exec = nullptr;
-    double sweep_tau = 0;
+    double sweep_key[5] = {0, 0, 0, 0, 0};  // tau, mover and metrop6 parameters the graph was captured with
 
     // ---- timing
     bool timing = false;

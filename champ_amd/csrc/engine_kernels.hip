@@ -9,6 +9,7 @@
 // AO->MO contractions are population-wide GEMMs.
 #include "een_poly.hpp"
 #include "engine_internal.hpp"
+#include "metrop6.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -38,6 +39,10 @@ struct MoveArgs {
     double *md_S, *md_Sn;              // [W] psi_D / (D_up,ref D_dn,ref), current / proposed
     int *md_cur;                       // [W][2] current buffer of the multideterminant state
     int ao32;                          // AO block stored as FP32 for the FP32 contraction
+    int imetro;                        // mode-0 mover: 1 metrop1, 6 metrop6
+    M6Params m6p;
+    double *m6rec;                     // [W][M6_REC]
+    const EcpParams *ecp;              // znuc per type and nloc (metrop6's cusp and theta range)
 };
 
 static MoveArgs move_args(champ_ctx *c, int iel, int mode, double tau)
@@ -60,6 +65,10 @@ static MoveArgs move_args(champ_ctx *c, int iel, int mode, double tau)
     a.md = c->md; a.md_vcol = c->md_vcol; a.md_vcoln = c->md_vcoln; a.md_S = c->md_S; a.md_Sn = c->md_Sn;
     a.md_cur = c->md_cur;
     a.ao32 = c->prec == CHAMP_PREC_MIXED && c->tune_gemm[0] == GEMM_F32;
+    a.imetro = c->imetro;
+    a.m6p = M6Params{c->m6_deltar, c->m6_deltat, c->m6_fbias};
+    a.m6rec = c->m6rec;
+    a.ecp = c->ecp;
     return a;
 }
 
@@ -106,8 +115,10 @@ __global__ void __launch_bounds__(256) k_drift_old(MoveArgs a)
 
 // ------------------------------------------------------------------------
 // Move kernel A, one wavefront per walker:
-//   mode 0: metrop1 proposal (drift at old position, 3 gauss draws;
-//           metrop_mov1_driftdif.f90:105-117); mode 1: xnew given.
+//   mode 0: proposal from the drift at the old position: metrop1 (3 gauss
+//           draws; metrop_mov1_driftdif.f90:105-117) or metrop6 (local
+//           spherical move about the nearest nucleus; metrop6.hpp);
+//   mode 1: xnew given.
 //   then distances(iel) (distances.f90:3-167), jastrow4e with iflag=0
 //   (jastrow4e.f90:3-358) and basis_fns(iel, ider=1) (basis_fns.f90:8-214)
 //   writing the AO block that the MO GEMM contracts (orbitalse).
@@ -162,7 +173,17 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
         if (a.mode == 0) {
             double vo[3];
             drift_old(a, w, lane, vo);
-            if (lane == 0) {
+            if (lane == 0 && a.imetro == 6) {
+                uint64_t s[4];
+                for (int k = 0; k < 4; k++) s[k] = a.rng[4 * (size_t)w + k];
+                const double xo[3] = {xw[iel], xw[NE + iel], xw[2 * NE + iel]};
+                double xn[3];
+                m6_propose(a.cent, a.iwctype, a.ecp->znuc, a.ecp->nloc, NC, a.m6p, xo, vo, s, xn,
+                           a.m6rec + (size_t)w * M6_REC);
+                for (int k = 0; k < 4; k++) a.rng[4 * (size_t)w + k] = s[k];
+                for (int k = 0; k < 3; k++) { a.xnew[3 * w + k] = xn[k]; a.vold[3 * w + k] = vo[k]; }
+                xn0 = xn[0]; xn1 = xn[1]; xn2 = xn[2];
+            } else if (lane == 0) {
                 const double tau = a.tau, rttau = sqrt(tau), adrift = 0.5;
                 uint64_t s[4];
                 for (int k = 0; k < 4; k++) s[k] = a.rng[4 * (size_t)w + k];
@@ -492,7 +513,26 @@ __global__ void __launch_bounds__(256) k_move_b(MoveArgs a)
                                              : 0.0 + detn * a.detiab[2 * w + 1 - iab] * (a.md ? a.md_Sn[w] : a.cdet0);
             const double psijn = a.psijn[w];
             int accept = 0;
-            if (a.mode == 0) {
+            if (a.mode == 0 && a.imetro == 6) {
+                const double psi2n = 2 * (log(fabs(psidn)) + psijn);
+                const double *xw = a.xT + (size_t)w * 3 * NE;
+                const double xo[3] = {xw[iel], xw[NE + iel], xw[2 * NE + iel]};
+                const double xn[3] = {a.xnew[3 * w], a.xnew[3 * w + 1], a.xnew[3 * w + 2]};
+                const double *rec = a.m6rec + (size_t)w * M6_REC;
+                const double p = m6_accept_prob(a.cent, a.iwctype, a.ecp->znuc, a.ecp->nloc, a.m6p, xo, xn, vn, psi2n,
+                                                a.psi2o[w], psidn == 0.0, rec);
+                uint64_t s[4];
+                for (int k = 0; k < 4; k++) s[k] = a.rng[4 * (size_t)w + k];
+                const double uacc = xo_uniform(s);
+                for (int k = 0; k < 4; k++) a.rng[4 * (size_t)w + k] = s[k];
+                accept = uacc < p;
+                if (accept) {
+                    a.psi2o[w] = psi2n;
+                    atomicAdd(&a.esum_dev[0], 1.0);
+                }
+                // psi is evaluated at proposals that pass the geometric test only
+                if (rt == 0.0 && rec[M6_LIVE] != 0.0) atomicOr(&a.status[w], CHAMP_WALKER_ZERO_RATIO);
+            } else if (a.mode == 0) {
                 const double tau = a.tau, adrift = 0.5;
                 const double psi2n = 2 * (log(fabs(psidn)) + psijn);
                 const double v2new = vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2];

@@ -72,6 +72,8 @@ def load_library(path: str = LIBPATH):
     L.champ_gpu_psie.argtypes = [vp, i32, dp, dp, dp, i32]
     L.champ_gpu_accept.argtypes = [vp, i32, C.POINTER(C.c_uint8)]
     L.champ_gpu_metrop1.argtypes = [vp, i32, C.c_double, C.POINTER(SweepStats)]
+    L.champ_gpu_metrop6.argtypes = [vp, i32, C.POINTER(SweepStats)]
+    L.champ_gpu_set_metrop6.argtypes = [vp, C.c_double, C.c_double, C.c_double]
     L.champ_gpu_get_walker_scalars.argtypes = [vp, dp, dp, dp, dp]
     L.champ_gpu_get_status.argtypes = [vp, C.POINTER(C.c_int32)]
     L.champ_gpu_array_size.argtypes = [vp, i32]
@@ -191,6 +193,15 @@ class Engine:
         st = SweepStats()
         self._check(self.L.champ_gpu_metrop1(self.ptr, int(nsweep), float(tau), C.byref(st)), "metrop1")
         return dict(accepted=st.accepted, proposed=st.proposed, esum=st.esum, esum2=st.esum2, nflagged=st.nflagged)
+
+    def metrop6(self, nsweep: int) -> Dict[str, float]:
+        """nsweep metrop6 sweeps (CHAMP's accelerated Metropolis mover)"""
+        st = SweepStats()
+        self._check(self.L.champ_gpu_metrop6(self.ptr, int(nsweep), C.byref(st)), "metrop6")
+        return dict(accepted=st.accepted, proposed=st.proposed, esum=st.esum, esum2=st.esum2, nflagged=st.nflagged)
+
+    def set_metrop6(self, deltar: float = 5.0, deltat: float = 1.0, fbias: float = 1.0):
+        self._check(self.L.champ_gpu_set_metrop6(self.ptr, float(deltar), float(deltat), float(fbias)), "set_metrop6")
 
     def vmc(self, nblk: int, nstep: int, nblkeq: int = 0, tau: float = 0.5, imetro: int = 1):
         """Blocked VMC (vmc.f90 with acuest/finwrt estimators): returns the

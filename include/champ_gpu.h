@@ -210,6 +210,21 @@ int champ_gpu_accept(champ_ctx *ctx, int32_t iel, const uint8_t *accept);
 int champ_gpu_metrop1(champ_ctx *ctx, int32_t nsweep, double tau,
                       champ_sweep_stats *stats);
 
+/* metrop6, CHAMP's default mover (imetro=6, metrop_mov1_slat.f90:3-738; the
+ * accelerated Metropolis method of Umrigar, PRL 71, 408 (1993)) over the
+ * whole population for nsweep sweeps: rotqua (if nloc>0), Ne single-electron
+ * moves sampled in local spherical coordinates about each electron's nearest
+ * nucleus with the drift-biased radial and azimuthal densities, geometric
+ * rejections and the reverse-move acceptance, then hpsi.  iguiding = 0,
+ * node_cutoff = 0.  Parameters from champ_gpu_set_metrop6 (CHAMP's defaults
+ * deltar 5, deltat 1, fbias 1 until then).                                   */
+int champ_gpu_metrop6(champ_ctx *ctx, int32_t nsweep, champ_sweep_stats *stats);
+
+/* deltar, deltat, fbias of metrop6 (the metropolis module of parser.f90:
+ * 402-404), with the reference's input checks (:774-787): deltar < 1 and
+ * deltat outside [0, 2] are reset to 2, fbias is truncated to [0, 2].       */
+int champ_gpu_set_metrop6(champ_ctx *ctx, double deltar, double deltat, double fbias);
+
 /* Result of a blocked VMC run (champ_gpu_vmc).                            */
 typedef struct champ_vmc_result {
     double efin;            /* total E: sum of E_L over the accumulated blocks / samples   */
@@ -227,8 +242,8 @@ typedef struct champ_vmc_result {
  * nblk blocks of nstep sweeps whose energies are accumulated per block as
  * acuest does (acuest.f90; the block weight is nwalk * nstep, as the sum over
  * CHAMP's one-walker ranks in acuest_reduce).  imetro = 1: metrop1 sweeps
- * (drift-diffusion, time step tau); imetro = 6 (metrop6) is refused with
- * CHAMP_ERR_UNSUPPORTED.  enow (may be NULL) [nblk]: the block averages.
+ * (drift-diffusion, time step tau); imetro = 6: metrop6 sweeps (tau unused,
+ * as in the reference).  enow (may be NULL) [nblk]: the block averages.
  * res (may be NULL).                                                        */
 int champ_gpu_vmc(champ_ctx *ctx, int32_t nblk, int32_t nstep, int32_t nblkeq, double tau, int32_t imetro,
                   double *enow, champ_vmc_result *res);

@@ -1439,6 +1439,290 @@ int orc_metrop1_sweep(const champ_system_t *s, oracle_walker_t *w, double tau)
     return acc;
 }
 
+/* ------------------------------------------------------------------------ */
+/* gammai (gammai.f90:3-108): incomplete gamma function of a = 3/2 or 5/2 with */
+/* xae = x^a e^-x given; a fixed number of series terms for x < a+4 (returns */
+/* gamma(a,x), iflag 1), else a fixed number of continued-fraction steps     */
+/* (returns -Gamma(a,x), the complement, iflag -1).                         */
+double orc_gammai(double a, double x, double xae, int *iflag)
+{
+    if (x < a + 4) {
+        *iflag = 1;
+        double ap = a, sum = 1 / a, del = sum;
+        const int iter = (int)(21 + 3 * (x - a));
+        for (int n = 1; n <= iter; n++) {
+            ap = ap + 1;
+            del = del * x / ap;
+            sum = sum + del;
+        }
+        return sum * xae;
+    }
+    *iflag = -1;
+    double a0 = 1, a1 = x, b0 = 0, b1 = 1, fac = 1, g = 0;
+    const double t = 20 + round(a - x);  /* nint: half away from zero */
+    const int iter = t > 7 ? (int)t : 7;
+    for (int n = 1; n <= iter; n++) {
+        const double an = (double)n, ana = an - a;
+        a0 = (a1 + a0 * ana) * fac;
+        b0 = (b1 + b0 * ana) * fac;
+        const double anf = an * fac;
+        a1 = x * a0 + anf * a1;
+        b1 = x * b0 + anf * b1;
+        fac = 1 / a1;
+        g = b1 * fac;
+    }
+    return -(g * xae);
+}
+
+/* Normalisation of metrop6's radial transition density (metrop_mov1_slat.f90:
+ * 256-293 forward, 490-527 reverse): the integral of
+ * sqrt(r) |1 + co r| exp(-zeta r) over [rbot, rtop] divided by bot, from the
+ * incomplete gamma functions, split at the zero r = -1/co of 1 + co r.       */
+double orc_m6_area(double zeta, double co, double rbot, double rtop, double bot)
+{
+    const double d5b2 = 2.5, d2b3 = .666666666666667, g5b2 = 1.329340388179137;
+    int iflagb, iflagt, iflagz;
+    const double rzero = -1 / co;
+    const double zrbot = zeta * rbot, zrtop = zeta * rtop;
+    const double zebot = sqrt(orc_powi(zrbot, 3)) * exp(-zrbot);
+    const double zetop = sqrt(orc_powi(zrtop, 3)) * exp(-zrtop);
+    const double g52bot = orc_gammai(d5b2, zrbot, zrbot * zebot, &iflagb);
+    const double g52top = orc_gammai(d5b2, zrtop, zrtop * zetop, &iflagt);
+    if (rzero < rbot || rzero > rtop) {
+        const double g52dif = iflagb * iflagt == 1 ? g52top - g52bot : g52top - g52bot + g5b2;
+        const double g32dif = d2b3 * (g52dif + zetop - zebot);
+        return fabs(g32dif + co * g52dif / zeta) / (bot * sqrt(orc_powi(zeta, 3)));
+    }
+    const double zrzer = zeta * rzero;
+    const double zezer = sqrt(orc_powi(zrzer, 3)) * exp(-zrzer);
+    const double g52zer = orc_gammai(d5b2, zrzer, zrzer * zezer, &iflagz);
+    const double g52dif1 = iflagb * iflagz == 1 ? g52zer - g52bot : g52zer - g52bot + g5b2;
+    const double g32dif1 = d2b3 * (g52dif1 + zezer - zebot);
+    const double g52dif2 = iflagt * iflagz == 1 ? g52top - g52zer : g52top - g52zer + g5b2;
+    const double g32dif2 = d2b3 * (g52dif2 + zetop - zezer);
+    return (fabs(g32dif1 + co * g52dif1 / zeta) + fabs(g32dif2 + co * g52dif2 / zeta)) /
+           (bot * sqrt(orc_powi(zeta, 3)));
+}
+
+/* zeta and co of the Slater-type radial density from the radial velocity   */
+/* (metrop_mov1_slat.f90:204-218 forward, 473-486 reverse)                  */
+static void m6_zeta_co(double zcusp, double vr, double rmin, double *zeta, double *co)
+{
+    const double eps = 1.e-10;
+    double root = (zcusp + vr) * (zcusp + vr - 4 / rmin), z;
+    if (root >= 0) {
+        root = sqrt(root);
+        z = 0.5 * (zcusp - vr - root);
+        if (z <= 0) z = 0.5 * (zcusp - vr + root);
+        if (z <= 0) z = eps;
+    } else {
+        z = vr < 0 ? -vr - eps : 1;
+    }
+    *zeta = z;
+    *co = (z + vr) / (1 - (z + vr) * rmin);
+}
+
+static double m6_thetamx(double r, double z, double deltat)
+{
+    return deltat + (2 - deltat) / (1 + orc_powi(z * r, 2));
+}
+
+/* nearest centre to point x (first index on ties) and its squared distance */
+static int m6_nearest(const champ_system_t *s, const double x[3], double *d2)
+{
+    int best = 0;
+    double dmin = 99.e9;
+    for (int j = 0; j < s->ncent; j++) {
+        double dist = 0;
+        for (int k = 0; k < 3; k++) dist = dist + orc_powi(x[k] - s->cent[3 * j + k], 2);
+        if (dist < dmin) { dmin = dist; best = j; }
+    }
+    *d2 = dmin;
+    return best;
+}
+
+/* One metrop6 sweep (metrop_mov1_slat.f90:3-738): the accelerated Metropolis
+ * move of every electron in local spherical coordinates about its nearest
+ * nucleus, iguiding = 0, node_cutoff = 0, one state; then hpsi.  The nearest
+ * nucleus, rvmino and rmino of electron i are functions of its position
+ * (dumper_more.f90:271-286 at start; nearestn/rvminn taken over on
+ * acceptance), so they are recomputed here rather than carried.  The
+ * rejection loops of the radial and azimuthal samplings are capped at 10^6
+ * trials (never reached in practice); the walker then stays put.  Returns
+ * the accepted moves. */
+int orc_metrop6_sweep(const champ_system_t *s, oracle_walker_t *w, double deltar, double deltat, double fbias)
+{
+    const double eps = 1.e-10, pi = 4.0 * atan(1.0);
+    const double deltri = 1 / deltar;
+    const int ne = w->nelec;
+    int acc = 0;
+    if (s->nloc > 0) orc_rotqua(s, w->rng, w->xq, w->yq, w->zq);
+    for (int i = 0; i < ne; i++) {
+        double xold[3], xnew[3], vold[3], vnew[3], xaxis[3], yaxis[3], zaxis[3], rvmino[3], rvminn[3], rvminon[3];
+        for (int k = 0; k < 3; k++) xold[k] = w->x[3 * i + k];
+        orc_determinante_grad(s, w, i, 1, vold);
+        double d2;
+        const int nearo = m6_nearest(s, xold, &d2);
+        for (int k = 0; k < 3; k++) rvmino[k] = xold[k] - s->cent[3 * nearo + k];
+        const double rmino = sqrt(orc_powi(rvmino[0], 2) + orc_powi(rvmino[1], 2) + orc_powi(rvmino[2], 2));
+        const double znear = s->znuc[s->iwctype[nearo]];
+        double zcusp = s->nloc == 0 ? znear : 0;
+        double fxop = 1;
+        double rbot = rmino * deltri, rtop = rmino * deltar;
+        double voldr = 0;
+        for (int k = 0; k < 3; k++) voldr = voldr + vold[k] * rvmino[k];
+        voldr = voldr / rmino;
+        double voldp = 0;
+        for (int k = 0; k < 3; k++) {
+            zaxis[k] = rvmino[k] / rmino;
+            xaxis[k] = vold[k] - voldr * zaxis[k];
+            voldp = voldp + orc_powi(xaxis[k], 2);
+        }
+        voldp = sqrt(voldp);
+        if (voldp < eps) {
+            xaxis[0] = eps * (1 - orc_powi(zaxis[0], 2));
+            xaxis[1] = eps * (-zaxis[0] * zaxis[1]);
+            xaxis[2] = eps * (-zaxis[0] * zaxis[2]);
+            voldp = eps * sqrt(1 - orc_powi(zaxis[0], 2));
+        }
+        for (int k = 0; k < 3; k++) xaxis[k] = xaxis[k] / voldp;
+        voldr = fmax(voldr, -2 * znear);
+        yaxis[0] = zaxis[1] * xaxis[2] - zaxis[2] * xaxis[1];
+        yaxis[1] = zaxis[2] * xaxis[0] - zaxis[0] * xaxis[2];
+        yaxis[2] = zaxis[0] * xaxis[1] - zaxis[1] * xaxis[0];
+        voldp = voldp * fbias;
+        double zeta, co;
+        m6_zeta_co(zcusp, voldr, rmino, &zeta, &co);
+        /* maximum of the radial density on [rbot, rtop] for rejection sampling */
+        double root = sqrt(orc_powi(1.5 * co - zeta, 2) + 2 * zeta * co);
+        double rmax1 = (1.5 * co - zeta + root) / (2 * zeta * co);
+        if (rmax1 < rbot) rmax1 = rbot;
+        if (rmax1 > rtop) rmax1 = rtop;
+        double fmx = sqrt(rmax1) * fabs(1 + co * rmax1) * exp(-zeta * rmax1);
+        if (co < 0) {
+            double rmax2 = (1.5 * co - zeta - root) / (2 * zeta * co);
+            if (rmax2 < rbot) rmax2 = rbot;
+            if (rmax2 > rtop) rmax2 = rtop;
+            const double fmax2 = sqrt(rmax2) * fabs(1 + co * rmax2) * exp(-zeta * rmax2);
+            fmx = fmx > fmax2 ? fmx : fmax2;
+        }
+        const double bot = sqrt(rmino) * fabs(1 + co * rmino) * exp(-zeta * rmino);
+        double rtry = 0, top = 0;
+        int ok = 0;
+        for (int trial = 0; trial < 1000000 && !ok; trial++) {
+            rtry = ((deltar - deltri) * orc_random_dp(w->rng) + deltri) * rmino;
+            top = sqrt(rtry) * fabs(1 + co * rtry) * exp(-zeta * rtry);
+            ok = top / fmx > orc_random_dp(w->rng);
+        }
+        fxop = fxop * top / bot;
+        double areao = orc_m6_area(zeta, co, rbot, rtop, bot);
+        /* cos(theta), then phi by rejection */
+        const double raver = 0.5 * (rmino + rtry);
+        const double deltt = m6_thetamx(raver, znear, deltat);
+        double costht = 1 - deltt * orc_random_dp(w->rng);
+        const double zprime = rtry * costht;
+        double sintht = sqrt(1 - costht * costht);
+        areao = areao * deltt;
+        double term = fmin(voldp * raver * sintht, 1.0);
+        fmx = 1 + term;
+        double phitry = 0, cosphi = 0;
+        int okp = 0;
+        for (int trial = 0; trial < 1000000 && !okp; trial++) {
+            phitry = pi * orc_random_dp(w->rng);
+            cosphi = cos(phitry);
+            top = 1 + term * cosphi;
+            okp = top > orc_random_dp(w->rng) * fmx;
+        }
+        fxop = fxop * top;
+        const double xprime = rtry * sintht * cos(phitry);
+        double yprime = sqrt(fmax(0.0, rtry * rtry - orc_powi(xprime, 2) - orc_powi(zprime, 2)));
+        if (orc_random_dp(w->rng) < 0.5) yprime = -yprime;
+        for (int k = 0; k < 3; k++) xnew[k] = xaxis[k] * xprime + yaxis[k] * yprime + zaxis[k] * zprime + s->cent[3 * nearo + k];
+        const double rminno = rtry;
+        /* geometric rejection: the reverse move must be a possible forward move */
+        const int nearn = m6_nearest(s, xnew, &d2);
+        const double rminn = sqrt(d2);
+        double rminon = 0, dot = 0;
+        for (int k = 0; k < 3; k++) {
+            rvminn[k] = xnew[k] - s->cent[3 * nearn + k];
+            rvminon[k] = xold[k] - s->cent[3 * nearn + k];
+            rminon = rminon + orc_powi(rvminon[k], 2);
+            dot = dot + rvminn[k] * rvminon[k];
+        }
+        rminon = sqrt(rminon);
+        dot = dot / (rminn * rminon);
+        costht = dot;
+        sintht = sqrt(1 - costht * costht);
+        const double ravern = 0.5 * (rminn + rminon);
+        const double znearn = s->znuc[s->iwctype[nearn]];
+        const double delttn = m6_thetamx(ravern, znearn, deltat);
+        double p = 0, psidn = 0, psijn = 0, psi2n = 0;
+        const int geo = rminon > rminn * deltar || dot < 1 - delttn;
+        if (!(ok && okp)) {
+            p = 0;  /* a rejection loop hit its cap: the walker stays */
+        } else if (!geo) {
+            const double rratio = rminno / rminon;
+            orc_psie(s, w, i, xnew, &psidn, &psijn, 0);
+            orc_determinante_grad(s, w, i, 0, vnew);
+            psi2n = 2 * (log(fabs(psidn)) + psijn);
+            /* reverse transition */
+            double fxnp = 1;
+            rbot = rminn * deltri;
+            rtop = rminn * deltar;
+            double vnewr = 0;
+            for (int k = 0; k < 3; k++) vnewr = vnewr + vnew[k] * rvminn[k];
+            vnewr = vnewr / rminn;
+            double vnewp = 0;
+            for (int k = 0; k < 3; k++) {
+                xaxis[k] = vnew[k] - vnewr * rvminn[k] / rminn;
+                vnewp = vnewp + orc_powi(xaxis[k], 2);
+            }
+            vnewp = sqrt(vnewp);
+            if (vnewp < eps) {
+                xaxis[0] = eps * (1 - orc_powi(rvminn[0], 2));
+                xaxis[1] = eps * (-rvminn[0] * rvminn[1]);
+                xaxis[2] = eps * (-rvminn[0] * rvminn[2]);
+                vnewp = eps * sqrt(1 - orc_powi(rvminn[0], 2) * (2 - orc_powi(rminn, 2)));
+            }
+            for (int k = 0; k < 3; k++) xaxis[k] = xaxis[k] / vnewp;
+            zcusp = s->nloc == 0 ? znearn : 0;
+            vnewr = fmax(vnewr, -2 * znearn);
+            vnewp = vnewp * fbias;
+            m6_zeta_co(zcusp, vnewr, rminn, &zeta, &co);
+            const double botn = sqrt(rminn) * fabs(1 + co * rminn) * exp(-zeta * rminn);
+            const double topn = sqrt(rminon) * fabs(1 + co * rminon) * exp(-zeta * rminon);
+            fxnp = fxnp * topn / botn;
+            double arean = orc_m6_area(zeta, co, rbot, rtop, botn);
+            arean = arean * delttn;
+            term = fmin(vnewp * ravern * sintht, 1.0);
+            cosphi = 0;
+            double rnorm = 0;
+            for (int k = 0; k < 3; k++) {
+                const double term2 = rvminon[k] / rminon - costht * rvminn[k] / rminn;
+                rnorm = rnorm + term2 * term2;
+                cosphi = cosphi + term2 * xaxis[k];
+            }
+            cosphi = cosphi / sqrt(rnorm);
+            fxnp = fxnp * (1 + term * cosphi);
+            p = orc_powi(rratio, 2) * exp(psi2n - w->psi2o) * fabs((fxnp * areao) / (fxop * arean));
+            if (psidn == 0.0) p = 0.0;
+            /* min(1, p); a NaN p rejects, as the reference's dmin1 (minsd) keeps it */
+            p = p < 1.0 ? p : (p >= 1.0 ? 1.0 : 0.0);
+        }
+        if (orc_random_dp(w->rng) < p) {
+            w->psi2o = psi2n;
+            w->psido = psidn; w->psijo = psijn;
+            orc_accept(s, w, i);
+            acc++;
+        } else if (!geo && ok && okp) {
+            orc_reject(s, w, i);
+        }
+    }
+    double psid, psij, ekin, energy;
+    orc_hpsi(s, w, &psid, &psij, &ekin, &energy);
+    return acc;
+}
+
 void orc_walker_scalars(const oracle_walker_t *w, double out[4])
 {
     out[0] = w->psido; out[1] = w->psijo; out[2] = w->eold; out[3] = w->psi2o;

@@ -65,6 +65,10 @@ void   orc_accept(const champ_system_t *s, oracle_walker_t *w, int iel);
 void   orc_reject(const champ_system_t *s, oracle_walker_t *w, int iel);
 /* one metrop1 sweep (rotqua first when nloc>0); returns accepted moves */
 int    orc_metrop1_sweep(const champ_system_t *s, oracle_walker_t *w, double tau);
+/* one metrop6 sweep (accelerated Metropolis, metrop_mov1_slat.f90), then hpsi */
+int    orc_metrop6_sweep(const champ_system_t *s, oracle_walker_t *w, double deltar, double deltat, double fbias);
+double orc_gammai(double a, double x, double xae, int *iflag);
+double orc_m6_area(double zeta, double co, double rbot, double rtop, double bot);
 void   orc_walker_scalars(const oracle_walker_t *w, double out[4]); /* psido psijo eold psi2o */
 
 /* state getters for checking */

@@ -47,6 +47,12 @@ def lib():
         L.orc_reject.argtypes = [vp, vp, C.c_int]
         L.orc_metrop1_sweep.argtypes = [vp, vp, C.c_double]
         L.orc_metrop1_sweep.restype = C.c_int
+        L.orc_metrop6_sweep.argtypes = [vp, vp, C.c_double, C.c_double, C.c_double]
+        L.orc_metrop6_sweep.restype = C.c_int
+        L.orc_gammai.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int)]
+        L.orc_gammai.restype = C.c_double
+        L.orc_m6_area.argtypes = [C.c_double] * 5
+        L.orc_m6_area.restype = C.c_double
         L.orc_walker_scalars.argtypes = [vp, dp]
         for f in ("orc_get_vj", "orc_get_vjn", "orc_get_slmi", "orc_get_detiab", "orc_get_orbn", "orc_get_parts"):
             getattr(L, f).argtypes = [vp, dp]
@@ -185,6 +191,10 @@ class Walker:
 
     def sweep(self, tau):
         return self.L.orc_metrop1_sweep(C.byref(self.csys), self.ptr, float(tau))
+
+    def sweep6(self, deltar=5.0, deltat=1.0, fbias=1.0):
+        """one metrop6 sweep (CHAMP's defaults deltar 5, deltat 1, fbias 1)"""
+        return self.L.orc_metrop6_sweep(C.byref(self.csys), self.ptr, float(deltar), float(deltat), float(fbias))
 
     def scalars(self):
         o = np.zeros(4)

@@ -110,7 +110,7 @@ def test_numerical_radial_tables(h2):
             assert abs(wfv[0] - h2.rwf[0, irb, i]) <= 1e-10 * max(1.0, abs(h2.rwf[0, irb, i])), (irb, i)
 
 
-def _oracle_vmc(sy, nw, neq, nsw, tau):
+def _oracle_vmc(sy, nw, neq, nsw, tau, imetro=1):
     cs = sy.as_ctypes()
     x = sy.initial_walkers(nw, seed=1)
     st = ffi.seeded_streams(S.seed_from_string(), nw)
@@ -120,11 +120,12 @@ def _oracle_vmc(sy, nw, neq, nsw, tau):
         wk.set_x(x[w])
         wk.set_rng(st[w])
         wk.hpsi()
+        sweep = (lambda: wk.sweep(tau)) if imetro == 1 else wk.sweep6
         for _ in range(neq):
-            wk.sweep(tau)
+            sweep()
         es = np.zeros(nsw)
         for k in range(nsw):
-            wk.sweep(tau)
+            sweep()
             es[k] = wk.scalars()["eold"]
         return es
 
@@ -144,4 +145,16 @@ def test_oracle_vmc_energy_matches_reference_ci(name, nw, neq, nsw, tau, h2, c4h
     e, err = _oracle_vmc(sy, nw, neq, nsw, tau)
     eref, eref_err = reference_energy(name)
     print(f"[{name}] oracle {e:.5f} +- {err:.5f}; reference {eref} +- {eref_err}")
+    assert abs(e - eref) <= 2.0 * np.hypot(err, eref_err)
+
+
+@pytest.mark.parametrize("name,nw,neq,nsw", [("H2", 64, 100, 500), ("C4H6_ci44", 32, 40, 100)])
+def test_oracle_metrop6_energy_matches_reference_ci(name, nw, neq, nsw, h2, c4h6):
+    """The same with the oracle's metrop6 (metrop_mov1_slat.f90, CHAMP's
+    default mover imetro=6 at its default deltar 5, deltat 1, fbias 1): the
+    mover these CI energies were produced with."""
+    sy = h2 if name == "H2" else c4h6
+    e, err = _oracle_vmc(sy, nw, neq, nsw, 0.0, imetro=6)
+    eref, eref_err = reference_energy(name)
+    print(f"[{name} metrop6] oracle {e:.5f} +- {err:.5f}; reference {eref} +- {eref_err}")
     assert abs(e - eref) <= 2.0 * np.hypot(err, eref_err)

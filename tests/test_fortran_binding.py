@@ -71,7 +71,9 @@ def test_fortran_driver_builds(tmp_path):
 def test_fortran_driver_matches_python_driver(require_gpu, tmp_path):
     """vmc_gpu (Fortran, through champ_gpu_mod) and Engine (Python, ctypes)
     on the same system file, walkers and seed: the block energies of their
-    metrop1 sweeps agree to the printed 17 digits."""
+    metrop1 sweeps agree to 1e-13 relative (the walkers follow the same
+    chains; the population's energy sum is accumulated with atomics, whose
+    order, and so the last bits of the sum, vary from run to run)."""
     if os.environ.get("CHAMP_ENGINE_LIB"):
         pytest.skip("the Fortran driver links the GPU library itself")
     from champ_amd.engine import Engine
@@ -95,4 +97,4 @@ def test_fortran_driver_matches_python_driver(require_gpu, tmp_path):
         e.hpsi()
         ep = [e.metrop1(nstep, tau)["esum"] / (nwalk * nstep) for _ in range(nblk)]
         print(f"[fortran driver] {name} fortran {ef} python {ep}", flush=True)
-        np.testing.assert_allclose(ef, ep, rtol=1e-15, atol=0)
+        np.testing.assert_allclose(ef, ep, rtol=1e-13, atol=0)

@@ -96,6 +96,33 @@ def test_ci_case_vmc_energy(require_gpu, name):
     assert d <= res[PREC_FP64]["eerr"]
 
 
+@pytest.mark.parametrize("name", ["H2", "C4H6_ci44"])
+def test_ci_case_vmc_energy_metrop6(require_gpu, name):
+    """The same CI cases sampled with CHAMP's default mover, metrop6 at its
+    default deltar 5, deltat 1, fbias 1 (parser.f90:398-404): the mover the
+    reference energies were produced with.  FP64 and mixed paths."""
+    sy = _case(name)
+    W, neq, nblk, nstep, _ = VMC_RUN[name]
+    eref, err_ref = reference_energy(name)
+    res = {}
+    for prec in (PREC_FP64, PREC_MIXED):
+        e = Engine(sy, W, precision=prec, lib_path=ENGINE_LIB)
+        e.set_walkers(sy.initial_walkers(W, seed=3))
+        e.seed_rng(S.seed_from_string())
+        e.hpsi()
+        r = e.vmc(nblk, nstep, nblkeq=neq, imetro=6)
+        res[prec] = r
+        print(f"[vmc6 {name} {'fp64' if prec == PREC_FP64 else 'mixed'}] E = {r['efin']:.6f} +- {r['eerr']:.6f} "
+              f"(sigma {r['sigma']:.4f}, Tcorr {r['tcorr']:.2f}, acceptance {r['accept']:.3f}); "
+              f"reference {eref} +- {err_ref}", flush=True)
+        assert r["nflagged"] == 0 and r["nsamples"] == W * nblk * nstep
+        assert abs(r["efin"] - eref) <= 2.0 * np.hypot(r["eerr"], err_ref)
+        e.close()
+    d = abs(res[PREC_MIXED]["efin"] - res[PREC_FP64]["efin"])
+    print(f"[vmc6 {name}] mixed - fp64 = {d:.2e} (one standard error {res[PREC_FP64]['eerr']:.2e})", flush=True)
+    assert d <= res[PREC_FP64]["eerr"]
+
+
 def test_vmc_block_estimators(require_gpu):
     """champ_gpu_vmc's estimators from its own block averages: efin is their
     mean (equal block weights) and eerr = sqrt(var_blocks / nblk), the
@@ -109,5 +136,5 @@ def test_vmc_block_estimators(require_gpu):
     b = r["enow"]
     assert abs(r["efin"] - b.mean()) <= 1e-12 * max(1.0, abs(b.mean()))
     assert abs(r["eerr"] - np.sqrt(np.mean(b * b) - b.mean() ** 2) / np.sqrt(len(b))) <= 1e-9 * r["eerr"]
-    with pytest.raises(RuntimeError, match="metrop6"):
-        e.vmc(1, 1, imetro=6)
+    r6 = e.vmc(4, 3, nblkeq=1, imetro=6)
+    assert r6["nsamples"] == 64 * 4 * 3 and 0.0 < r6["accept"] < 1.0

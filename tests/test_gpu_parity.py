@@ -144,6 +144,40 @@ def test_metrop1_trajectory(require_gpu, name):
         assert np.array_equal(rg[w], pr.orc[w].get_rng())
 
 
+@pytest.mark.parametrize("name", ["small", "small_ae", "small_md"])
+def test_metrop6_trajectory(require_gpu, name):
+    """Two metrop6 sweeps (CHAMP's accelerated mover, metrop_mov1_slat.f90)
+    from the same seeds, at its default parameters and at others: the GPU
+    population follows the oracle chains draw for draw (equal RNG states and
+    acceptance counts), through the rejection samplings of r and phi, the
+    geometric rejections and the reverse-move densities."""
+    from champ_amd import system as S
+    from oracle import ffi
+    for deltar, deltat, fbias in ((5.0, 1.0, 1.0), (3.0, 1.6, 0.7)):
+        pr = Pair(name, 6, seed=9)
+        streams = ffi.seeded_streams(S.seed_from_string(), pr.nwalk)
+        pr.eng.set_rng(streams)
+        for w in range(pr.nwalk):
+            pr.orc[w].set_rng(streams[w])
+        _hpsi_both(pr)
+        pr.eng.set_metrop6(deltar, deltat, fbias)
+        st = pr.eng.metrop6(2)
+        acc = 0
+        for w in pr.orc:
+            acc += w.sweep6(deltar, deltat, fbias)
+            acc += w.sweep6(deltar, deltat, fbias)
+        print(f"[metrop6 {name} {deltar} {deltat} {fbias}] accepted {st['accepted']} of {st['proposed']}", flush=True)
+        assert st["accepted"] == acc and st["nflagged"] == 0
+        xg = pr.eng.get_walkers()
+        sc = pr.eng.scalars()
+        for w in range(pr.nwalk):
+            assert np.max(np.abs(xg[w] - pr.orc[w].get_x())) < 1e-9
+            assert abs(sc["eold"][w] - pr.orc[w].scalars()["eold"]) < 1e-9 * max(1.0, abs(sc["eold"][w]))
+        rg = pr.eng.get_rng()
+        for w in range(pr.nwalk):
+            assert np.array_equal(rg[w], pr.orc[w].get_rng())
+
+
 @pytest.mark.parametrize("name,nwalk", [("small", 4), ("c22_md", 3)])
 def test_autotune_keeps_parity(require_gpu, name, nwalk):
     """The autotuner's kernel choices (GEMM kinds, quadrature form) leave hpsi

@@ -629,3 +629,45 @@ def test_een_coefficient_order_one_hot():
         w.set_x(x)
         psij = w.hpsi()["psij"]
         assert abs(psij - 0.7 * u ** k * (xa ** l + ya ** l) * (xa * ya) ** m) < 1e-14, (t, (k, l, m))
+
+
+# ----------------------------------------------------------------- metrop6
+@pytest.mark.parametrize("a", [1.5, 2.5])
+def test_gammai_matches_scipy(a):
+    """gammai (gammai.f90): the lower incomplete gamma function for x < a+4
+    (iflag 1) and minus the upper one beyond (iflag -1), with its fixed term
+    counts, against scipy's regularised functions times Gamma(a)."""
+    from scipy import special
+    L = ffi.lib()
+    fl = C.c_int()
+    G = special.gamma(a)
+    for x in np.concatenate([np.linspace(1e-3, 40.0, 401), [a + 4 - 1e-9, a + 4, 60.0, 100.0]]):
+        g = L.orc_gammai(a, float(x), float(x ** a * np.exp(-x)), C.byref(fl))
+        ref = special.gammainc(a, x) * G if x < a + 4 else -special.gammaincc(a, x) * G
+        assert fl.value == (1 if x < a + 4 else -1)
+        assert abs(g - ref) <= 1e-12 * abs(ref), (a, x, g, ref)
+
+
+def test_metrop6_radial_area_matches_quadrature():
+    """The normalisation metrop6 forms from incomplete gamma functions
+    (metrop_mov1_slat.f90:256-293) equals the integral of
+    sqrt(r)|1 + co r| exp(-zeta r) over [rmin/5, 5 rmin] by adaptive
+    quadrature, divided by the density at rmin; cases with the zero of
+    1 + co r inside and outside the interval."""
+    from scipy import integrate
+    L = ffi.lib()
+    rng = np.random.default_rng(1)
+    inside = 0
+    for _ in range(300):
+        zeta, co, rmin = rng.uniform(0.05, 6.0), rng.uniform(-3.0, 3.0), rng.uniform(0.05, 4.0)
+        rb, rt = rmin / 5, rmin * 5
+        bot = math.sqrt(rmin) * abs(1 + co * rmin) * math.exp(-zeta * rmin)
+        if bot == 0.0:
+            continue
+        pts = [-1 / co] if rb < -1 / co < rt else None
+        inside += pts is not None
+        v, _ = integrate.quad(lambda r: math.sqrt(r) * abs(1 + co * r) * math.exp(-zeta * r), rb, rt, points=pts,
+                              epsabs=0, epsrel=1e-13, limit=200)
+        a = L.orc_m6_area(zeta, co, rb, rt, bot)
+        assert abs(a - v / bot) <= 1e-12 * (v / bot), (zeta, co, rmin)
+    assert inside > 20
