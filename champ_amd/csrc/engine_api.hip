@@ -122,6 +122,35 @@ using namespace champ;
 
 extern "C" const char *champ_gpu_last_error(void) { return g_err.c_str(); }
 
+// ------------------------------------------------------------------------
+// Static scratch of champ_gpu_basis_fns_noalloc: the staged positions and AO
+// values of one call, 16 MiB per device, held until the process ends.  In
+// host code `scratch` names a host-side stand-in, not device memory: its
+// address on each device comes from hipGetSymbolAddress, taken once per
+// device at champ_gpu_init with that device current.
+constexpr size_t NOALLOC_DOUBLES = (size_t)2 << 20;  // 16 MiB
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+constexpr int NOALLOC_MAXDEV = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[NOALLOC_MAXDEV];
+
+static int noalloc_scratch(int dev, double **out)
+{
+    if (dev < 0 || dev >= NOALLOC_MAXDEV) { set_error("init: device index beyond the scratch table"); return CHAMP_ERR_UNSUPPORTED; }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("init: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return CHAMP_ERR_HIP;
+        }
+        g_scratch_dev[dev] = (double *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return CHAMP_OK;
+}
+
 extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t nwalk, int32_t precision)
 {
     if (!ctxp || !s || nwalk <= 0) { set_error("init: bad arguments"); return CHAMP_ERR_ARG; }
@@ -148,6 +177,8 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
     if (hipGetDevice(&dev) != hipSuccess) { set_error("init: no HIP device"); return CHAMP_ERR_HIP; }
     champ_ctx *c = new champ_ctx();
     *ctxp = c;
+    c->dev = dev;
+    if (int rc = noalloc_scratch(dev, &c->scratch)) { champ_gpu_free(c); *ctxp = nullptr; return rc; }
     c->W = nwalk; c->NE = s->nelec; c->NUP = s->nup; c->NDN = s->nelec - s->nup;
     c->NM = std::max(c->NUP, c->NDN); c->NC = s->ncent; c->NCT = s->nctype; c->NB = s->nbasis; c->NO = s->norb;
     c->NQ = s->nloc ? s->nquad : 0; c->nloc = s->nloc; c->prec = precision; c->cdet0 = s->cdet[0];
@@ -977,6 +1008,25 @@ extern "C" int champ_gpu_basis_fns(champ_ctx *c, int32_t npt, const double *pos,
     hipFree(dpos);
     hipFree(dout);
     return rc;
+}
+
+extern "C" int champ_gpu_basis_fns_noalloc(champ_ctx *c, int32_t npt, const double *pos, double *phin)
+{
+    if (!c || npt <= 0 || !pos || !phin) return CHAMP_ERR_ARG;
+    const size_t npos = ((size_t)npt * 3 + 31) / 32 * 32;  // the values start 256-byte aligned
+    if (npos + (size_t)npt * 5 * c->NB > NOALLOC_DOUBLES) {
+        set_error("basis_fns_noalloc: npt * (3 + 5 nbasis) doubles exceed the 16 MiB scratch (at most " +
+                  std::to_string((NOALLOC_DOUBLES - 32) / (3 + 5 * (size_t)c->NB)) + " points here)");
+        return CHAMP_ERR_ARG;
+    }
+    if (!c->scratch) { set_error("basis_fns_noalloc: no scratch address for this context's device"); return CHAMP_ERR_STATE; }
+    double *dpos = c->scratch, *dout = c->scratch + npos;
+    HIPCHECK(hipMemcpyAsync(dpos, pos, (size_t)npt * 3 * 8, hipMemcpyHostToDevice, c->stream));
+    int rc = launch_basis_points(c, npt, dpos, dout);
+    if (rc) return rc;
+    HIPCHECK(hipMemcpyAsync(phin, dout, (size_t)npt * 5 * c->NB * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return CHAMP_OK;
 }
 
 static const char *kclass_name[K_NCLASS] = {"move_jastrow_ao", "move_mo_gemm", "move_ratio_commit", "hpsi_jastrow",

@@ -41,6 +41,8 @@ struct champ_ctx {
     int imetro = 1;    // mover of the sweep kernels: 1 metrop1 (drift-diffusion), 6 metrop6 (accelerated)
     double m6_deltar = 5.0, m6_deltat = 1.0, m6_fbias = 1.0;  // metrop6 parameters (parser.f90:402-404)
     hipStream_t stream = nullptr;
+    int dev = 0;                 // device the context was created on
+    double *scratch = nullptr;   // that device's address of the static scratch of champ_gpu_basis_fns_noalloc
 
     // ---- tables
     double *cent = nullptr;          // [NC][3]

@@ -80,6 +80,7 @@ def load_library(path: str = LIBPATH):
     L.champ_gpu_array_size.restype = C.c_int64
     L.champ_gpu_get_array.argtypes = [vp, i32, dp]
     L.champ_gpu_basis_fns.argtypes = [vp, i32, dp, dp]
+    L.champ_gpu_basis_fns_noalloc.argtypes = [vp, i32, dp, dp]
     L.champ_gpu_kernel_times.argtypes = [vp, C.POINTER(i32), C.c_char_p, dp, C.POINTER(C.c_int64)]
     L.champ_gpu_set_graphs.argtypes = [vp, i32]
     L.champ_gpu_autotune.argtypes = [vp, C.POINTER(i32)]
@@ -238,6 +239,13 @@ class Engine:
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         out = np.zeros((pos.shape[0], 5, self.system.nbasis))
         self._check(self.L.champ_gpu_basis_fns(self.ptr, pos.shape[0], _p(pos), _p(out)), "basis_fns")
+        return out
+
+    def basis_fns_noalloc(self, pos: np.ndarray) -> np.ndarray:
+        """basis_fns staged in the library's static device scratch (no device allocation)"""
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros((pos.shape[0], 5, self.system.nbasis))
+        self._check(self.L.champ_gpu_basis_fns_noalloc(self.ptr, pos.shape[0], _p(pos), _p(out)), "basis_fns_noalloc")
         return out
 
     # ---------------------------------------------------------------- timing

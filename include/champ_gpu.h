@@ -310,6 +310,18 @@ int champ_gpu_get_array(champ_ctx *ctx, int32_t which, double *out);
 int champ_gpu_basis_fns(champ_ctx *ctx, int32_t npt, const double *pos,
                         double *phin);
 
+/* champ_gpu_basis_fns without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (tight loops,
+ * code that forbids hipMalloc).  The points and values are staged in a static
+ * 16 MiB device array per GPU (held until the process ends), so npt is
+ * limited to npt * (3 + 5 nbasis) doubles within it (about 690 points at 600
+ * AOs); larger requests return CHAMP_ERR_ARG.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (from several host threads or
+ * contexts) race and must be serialised by the caller.  Same outputs as
+ * champ_gpu_basis_fns.                                                       */
+int champ_gpu_basis_fns_noalloc(champ_ctx *ctx, int32_t npt, const double *pos,
+                                double *phin);
+
 /* Timing of the last metrop1 call: per-kernel-class milliseconds measured
  * with HIP events on the context stream.  names/ms sized by *n on input.      */
 int champ_gpu_kernel_times(champ_ctx *ctx, int32_t *n, char *names /* n*32 */,

@@ -22,6 +22,7 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __align__(n) alignas(n)
 #define HIP_DYNAMIC_SHARED(type, var) type *var = (type *)emu::dyn_shared();
 
 using std::isfinite;
@@ -187,6 +188,9 @@ hipError_t hipStreamDestroy(hipStream_t s);
 hipError_t hipStreamSynchronize(hipStream_t s);
 hipError_t hipDeviceSynchronize();
 hipError_t hipGetDevice(int *d);
+// a __device__ variable is host memory here: its "device address" is its address
+#define HIP_SYMBOL(x) ((const void *)&(x))
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return hipSuccess; }
 hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int dev);
 hipError_t hipFuncSetAttribute(const void *f, hipFuncAttribute a, int v);
 template <typename F>
