@@ -251,6 +251,17 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
         TRY(upload(&c->ri_cent, ric.data(), ric.size())); TRY(upload(&c->ri_type, rit.data(), rit.size()));
         TRY(upload(&c->ri_rad, rir.data(), rir.size())); TRY(upload(&c->ri_rmax, rirm.data(), rirm.size()));
         TRY(upload(&c->ao_ri, aori.data(), NB));
+        // each instance's AOs, and the instances in decreasing order of their AO count
+        std::vector<int> aoff(c->NRI + 1, 0), aol(NB), perm(c->NRI);
+        for (int m = 0; m < NB; m++) aoff[aori[m] + 1]++;
+        for (int r = 0; r < c->NRI; r++) aoff[r + 1] += aoff[r];
+        std::vector<int> fill(aoff.begin(), aoff.end() - 1);
+        for (int m = 0; m < NB; m++) aol[fill[aori[m]]++] = m;
+        for (int r = 0; r < c->NRI; r++) perm[r] = r;
+        std::stable_sort(perm.begin(), perm.end(),
+                         [&](int x, int y) { return aoff[x + 1] - aoff[x] > aoff[y + 1] - aoff[y]; });
+        TRY(upload(&c->ri_perm, perm.data(), c->NRI)); TRY(upload(&c->ri_aoff, aoff.data(), c->NRI + 1));
+        TRY(upload(&c->ri_aol, aol.data(), NB));
     }
     TRY(upload(&c->coef, s->coef, (size_t)NO * NB));
     if (precision == CHAMP_PREC_MIXED) {
@@ -416,7 +427,8 @@ extern "C" int champ_gpu_free(champ_ctx *c)
                     c->vjnT, c->fsumn, c->aomove, c->orbn, c->ratio, c->psidn, c->psijn, c->vnew, c->vold, c->acc,
                     c->esum_dev, c->aoall, c->orball, c->wmat, c->pair_count, c->pair_off, c->pair_total, c->grp_off,
                     c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->rrT, c->nxquad, c->ri_cent, c->ri_type,
-                    c->ri_rad, c->ri_rmax, c->ao_ri, c->gref, c->lu_ws, c->m6rec};
+                    c->ri_rad, c->ri_rmax, c->ao_ri, c->gref, c->lu_ws, c->m6rec, c->ri_perm,
+                    c->ri_aoff, c->ri_aol};
     for (void *p : ptrs) if (p) hipFree(p);
     md_free(c);
     if (c->sweep_exec) hipGraphExecDestroy(c->sweep_exec);

@@ -265,7 +265,9 @@ __global__ void __launch_bounds__(512) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
 // (walker, electron), lanes over AOs.  Chunk of walkers [w0, w0+nw).
 // T: arithmetic type of the radial functions and AO values (double on the
 // FP64 path, float on the mixed path); the AO block is stored as double.
-template <typename T>
+// NCOMP 5: value, gradient and Laplacian; 1: values only (the periodic
+// recompute of the Slater inverse inside a sweep).
+template <typename T, int NCOMP>
 __global__ void __launch_bounds__(256) k_hpsi_ao(HArgs a)
 {
     HIP_DYNAMIC_SHARED(double, lds)
@@ -276,20 +278,29 @@ __global__ void __launch_bounds__(256) k_hpsi_ao(HArgs a)
     const int NE = a.NE, NRI = a.NRI;
     const double *xw = a.xT + (size_t)w * 3 * NE;
     const double *rw = a.r_en + (size_t)w * a.NC * NE;
-    double *out = a.aoall + (size_t)gw * 5 * a.NB;
+    double *out = a.aoall + (size_t)gw * NCOMP * a.NB;
     const double x0 = xw[e], x1 = xw[NE + e], x2 = xw[2 * NE + e];
     // radial functions (R, R', R'') once per (centre, shell), in the wave's LDS
     T *R = (T *)lds + (size_t)wv * 3 * NRI;
     for (int ri = lane; ri < NRI; ri += WAVE) {
         const double r = rw[(size_t)a.ri_cent[ri] * NE + e];
         T wfv[3] = {T(0), T(0), T(0)};
-        if (r <= a.ri_rmax[ri]) splfit_jx<T>(a.tab, a.ri_type[ri], a.ri_rad[ri], (T)r, grid_jx(a.tab, r), wfv, 2);
+        if (r <= a.ri_rmax[ri])
+            splfit_jx<T>(a.tab, a.ri_type[ri], a.ri_rad[ri], (T)r, grid_jx(a.tab, r), wfv, NCOMP == 1 ? 0 : 2);
         R[3 * ri] = wfv[0]; R[3 * ri + 1] = wfv[1]; R[3 * ri + 2] = wfv[2];
     }
     __syncwarp();
     for (int m = lane; m < a.NB; m += WAVE) {
         const int c = a.ao_cent[m];
         const T *Rm = R + 3 * a.ao_ri[m];
+        if constexpr (NCOMP == 1) {
+            T o0 = T(0);
+            if (Rm[0] != T(0))
+                o0 = slm_value<T>(slm_decode(a.ao_slm[m]), (T)(x0 - a.cent[3 * c]), (T)(x1 - a.cent[3 * c + 1]),
+                                  (T)(x2 - a.cent[3 * c + 2])) * Rm[0];
+            out[m] = (double)o0;
+            continue;
+        }
         T o[5] = {T(0), T(0), T(0), T(0), T(0)};
         if (Rm[0] != T(0) || Rm[1] != T(0) || Rm[2] != T(0))  // else beyond the cut radius: exactly zero
             ao_combine<T>(a.ao_slm[m], (T)(x0 - a.cent[3 * c]), (T)(x1 - a.cent[3 * c + 1]),
@@ -323,11 +334,12 @@ __global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
     const int ld = nel + 1;
     double *L = GLB ? a.lu_ws + (size_t)blockIdx.x * a.NM * (a.NM + 1) : lds;
     double *col = GLB ? lds : L + (size_t)nel * ld;
-    const double *ob = a.orball + (size_t)wc * NE * 5 * NO;
+    const int oc = a.ocomp;
+    const double *ob = a.orball + (size_t)wc * NE * oc * NO;
     if (tid == 0) s_bad = 0;
     for (int t = tid; t < nel * nel; t += bd) {
         const int i = t / nel, j = t % nel;
-        L[i * ld + j] = ob[((size_t)(ish + i) * 5) * NO + a.iworbd[ish + j]];
+        L[i * ld + j] = ob[((size_t)(ish + i) * oc) * NO + a.iworbd[ish + j]];
     }
     __syncthreads();
     const int nd = nel > 0 ? nel : 1;  // element stepping of the elimination (no elimination when nel = 0)
@@ -404,6 +416,7 @@ __global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
         if (s_bad) atomicOr(&a.status[w], CHAMP_WALKER_SINGULAR);
         if (nonfinite) atomicOr(&a.status[w], CHAMP_WALKER_NONFINITE);
     }
+    if (oc < 5) return;  // orbital values only: the inverse and determinants are the whole result
     // ddx, d2dx2 (determinant.f90:81-88)
     for (int i = tid; i < nel; i += bd) {
         const double *orbi = ob + (size_t)(ish + i) * 5 * NO;
@@ -598,9 +611,14 @@ __global__ void __launch_bounds__(256) k_hpsi_eloc(HArgs a)
 // walker, in chunks of walkers (orbitals of all electrons are large): the
 // determinant part of hpsi, also used on its own for the periodic FP64
 // refresh of the inverse on the mixed path.
-static int launch_orbitals_slater(champ_ctx *c, HArgs a)
+// values_only: orbital values alone, and from them the inverse and the spin
+// determinants (the periodic recompute inside a sweep of a single-determinant
+// trial function); otherwise also gradients and Laplacians, ddx, d2dx2, the
+// orbital gradients of the moves and the multideterminant state.
+static int launch_orbitals_slater(champ_ctx *c, HArgs a, bool values_only = false)
 {
     int slot;
+    a.ocomp = values_only ? 1 : 5;
     for (int w0 = 0; w0 < c->W; w0 += c->hpsi_chunk) {
         a.w0 = w0;
         a.nw = std::min(c->hpsi_chunk, c->W - w0);
@@ -610,12 +628,16 @@ static int launch_orbitals_slater(champ_ctx *c, HArgs a)
         // periodic refresh recompute from scratch, and the Slater matrices of a
         // trial function can be ill-conditioned enough to turn FP32 AO rounding
         // into 1e-3 relative errors of the determinant
-        hipLaunchKernelGGL(k_hpsi_ao<double>, dim3((unsigned)((nwave + 3) / 4)), dim3(256),
-                           (size_t)4 * 3 * c->NRI * sizeof(double), c->stream, a);
+        if (values_only)
+            hipLaunchKernelGGL((k_hpsi_ao<double, 1>), dim3((unsigned)((nwave + 3) / 4)), dim3(256),
+                               (size_t)4 * 3 * c->NRI * sizeof(double), c->stream, a);
+        else
+            hipLaunchKernelGGL((k_hpsi_ao<double, 5>), dim3((unsigned)((nwave + 3) / 4)), dim3(256),
+                               (size_t)4 * 3 * c->NRI * sizeof(double), c->stream, a);
         tend(c, slot);
         HIPCHECK(hipGetLastError());
         tbegin(c, K_HPSI_GEMM, &slot);
-        int rcg = launch_gemm(c, true, (int)(nwave * 5), c->NO, c->NB, c->aoall, c->NB, 0, c->coef, c->NB, 0,
+        int rcg = launch_gemm(c, true, (int)(nwave * a.ocomp), c->NO, c->NB, c->aoall, c->NB, 0, c->coef, c->NB, 0,
                               c->orball, c->NO, 0, 1, c->tune_gemm[1]);
         tend(c, slot);
         if (rcg) return rcg;
@@ -635,7 +657,7 @@ static int launch_orbitals_slater(champ_ctx *c, HArgs a)
         }
         tend(c, slot);
         HIPCHECK(hipGetLastError());
-        if (c->md) {
+        if (c->md && !values_only) {
             tbegin(c, K_MD, &slot);
             const int rcm = launch_md_hpsi(c, w0, a.nw);
             tend(c, slot);
@@ -658,7 +680,10 @@ __global__ void k_refresh_psi(HArgs a)
 int launch_slater_refresh(champ_ctx *c)
 {
     HArgs a = hargs(c);
-    const int rc = launch_orbitals_slater(c, a);
+    // single determinant: the inverse and determinants from orbital values
+    // alone (the orbital gradients of the moves are kept as the moves wrote
+    // them); multideterminant: the full recompute, which rebuilds its state
+    const int rc = launch_orbitals_slater(c, a, c->md == 0);
     if (rc) return rc;
     hipLaunchKernelGGL(k_refresh_psi, dim3((c->W + 255) / 256), dim3(256), 0, c->stream, a);
     HIPCHECK(hipGetLastError());

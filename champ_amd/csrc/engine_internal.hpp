@@ -54,6 +54,9 @@ struct champ_ctx {
     int *ri_cent = nullptr, *ri_type = nullptr, *ri_rad = nullptr;  // [NRI]
     double *ri_rmax = nullptr;                                      // [NRI]
     int *ao_ri = nullptr;                                           // [NB] radial instance of each AO
+    int *ri_perm = nullptr;   // [NRI] instances in decreasing order of their AO count
+    int *ri_aoff = nullptr;   // [NRI+1] offsets of each instance's AOs in ri_aol
+    int *ri_aol = nullptr;    // [NB] AOs grouped by radial instance
     double *gref = nullptr;                                         // [nr][2] grid node radii
     int lu_every = 0;  // moves between FP64 recomputes of the Slater inverse inside a sweep (0: only in hpsi)
     int *ao_cent = nullptr, *ao_type = nullptr, *ao_rad = nullptr, *ao_slm = nullptr;  // [NB]

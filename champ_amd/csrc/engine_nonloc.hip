@@ -338,9 +338,11 @@ template <> struct Mma<float> {
     static __device__ __forceinline__ int row(int r, int sl) { return 4 * sl + r; }
 };
 
+// with_ee: the item also adds the e-e terms b(u_qj) of its (point, partner)
+// entries (the first of the two halves of a split tile, or a whole tile)
 template <int N, typename T>
 __device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int c_lo, int c_hi, int npts,
-                                         const double *XP, const T *XQ, const int *IPT, double *swv)
+                                         const double *XP, const T *XQ, const int *IPT, double *swv, bool with_ee)
 {
     constexpr EenTable<N> ET{};
     constexpr int NA = een_naz<N>(), KR = ET.kmax + 1;
@@ -403,11 +405,13 @@ __device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int c_lo
                 T r2 = 0;
                 r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
                 const T rjq = sqrt(r2);
-                const T u = switched_from<T>(kap, rjq, scale_r<T>((T)kap, rjq));
+                const T rr = scale_r<T>((T)kap, rjq);
+                const T u = switched_from<T>(kap, rjq, rr);
                 T pv2 = acc[mt][KR - 1][r];
 #pragma unroll
                 for (int k = KR - 2; k >= 0; k--) pv2 = pv2 * u + acc[mt][k][r];
                 v = (double)pv2;
+                if (with_ee) v += (double)ee_term<T>(jp, a.NUP, IPT[pt], j, rr, u);
             }
             // sum over the 16 partners (lanes of equal l / 16) into the wave's point slot
             v += __shfl_xor(v, 1, WAVE); v += __shfl_xor(v, 2, WAVE);
@@ -416,24 +420,20 @@ __device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int c_lo
         }
 }
 
-// Quadrature points per pass of the psi-ratio phase (a whole pair's 12 by
-// default).  Smaller passes shrink that phase's LDS tables so that more blocks
-// stay resident (6 points: 3 blocks per CU instead of 2 at c160), but the
-// kernel is bound by instruction issue, not by occupancy: 6 and 4 points per
-// pass measured 3% and 7% slower.
-#ifndef CHAMP_MMA_QH
-#define CHAMP_MMA_QH 12
-#endif
-constexpr int MMA_QH = CHAMP_MMA_QH;
-static_assert(MMA_QH >= 1 && MMA_QH <= 12, "psi-ratio pass size");
+// psi-ratio phase: lanes over (point, radial-instance slot), MMA_PS slots
+constexpr int MMA_PS = MMA_THREADS / MMA_MP;
+static_assert(MMA_PS * MMA_MP <= 2 * MMA_TG * MMA_MP, "psi-ratio partials fit the point-phase partials");
 
-// LDS of k_nonloc_mma in doubles: XP, SP, TG partials, SW, FI, IPT/PI (as
-// doubles), XQ (tsize bytes per entry), then GEO + RAD of one psi-ratio pass
-__host__ __device__ __forceinline__ size_t mma_lds_doubles(int NC, int NQ, int NRI, int tsize)
+// LDS of k_nonloc_mma in doubles: XP, SP, TG partials (reused for the
+// psi-ratio partials), SW, FI, IPT/PI (as doubles), XQ (tsize bytes per
+// entry), the point-centre distances (double) and grid indices (int), then
+// the radial instances in slot order (8 ints and the cut radius each) and the
+// AOs in slot order (slm code and AO index)
+__host__ __device__ __forceinline__ size_t mma_lds_doubles(int NC, int NB, int NRI, int tsize)
 {
-    const size_t qh = NQ < MMA_QH ? NQ : MMA_QH;
     return 3 * MMA_MP + 3 * MMA_MP + 2 * MMA_TG * MMA_MP + MMA_WAVES * MMA_MP + MMA_GP + MMA_MP + 4 * MMA_GP +
-           ((size_t)NC * MMA_MP * tsize + 7) / 8 + qh * NC * 5 + qh * NRI;
+           ((size_t)NC * MMA_MP * tsize + 7) / 8 + (size_t)NC * MMA_MP + ((size_t)NC * MMA_MP * 4 + 7) / 8 +
+           (size_t)NRI * 4 + (size_t)NRI + (size_t)NB;
 }
 
 #ifndef CHAMP_MMA_WPE
@@ -460,9 +460,26 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
     int *IPT = (int *)(FI + MMA_GP);                           // [MP] electron of each point
     int *PI = IPT + 2 * MMA_MP;                                // [GP][4]: i, ic
     T *XQ = (T *)(PI + 8 * MMA_GP);                            // [NC][MP] switched point-centre distances
-    const int QH = NQ < MMA_QH ? NQ : MMA_QH;                  // points per psi-ratio pass
-    double *GEO = (double *)PI + 4 * MMA_GP + ((size_t)NC * MMA_MP * sizeof(T) + 7) / 8;  // phase C: [QH][NC][5]
-    double *RAD = GEO + (size_t)QH * NC * 5;                   //          [QH][NRI]
+    double *GR = (double *)PI + 4 * MMA_GP + ((size_t)NC * MMA_MP * sizeof(T) + 7) / 8;  // [MP][NC] point-centre r
+    int *GI = (int *)(GR + (size_t)NC * MMA_MP);               // [MP][NC] grid interval of r
+    double *PS = TP;                                           // [PS][MP] psi-ratio partials
+    // the radial instances in slot order (centre, type, radial function, AO range)
+    // and their AOs, staged once per block: the psi-ratio lanes read them per
+    // (point, instance), so from global memory their latency dominated that phase
+    int *SMI = GI + (size_t)NC * MMA_MP;                       // [NRI][8]
+    double *SRM = (double *)(SMI + (size_t)NRI * 8);           // [NRI] cut radius
+    int *SAO = (int *)(SRM + NRI);                             // [NB][2] slm code, AO index
+    for (int sidx = tid; sidx < NRI; sidx += MMA_THREADS) {
+        const int ri = a.ri_perm[sidx];
+        SMI[8 * sidx] = a.ri_cent[ri]; SMI[8 * sidx + 1] = a.ri_type[ri]; SMI[8 * sidx + 2] = a.ri_rad[ri];
+        SMI[8 * sidx + 3] = a.ri_aoff[ri]; SMI[8 * sidx + 4] = a.ri_aoff[ri + 1];
+        SRM[sidx] = a.ri_rmax[ri];
+    }
+    for (int k = tid; k < NB; k += MMA_THREADS) {
+        const int m = a.ri_aol[k];
+        SAO[2 * k] = a.ao_slm[m];
+        SAO[2 * k + 1] = m;
+    }
 
     const int G = gridDim.x, bk = blockIdx.x;
     const int nx = (G % 8 == 0) ? 8 : 1, gx = G / nx, xcd = bk % nx, g0 = bk / nx;
@@ -527,7 +544,9 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
                     }
                     XQ[(size_t)b * MMA_MP + pt] = xq;
                 }
-                if (live)
+                // with the e-e-n terms on the matrix cores (NORDC > 1) the e-e
+                // terms are added in their epilogue, which forms u_qj anyway
+                if (NORDC <= 1 && live)
                     for (int j = tg; j < NE; j += MMA_TG) {
                         if (j == i) continue;
                         const T d0 = (T)(xw[j] - px), d1 = (T)(xw[NE + j] - py), d2 = (T)(xw[2 * NE + j] - pz);
@@ -565,10 +584,11 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
                 for (int h = wv * per; h < h_hi;) {
                     const int tile = h / 2;
                     if (h % 2 == 0 && h + 1 < h_hi) {
-                        mma_item<NORDC, T>(a, w, tile * 16, 0, NC, npts, XP, XQ, IPT, swv);
+                        mma_item<NORDC, T>(a, w, tile * 16, 0, NC, npts, XP, XQ, IPT, swv, true);
                         h += 2;
                     } else {
-                        mma_item<NORDC, T>(a, w, tile * 16, h % 2 ? ncm : 0, h % 2 ? NC : ncm, npts, XP, XQ, IPT, swv);
+                        mma_item<NORDC, T>(a, w, tile * 16, h % 2 ? ncm : 0, h % 2 ? NC : ncm, npts, XP, XQ, IPT, swv,
+                                           h % 2 == 0);
                         h += 1;
                     }
                     __syncwarp();
@@ -582,66 +602,53 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
             SP[MMA_MP + pt] = v;
         }
 
-        // ---- psi ratio of every point, QH points of one pair per pass
-        for (int pq = 0; pq < np * ((NQ + QH - 1) / QH); pq++) {
-            const int npass = (NQ + QH - 1) / QH, p = pq / npass, q0 = (pq % npass) * QH, nq = min(QH, NQ - q0);
-            const int i = PI[4 * p], ic = PI[4 * p + 1];
-            __syncthreads();  // the previous pass's GEO, RAD and SW are free
-            for (int t = tid; t < nq * NC; t += MMA_THREADS) {
-                const int q = t / NC, c = t % NC;
-                const double *xp = XP + 3 * (p * NQ + q0 + q);
-                const double e0 = xp[0] - a.cent[3 * c], e1 = xp[1] - a.cent[3 * c + 1], e2 = xp[2] - a.cent[3 * c + 2];
-                double r2 = 0;
-                r2 = r2 + e0 * e0; r2 = r2 + e1 * e1; r2 = r2 + e2 * e2;
-                const double rqd = c == ic ? a.r_en[((size_t)w * NC + ic) * NE + i] : sqrt(r2);
-                double *g = GEO + ((size_t)q * NC + c) * 5;
-                g[0] = e0; g[1] = e1; g[2] = e2; g[3] = rqd;
-                g[4] = (double)grid_jx(a.tab, rqd);
-            }
-            __syncthreads();
-            // one radial instance per thread, its table indices read once for all points
-            for (int ri = tid; ri < NRI; ri += MMA_THREADS) {
-                const int cen = a.ri_cent[ri], ty = a.ri_type[ri], rd = a.ri_rad[ri];
-                const double rmax = a.ri_rmax[ri];
-                for (int q = 0; q < nq; q++) {
-                    const double *g = GEO + ((size_t)q * NC + cen) * 5;
+        // ---- psi ratio of every point: sum_m phi_m(x_q) W[i][m].  The point-centre
+        // distances and grid intervals first, then lanes over (point, radial
+        // instance): a lane forms the instance's radial value at its point and
+        // dots the instance's angular factors with the W row.  Instances are
+        // taken in order of their AO count (ri_perm), so the lanes of a wave
+        // mostly run the same number of AOs.
+        __syncthreads();  // SW and the point-phase partials are free
+        for (int t = tid; t < npts * NC; t += MMA_THREADS) {
+            const int q = t / NC, c = t % NC, p = q / NQ;
+            const double *xp = XP + 3 * q;
+            const double e0 = xp[0] - a.cent[3 * c], e1 = xp[1] - a.cent[3 * c + 1], e2 = xp[2] - a.cent[3 * c + 2];
+            double r2 = 0;
+            r2 = r2 + e0 * e0; r2 = r2 + e1 * e1; r2 = r2 + e2 * e2;
+            const double rqd = c == PI[4 * p + 1] ? a.r_en[((size_t)w * NC + c) * NE + PI[4 * p]] : sqrt(r2);
+            GR[t] = rqd;
+            GI[t] = grid_jx(a.tab, rqd);
+        }
+        __syncthreads();
+        {
+            const int q = tid % MMA_MP, slot = tid / MMA_MP;
+            double part = 0.0;
+            if (slot < MMA_PS && q < npts) {
+                const double px = XP[3 * q], py = XP[3 * q + 1], pz = XP[3 * q + 2];
+                const double *Wm = a.wmat + ((size_t)w * NE + IPT[q]) * NB;
+                const double *gr = GR + (size_t)q * NC;
+                const int *gi = GI + (size_t)q * NC;
+                for (int sidx = slot; sidx < NRI; sidx += MMA_PS) {
+                    const int *mi = SMI + 8 * sidx;
+                    const int cen = mi[0];
+                    const double r = gr[cen];
+                    if (!(r <= SRM[sidx])) continue;  // beyond the shell's cut radius: its AOs are exactly zero
                     double wfv[3] = {0.0, 0.0, 0.0};
-                    if (g[3] <= rmax) splfit_jx<double>(a.tab, ty, rd, g[3], (int)g[4], wfv, 0);
-                    RAD[(size_t)q * NRI + ri] = wfv[0];
+                    splfit_jx<double>(a.tab, mi[1], mi[2], r, gi[cen], wfv, 0);
+                    const double d0 = px - a.cent[3 * cen], d1 = py - a.cent[3 * cen + 1], d2 = pz - a.cent[3 * cen + 2];
+                    double sa = 0.0;
+                    for (int k = mi[3]; k < mi[4]; k++)
+                        sa += slm_value<double>(slm_decode(SAO[2 * k]), d0, d1, d2) * Wm[SAO[2 * k + 1]];
+                    part += sa * wfv[0];
                 }
             }
-            __syncthreads();
-            const double *Wm = a.wmat + ((size_t)w * NE + i) * NB;
-            constexpr int QM = MMA_QH;
-            double dr[QM];
-#pragma unroll
-            for (int q = 0; q < QM; q++) dr[q] = 0.0;
-            for (int m = tid; m < NB; m += MMA_THREADS) {
-                const int c = a.ao_cent[m], ri = a.ao_ri[m];
-                const SlmMono mono = slm_decode(a.ao_slm[m]);  // once per AO, branch-free
-                const double wm = Wm[m];
-#pragma unroll
-                for (int q = 0; q < QM; q++) {
-                    if (q >= nq) break;
-                    const double rad = RAD[(size_t)q * NRI + ri];
-                    if (rad == 0.0) continue;  // beyond the shell's cut radius: the AO is exactly zero
-                    const double *g = GEO + ((size_t)q * NC + c) * 5;
-                    const double y = slm_value<double>(mono, g[0], g[1], g[2]);
-                    dr[q] += (y * rad) * wm;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < QM; q++) {
-                if (q >= nq) break;
-                const double v = wave_sum(dr[q]);
-                if (lane == 0) SW[wv * MMA_MP + q] = v;
-            }
-            __syncthreads();
-            for (int q = tid; q < nq; q += MMA_THREADS) {
-                double v = 0.0;
-                for (int k = 0; k < MMA_WAVES; k++) v += SW[k * MMA_MP + q];
-                SP[2 * MMA_MP + p * NQ + q0 + q] = v;
-            }
+            PS[tid] = part;
+        }
+        __syncthreads();
+        for (int q = tid; q < npts; q += MMA_THREADS) {
+            double v = 0.0;
+            for (int k = 0; k < MMA_PS; k++) v += PS[k * MMA_MP + q];
+            SP[2 * MMA_MP + q] = v;
         }
         __syncthreads();
 
@@ -674,7 +681,7 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
 template <int NORDC, typename T>
 static int launch_mma(champ_ctx *c, const HArgs &a)
 {
-    const size_t lds = mma_lds_doubles(c->NC, c->NQ, c->NRI, (int)sizeof(T)) * sizeof(double);
+    const size_t lds = mma_lds_doubles(c->NC, c->NB, c->NRI, (int)sizeof(T)) * sizeof(double);
     if (lds > 160 * 1024) {
         set_error("nonloc: the quadrature kernel's LDS tables exceed 160 KB for this system");
         return CHAMP_ERR_UNSUPPORTED;
@@ -739,7 +746,7 @@ int launch_nonloc(champ_ctx *c, const HArgs &a)
     // its LDS within 160 KB
     const bool mixed = c->prec == CHAMP_PREC_MIXED;
     const bool mma = CHAMP_NL_MMA && c->tune_nonloc != 1 && c->NQ <= 12 && c->hjas.nordc >= 2 && c->hjas.nordc <= 5 &&
-                     mma_lds_doubles(c->NC, c->NQ, c->NRI, mixed ? 4 : 8) * 8 <= 160 * 1024;
+                     mma_lds_doubles(c->NC, c->NB, c->NRI, mixed ? 4 : 8) * 8 <= 160 * 1024;
     if (mma && mixed) {
         switch (c->hjas.nordc) {
         case 2: return launch_mma<2, float>(c, a);

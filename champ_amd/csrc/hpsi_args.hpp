@@ -27,12 +27,14 @@ struct HArgs {
     int NRI;      // radial instances (centre, radial function)
     const int *ri_cent, *ri_type, *ri_rad, *ao_ri;
     const double *ri_rmax;
+    const int *ri_perm, *ri_aoff, *ri_aol;  // instances by AO count; each instance's AOs (CSR)
     double *xq, *yq, *zq;
     double *esum_dev;
     int *status;
     int md;                 // multideterminant: psi_D = D_up,ref D_dn,ref md_S
     const double *md_S;     // [W]
     double *lu_ws;          // [2 hpsi_chunk][NM (NM+1)] Slater work matrices when they exceed LDS, else null
+    int ocomp;              // components per electron in aoall / orball: 5 (value, gradient, Laplacian) or 1
 };
 
 inline HArgs hargs(champ_ctx *c)
@@ -53,9 +55,11 @@ inline HArgs hargs(champ_ctx *c)
     a.rrT = c->rrT;
     a.NRI = c->NRI; a.ri_cent = c->ri_cent; a.ri_type = c->ri_type; a.ri_rad = c->ri_rad; a.ao_ri = c->ao_ri;
     a.ri_rmax = c->ri_rmax;
+    a.ri_perm = c->ri_perm; a.ri_aoff = c->ri_aoff; a.ri_aol = c->ri_aol;
     a.xq = c->xq; a.yq = c->yq; a.zq = c->zq; a.esum_dev = c->esum_dev; a.status = c->status;
     a.md = c->md; a.md_S = c->md_S;
     a.lu_ws = c->lu_ws;
+    a.ocomp = 5;
     return a;
 }
 
