@@ -431,6 +431,7 @@ extern "C" int champ_gpu_free(champ_ctx *c)
                     c->ri_aoff, c->ri_aol};
     for (void *p : ptrs) if (p) hipFree(p);
     md_free(c);
+    dmc_free(c);
     if (c->sweep_exec) hipGraphExecDestroy(c->sweep_exec);
     for (auto &t : c->ev_pool) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     if (c->stream) hipStreamDestroy(c->stream);
@@ -610,7 +611,7 @@ extern "C" int champ_gpu_accept(champ_ctx *c, int32_t iel, const uint8_t *accept
 
 // One single-electron move of every walker (drift, proposal, ratio, decision,
 // commit) and the periodic refresh of the Slater inverses.
-static int launch_move_step(champ_ctx *c, int iel)
+int launch_move_step(champ_ctx *c, int iel)
 {
     int rc;
     if ((rc = launch_md_pre_timed(c, iel))) return rc;

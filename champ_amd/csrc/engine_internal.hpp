@@ -11,6 +11,8 @@
 
 namespace champ {
 
+struct DmcState;  // host side of a DMC population (engine_dmc.hip)
+
 struct EcpParams {
     int lpot[CHAMP_MAXCT];
     int nterm[CHAMP_MAXCT][CHAMP_MPS_L];
@@ -105,6 +107,18 @@ struct champ_ctx {
     double *orbn = nullptr;    // [W][4][NO]
     double *ratio = nullptr, *psidn = nullptr, *psijn = nullptr, *vnew = nullptr /*[W][3]*/, *vold = nullptr;
     double *m6rec = nullptr;   // [W][M6_REC] metrop6 proposal record
+
+    // ---- DMC (dmc_ps): per-walker state of the population (allocated by
+    // champ_gpu_dmc_init; W is the capacity, nwalk the live walkers, which
+    // occupy slots 0 .. nwalk-1) and the host-side population control
+    double *dmc_acc = nullptr;   // [W][DMC_ACC] move accumulators of the current step
+    int *dmc_age = nullptr;      // [W] steps since the walker last moved (iage)
+    double *dmc_wt = nullptr;    // [W] weights
+    double *dmc_fratio = nullptr;  // [W] velocity ratio of the last step (fratio)
+    double *dmc_eprev = nullptr; // [W] local energy of the previous step
+    double *dmc_out = nullptr;   // [W][DMC_OUT] per-walker terms of the step's estimators
+    int dmc_on = 0, dmc_W = 0;   // DMC initialised; capacity (the context's walker count at init)
+    champ::DmcState *dmc = nullptr;
     unsigned char *acc = nullptr;
     double *esum_dev = nullptr;  // [4] accepted, esum, esum2, nflag
 
@@ -163,6 +177,10 @@ struct champ_ctx {
 };
 
 namespace champ {
+// DMC move accumulators per walker (engine_kernels.hip k_move_b, engine_dmc.hip)
+enum { DMC_PP = 0, DMC_DFUS2AC, DMC_DFUS2UN, DMC_PSUM, DMC_NODECR, DMC_IACCEPT, DMC_ACC = 8 };
+// per-walker estimator terms of a DMC step
+enum { DMC_O_WT = 0, DMC_O_WE, DMC_O_PE, DMC_O_TPB, DMC_O_TAU, DMC_O_PSUM, DMC_O_NODECR, DMC_OUT = 8 };
 // kernel launchers (engine_kernels.hip)
 int launch_move_a(champ_ctx *c, int iel, int mode);
 int launch_move_gemm(champ_ctx *c);
@@ -182,6 +200,7 @@ int launch_wmat(champ_ctx *c);
 // FP32 A [M][K] times FP32 B^T (B stored [N][K]) into FP64 C (matrix cores)
 int launch_gemm_f32(champ_ctx *c, int M, int N, int K, const float *A, int lda, const float *B, int ldb, double *Cm,
                     int ldc);
+void dmc_free(champ_ctx *c);
 int md_init(champ_ctx *c, const champ_system_t *s);
 void md_free(champ_ctx *c);
 int launch_md_hpsi(champ_ctx *c, int w0, int nw);

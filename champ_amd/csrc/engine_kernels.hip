@@ -39,8 +39,10 @@ struct MoveArgs {
     double *md_S, *md_Sn;              // [W] psi_D / (D_up,ref D_dn,ref), current / proposed
     int *md_cur;                       // [W][2] current buffer of the multideterminant state
     int ao32;                          // AO block stored as FP32 for the FP32 contraction
-    int imetro;                        // mode-0 mover: 1 metrop1, 6 metrop6
+    int imetro;                        // mode-0 mover: 1 metrop1, 6 metrop6, 2 DMC (dmc_ps)
     M6Params m6p;
+    double *dmc_acc;                   // [W][DMC_ACC] DMC move accumulators of the current step
+    int *dmc_age;                      // [W] steps since the walker last moved
     double *m6rec;                     // [W][M6_REC]
     const EcpParams *ecp;              // znuc per type and nloc (metrop6's cusp and theta range)
 };
@@ -69,6 +71,8 @@ static MoveArgs move_args(champ_ctx *c, int iel, int mode, double tau)
     a.m6p = M6Params{c->m6_deltar, c->m6_deltat, c->m6_fbias};
     a.m6rec = c->m6rec;
     a.ecp = c->ecp;
+    a.dmc_acc = c->dmc_acc;
+    a.dmc_age = c->dmc_age;
     return a;
 }
 
@@ -182,6 +186,26 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                            a.m6rec + (size_t)w * M6_REC);
                 for (int k = 0; k < 4; k++) a.rng[4 * (size_t)w + k] = s[k];
                 for (int k = 0; k < 3; k++) { a.xnew[3 * w + k] = xn[k]; a.vold[3 * w + k] = vo[k]; }
+                xn0 = xn[0]; xn1 = xn[1]; xn2 = xn[2];
+            } else if (lane == 0 && a.imetro == 2) {
+                // dmc_ps drift-diffusion proposal (dmc_ps_mov1.f90:291-303): the
+                // drift is vavvt v, without metrop1's extra factor tau
+                const double tau = a.tau, rttau = sqrt(tau), adrift = 0.5;
+                uint64_t s[4];
+                for (int k = 0; k < 4; k++) s[k] = a.rng[4 * (size_t)w + k];
+                const double v2old = vo[0] * vo[0] + vo[1] * vo[1] + vo[2] * vo[2];
+                const double vavvt = (sqrt(1.0 + 2.0 * adrift * v2old * tau) - 1.0) / (adrift * v2old);
+                double dfus2o = 0.0, xn[3];
+                for (int k = 0; k < 3; k++) {
+                    const double drift = vavvt * vo[k];
+                    const double dfus = xo_gauss(s) * rttau;
+                    const double dx = drift + dfus;
+                    dfus2o = dfus2o + dfus * dfus;
+                    xn[k] = xw[k * NE + iel] + dx;
+                }
+                for (int k = 0; k < 4; k++) a.rng[4 * (size_t)w + k] = s[k];
+                for (int k = 0; k < 3; k++) { a.xnew[3 * w + k] = xn[k]; a.vold[3 * w + k] = vo[k]; }
+                a.dfus2o[w] = dfus2o;
                 xn0 = xn[0]; xn1 = xn[1]; xn2 = xn[2];
             } else if (lane == 0) {
                 const double tau = a.tau, rttau = sqrt(tau), adrift = 0.5;
@@ -532,6 +556,45 @@ __global__ void __launch_bounds__(256) k_move_b(MoveArgs a)
                 }
                 // psi is evaluated at proposals that pass the geometric test only
                 if (rt == 0.0 && rec[M6_LIVE] != 0.0) atomicOr(&a.status[w], CHAMP_WALKER_ZERO_RATIO);
+            } else if (a.mode == 0 && a.imetro == 2) {
+                // dmc_ps acceptance (dmc_ps_mov1.f90:325-399), ipq = 1, icross = 1,
+                // iacc_rej = 1, node_cutoff = 0: node crossings are counted, not rejected
+                const double tau = a.tau, adrift = 0.5;
+                const double *xw = a.xT + (size_t)w * 3 * NE;
+                double *ac = a.dmc_acc + (size_t)w * DMC_ACC;
+                const double psido = a.psido[w];
+                if (psidn * psido <= 0.0) ac[DMC_NODECR] += 1.0;
+                const double v2new = vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2];
+                const double vavvt = (sqrt(1.0 + 2.0 * adrift * v2new * tau) - 1.0) / (adrift * v2new);
+                double dfus2n = 0.0;
+                for (int k = 0; k < 3; k++) {
+                    const double drift = vavvt * vn[k];
+                    const double xbac = a.xnew[3 * w + k] + drift;
+                    const double dfus = xbac - xw[k * NE + iel];
+                    dfus2n = dfus2n + dfus * dfus;
+                }
+                const double dfus2o = a.dfus2o[w];
+                const double rat = psidn / psido;
+                double p = rat * rat * exp(2 * (psijn - a.psijo[w])) * exp((dfus2o - dfus2n) / (2.0 * tau));
+                const int age = a.dmc_age[w];
+                if (age > 50) p = p * pow(1.1, (double)(age - 50));
+                ac[DMC_PP] = ac[DMC_PP] * p;
+                p = p < 1.0 ? p : (p >= 1.0 ? 1.0 : 0.0);  // dmin1(one, p); a NaN p never passes
+                ac[DMC_PSUM] += p;
+                ac[DMC_DFUS2AC] += p * dfus2o;
+                ac[DMC_DFUS2UN] += dfus2o;
+                uint64_t s[4];
+                for (int k = 0; k < 4; k++) s[k] = a.rng[4 * (size_t)w + k];
+                const double uacc = xo_uniform(s);
+                for (int k = 0; k < 4; k++) a.rng[4 * (size_t)w + k] = s[k];
+                accept = uacc < p;
+                if (accept) {
+                    a.psi2o[w] = 2 * (log(fabs(psidn)) + psijn);
+                    atomicAdd(&a.esum_dev[0], 1.0);
+                    a.dmc_age[w] = 0;
+                    ac[DMC_IACCEPT] = 1.0;
+                }
+                if (rt == 0.0) atomicOr(&a.status[w], CHAMP_WALKER_ZERO_RATIO);
             } else if (a.mode == 0) {
                 const double tau = a.tau, adrift = 0.5;
                 const double psi2n = 2 * (log(fabs(psidn)) + psijn);

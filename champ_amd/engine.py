@@ -40,6 +40,18 @@ class VmcResult(C.Structure):
                 ("accept", C.c_double), ("nsamples", C.c_int64), ("nflagged", C.c_int64)]
 
 
+class DmcStepStats(C.Structure):
+    _fields_ = [("nwalk", C.c_int32), ("pad", C.c_int32), ("wsum", C.c_double), ("esum", C.c_double),
+                ("wgsum", C.c_double), ("egsum", C.c_double), ("eest", C.c_double), ("eigv", C.c_double),
+                ("fprod", C.c_double), ("accept", C.c_double), ("nodecr", C.c_int64), ("nbranch", C.c_int64)]
+
+
+class DmcResult(C.Structure):
+    _fields_ = [("egave", C.c_double), ("egerr", C.c_double), ("eest", C.c_double), ("accept", C.c_double),
+                ("tau_eff", C.c_double), ("nwalk", C.c_int32), ("nwalk_max", C.c_int32), ("nbranch", C.c_int64),
+                ("nodecr", C.c_int64)]
+
+
 class SweepStats(C.Structure):
     _fields_ = [("accepted", C.c_double), ("proposed", C.c_double), ("esum", C.c_double), ("esum2", C.c_double),
                 ("nflagged", C.c_int64)]
@@ -73,6 +85,10 @@ def load_library(path: str = LIBPATH):
     L.champ_gpu_accept.argtypes = [vp, i32, C.POINTER(C.c_uint8)]
     L.champ_gpu_metrop1.argtypes = [vp, i32, C.c_double, C.POINTER(SweepStats)]
     L.champ_gpu_metrop6.argtypes = [vp, i32, C.POINTER(SweepStats)]
+    L.champ_gpu_dmc_init.argtypes = [vp, i32, C.c_double, C.c_double, i32, C.POINTER(C.c_uint64)]
+    L.champ_gpu_dmc_step.argtypes = [vp, C.POINTER(DmcStepStats)]
+    L.champ_gpu_dmc.argtypes = [vp, i32, i32, i32, dp, C.POINTER(DmcResult)]
+    L.champ_gpu_dmc_weights.argtypes = [vp, dp, C.POINTER(i32)]
     L.champ_gpu_set_metrop6.argtypes = [vp, C.c_double, C.c_double, C.c_double]
     L.champ_gpu_get_walker_scalars.argtypes = [vp, dp, dp, dp, dp]
     L.champ_gpu_get_status.argtypes = [vp, C.POINTER(C.c_int32)]
@@ -200,6 +216,34 @@ class Engine:
         st = SweepStats()
         self._check(self.L.champ_gpu_metrop6(self.ptr, int(nsweep), C.byref(st)), "metrop6")
         return dict(accepted=st.accepted, proposed=st.proposed, esum=st.esum, esum2=st.esum2, nflagged=st.nflagged)
+
+    # ---------------------------------------------------------------- DMC
+    def dmc_init(self, nconf: int, tau: float, etrial: float, pop_state, nfprod: int = 100):
+        """DMC population of the first nconf walkers (the context's walker count is the capacity)"""
+        st = np.ascontiguousarray(pop_state, dtype=np.uint64).reshape(4)
+        self._check(self.L.champ_gpu_dmc_init(self.ptr, int(nconf), float(tau), float(etrial), int(nfprod),
+                                              st.ctypes.data_as(C.POINTER(C.c_uint64))), "dmc_init")
+
+    def dmc_step(self) -> Dict[str, float]:
+        s = DmcStepStats()
+        self._check(self.L.champ_gpu_dmc_step(self.ptr, C.byref(s)), "dmc_step")
+        return {f: getattr(s, f) for f, _ in DmcStepStats._fields_ if f != "pad"}
+
+    def dmc(self, nblk: int, nstep: int, nblkeq: int = 0) -> Dict[str, float]:
+        r = DmcResult()
+        eg = np.zeros(int(nblk))
+        self._check(self.L.champ_gpu_dmc(self.ptr, int(nblk), int(nstep), int(nblkeq), _p(eg), C.byref(r)), "dmc")
+        out = {f: getattr(r, f) for f, _ in DmcResult._fields_}
+        out["egnow"] = eg
+        return out
+
+    def dmc_population(self):
+        """(weights, positions, local energies) of the live walkers"""
+        n = C.c_int32()
+        wt = np.zeros(self.nwalk)
+        self._check(self.L.champ_gpu_dmc_weights(self.ptr, _p(wt), C.byref(n)), "dmc_weights")
+        k = n.value
+        return wt[:k], self.get_walkers()[:k], self.scalars()["eold"][:k]
 
     def set_metrop6(self, deltar: float = 5.0, deltat: float = 1.0, fbias: float = 1.0):
         self._check(self.L.champ_gpu_set_metrop6(self.ptr, float(deltar), float(deltat), float(fbias)), "set_metrop6")

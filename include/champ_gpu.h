@@ -248,6 +248,61 @@ typedef struct champ_vmc_result {
 int champ_gpu_vmc(champ_ctx *ctx, int32_t nblk, int32_t nstep, int32_t nblkeq, double tau, int32_t imetro,
                   double *enow, champ_vmc_result *res);
 
+/* --------------------------------------------------------------------------
+ * Diffusion Monte Carlo (dmc.f90 with the dmc_ps mover, dmc_ps_mov1.f90:4-838)
+ * at CHAMP's default controls: idmc = 2, icasula = 0 (the non-local
+ * pseudopotential in the locality approximation), ipq = 1, itau_eff = 1,
+ * iacc_rej = 1, icross = 1, icut_br = 0, icut_e = 0, nforce = 1,
+ * node_cutoff = 0.  Systems with a pseudopotential only (CHAMP's all-electron
+ * mover dmc_ae is not provided: CHAMP_ERR_UNSUPPORTED).  The context's walker
+ * count at the first champ_gpu_dmc_init is the population's capacity (MWALK);
+ * from then on the context's walkers are the live population, slots
+ * 0 .. nwalk-1.  Each walker slot draws its moves from its own stream; the
+ * branching (splitj) uniforms come from the population stream given here
+ * (CHAMP draws both from one stream per MPI rank).
+ * ------------------------------------------------------------------------ */
+typedef struct champ_dmc_step_stats {
+    int32_t nwalk;          /* live walkers after the step's branching           */
+    int32_t pad;
+    double wsum, esum;      /* sum of weights and of weight * E_L (wsum1, esum1)  */
+    double wgsum, egsum;    /* the same times fprod (growth estimator)           */
+    double eest, eigv, fprod;  /* energy estimate, population factor, fprod     */
+    double accept;          /* mean acceptance probability of the step's moves   */
+    int64_t nodecr;         /* node crossings of the step                        */
+    int64_t nbranch;        /* branching events since the last zerest            */
+} champ_dmc_step_stats;
+
+typedef struct champ_dmc_result {
+    double egave, egerr;    /* growth estimator of the energy and its error
+                               (finwrt.f90:228-229: egcum/wgcum, errg)            */
+    double eest;            /* energy estimate at the end                        */
+    double accept;          /* acceptance of the accumulated moves               */
+    double tau_eff;         /* taucum / wgcum                                    */
+    int32_t nwalk, nwalk_max;
+    int64_t nbranch, nodecr;
+} champ_dmc_result;
+
+/* init (dmc/init.f90:3-191): the first nconf walkers (set by set_walkers)
+ * start with weight 1; hpsi of each; eest = etrial, eigv = fprod = 1,
+ * nfprod population-control steps (parser.f90:440), time step tau.
+ * pop_state: the xoshiro256** state of the branching stream.                 */
+int champ_gpu_dmc_init(champ_ctx *ctx, int32_t nconf, double tau, double etrial, int32_t nfprod,
+                       const uint64_t pop_state[4]);
+
+/* One DMC step of the population: rotqua, dmc_ps (every electron of every
+ * walker), the reweighting, splitj (splitj.f90:3-233) and acues1
+ * (acues1.f90:3-95).  stats may be NULL.                                      */
+int champ_gpu_dmc_step(champ_ctx *ctx, champ_dmc_step_stats *stats);
+
+/* dmc (dmc.f90:3-125): 2*nblkeq equilibration blocks (zerest after each
+ * half), then nblk blocks of nstep steps; egnow (may be NULL) [nblk]: the
+ * block energies (acuest.f90); res (may be NULL).                           */
+int champ_gpu_dmc(champ_ctx *ctx, int32_t nblk, int32_t nstep, int32_t nblkeq, double *egnow,
+                  champ_dmc_result *res);
+
+/* The live walker count and their weights [nwalk] (wt may be NULL).          */
+int champ_gpu_dmc_weights(champ_ctx *ctx, double *wt, int32_t *nwalk);
+
 /* metrop1 over n populations at once, typically different molecules (each
  * context its own system, walker count and precision): every context's
  * kernels run on its own stream in the order of its own champ_gpu_metrop1,

@@ -1723,6 +1723,269 @@ int orc_metrop6_sweep(const champ_system_t *s, oracle_walker_t *w, double deltar
     return acc;
 }
 
+/* ------------------------------------------------------------------------ */
+/* DMC (dmc.f90 with dmc_ps, dmc_ps_mov1.f90:4-838) at the default controls */
+/* idmc = 2, icasula = 0, ipq = 1, itau_eff = 1, iacc_rej = 1, icross = 1,  */
+/* icut_br = 0, icut_e = 0, nforce = 1, node_cutoff = 0.  One stream per    */
+/* walker slot (moves and quadrature rotation) and one population stream    */
+/* for splitj; a copied walker keeps its destination slot's stream.         */
+
+/* deep copy of one walker's state except its stream */
+static void walker_copy_state(oracle_walker_t *d, const oracle_walker_t *s, const champ_system_t *sy)
+{
+    const int ne = s->nelec, nc = s->ncent, nm = s->nmax, nb = s->nbasis, no = s->norb, np = s->npair;
+#define CP(f, n) memcpy(d->f, s->f, sizeof(double) * (size_t)(n))
+    CP(x, 3 * ne); CP(xnew_e, 3); CP(r_en, ne * nc); CP(rvec_en, 3 * ne * nc); CP(r_ee, np); CP(rvec_ee, 3 * np);
+    CP(r_en_sav, nc); CP(rvec_en_sav, 3 * nc); CP(r_ee_sav, ne); CP(rvec_ee_sav, 3 * ne);
+    CP(fjo, 3 * ne); CP(fjn, 3 * ne); CP(fso, ne * ne); CP(fsn, ne * ne); CP(fijo, 3 * ne * ne); CP(fijn, 3 * ne * ne);
+    CP(d2ijo, ne * ne); CP(d2ijn, ne * ne); CP(vj, 3 * ne); CP(vjn, 3 * ne); CP(orb, ne * no); CP(dorb, 3 * ne * no);
+    CP(ddorb, ne * no); CP(orbn, no); CP(dorbn, 3 * no); CP(ddorbn, no); CP(slmi, 2 * nm * nm); CP(slmin, nm * nm);
+    CP(fp, 3 * nm * nm); CP(fpp, nm * nm); CP(ddx, 3 * ne); CP(d2dx2, ne); CP(phin, nb); CP(dphin, 3 * nb);
+    CP(d2phin, nb); CP(vps, ne * nc * CHAMP_MPS_L);
+    if (s->ndet > 1) {
+        CP(md_inv, s->ndet * 2 * nm * nm); CP(md_det, s->ndet * 2); CP(md_invn, s->ndet * nm * nm);
+        CP(md_detn, s->ndet); CP(vmd, ne * no);
+    }
+#undef CP
+    (void)sy;
+    memcpy(d->xq, s->xq, sizeof(d->xq)); memcpy(d->yq, s->yq, sizeof(d->yq)); memcpy(d->zq, s->zq, sizeof(d->zq));
+    d->fsumo = s->fsumo; d->d2o = s->d2o; d->fsumn = s->fsumn; d->d2n = s->d2n;
+    d->detiab[0] = s->detiab[0]; d->detiab[1] = s->detiab[1]; d->detn = s->detn;
+    d->psid = s->psid; d->psij = s->psij; d->ekin = s->ekin; d->energy = s->energy; d->pe_local = s->pe_local;
+    d->d2j = s->d2j; d->vpsp_det[0] = s->vpsp_det[0]; d->vpsp_det[1] = s->vpsp_det[1]; d->nxquad = s->nxquad;
+    d->psido = s->psido; d->psijo = s->psijo; d->psi2o = s->psi2o; d->eold = s->eold; d->ratio_kref = s->ratio_kref;
+    d->md_psidn = s->md_psidn;
+}
+
+struct orc_dmc {
+    int cap, nwalk, nconf, nfprod, ipass, first;
+    double tau, etrial, eigv, eest, fprod, wdsumo;
+    double *ff, *wtgen, *wt, *fratio;
+    int *age;
+    uint64_t pop[4];
+    oracle_walker_t **w;
+    double wgsum, egsum, tausum, wgcum, egcum, taucum, wgcm2, egcm2;
+    long long nbrnch;
+};
+
+/* dmc_eloc_cutoff for nloc > 0 (branching.f90:226-238), tratio = 1 */
+static double dmc_fratio(const oracle_walker_t *w, double tau)
+{
+    const double adrift = 0.5;
+    double vav2sum = 0, v2sum = 0;
+    for (int i = 0; i < w->nelec; i++) {
+        double v[3];
+        for (int k = 0; k < 3; k++) v[k] = w->ddx[3 * i + k] + w->vj[3 * i + k];
+        const double v2 = orc_powi(v[0], 2) + orc_powi(v[1], 2) + orc_powi(v[2], 2);
+        const double vavvt = (sqrt(1.0 + 2.0 * adrift * v2 * tau) - 1.0) / (adrift * v2);
+        const double vavvn = vavvt / tau;
+        vav2sum = vav2sum + orc_powi(vavvn, 2) * v2;
+        v2sum = v2sum + v2;
+    }
+    return sqrt(vav2sum / v2sum);
+}
+
+orc_dmc_t *orc_dmc_new(const champ_system_t *s, int cap, const double *x0, const uint64_t *rng0, int nconf, double tau,
+                       double etrial, int nfprod, const uint64_t pop[4])
+{
+    orc_dmc_t *d = (orc_dmc_t *)xcalloc(1, sizeof(*d));
+    d->cap = cap; d->nwalk = nconf; d->nconf = nconf; d->nfprod = nfprod; d->tau = tau; d->etrial = etrial;
+    d->eigv = 1; d->eest = etrial; d->fprod = 1; d->wdsumo = nconf; d->first = 1;
+    d->ff = xcalloc(nfprod + 1, 8); d->wtgen = xcalloc(nfprod + 1, 8);
+    for (int i = 0; i <= nfprod; i++) { d->ff[i] = 1; d->wtgen[i] = nconf; }
+    d->wt = xcalloc(cap, 8); d->fratio = xcalloc(cap, 8); d->age = (int *)xcalloc(cap, sizeof(int));
+    memcpy(d->pop, pop, 32);
+    d->w = (oracle_walker_t **)xcalloc(cap, sizeof(*d->w));
+    for (int iw = 0; iw < cap; iw++) {
+        d->w[iw] = orc_walker_new(s);
+        orc_walker_set_x(d->w[iw], x0 + (size_t)iw * 3 * s->nelec);
+        orc_walker_set_rng(d->w[iw], rng0 + 4 * (size_t)iw);
+    }
+    for (int iw = 0; iw < nconf; iw++) {
+        double a, b, c2, e;
+        d->wt[iw] = 1;
+        orc_hpsi(s, d->w[iw], &a, &b, &c2, &e);
+    }
+    return d;
+}
+
+void orc_dmc_free(orc_dmc_t *d)
+{
+    if (!d) return;
+    for (int iw = 0; iw < d->cap; iw++) orc_walker_free(d->w[iw]);
+    free(d->w); free(d->ff); free(d->wtgen); free(d->wt); free(d->fratio); free(d->age);
+    free(d);
+}
+
+/* splitj (splitj.f90:3-233) with deep copies of the walker states */
+static int dmc_splitj(const champ_system_t *s, orc_dmc_t *d)
+{
+    int *iwundr = (int *)xcalloc(d->cap + 1, sizeof(int));
+    int iunder = 0, ipair = 0, iw2 = 0;
+    for (int iw = 0; iw < d->nwalk; iw++) {
+        if (d->wt[iw] < 0.5) {
+            if (d->wt[iw] == 0) {
+                d->nbrnch++;
+                iwundr[iunder++] = iw;
+            } else if (ipair == 0) {
+                ipair = 1;
+                iw2 = iw;
+            } else {
+                d->nbrnch++;
+                ipair = 0;
+                const double wttot = d->wt[iw] + d->wt[iw2];
+                if (orc_random_dp(d->pop) > d->wt[iw] / wttot) {
+                    d->wt[iw2] = wttot;
+                    iwundr[iunder++] = iw;
+                } else {
+                    d->wt[iw] = wttot;
+                    iwundr[iunder++] = iw2;
+                }
+            }
+        }
+    }
+    int nwalk2 = d->nwalk;
+    for (int iw = 0; iw < d->nwalk; iw++) {
+        if (d->wt[iw] >= 2) {
+            d->nbrnch++;
+            int dst;
+            if (iunder > 0) dst = iwundr[--iunder];
+            else {
+                dst = nwalk2++;
+                if (nwalk2 > d->cap) { free(iwundr); return -1; }
+            }
+            d->wt[iw] = d->wt[iw] * 0.5;
+            d->wt[dst] = d->wt[iw];
+            d->age[dst] = d->age[iw];
+            d->fratio[dst] = d->fratio[iw];
+            walker_copy_state(d->w[dst], d->w[iw], s);
+        }
+    }
+    for (int j = iunder - 1; j >= 0; j--) {
+        const int dst = iwundr[j], src = nwalk2 - 1;
+        nwalk2--;
+        d->wt[dst] = d->wt[src];
+        d->age[dst] = d->age[src];
+        d->fratio[dst] = d->fratio[src];
+        walker_copy_state(d->w[dst], d->w[src], s);
+    }
+    free(iwundr);
+    d->nwalk = nwalk2;
+    return nwalk2;
+}
+
+/* one DMC step of the population; out: wsum1, esum1, wgsum1, egsum1, eest,
+ * eigv, fprod, nodecr.  Returns the new walker count (-1: capacity).      */
+int orc_dmc_step(const champ_system_t *s, orc_dmc_t *d, double out[8])
+{
+    const double adrift = 0.5, tau = d->tau, rttau = sqrt(tau);
+    const int ne = s->nelec;
+    d->ipass++;
+    const int ipmod = d->ipass % d->nfprod;
+    const double ginv = tau < 1.0 ? tau : 1.0;
+    const double ffn = d->eigv * pow(d->wdsumo / d->nconf, ginv), ffi = 1 / ffn;
+    d->fprod = d->fprod * ffn / d->ff[ipmod];
+    d->ff[ipmod] = ffn;
+    if (d->first) {
+        for (int iw = 0; iw < d->nwalk; iw++) d->fratio[iw] = dmc_fratio(d->w[iw], tau);
+        d->first = 0;
+    }
+    double wsum1 = 0, esum1 = 0, nod = 0;
+    for (int iw = 0; iw < d->nwalk; iw++) {
+        oracle_walker_t *w = d->w[iw];
+        if (s->nloc > 0) orc_rotqua(s, w->rng, w->xq, w->yq, w->zq);
+        double dfus2ac = 0, dfus2un = 0;
+        int iaccept = 0;
+        for (int i = 0; i < ne; i++) {
+            double vold[3], vnew[3], xold[3], xnew[3];
+            for (int k = 0; k < 3; k++) xold[k] = w->x[3 * i + k];
+            orc_determinante_grad(s, w, i, 1, vold);
+            const double v2old = orc_powi(vold[0], 2) + orc_powi(vold[1], 2) + orc_powi(vold[2], 2);
+            double vavvt = (sqrt(1.0 + 2.0 * adrift * v2old * tau) - 1.0) / (adrift * v2old);
+            double dfus2o = 0;
+            for (int k = 0; k < 3; k++) {
+                const double drift = vavvt * vold[k];
+                const double dfus = orc_gauss(w->rng) * rttau;
+                const double dx = drift + dfus;
+                dfus2o = dfus2o + orc_powi(dfus, 2);
+                xnew[k] = xold[k] + dx;
+            }
+            double psidn, psijn;
+            orc_psie(s, w, i, xnew, &psidn, &psijn, 0);
+            orc_determinante_grad(s, w, i, 0, vnew);
+            if (psidn * w->psido <= 0) nod++;
+            const double v2new = orc_powi(vnew[0], 2) + orc_powi(vnew[1], 2) + orc_powi(vnew[2], 2);
+            vavvt = (sqrt(1.0 + 2.0 * adrift * v2new * tau) - 1.0) / (adrift * v2new);
+            double dfus2n = 0;
+            for (int k = 0; k < 3; k++) {
+                const double drift = vavvt * vnew[k];
+                const double dfus = xnew[k] + drift - xold[k];
+                dfus2n = dfus2n + orc_powi(dfus, 2);
+            }
+            double p = orc_powi(psidn / w->psido, 2) * exp(2 * (psijn - w->psijo)) * exp((dfus2o - dfus2n) / (2 * tau));
+            if (d->age[iw] > 50) p = p * pow(1.1, d->age[iw] - 50);
+            p = p < 1.0 ? p : (p >= 1.0 ? 1.0 : 0.0);
+            dfus2ac = dfus2ac + p * dfus2o;
+            dfus2un = dfus2un + dfus2o;
+            if (orc_random_dp(w->rng) < p) {
+                iaccept = 1;
+                d->age[iw] = 0;
+                w->psi2o = 2 * (log(fabs(psidn)) + psijn);
+                w->psido = psidn; w->psijo = psijn;
+                orc_accept(s, w, i);
+            } else {
+                orc_reject(s, w, i);
+            }
+        }
+        const double taunow = tau * dfus2ac / dfus2un;
+        const double eprev = w->eold;
+        double psid, psij, ekin, enew;
+        orc_hpsi(s, w, &psid, &psij, &ekin, &enew);
+        const double fration = dmc_fratio(w, tau);
+        const double deo = d->eest - eprev, den = d->eest - enew;
+        const double ewto = d->eest - deo * d->fratio[iw], ewtn = d->eest - den * fration;
+        const double expon = (d->etrial - 0.5 * (ewto + ewtn)) * taunow;
+        const double dwt = exp(expon) * ffi;
+        d->wt[iw] = d->wt[iw] * dwt;
+        const double wtnow = d->wt[iw], wtg = wtnow * d->fprod;
+        d->tausum = d->tausum + wtg * taunow;
+        d->fratio[iw] = fration;
+        if (!iaccept) d->age[iw]++;
+        wsum1 = wsum1 + wtnow;
+        esum1 = esum1 + wtnow * enew;
+    }
+    const double wgsum1 = wsum1 * d->fprod, egsum1 = esum1 * d->fprod;
+    if (dmc_splitj(s, d) < 0) return -1;
+    d->wgsum += wgsum1; d->egsum += egsum1;
+    d->eest = (d->egcum + d->egsum) / (d->wgcum + d->wgsum);
+    d->eigv = exp((d->etrial - d->eest) * (d->taucum + d->tausum) / (d->wgcum + d->wgsum));
+    d->wdsumo = wsum1;
+    d->wtgen[ipmod] = wsum1;
+    out[0] = wsum1; out[1] = esum1; out[2] = wgsum1; out[3] = egsum1; out[4] = d->eest; out[5] = d->eigv;
+    out[6] = d->fprod; out[7] = nod;
+    return d->nwalk;
+}
+
+/* end of block (acuest): block energy egsum/wgsum, cumulative sums */
+double orc_dmc_block(orc_dmc_t *d)
+{
+    const double en = d->egsum / d->wgsum;
+    d->wgcum += d->wgsum; d->egcum += d->egsum; d->taucum += d->tausum;
+    d->wgcm2 += d->wgsum * d->wgsum; d->egcm2 += d->egsum * en;
+    d->wgsum = d->egsum = d->tausum = 0;
+    return en;
+}
+
+int orc_dmc_get(const orc_dmc_t *d, double *wt, double *x, double *eold)
+{
+    for (int iw = 0; iw < d->nwalk; iw++) {
+        if (wt) wt[iw] = d->wt[iw];
+        if (x) orc_walker_get_x(d->w[iw], x + (size_t)iw * 3 * d->w[iw]->nelec);
+        if (eold) eold[iw] = d->w[iw]->eold;
+    }
+    return d->nwalk;
+}
+
 void orc_walker_scalars(const oracle_walker_t *w, double out[4])
 {
     out[0] = w->psido; out[1] = w->psijo; out[2] = w->eold; out[3] = w->psi2o;

@@ -69,6 +69,14 @@ int    orc_metrop1_sweep(const champ_system_t *s, oracle_walker_t *w, double tau
 int    orc_metrop6_sweep(const champ_system_t *s, oracle_walker_t *w, double deltar, double deltat, double fbias);
 double orc_gammai(double a, double x, double xae, int *iflag);
 double orc_m6_area(double zeta, double co, double rbot, double rtop, double bot);
+/* DMC population (dmc.f90 / dmc_ps / splitj / acues1 at default controls) */
+typedef struct orc_dmc orc_dmc_t;
+orc_dmc_t *orc_dmc_new(const champ_system_t *s, int cap, const double *x0, const uint64_t *rng0, int nconf, double tau,
+                       double etrial, int nfprod, const uint64_t pop[4]);
+void   orc_dmc_free(orc_dmc_t *d);
+int    orc_dmc_step(const champ_system_t *s, orc_dmc_t *d, double out[8]);
+double orc_dmc_block(orc_dmc_t *d);
+int    orc_dmc_get(const orc_dmc_t *d, double *wt, double *x, double *eold);
 void   orc_walker_scalars(const oracle_walker_t *w, double out[4]); /* psido psijo eold psi2o */
 
 /* state getters for checking */

@@ -48,6 +48,15 @@ def lib():
         L.orc_metrop1_sweep.argtypes = [vp, vp, C.c_double]
         L.orc_metrop1_sweep.restype = C.c_int
         L.orc_metrop6_sweep.argtypes = [vp, vp, C.c_double, C.c_double, C.c_double]
+        L.orc_dmc_new.argtypes = [vp, C.c_int, dp, u64p, C.c_int, C.c_double, C.c_double, C.c_int, u64p]
+        L.orc_dmc_new.restype = vp
+        L.orc_dmc_free.argtypes = [vp]
+        L.orc_dmc_step.argtypes = [vp, vp, dp]
+        L.orc_dmc_step.restype = C.c_int
+        L.orc_dmc_block.argtypes = [vp]
+        L.orc_dmc_block.restype = C.c_double
+        L.orc_dmc_get.argtypes = [vp, dp, dp, dp]
+        L.orc_dmc_get.restype = C.c_int
         L.orc_metrop6_sweep.restype = C.c_int
         L.orc_gammai.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int)]
         L.orc_gammai.restype = C.c_double
@@ -261,3 +270,45 @@ def ref_rng_lib():
     L.xoroshiro_set_state.argtypes = [C.POINTER(C.c_uint32)]
     L.xoroshiro_get_state.argtypes = [C.POINTER(C.c_uint32)]
     return L
+
+
+class Dmc:
+    """The oracle's DMC population (orc_dmc_*): cap walker slots with their own
+    streams, the first nconf live, and the population stream of splitj."""
+
+    def __init__(self, system, x0, rng0, nconf, tau, etrial, pop, nfprod=100, csys=None):
+        self.L = lib()
+        self.sys = system
+        self.csys = csys if csys is not None else system.as_ctypes()
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        rng0 = np.ascontiguousarray(rng0, dtype=np.uint64)
+        pop = np.ascontiguousarray(pop, dtype=np.uint64)
+        self.cap = x0.shape[0]
+        self.ptr = self.L.orc_dmc_new(C.byref(self.csys), self.cap, P(x0), U(rng0), int(nconf), float(tau),
+                                      float(etrial), int(nfprod), U(pop))
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.L.orc_dmc_free(self.ptr)
+            self.ptr = None
+
+    def step(self):
+        out = np.zeros(8)
+        n = self.L.orc_dmc_step(C.byref(self.csys), self.ptr, P(out))
+        if n < 0:
+            raise RuntimeError("oracle DMC: walker capacity exceeded")
+        keys = ("wsum", "esum", "wgsum", "egsum", "eest", "eigv", "fprod", "nodecr")
+        d = dict(zip(keys, out))
+        d["nwalk"] = n
+        return d
+
+    def block(self):
+        return self.L.orc_dmc_block(self.ptr)
+
+    def population(self):
+        ne = self.sys.nelec
+        wt = np.zeros(self.cap)
+        x = np.zeros((self.cap, ne, 3))
+        e = np.zeros(self.cap)
+        n = self.L.orc_dmc_get(self.ptr, P(wt), P(x), P(e))
+        return wt[:n], x[:n], e[:n]

@@ -8,6 +8,8 @@ the energies the CI scripts compare against:
           (vmc_noopt_ci44.inp; energies from
           VMC_C4H6_ci44_pVQZ_20-dets_12-csf_energy-only_butadiene.sh:8-9, :19-20;
           the CHAMP-v2 orbital file and basis info its conversion step reads)
+  C4H6_dmc  tests/CI_test/DMC-C4H6-ci1010_pVTZ-500-dets (dmc.inp; energies from
+          DMC_C4H6_ci1010_pVTZ_500-dets_butadiene.sh:8-9, :16-17)
 
 Run in the build container only (the reference tree is not on GPU hosts):
     python tests/golden/champ_inputs/make_fixtures.py
@@ -33,6 +35,10 @@ CASES = {
                    "pool/BFD-Q.basis.C", "pool/BFD-Q.basis.H", "pool/BFD.gauss_ecp.dat.C",
                    "pool/BFD.gauss_ecp.dat.H", "conversion/champ_v2_gamess_ci44_pVQZ_cart.lcao",
                    "conversion/champ_v2_gamess_ci44_pVQZ.bfinfo"]),
+    "C4H6_dmc": ("DMC-C4H6-ci1010_pVTZ-500-dets", "DMC_C4H6_ci1010_pVTZ_500-dets_butadiene.sh",
+                 ["dmc.inp", "TZ_1M_500.det", "champ_v3_trexio_order_ci1010_pVTZ_1_orbitals.lcao",
+                  "jastrow_good_b3lyp.0", "pool/champ_v3_butadiene.xyz", "pool/champ_v3_BFD-T_basis_pointers.bfinfo",
+                  "pool/BFD-T.basis.C", "pool/BFD-T.basis.H", "pool/BFD.gauss_ecp.dat.C", "pool/BFD.gauss_ecp.dat.H"]),
 }
 
 
@@ -51,7 +57,11 @@ def main():
         for f in files:
             os.makedirs(os.path.dirname(os.path.join(dst, f)), exist_ok=True)
             if os.path.getsize(os.path.join(src, f)) > 100_000:
-                with open(os.path.join(src, f), "rb") as fi, gzip.open(os.path.join(dst, f) + ".gz", "wb") as fo:
+                out = os.path.join(dst, f) + ".gz"
+                if os.path.exists(out):  # gzip output carries a timestamp: keep committed fixtures as they are
+                    continue
+                with open(os.path.join(src, f), "rb") as fi, open(out, "wb") as raw, \
+                        gzip.GzipFile(fileobj=raw, mode="wb", mtime=0) as fo:
                     shutil.copyfileobj(fi, fo)
             else:
                 shutil.copy(os.path.join(src, f), os.path.join(dst, f))

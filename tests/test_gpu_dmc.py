@@ -1,0 +1,90 @@
+"""DMC on the GPU (champ_gpu_dmc_*: dmc.f90 with the dmc_ps mover, splitj,
+acues1/acuest at CHAMP's default controls).
+
+* the population against the oracle's DMC step by step: walker counts after
+  branching, the weight and energy sums, eest, eigv, fprod, node crossings,
+  then every live walker's weight, position and local energy;
+* the DMC energy of the reference's own DMC CI case (butadiene, 500
+  determinants, BFD pseudopotential, tau 0.05, etrial -26.3;
+  tests/CI_test/DMC-C4H6-ci1010_pVTZ-500-dets) against the energy that case's
+  CI script asserts, with the reference's comparison rule
+  (|E - E_ref| <= 2 sqrt(err^2 + err_ref^2), tools/compare_value.py).  The
+  reference ran T-moves (icasula = -1); this engine uses the locality
+  approximation (icasula = 0), whose difference is well inside that bound.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from champ_amd import champ_io as io
+from champ_amd import system as S
+from champ_amd.engine import PREC_FP64, PREC_MIXED, Engine
+from oracle import ffi
+from tests.gpu_helpers import ENGINE_LIB, rel
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _etrial(sy, x):
+    es = []
+    for xw in x:
+        w = ffi.walker(sy)
+        w.set_x(xw)
+        es.append(w.hpsi()["energy"])
+    return float(np.mean(es))
+
+
+@pytest.mark.parametrize("name", ["small", "small_md"])
+def test_dmc_population_matches_oracle(require_gpu, name):
+    sy = S.config(name)
+    cap, nconf, tau, nsteps = 24, 12, 0.05, 5
+    x0 = sy.initial_walkers(cap, seed=5)
+    st = ffi.seeded_streams(S.seed_from_string(), cap + 1)
+    etrial = _etrial(sy, x0[:nconf])
+    eng = Engine(sy, cap, lib_path=ENGINE_LIB)
+    eng.set_walkers(x0)
+    eng.set_rng(st[:cap])
+    eng.dmc_init(nconf, tau, etrial, st[cap])
+    orc = ffi.Dmc(sy, x0, st[:cap], nconf, tau, etrial, st[cap])
+    for k in range(nsteps):
+        g, o = eng.dmc_step(), orc.step()
+        print(f"[dmc {name}] step {k}: nwalk {g['nwalk']} / {o['nwalk']}, wsum {g['wsum']:.10f} / {o['wsum']:.10f}, "
+              f"eest {g['eest']:.8f} / {o['eest']:.8f}", flush=True)
+        assert g["nwalk"] == o["nwalk"] and g["nodecr"] == o["nodecr"]
+        for key in ("wsum", "esum", "wgsum", "egsum", "eest", "eigv", "fprod"):
+            assert rel(g[key], o[key]) < 1e-9, (k, key, g[key], o[key])
+    wg, xg, eg = eng.dmc_population()
+    wo, xo, eo = orc.population()
+    assert len(wg) == len(wo)
+    assert rel(wg, wo) < 1e-9
+    assert np.max(np.abs(xg - xo)) < 1e-9
+    assert np.max(np.abs(eg - eo) / np.maximum(1.0, np.abs(eo))) < 1e-9
+
+
+def test_dmc_energy_matches_reference_ci(require_gpu):
+    sy = io.load_input(os.path.join(HERE, "golden", "champ_inputs", "C4H6_dmc", "dmc.inp"), name="C4H6_dmc")
+    import json
+    with open(os.path.join(HERE, "golden", "champ_inputs", "C4H6_dmc", "reference_energies.json")) as fh:
+        ref = [e for e in json.load(fh)["total_E"] if e["cores"] == 1][0]
+    cap, nconf = 3072, 1536
+    res = {}
+    for prec in (PREC_FP64, PREC_MIXED):
+        eng = Engine(sy, cap, precision=prec, lib_path=ENGINE_LIB)
+        eng.set_walkers(sy.initial_walkers(cap, seed=3))
+        eng.seed_rng(S.seed_from_string())
+        eng.hpsi()
+        eng.vmc(2, 10, nblkeq=0, tau=0.2)  # walkers drawn from |psi|^2 first
+        pop = ffi.seeded_streams(S.seed_from_string(), cap + 1)[cap]
+        eng.dmc_init(nconf, 0.05, -26.3, pop)
+        r = eng.dmc(12, 20, nblkeq=2)
+        res[prec] = r
+        print(f"[dmc C4H6 {'fp64' if prec == PREC_FP64 else 'mixed'}] E = {r['egave']:.6f} +- {r['egerr']:.6f}; "
+              f"reference {ref['energy']} +- {ref['error']}; acceptance {r['accept']:.4f}; tau_eff {r['tau_eff']:.4f}; "
+              f"walkers {r['nwalk']} (max {r['nwalk_max']}); branchings {r['nbranch']}; node crossings {r['nodecr']}",
+              flush=True)
+        assert abs(r["egave"] - ref["energy"]) <= 2.0 * np.hypot(r["egerr"], ref["error"])
+        assert 0.9 < r["accept"] < 1.0 and 0.5 * nconf < r["nwalk"] < 2 * nconf
+        eng.close()
