@@ -317,7 +317,13 @@ __global__ void __launch_bounds__(256) k_hpsi_ao(HArgs a)
 // GLB: the work matrix of more than ~140 electrons of one spin does not fit
 // in LDS; it then lives in the block's slice of a global work buffer (L2
 // resident during the elimination), with the same arithmetic in the same order.
-template <bool GLB>
+// R > 0 (up to 16 R electrons per spin, LDS form only): the elimination runs
+// on register tiles.  Thread (ti, tj) of a 16 x 16 grid holds the elements
+// (ti + 16 a, tj + 16 b), a, b < R; each pivot step exchanges only the pivot
+// column and the two swapped rows through LDS (double-buffered), so the
+// updates are register FMAs instead of dependent LDS round trips.  Same
+// arithmetic and order per element as the LDS form.
+template <bool GLB, int R>
 __global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
 {
     // static arrays: SLATER_STATIC_LDS bytes (hpsi_args.hpp)
@@ -342,9 +348,119 @@ __global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
         L[i * ld + j] = ob[((size_t)(ish + i) * oc) * NO + a.iworbd[ish + j]];
     }
     __syncthreads();
+    if constexpr (R > 0) {
+        static_assert(!GLB, "register tiles with the LDS work matrix only");
+        const int ti = tid / 16, tj = tid % 16, NMX = a.NM;
+        double m[R][R];
+#pragma unroll
+        for (int ra = 0; ra < R; ra++)
+#pragma unroll
+            for (int rb = 0; rb < R; rb++) {
+                const int i = ti + 16 * ra, j = tj + 16 * rb;
+                m[ra][rb] = (i < nel && j < nel) ? L[i * ld + j] : 0.0;
+            }
+        double *XB = col;  // [2 parities][C, RK, RP][NM]
+        for (int k = 0; k < nel; k++) {
+            double *Cb = XB + (size_t)(k & 1) * 3 * NMX, *Rk = Cb + NMX, *Rp = Rk + NMX;
+            // the pivot column
+            if (tj == k % 16) {
+#pragma unroll
+                for (int ra = 0; ra < R; ra++) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int rb = 0; rb < R; rb++) v = (rb == k / 16) ? m[ra][rb] : v;
+                    const int i = ti + 16 * ra;
+                    if (i < nel) Cb[i] = v;
+                }
+            }
+            __syncthreads();
+            if (wv == 0) {
+                double best = -1.0; int bi = k;
+                for (int i = k + lane; i < nel; i += WAVE) {
+                    const double v = fabs(Cb[i]);
+                    if (v > best) { best = v; bi = i; }
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ob2 = __shfl_xor(best, off, WAVE);
+                    const int oi = __shfl_xor(bi, off, WAVE);
+                    if (ob2 > best || (ob2 == best && oi < bi)) { best = ob2; bi = oi; }
+                }
+                if (lane == 0) {
+                    s_p = bi; perm[k] = bi;
+                    piv[k] = Cb[bi]; s_akk = Cb[k];
+                    if (piv[k] == 0.0) s_bad = 1;
+                }
+            }
+            __syncthreads();
+            const int p = s_p;
+            // rows k and p before the swap
+            if (ti == k % 16 || ti == p % 16) {
+#pragma unroll
+                for (int rb = 0; rb < R; rb++) {
+                    const int j = tj + 16 * rb;
+                    if (j >= nel) continue;
+                    double vk = 0.0, vp = 0.0;
+#pragma unroll
+                    for (int ra = 0; ra < R; ra++) {
+                        vk = (ti + 16 * ra == k) ? m[ra][rb] : vk;
+                        vp = (ti + 16 * ra == p) ? m[ra][rb] : vp;
+                    }
+                    if (ti == k % 16) Rk[j] = vk;
+                    if (ti == p % 16) Rp[j] = vp;
+                }
+            }
+            __syncthreads();
+            const double pinv = 1.0 / piv[k], akk = s_akk;
+            double rj[R], rkj[R];
+#pragma unroll
+            for (int rb = 0; rb < R; rb++) {
+                const int j = tj + 16 * rb;
+                rj[rb] = j < nel ? ((j == k) ? pinv : Rp[j] * pinv) : 0.0;
+                rkj[rb] = j < nel ? Rk[j] : 0.0;
+            }
+#pragma unroll
+            for (int ra = 0; ra < R; ra++) {
+                const int i = ti + 16 * ra;
+                if (i >= nel) continue;
+                if (i == k) {
+#pragma unroll
+                    for (int rb = 0; rb < R; rb++) m[ra][rb] = rj[rb];
+                    continue;
+                }
+                const double ci = (i == p) ? akk : Cb[i];
+#pragma unroll
+                for (int rb = 0; rb < R; rb++) {
+                    const int j = tj + 16 * rb;
+                    double base = (i == p) ? rkj[rb] : m[ra][rb];
+                    base = (j == k) ? 0.0 : base;
+                    m[ra][rb] = base - rj[rb] * ci;
+                }
+            }
+        }
+        // undo the column order of the row swaps: the final inverse's column c is
+        // the eliminated matrix's column sig(c), sig = swaps (k, perm[k]) applied
+        // in reverse order; each element goes to its place in the LDS matrix
+        int *sinv = (int *)XB;  // [NM] (the exchange buffers are free now)
+        __syncthreads();
+        if (tid == 0) {
+            int *sig = sinv + NMX;
+            for (int c = 0; c < nel; c++) sig[c] = c;
+            for (int k = nel - 1; k >= 0; k--) { const int t = sig[k]; sig[k] = sig[perm[k]]; sig[perm[k]] = t; }
+            for (int c = 0; c < nel; c++) sinv[sig[c]] = c;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ra = 0; ra < R; ra++)
+#pragma unroll
+            for (int rb = 0; rb < R; rb++) {
+                const int i = ti + 16 * ra, j = tj + 16 * rb;
+                if (i < nel && j < nel) L[i * ld + sinv[j]] = m[ra][rb];
+            }
+        __syncthreads();
+    }
     const int nd = nel > 0 ? nel : 1;  // element stepping of the elimination (no elimination when nel = 0)
     const int i0 = tid / nd, j0 = tid % nd, si = bd / nd, sj = bd % nd;
-    for (int k = 0; k < nel; k++) {
+    for (int k = 0; k < (R > 0 ? 0 : nel); k++) {
         if (wv == 0) {
             double best = -1.0; int bi = k;
             for (int i = k + lane; i < nel; i += WAVE) {
@@ -387,7 +503,7 @@ __global__ void __launch_bounds__(256) k_hpsi_slater(HArgs a)
         }
         __syncthreads();
     }
-    for (int k = nel - 1; k >= 0; k--) {
+    for (int k = (R > 0 ? -1 : nel - 1); k >= 0; k--) {
         const int p = perm[k];
         if (p != k)
             for (int i = tid; i < nel; i += bd) { const double t = L[i * ld + k]; L[i * ld + k] = L[i * ld + p]; L[i * ld + p] = t; }
@@ -611,6 +727,38 @@ __global__ void __launch_bounds__(256) k_hpsi_eloc(HArgs a)
 // walker, in chunks of walkers (orbitals of all electrons are large): the
 // determinant part of hpsi, also used on its own for the periodic FP64
 // refresh of the inverse on the mixed path.
+// Register tiles of the Slater inverse: 16 R >= electrons per spin, R <= 8
+// (64 doubles per thread); 0 above that (the LDS elimination)
+#ifndef CHAMP_SLATER_TILES
+#define CHAMP_SLATER_TILES 1
+#endif
+static int slater_tiles(int nmx) { return (CHAMP_SLATER_TILES && nmx <= 128) ? (nmx + 15) / 16 : 0; }
+
+template <int R>
+static int launch_slater_tiles_r(champ_ctx *c, const HArgs &a, size_t lds)
+{
+    if (lds + SLATER_STATIC_LDS > 64 * 1024)
+        HIPCHECK(hipFuncSetAttribute((const void *)k_hpsi_slater<false, R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds));
+    hipLaunchKernelGGL((k_hpsi_slater<false, R>), dim3(2 * a.nw), dim3(256), lds, c->stream, a);
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
+static int launch_slater_tiles(champ_ctx *c, const HArgs &a, int rt, size_t lds)
+{
+    switch (rt) {
+    case 1: return launch_slater_tiles_r<1>(c, a, lds);
+    case 2: return launch_slater_tiles_r<2>(c, a, lds);
+    case 3: return launch_slater_tiles_r<3>(c, a, lds);
+    case 4: return launch_slater_tiles_r<4>(c, a, lds);
+    case 5: return launch_slater_tiles_r<5>(c, a, lds);
+    case 6: return launch_slater_tiles_r<6>(c, a, lds);
+    case 7: return launch_slater_tiles_r<7>(c, a, lds);
+    default: return launch_slater_tiles_r<8>(c, a, lds);
+    }
+}
+
 // values_only: orbital values alone, and from them the inverse and the spin
 // determinants (the periodic recompute inside a sweep of a single-determinant
 // trial function); otherwise also gradients and Laplacians, ddx, d2dx2, the
@@ -644,16 +792,24 @@ static int launch_orbitals_slater(champ_ctx *c, HArgs a, bool values_only = fals
         const int nmx = std::max(c->NUP, c->NDN);
         tbegin(c, K_HPSI_SLATER, &slot);
         if (a.lu_ws) {
-            hipLaunchKernelGGL(k_hpsi_slater<true>, dim3(2 * a.nw), dim3(256), (size_t)nmx * sizeof(double),
+            hipLaunchKernelGGL((k_hpsi_slater<true, 0>), dim3(2 * a.nw), dim3(256), (size_t)nmx * sizeof(double),
                                c->stream, a);
         } else {
             // dynamic work matrix; with SLATER_STATIC_LDS it fits one block's LDS
             // (otherwise init allocated lu_ws and the branch above is taken)
-            const size_t lds = (size_t)(nmx * (nmx + 1) + nmx) * sizeof(double);
-            if (lds + SLATER_STATIC_LDS > 64 * 1024)
-                HIPCHECK(hipFuncSetAttribute((const void *)k_hpsi_slater<false>,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL(k_hpsi_slater<false>, dim3(2 * a.nw), dim3(256), lds, c->stream, a);
+            const int rt = slater_tiles(nmx);
+            if (rt > 0) {
+                // register tiles: the work matrix plus the double-buffered pivot column and rows
+                const size_t lds = (size_t)(nmx * (nmx + 1) + 6 * nmx) * sizeof(double);
+                const int rc = launch_slater_tiles(c, a, rt, lds);
+                if (rc) return rc;
+            } else {
+                const size_t lds = (size_t)(nmx * (nmx + 1) + nmx) * sizeof(double);
+                if (lds + SLATER_STATIC_LDS > 64 * 1024)
+                    HIPCHECK(hipFuncSetAttribute((const void *)k_hpsi_slater<false, 0>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                hipLaunchKernelGGL((k_hpsi_slater<false, 0>), dim3(2 * a.nw), dim3(256), lds, c->stream, a);
+            }
         }
         tend(c, slot);
         HIPCHECK(hipGetLastError());
