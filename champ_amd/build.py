@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libchamp_gpu.so")
 OBJDIR = os.path.join(HERE, "_build")
 SOURCES = ["engine_api.hip", "engine_kernels.hip", "engine_hpsi.hip", "engine_gemm.hip", "engine_nonloc.hip",
-           "engine_multidet.hip", "engine_dmc.hip"]
+           "engine_multidet.hip", "engine_dmc.hip", "engine_deriv.hip"]
 ARCH = os.environ.get("CHAMP_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

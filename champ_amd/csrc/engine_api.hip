@@ -432,6 +432,7 @@ extern "C" int champ_gpu_free(champ_ctx *c)
     for (void *p : ptrs) if (p) hipFree(p);
     md_free(c);
     dmc_free(c);
+    deriv_free(c);
     if (c->sweep_exec) hipGraphExecDestroy(c->sweep_exec);
     for (auto &t : c->ev_pool) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     if (c->stream) hipStreamDestroy(c->stream);

@@ -119,6 +119,11 @@ struct champ_ctx {
     double *dmc_out = nullptr;   // [W][DMC_OUT] per-walker terms of the step's estimators
     int dmc_on = 0, dmc_W = 0;   // DMC initialised; capacity (the context's walker count at init)
     champ::DmcState *dmc = nullptr;
+
+    // ---- parameter derivatives (engine_deriv.hip): d ln psi / d p of the
+    // current walkers, parameter-major [nparm][W], and the SR sums' scratch
+    double *der_ot = nullptr, *der_s = nullptr, *der_v = nullptr;
+    int der_cap = 0;
     unsigned char *acc = nullptr;
     double *esum_dev = nullptr;  // [4] accepted, esum, esum2, nflag
 
@@ -138,6 +143,7 @@ struct champ_ctx {
     int md = 0, md_ndet = 0, md_NV = 0, md_NU = 0, md_nu[2] = {0, 0}, md_nv[2] = {0, 0};
     int *md_du = nullptr, *md_dd = nullptr;   // [ndet] unique spin determinant of each det
     double *md_c = nullptr;                   // [ndet] cdet with the reordering phase
+    double *md_ph = nullptr;                  // [ndet] the reordering phase (+-1) of each determinant
     int *md_rank[2] = {nullptr, nullptr}, *md_hole[2] = {nullptr, nullptr}, *md_part[2] = {nullptr, nullptr};  // [nu][4]
     int *md_virt[2] = {nullptr, nullptr};     // [nv] orbital of each particle index
     int *md_woff[2] = {nullptr, nullptr}, *md_woth[2] = {nullptr, nullptr};  // dets of each spin determinant (CSR):
@@ -201,6 +207,7 @@ int launch_wmat(champ_ctx *c);
 int launch_gemm_f32(champ_ctx *c, int M, int N, int K, const float *A, int lda, const float *B, int ldb, double *Cm,
                     int ldc);
 void dmc_free(champ_ctx *c);
+void deriv_free(champ_ctx *c);
 int md_init(champ_ctx *c, const champ_system_t *s);
 void md_free(champ_ctx *c);
 int launch_md_hpsi(champ_ctx *c, int w0, int nw);

@@ -560,7 +560,7 @@ int md_init(champ_ctx *c, const champ_system_t *s)
     if (!c->md) return CHAMP_OK;
     const int NE = c->NE, NO = c->NO, ndet = s->ndet;
     c->md_ndet = ndet;
-    std::vector<double> cd(ndet);
+    std::vector<double> cd(ndet), ph(ndet, 1.0);
     for (int k = 0; k < ndet; k++) cd[k] = s->cdet[k];
     std::vector<int> uidx[2];
     struct SpinTab {
@@ -612,7 +612,7 @@ int md_init(champ_ctx *c, const champ_system_t *s)
                 for (int j = i; !vis[j]; j = perm[j]) { vis[j] = 1; len++; }
                 parity ^= (len - 1) & 1;
             }
-            if (parity) cd[k] = -cd[k];
+            if (parity) { cd[k] = -cd[k]; ph[k] = -ph[k]; }
             std::vector<int> key(holes);
             key.insert(key.end(), parts.begin(), parts.end());
             auto it = seen.find(key);
@@ -720,6 +720,7 @@ int md_init(champ_ctx *c, const champ_system_t *s)
     if ((rc = md_upload(&c->md_du, uidx[0]))) return rc;
     if ((rc = md_upload(&c->md_dd, uidx[1]))) return rc;
     if ((rc = md_upload(&c->md_c, cd))) return rc;
+    if ((rc = md_upload(&c->md_ph, ph))) return rc;
     const size_t NU = c->md_NU;
     if ((rc = md_zeros(&c->md_aa, (size_t)W * 4 * NM * NV))) return rc;
     if ((rc = md_zeros(&c->md_D, (size_t)W * 4 * NU))) return rc;
@@ -740,7 +741,7 @@ int md_init(champ_ctx *c, const champ_system_t *s)
 
 void md_free(champ_ctx *c)
 {
-    void *ps[] = {c->md_du, c->md_dd, c->md_c, c->md_aa, c->md_D, c->md_tb, c->md_G, c->md_S, c->md_Sn, c->md_cur,
+    void *ps[] = {c->md_du, c->md_dd, c->md_c, c->md_ph, c->md_aa, c->md_D, c->md_tb, c->md_G, c->md_S, c->md_Sn, c->md_cur,
                   c->md_vcol, c->md_vcoln, c->md_V, c->md_Y, c->md_Z, c->md_Wt, c->md_g};
     for (void *p : ps) if (p) hipFree(p);
     for (int sp = 0; sp < 2; sp++) {

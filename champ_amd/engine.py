@@ -40,6 +40,9 @@ class VmcResult(C.Structure):
                 ("accept", C.c_double), ("nsamples", C.c_int64), ("nflagged", C.c_int64)]
 
 
+DERIV_JAS_EN, DERIV_JAS_EE, DERIV_JAS_EEN, DERIV_CI = 1, 2, 4, 8
+
+
 class DmcStepStats(C.Structure):
     _fields_ = [("nwalk", C.c_int32), ("pad", C.c_int32), ("wsum", C.c_double), ("esum", C.c_double),
                 ("wgsum", C.c_double), ("egsum", C.c_double), ("eest", C.c_double), ("eigv", C.c_double),
@@ -89,6 +92,9 @@ def load_library(path: str = LIBPATH):
     L.champ_gpu_dmc_step.argtypes = [vp, C.POINTER(DmcStepStats)]
     L.champ_gpu_dmc.argtypes = [vp, i32, i32, i32, dp, C.POINTER(DmcResult)]
     L.champ_gpu_dmc_weights.argtypes = [vp, dp, C.POINTER(i32)]
+    L.champ_gpu_deriv_count.argtypes = [vp, i32, C.POINTER(i32)]
+    L.champ_gpu_deriv_psi.argtypes = [vp, i32, dp]
+    L.champ_gpu_sr_sums.argtypes = [vp, i32, dp, dp, dp, dp]
     L.champ_gpu_set_metrop6.argtypes = [vp, C.c_double, C.c_double, C.c_double]
     L.champ_gpu_get_walker_scalars.argtypes = [vp, dp, dp, dp, dp]
     L.champ_gpu_get_status.argtypes = [vp, C.POINTER(C.c_int32)]
@@ -244,6 +250,26 @@ class Engine:
         self._check(self.L.champ_gpu_dmc_weights(self.ptr, _p(wt), C.byref(n)), "dmc_weights")
         k = n.value
         return wt[:k], self.get_walkers()[:k], self.scalars()["eold"][:k]
+
+    # ---------------------------------------------------------------- derivatives
+    def deriv_count(self, kinds: int) -> int:
+        n = C.c_int32()
+        self._check(self.L.champ_gpu_deriv_count(self.ptr, int(kinds), C.byref(n)), "deriv_count")
+        return n.value
+
+    def deriv_psi(self, kinds: int) -> np.ndarray:
+        """d ln psi / d p [nwalk][nparm] at the current walkers"""
+        o = np.zeros((self.nwalk, self.deriv_count(kinds)))
+        self._check(self.L.champ_gpu_deriv_psi(self.ptr, int(kinds), _p(o)), "deriv_psi")
+        return o
+
+    def sr_sums(self, kinds: int):
+        """(sum O O^T, sum O, sum E_L O, sum E_L) over the current walkers"""
+        n = self.deriv_count(kinds)
+        s, os_, eo = np.zeros((n, n)), np.zeros(n), np.zeros(n)
+        e = C.c_double()
+        self._check(self.L.champ_gpu_sr_sums(self.ptr, int(kinds), _p(s), _p(os_), _p(eo), C.byref(e)), "sr_sums")
+        return s, os_, eo, e.value
 
     def set_metrop6(self, deltar: float = 5.0, deltat: float = 1.0, fbias: float = 1.0):
         self._check(self.L.champ_gpu_set_metrop6(self.ptr, float(deltar), float(deltat), float(fbias)), "set_metrop6")

@@ -303,6 +303,34 @@ int champ_gpu_dmc(champ_ctx *ctx, int32_t nblk, int32_t nstep, int32_t nblkeq, d
 /* The live walker count and their weights [nwalk] (wt may be NULL).          */
 int champ_gpu_dmc_weights(champ_ctx *ctx, double *wt, int32_t *nwalk);
 
+/* --------------------------------------------------------------------------
+ * Parameter derivatives d ln psi / d p at the walkers' current configurations
+ * (deriv_jastrow4.f90:3-667 gvalue for the linear Jastrow parameters;
+ * optci.f90 for the determinant coefficients) and the population sums of
+ * stochastic reconfiguration (optwf_sr.f90:71-470).  They use the state the
+ * last hpsi left (metrop1/metrop6 sweeps end with one).  Parameter groups, in
+ * this order, selected by the bits of `kinds`:
+ *   CHAMP_DERIV_JAS_EN   a4[o] per centre type, o = 2..norda (polynomial part)
+ *   CHAMP_DERIV_JAS_EE   b[o] per spin class (nspin2b), o = 2..nordb
+ *   CHAMP_DERIV_JAS_EEN  c[t] per centre type, every term of order nordc in
+ *                        jastrow4.f90's order (no cusp elimination: a caller
+ *                        that keeps the cusp conditions maps these through
+ *                        its dependency matrix)
+ *   CHAMP_DERIV_CI       cdet[k] of every determinant (ndet > 1 only)
+ * ------------------------------------------------------------------------ */
+#define CHAMP_DERIV_JAS_EN  1
+#define CHAMP_DERIV_JAS_EE  2
+#define CHAMP_DERIV_JAS_EEN 4
+#define CHAMP_DERIV_CI      8
+int champ_gpu_deriv_count(champ_ctx *ctx, int32_t kinds, int32_t *nparm);
+
+/* O [nwalk][nparm]: d ln psi / d p of every walker.                         */
+int champ_gpu_deriv_psi(champ_ctx *ctx, int32_t kinds, double *o);
+
+/* Sums over the walkers of O_k O_l (s [nparm][nparm], on the matrix cores),
+ * O_k (osum [nparm]), E_L O_k (eosum [nparm]) and E_L (esum); any may be NULL. */
+int champ_gpu_sr_sums(champ_ctx *ctx, int32_t kinds, double *s, double *osum, double *eosum, double *esum);
+
 /* metrop1 over n populations at once, typically different molecules (each
  * context its own system, walker count and precision): every context's
  * kernels run on its own stream in the order of its own champ_gpu_metrop1,
