@@ -254,6 +254,47 @@ def read_determinants(path: str, nelec: int):
     return cdet, iworbd
 
 
+def read_csf(path: str):
+    """The csf / csfmap sections of a determinant file: (ccsf [ncsf], per CSF
+    the list of (determinant, coefficient), determinants 0-based), or None
+    when the file has no CSFs."""
+    r = _records(path)
+    if not r.skip_to("csf"):
+        return None
+    ncsf = r.keyword(1)[0]
+    ccsf = np.array(r.read(ncsf))
+    if not r.skip_to("csfmap"):
+        return None
+    w = r.keyword(3)
+    terms = []
+    for _ in range(w[0]):
+        nterm = r.read(1, int)[0]
+        t = []
+        for _ in range(nterm):
+            idt, cx = r.read(2)
+            t.append((int(idt) - 1, cx))
+        terms.append(t)
+    return ccsf, terms
+
+
+def read_jastrow_der(path: str, nctype: int, nspin2b: int = 1):
+    """The jasderiv block (read_data_parser.f90:1521-1700): which Jastrow
+    coefficients are optimised, CHAMP's 1-based indices per centre type (a,
+    c) and per spin class (b): dict(iwjasa, iwjasb, iwjasc) of lists."""
+    r = _records(path)
+    if not r.skip_to("jasderiv"):
+        raise ValueError(f"{path}: no jasderiv line")
+    r.line()
+    counts = r.read(2 * nctype + nspin2b + nctype, int)
+    nparma, nparmb = counts[:nctype], counts[nctype:nctype + nspin2b]
+    nparmc = counts[nctype + nspin2b:2 * nctype + nspin2b]
+    # list-directed reads: an empty list still takes its record
+    iwjasa = [r.read(n, int) for n in nparma]
+    iwjasb = [r.read(n, int) for n in nparmb]
+    iwjasc = [r.read(n, int) for n in nparmc]
+    return dict(iwjasa=iwjasa, iwjasb=iwjasb, iwjasc=iwjasc)
+
+
 def read_jastrow(path: str, nctype: int, nspin2b: int = 1):
     """ijas = 4 parameters: (norda, nordb, nordc, scalek, a4[nctype][norda+1],
     b[nspin2b][nordb+1], c[nctype][nterms4(nordc)])."""

@@ -151,6 +151,32 @@ static int noalloc_scratch(int dev, double **out)
     return CHAMP_OK;
 }
 
+// The Jastrow coefficients of JasParams from the system-array layouts
+// (a4 [nctype][norda+1], b [nspin2b][nordb+1], c [nctype][nterms]), with the
+// asymptotic values subtracted by jastrow4 (set_scale_dist, scale_dist.f90:3-60).
+static void jas_coeffs(JasParams &jp, const double *a4, const double *b, const double *cc)
+{
+    const double ar = jp.asymp_r;
+    const int nt = jp.nterms;
+    for (int it = 0; it < jp.nctype; it++) {
+        for (int p = 0; p <= jp.norda; p++) jp.a4[it][p] = a4[it * (jp.norda + 1) + p];
+        for (int p = 0; p < nt; p++) jp.c[it][p] = cc[it * nt + p];
+        double ja = jp.a4[it][0] * ar / (1 + jp.a4[it][1] * ar);
+        for (int iord = 2; iord <= jp.norda; iord++) ja = ja + jp.a4[it][iord] * hpowi(ar, iord);
+        jp.ja[it] = ja;
+    }
+    for (int isp = 0; isp < jp.nspin2b; isp++)
+        for (int p = 0; p <= jp.nordb; p++) jp.b[isp][p] = b[isp * (jp.nordb + 1) + p];
+    for (int i = 0; i < 2; i++) {
+        double sspinn; int isp;
+        if (i == 0) { sspinn = 1; isp = 0; }
+        else { sspinn = (jp.nspin2b == 1 && jp.nocuspb == 0) ? 0.5 : 1.0; isp = jp.nspin2b - 1; }
+        double jb = sspinn * jp.b[isp][0] * ar / (1 + jp.b[isp][1] * ar);
+        for (int iord = 2; iord <= jp.nordb; iord++) jb = jb + jp.b[isp][iord] * hpowi(ar, iord);
+        jp.jb[i] = jb;
+    }
+}
+
 extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t nwalk, int32_t precision)
 {
     if (!ctxp || !s || nwalk <= 0) { set_error("init: bad arguments"); return CHAMP_ERR_ARG; }
@@ -294,24 +320,7 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
             }
         }
     jp.nterms = nt;
-    const double ar = jp.asymp_r;
-    for (int it = 0; it < NCT; it++) {
-        for (int p = 0; p <= s->norda; p++) jp.a4[it][p] = s->a4[it * (s->norda + 1) + p];
-        for (int p = 0; p < nt; p++) jp.c[it][p] = s->c[it * nt + p];
-        double ja = jp.a4[it][0] * ar / (1 + jp.a4[it][1] * ar);
-        for (int iord = 2; iord <= s->norda; iord++) ja = ja + jp.a4[it][iord] * hpowi(ar, iord);
-        jp.ja[it] = ja;
-    }
-    for (int isp = 0; isp < s->nspin2b; isp++)
-        for (int p = 0; p <= s->nordb; p++) jp.b[isp][p] = s->b[isp * (s->nordb + 1) + p];
-    for (int i = 0; i < 2; i++) {
-        double sspinn; int isp;
-        if (i == 0) { sspinn = 1; isp = 0; }
-        else { sspinn = (s->nspin2b == 1 && s->nocuspb == 0) ? 0.5 : 1.0; isp = s->nspin2b - 1; }
-        double jb = sspinn * jp.b[isp][0] * ar / (1 + jp.b[isp][1] * ar);
-        for (int iord = 2; iord <= s->nordb; iord++) jb = jb + jp.b[isp][iord] * hpowi(ar, iord);
-        jp.jb[i] = jb;
-    }
+    jas_coeffs(jp, s->a4, s->b, s->c);
     TRY(upload(&c->jas, &c->hjas, 1));
     // ---- ECP
     EcpParams &ep = c->hecp;
@@ -450,6 +459,54 @@ extern "C" int champ_gpu_set_walkers(champ_ctx *c, const double *x)
     HIPCHECK(hipMemcpyAsync(c->xT, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
     return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_set_params(champ_ctx *c, const double *a4, const double *b, const double *cc,
+                                    const double *cdet)
+{
+    if (!c) return CHAMP_ERR_ARG;
+    JasParams jp = c->hjas;
+    // check_parms_jas (optwf_handle_wf.f90:1286-1336): the Pade denominators
+    // a4[1] and b[1] must stay above -scalek
+    const double scalem = -jp.kap;
+    if (a4)
+        for (int it = 0; it < jp.nctype; it++)
+            if (jp.norda >= 1 && a4[it * (jp.norda + 1) + 1] <= scalem) {
+                set_error("set_params: a4[1] <= -scalek");
+                return CHAMP_ERR_ARG;
+            }
+    if (b)
+        for (int isp = 0; isp < jp.nspin2b; isp++)
+            if (jp.nordb >= 1 && b[isp * (jp.nordb + 1) + 1] <= scalem) {
+                set_error("set_params: b[1] <= -scalek");
+                return CHAMP_ERR_ARG;
+            }
+    if (a4 || b || cc) {
+        std::vector<double> ha((size_t)jp.nctype * (jp.norda + 1)), hb((size_t)jp.nspin2b * (jp.nordb + 1)),
+            hc((size_t)jp.nctype * std::max(jp.nterms, 1));
+        for (int it = 0; it < jp.nctype; it++) {
+            for (int p = 0; p <= jp.norda; p++) ha[it * (jp.norda + 1) + p] = a4 ? a4[it * (jp.norda + 1) + p] : jp.a4[it][p];
+            for (int t = 0; t < jp.nterms; t++) hc[it * jp.nterms + t] = cc ? cc[it * jp.nterms + t] : jp.c[it][t];
+        }
+        for (int isp = 0; isp < jp.nspin2b; isp++)
+            for (int p = 0; p <= jp.nordb; p++) hb[isp * (jp.nordb + 1) + p] = b ? b[isp * (jp.nordb + 1) + p] : jp.b[isp][p];
+        jas_coeffs(jp, ha.data(), hb.data(), hc.data());
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->hjas = jp;
+        HIPCHECK(hipMemcpy(c->jas, &c->hjas, sizeof(JasParams), hipMemcpyHostToDevice));
+    }
+    if (cdet) {
+        if (c->md) {
+            if (int rc = md_set_cdet(c, cdet)) return rc;
+        } else {
+            c->cdet0 = cdet[0];
+        }
+    }
+    // kernel arguments captured in a sweep graph hold the old coefficients
+    if (c->sweep_exec) { hipGraphExecDestroy(c->sweep_exec); c->sweep_exec = nullptr; }
+    int rc = launch_hpsi_impl(c, 0);
+    if (!rc) HIPCHECK(hipStreamSynchronize(c->stream));
+    return rc;
 }
 
 extern "C" int champ_gpu_get_walkers(champ_ctx *c, double *x)

@@ -1,25 +1,36 @@
 // engine_deriv.hip -- derivatives of ln psi with respect to the wave
 // function's parameters at the walkers' current configurations (the gvalue
-// of deriv_jastrow4.f90:3-667 for the linear Jastrow parameters, and the
-// determinant ratios of optci.f90), and the population sums of stochastic
-// reconfiguration (optwf_sr.f90:71-470: <O_k>, <E_L O_k>, <O_k O_l>).
+// of deriv_jastrow4.f90:3-667 for the Jastrow coefficients, and the
+// determinant ratios of optci.f90:55-57), and the population sums of
+// stochastic reconfiguration (optwf_sr.f90:504-620: <O_k>, <E_L O_k>,
+// <O_k O_l>), and the SR parameter step of optwf_sr.f90:201-215 with the
+// preconditioned conjugate gradients of optwf_sr_more.f90:31-160.
 //
-// Parameters, in this order (kinds selects the groups):
-//   e-n   a4[o] of each centre type, o = 2..norda (the polynomial part):
-//         O = sum over electrons and centres of the type of R^o - R_inf^o
-//   e-e   b[o] of each spin class, o = 2..nordb: O = sum over pairs of u^o - u_inf^o
-//   e-e-n c_t of each centre type, every term t of the order-nordc
+// Raw parameters, in the order of the system arrays (kinds selects groups):
+//   e-n   a4[it][p], p = 0..norda, of each centre type it:
+//         p = 0: sum of R/(1 + a4[1] R) minus its value at R_inf,
+//         p = 1: sum of -a4[0] R^2/(1 + a4[1] R)^2 minus its value at R_inf,
+//         p >= 2: sum of R^p - R_inf^p, over electrons and centres of type it
+//   e-e   b[isb][p], p = 0..nordb, of each spin class, the same three forms in
+//         the pair's scaled distance u, the p = 0 and 1 forms times the pair's
+//         spin factor sspinn (1/2 for like spins when nspin2b = 1 and the
+//         cusp is imposed, else 1)
+//   e-e-n c[it][t] of each centre type, every term t of the order-nordc
 //         polynomial (jastrow4.f90:203-213 order, no cusp elimination):
 //         O = sum over pairs and centres of the type of u^k (x^l + y^l) (x y)^m
 //         with the switched distances u, x, y
-//   CI    cdet_k of every determinant (ndet > 1): O = D_up,k D_dn,k / psi_D
+//   CI    cdet[k] of every determinant (ndet > 1): O = D_up,k D_dn,k / psi_D
+// A parameter map (champ_gpu_set_deriv_map) turns these into a caller's own
+// parameters, e.g. CHAMP's iwjasa/iwjasb/iwjasc lists with the e-e-n cusp
+// dependencies and the CSF combinations: O' = O M.
 // The sums use the distances hpsi left in the walker state, so hpsi (or a
-// metrop1 sweep, which ends with one) must precede a call.
+// sweep, which ends with one) must precede a call.
 #include "een_poly.hpp"
 #include "engine_internal.hpp"
 #include "hpsi_args.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 namespace champ {
@@ -33,20 +44,40 @@ static DerLayout der_layout(const champ_ctx *c, int kinds)
 {
     const JasParams &j = c->hjas;
     DerLayout L;
-    L.na = (kinds & CHAMP_DERIV_JAS_EN) ? j.nctype * std::max(0, j.norda - 1) : 0;
-    L.nb = (kinds & CHAMP_DERIV_JAS_EE) ? j.nspin2b * std::max(0, j.nordb - 1) : 0;
+    L.na = (kinds & CHAMP_DERIV_JAS_EN) ? j.nctype * (j.norda + 1) : 0;
+    L.nb = (kinds & CHAMP_DERIV_JAS_EE) ? j.nspin2b * (j.nordb + 1) : 0;
     L.nc = (kinds & CHAMP_DERIV_JAS_EEN) ? j.nctype * j.nterms : 0;
     L.nci = ((kinds & CHAMP_DERIV_CI) && c->md) ? c->md_ndet : 0;
     L.n = L.na + L.nb + L.nc + L.nci;
     return L;
 }
 
+static void deriv_free_scratch(champ_ctx *c)
+{
+    void *ps[] = {c->der_ot, c->der_s, c->der_v, c->der_om};
+    for (void *p : ps) if (p) hipFree(p);
+    c->der_ot = c->der_s = c->der_v = c->der_om = nullptr;
+    c->der_cap = c->der_scap = c->der_om_cap = 0;
+}
+
 void deriv_free(champ_ctx *c)
 {
-    void *ps[] = {c->der_ot, c->der_s, c->der_v};
-    for (void *p : ps) if (p) hipFree(p);
-    c->der_ot = c->der_s = c->der_v = nullptr;
-    c->der_cap = 0;
+    deriv_free_scratch(c);
+    if (c->der_map) hipFree(c->der_map);
+    c->der_map = nullptr;
+    c->der_map_kinds = c->der_map_nraw = c->der_map_nout = 0;
+}
+
+// d/dp of the e-n (e-e) form f(r) = q0 r / (1 + q1 r) + sum_{p>=2} q_p r^p
+// at the scaled distance r: p = 0 and 1 are the Pade coefficients.
+__device__ __forceinline__ double pade_deriv(int p, double q0, double q1, double r)
+{
+    if (p == 0) return r / (1.0 + q1 * r);
+    if (p == 1) {
+        const double bot = 1.0 + q1 * r;
+        return -q0 * r * r / (bot * bot);
+    }
+    return powi(r, p);
 }
 
 struct DerArgs {
@@ -74,29 +105,33 @@ __global__ void __launch_bounds__(256) k_deriv_jastrow(DerArgs a)
     const double *rrw = a.rr_en + (size_t)w * NC * NE;
     const double *xw = a.xT + (size_t)w * 3 * NE;
     int off = 0;
-    // e-n: a4[o] of type it
+    // e-n: a4[p] of type it
     if (a.na) {
-        for (int it = 0; it < jp.nctype; it++)
-            for (int o = 2; o <= jp.norda; o++) {
+        for (int it = 0; it < jp.nctype; it++) {
+            const double a0 = jp.a4[it][0], a1 = jp.a4[it][1];
+            for (int p = 0; p <= jp.norda; p++) {
                 double s = 0.0;
-                const double pinf = powi(ainf, o);
+                const double pinf = pade_deriv(p, a0, a1, ainf);
                 for (int t = tid; t < NC * NE; t += bd) {
                     const int cc = t / NE;
                     if (a.iwctype[cc] != it) continue;
-                    s += powi(rrw[t], o) - pinf;
+                    s += pade_deriv(p, a0, a1, rrw[t]) - pinf;
                 }
                 s = block_sum(s, red);
-                if (tid == 0) a.ot[(size_t)(off + it * (jp.norda - 1) + (o - 2)) * a.W + w] = s;
+                if (tid == 0) a.ot[(size_t)(off + it * (jp.norda + 1) + p) * a.W + w] = s;
             }
+        }
         off += a.na;
     }
     const int npair = NE * (NE - 1) / 2;
-    // e-e: b[o] of spin class isb (like-spin pairs are class 1 when nspin2b = 2)
+    // e-e: b[p] of spin class isb (like-spin pairs are class 1 when nspin2b = 2)
     if (a.nb) {
-        for (int isb = 0; isb < jp.nspin2b; isb++)
-            for (int o = 2; o <= jp.nordb; o++) {
+        const double slike = (jp.nspin2b == 1 && jp.nocuspb == 0) ? 0.5 : 1.0;
+        for (int isb = 0; isb < jp.nspin2b; isb++) {
+            const double b0 = jp.b[isb][0], b1 = jp.b[isb][1];
+            for (int p = 0; p <= jp.nordb; p++) {
                 double s = 0.0;
-                const double pinf = powi(ainf, o);
+                const double pinf = pade_deriv(p, b0, b1, ainf);
                 for (int t = tid; t < npair; t += bd) {
                     // pair t = i (i - 1) / 2 + j, j < i
                     int i = (int)((1.0 + sqrt(1.0 + 8.0 * t)) * 0.5);
@@ -108,11 +143,13 @@ __global__ void __launch_bounds__(256) k_deriv_jastrow(DerArgs a)
                     const double d0 = xw[i] - xw[j], d1 = xw[NE + i] - xw[NE + j], d2 = xw[2 * NE + i] - xw[2 * NE + j];
                     double r2 = 0;
                     r2 = r2 + d0 * d0; r2 = r2 + d1 * d1; r2 = r2 + d2 * d2;
-                    s += powi(scale_r(kap, sqrt(r2)), o) - pinf;
+                    const double g = pade_deriv(p, b0, b1, scale_r(kap, sqrt(r2))) - pinf;
+                    s += (p < 2 && like) ? slike * g : g;
                 }
                 s = block_sum(s, red);
-                if (tid == 0) a.ot[(size_t)(off + isb * (jp.nordb - 1) + (o - 2)) * a.W + w] = s;
+                if (tid == 0) a.ot[(size_t)(off + isb * (jp.nordb + 1) + p) * a.W + w] = s;
             }
+        }
         off += a.nb;
     }
     // e-e-n: c_t of type it
@@ -181,17 +218,40 @@ __global__ void k_deriv_sums(int n, int W, const double *ot, const double *eold,
     v[n + k] = se;
 }
 
-static int deriv_compute(champ_ctx *c, int kinds, DerLayout &L)
+template <typename T>
+static int grow(T **p, int *cap, int need, size_t per)
 {
-    L = der_layout(c, kinds);
+    if (need <= *cap) return CHAMP_OK;
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    HIPCHECK(hipMalloc((void **)p, (size_t)need * per));
+    *cap = need;
+    return CHAMP_OK;
+}
+
+// O of the selected parameters, parameter-major: *ot = [n][W] on the device.
+// With CHAMP_DERIV_MAPPED the raw parameters are the map's, and the result is
+// Mt [nout][nraw] x O [nraw][W] (a matrix-core GEMM).
+static int deriv_compute(champ_ctx *c, int kinds, const double **ot, int *n)
+{
+    const bool mapped = (kinds & CHAMP_DERIV_MAPPED) != 0;
+    if (mapped && !c->der_map) { set_error("deriv: CHAMP_DERIV_MAPPED without a parameter map"); return CHAMP_ERR_ARG; }
+    const int raw = mapped ? c->der_map_kinds : kinds;
+    DerLayout L = der_layout(c, raw);
     if (L.n == 0) { set_error("deriv: no parameters selected (kinds) for this trial function"); return CHAMP_ERR_ARG; }
     if (c->hjas.nordc > 5) { set_error("deriv: e-e-n order above 5 is not supported"); return CHAMP_ERR_UNSUPPORTED; }
-    if (L.n > c->der_cap) {
-        deriv_free(c);
-        HIPCHECK(hipMalloc(&c->der_ot, (size_t)L.n * c->W * sizeof(double)));
-        HIPCHECK(hipMalloc(&c->der_s, (size_t)L.n * L.n * sizeof(double)));
-        HIPCHECK(hipMalloc(&c->der_v, (size_t)2 * L.n * sizeof(double)));
-        c->der_cap = L.n;
+    const int nout = mapped ? c->der_map_nout : L.n;
+    int rc;
+    if ((rc = grow(&c->der_ot, &c->der_cap, L.n, (size_t)c->W * sizeof(double)))) return rc;
+    if (nout > c->der_scap) {
+        if (c->der_s) hipFree(c->der_s);
+        if (c->der_v) hipFree(c->der_v);
+        c->der_s = c->der_v = nullptr;
+        c->der_scap = 0;
+        HIPCHECK(hipMalloc(&c->der_s, (size_t)nout * nout * sizeof(double)));
+        HIPCHECK(hipMalloc(&c->der_v, (size_t)2 * nout * sizeof(double)));
+        c->der_scap = nout;
     }
     DerArgs a;
     a.W = c->W; a.NE = c->NE; a.NUP = c->NUP; a.NC = c->NC;
@@ -215,7 +275,89 @@ static int deriv_compute(champ_ctx *c, int kinds, DerLayout &L)
                            L.na + L.nb + L.nc);
         HIPCHECK(hipGetLastError());
     }
+    if (mapped) {
+        if ((rc = grow(&c->der_om, &c->der_om_cap, nout, (size_t)c->W * sizeof(double)))) return rc;
+        if ((rc = launch_gemm(c, false, nout, c->W, L.n, c->der_map, L.n, 0, c->der_ot, c->W, 0, c->der_om, c->W, 0, 1)))
+            return rc;
+        *ot = c->der_om;
+    } else {
+        *ot = c->der_ot;
+    }
+    *n = nout;
     return CHAMP_OK;
+}
+
+// ---- SR step (host): sr_hs, optwf_sr.f90:504-620 with one state and the
+// walker weights summed in wsum; pcg + asolve + atimes_n,
+// optwf_sr_more.f90:31-160, on the dense S of the sums.
+struct SrSystem {
+    int n;
+    std::vector<double> oo, o, s_diag, s_ii_inv, h;  // oo [n][n] = <O_k O_l>, o = <O_k>
+};
+
+static void sr_hs(SrSystem &A, double adiag, const std::vector<double> &eo, double e)
+{
+    const int n = A.n;
+    double smax = 0.0;
+    for (int k = 0; k < n; k++) {
+        const double aux = A.oo[(size_t)k * n + k] - A.o[k] * A.o[k];
+        A.s_diag[k] = aux * adiag;
+        A.s_ii_inv[k] = aux + A.s_diag[k];
+        A.h[k] = -2.0 * (eo[k] - A.o[k] * e);
+        if (A.s_ii_inv[k] > smax) smax = A.s_ii_inv[k];
+    }
+    for (int k = 0; k < n; k++)
+        A.s_ii_inv[k] = (A.s_ii_inv[k] / smax > 1e-14) ? 1.0 / A.s_ii_inv[k] : 0.0;
+}
+
+// r = S z with S_kl = <O_k O_l> - <O_k><O_l> + s_diag_k delta_kl
+static void sr_atimes(const SrSystem &A, const double *z, double *r)
+{
+    const int n = A.n;
+    double oz = 0.0;
+    for (int k = 0; k < n; k++) oz += A.o[k] * z[k];
+    for (int k = 0; k < n; k++) {
+        double t = 0.0;
+        const double *row = &A.oo[(size_t)k * n];
+        for (int l = 0; l < n; l++) t += row[l] * z[l];
+        r[k] = t - A.o[k] * oz + A.s_diag[k] * z[k];
+    }
+}
+
+static int sr_pcg(const SrSystem &A, double eps, std::vector<double> &x)
+{
+    const int n = A.n, imax = n, imod = 50;
+    std::vector<double> r(n), d(n), q(n), sv(n);
+    auto dot = [n](const double *u, const double *v) {
+        double t = 0.0;
+        for (int k = 0; k < n; k++) t += u[k] * v[k];
+        return t;
+    };
+    x.assign(n, 0.0);
+    sr_atimes(A, x.data(), r.data());
+    for (int k = 0; k < n; k++) r[k] = A.h[k] - r[k];
+    for (int k = 0; k < n; k++) d[k] = r[k] * A.s_ii_inv[k];
+    double delta_new = dot(d.data(), r.data());
+    const double delta_0 = delta_new * eps * eps;
+    int i = 0;
+    for (; i < imax; i++) {
+        if (delta_new < delta_0) return i;
+        sr_atimes(A, d.data(), q.data());
+        const double alpha = delta_new / dot(d.data(), q.data());
+        for (int k = 0; k < n; k++) x[k] += alpha * d[k];
+        if (i % imod == 0) {
+            sr_atimes(A, x.data(), r.data());
+            for (int k = 0; k < n; k++) r[k] = A.h[k] - r[k];
+        } else {
+            for (int k = 0; k < n; k++) r[k] -= alpha * q[k];
+        }
+        for (int k = 0; k < n; k++) sv[k] = r[k] * A.s_ii_inv[k];
+        const double delta_old = delta_new;
+        delta_new = dot(r.data(), sv.data());
+        const double beta = delta_new / delta_old;
+        for (int k = 0; k < n; k++) d[k] = sv[k] + beta * d[k];
+    }
+    return i;
 }
 
 }  // namespace champ
@@ -225,22 +367,49 @@ using namespace champ;
 extern "C" int champ_gpu_deriv_count(champ_ctx *c, int32_t kinds, int32_t *nparm)
 {
     if (!c || !nparm) return CHAMP_ERR_ARG;
-    *nparm = der_layout(c, kinds).n;
+    if (kinds & CHAMP_DERIV_MAPPED) {
+        if (!c->der_map) { set_error("deriv: CHAMP_DERIV_MAPPED without a parameter map"); return CHAMP_ERR_ARG; }
+        *nparm = c->der_map_nout;
+    } else {
+        *nparm = der_layout(c, kinds).n;
+    }
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_set_deriv_map(champ_ctx *c, int32_t kinds, int32_t nout, const double *map)
+{
+    if (!c) return CHAMP_ERR_ARG;
+    if (c->der_map) hipFree(c->der_map);
+    c->der_map = nullptr;
+    c->der_map_kinds = c->der_map_nraw = c->der_map_nout = 0;
+    if (!map || nout == 0) return CHAMP_OK;
+    if (nout < 0 || (kinds & ~15)) { set_error("set_deriv_map: bad kinds or nout"); return CHAMP_ERR_ARG; }
+    const int nraw = der_layout(c, kinds).n;
+    if (nraw == 0) { set_error("set_deriv_map: no parameters selected (kinds) for this trial function"); return CHAMP_ERR_ARG; }
+    std::vector<double> mt((size_t)nout * nraw);
+    for (int k = 0; k < nraw; k++)
+        for (int j = 0; j < nout; j++) mt[(size_t)j * nraw + k] = map[(size_t)k * nout + j];
+    HIPCHECK(hipMalloc(&c->der_map, mt.size() * sizeof(double)));
+    HIPCHECK(hipMemcpy(c->der_map, mt.data(), mt.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->der_map_kinds = kinds;
+    c->der_map_nraw = nraw;
+    c->der_map_nout = nout;
     return CHAMP_OK;
 }
 
 extern "C" int champ_gpu_deriv_psi(champ_ctx *c, int32_t kinds, double *o)
 {
     if (!c) return CHAMP_ERR_ARG;
-    DerLayout L;
-    int rc = deriv_compute(c, kinds, L);
+    const double *ot = nullptr;
+    int n = 0;
+    int rc = deriv_compute(c, kinds, &ot, &n);
     if (rc) return rc;
     if (o) {
-        std::vector<double> h((size_t)L.n * c->W);
-        HIPCHECK(hipMemcpyAsync(h.data(), c->der_ot, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        std::vector<double> h((size_t)n * c->W);
+        HIPCHECK(hipMemcpyAsync(h.data(), ot, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
         for (int w = 0; w < c->W; w++)
-            for (int k = 0; k < L.n; k++) o[(size_t)w * L.n + k] = h[(size_t)k * c->W + w];
+            for (int k = 0; k < n; k++) o[(size_t)w * n + k] = h[(size_t)k * c->W + w];
     }
     return CHAMP_OK;
 }
@@ -248,16 +417,17 @@ extern "C" int champ_gpu_deriv_psi(champ_ctx *c, int32_t kinds, double *o)
 extern "C" int champ_gpu_sr_sums(champ_ctx *c, int32_t kinds, double *s, double *osum, double *eosum, double *esum)
 {
     if (!c) return CHAMP_ERR_ARG;
-    DerLayout L;
-    int rc = deriv_compute(c, kinds, L);
+    const double *ot = nullptr;
+    int n = 0;
+    int rc = deriv_compute(c, kinds, &ot, &n);
     if (rc) return rc;
-    const int n = L.n, W = c->W;
+    const int W = c->W;
     // <O_k O_l> numerators: the parameter-major O times its transpose on the matrix cores
     if (s) {
-        if ((rc = launch_gemm(c, true, n, n, W, c->der_ot, W, 0, c->der_ot, W, 0, c->der_s, n, 0, 1))) return rc;
+        if ((rc = launch_gemm(c, true, n, n, W, ot, W, 0, ot, W, 0, c->der_s, n, 0, 1))) return rc;
         HIPCHECK(hipMemcpyAsync(s, c->der_s, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     }
-    hipLaunchKernelGGL(k_deriv_sums, dim3((n + 63) / 64), dim3(64), 0, c->stream, n, W, c->der_ot, c->eold, c->der_v);
+    hipLaunchKernelGGL(k_deriv_sums, dim3((n + 63) / 64), dim3(64), 0, c->stream, n, W, ot, c->eold, c->der_v);
     HIPCHECK(hipGetLastError());
     std::vector<double> v((size_t)2 * n), e(W);
     HIPCHECK(hipMemcpyAsync(v.data(), c->der_v, v.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -270,5 +440,41 @@ extern "C" int champ_gpu_sr_sums(champ_ctx *c, int32_t kinds, double *s, double 
         for (int w = 0; w < W; w++) t += e[w];
         *esum = t;
     }
+    return CHAMP_OK;
+}
+
+extern "C" int champ_sr_solve(int32_t nparm, double wsum, const double *s, const double *osum, const double *eosum,
+                              double esum, double sr_tau, double sr_eps, double sr_adiag, double dparm_norm_min,
+                              double *dparm, champ_sr_info *info)
+{
+    if (nparm <= 0 || !(wsum > 0.0) || !s || !osum || !eosum || !dparm) { set_error("sr_solve: bad arguments"); return CHAMP_ERR_ARG; }
+    SrSystem A;
+    const int n = nparm;
+    A.n = n;
+    A.oo.resize((size_t)n * n);
+    A.o.resize(n);
+    A.s_diag.resize(n);
+    A.s_ii_inv.resize(n);
+    A.h.resize(n);
+    std::vector<double> eo(n);
+    for (size_t k = 0; k < (size_t)n * n; k++) A.oo[k] = s[k] / wsum;
+    for (int k = 0; k < n; k++) { A.o[k] = osum[k] / wsum; eo[k] = eosum[k] / wsum; }
+    const double e = esum / wsum;
+    double adiag = sr_adiag, norm = 0.0;
+    int iters = 0, tries = 0;
+    std::vector<double> x;
+    for (;; tries++) {
+        sr_hs(A, adiag, eo, e);
+        iters = sr_pcg(A, sr_eps, x);
+        norm = 0.0;
+        for (int k = 0; k < n; k++) { dparm[k] = -sr_tau * x[k]; norm += dparm[k] * dparm[k]; }
+        norm = std::sqrt(norm / n);
+        // test_solution_parm (optwf_handle_wf.f90:1338-1365): a step whose rms
+        // exceeds dparm_norm_min is redone with ten times the diagonal shift
+        if (!(adiag > 0.0) || norm <= dparm_norm_min) break;
+        if (tries == 30) { set_error("sr_solve: no step below dparm_norm_min"); return CHAMP_ERR_NUMERIC; }
+        adiag *= 10.0;
+    }
+    if (info) { info->adiag = adiag; info->dparm_norm = norm; info->cg_iters = iters; info->retries = tries; }
     return CHAMP_OK;
 }

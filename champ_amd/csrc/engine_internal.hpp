@@ -123,7 +123,11 @@ struct champ_ctx {
     // ---- parameter derivatives (engine_deriv.hip): d ln psi / d p of the
     // current walkers, parameter-major [nparm][W], and the SR sums' scratch
     double *der_ot = nullptr, *der_s = nullptr, *der_v = nullptr;
-    int der_cap = 0;
+    int der_cap = 0, der_scap = 0;
+    // parameter map (champ_gpu_set_deriv_map): Mt [nout][nraw] over the raw
+    // parameters of der_map_kinds, and the mapped O [nout][W]
+    double *der_map = nullptr, *der_om = nullptr;
+    int der_map_kinds = 0, der_map_nraw = 0, der_map_nout = 0, der_om_cap = 0;
     unsigned char *acc = nullptr;
     double *esum_dev = nullptr;  // [4] accepted, esum, esum2, nflag
 
@@ -144,6 +148,8 @@ struct champ_ctx {
     int *md_du = nullptr, *md_dd = nullptr;   // [ndet] unique spin determinant of each det
     double *md_c = nullptr;                   // [ndet] cdet with the reordering phase
     double *md_ph = nullptr;                  // [ndet] the reordering phase (+-1) of each determinant
+    std::vector<double> md_hph;               // host copy of md_ph (coefficient updates)
+    std::vector<int> md_hwdet[2];             // determinant of each w-matrix entry per spin (md_wcf order)
     int *md_rank[2] = {nullptr, nullptr}, *md_hole[2] = {nullptr, nullptr}, *md_part[2] = {nullptr, nullptr};  // [nu][4]
     int *md_virt[2] = {nullptr, nullptr};     // [nv] orbital of each particle index
     int *md_woff[2] = {nullptr, nullptr}, *md_woth[2] = {nullptr, nullptr};  // dets of each spin determinant (CSR):
@@ -210,6 +216,7 @@ void dmc_free(champ_ctx *c);
 void deriv_free(champ_ctx *c);
 int md_init(champ_ctx *c, const champ_system_t *s);
 void md_free(champ_ctx *c);
+int md_set_cdet(champ_ctx *c, const double *cdet);
 int launch_md_hpsi(champ_ctx *c, int w0, int nw);
 int launch_md_pre(champ_ctx *c, int iel);
 int launch_md_post(champ_ctx *c, int iel);

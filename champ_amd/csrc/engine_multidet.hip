@@ -681,6 +681,7 @@ int md_init(champ_ctx *c, const champ_system_t *s)
         std::vector<double> wcf(ndet);
         std::vector<int> woth(ndet);
         for (int e = 0; e < ndet; e++) { wcf[e] = cd[wdet[e]]; woth[e] = uidx[1 - sp][wdet[e]]; }
+        c->md_hwdet[sp] = wdet;
         // cofactor entries gathered into each cell (v, j) of Y (CSR over v * NM + j)
         std::vector<std::vector<int>> cell((size_t)NV * NM);
         for (int u = 1; u < nu; u++)
@@ -721,6 +722,7 @@ int md_init(champ_ctx *c, const champ_system_t *s)
     if ((rc = md_upload(&c->md_dd, uidx[1]))) return rc;
     if ((rc = md_upload(&c->md_c, cd))) return rc;
     if ((rc = md_upload(&c->md_ph, ph))) return rc;
+    c->md_hph = ph;
     const size_t NU = c->md_NU;
     if ((rc = md_zeros(&c->md_aa, (size_t)W * 4 * NM * NV))) return rc;
     if ((rc = md_zeros(&c->md_D, (size_t)W * 4 * NU))) return rc;
@@ -736,6 +738,26 @@ int md_init(champ_ctx *c, const champ_system_t *s)
     if ((rc = md_zeros(&c->md_Z, (size_t)W * NV * NM))) return rc;
     if ((rc = md_zeros(&c->md_Wt, (size_t)W * NU))) return rc;
     if ((rc = md_zeros(&c->md_g, (size_t)W * NV))) return rc;
+    return CHAMP_OK;
+}
+
+// New determinant coefficients: the reordering phases and the w-matrix
+// order of init stay, only the coefficient tables are rewritten.
+int md_set_cdet(champ_ctx *c, const double *cdet)
+{
+    const int ndet = c->md_ndet;
+    std::vector<double> cd(ndet);
+    for (int k = 0; k < ndet; k++) cd[k] = cdet[k] * c->md_hph[k];
+    std::vector<double> wcf[2];
+    for (int sp = 0; sp < 2; sp++) {
+        wcf[sp].resize(ndet);
+        for (int e = 0; e < ndet; e++) wcf[sp][e] = cd[c->md_hwdet[sp][e]];
+    }
+    HIPCHECK(hipMemcpyAsync(c->md_c, cd.data(), ndet * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    for (int sp = 0; sp < 2; sp++)
+        HIPCHECK(hipMemcpyAsync(c->md_wcf[sp], wcf[sp].data(), ndet * sizeof(double), hipMemcpyHostToDevice,
+                                c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
     return CHAMP_OK;
 }
 

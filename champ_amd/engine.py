@@ -40,7 +40,11 @@ class VmcResult(C.Structure):
                 ("accept", C.c_double), ("nsamples", C.c_int64), ("nflagged", C.c_int64)]
 
 
-DERIV_JAS_EN, DERIV_JAS_EE, DERIV_JAS_EEN, DERIV_CI = 1, 2, 4, 8
+DERIV_JAS_EN, DERIV_JAS_EE, DERIV_JAS_EEN, DERIV_CI, DERIV_MAPPED = 1, 2, 4, 8, 16
+
+
+class SrInfo(C.Structure):
+    _fields_ = [("adiag", C.c_double), ("dparm_norm", C.c_double), ("cg_iters", C.c_int32), ("retries", C.c_int32)]
 
 
 class DmcStepStats(C.Structure):
@@ -95,6 +99,10 @@ def load_library(path: str = LIBPATH):
     L.champ_gpu_deriv_count.argtypes = [vp, i32, C.POINTER(i32)]
     L.champ_gpu_deriv_psi.argtypes = [vp, i32, dp]
     L.champ_gpu_sr_sums.argtypes = [vp, i32, dp, dp, dp, dp]
+    L.champ_gpu_set_deriv_map.argtypes = [vp, i32, i32, dp]
+    L.champ_gpu_set_params.argtypes = [vp, dp, dp, dp, dp]
+    L.champ_sr_solve.argtypes = [i32, C.c_double, dp, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, dp, C.POINTER(SrInfo)]
     L.champ_gpu_set_metrop6.argtypes = [vp, C.c_double, C.c_double, C.c_double]
     L.champ_gpu_get_walker_scalars.argtypes = [vp, dp, dp, dp, dp]
     L.champ_gpu_get_status.argtypes = [vp, C.POINTER(C.c_int32)]
@@ -117,6 +125,26 @@ def load_library(path: str = LIBPATH):
 
 def _p(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def sr_solve(s, osum, eosum, esum: float, wsum: float, sr_tau: float = 0.02, sr_eps: float = 1e-3,
+             sr_adiag: float = 0.01, dparm_norm_min: float = 1.0, lib_path: str = LIBPATH):
+    """The SR step from the population sums (champ_sr_solve, host only):
+    returns (dparm, info); new parameters are p - dparm."""
+    L = load_library(lib_path)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    osum = np.ascontiguousarray(osum, dtype=np.float64)
+    eosum = np.ascontiguousarray(eosum, dtype=np.float64)
+    n = len(osum)
+    if s.shape != (n, n) or eosum.shape != (n,):
+        raise ValueError("sr_solve: s must be [n][n] and osum, eosum [n]")
+    dparm = np.zeros(n)
+    info = SrInfo()
+    rc = L.champ_sr_solve(n, float(wsum), _p(s), _p(osum), _p(eosum), float(esum), float(sr_tau), float(sr_eps),
+                          float(sr_adiag), float(dparm_norm_min), _p(dparm), C.byref(info))
+    if rc != 0:
+        raise RuntimeError(f"sr_solve failed ({rc}): {L.champ_gpu_last_error().decode()}")
+    return dparm, dict(adiag=info.adiag, dparm_norm=info.dparm_norm, cg_iters=info.cg_iters, retries=info.retries)
 
 
 class Engine:
@@ -262,6 +290,27 @@ class Engine:
         o = np.zeros((self.nwalk, self.deriv_count(kinds)))
         self._check(self.L.champ_gpu_deriv_psi(self.ptr, int(kinds), _p(o)), "deriv_psi")
         return o
+
+    def set_deriv_map(self, kinds: int, m: Optional[np.ndarray]):
+        """O' = O m for deriv_psi / sr_sums called with DERIV_MAPPED; m [nraw][nout]
+        over the raw parameters of kinds (None removes the map)"""
+        if m is None:
+            self._check(self.L.champ_gpu_set_deriv_map(self.ptr, int(kinds), 0, None), "set_deriv_map")
+            return
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        if m.ndim != 2 or m.shape[0] != self.deriv_count(kinds):
+            raise ValueError("set_deriv_map: m must be [nraw][nout] for the raw parameters of kinds")
+        self._check(self.L.champ_gpu_set_deriv_map(self.ptr, int(kinds), m.shape[1], _p(m)), "set_deriv_map")
+
+    def set_params(self, a4=None, b=None, c=None, cdet=None):
+        """New Jastrow / determinant coefficients (system-array layouts; None keeps);
+        the walkers' state is recomputed"""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (a4, b, c, cdet)]
+        sy = self.system
+        for a, ref in zip(arrs, (sy.a4, sy.b, sy.c, sy.cdet)):
+            if a is not None and a.size != np.asarray(ref).size:
+                raise ValueError("set_params: array size differs from the system's")
+        self._check(self.L.champ_gpu_set_params(self.ptr, *[None if a is None else _p(a) for a in arrs]), "set_params")
 
     def sr_sums(self, kinds: int):
         """(sum O O^T, sum O, sum E_L O, sum E_L) over the current walkers"""

@@ -41,6 +41,28 @@ module champ_gpu_mod
     integer(c_int64_t) :: nflagged
   end type
 
+  type, bind(c), public :: champ_vmc_result
+    real(c_double) :: efin, eerr, sigma, tcorr, accept
+    integer(c_int64_t) :: nsamples, nflagged
+  end type
+
+  type, bind(c), public :: champ_dmc_step_stats
+    integer(c_int32_t) :: nwalk, pad
+    real(c_double) :: wsum, esum, wgsum, egsum, eest, eigv, fprod, accept
+    integer(c_int64_t) :: nodecr, nbranch
+  end type
+
+  type, bind(c), public :: champ_sr_info
+    real(c_double) :: adiag, dparm_norm
+    integer(c_int32_t) :: cg_iters, retries
+  end type
+
+  type, bind(c), public :: champ_dmc_result
+    real(c_double) :: egave, egerr, eest, accept, tau_eff
+    integer(c_int32_t) :: nwalk, nwalk_max
+    integer(c_int64_t) :: nbranch, nodecr
+  end type
+
   interface
     integer(c_int) function champ_gpu_init(ctx, sys, nwalk, precision) bind(c, name="champ_gpu_init")
       import :: c_int, c_ptr, c_int32_t, champ_system_t
@@ -130,12 +152,105 @@ module champ_gpu_mod
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: psido(*), psijo(*), eold(*), psi2o(*)
     end function
+    integer(c_int) function champ_gpu_metrop6(ctx, nsweep, stats) bind(c, name="champ_gpu_metrop6")
+      import :: c_int, c_ptr, c_int32_t, champ_sweep_stats
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsweep
+      type(champ_sweep_stats), intent(out) :: stats
+    end function
+    integer(c_int) function champ_gpu_set_metrop6(ctx, deltar, deltat, fbias) bind(c, name="champ_gpu_set_metrop6")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltar, deltat, fbias
+    end function
+    integer(c_int) function champ_gpu_vmc(ctx, nblk, nstep, nblkeq, tau, imetro, enow, res) &
+        bind(c, name="champ_gpu_vmc")
+      import :: c_int, c_ptr, c_int32_t, c_double, champ_vmc_result
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nblk, nstep, nblkeq
+      real(c_double), value :: tau
+      integer(c_int32_t), value :: imetro
+      real(c_double), intent(out) :: enow(*)
+      type(champ_vmc_result), intent(out) :: res
+    end function
+    integer(c_int) function champ_gpu_dmc_init(ctx, nconf, tau, etrial, nfprod, pop_state) &
+        bind(c, name="champ_gpu_dmc_init")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nconf
+      real(c_double), value :: tau, etrial
+      integer(c_int32_t), value :: nfprod
+      integer(c_int64_t), intent(in) :: pop_state(4)
+    end function
+    integer(c_int) function champ_gpu_dmc_step(ctx, stats) bind(c, name="champ_gpu_dmc_step")
+      import :: c_int, c_ptr, champ_dmc_step_stats
+      type(c_ptr), value :: ctx
+      type(champ_dmc_step_stats), intent(out) :: stats
+    end function
+    integer(c_int) function champ_gpu_dmc(ctx, nblk, nstep, nblkeq, egnow, res) bind(c, name="champ_gpu_dmc")
+      import :: c_int, c_ptr, c_int32_t, c_double, champ_dmc_result
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nblk, nstep, nblkeq
+      real(c_double), intent(out) :: egnow(*)
+      type(champ_dmc_result), intent(out) :: res
+    end function
+    integer(c_int) function champ_gpu_dmc_weights(ctx, wt, nwalk) bind(c, name="champ_gpu_dmc_weights")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: wt(*)
+      integer(c_int32_t), intent(out) :: nwalk
+    end function
+    integer(c_int) function champ_gpu_deriv_count(ctx, kinds, nparm) bind(c, name="champ_gpu_deriv_count")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: kinds
+      integer(c_int32_t), intent(out) :: nparm
+    end function
+    integer(c_int) function champ_gpu_deriv_psi(ctx, kinds, o) bind(c, name="champ_gpu_deriv_psi")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: kinds
+      real(c_double), intent(out) :: o(*)        ! (nparm, nwalk)
+    end function
+    integer(c_int) function champ_gpu_sr_sums(ctx, kinds, s, osum, eosum, esum) bind(c, name="champ_gpu_sr_sums")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: kinds
+      real(c_double), intent(out) :: s(*), osum(*), eosum(*)
+      real(c_double), intent(out) :: esum
+    end function
+    integer(c_int) function champ_gpu_set_deriv_map(ctx, kinds, nout, map) bind(c, name="champ_gpu_set_deriv_map")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: kinds, nout
+      real(c_double), intent(in) :: map(*)      ! (nout, nraw)
+    end function
+    integer(c_int) function champ_gpu_set_params(ctx, a4, b, c, cdet) bind(c, name="champ_gpu_set_params")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: a4, b, c, cdet       ! c_loc of the arrays, or c_null_ptr to keep
+    end function
+    integer(c_int) function champ_sr_solve(nparm, wsum, s, osum, eosum, esum, sr_tau, sr_eps, sr_adiag, &
+        dparm_norm_min, dparm, info) bind(c, name="champ_sr_solve")
+      import :: c_int, c_int32_t, c_double, champ_sr_info
+      integer(c_int32_t), value :: nparm
+      real(c_double), value :: wsum
+      real(c_double), intent(in) :: s(*), osum(*), eosum(*)
+      real(c_double), value :: esum, sr_tau, sr_eps, sr_adiag, dparm_norm_min
+      real(c_double), intent(out) :: dparm(*)
+      type(champ_sr_info), intent(out) :: info
+    end function
   end interface
 
   public :: champ_gpu_init, champ_gpu_free, champ_gpu_set_walkers, champ_gpu_get_walkers, champ_gpu_seed_rng
   public :: champ_gpu_rotqua, champ_gpu_hpsi, champ_gpu_compute_determinante_grad, champ_gpu_psie
   public :: champ_gpu_accept, champ_gpu_metrop1, champ_gpu_get_walker_scalars, champ_gpu_autotune
-  public :: champ_gpu_set_graphs, champ_gpu_metrop1_batch
+  public :: champ_gpu_set_graphs, champ_gpu_metrop1_batch, champ_gpu_metrop6, champ_gpu_set_metrop6
+  public :: champ_gpu_vmc, champ_gpu_dmc_init, champ_gpu_dmc_step, champ_gpu_dmc, champ_gpu_dmc_weights
+  public :: champ_gpu_deriv_count, champ_gpu_deriv_psi, champ_gpu_sr_sums, champ_gpu_set_deriv_map
+  public :: champ_gpu_set_params, champ_sr_solve
+  integer(c_int32_t), parameter, public :: CHAMP_DERIV_JAS_EN = 1, CHAMP_DERIV_JAS_EE = 2
+  integer(c_int32_t), parameter, public :: CHAMP_DERIV_JAS_EEN = 4, CHAMP_DERIV_CI = 8, CHAMP_DERIV_MAPPED = 16
   public :: gpu_check, gpu_hpsi, gpu_psie, gpu_compute_determinante_grad, gpu_metrop1
 
 contains

@@ -1,8 +1,9 @@
 !> Population VMC driver in Fortran on top of the GPU engine: the block/step
-!> loop of vmc.f90:103-134 with metrop1 moves (metrop_mov1_driftdif.f90)
+!> loop of vmc.f90:103-134 with metrop1 moves (metrop_mov1_driftdif.f90) or
+!> metrop6 moves (metrop_mov1_slat.f90, CHAMP's default mover, tau unused)
 !> executed for all walkers at once through champ_gpu_mod.
 !>
-!> usage: vmc_gpu <system.bin> <nblk> <nstep> <tau> [seed32]
+!> usage: vmc_gpu <system.bin> <nblk> <nstep> <tau> [seed32 [imetro]]
 !> The system file holds the tables of champ_system_t and the initial walker
 !> positions (written by champ_amd.system.System.write_binary).
 program vmc_gpu
@@ -24,7 +25,7 @@ program vmc_gpu
   character(len=32) :: cseed
   integer(c_int32_t) :: irn(8)
   integer :: u, nelec, ncent, nctype, nr, nrbm, nbm, nbasis, norb, ndet, norda, nordb, nordc, nspin2b, nquad
-  integer :: nterms, nwalk, nblk, nstep, iblk
+  integer :: nterms, nwalk, nblk, nstep, iblk, imetro
   real(c_double) :: tau, eblk, esum, esum2, acc, t0, t1, moves
   integer(c_int64_t) :: c0, c1, crate
 
@@ -34,6 +35,14 @@ program vmc_gpu
   call get_command_argument(4, arg); read(arg, *) tau
   cseed = '18123437465549275475255231234865'
   if (command_argument_count() >= 5) call get_command_argument(5, cseed)
+  imetro = 1
+  if (command_argument_count() >= 6) then
+    call get_command_argument(6, arg); read(arg, *) imetro
+  end if
+  if (imetro /= 1 .and. imetro /= 6) then
+    write(*,'(a,i0)') ' vmc_gpu: imetro must be 1 or 6, got ', imetro
+    error stop 1
+  end if
 
   open(newunit=u, file=trim(path), access='stream', form='unformatted', status='old')
   read(u) hdr
@@ -83,7 +92,11 @@ program vmc_gpu
   esum = 0; esum2 = 0; acc = 0
   call system_clock(c0, crate)
   do iblk = 1, nblk
-    call gpu_metrop1(ctx, nstep, tau, st)
+    if (imetro == 6) then
+      call gpu_check(champ_gpu_metrop6(ctx, int(nstep, c_int32_t), st), 'metrop6')
+    else
+      call gpu_metrop1(ctx, nstep, tau, st)
+    end if
     eblk = st%esum / (nwalk * nstep)
     esum = esum + eblk
     esum2 = esum2 + eblk * eblk

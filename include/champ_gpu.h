@@ -40,6 +40,7 @@ extern "C" {
 #define CHAMP_ERR_UNSUPPORTED -3
 #define CHAMP_ERR_NOMEM       -4
 #define CHAMP_ERR_STATE       -5
+#define CHAMP_ERR_NUMERIC     -6
 
 /* ---- per-walker status bits ---------------------------------------------- */
 #define CHAMP_WALKER_SINGULAR   1  /* LU pivot was exactly zero (matinv.f90:59)   */
@@ -305,24 +306,37 @@ int champ_gpu_dmc_weights(champ_ctx *ctx, double *wt, int32_t *nwalk);
 
 /* --------------------------------------------------------------------------
  * Parameter derivatives d ln psi / d p at the walkers' current configurations
- * (deriv_jastrow4.f90:3-667 gvalue for the linear Jastrow parameters;
- * optci.f90 for the determinant coefficients) and the population sums of
- * stochastic reconfiguration (optwf_sr.f90:71-470).  They use the state the
- * last hpsi left (metrop1/metrop6 sweeps end with one).  Parameter groups, in
- * this order, selected by the bits of `kinds`:
- *   CHAMP_DERIV_JAS_EN   a4[o] per centre type, o = 2..norda (polynomial part)
- *   CHAMP_DERIV_JAS_EE   b[o] per spin class (nspin2b), o = 2..nordb
- *   CHAMP_DERIV_JAS_EEN  c[t] per centre type, every term of order nordc in
- *                        jastrow4.f90's order (no cusp elimination: a caller
- *                        that keeps the cusp conditions maps these through
- *                        its dependency matrix)
+ * (deriv_jastrow4.f90:3-667 gvalue for the Jastrow coefficients; optci.f90:
+ * 55-57 for the determinant coefficients) and the population sums of
+ * stochastic reconfiguration (optwf_sr.f90:504-620).  They use the state the
+ * last hpsi left (metrop1/metrop6 sweeps end with one).  Raw parameters, in
+ * this order and in the layouts of champ_system_t's arrays, selected by the
+ * bits of `kinds`:
+ *   CHAMP_DERIV_JAS_EN   a4[it][p], p = 0..norda, per centre type (p = 0, 1:
+ *                        the Pade numerator and denominator, CHAMP's a(1),
+ *                        a(2); p >= 2 the polynomial, CHAMP's a(p+1))
+ *   CHAMP_DERIV_JAS_EE   b[isb][p], p = 0..nordb, per spin class (nspin2b)
+ *   CHAMP_DERIV_JAS_EEN  c[it][t] per centre type, every term of order nordc
+ *                        in jastrow4.f90's order (no cusp elimination)
  *   CHAMP_DERIV_CI       cdet[k] of every determinant (ndet > 1 only)
+ *   CHAMP_DERIV_MAPPED   the parameters of the map set by
+ *                        champ_gpu_set_deriv_map (other bits ignored)
  * ------------------------------------------------------------------------ */
 #define CHAMP_DERIV_JAS_EN  1
 #define CHAMP_DERIV_JAS_EE  2
 #define CHAMP_DERIV_JAS_EEN 4
 #define CHAMP_DERIV_CI      8
+#define CHAMP_DERIV_MAPPED  16
 int champ_gpu_deriv_count(champ_ctx *ctx, int32_t kinds, int32_t *nparm);
+
+/* A linear map from the raw parameters of `kinds` (nraw of them) to a
+ * caller's nout parameters: O' = O map, map [nraw][nout] row-major.  This is
+ * how CHAMP's own parameter list is obtained: the iwjasa / iwjasb / iwjasc
+ * selections of the jastrow_der file (read_data_parser.f90:1521-1700), the
+ * e-e-n coefficients fixed by the cusp conditions folded into the free ones
+ * (deriv_jastrow4.f90:394-460 with cdep of checkdepend4), and the CSF
+ * combinations of optci.f90:70-80.  map NULL or nout 0 removes the map.      */
+int champ_gpu_set_deriv_map(champ_ctx *ctx, int32_t kinds, int32_t nout, const double *map);
 
 /* O [nwalk][nparm]: d ln psi / d p of every walker.                         */
 int champ_gpu_deriv_psi(champ_ctx *ctx, int32_t kinds, double *o);
@@ -330,6 +344,36 @@ int champ_gpu_deriv_psi(champ_ctx *ctx, int32_t kinds, double *o);
 /* Sums over the walkers of O_k O_l (s [nparm][nparm], on the matrix cores),
  * O_k (osum [nparm]), E_L O_k (eosum [nparm]) and E_L (esum); any may be NULL. */
 int champ_gpu_sr_sums(champ_ctx *ctx, int32_t kinds, double *s, double *osum, double *eosum, double *esum);
+
+/* New wave-function parameters (compute_jastrow / compute_ci,
+ * optwf_handle_wf.f90:1050-1104, 1214-1284, after the caller has applied the
+ * step and, for the e-e-n coefficients, cuspexact4): arrays in the layouts of
+ * champ_system_t; a NULL array keeps its values.  The Pade denominators
+ * a4[it][1] and b[isb][1] must exceed -scalek (check_parms_jas,
+ * optwf_handle_wf.f90:1286-1336): CHAMP_ERR_ARG otherwise.  Every walker's
+ * state is recomputed (hpsi) before the call returns.                       */
+int champ_gpu_set_params(champ_ctx *ctx, const double *a4, const double *b, const double *c,
+                         const double *cdet);
+
+typedef struct champ_sr_info {
+    double adiag;           /* the diagonal shift finally used                   */
+    double dparm_norm;      /* rms of the step                                   */
+    int32_t cg_iters;       /* conjugate-gradient iterations of the last solve   */
+    int32_t retries;        /* times the shift was multiplied by ten             */
+} champ_sr_info;
+
+/* The SR step from the sums (host only, no GPU work): sr_hs
+ * (optwf_sr.f90:504-620) with S_kl = <O_k O_l> - <O_k><O_l> plus sr_adiag
+ * times its diagonal, h = -2 (<E O_k> - <E><O_k>), the Jacobi-preconditioned
+ * conjugate gradients of pcg (optwf_sr_more.f90:31-97) to relative residual
+ * sr_eps, dparm = -sr_tau x (optwf_sr.f90:213-214), and the retry of
+ * optwf_sr.f90:215-231 with ten times the shift while the rms of dparm
+ * exceeds dparm_norm_min (CHAMP's default 1).  The sums are those of
+ * champ_gpu_sr_sums over walkers of total weight wsum.  New parameters are
+ * p - dparm (compute_jastrow subtracts).  info may be NULL.                  */
+int champ_sr_solve(int32_t nparm, double wsum, const double *s, const double *osum, const double *eosum,
+                   double esum, double sr_tau, double sr_eps, double sr_adiag, double dparm_norm_min,
+                   double *dparm, champ_sr_info *info);
 
 /* metrop1 over n populations at once, typically different molecules (each
  * context its own system, walker count and precision): every context's

@@ -6,6 +6,9 @@
 
 #define P(f) printf("%s %zu\n", #f, offsetof(champ_system_t, f))
 #define Q(f) printf("stats.%s %zu\n", #f, offsetof(champ_sweep_stats, f))
+#define V(f) printf("vmc.%s %zu\n", #f, offsetof(champ_vmc_result, f))
+#define D(f) printf("dmcstep.%s %zu\n", #f, offsetof(champ_dmc_step_stats, f))
+#define R(f) printf("dmcres.%s %zu\n", #f, offsetof(champ_dmc_result, f))
 
 int main(void)
 {
@@ -17,5 +20,15 @@ int main(void)
     printf("sizeof_system %zu\n", sizeof(champ_system_t));
     Q(accepted); Q(proposed); Q(esum); Q(esum2); Q(nflagged);
     printf("sizeof_stats %zu\n", sizeof(champ_sweep_stats));
+    V(efin); V(eerr); V(sigma); V(tcorr); V(accept); V(nsamples); V(nflagged);
+    printf("sizeof_vmc %zu\n", sizeof(champ_vmc_result));
+    D(nwalk); D(pad); D(wsum); D(esum); D(wgsum); D(egsum); D(eest); D(eigv); D(fprod); D(accept); D(nodecr);
+    D(nbranch);
+    printf("sizeof_dmcstep %zu\n", sizeof(champ_dmc_step_stats));
+    R(egave); R(egerr); R(eest); R(accept); R(tau_eff); R(nwalk); R(nwalk_max); R(nbranch); R(nodecr);
+    printf("sizeof_dmcres %zu\n", sizeof(champ_dmc_result));
+    printf("sr.adiag %zu\nsr.dparm_norm %zu\nsr.cg_iters %zu\nsr.retries %zu\n", offsetof(champ_sr_info, adiag),
+           offsetof(champ_sr_info, dparm_norm), offsetof(champ_sr_info, cg_iters), offsetof(champ_sr_info, retries));
+    printf("sizeof_sr %zu\n", sizeof(champ_sr_info));
     return 0;
 }

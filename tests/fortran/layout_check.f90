@@ -7,6 +7,10 @@ program layout_check
   implicit none
   type(champ_system_t), target :: s
   type(champ_sweep_stats), target :: st
+  type(champ_vmc_result), target :: v
+  type(champ_dmc_step_stats), target :: d
+  type(champ_dmc_result), target :: r
+  type(champ_sr_info), target :: q
   integer(c_intptr_t) :: b
 
   b = transfer(c_loc(s), b)
@@ -33,6 +37,33 @@ program layout_check
   call p('stats.esum', c_loc(st%esum)); call p('stats.esum2', c_loc(st%esum2))
   call p('stats.nflagged', c_loc(st%nflagged))
   write(*,'(a,1x,i0)') 'sizeof_stats', c_sizeof(st)
+
+  b = transfer(c_loc(v), b)
+  call p('vmc.efin', c_loc(v%efin)); call p('vmc.eerr', c_loc(v%eerr)); call p('vmc.sigma', c_loc(v%sigma))
+  call p('vmc.tcorr', c_loc(v%tcorr)); call p('vmc.accept', c_loc(v%accept))
+  call p('vmc.nsamples', c_loc(v%nsamples)); call p('vmc.nflagged', c_loc(v%nflagged))
+  write(*,'(a,1x,i0)') 'sizeof_vmc', c_sizeof(v)
+
+  b = transfer(c_loc(d), b)
+  call p('dmcstep.nwalk', c_loc(d%nwalk)); call p('dmcstep.pad', c_loc(d%pad)); call p('dmcstep.wsum', c_loc(d%wsum))
+  call p('dmcstep.esum', c_loc(d%esum)); call p('dmcstep.wgsum', c_loc(d%wgsum))
+  call p('dmcstep.egsum', c_loc(d%egsum)); call p('dmcstep.eest', c_loc(d%eest))
+  call p('dmcstep.eigv', c_loc(d%eigv)); call p('dmcstep.fprod', c_loc(d%fprod))
+  call p('dmcstep.accept', c_loc(d%accept)); call p('dmcstep.nodecr', c_loc(d%nodecr))
+  call p('dmcstep.nbranch', c_loc(d%nbranch))
+  write(*,'(a,1x,i0)') 'sizeof_dmcstep', c_sizeof(d)
+
+  b = transfer(c_loc(r), b)
+  call p('dmcres.egave', c_loc(r%egave)); call p('dmcres.egerr', c_loc(r%egerr)); call p('dmcres.eest', c_loc(r%eest))
+  call p('dmcres.accept', c_loc(r%accept)); call p('dmcres.tau_eff', c_loc(r%tau_eff))
+  call p('dmcres.nwalk', c_loc(r%nwalk)); call p('dmcres.nwalk_max', c_loc(r%nwalk_max))
+  call p('dmcres.nbranch', c_loc(r%nbranch)); call p('dmcres.nodecr', c_loc(r%nodecr))
+  write(*,'(a,1x,i0)') 'sizeof_dmcres', c_sizeof(r)
+
+  b = transfer(c_loc(q), b)
+  call p('sr.adiag', c_loc(q%adiag)); call p('sr.dparm_norm', c_loc(q%dparm_norm))
+  call p('sr.cg_iters', c_loc(q%cg_iters)); call p('sr.retries', c_loc(q%retries))
+  write(*,'(a,1x,i0)') 'sizeof_sr', c_sizeof(q)
 
 contains
 

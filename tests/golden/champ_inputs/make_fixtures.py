@@ -25,11 +25,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 CASES = {
     "H2": ("VMC-H2", "VMC_H2_STO-CVB1_energy_hydrogen.sh",
-           ["revised_vmc.inp", "rhf.det", "champ_v3_trexio_order_rhf.sto_cvb1.lcao", "jastrow.0",
+           ["revised_vmc.inp", "rhf.det", "champ_v3_trexio_order_rhf.sto_cvb1.lcao", "jastrow.0", "jastrow.der",
             "pool/champ_v3_h2.xyz", "pool/champ_v3_sto_cvb1_basis_pointers.bfinfo", "pool/sto_cvb1.basis.H"]),
     "C4H6_ci44": ("VMC-C4H6-ci44_pVQZ_20_dets_12_csf_noopt",
                   "VMC_C4H6_ci44_pVQZ_20-dets_12-csf_energy-only_butadiene.sh",
-                  ["vmc_noopt_ci44.inp", "champ_v2_gamess_ci44_pVQZ_cart.det", "jastrow_good_b3lyp.0",
+                  ["vmc_noopt_ci44.inp", "champ_v2_gamess_ci44_pVQZ_cart.det", "jastrow_good_b3lyp.0", "jastrow.der",
                    "pool/champ_v3_butadiene_champ_v2_gamess_ci44_pVQZ_1_geom.xyz",
                    "pool/champ_v3_champ_v2_gamess_ci44_pVQZ_basis_pointers.bfinfo",
                    "pool/BFD-Q.basis.C", "pool/BFD-Q.basis.H", "pool/BFD.gauss_ecp.dat.C",

@@ -31,8 +31,9 @@ def _layout(cmd, exe):
 
 @pytest.mark.skipif(not os.path.exists(LIBPATH), reason="engine library not built")
 def test_bind_c_types_match_the_c_layout(tmp_path):
-    """Offsets of every component of champ_system_t and champ_sweep_stats,
-    and their sizes, as amdflang lays out the bind(c) types and as the C
+    """Offsets of every component of champ_system_t, champ_sweep_stats,
+    champ_vmc_result, champ_dmc_step_stats, champ_dmc_result and
+    champ_sr_info, and their sizes, as amdflang lays out the bind(c) types and as the C
     compiler lays out the header's structs."""
     fexe, cexe = str(tmp_path / "lf"), str(tmp_path / "lc")
     f = _layout([FC, "-O0", "-J", str(tmp_path), MOD, os.path.join(ROOT, "tests", "fortran", "layout_check.f90"),
@@ -40,7 +41,7 @@ def test_bind_c_types_match_the_c_layout(tmp_path):
                  "-o", fexe], fexe)
     c = _layout(["gcc", "-I", os.path.dirname(HDR), os.path.join(ROOT, "tests", "fortran", "layout_check.c"),
                  "-o", cexe], cexe)
-    assert len(c) == 54
+    assert len(c) == 54 + 8 + 13 + 10 + 5
     assert f == c
 
 
@@ -51,7 +52,7 @@ def _bind_names():
 @pytest.mark.skipif(not os.path.exists(LIBPATH), reason="engine library not built")
 def test_bind_c_names_are_exported_and_declared():
     names = _bind_names()
-    assert len(names) >= 16
+    assert len(names) >= 26
     syms = subprocess.run(["nm", "-D", "--defined-only", LIBPATH], check=True, capture_output=True,
                           text=True).stdout.split()
     hdr = open(HDR).read()
@@ -71,7 +72,7 @@ def test_fortran_driver_builds(tmp_path):
 def test_fortran_driver_matches_python_driver(require_gpu, tmp_path):
     """vmc_gpu (Fortran, through champ_gpu_mod) and Engine (Python, ctypes)
     on the same system file, walkers and seed: the block energies of their
-    metrop1 sweeps agree to 1e-13 relative (the walkers follow the same
+    metrop1 and metrop6 sweeps agree to 1e-13 relative (the walkers follow the same
     chains; the population's energy sum is accumulated with atomics, whose
     order, and so the last bits of the sum, vary from run to run)."""
     if os.environ.get("CHAMP_ENGINE_LIB"):
@@ -79,15 +80,15 @@ def test_fortran_driver_matches_python_driver(require_gpu, tmp_path):
     from champ_amd.engine import Engine
     if not os.access(B.FEXE, os.X_OK):
         raise RuntimeError(f"{B.FEXE} is not built: run __graft_entry__.build() first")
-    for name, nwalk in (("small", 16), ("c22", 8)):
+    for name, nwalk, imetro in (("small", 16, 1), ("c22", 8, 1), ("small", 16, 6)):
         sy = S.config(name)
         x0 = sy.initial_walkers(nwalk, seed=21)
         path = str(tmp_path / f"{name}.bin")
         sy.write_binary(path, x0)
         nblk, nstep, tau = 3, 2, 0.3
-        print(f"[fortran driver] {name} nwalk={nwalk}", flush=True)
-        r = subprocess.run([B.FEXE, path, str(nblk), str(nstep), str(tau)], capture_output=True, text=True,
-                           timeout=120)
+        print(f"[fortran driver] {name} nwalk={nwalk} imetro={imetro}", flush=True)
+        r = subprocess.run([B.FEXE, path, str(nblk), str(nstep), str(tau), "18123437465549275475255231234865",
+                            str(imetro)], capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stdout + r.stderr
         ef = [float(m) for m in re.findall(r"block\s+\d+\s+E_L\s+(\S+)", r.stdout)]
         assert len(ef) == nblk, r.stdout
@@ -95,6 +96,9 @@ def test_fortran_driver_matches_python_driver(require_gpu, tmp_path):
         e.set_walkers(x0)
         e.seed_rng(S.seed_from_string())
         e.hpsi()
-        ep = [e.metrop1(nstep, tau)["esum"] / (nwalk * nstep) for _ in range(nblk)]
+        if imetro == 6:
+            ep = [e.metrop6(nstep)["esum"] / (nwalk * nstep) for _ in range(nblk)]
+        else:
+            ep = [e.metrop1(nstep, tau)["esum"] / (nwalk * nstep) for _ in range(nblk)]
         print(f"[fortran driver] {name} fortran {ef} python {ep}", flush=True)
         np.testing.assert_allclose(ef, ep, rtol=1e-13, atol=0)

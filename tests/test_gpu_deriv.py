@@ -1,14 +1,21 @@
 """Parameter derivatives d ln psi / d p on the GPU (champ_gpu_deriv_psi: the
-gvalue of deriv_jastrow4.f90 for the linear Jastrow parameters and the
+gvalue of deriv_jastrow4.f90 for the Jastrow coefficients and the
 determinant ratios of optci.f90) against central finite differences of the
-oracle's ln |psi| with the parameter moved in the system tables, and the
-stochastic-reconfiguration sums (champ_gpu_sr_sums) against the same sums of
-those derivatives."""
+oracle's ln |psi| with the parameter moved in the system tables; CHAMP's own
+parameter list (jastrow_der selections, cusp-fixed e-e-n coefficients
+following the free ones, determinant coefficients but the first) through the
+GPU parameter map against finite differences along the optimiser's own
+update; new parameters (champ_gpu_set_params) against the oracle's hpsi; and
+the stochastic-reconfiguration sums (champ_gpu_sr_sums) against the same sums
+of the derivatives."""
+import dataclasses
+
 import numpy as np
 import pytest
 
+from champ_amd import optwf
 from champ_amd import system as S
-from champ_amd.engine import DERIV_CI, DERIV_JAS_EE, DERIV_JAS_EEN, DERIV_JAS_EN, Engine
+from champ_amd.engine import DERIV_CI, DERIV_JAS_EE, DERIV_JAS_EEN, DERIV_JAS_EN, DERIV_MAPPED, Engine
 from oracle import ffi
 from tests.gpu_helpers import ENGINE_LIB
 
@@ -21,10 +28,10 @@ def _params(sy):
     """(array attribute, index) of each parameter in the engine's order"""
     out = []
     for it in range(sy.nctype):
-        for o in range(2, sy.norda + 1):
+        for o in range(0, sy.norda + 1):
             out.append(("a4", (it, o)))
     for isb in range(sy.nspin2b):
-        for o in range(2, sy.nordb + 1):
+        for o in range(0, sy.nordb + 1):
             out.append(("b", (isb, o)))
     for it in range(sy.nctype):
         for t in range(sy.c.shape[1]):
@@ -89,3 +96,105 @@ def test_sr_sums_are_the_derivative_sums(require_gpu, name):
     assert np.allclose(os_, o.sum(axis=0), rtol=1e-12, atol=1e-13)
     assert np.allclose(eo, e @ o, rtol=1e-12, atol=1e-12 * np.max(np.abs(e @ o)))
     assert abs(es - e.sum()) <= 1e-12 * abs(e.sum())
+
+
+def _champ_list(sy, ci):
+    free = [t + 1 for t in range(S.nterms4(sy.nordc)) if t not in optwf.cusp_fixed_terms(sy.nordc)]
+    der = dict(iwjasa=[[3, 4, 5, 6]] * sy.nctype, iwjasb=[[2, 3, 4, 5, 6]] * sy.nspin2b, iwjasc=[free] * sy.nctype)
+    return optwf.ChampParams.build(sy, der, ci=ci)
+
+
+def _with(sy, arrays):
+    a4, b, c, cdet = arrays
+    return dataclasses.replace(sy, a4=a4, b=b, c=c, cdet=cdet)
+
+
+@pytest.mark.parametrize("name,ci", [("small", False), ("small_md", True)])
+def test_champ_parameter_list_matches_finite_differences(require_gpu, name, ci):
+    """O of CHAMP's parameters (the map applied on the GPU) against
+    (ln psi(p - h e_k) - ln psi(p + h e_k)) / 2h along the optimiser's update
+    (ChampParams.apply subtracts: p_new = p - dparm), cuspexact4 included."""
+    sy = S.config(name)
+    nw = 3
+    x = sy.initial_walkers(nw, seed=23)
+    p = _champ_list(sy, ci)
+    eng = Engine(sy, nw, lib_path=ENGINE_LIB)
+    eng.set_walkers(x)
+    eng.hpsi()
+    eng.set_deriv_map(p.kinds, p.m)
+    assert eng.deriv_count(DERIV_MAPPED) == p.nparm
+    o = eng.deriv_psi(DERIV_MAPPED)
+    raw = eng.deriv_psi(p.kinds)
+    np.testing.assert_allclose(o, raw @ p.m, rtol=1e-13, atol=1e-13 * max(1.0, float(np.max(np.abs(o)))))
+    for k in range(p.nparm):
+        h = 1e-5
+        e = np.zeros(p.nparm)
+        e[k] = h
+        lp = _lnpsi(_with(sy, p.apply(sy, -e)), x)
+        lm = _lnpsi(_with(sy, p.apply(sy, e)), x)
+        fd = (lp - lm) / (2 * h)
+        err = float(np.max(np.abs(o[:, k] - fd))) / max(1.0, float(np.max(np.abs(fd))))
+        assert err < 1e-6, (name, p.names[k], o[:, k], fd)
+    print(f"[deriv {name}] {p.nparm} CHAMP parameters match finite differences", flush=True)
+
+
+@pytest.mark.parametrize("name,ci", [("small", False), ("small_md", True)])
+def test_set_params_matches_the_oracle(require_gpu, name, ci):
+    """After an SR-sized step, the engine's recomputed state equals the
+    oracle's hpsi with the new coefficients."""
+    sy = S.config(name)
+    nw = 4
+    x = sy.initial_walkers(nw, seed=29)
+    p = _champ_list(sy, ci)
+    eng = Engine(sy, nw, lib_path=ENGINE_LIB)
+    eng.set_walkers(x)
+    eng.hpsi()
+    dp = np.random.default_rng(5).normal(size=p.nparm) * 0.02
+    new = p.apply(sy, dp)
+    eng.set_params(*new)
+    sc = eng.scalars()
+    sy2 = _with(sy, new)
+    cs = sy2.as_ctypes()
+    for w in range(nw):
+        ow = ffi.walker(sy2, cs)
+        ow.set_x(x[w])
+        hh = ow.hpsi()
+        assert abs(sc["psido"][w] - hh["psid"]) <= 1e-10 * abs(hh["psid"])
+        assert abs(sc["psijo"][w] - hh["psij"]) <= 1e-10 * max(1.0, abs(hh["psij"]))
+        assert abs(sc["eold"][w] - hh["energy"]) <= 1e-10 * max(1.0, abs(hh["energy"]))
+    # a Pade denominator at or below -scalek is refused, the state kept
+    bad = new[0].copy()
+    bad[0, 1] = -sy.scalek
+    with pytest.raises(RuntimeError):
+        eng.set_params(bad, None, None, None)
+    np.testing.assert_array_equal(eng.scalars()["eold"], sc["eold"])
+
+
+def test_sr_step_through_the_map(require_gpu):
+    """A few SR iterations on c22_md with CHAMP's Jastrow and CI list: the
+    mapped S equals M^T S_raw M, and every step keeps the population finite."""
+    from champ_amd.engine import sr_solve
+    sy = S.config("c22_md")
+    nw = 128
+    p = _champ_list(sy, True)
+    eng = Engine(sy, nw, lib_path=ENGINE_LIB)
+    eng.set_walkers(sy.initial_walkers(nw, seed=8))
+    eng.seed_rng(S.seed_from_string())
+    eng.hpsi()
+    eng.set_deriv_map(p.kinds, p.m)
+    cur = sy
+    for it in range(3):
+        eng.metrop1(4, 0.3)
+        s, os_, eo, es = eng.sr_sums(DERIV_MAPPED)
+        sr, osr, eor, esr = eng.sr_sums(p.kinds)
+        np.testing.assert_allclose(s, p.m.T @ sr @ p.m, rtol=1e-11, atol=1e-11 * np.max(np.abs(s)))
+        np.testing.assert_allclose(os_, p.m.T @ osr, rtol=1e-11, atol=1e-11 * max(1.0, np.max(np.abs(os_))))
+        assert es == esr
+        dparm, info = sr_solve(s, os_, eo, es, nw, sr_tau=0.02, sr_eps=1e-3, sr_adiag=0.01, lib_path=ENGINE_LIB)
+        assert np.all(np.isfinite(dparm)) and info["dparm_norm"] <= 1.0
+        new = p.apply(cur, dparm)
+        eng.set_params(*new)
+        cur = _with(cur, new)
+        assert np.all(np.isfinite(eng.scalars()["eold"]))
+        print(f"[sr c22_md] iteration {it}: <E> {es / nw:.6f} |dparm| {info['dparm_norm']:.3e} "
+              f"cg {info['cg_iters']}", flush=True)

@@ -1,0 +1,113 @@
+"""CHAMP's optimisation parameter list (champ_amd.optwf) and the SR step
+(champ_sr_solve, host code of the engine library; no GPU work).
+
+Pins: the free e-e-n coefficients of the cusp rule against the lists the
+reference's own jastrow_der inputs hold; the parameter map against the
+parameter update (the raw coefficients move by exactly -M dparm); the SR
+step against a direct dense solve of the same regularised system."""
+import os
+
+import numpy as np
+import pytest
+
+from champ_amd import champ_io as io
+from champ_amd import optwf
+from champ_amd import system as S
+from champ_amd.engine import LIBPATH, sr_solve
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = os.path.join(HERE, "golden", "champ_inputs")
+
+
+def _listed_c(path):
+    """The e-e-n index list written on the '(c(iparmj),...' lines of a jastrow_der file."""
+    out = []
+    for line in open(path):
+        if "(c(iparmj)" in line:
+            out.append([int(t) for t in line.split("(")[0].split()])
+    return out
+
+
+@pytest.mark.parametrize("case,nctype", [("H2", 1), ("C4H6_ci44", 2)])
+def test_jastrow_der_reader_and_free_een_terms(case, nctype):
+    path = os.path.join(FIX, case, "jastrow.der")
+    d = io.read_jastrow_der(path, nctype, 1)
+    assert d["iwjasa"] == [[3, 4, 5, 6]] * nctype
+    assert d["iwjasb"] == [[2, 3, 4, 5, 6]]
+    assert d["iwjasc"] == [[]] * nctype  # nparmc = 0: the lists are read as empty records
+    # the lists written for nordc = 5 are exactly the coefficients the cusp rule leaves free
+    free = [t + 1 for t in range(S.nterms4(5)) if t not in optwf.cusp_fixed_terms(5)]
+    for lst in _listed_c(path):
+        assert lst == free
+    # and the counts the reader allows per order (read_data_parser.f90: 2, 7, 15, 27, 43 for nordc 3..7)
+    assert [S.nterms4(n) - len(optwf.cusp_fixed_terms(n)) for n in (3, 4, 5)] == [2, 7, 15]
+
+
+def test_csf_reader_rebuilds_cdet():
+    det = os.path.join(FIX, "C4H6_ci44", "champ_v2_gamess_ci44_pVQZ_cart.det")
+    ccsf, terms = io.read_csf(det)
+    cdet, _ = io.read_determinants(det, 22)
+    rebuilt = np.zeros_like(cdet)
+    for c, t in zip(ccsf, terms):
+        for idet, cx in t:
+            rebuilt[idet] += c * cx
+    np.testing.assert_allclose(rebuilt, cdet, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("name,ci", [("small", False), ("small_md", True)])
+def test_parameter_map_matches_the_update(name, ci):
+    """apply(dparm) moves the raw coefficients by exactly -M dparm: the map the
+    derivatives go through and the step the optimiser takes are the same
+    linear relation (cuspexact4 included)."""
+    sy = S.config(name)
+    free = [t + 1 for t in range(S.nterms4(sy.nordc)) if t not in optwf.cusp_fixed_terms(sy.nordc)]
+    der = dict(iwjasa=[[3, 4, 5, 6]] * sy.nctype, iwjasb=[[2, 3, 4, 5, 6]] * sy.nspin2b,
+               iwjasc=[free] * sy.nctype)
+    p = optwf.ChampParams.build(sy, der, ci=ci)
+    assert p.nparm == 4 * sy.nctype + 5 * sy.nspin2b + len(free) * sy.nctype + ((len(sy.cdet) - 1) if ci else 0)
+    rng = np.random.default_rng(3)
+    dp = rng.normal(size=p.nparm) * 1e-3
+    a4, b, c, cdet = p.apply(sy, dp)
+    raw_old = np.concatenate([sy.a4.ravel(), sy.b.ravel(), np.asarray(sy.c).ravel()] + ([sy.cdet] if ci else []))
+    raw_new = np.concatenate([a4.ravel(), b.ravel(), c.ravel()] + ([cdet] if ci else []))
+    np.testing.assert_allclose(raw_new - raw_old, -(p.m @ dp), rtol=0, atol=1e-15)
+    # the new e-e-n coefficients still satisfy the cusp conditions
+    np.testing.assert_allclose(S.cusp_fix_c(c, sy.nordc), c, rtol=0, atol=1e-15)
+
+
+def _dense_step(s, osum, eosum, esum, w, tau, adiag):
+    oo, o, eo, e = s / w, osum / w, eosum / w, esum / w
+    sm = oo - np.outer(o, o)
+    sm = sm + np.diag(np.diag(sm) * adiag)
+    h = -2.0 * (eo - o * e)
+    return -tau * np.linalg.solve(sm, h)
+
+
+@pytest.mark.skipif(not os.path.exists(LIBPATH), reason="engine library not built")
+def test_sr_solve_matches_a_dense_solve():
+    rng = np.random.default_rng(11)
+    w, n = 400, 14
+    # a well-conditioned S, so that pcg converges within its nparm iterations
+    o = rng.normal(size=(w, n)) * np.linspace(0.5, 2.0, n) + rng.normal(size=n)
+    e = -3.0 + 0.5 * rng.normal(size=w) + 0.2 * o[:, 0]
+    s, osum, eosum, esum = o.T @ o, o.sum(0), e @ o, e.sum()
+    dparm, info = sr_solve(s, osum, eosum, esum, w, sr_tau=0.05, sr_eps=1e-12, sr_adiag=0.01, dparm_norm_min=1e9)
+    ref = _dense_step(s, osum, eosum, esum, w, 0.05, 0.01)
+    np.testing.assert_allclose(dparm, ref, rtol=1e-9, atol=1e-12 * np.max(np.abs(ref)))
+    assert info["retries"] == 0 and info["adiag"] == 0.01 and 1 <= info["cg_iters"] < n
+    # a step above dparm_norm_min is redone with ten times the shift until it is not
+    big, info2 = sr_solve(s, osum, eosum, esum, w, sr_tau=50.0, sr_eps=1e-12, sr_adiag=0.01, dparm_norm_min=0.5)
+    assert info2["retries"] >= 1
+    assert info2["adiag"] == pytest.approx(0.01 * 10 ** info2["retries"])
+    assert info2["dparm_norm"] <= 0.5
+    np.testing.assert_allclose(big, _dense_step(s, osum, eosum, esum, w, 50.0, info2["adiag"]), rtol=1e-8)
+    # the default tolerance stops early but still descends: h . dparm has the sign of -g . step
+    d3, _ = sr_solve(s, osum, eosum, esum, w)
+    g = 2.0 * (eosum / w - osum / w * esum / w)
+    assert float(g @ d3) > 0.0  # p - dparm lowers the energy to first order
+
+
+@pytest.mark.skipif(not os.path.exists(LIBPATH), reason="engine library not built")
+def test_sr_solve_rejects_bad_arguments():
+    with pytest.raises(RuntimeError):
+        sr_solve(np.eye(2), np.zeros(2), np.zeros(2), 0.0, 0.0)
