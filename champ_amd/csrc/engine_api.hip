@@ -386,6 +386,7 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
         TRY(dalloc(&c->pair_e, (size_t)W * c->pair_cap)); TRY(dalloc(&c->pair_c, (size_t)W * c->pair_cap));
         TRY(dalloc(&c->pair_vps, (size_t)W * c->pair_cap * CHAMP_MPS_L));
         TRY(dalloc(&c->pair_val, (size_t)W * c->pair_cap));
+        TRY(dalloc(&c->pair_psi, (size_t)W * c->pair_cap * std::max(c->NQ, 1)));
         TRY(dalloc(&c->rrT, (size_t)W * NE * NC));
     }
     TRY(dalloc(&c->pair_count, W)); TRY(dalloc(&c->pair_off, (size_t)W + 1)); TRY(dalloc(&c->pair_total, 2));
@@ -435,7 +436,7 @@ extern "C" int champ_gpu_free(champ_ctx *c)
                     c->rng, c->xq, c->yq, c->zq, c->xnew, c->dfus2o, c->rnew, c->rrnew, c->fsn_row, c->g1n, c->g2n,
                     c->vjnT, c->fsumn, c->aomove, c->orbn, c->ratio, c->psidn, c->psijn, c->vnew, c->vold, c->acc,
                     c->esum_dev, c->aoall, c->orball, c->wmat, c->pair_count, c->pair_off, c->pair_total, c->grp_off,
-                    c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->rrT, c->nxquad, c->ri_cent, c->ri_type,
+                    c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->pair_psi, c->rrT, c->nxquad, c->ri_cent, c->ri_type,
                     c->ri_rad, c->ri_rmax, c->ao_ri, c->gref, c->lu_ws, c->m6rec, c->ri_perm,
                     c->ri_aoff, c->ri_aol};
     for (void *p : ptrs) if (p) hipFree(p);

@@ -140,6 +140,7 @@ struct champ_ctx {
     int *pair_e = nullptr, *pair_c = nullptr;  // [W*pair_cap]
     double *pair_vps = nullptr;                // [W*pair_cap][MPS_L]
     double *pair_val = nullptr;                // [W*pair_cap]
+    double *pair_psi = nullptr;                // [W*pair_cap][NQ] psi ratio at each quadrature point
     double *rrT = nullptr;                     // [W][NE][NC] rr_en with the centre innermost
     int *nxquad = nullptr;                     // [W]
 

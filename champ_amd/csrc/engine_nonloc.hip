@@ -469,16 +469,19 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
     int *SMI = GI + (size_t)NC * MMA_MP;                       // [NRI][8]
     double *SRM = (double *)(SMI + (size_t)NRI * 8);           // [NRI] cut radius
     int *SAO = (int *)(SRM + NRI);                             // [NB][2] slm code, AO index
-    for (int sidx = tid; sidx < NRI; sidx += MMA_THREADS) {
-        const int ri = a.ri_perm[sidx];
-        SMI[8 * sidx] = a.ri_cent[ri]; SMI[8 * sidx + 1] = a.ri_type[ri]; SMI[8 * sidx + 2] = a.ri_rad[ri];
-        SMI[8 * sidx + 3] = a.ri_aoff[ri]; SMI[8 * sidx + 4] = a.ri_aoff[ri + 1];
-        SRM[sidx] = a.ri_rmax[ri];
-    }
-    for (int k = tid; k < NB; k += MMA_THREADS) {
-        const int m = a.ri_aol[k];
-        SAO[2 * k] = a.ao_slm[m];
-        SAO[2 * k + 1] = m;
+    const bool psi_ext = a.psi_ext != 0;  // psi ratios from k_nonloc_psi
+    if (!psi_ext) {
+        for (int sidx = tid; sidx < NRI; sidx += MMA_THREADS) {
+            const int ri = a.ri_perm[sidx];
+            SMI[8 * sidx] = a.ri_cent[ri]; SMI[8 * sidx + 1] = a.ri_type[ri]; SMI[8 * sidx + 2] = a.ri_rad[ri];
+            SMI[8 * sidx + 3] = a.ri_aoff[ri]; SMI[8 * sidx + 4] = a.ri_aoff[ri + 1];
+            SRM[sidx] = a.ri_rmax[ri];
+        }
+        for (int k = tid; k < NB; k += MMA_THREADS) {
+            const int m = a.ri_aol[k];
+            SAO[2 * k] = a.ao_slm[m];
+            SAO[2 * k + 1] = m;
+        }
     }
 
     const int G = gridDim.x, bk = blockIdx.x;
@@ -609,6 +612,7 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
         // taken in order of their AO count (ri_perm), so the lanes of a wave
         // mostly run the same number of AOs.
         __syncthreads();  // SW and the point-phase partials are free
+        if (!psi_ext) {
         for (int t = tid; t < npts * NC; t += MMA_THREADS) {
             const int q = t / NC, c = t % NC, p = q / NQ;
             const double *xp = XP + 3 * q;
@@ -650,6 +654,7 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
             for (int k = 0; k < MMA_PS; k++) v += PS[k * MMA_MP + q];
             SP[2 * MMA_MP + q] = v;
         }
+        }  // !psi_ext
         __syncthreads();
 
         // ---- the vpsp_det terms (nonloc.f90:248-274), one thread per pair
@@ -671,17 +676,165 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
                 for (int l = 1; l <= lp - 1; l++)
                     term_radial = term_radial + yl0(l, costh) * a.pair_vps[slot * CHAMP_MPS_L + l - 1];
                 const double trj = term_radial * a.ecp->wq[q] * exp(fsumn);
-                pair_sum = pair_sum + trj * SP[2 * MMA_MP + pt];
+                pair_sum = pair_sum + trj * (psi_ext ? a.pair_psi[slot * NQ + q] : SP[2 * MMA_MP + pt]);
             }
             a.pair_val[slot] = pair_sum;
         }
     }
 }
 
-template <int NORDC, typename T>
-static int launch_mma(champ_ctx *c, const HArgs &a)
+// ------------------------------------------------------------------------
+// The psi ratio of every quadrature point on its own: sum_m phi_m(x_q) W[i][m]
+// (the orbital part of nonloc.f90:224-247, contracted with the inverse row
+// once per electron: wmat).  The same groups of MMA_GP pairs and the same lane
+// form as the psi phase of k_nonloc_mma, in a kernel of few registers and
+// small LDS, so that more waves hide its table and W loads.  Results go to
+// pair_psi[slot][q] for k_nonloc_mma (psi_ext).
+constexpr int PSI_MP = MMA_MP, PSI_THREADS = MMA_THREADS;
+
+__host__ __device__ __forceinline__ size_t psi_lds_bytes(int NC, int NB, int NRI)
 {
-    const size_t lds = mma_lds_doubles(c->NC, c->NB, c->NRI, (int)sizeof(T)) * sizeof(double);
+    const size_t dbl = 3 * PSI_MP + (size_t)PSI_MP * NC + NRI + MMA_PS * PSI_MP;
+    const size_t ints = PSI_MP + 2 * MMA_GP + (size_t)PSI_MP * NC + 5 * (size_t)NRI + 2 * (size_t)NB;
+    return dbl * 8 + ((ints * 4 + 7) / 8) * 8;
+}
+
+__global__ void __launch_bounds__(PSI_THREADS) k_nonloc_psi(HArgs a)
+{
+    HIP_DYNAMIC_SHARED(double, lds)
+    const int tid = threadIdx.x;
+    const int NE = a.NE, NC = a.NC, NQ = a.NQ, NB = a.NB, NRI = a.NRI;
+    double *XP = lds;                                  // [MP][3]
+    double *GR = XP + 3 * PSI_MP;                      // [MP][NC] point-centre r
+    double *SRM = GR + (size_t)PSI_MP * NC;            // [NRI] cut radius in slot order
+    double *PS = SRM + NRI;                            // [PS][MP] partials
+    int *IPT = (int *)(PS + MMA_PS * PSI_MP);          // [MP] electron of each point
+    int *PI = IPT + PSI_MP;                            // [GP][2]: i, ic
+    int *GI = PI + 2 * MMA_GP;                         // [MP][NC] grid interval of r
+    int *SMI = GI + (size_t)PSI_MP * NC;               // [NRI][5] centre, type, radial fn, AO range
+    int *SAO = SMI + 5 * (size_t)NRI;                  // [NB][2] slm code, AO index
+    for (int sidx = tid; sidx < NRI; sidx += PSI_THREADS) {
+        const int ri = a.ri_perm[sidx];
+        SMI[5 * sidx] = a.ri_cent[ri]; SMI[5 * sidx + 1] = a.ri_type[ri]; SMI[5 * sidx + 2] = a.ri_rad[ri];
+        SMI[5 * sidx + 3] = a.ri_aoff[ri]; SMI[5 * sidx + 4] = a.ri_aoff[ri + 1];
+        SRM[sidx] = a.ri_rmax[ri];
+    }
+    for (int k = tid; k < NB; k += PSI_THREADS) {
+        const int m = a.ri_aol[k];
+        SAO[2 * k] = a.ao_slm[m];
+        SAO[2 * k + 1] = m;
+    }
+    const int G = gridDim.x, bk = blockIdx.x;
+    const int nx = (G % 8 == 0) ? 8 : 1, gx = G / nx, xcd = bk % nx, g0 = bk / nx;
+    const int total = a.pair_total[1];
+    const int per_x = (total + nx - 1) / nx;
+    const int x_lo = xcd * per_x, x_hi = min(total, x_lo + per_x);
+    for (int gi = x_lo + g0; gi < x_hi; gi += gx) {
+        int lo = 0, hi = a.W - 1;
+        while (lo < hi) {  // walker: largest w with grp_off[w] <= gi
+            const int mid = (lo + hi + 1) / 2;
+            if (a.grp_off[mid] <= gi) lo = mid; else hi = mid - 1;
+        }
+        const int w = lo;
+        const int p0 = MMA_GP * (gi - a.grp_off[w]);
+        const int np = min(MMA_GP, a.pair_count[w] - p0), npts = np * NQ;
+        __syncthreads();  // LDS of the previous group is free
+        if (tid < np) {
+            const size_t slot = (size_t)w * a.cap + p0 + tid;
+            PI[2 * tid] = a.pair_e[slot];
+            PI[2 * tid + 1] = a.pair_c[slot];
+        }
+        __syncthreads();
+        for (int t = tid; t < PSI_MP; t += PSI_THREADS) IPT[t] = t < npts ? PI[2 * (t / NQ)] : -1;
+        for (int t = tid; t < 3 * npts; t += PSI_THREADS) {
+            const int pt = t / 3, k = t % 3, p = pt / NQ, q = pt % NQ;
+            const int i = PI[2 * p], ic = PI[2 * p + 1];
+            const double ren = a.r_en[((size_t)w * NC + ic) * NE + i];
+            const double *qv = (k == 0 ? a.xq : (k == 1 ? a.yq : a.zq)) + (size_t)w * NQ;
+            XP[t] = ren * qv[q] + a.cent[3 * ic + k];
+        }
+        __syncthreads();
+        for (int t = tid; t < npts * NC; t += PSI_THREADS) {
+            const int q = t / NC, c = t % NC, p = q / NQ;
+            const double *xp = XP + 3 * q;
+            const double e0 = xp[0] - a.cent[3 * c], e1 = xp[1] - a.cent[3 * c + 1], e2 = xp[2] - a.cent[3 * c + 2];
+            double r2 = 0;
+            r2 = r2 + e0 * e0; r2 = r2 + e1 * e1; r2 = r2 + e2 * e2;
+            const double rqd = c == PI[2 * p + 1] ? a.r_en[((size_t)w * NC + c) * NE + PI[2 * p]] : sqrt(r2);
+            GR[t] = rqd;
+            GI[t] = grid_jx(a.tab, rqd);
+        }
+        __syncthreads();
+        {
+            const int q = tid % PSI_MP, slot = tid / PSI_MP;
+            if (slot < MMA_PS) {
+                double part = 0.0;
+                if (q < npts) {
+                    const double px = XP[3 * q], py = XP[3 * q + 1], pz = XP[3 * q + 2];
+                    const double *Wm = a.wmat + ((size_t)w * NE + IPT[q]) * NB;
+                    const double *gr = GR + (size_t)q * NC;
+                    const int *gi = GI + (size_t)q * NC;
+                    for (int sidx = slot; sidx < NRI; sidx += MMA_PS) {
+                        const int *mi = SMI + 5 * sidx;
+                        const int cen = mi[0];
+                        const double r = gr[cen];
+                        if (!(r <= SRM[sidx])) continue;  // beyond the shell's cut radius: its AOs are exactly zero
+                        double wfv[3] = {0.0, 0.0, 0.0};
+                        splfit_jx<double>(a.tab, mi[1], mi[2], r, gi[cen], wfv, 0);
+                        const double d0 = px - a.cent[3 * cen], d1 = py - a.cent[3 * cen + 1], d2 = pz - a.cent[3 * cen + 2];
+                        double sa = 0.0;
+                        for (int k = mi[3]; k < mi[4]; k++)
+                            sa += slm_value<double>(slm_decode(SAO[2 * k]), d0, d1, d2) * Wm[SAO[2 * k + 1]];
+                        part += sa * wfv[0];
+                    }
+                }
+                PS[tid] = part;
+            }
+        }
+        __syncthreads();
+        for (int q = tid; q < npts; q += PSI_THREADS) {
+            double v = 0.0;
+            for (int k = 0; k < MMA_PS; k++) v += PS[k * PSI_MP + q];
+            a.pair_psi[((size_t)w * a.cap + p0 + q / NQ) * NQ + q % NQ] = v;
+        }
+    }
+}
+
+#ifndef CHAMP_NL_SPLIT
+#define CHAMP_NL_SPLIT 1
+#endif
+
+static int launch_psi(champ_ctx *c, const HArgs &a)
+{
+    const size_t lds = psi_lds_bytes(c->NC, c->NB, c->NRI);
+    if (lds > 64 * 1024)
+        HIPCHECK(hipFuncSetAttribute((const void *)k_nonloc_psi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int dev = 0, ncu = 256, per_cu = 1;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonloc_psi, PSI_THREADS, lds));
+    per_cu = std::max(1, per_cu);
+    hipLaunchKernelGGL(k_nonloc_psi, dim3(ncu * per_cu), dim3(PSI_THREADS), lds, c->stream, a);
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
+}
+
+template <int NORDC, typename T>
+static int launch_mma(champ_ctx *c, const HArgs &a0)
+{
+    HArgs a = a0;
+    const bool split = CHAMP_NL_SPLIT && psi_lds_bytes(c->NC, c->NB, c->NRI) <= 160 * 1024;
+    if (split) {
+        if (int rc = launch_psi(c, a)) return rc;
+        a.psi_ext = 1;
+    }
+    // without the psi phase the tables after XQ (point-centre distances,
+    // instances, AOs) are not used
+    const size_t lds = (mma_lds_doubles(c->NC, c->NB, c->NRI, (int)sizeof(T)) -
+                        (split ? (size_t)c->NC * MMA_MP + ((size_t)c->NC * MMA_MP * 4 + 7) / 8 + (size_t)c->NRI * 4 +
+                                     (size_t)c->NRI + (size_t)c->NB
+                               : 0)) *
+                       sizeof(double);
     if (lds > 160 * 1024) {
         set_error("nonloc: the quadrature kernel's LDS tables exceed 160 KB for this system");
         return CHAMP_ERR_UNSUPPORTED;
