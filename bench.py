@@ -45,6 +45,8 @@ sys.path.insert(0, ROOT)
 #   move (jastrow4e + AO):  (Ne-1)*Nc*(25T+60) + AO
 #   hpsi_nonloc (per quadrature point): (Ne-1)*(Nc*(4T+40)+60) + Nc*40 + Nao*42
 #   hpsi_jastrow: Ne(Ne-1)/2 * Nc * (54T+80)
+#   hpsi_nonloc_psi (the psi ratio at each quadrature point, its own kernel):
+#       Nao*42 per point
 # Executed matrix-core work of the quadrature kernel (k_nonloc_mma): every
 # group of up to four active pairs of a walker runs, for each 16-partner tile
 # and each step of four centres, 3 row tiles x naz table entries of one
@@ -230,6 +232,8 @@ def per_kernel(sy, W, kt, nxq, path, acc):
         "hpsi_slater_lu": (wchunk * 2 * 2.0 * ns ** 3, "fp64-valu", "TFLOP/s", "2 Nsigma^3 per spin matrix", True),
         "hpsi_nonloc": (mfma_flops_nonloc(sy, nxq), mfma, "TFLOP/s", "executed 16x16x4 MFMA flop (groups x "
                         "partner tiles x centre quads x 3 row tiles x naz)", True),
+        "hpsi_nonloc_psi": (float(np.sum(nxq)) * nb * 42, "fp64-valu", "TFLOP/s",
+                            "reference count Nao*42 per quadrature point (AO values at the point, dot with W)", False),
     }
     out = {}
     for k, (ms, n) in kt.items():

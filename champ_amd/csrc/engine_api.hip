@@ -1103,7 +1103,7 @@ extern "C" int champ_gpu_basis_fns_noalloc(champ_ctx *c, int32_t npt, const doub
 
 static const char *kclass_name[K_NCLASS] = {"move_jastrow_ao", "move_mo_gemm", "move_ratio_commit", "hpsi_jastrow",
                                             "hpsi_ao", "hpsi_mo_gemm", "hpsi_slater_lu", "hpsi_ecp_prep", "hpsi_nonloc",
-                                            "hpsi_eloc", "multidet", "other"};
+                                            "hpsi_nonloc_psi", "hpsi_eloc", "multidet", "other"};
 
 extern "C" int champ_gpu_kernel_times(champ_ctx *c, int32_t *n, char *names, double *ms, int64_t *launches)
 {

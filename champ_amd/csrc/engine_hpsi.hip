@@ -932,9 +932,8 @@ int launch_hpsi_impl(champ_ctx *c, int accumulate)
         const int rcw = launch_wmat(c);
         tend(c, slot);
         if (rcw) return rcw;
-        tbegin(c, K_HPSI_NONLOC, &slot);  // the quadrature kernel alone
+        // timed inside: the psi-ratio kernel and the quadrature kernel apart
         const int rcn = launch_nonloc(c, a);
-        tend(c, slot);
         if (rcn) return rcn;
     }
     tbegin(c, K_HPSI_ELOC, &slot);

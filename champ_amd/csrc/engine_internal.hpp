@@ -30,7 +30,7 @@ struct Timer {
 };
 
 enum KClass { K_MOVE_A = 0, K_MOVE_GEMM, K_MOVE_B, K_HPSI_JAS, K_HPSI_AO, K_HPSI_GEMM, K_HPSI_SLATER,
-              K_HPSI_ECP, K_HPSI_NONLOC, K_HPSI_ELOC, K_MD, K_OTHER, K_NCLASS };
+              K_HPSI_ECP, K_HPSI_NONLOC, K_HPSI_PSI, K_HPSI_ELOC, K_MD, K_OTHER, K_NCLASS };
 
 }  // namespace champ
 

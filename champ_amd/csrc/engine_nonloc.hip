@@ -342,14 +342,14 @@ template <> struct Mma<float> {
 // entries (the first of the two halves of a split tile, or a whole tile)
 template <int N, typename T>
 __device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int c_lo, int c_hi, int npts,
-                                         const double *XP, const T *XQ, const int *IPT, double *swv, bool with_ee)
+                                         const double *XP, const T *XQ, const int *IPT, double *swv, bool with_ee,
+                                         const JasParams &jp, const int *sT)
 {
     constexpr EenTable<N> ET{};
     constexpr int NA = een_naz<N>(), KR = ET.kmax + 1;
     typedef typename Mma<T>::acc Acc;
     const int lane = threadIdx.x % WAVE, sl = lane / 16, ln = lane % 16;
     const int NC = a.NC, NE = a.NE;
-    const JasParams &jp = *a.jas;
     const double kap = jp.kap;
     const int j = j0 + ln;
     const bool jv = j < NE;
@@ -367,29 +367,32 @@ __device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int c_lo
         if (b + 4 < c_hi && jv) ynext = rrw[(size_t)(b + 4) * NE + j];  // next centre's distance in flight
         // the partner's table entries at centre b: built once, used by every row tile
         T tab[NA];
-        een_tables<N, 0, T>(jp.c[bv ? a.iwctype[b] : 0], (T)(1.0 - kap * rrb), tab);
+        een_tables<N, 0, T>(jp.c[bv ? sT[b] : 0], (T)(1.0 - kap * rrb), tab);
         if (!(bv && jv)) {
 #pragma unroll
             for (int e = 0; e < NA; e++) tab[e] = T(0);
         }
+        T xp[MMA_MT][N + 1];
 #pragma unroll
         for (int mt = 0; mt < MMA_MT; mt++) {
             const int pta = mt * 16 + ln;
             const bool live_a = bv && pta < npts;
-            T xp[N + 1];
             const T x = live_a ? XQ[(size_t)b * MMA_MP + pta] : T(0);
-            xp[0] = live_a ? T(1) : T(0);
+            xp[mt][0] = live_a ? T(1) : T(0);
 #pragma unroll
-            for (int p = 1; p <= N; p++) xp[p] = xp[p - 1] * x;
-#pragma unroll
-            for (int k = 0; k < KR; k++)
-#pragma unroll
-                for (int n = 0; n <= N; n++) {
-                    const int e = ET.idx[k][n];
-                    if (e < 0) continue;
-                    acc[mt][k] = Mma<T>::op(xp[n], tab[e], acc[mt][k]);
-                }
+            for (int p = 1; p <= N; p++) xp[mt][p] = xp[mt][p - 1] * x;
         }
+        // consecutive MFMAs go to different accumulators (row tile, power of
+        // u), so none waits for the previous one's result
+#pragma unroll
+        for (int n = 0; n <= N; n++)
+#pragma unroll
+            for (int k = 0; k < KR; k++) {
+                const int e = ET.idx[k][n];
+                if (e < 0) continue;
+#pragma unroll
+                for (int mt = 0; mt < MMA_MT; mt++) acc[mt][k] = Mma<T>::op(xp[mt][n], tab[e], acc[mt][k]);
+            }
     }
     const double *xw = a.xT + (size_t)w * 3 * NE;
     const double xj = jv ? xw[j] : 0.0, yj = jv ? xw[NE + j] : 0.0, zj = jv ? xw[2 * NE + j] : 0.0;
@@ -449,9 +452,18 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
     static_assert(MMA_GP * 12 <= MMA_MP, "group size");
     const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
     const int NE = a.NE, NC = a.NC, NQ = a.NQ, NB = a.NB, NRI = a.NRI;
-    const JasParams &jp = *a.jas;
-    const double kapd = jp.kap;
-    const T kap = (T)jp.kap;
+    // the Jastrow parameters and the centres' types in LDS (read before use
+    // behind the group loop's first barrier): every centre step of every item
+    // reads a type's e-e-n coefficients, every epilogue entry the e-e ones
+    __shared__ JasParams sJ;
+    __shared__ int sT[128];
+    static_assert(sizeof(JasParams) % 8 == 0, "JasParams copied as 8-byte words");
+    for (int t = tid; t < (int)(sizeof(JasParams) / 8); t += MMA_THREADS)
+        reinterpret_cast<unsigned long long *>(&sJ)[t] = reinterpret_cast<const unsigned long long *>(a.jas)[t];
+    for (int t = tid; t < NC; t += MMA_THREADS) sT[t] = a.iwctype[t];
+    const JasParams &jp = sJ;
+    const double kapd = a.jas->kap;  // read directly: sJ is complete only after a barrier
+    const T kap = (T)kapd;
     double *XP = lds;                                          // [MP][3]
     double *SP = XP + 3 * MMA_MP;                              // [3][MP]: e-e + e-n, e-e-n, psi ratio
     double *TP = SP + 3 * MMA_MP;                              // [2][TG][MP] partial sums
@@ -587,11 +599,11 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
                 for (int h = wv * per; h < h_hi;) {
                     const int tile = h / 2;
                     if (h % 2 == 0 && h + 1 < h_hi) {
-                        mma_item<NORDC, T>(a, w, tile * 16, 0, NC, npts, XP, XQ, IPT, swv, true);
+                        mma_item<NORDC, T>(a, w, tile * 16, 0, NC, npts, XP, XQ, IPT, swv, true, sJ, sT);
                         h += 2;
                     } else {
                         mma_item<NORDC, T>(a, w, tile * 16, h % 2 ? ncm : 0, h % 2 ? NC : ncm, npts, XP, XQ, IPT, swv,
-                                           h % 2 == 0);
+                                           h % 2 == 0, sJ, sT);
                         h += 1;
                     }
                     __syncwarp();
@@ -824,8 +836,12 @@ static int launch_mma(champ_ctx *c, const HArgs &a0)
 {
     HArgs a = a0;
     const bool split = CHAMP_NL_SPLIT && psi_lds_bytes(c->NC, c->NB, c->NRI) <= 160 * 1024;
+    int slot;
     if (split) {
-        if (int rc = launch_psi(c, a)) return rc;
+        tbegin(c, K_HPSI_PSI, &slot);
+        const int rc = launch_psi(c, a);
+        tend(c, slot);
+        if (rc) return rc;
         a.psi_ext = 1;
     }
     // without the psi phase the tables after XQ (point-centre distances,
@@ -846,7 +862,9 @@ static int launch_mma(champ_ctx *c, const HArgs &a0)
     HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonloc_mma<NORDC, T>, MMA_THREADS, lds));
     per_cu = std::max(1, per_cu);
+    tbegin(c, K_HPSI_NONLOC, &slot);
     hipLaunchKernelGGL((k_nonloc_mma<NORDC, T>), dim3(ncu * per_cu), dim3(MMA_THREADS), lds, c->stream, a);
+    tend(c, slot);
     HIPCHECK(hipGetLastError());
     return CHAMP_OK;
 }
@@ -868,7 +886,10 @@ static int launch_k(champ_ctx *c, const HArgs &a)
     HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nonloc<NORDC, QB, T>, NL_THREADS, lds));
     per_cu = std::max(1, per_cu);
+    int slot;
+    tbegin(c, K_HPSI_NONLOC, &slot);
     hipLaunchKernelGGL((k_nonloc<NORDC, QB, T>), dim3(ncu * per_cu), dim3(NL_THREADS), lds, c->stream, a);
+    tend(c, slot);
     HIPCHECK(hipGetLastError());
     return CHAMP_OK;
 }
