@@ -305,6 +305,17 @@ struct JasParams {
 template <typename T>
 __device__ __forceinline__ T scale_r(T kap, T r) { return (T(1) - exp(-kap * r)) / kap; }
 
+// The partner side of an (electron, centre) pair as the FP32 move kernel uses
+// it: the switched scaled distance x = 1 - kap rr and the factor
+// -kap x / r of its gradient; (0, 0) at the asymptote rr = 1/kap, so x == 0
+// marks it (jastrow4e skips a pair whose two distances are both there).
+__device__ __forceinline__ float2 enp_pack(double kap, double asymp_r, double r, double rr)
+{
+    if (rr == asymp_r) return make_float2(0.0f, 0.0f);
+    const double sw = 1.0 - kap * rr;
+    return make_float2((float)sw, (float)(-kap * sw / r));
+}
+
 // e-e-n sums for one pair and one centre: jastrow4.f90:203-262 (fc, fu, fi,
 // fj and, with want2, the second derivatives).  uu/ri/rj are the switched
 // scaled distances.  Power tables have zeros at negative indices.

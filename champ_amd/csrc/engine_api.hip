@@ -352,6 +352,7 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
     // ---- walker state
     TRY(dalloc(&c->xT, (size_t)W * 3 * NE));
     TRY(dalloc(&c->r_en, (size_t)W * NC * NE)); TRY(dalloc(&c->rr_en, (size_t)W * NC * NE));
+    if (precision == CHAMP_PREC_MIXED) TRY(dalloc(&c->enp, (size_t)W * NC * NE));
     TRY(dalloc(&c->fso, (size_t)W * NE * NE));
     TRY(dalloc(&c->g1, (size_t)W * NE * 3 * NE)); TRY(dalloc(&c->g2, (size_t)W * NE * 3 * NE));
     TRY(dalloc(&c->vjT, (size_t)W * 3 * NE));
@@ -438,7 +439,7 @@ extern "C" int champ_gpu_free(champ_ctx *c)
                     c->esum_dev, c->aoall, c->orball, c->wmat, c->pair_count, c->pair_off, c->pair_total, c->grp_off,
                     c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->pair_psi, c->rrT, c->nxquad, c->ri_cent, c->ri_type,
                     c->ri_rad, c->ri_rmax, c->ao_ri, c->gref, c->lu_ws, c->m6rec, c->ri_perm,
-                    c->ri_aoff, c->ri_aol};
+                    c->ri_aoff, c->ri_aol, c->enp};
     for (void *p : ptrs) if (p) hipFree(p);
     md_free(c);
     dmc_free(c);

@@ -291,6 +291,7 @@ extern "C" int champ_gpu_dmc_init(champ_ctx *c, int32_t nconf, double tau, doubl
         add(c->ddx, NE * 3 * 8); add(c->d2dx2, NE * 8); add(c->psido, 8); add(c->psijo, 8); add(c->psi2o, 8);
         add(c->eold, 8); add(c->ekin, 8); add(c->pe_local, 8); add(c->vpsp, 16); add(c->status, 4);
         add(c->dmc_age, 4); add(c->dmc_fratio, 8);
+        add(c->enp, NC * NE * 8);  // mixed path only (null otherwise)
         if (c->md) {
             add(c->md_aa, (size_t)2 * 2 * NM * c->md_NV * 8); add(c->md_D, (size_t)2 * 2 * c->md_NU * 8);
             add(c->md_S, 8); add(c->md_cur, 8); add(c->md_V, NE * NO * 8);

@@ -46,8 +46,10 @@ __global__ void __launch_bounds__(512) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
         double r2 = 0;
         r2 = r2 + rv0 * rv0; r2 = r2 + rv1 * rv1; r2 = r2 + rv2 * rv2;
         const double r = sqrt(r2);
+        const double rr = scale_r(kap, r);
         rw[t] = r;
-        rrw[t] = scale_r(kap, r);
+        rrw[t] = rr;
+        if (a.enp) a.enp[(size_t)w * NC * NE + t] = enp_pack(kap, asymp_r, r, rr);
     }
     __syncthreads();
 

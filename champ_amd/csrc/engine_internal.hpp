@@ -141,6 +141,7 @@ struct champ_ctx {
     double *pair_vps = nullptr;                // [W*pair_cap][MPS_L]
     double *pair_val = nullptr;                // [W*pair_cap]
     double *pair_psi = nullptr;                // [W*pair_cap][NQ] psi ratio at each quadrature point
+    float2 *enp = nullptr;                     // mixed path: [W][NC][NE] partner pair table (enp_pack)
     double *rrT = nullptr;                     // [W][NE][NC] rr_en with the centre innermost
     int *nxquad = nullptr;                     // [W]
 

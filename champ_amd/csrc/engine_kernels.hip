@@ -45,6 +45,7 @@ struct MoveArgs {
     int *dmc_age;                      // [W] steps since the walker last moved
     double *m6rec;                     // [W][M6_REC]
     const EcpParams *ecp;              // znuc per type and nloc (metrop6's cusp and theta range)
+    float2 *enp;                       // mixed path: [W][NC][NE] partner pair table (enp_pack), else null
 };
 
 static MoveArgs move_args(champ_ctx *c, int iel, int mode, double tau)
@@ -57,6 +58,7 @@ static MoveArgs move_args(champ_ctx *c, int iel, int mode, double tau)
     a.iworbd = c->iworbd; a.jas = c->jas;
     a.NRI = c->NRI; a.ri_cent = c->ri_cent; a.ri_type = c->ri_type; a.ri_rad = c->ri_rad; a.ao_ri = c->ao_ri;
     a.ri_rmax = c->ri_rmax;
+    a.enp = c->enp;
     a.xT = c->xT; a.r_en = c->r_en; a.rr_en = c->rr_en; a.fso = c->fso; a.g1 = c->g1; a.g2 = c->g2;
     a.vjT = c->vjT; a.fsumo = c->fsumo; a.dorb = c->dorb; a.slmi = c->slmi; a.detiab = c->detiab;
     a.psido = c->psido; a.psijo = c->psijo; a.psi2o = c->psi2o; a.rng = c->rng;
@@ -352,25 +354,39 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                     jcl[s] = j < NE ? j : 0;
                 }
                 constexpr int MOVE_PF = move_pf<T>();
+                // FP32 path: the partner pair table (x, gradient factor), 8
+                // bytes per pair instead of the two FP64 distances
+                constexpr bool F32 = sizeof(T) == sizeof(float);
+                const float2 *ew = F32 ? a.enp + (size_t)w * NC * NE : nullptr;
                 double rjc[MOVE_PF][MJB], rrjc[MOVE_PF][MJB], rjn[MOVE_PF][MJB], rrjn[MOVE_PF][MJB];
+                float2 ejc[MOVE_PF][MJB], ejn[MOVE_PF][MJB];
 #pragma unroll
                 for (int p = 0; p < MOVE_PF; p++)
 #pragma unroll
                     for (int s = 0; s < MJB; s++) {
                         const int c = p < NC ? p : 0;
-                        rjn[p][s] = rw[(size_t)c * NE + jcl[s]];
-                        rrjn[p][s] = rrw[(size_t)c * NE + jcl[s]];
+                        if constexpr (F32) {
+                            ejn[p][s] = ew[(size_t)c * NE + jcl[s]];
+                        } else {
+                            rjn[p][s] = rw[(size_t)c * NE + jcl[s]];
+                            rrjn[p][s] = rrw[(size_t)c * NE + jcl[s]];
+                        }
                     }
                 for (int c0 = 0; c0 < NC; c0 += MOVE_PF) {
 #pragma unroll
                     for (int p = 0; p < MOVE_PF; p++)
 #pragma unroll
                         for (int s = 0; s < MJB; s++) {
-                            rjc[p][s] = rjn[p][s]; rrjc[p][s] = rrjn[p][s];
                             const int cn = c0 + MOVE_PF + p;
-                            if (cn < NC) {
-                                rjn[p][s] = rw[(size_t)cn * NE + jcl[s]];
-                                rrjn[p][s] = rrw[(size_t)cn * NE + jcl[s]];
+                            if constexpr (F32) {
+                                ejc[p][s] = ejn[p][s];
+                                if (cn < NC) ejn[p][s] = ew[(size_t)cn * NE + jcl[s]];
+                            } else {
+                                rjc[p][s] = rjn[p][s]; rrjc[p][s] = rrjn[p][s];
+                                if (cn < NC) {
+                                    rjn[p][s] = rw[(size_t)cn * NE + jcl[s]];
+                                    rrjn[p][s] = rrw[(size_t)cn * NE + jcl[s]];
+                                }
                             }
                         }
 #pragma unroll
@@ -386,14 +402,21 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                         for (int s = 0; s < MJB; s++) {
                             if (j0 + s * WAVE >= NE) break;
                             if (!on[s]) continue;
-                            if (rrm == asymp_r && rrjc[p][s] == asymp_r) continue;
-                            const T rjo = (T)rjc[p][s];
-                            const T swj = (T)(1.0 - kap * rrjc[p][s]);
+                            T swj;
+                            if constexpr (F32) {
+                                swj = ejc[p][s].x;
+                                if (rrm == asymp_r && swj == 0.0f) continue;
+                            } else {
+                                if (rrm == asymp_r && rrjc[p][s] == asymp_r) continue;
+                                swj = (T)(1.0 - kap * rrjc[p][s]);
+                            }
                             const EenOut<T> e = een_eval_tab<NORDC, 1, T>(tab, uu1[s], swj);
                             fs[s] += e.f;
                             sfu[s] += e.fu;
                             const T fmv = e.fx * fmf;
-                            const T fot = e.fy * (-tkap * swj) / rjo;
+                            T fot;
+                            if constexpr (F32) fot = e.fy * ejc[p][s].y;
+                            else fot = e.fy * (-tkap * swj) / (T)rjc[p][s];
                             gm[s][0] += fmv * rvm0; gm[s][1] += fmv * rvm1; gm[s][2] += fmv * rvm2;
                             go[s][0] += fot * (xj[s][0] - cx); go[s][1] += fot * (xj[s][1] - cy);
                             go[s][2] += fot * (xj[s][2] - cz);
@@ -664,8 +687,10 @@ __global__ void __launch_bounds__(256) k_move_b(MoveArgs a)
     // position and distances
     if (tid < 3) a.xT[((size_t)w * 3 + tid) * NE + iel] = a.xnew[3 * w + tid];
     for (int c = tid; c < a.NC; c += blockDim.x) {
-        a.r_en[((size_t)w * a.NC + c) * NE + iel] = a.rnew[(size_t)w * a.NC + c];
-        a.rr_en[((size_t)w * a.NC + c) * NE + iel] = a.rrnew[(size_t)w * a.NC + c];
+        const double rn = a.rnew[(size_t)w * a.NC + c], rrn = a.rrnew[(size_t)w * a.NC + c];
+        a.r_en[((size_t)w * a.NC + c) * NE + iel] = rn;
+        a.rr_en[((size_t)w * a.NC + c) * NE + iel] = rrn;
+        if (a.enp) a.enp[((size_t)w * a.NC + c) * NE + iel] = enp_pack(a.jas->kap, a.jas->asymp_r, rn, rrn);
     }
     // jassav (iflag = 0)
     const double *fsn = a.fsn_row + (size_t)w * NE;

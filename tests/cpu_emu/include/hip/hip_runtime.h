@@ -117,6 +117,12 @@ inline int __builtin_amdgcn_readfirstlane(int v)
 // and holds D[4 r + l / 16][l % 16] in accumulator element r = 0..3 (the
 // layout measured on gfx950 with a one-hot probe of A and B)
 #define CHAMP_DBL4_DEFINED
+// HIP's float2 vector type and its constructor
+struct float2 {
+    float x, y;
+};
+inline float2 make_float2(float x, float y) { return float2{x, y}; }
+
 struct dbl4 {
     double v[4];
     double &operator[](int i) { return v[i]; }
