@@ -115,24 +115,37 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int M, int N, int K, const do
     dbl4 acc[WN];
 #pragma unroll
     for (int t = 0; t < WN; t++) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    // the operands of chunk k0 + 16 are loaded while chunk k0 runs on the
+    // matrix cores (one chunk of software pipelining)
+    auto load = [&](int k0, double *a, double (*b)[4]) {
         const int kb = k0 + kq;
-        double a[4];
 #pragma unroll
         for (int s = 0; s < 4; s++) a[s] = (rok && kb + s < K) ? Ar[kb + s] : 0.0;
 #pragma unroll
-        for (int t = 0; t < WN; t++) {
-            double b[4];
+        for (int t = 0; t < WN; t++)
 #pragma unroll
             for (int s = 0; s < 4; s++) {
                 const bool ok = cok[t] && kb + s < K;
                 const long long off = BT ? (long long)(cok[t] ? col[t] : 0) * ldb + (kb + s < K ? kb + s : 0)
                                          : (long long)(kb + s < K ? kb + s : 0) * ldb + (cok[t] ? col[t] : 0);
-                b[s] = ok ? B[off] : 0.0;
+                b[t][s] = ok ? B[off] : 0.0;
             }
+    };
+    double an[4], bn[WN][4];
+    load(0, an, bn);
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        double a[4], b[WN][4];
 #pragma unroll
-            for (int s = 0; s < 4; s++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc[t], 0, 0, 0);
-        }
+        for (int s = 0; s < 4; s++) a[s] = an[s];
+#pragma unroll
+        for (int t = 0; t < WN; t++)
+#pragma unroll
+            for (int s = 0; s < 4; s++) b[t][s] = bn[t][s];
+        if (k0 + 16 < K) load(k0 + 16, an, bn);
+#pragma unroll
+        for (int t = 0; t < WN; t++)
+#pragma unroll
+            for (int s = 0; s < 4; s++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[t][s], acc[t], 0, 0, 0);
     }
 #pragma unroll
     for (int t = 0; t < WN; t++) {

@@ -725,6 +725,8 @@ __global__ void __launch_bounds__(PSI_THREADS) k_nonloc_psi(HArgs a)
     int *GI = PI + 2 * MMA_GP;                         // [MP][NC] grid interval of r
     int *SMI = GI + (size_t)PSI_MP * NC;               // [NRI][5] centre, type, radial fn, AO range
     int *SAO = SMI + 5 * (size_t)NRI;                  // [NB][2] slm code, AO index
+    __shared__ double sCent[3 * 128];                  // nuclear positions (ncent <= 128)
+    for (int t = tid; t < 3 * NC; t += PSI_THREADS) sCent[t] = a.cent[t];
     for (int sidx = tid; sidx < NRI; sidx += PSI_THREADS) {
         const int ri = a.ri_perm[sidx];
         SMI[5 * sidx] = a.ri_cent[ri]; SMI[5 * sidx + 1] = a.ri_type[ri]; SMI[5 * sidx + 2] = a.ri_rad[ri];
@@ -763,13 +765,13 @@ __global__ void __launch_bounds__(PSI_THREADS) k_nonloc_psi(HArgs a)
             const int i = PI[2 * p], ic = PI[2 * p + 1];
             const double ren = a.r_en[((size_t)w * NC + ic) * NE + i];
             const double *qv = (k == 0 ? a.xq : (k == 1 ? a.yq : a.zq)) + (size_t)w * NQ;
-            XP[t] = ren * qv[q] + a.cent[3 * ic + k];
+            XP[t] = ren * qv[q] + sCent[3 * ic + k];
         }
         __syncthreads();
         for (int t = tid; t < npts * NC; t += PSI_THREADS) {
             const int q = t / NC, c = t % NC, p = q / NQ;
             const double *xp = XP + 3 * q;
-            const double e0 = xp[0] - a.cent[3 * c], e1 = xp[1] - a.cent[3 * c + 1], e2 = xp[2] - a.cent[3 * c + 2];
+            const double e0 = xp[0] - sCent[3 * c], e1 = xp[1] - sCent[3 * c + 1], e2 = xp[2] - sCent[3 * c + 2];
             double r2 = 0;
             r2 = r2 + e0 * e0; r2 = r2 + e1 * e1; r2 = r2 + e2 * e2;
             const double rqd = c == PI[2 * p + 1] ? a.r_en[((size_t)w * NC + c) * NE + PI[2 * p]] : sqrt(r2);
@@ -793,7 +795,7 @@ __global__ void __launch_bounds__(PSI_THREADS) k_nonloc_psi(HArgs a)
                         if (!(r <= SRM[sidx])) continue;  // beyond the shell's cut radius: its AOs are exactly zero
                         double wfv[3] = {0.0, 0.0, 0.0};
                         splfit_jx<double>(a.tab, mi[1], mi[2], r, gi[cen], wfv, 0);
-                        const double d0 = px - a.cent[3 * cen], d1 = py - a.cent[3 * cen + 1], d2 = pz - a.cent[3 * cen + 2];
+                        const double d0 = px - sCent[3 * cen], d1 = py - sCent[3 * cen + 1], d2 = pz - sCent[3 * cen + 2];
                         double sa = 0.0;
                         for (int k = mi[3]; k < mi[4]; k++)
                             sa += slm_value<double>(slm_decode(SAO[2 * k]), d0, d1, d2) * Wm[SAO[2 * k + 1]];
