@@ -359,12 +359,15 @@ __device__ __forceinline__ void mma_item(const HArgs &a, int w, int j0, int c_lo
     for (int mt = 0; mt < MMA_MT; mt++)
 #pragma unroll
         for (int k = 0; k < KR; k++) acc[mt][k] = Mma<T>::zero();
+    // partner distances two centre steps ahead
     double ynext = (jv && c_lo + sl < c_hi) ? rrw[(size_t)(c_lo + sl) * NE + j] : 0.0;
+    double ynext2 = (jv && c_lo + sl + 4 < c_hi) ? rrw[(size_t)(c_lo + sl + 4) * NE + j] : 0.0;
     for (int b0 = c_lo; b0 < c_hi; b0 += 4) {
         const int b = b0 + sl;
         const bool bv = b < c_hi;
         const double rrb = ynext;
-        if (b + 4 < c_hi && jv) ynext = rrw[(size_t)(b + 4) * NE + j];  // next centre's distance in flight
+        ynext = ynext2;
+        if (b + 8 < c_hi && jv) ynext2 = rrw[(size_t)(b + 8) * NE + j];
         // the partner's table entries at centre b: built once, used by every row tile
         T tab[NA];
         een_tables<N, 0, T>(jp.c[bv ? sT[b] : 0], (T)(1.0 - kap * rrb), tab);
@@ -669,28 +672,32 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
         }  // !psi_ext
         __syncthreads();
 
-        // ---- the vpsp_det terms (nonloc.f90:248-274), one thread per pair
-        if (tid < np) {
-            const int p = tid, i = PI[4 * p], ic = PI[4 * p + 1];
+        // ---- the vpsp_det terms (nonloc.f90:248-274): a thread per point
+        // forms its term, then a thread per pair sums its points in order
+        double *TRJ = TP;  // [MP], free once the point sums are in SP
+        if (tid < npts) {
+            const int pt = tid, p = pt / NQ, q = pt % NQ, i = PI[4 * p], ic = PI[4 * p + 1];
             const size_t slot = (size_t)w * a.cap + p0 + p;
-            const int lp = a.ecp->lpot[a.iwctype[ic]];
+            const int lp = a.ecp->lpot[sT[ic]];
             const double cx = a.cent[3 * ic], cy = a.cent[3 * ic + 1], cz = a.cent[3 * ic + 2];
             const double rv0 = xw[i] - cx, rv1 = xw[NE + i] - cy, rv2 = xw[2 * NE + i] - cz;
             const double ri = 1.0 / a.r_en[((size_t)w * NC + ic) * NE + i];
             const double *qx = a.xq + (size_t)w * NQ, *qy = a.yq + (size_t)w * NQ, *qz = a.zq + (size_t)w * NQ;
+            const double fsumn = SP[pt] + SP[MMA_MP + pt] - FI[p];
+            double costh = rv0 * qx[q] + rv1 * qy[q] + rv2 * qz[q];
+            costh = costh * ri;
+            double term_radial = 0.0;
+            for (int l = 1; l <= lp - 1; l++)
+                term_radial = term_radial + yl0(l, costh) * a.pair_vps[slot * CHAMP_MPS_L + l - 1];
+            const double trj = term_radial * a.ecp->wq[q] * exp(fsumn);
+            TRJ[pt] = trj * (psi_ext ? a.pair_psi[slot * NQ + q] : SP[2 * MMA_MP + pt]);
+        }
+        __syncthreads();
+        if (tid < np) {
+            const int p = tid;
             double pair_sum = 0.0;
-            for (int q = 0; q < NQ; q++) {
-                const int pt = p * NQ + q;
-                const double fsumn = SP[pt] + SP[MMA_MP + pt] - FI[p];
-                double costh = rv0 * qx[q] + rv1 * qy[q] + rv2 * qz[q];
-                costh = costh * ri;
-                double term_radial = 0.0;
-                for (int l = 1; l <= lp - 1; l++)
-                    term_radial = term_radial + yl0(l, costh) * a.pair_vps[slot * CHAMP_MPS_L + l - 1];
-                const double trj = term_radial * a.ecp->wq[q] * exp(fsumn);
-                pair_sum = pair_sum + trj * (psi_ext ? a.pair_psi[slot * NQ + q] : SP[2 * MMA_MP + pt]);
-            }
-            a.pair_val[slot] = pair_sum;
+            for (int q = 0; q < NQ; q++) pair_sum = pair_sum + TRJ[p * NQ + q];
+            a.pair_val[(size_t)w * a.cap + p0 + p] = pair_sum;
         }
     }
 }
