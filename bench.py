@@ -195,12 +195,13 @@ def run_hetero(args):
     return out
 
 
-# HBM bytes per launch of the quadrature kernel at the default workload (c160,
-# 4096 walkers, the timed sweeps after ten warm-up sweeps): FETCH_SIZE x 2 +
-# WRITE_SIZE x 1 from two rocprofv3 --pmc passes over the default bench,
+# HBM bytes per launch of the matrix-core quadrature kernel (k_nonloc_mma,
+# the psi ratio excluded: it has its own kernel) at the default workload
+# (c160, 4096 walkers, the timed sweep after ten warm-up sweeps): FETCH_SIZE
+# x 2 + WRITE_SIZE x 1 from two rocprofv3 --pmc passes over the default bench,
 # scaled by the same counters on a device-to-device copy of known size
 # (profiles/k_nonloc_c160_traffic.md)
-NONLOC_TRAFFIC = {("c160", 4096, "fp64"): 3.46e9, ("c160", 4096, "mixed"): 3.45e9}
+NONLOC_TRAFFIC = {("c160", 4096, "fp64"): 1.29e9, ("c160", 4096, "mixed"): 1.20e9}
 
 PEAK = {"fp64-mfma": 78.6, "fp32-mfma": 157.3, "fp64-valu": 78.6,
         # FP32 without packed math (this build never emits v_pk_* FP32): one
