@@ -305,6 +305,17 @@ struct JasParams {
 template <typename T>
 __device__ __forceinline__ T scale_r(T kap, T r) { return (T(1) - exp(-kap * r)) / kap; }
 
+// Electron-centre distance of electron e of a walker (xw [3][NE]): the one
+// expression every kernel that forms r_en uses, so recomputed values equal
+// the stored ones exactly.
+__device__ __forceinline__ double en_distance(const double *xw, int NE, int e, const double *cent, int c)
+{
+    const double rv0 = xw[e] - cent[3 * c], rv1 = xw[NE + e] - cent[3 * c + 1], rv2 = xw[2 * NE + e] - cent[3 * c + 2];
+    double r2 = 0;
+    r2 = r2 + rv0 * rv0; r2 = r2 + rv1 * rv1; r2 = r2 + rv2 * rv2;
+    return sqrt(r2);
+}
+
 // The partner side of an (electron, centre) pair as the FP32 move kernel uses
 // it: the switched scaled distance x = 1 - kap rr and the factor
 // -kap x / r of its gradient; (0, 0) at the asymptote rr = 1/kap, so x == 0

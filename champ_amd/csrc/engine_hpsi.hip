@@ -42,10 +42,7 @@ __global__ void __launch_bounds__(512) CHAMP_WAVES_PER_EU(CHAMP_HJAS_WPE) k_hpsi
 
     for (int t = tid; t < NC * NE; t += bd) {
         const int c = t / NE, e = t % NE;
-        const double rv0 = xw[e] - a.cent[3 * c], rv1 = xw[NE + e] - a.cent[3 * c + 1], rv2 = xw[2 * NE + e] - a.cent[3 * c + 2];
-        double r2 = 0;
-        r2 = r2 + rv0 * rv0; r2 = r2 + rv1 * rv1; r2 = r2 + rv2 * rv2;
-        const double r = sqrt(r2);
+        const double r = en_distance(xw, NE, e, a.cent, c);
         const double rr = scale_r(kap, r);
         rw[t] = r;
         rrw[t] = rr;
@@ -279,13 +276,14 @@ __global__ void __launch_bounds__(256) k_hpsi_ao(HArgs a)
     const int wc = (int)(gw / a.NE), e = (int)(gw % a.NE), w = a.w0 + wc;
     const int NE = a.NE, NRI = a.NRI;
     const double *xw = a.xT + (size_t)w * 3 * NE;
-    const double *rw = a.r_en + (size_t)w * a.NC * NE;
     double *out = a.aoall + (size_t)gw * NCOMP * a.NB;
     const double x0 = xw[e], x1 = xw[NE + e], x2 = xw[2 * NE + e];
-    // radial functions (R, R', R'') once per (centre, shell), in the wave's LDS
+    // radial functions (R, R', R'') once per (centre, shell), in the wave's LDS.
+    // The distances are formed here (the expression that wrote r_en): faster
+    // than reading them back from r_en.
     T *R = (T *)lds + (size_t)wv * 3 * NRI;
     for (int ri = lane; ri < NRI; ri += WAVE) {
-        const double r = rw[(size_t)a.ri_cent[ri] * NE + e];
+        const double r = en_distance(xw, NE, e, a.cent, a.ri_cent[ri]);
         T wfv[3] = {T(0), T(0), T(0)};
         if (r <= a.ri_rmax[ri])
             splfit_jx<T>(a.tab, a.ri_type[ri], a.ri_rad[ri], (T)r, grid_jx(a.tab, r), wfv, NCOMP == 1 ? 0 : 2);
@@ -306,7 +304,7 @@ __global__ void __launch_bounds__(256) k_hpsi_ao(HArgs a)
         T o[5] = {T(0), T(0), T(0), T(0), T(0)};
         if (Rm[0] != T(0) || Rm[1] != T(0) || Rm[2] != T(0))  // else beyond the cut radius: exactly zero
             ao_combine<T>(a.ao_slm[m], (T)(x0 - a.cent[3 * c]), (T)(x1 - a.cent[3 * c + 1]),
-                          (T)(x2 - a.cent[3 * c + 2]), (T)rw[(size_t)c * NE + e], Rm, 2, o);
+                          (T)(x2 - a.cent[3 * c + 2]), (T)en_distance(xw, NE, e, a.cent, c), Rm, 2, o);
         for (int k = 0; k < 5; k++) out[(size_t)k * a.NB + m] = (double)o[k];
     }
 }

@@ -204,6 +204,10 @@ inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, F, int, s
 hipError_t hipGetLastError();
 const char *hipGetErrorString(hipError_t e);
 hipError_t hipEventCreate(hipEvent_t *e);
+#define hipEventDisableTiming 0x2
+hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned flags);
+// launches run in issue order on the host, so a cross-stream wait has nothing to wait for
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned flags);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = nullptr);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
