@@ -208,8 +208,11 @@ extern "C" int champ_gpu_init(champ_ctx **ctxp, const champ_system_t *s, int32_t
     c->W = nwalk; c->NE = s->nelec; c->NUP = s->nup; c->NDN = s->nelec - s->nup;
     c->NM = std::max(c->NUP, c->NDN); c->NC = s->ncent; c->NCT = s->nctype; c->NB = s->nbasis; c->NO = s->norb;
     c->NQ = s->nloc ? s->nquad : 0; c->nloc = s->nloc; c->prec = precision; c->cdet0 = s->cdet[0];
-    // mixed path: the FP32-fed Sherman-Morrison inverse is recomputed in FP64 every 80 moves
-    c->lu_every = precision == CHAMP_PREC_MIXED ? 80 : 0;
+    // mixed path: the FP32-fed Sherman-Morrison inverse is recomputed in FP64
+    // every 80 moves for a single determinant (orbital values and one LU); a
+    // multideterminant recompute rebuilds the whole excitation state, as much
+    // as a quarter of a sweep at 5000 determinants, so there it is left to hpsi
+    c->lu_every = (precision == CHAMP_PREC_MIXED && s->ndet == 1) ? 80 : 0;
     if (c->NM > 400) { set_error("init: more than 400 electrons of one spin"); delete c; *ctxp = nullptr; return CHAMP_ERR_UNSUPPORTED; }
     if (hipStreamCreate(&c->stream) != hipSuccess) { set_error("init: stream"); delete c; *ctxp = nullptr; return CHAMP_ERR_HIP; }
 

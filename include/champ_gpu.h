@@ -174,9 +174,10 @@ int champ_gpu_hpsi(champ_ctx *ctx, double *psid, double *psij, double *ekin,
 int champ_gpu_determinant(champ_ctx *ctx, double *psid);
 
 /* Moves between the FP64 recomputes of the Slater inverse inside a metrop1
- * sweep (0: only the one in hpsi).  Defaults: 80 on the mixed path, 0 on the
- * FP64 path.  No counterpart in the reference, which rebuilds the inverse
- * once per sweep in hpsi.                                                     */
+ * sweep (0: only the one in hpsi).  Defaults: 80 on the mixed path with one
+ * determinant, 0 on the FP64 path and for multideterminant trial functions
+ * (whose recompute rebuilds the whole excitation state).  No counterpart in
+ * the reference, which rebuilds the inverse once per sweep in hpsi.          */
 int champ_gpu_set_lu_every(champ_ctx *ctx, int32_t nmoves);
 
 /* With ndet > 1 the multideterminant parts run inside these same calls:
