@@ -514,6 +514,32 @@ extern "C" int champ_gpu_set_params(champ_ctx *c, const double *a4, const double
     return rc;
 }
 
+extern "C" int champ_gpu_set_orbitals(champ_ctx *c, const double *coef)
+{
+    if (!c || !coef) return CHAMP_ERR_ARG;
+    const int NO = c->NO, NB = c->NB, NE = c->NE;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipMemcpy(c->coef, coef, (size_t)NO * NB * sizeof(double), hipMemcpyHostToDevice));
+    if (c->coef32) {
+        std::vector<float> cf((size_t)NO * NB);
+        for (size_t t = 0; t < cf.size(); t++) cf[t] = (float)coef[t];
+        HIPCHECK(hipMemcpy(c->coef32, cf.data(), cf.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    // the occupied rows of the reference determinant per spin (the ECP rows' GEMM)
+    std::vector<int> iw(NE);
+    HIPCHECK(hipMemcpy(iw.data(), c->iworbd, (size_t)NE * sizeof(int), hipMemcpyDeviceToHost));
+    for (int iab = 0; iab < 2; iab++) {
+        const int nel = iab ? c->NDN : c->NUP, ish = iab ? c->NUP : 0;
+        std::vector<double> h((size_t)std::max(nel, 1) * NB, 0.0);
+        for (int j = 0; j < nel; j++) std::memcpy(&h[(size_t)j * NB], coef + (size_t)iw[ish + j] * NB, sizeof(double) * NB);
+        HIPCHECK(hipMemcpy(c->cocc[iab], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (c->sweep_exec) { hipGraphExecDestroy(c->sweep_exec); c->sweep_exec = nullptr; }
+    int rc = launch_hpsi_impl(c, 0);
+    if (!rc) HIPCHECK(hipStreamSynchronize(c->stream));
+    return rc;
+}
+
 extern "C" int champ_gpu_get_walkers(champ_ctx *c, double *x)
 {
     if (!c || !x) return CHAMP_ERR_ARG;

@@ -20,6 +20,12 @@
 //         O = sum over pairs and centres of the type of u^k (x^l + y^l) (x y)^m
 //         with the switched distances u, x, y
 //   CI    cdet[k] of every determinant (ndet > 1): O = D_up,k D_dn,k / psi_D
+//   ORB   the orbital terms (io, jo) set by champ_gpu_set_orb_terms, the
+//         excitation phi_io += kappa phi_jo of compute_lcao
+//         (optwf_handle_wf.f90:1128-1134): O = sum_e V[e][io] phi_jo(r_e)
+//         with V[e][o] = d ln psi_D / d phi_o(r_e) (the reference inverse
+//         for one determinant, the multideterminant V otherwise), the
+//         orbprim of optorb_deriv (optorb.f90:5-97)
 // A parameter map (champ_gpu_set_deriv_map) turns these into a caller's own
 // parameters, e.g. CHAMP's iwjasa/iwjasb/iwjasc lists with the e-e-n cusp
 // dependencies and the CSF combinations: O' = O M.
@@ -36,8 +42,8 @@
 namespace champ {
 
 struct DerLayout {
-    int na, nb, nc, nci;  // parameters per group
-    int n;                // all of them
+    int na, nb, nc, nci, norb;  // parameters per group
+    int n;                      // all of them
 };
 
 static DerLayout der_layout(const champ_ctx *c, int kinds)
@@ -48,7 +54,8 @@ static DerLayout der_layout(const champ_ctx *c, int kinds)
     L.nb = (kinds & CHAMP_DERIV_JAS_EE) ? j.nspin2b * (j.nordb + 1) : 0;
     L.nc = (kinds & CHAMP_DERIV_JAS_EEN) ? j.nctype * j.nterms : 0;
     L.nci = ((kinds & CHAMP_DERIV_CI) && c->md) ? c->md_ndet : 0;
-    L.n = L.na + L.nb + L.nc + L.nci;
+    L.norb = (kinds & CHAMP_DERIV_ORB) ? c->der_norb : 0;
+    L.n = L.na + L.nb + L.nc + L.nci + L.norb;
     return L;
 }
 
@@ -64,6 +71,10 @@ void deriv_free(champ_ctx *c)
 {
     deriv_free_scratch(c);
     if (c->der_map) hipFree(c->der_map);
+    if (c->der_orb) hipFree(c->der_orb);
+    if (c->der_occ) hipFree(c->der_occ);
+    c->der_orb = c->der_occ = nullptr;
+    c->der_norb = 0;
     c->der_map = nullptr;
     c->der_map_kinds = c->der_map_nraw = c->der_map_nout = 0;
 }
@@ -206,6 +217,39 @@ __global__ void k_deriv_ci(DerArgs a, int off)
     a.ot[(size_t)(off + k) * a.W + w] = a.ph[k] * Du[a.du[k]] * Dd[a.dd[k]] / a.S[w];
 }
 
+// Orbital terms: a wave per term, lanes over electrons, for the walkers
+// [w0, w0 + nw) whose orbital values are in orb ([nw][NE][NO]).
+struct DerOrbArgs {
+    int W, NE, NUP, NM, NO, w0, nterm, off, md;
+    const int *terms, *occ;
+    const double *orb, *slmi, *V;
+    double *ot;
+};
+
+__global__ void __launch_bounds__(256) k_deriv_orb(DerOrbArgs a)
+{
+    const int wc = blockIdx.x, w = a.w0 + wc, lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+    const int nwv = blockDim.x / WAVE, NE = a.NE, NO = a.NO;
+    const double *orb = a.orb + (size_t)wc * NE * NO;
+    for (int k = wv; k < a.nterm; k += nwv) {
+        const int io = a.terms[2 * k], jo = a.terms[2 * k + 1];
+        double s = 0.0;
+        for (int e = lane; e < NE; e += WAVE) {
+            double v;
+            if (a.md) {
+                v = a.V[((size_t)w * NE + e) * NO + io];
+            } else {
+                const int sp = e < a.NUP ? 0 : 1, el = e - (sp ? a.NUP : 0), j = a.occ[sp * NO + io];
+                // slmi row el holds (A^-1)[j][el] at column j: d ln det / d phi_{o_j}(r_e)
+                v = j >= 0 ? a.slmi[((size_t)w * 2 + sp) * a.NM * a.NM + (size_t)el * a.NM + j] : 0.0;
+            }
+            s += v * orb[(size_t)e * NO + jo];
+        }
+        s = wave_sum(s);
+        if (lane == 0) a.ot[(size_t)(a.off + k) * a.W + w] = s;
+    }
+}
+
 // sums over the walkers in walker order: v[k] = sum_w O_kw, v[n + k] = sum_w E_w O_kw
 __global__ void k_deriv_sums(int n, int W, const double *ot, const double *eold, double *v)
 {
@@ -274,6 +318,21 @@ static int deriv_compute(champ_ctx *c, int kinds, const double **ot, int *n)
         hipLaunchKernelGGL(k_deriv_ci, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, a,
                            L.na + L.nb + L.nc);
         HIPCHECK(hipGetLastError());
+    }
+    if (L.norb) {
+        DerOrbArgs o;
+        o.W = c->W; o.NE = c->NE; o.NUP = c->NUP; o.NM = c->NM; o.NO = c->NO; o.nterm = L.norb;
+        o.off = L.na + L.nb + L.nc + L.nci; o.md = c->md;
+        o.terms = c->der_orb; o.occ = c->der_occ; o.orb = c->orball; o.slmi = c->slmi; o.V = c->md_V;
+        o.ot = c->der_ot;
+        // orbital values chunk by chunk in the hpsi scratch (hpsi rewrites it)
+        for (int w0 = 0; w0 < c->W; w0 += c->hpsi_chunk) {
+            const int nw = std::min(c->hpsi_chunk, c->W - w0);
+            if ((rc = launch_orbital_values(c, w0, nw))) return rc;
+            o.w0 = w0;
+            hipLaunchKernelGGL(k_deriv_orb, dim3(nw), dim3(256), 0, c->stream, o);
+            HIPCHECK(hipGetLastError());
+        }
     }
     if (mapped) {
         if ((rc = grow(&c->der_om, &c->der_om_cap, nout, (size_t)c->W * sizeof(double)))) return rc;
@@ -383,7 +442,11 @@ extern "C" int champ_gpu_set_deriv_map(champ_ctx *c, int32_t kinds, int32_t nout
     c->der_map = nullptr;
     c->der_map_kinds = c->der_map_nraw = c->der_map_nout = 0;
     if (!map || nout == 0) return CHAMP_OK;
-    if (nout < 0 || (kinds & ~15)) { set_error("set_deriv_map: bad kinds or nout"); return CHAMP_ERR_ARG; }
+    const int raw_bits = CHAMP_DERIV_JAS_EN | CHAMP_DERIV_JAS_EE | CHAMP_DERIV_JAS_EEN | CHAMP_DERIV_CI | CHAMP_DERIV_ORB;
+    if (nout < 0 || (kinds & ~raw_bits)) {
+        set_error("set_deriv_map: bad kinds or nout");
+        return CHAMP_ERR_ARG;
+    }
     const int nraw = der_layout(c, kinds).n;
     if (nraw == 0) { set_error("set_deriv_map: no parameters selected (kinds) for this trial function"); return CHAMP_ERR_ARG; }
     std::vector<double> mt((size_t)nout * nraw);
@@ -394,6 +457,37 @@ extern "C" int champ_gpu_set_deriv_map(champ_ctx *c, int32_t kinds, int32_t nout
     c->der_map_kinds = kinds;
     c->der_map_nraw = nraw;
     c->der_map_nout = nout;
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_set_orb_terms(champ_ctx *c, int32_t nterm, const int32_t *ideriv)
+{
+    if (!c || nterm < 0 || (nterm > 0 && !ideriv)) return CHAMP_ERR_ARG;
+    for (int k = 0; k < 2 * nterm; k++)
+        if (ideriv[k] < 0 || ideriv[k] >= c->NO) { set_error("set_orb_terms: orbital index outside 0..norb-1"); return CHAMP_ERR_ARG; }
+    if (c->der_orb) hipFree(c->der_orb);
+    if (c->der_occ) hipFree(c->der_occ);
+    c->der_orb = c->der_occ = nullptr;
+    c->der_norb = 0;
+    if (nterm == 0) return CHAMP_OK;
+    // the slot of each orbital in the reference determinant, per spin
+    std::vector<int> iw(c->NE), occ((size_t)2 * c->NO, -1);
+    HIPCHECK(hipMemcpy(iw.data(), c->iworbd, (size_t)c->NE * sizeof(int), hipMemcpyDeviceToHost));
+    for (int e = 0; e < c->NE; e++) {
+        const int sp = e < c->NUP ? 0 : 1;
+        occ[(size_t)sp * c->NO + iw[e]] = e - (sp ? c->NUP : 0);
+    }
+    HIPCHECK(hipMalloc(&c->der_orb, (size_t)2 * nterm * sizeof(int)));
+    HIPCHECK(hipMemcpy(c->der_orb, ideriv, (size_t)2 * nterm * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMalloc(&c->der_occ, occ.size() * sizeof(int)));
+    HIPCHECK(hipMemcpy(c->der_occ, occ.data(), occ.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->der_norb = nterm;
+    // a map over the previous raw layout no longer fits
+    if (c->der_map && (c->der_map_kinds & CHAMP_DERIV_ORB)) {
+        hipFree(c->der_map);
+        c->der_map = nullptr;
+        c->der_map_kinds = c->der_map_nraw = c->der_map_nout = 0;
+    }
     return CHAMP_OK;
 }
 

@@ -823,6 +823,23 @@ static int launch_orbitals_slater(champ_ctx *c, HArgs a, bool values_only = fals
     return CHAMP_OK;
 }
 
+// Orbital values of every electron of walkers [w0, w0 + nw) into orball
+// ([nw][NE][NO]): values-only AOs and one GEMM (the orbital-rotation
+// derivatives read them).
+int launch_orbital_values(champ_ctx *c, int w0, int nw)
+{
+    HArgs a = hargs(c);
+    a.w0 = w0;
+    a.nw = nw;
+    a.ocomp = 1;
+    const long long nwave = (long long)nw * c->NE;
+    hipLaunchKernelGGL((k_hpsi_ao<double, 1>), dim3((unsigned)((nwave + 3) / 4)), dim3(256),
+                       (size_t)4 * 3 * c->NRI * sizeof(double), c->stream, a);
+    HIPCHECK(hipGetLastError());
+    return launch_gemm(c, true, (int)nwave, c->NO, c->NB, c->aoall, c->NB, 0, c->coef, c->NB, 0, c->orball, c->NO, 0, 1,
+                       c->tune_gemm[1]);
+}
+
 // psido / psi2o from refreshed determinants (the Jastrow part is unchanged)
 __global__ void k_refresh_psi(HArgs a)
 {

@@ -127,6 +127,10 @@ struct champ_ctx {
     // parameter map (champ_gpu_set_deriv_map): Mt [nout][nraw] over the raw
     // parameters of der_map_kinds, and the mapped O [nout][W]
     double *der_map = nullptr, *der_om = nullptr;
+    // orbital terms (champ_gpu_set_orb_terms): [nterm][2] (io, jo) and the slot
+    // of each orbital in the reference determinant per spin [2][NO] (-1: empty)
+    int *der_orb = nullptr, *der_occ = nullptr;
+    int der_norb = 0;
     int der_map_kinds = 0, der_map_nraw = 0, der_map_nout = 0, der_om_cap = 0;
     unsigned char *acc = nullptr;
     double *esum_dev = nullptr;  // [4] accepted, esum, esum2, nflag
@@ -219,6 +223,7 @@ void deriv_free(champ_ctx *c);
 int md_init(champ_ctx *c, const champ_system_t *s);
 void md_free(champ_ctx *c);
 int md_set_cdet(champ_ctx *c, const double *cdet);
+int launch_orbital_values(champ_ctx *c, int w0, int nw);
 int launch_md_hpsi(champ_ctx *c, int w0, int nw);
 int launch_md_pre(champ_ctx *c, int iel);
 int launch_md_post(champ_ctx *c, int iel);

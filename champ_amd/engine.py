@@ -40,7 +40,7 @@ class VmcResult(C.Structure):
                 ("accept", C.c_double), ("nsamples", C.c_int64), ("nflagged", C.c_int64)]
 
 
-DERIV_JAS_EN, DERIV_JAS_EE, DERIV_JAS_EEN, DERIV_CI, DERIV_MAPPED = 1, 2, 4, 8, 16
+DERIV_JAS_EN, DERIV_JAS_EE, DERIV_JAS_EEN, DERIV_CI, DERIV_MAPPED, DERIV_ORB = 1, 2, 4, 8, 16, 32
 
 
 class SrInfo(C.Structure):
@@ -101,6 +101,8 @@ def load_library(path: str = LIBPATH):
     L.champ_gpu_sr_sums.argtypes = [vp, i32, dp, dp, dp, dp]
     L.champ_gpu_set_deriv_map.argtypes = [vp, i32, i32, dp]
     L.champ_gpu_set_params.argtypes = [vp, dp, dp, dp, dp]
+    L.champ_gpu_set_orb_terms.argtypes = [vp, i32, C.POINTER(i32)]
+    L.champ_gpu_set_orbitals.argtypes = [vp, dp]
     L.champ_sr_solve.argtypes = [i32, C.c_double, dp, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                  C.c_double, dp, C.POINTER(SrInfo)]
     L.champ_gpu_set_metrop6.argtypes = [vp, C.c_double, C.c_double, C.c_double]
@@ -311,6 +313,19 @@ class Engine:
             if a is not None and a.size != np.asarray(ref).size:
                 raise ValueError("set_params: array size differs from the system's")
         self._check(self.L.champ_gpu_set_params(self.ptr, *[None if a is None else _p(a) for a in arrs]), "set_params")
+
+    def set_orb_terms(self, terms):
+        """Orbital terms (io, jo) of DERIV_ORB (0-based; phi_io += kappa phi_jo)"""
+        t = np.ascontiguousarray(np.asarray(terms, dtype=np.int32).reshape(-1, 2))
+        self._check(self.L.champ_gpu_set_orb_terms(self.ptr, len(t), t.ctypes.data_as(C.POINTER(C.c_int32))),
+                    "set_orb_terms")
+
+    def set_orbitals(self, coef):
+        """New orbital coefficients [norb][nbasis]; the walkers' state is recomputed"""
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        if coef.shape != np.asarray(self.system.coef).shape:
+            raise ValueError("set_orbitals: coef must have the system's shape")
+        self._check(self.L.champ_gpu_set_orbitals(self.ptr, _p(coef)), "set_orbitals")
 
     def sr_sums(self, kinds: int):
         """(sum O O^T, sum O, sum E_L O, sum E_L) over the current walkers"""

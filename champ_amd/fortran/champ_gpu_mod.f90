@@ -225,6 +225,17 @@ module champ_gpu_mod
       integer(c_int32_t), value :: kinds, nout
       real(c_double), intent(in) :: map(*)      ! (nout, nraw)
     end function
+    integer(c_int) function champ_gpu_set_orb_terms(ctx, nterm, ideriv) bind(c, name="champ_gpu_set_orb_terms")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nterm
+      integer(c_int32_t), intent(in) :: ideriv(2, *)   ! (io, jo), 0-based
+    end function
+    integer(c_int) function champ_gpu_set_orbitals(ctx, coef) bind(c, name="champ_gpu_set_orbitals")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: coef(*)          ! (nbasis, norb)
+    end function
     integer(c_int) function champ_gpu_set_params(ctx, a4, b, c, cdet) bind(c, name="champ_gpu_set_params")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
@@ -248,9 +259,10 @@ module champ_gpu_mod
   public :: champ_gpu_set_graphs, champ_gpu_metrop1_batch, champ_gpu_metrop6, champ_gpu_set_metrop6
   public :: champ_gpu_vmc, champ_gpu_dmc_init, champ_gpu_dmc_step, champ_gpu_dmc, champ_gpu_dmc_weights
   public :: champ_gpu_deriv_count, champ_gpu_deriv_psi, champ_gpu_sr_sums, champ_gpu_set_deriv_map
-  public :: champ_gpu_set_params, champ_sr_solve
+  public :: champ_gpu_set_params, champ_sr_solve, champ_gpu_set_orb_terms, champ_gpu_set_orbitals
   integer(c_int32_t), parameter, public :: CHAMP_DERIV_JAS_EN = 1, CHAMP_DERIV_JAS_EE = 2
   integer(c_int32_t), parameter, public :: CHAMP_DERIV_JAS_EEN = 4, CHAMP_DERIV_CI = 8, CHAMP_DERIV_MAPPED = 16
+  integer(c_int32_t), parameter, public :: CHAMP_DERIV_ORB = 32
   public :: gpu_check, gpu_hpsi, gpu_psie, gpu_compute_determinante_grad, gpu_metrop1
 
 contains

@@ -10,7 +10,9 @@ coefficients but the first (or the largest CSF) (optci.f90:63-80,
 parser.f90:1096-1113).  :class:`ChampParams` holds that selection as a matrix
 M [nraw][nparm] (O_champ = O_raw M), which the engine applies on the GPU
 (``Engine.set_deriv_map``), and applies a step the way compute_jastrow /
-compute_ci do (optwf_handle_wf.f90:1050-1104, 1214-1284)."""
+compute_ci do (optwf_handle_wf.f90:1050-1104, 1214-1284).  Orbital
+excitations (io, jo) of optorb come last (phi_io += kappa phi_jo, the update
+of compute_lcao, optwf_handle_wf.f90:1106-1150, with orbitals_ortho off)."""
 from dataclasses import dataclass
 from typing import List, Optional, Tuple
 
@@ -18,7 +20,7 @@ import numpy as np
 
 from . import champ_io as io
 from . import system as S
-from .engine import DERIV_CI, DERIV_JAS_EE, DERIV_JAS_EEN, DERIV_JAS_EN
+from .engine import DERIV_CI, DERIV_JAS_EE, DERIV_JAS_EEN, DERIV_JAS_EN, DERIV_ORB
 
 
 def cusp_jacobian(nordc: int) -> np.ndarray:
@@ -53,17 +55,21 @@ class ChampParams:
     ci: bool
     csf: Optional[Tuple[np.ndarray, list]] = None
     maxcsf: int = 0
+    orb_terms: Optional[np.ndarray] = None  # [nterm][2] (io, jo), 0-based
 
     @property
     def nparm(self) -> int:
         return self.m.shape[1]
 
     @staticmethod
-    def build(sy: S.System, der: Optional[dict] = None, ci: bool = False, csf=None) -> "ChampParams":
+    def build(sy: S.System, der: Optional[dict] = None, ci: bool = False, csf=None,
+              orb_terms=None) -> "ChampParams":
         """The parameter list of optwf with ioptjas = 1 (the lists of ``der``,
         from :func:`champ_io.read_jastrow_der`; None = no Jastrow
-        parameters) and ioptci = ``ci`` (``csf``: :func:`champ_io.read_csf`
-        of the determinant file, None for plain determinants)."""
+        parameters), ioptci = ``ci`` (``csf``: :func:`champ_io.read_csf`
+        of the determinant file, None for plain determinants) and ioptorb
+        with the excitations ``orb_terms`` ([nterm][2] (io, jo), 0-based;
+        the engine needs the same list through Engine.set_orb_terms)."""
         nct, nsb = sy.nctype, sy.nspin2b
         na, nb = sy.norda + 1, sy.nordb + 1
         nt = S.nterms4(sy.nordc) if sy.nordc > 1 else 0
@@ -80,9 +86,12 @@ class ChampParams:
             kinds |= DERIV_JAS_EEN
         if ci and ndet > 1:
             kinds |= DERIV_CI
+        terms = None if orb_terms is None else np.asarray(orb_terms, dtype=np.int32).reshape(-1, 2)
+        if terms is not None and len(terms):
+            kinds |= DERIV_ORB
         off, o = {}, 0
         for k, size in ((DERIV_JAS_EN, nct * na), (DERIV_JAS_EE, nsb * nb), (DERIV_JAS_EEN, nct * nt),
-                        (DERIV_CI, ndet if ndet > 1 else 0)):
+                        (DERIV_CI, ndet if ndet > 1 else 0), (DERIV_ORB, 0 if terms is None else len(terms))):
             if kinds & k:
                 off[k] = o
                 o += size
@@ -124,9 +133,9 @@ class ChampParams:
                     cols.append(unit(off[DERIV_CI] + k))
                     names.append(f"cdet{k + 1}")
             else:
-                ccsf, terms = csf
+                ccsf, csf_terms = csf
                 maxcsf = int(np.argmax(np.abs(ccsf)))  # first of the largest, parser.f90:1096-1109
-                for icsf, t in enumerate(terms):
+                for icsf, t in enumerate(csf_terms):
                     if icsf == maxcsf:
                         continue
                     v = np.zeros(nraw)
@@ -134,9 +143,13 @@ class ChampParams:
                         v[off[DERIV_CI] + idet] += cx
                     cols.append(v)
                     names.append(f"csf{icsf + 1}")
+        if kinds & DERIV_ORB:
+            for k, (io, jo) in enumerate(terms):
+                cols.append(unit(off[DERIV_ORB] + k))
+                names.append(f"orb{io + 1}->{jo + 1}")
         m = np.array(cols).T if cols else np.zeros((nraw, 0))
         return ChampParams(kinds, np.ascontiguousarray(m), names, iwjasa, iwjasb, iwjasc, bool(kinds & DERIV_CI),
-                           csf, maxcsf)
+                           csf, maxcsf, terms if kinds & DERIV_ORB else None)
 
     def apply(self, sy: S.System, dparm: np.ndarray):
         """(a4, b, c, cdet) after the step p - dparm (compute_jastrow and
@@ -173,5 +186,20 @@ class ChampParams:
                     for idet, cx in t:
                         cdet[idet] -= dparm[i] * cx
                     i += 1
+        if self.orb_terms is not None:
+            i += len(self.orb_terms)
         assert i == self.nparm
         return a4, b, c, cdet
+
+    def orbitals(self, sy: S.System, dparm: np.ndarray) -> np.ndarray:
+        """The orbital coefficients after the step (compute_lcao: coef[io] -=
+        dparm * coef[jo] with the coefficients before the step)."""
+        coef = np.array(sy.coef, dtype=np.float64)
+        if self.orb_terms is None:
+            return coef
+        dparm = np.asarray(dparm, dtype=np.float64)
+        k0 = self.nparm - len(self.orb_terms)
+        out = coef.copy()
+        for k, (io, jo) in enumerate(self.orb_terms):
+            out[io] -= dparm[k0 + k] * coef[jo]
+        return out

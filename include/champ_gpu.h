@@ -320,6 +320,7 @@ int champ_gpu_dmc_weights(champ_ctx *ctx, double *wt, int32_t *nwalk);
  *   CHAMP_DERIV_JAS_EEN  c[it][t] per centre type, every term of order nordc
  *                        in jastrow4.f90's order (no cusp elimination)
  *   CHAMP_DERIV_CI       cdet[k] of every determinant (ndet > 1 only)
+ *   CHAMP_DERIV_ORB      the orbital terms of champ_gpu_set_orb_terms
  *   CHAMP_DERIV_MAPPED   the parameters of the map set by
  *                        champ_gpu_set_deriv_map (other bits ignored)
  * ------------------------------------------------------------------------ */
@@ -328,6 +329,7 @@ int champ_gpu_dmc_weights(champ_ctx *ctx, double *wt, int32_t *nwalk);
 #define CHAMP_DERIV_JAS_EEN 4
 #define CHAMP_DERIV_CI      8
 #define CHAMP_DERIV_MAPPED  16
+#define CHAMP_DERIV_ORB     32
 int champ_gpu_deriv_count(champ_ctx *ctx, int32_t kinds, int32_t *nparm);
 
 /* A linear map from the raw parameters of `kinds` (nraw of them) to a
@@ -338,6 +340,14 @@ int champ_gpu_deriv_count(champ_ctx *ctx, int32_t kinds, int32_t *nparm);
  * (deriv_jastrow4.f90:394-460 with cdep of checkdepend4), and the CSF
  * combinations of optci.f90:70-80.  map NULL or nout 0 removes the map.      */
 int champ_gpu_set_deriv_map(champ_ctx *ctx, int32_t kinds, int32_t nout, const double *map);
+
+/* The orbital terms of CHAMP_DERIV_ORB: ideriv [nterm][2] = (io, jo),
+ * 0-based orbital indices, each the excitation phi_io += kappa phi_jo that
+ * compute_lcao applies (optwf_handle_wf.f90:1128-1134, orbitals_ortho off;
+ * CHAMP's ideriv list of optorb_define, optorb.f90:838-1060).  The derivative
+ * is sum_e V[e][io] phi_jo(r_e), V = d ln psi_D / d phi(r_e) (optorb_deriv,
+ * optorb.f90:5-97, orbprim).  nterm 0 removes the terms.                      */
+int champ_gpu_set_orb_terms(champ_ctx *ctx, int32_t nterm, const int32_t *ideriv);
 
 /* O [nwalk][nparm]: d ln psi / d p of every walker.                         */
 int champ_gpu_deriv_psi(champ_ctx *ctx, int32_t kinds, double *o);
@@ -355,6 +365,11 @@ int champ_gpu_sr_sums(champ_ctx *ctx, int32_t kinds, double *s, double *osum, do
  * state is recomputed (hpsi) before the call returns.                       */
 int champ_gpu_set_params(champ_ctx *ctx, const double *a4, const double *b, const double *c,
                          const double *cdet);
+
+/* New orbital coefficients coef [norb][nbasis] (the orbital part of
+ * compute_lcao, optwf_handle_wf.f90:1106-1150); every walker's state is
+ * recomputed (hpsi) before the call returns.                                */
+int champ_gpu_set_orbitals(champ_ctx *ctx, const double *coef);
 
 typedef struct champ_sr_info {
     double adiag;           /* the diagonal shift finally used                   */

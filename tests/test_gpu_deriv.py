@@ -15,7 +15,7 @@ import pytest
 
 from champ_amd import optwf
 from champ_amd import system as S
-from champ_amd.engine import DERIV_CI, DERIV_JAS_EE, DERIV_JAS_EEN, DERIV_JAS_EN, DERIV_MAPPED, Engine
+from champ_amd.engine import DERIV_CI, DERIV_JAS_EE, DERIV_JAS_EEN, DERIV_JAS_EN, DERIV_MAPPED, DERIV_ORB, Engine
 from oracle import ffi
 from tests.gpu_helpers import ENGINE_LIB
 
@@ -198,3 +198,78 @@ def test_sr_step_through_the_map(require_gpu):
         assert np.all(np.isfinite(eng.scalars()["eold"]))
         print(f"[sr c22_md] iteration {it}: <E> {es / nw:.6f} |dparm| {info['dparm_norm']:.3e} "
               f"cg {info['cg_iters']}", flush=True)
+
+
+def _orb_system(name):
+    """The test system; a single determinant gets three extra (virtual)
+    orbitals so that excitations out of the occupied ones exist."""
+    sy = S.config(name)
+    if len(sy.cdet) == 1:
+        rng = np.random.default_rng(12)
+        extra = rng.normal(0.0, 0.3, size=(3, sy.coef.shape[1]))
+        sy = dataclasses.replace(sy, coef=np.ascontiguousarray(np.vstack([sy.coef, extra])))
+    return sy
+
+
+@pytest.mark.parametrize("name", ["c22", "small_md"])
+def test_orbital_terms_match_finite_differences(require_gpu, name):
+    """d ln psi / d kappa of phi_io += kappa phi_jo (optorb's excitations,
+    compute_lcao's update) against finite differences of the oracle's
+    ln |psi| with the orbital coefficients changed that way; occupied ->
+    virtual, occupied -> occupied (zero for one determinant) and virtual ->
+    occupied terms."""
+    sy = _orb_system(name)
+    nw = 3
+    x = sy.initial_walkers(nw, seed=31)
+    occ = sorted(set(int(o) for o in np.asarray(sy.iworbd)[0]))
+    virt = [o for o in range(sy.coef.shape[0]) if o not in occ]
+    terms = [(occ[0], virt[0]), (occ[-1], virt[-1]), (occ[1], occ[2]), (virt[0], occ[0]), (occ[2], virt[1])]
+    eng = Engine(sy, nw, lib_path=ENGINE_LIB)
+    eng.set_walkers(x)
+    eng.hpsi()
+    eng.set_orb_terms(terms)
+    o = eng.deriv_psi(DERIV_ORB)
+    assert o.shape == (nw, len(terms))
+    coef0 = np.array(sy.coef)
+    def cfd(io, jo, h):
+        cp, cm = coef0.copy(), coef0.copy()
+        cp[io] += h * coef0[jo]
+        cm[io] -= h * coef0[jo]
+        return (_lnpsi(dataclasses.replace(sy, coef=cp), x) - _lnpsi(dataclasses.replace(sy, coef=cm), x)) / (2 * h)
+
+    for k, (io, jo) in enumerate(terms):
+        # Richardson-extrapolated central differences: near a node |O| reaches
+        # hundreds, and the plain central difference's third-derivative error
+        # (h^2 / 6 of it) with it
+        h = 1e-5
+        fd = (4.0 * cfd(io, jo, h / 2) - cfd(io, jo, h)) / 3.0
+        err = np.abs(o[:, k] - fd) / np.maximum(1.0, np.abs(fd))
+        assert float(np.max(err)) < 1e-6, (name, io, jo, o[:, k], fd)
+    print(f"[deriv orb {name}] {len(terms)} orbital terms match finite differences", flush=True)
+
+
+def test_set_orbitals_matches_the_oracle(require_gpu):
+    """A step along orbital terms (ChampParams.orbitals, compute_lcao's update)
+    loaded with champ_gpu_set_orbitals: the recomputed state equals the
+    oracle's hpsi with the new coefficients."""
+    sy = _orb_system("c22")
+    nw = 3
+    x = sy.initial_walkers(nw, seed=37)
+    occ = sorted(set(int(o) for o in np.asarray(sy.iworbd)[0]))
+    terms = [(occ[0], sy.coef.shape[0] - 1), (occ[3], sy.coef.shape[0] - 2)]
+    p = optwf.ChampParams.build(sy, None, orb_terms=terms)
+    assert p.nparm == 2 and p.kinds == DERIV_ORB
+    eng = Engine(sy, nw, lib_path=ENGINE_LIB)
+    eng.set_walkers(x)
+    eng.hpsi()
+    coef = p.orbitals(sy, np.array([0.05, -0.03]))
+    eng.set_orbitals(coef)
+    sc = eng.scalars()
+    sy2 = dataclasses.replace(sy, coef=coef)
+    cs = sy2.as_ctypes()
+    for w in range(nw):
+        ow = ffi.walker(sy2, cs)
+        ow.set_x(x[w])
+        hh = ow.hpsi()
+        assert abs(sc["psido"][w] - hh["psid"]) <= 1e-10 * abs(hh["psid"])
+        assert abs(sc["eold"][w] - hh["energy"]) <= 1e-10 * max(1.0, abs(hh["energy"]))

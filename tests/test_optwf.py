@@ -111,3 +111,24 @@ def test_sr_solve_matches_a_dense_solve():
 def test_sr_solve_rejects_bad_arguments():
     with pytest.raises(RuntimeError):
         sr_solve(np.eye(2), np.zeros(2), np.zeros(2), 0.0, 0.0)
+
+
+def test_orbital_step_is_the_map_column():
+    """ChampParams with orbital terms: the terms come last, one raw parameter
+    each, and orbitals() applies compute_lcao's one-sided update with the
+    coefficients from before the step."""
+    sy = S.config("small_md")
+    terms = [(0, 5), (2, 7), (5, 0)]
+    p = optwf.ChampParams.build(sy, None, ci=True, orb_terms=terms)
+    nci = len(sy.cdet) - 1
+    assert p.nparm == nci + 3 and p.names[-3:] == ["orb1->6", "orb3->8", "orb6->1"]
+    assert np.array_equal(p.m[-3:, -3:], np.eye(3))
+    dp = np.zeros(p.nparm)
+    dp[-3:] = [0.1, -0.2, 0.3]
+    coef = p.orbitals(sy, dp)
+    ref = np.array(sy.coef, dtype=float)
+    exp = ref.copy()
+    exp[0] -= 0.1 * ref[5]
+    exp[2] += 0.2 * ref[7]
+    exp[5] -= 0.3 * ref[0]
+    np.testing.assert_array_equal(coef, exp)
