@@ -713,6 +713,17 @@ int launch_move_step(champ_ctx *c, int iel)
     return CHAMP_OK;
 }
 
+// Electron iel of every walker moved to xnew and committed where acc[w] is
+// set (psie + jassav/detsav; DMC's T-moves), the others left as they are.
+int launch_forced_move(champ_ctx *c, int iel)
+{
+    int rc;
+    if ((rc = launch_move_a(c, iel, 1))) return rc;
+    if ((rc = launch_move_gemm(c))) return rc;
+    if ((rc = launch_md_post_timed(c, iel))) return rc;
+    return launch_move_b(c, iel, 2);
+}
+
 // One metrop1 sweep: rotqua, Ne single-electron moves, hpsi accumulating the
 // local energy.  Every launch argument depends only on the context and tau.
 static int launch_sweep(champ_ctx *c)

@@ -30,6 +30,7 @@
 
 // one single-electron move of every walker with the context's mover (engine_api.hip)
 int launch_move_step(champ_ctx *c, int iel);
+int launch_forced_move(champ_ctx *c, int iel);
 
 namespace champ {
 
@@ -56,6 +57,10 @@ struct DmcState {
     std::vector<std::pair<char *, size_t>> arrays;
     void *d_tab = nullptr;   // device copy of `arrays`
     int *d_pairs = nullptr;  // [2 W] (source, destination) slots of one step's copies
+    // T-moves (icasula = -1): per walker the electron moved (-1: none) and its point
+    int icasula = 0;
+    int *d_tm_iel = nullptr;
+    double *d_tm_x = nullptr;
 };
 
 void dmc_free(champ_ctx *c)
@@ -63,6 +68,8 @@ void dmc_free(champ_ctx *c)
     if (c->dmc) {
         hipFree(c->dmc->d_tab);
         hipFree(c->dmc->d_pairs);
+        if (c->dmc->d_tm_iel) hipFree(c->dmc->d_tm_iel);
+        if (c->dmc->d_tm_x) hipFree(c->dmc->d_tm_x);
         delete c->dmc;
         c->dmc = nullptr;
     }
@@ -304,6 +311,8 @@ extern "C" int champ_gpu_dmc_init(champ_ctx *c, int32_t nconf, double tau, doubl
         HIPCHECK(hipMalloc(&D.d_tab, tab.size() * sizeof(CopyTab)));
         HIPCHECK(hipMemcpy(D.d_tab, tab.data(), tab.size() * sizeof(CopyTab), hipMemcpyHostToDevice));
         HIPCHECK(hipMalloc(&D.d_pairs, 2 * W * sizeof(int)));
+        HIPCHECK(hipMalloc(&D.d_tm_iel, W * sizeof(int)));
+        HIPCHECK(hipMalloc(&D.d_tm_x, 3 * W * sizeof(double)));
         c->dmc_on = 1;
     }
     DmcState &D = *c->dmc;
@@ -347,6 +356,79 @@ static int dmc_zerest(champ_ctx *c)
     return CHAMP_OK;
 }
 
+// ------------------------------------------------------------------------
+// T-moves with icasula = -1 (nonloc_grid.f90:60-137 over all electrons,
+// dmc_ps_mov1.f90:701-730): a thread per walker.  t_vpsp of every active
+// (electron, centre) pair and quadrature point is in tv (written by the
+// step's hpsi); positive ones count as zero.  t_norm = 1 + tau sum(-t); the
+// walker's stream draws p (only when t_norm != 1) and the first (electron,
+// point, centre), in that order, whose cumulative -tau t / t_norm exceeds p
+// is the move.  The pair list is electron-major with the centres ascending.
+__global__ void k_tmove_select(int W, int NE, int NC, int NQ, int cap, double tau, const int *pair_count,
+                               const int *pair_e, const int *pair_c, const double *tv, const double *r_en,
+                               const double *cent, const double *xq, const double *yq, const double *zq,
+                               uint64_t *rng, int *tm_iel, double *tm_x)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    const int n = pair_count[w];
+    const size_t base = (size_t)w * cap;
+    double tn = 0.0;
+    for (int p0 = 0; p0 < n;) {
+        const int e = pair_e[base + p0];
+        int p1 = p0;
+        while (p1 < n && pair_e[base + p1] == e) p1++;
+        for (int iq = 0; iq < NQ; iq++)
+            for (int p = p0; p < p1; p++) {
+                double t = tv[(base + p) * NQ + iq];
+                if (t > 0.0) t = 0.0;
+                tn = tn - t;
+            }
+        p0 = p1;
+    }
+    tn = 1.0 + tn * tau;
+    tm_iel[w] = -1;
+    if (tn == 1.0) return;
+    const double tni = 1.0 / tn;
+    uint64_t s[4];
+    for (int k = 0; k < 4; k++) s[k] = rng[4 * (size_t)w + k];
+    const double u = xo_uniform(s);
+    for (int k = 0; k < 4; k++) rng[4 * (size_t)w + k] = s[k];
+    double cum = 0.0;
+    for (int p0 = 0; p0 < n;) {
+        const int e = pair_e[base + p0];
+        int p1 = p0;
+        while (p1 < n && pair_e[base + p1] == e) p1++;
+        for (int iq = 0; iq < NQ; iq++)
+            for (int p = p0; p < p1; p++) {
+                double t = tv[(base + p) * NQ + iq];
+                if (t > 0.0) t = 0.0;
+                cum = cum - tau * t * tni;
+                if (cum > u) {
+                    const int ic = pair_c[base + p];
+                    const double ren = r_en[((size_t)w * NC + ic) * NE + e];
+                    const double q[3] = {xq[(size_t)w * NQ + iq], yq[(size_t)w * NQ + iq], zq[(size_t)w * NQ + iq]};
+                    for (int k = 0; k < 3; k++) tm_x[3 * (size_t)w + k] = ren * q[k] + cent[3 * ic + k];
+                    tm_iel[w] = e;
+                    return;
+                }
+            }
+        p0 = p1;
+    }
+}
+
+// the walkers whose T-move moves electron iel: its point as xnew and a
+// commit; every other walker proposes its electron's own position, unused
+__global__ void k_tmove_prep(int W, int NE, int iel, const int *tm_iel, const double *tm_x, const double *xT,
+                             double *xnew, unsigned char *acc)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    const bool mv = tm_iel[w] == iel;
+    for (int k = 0; k < 3; k++) xnew[3 * w + k] = mv ? tm_x[3 * (size_t)w + k] : xT[((size_t)w * 3 + k) * NE + iel];
+    acc[w] = mv ? 1 : 0;
+}
+
 extern "C" int champ_gpu_dmc_step(champ_ctx *c, champ_dmc_step_stats *st)
 {
     if (!c || !c->dmc_on) { set_error("dmc_step: champ_gpu_dmc_init first"); return CHAMP_ERR_STATE; }
@@ -376,11 +458,24 @@ extern "C" int champ_gpu_dmc_step(champ_ctx *c, champ_dmc_step_stats *st)
     for (int iel = 0; iel < c->NE; iel++)
         if ((rc = launch_move_step(c, iel))) { c->imetro = 1; return rc; }
     c->imetro = 1;
-    if ((rc = launch_hpsi_impl(c, 0))) return rc;
+    const bool tmoves = D.icasula < 0 && c->nloc > 0;
+    c->tstore = tmoves ? 1 : 0;
+    rc = launch_hpsi_impl(c, 0);
+    c->tstore = 0;
+    if (rc) return rc;
     da = dmc_args(c);
     da.ffi = ffi;
     hipLaunchKernelGGL(k_dmc_weight<1>, dim3((W + 3) / 4), dim3(256), 0, c->stream, da);
     HIPCHECK(hipGetLastError());
+    std::vector<int> tm_iel;
+    if (tmoves) {
+        hipLaunchKernelGGL(k_tmove_select, dim3((W + 255) / 256), dim3(256), 0, c->stream, W, c->NE, c->NC, c->NQ,
+                           c->pair_cap, D.tau, c->pair_count, c->pair_e, c->pair_c, c->pair_psi, c->r_en, c->cent,
+                           c->xq, c->yq, c->zq, c->rng, D.d_tm_iel, D.d_tm_x);
+        HIPCHECK(hipGetLastError());
+        tm_iel.resize(W);
+        HIPCHECK(hipMemcpyAsync(tm_iel.data(), D.d_tm_iel, (size_t)W * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    }
     std::vector<double> out((size_t)W * DMC_OUT), wt(c->dmc_W);
     std::vector<int> age(c->dmc_W);
     HIPCHECK(hipMemcpyAsync(out.data(), c->dmc_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -404,6 +499,19 @@ extern "C" int champ_gpu_dmc_step(champ_ctx *c, champ_dmc_step_stats *st)
     D.nodecr += nod;
     const double wfsum1 = wsum1 * ffn, efsum1 = esum1 * ffn;
     const double wgsum1 = wsum1 * D.fprod, egsum1 = esum1 * D.fprod;
+    // the T-moves, electron by electron over the electrons some walker moves
+    if (tmoves) {
+        std::vector<char> used(c->NE, 0);
+        for (int w = 0; w < W; w++)
+            if (tm_iel[w] >= 0) used[tm_iel[w]] = 1;
+        for (int iel = 0; iel < c->NE; iel++) {
+            if (!used[iel]) continue;
+            hipLaunchKernelGGL(k_tmove_prep, dim3((W + 255) / 256), dim3(256), 0, c->stream, W, c->NE, iel, D.d_tm_iel,
+                               D.d_tm_x, c->xT, c->xnew, c->acc);
+            HIPCHECK(hipGetLastError());
+            if ((rc = launch_forced_move(c, iel))) return rc;
+        }
+    }
     // branching
     std::vector<std::pair<int, int>> copies;
     std::string err;
@@ -501,5 +609,13 @@ extern "C" int champ_gpu_dmc_weights(champ_ctx *c, double *wt, int32_t *nwalk)
     if (!c || !c->dmc_on) { set_error("dmc_weights: champ_gpu_dmc_init first"); return CHAMP_ERR_STATE; }
     if (nwalk) *nwalk = c->W;
     if (wt) HIPCHECK(hipMemcpy(wt, c->dmc_wt, (size_t)c->W * sizeof(double), hipMemcpyDeviceToHost));
+    return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_dmc_set_icasula(champ_ctx *c, int32_t icasula)
+{
+    if (!c || !c->dmc_on) { set_error("dmc_set_icasula: champ_gpu_dmc_init first"); return CHAMP_ERR_STATE; }
+    if (icasula != 0 && icasula != -1) { set_error("dmc_set_icasula: icasula 0 or -1 is provided"); return CHAMP_ERR_UNSUPPORTED; }
+    c->dmc->icasula = icasula;
     return CHAMP_OK;
 }

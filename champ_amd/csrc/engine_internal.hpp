@@ -146,6 +146,7 @@ struct champ_ctx {
     double *pair_val = nullptr;                // [W*pair_cap]
     double *pair_psi = nullptr;                // [W*pair_cap][NQ] psi ratio at each quadrature point
     float2 *enp = nullptr;                     // mixed path: [W][NC][NE] partner pair table (enp_pack)
+    int tstore = 0;                            // hpsi keeps each quadrature point's t_vpsp in pair_psi (T-moves)
     double *rrT = nullptr;                     // [W][NE][NC] rr_en with the centre innermost
     int *nxquad = nullptr;                     // [W]
 

@@ -289,6 +289,8 @@ __global__ void __launch_bounds__(NL_THREADS) CHAMP_WAVES_PER_EU(CHAMP_NL_WPE) k
                         term_radial = term_radial + yl0(l, costh) * a.pair_vps[slot * CHAMP_MPS_L + l - 1];
                     const double trj = term_radial * a.ecp->wq[q] * exp(fsumn);
                     pair_sum = pair_sum + trj * drq;
+                    // t_vpsp of the point (nonloc.f90:297) for DMC's T-moves
+                    if (a.tstore) a.pair_psi[slot * NQ + q] = trj * drq;
                 }
             }
             __syncthreads();  // S, XQ, U, GEO and RAD are rewritten by the next point block
@@ -691,6 +693,8 @@ __global__ void __launch_bounds__(MMA_THREADS) CHAMP_WAVES_PER_EU(CHAMP_MMA_WPE)
                 term_radial = term_radial + yl0(l, costh) * a.pair_vps[slot * CHAMP_MPS_L + l - 1];
             const double trj = term_radial * a.ecp->wq[q] * exp(fsumn);
             TRJ[pt] = trj * (psi_ext ? a.pair_psi[slot * NQ + q] : SP[2 * MMA_MP + pt]);
+            // t_vpsp of the point (nonloc.f90:297) for DMC's T-moves
+            if (a.tstore) a.pair_psi[slot * NQ + q] = TRJ[pt];
         }
         __syncthreads();
         if (tid < np) {

@@ -25,6 +25,7 @@ struct HArgs {
     double *pair_vps, *pair_val;
     double *pair_psi;  // [W*cap][NQ] psi ratios from k_nonloc_psi (psi_ext), else unused
     float2 *enp;       // mixed path: the FP32 partner pair table, else null
+    int tstore;        // keep each point's t_vpsp (its term of the ECP sum) in pair_psi
     int psi_ext;
     double *rrT;  // [W][NE][NC] scaled e-n distances, centre index innermost
     int NRI;      // radial instances (centre, radial function)
@@ -57,6 +58,7 @@ inline HArgs hargs(champ_ctx *c)
     a.pair_c = c->pair_c; a.nxquad = c->nxquad; a.grp_off = c->grp_off; a.pair_vps = c->pair_vps; a.pair_val = c->pair_val;
     a.pair_psi = c->pair_psi; a.psi_ext = 0;
     a.enp = c->enp;
+    a.tstore = c->tstore;
     a.rrT = c->rrT;
     a.NRI = c->NRI; a.ri_cent = c->ri_cent; a.ri_type = c->ri_type; a.ri_rad = c->ri_rad; a.ao_ri = c->ao_ri;
     a.ri_rmax = c->ri_rmax;

@@ -96,6 +96,7 @@ def load_library(path: str = LIBPATH):
     L.champ_gpu_dmc_step.argtypes = [vp, C.POINTER(DmcStepStats)]
     L.champ_gpu_dmc.argtypes = [vp, i32, i32, i32, dp, C.POINTER(DmcResult)]
     L.champ_gpu_dmc_weights.argtypes = [vp, dp, C.POINTER(i32)]
+    L.champ_gpu_dmc_set_icasula.argtypes = [vp, i32]
     L.champ_gpu_deriv_count.argtypes = [vp, i32, C.POINTER(i32)]
     L.champ_gpu_deriv_psi.argtypes = [vp, i32, dp]
     L.champ_gpu_sr_sums.argtypes = [vp, i32, dp, dp, dp, dp]
@@ -259,6 +260,10 @@ class Engine:
         st = np.ascontiguousarray(pop_state, dtype=np.uint64).reshape(4)
         self._check(self.L.champ_gpu_dmc_init(self.ptr, int(nconf), float(tau), float(etrial), int(nfprod),
                                               st.ctypes.data_as(C.POINTER(C.c_uint64))), "dmc_init")
+
+    def dmc_set_icasula(self, icasula: int):
+        """0: locality approximation; -1: one T-move per walker after each step"""
+        self._check(self.L.champ_gpu_dmc_set_icasula(self.ptr, int(icasula)), "dmc_set_icasula")
 
     def dmc_step(self) -> Dict[str, float]:
         s = DmcStepStats()

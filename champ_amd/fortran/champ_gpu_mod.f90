@@ -194,6 +194,11 @@ module champ_gpu_mod
       real(c_double), intent(out) :: egnow(*)
       type(champ_dmc_result), intent(out) :: res
     end function
+    integer(c_int) function champ_gpu_dmc_set_icasula(ctx, icasula) bind(c, name="champ_gpu_dmc_set_icasula")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: icasula
+    end function
     integer(c_int) function champ_gpu_dmc_weights(ctx, wt, nwalk) bind(c, name="champ_gpu_dmc_weights")
       import :: c_int, c_ptr, c_int32_t, c_double
       type(c_ptr), value :: ctx
@@ -258,6 +263,7 @@ module champ_gpu_mod
   public :: champ_gpu_accept, champ_gpu_metrop1, champ_gpu_get_walker_scalars, champ_gpu_autotune
   public :: champ_gpu_set_graphs, champ_gpu_metrop1_batch, champ_gpu_metrop6, champ_gpu_set_metrop6
   public :: champ_gpu_vmc, champ_gpu_dmc_init, champ_gpu_dmc_step, champ_gpu_dmc, champ_gpu_dmc_weights
+  public :: champ_gpu_dmc_set_icasula
   public :: champ_gpu_deriv_count, champ_gpu_deriv_psi, champ_gpu_sr_sums, champ_gpu_set_deriv_map
   public :: champ_gpu_set_params, champ_sr_solve, champ_gpu_set_orb_terms, champ_gpu_set_orbitals
   integer(c_int32_t), parameter, public :: CHAMP_DERIV_JAS_EN = 1, CHAMP_DERIV_JAS_EE = 2

@@ -253,7 +253,8 @@ int champ_gpu_vmc(champ_ctx *ctx, int32_t nblk, int32_t nstep, int32_t nblkeq, d
 /* --------------------------------------------------------------------------
  * Diffusion Monte Carlo (dmc.f90 with the dmc_ps mover, dmc_ps_mov1.f90:4-838)
  * at CHAMP's default controls: idmc = 2, icasula = 0 (the non-local
- * pseudopotential in the locality approximation), ipq = 1, itau_eff = 1,
+ * pseudopotential in the locality approximation; -1, T-moves, through
+ * champ_gpu_dmc_set_icasula), ipq = 1, itau_eff = 1,
  * iacc_rej = 1, icross = 1, icut_br = 0, icut_e = 0, nforce = 1,
  * node_cutoff = 0.  Systems with a pseudopotential only (CHAMP's all-electron
  * mover dmc_ae is not provided: CHAMP_ERR_UNSUPPORTED).  The context's walker
@@ -301,6 +302,15 @@ int champ_gpu_dmc_step(champ_ctx *ctx, champ_dmc_step_stats *stats);
  * block energies (acuest.f90); res (may be NULL).                           */
 int champ_gpu_dmc(champ_ctx *ctx, int32_t nblk, int32_t nstep, int32_t nblkeq, double *egnow,
                   champ_dmc_result *res);
+
+/* icasula of the dmc module (parser.f90): 0, the locality approximation (the
+ * default), or -1, one Casula T-move per walker after each step's moves and
+ * hpsi (dmc_ps_mov1.f90:701-730, nonloc_grid.f90:3-137 over all electrons:
+ * heat-bath choice among the quadrature points with negative t_vpsp,
+ * nonloc.f90:297, in the order electron, point, centre; the walker's stream
+ * draws the choice).  The local energy and the weights are those of the
+ * configuration before the T-move, as in the reference.                    */
+int champ_gpu_dmc_set_icasula(champ_ctx *ctx, int32_t icasula);
 
 /* The live walker count and their weights [nwalk] (wt may be NULL).          */
 int champ_gpu_dmc_weights(champ_ctx *ctx, double *wt, int32_t *nwalk);

@@ -82,6 +82,9 @@ struct oracle_walker {
      * vmd[i][o] = d ln psi_D / d phi_o(r_i) */
     int ndet;
     double *md_inv, *md_det, *md_invn, *md_detn, *vmd, md_psidn;
+    /* t_vpsp (nonloc.f90:297): [i][ic][iq] det_ratio * term_radial_jas, zero
+     * for pairs beyond the cut-off; the T-move weights of DMC */
+    double *tvpsp;
 };
 
 /* ------------------------------------------------------------------------ */
@@ -493,6 +496,7 @@ oracle_walker_t *orc_walker_new(const champ_system_t *s)
     w->npair = ne * (ne - 1) / 2;
     const int nm = w->nmax, nb = s->nbasis, no = s->norb;
     w->x = xcalloc(3 * ne, 8); w->xnew_e = xcalloc(3, 8);
+    w->tvpsp = xcalloc((size_t)ne * nc * (s->nquad > 0 ? s->nquad : 1), 8);
     w->r_en = xcalloc((size_t)ne * nc, 8); w->rvec_en = xcalloc((size_t)3 * ne * nc, 8);
     w->r_ee = xcalloc(w->npair, 8); w->rvec_ee = xcalloc((size_t)3 * w->npair, 8);
     w->r_en_sav = xcalloc(nc, 8); w->rvec_en_sav = xcalloc(3 * nc, 8);
@@ -530,7 +534,8 @@ void orc_walker_free(oracle_walker_t *w)
                     w->r_ee_sav, w->rvec_ee_sav, w->fjo, w->fjn, w->fso, w->fsn, w->fijo, w->fijn,
                     w->d2ijo, w->d2ijn, w->vj, w->vjn, w->orb, w->dorb, w->ddorb, w->orbn, w->dorbn,
                     w->ddorbn, w->slmi, w->slmin, w->fp, w->fpp, w->ddx, w->d2dx2, w->phin, w->dphin,
-                    w->d2phin, w->vps, w->asymp_jasa, w->md_inv, w->md_det, w->md_invn, w->md_detn, w->vmd};
+                    w->d2phin, w->vps, w->asymp_jasa, w->md_inv, w->md_det, w->md_invn, w->md_detn, w->vmd,
+                    w->tvpsp};
     for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
     free(w);
 }
@@ -1218,6 +1223,7 @@ static void nonloc_pot(const champ_system_t *s, oracle_walker_t *w, double *pe)
 
     w->vpsp_det[0] = w->vpsp_det[1] = 0.0;
     w->nxquad = 0;
+    memset(w->tvpsp, 0, sizeof(double) * (size_t)ne * nc * (s->nquad > 0 ? s->nquad : 1));
     double *rr_en = (double *)malloc(sizeof(double) * ne * nc);   /* scale_dist of r_en */
     for (int ic = 0; ic < nc; ic++)
         for (int i = 0; i < ne; i++) rr_en[(size_t)i * nc + ic] = (1.0 - exp(-kap * w->r_en[(size_t)i * nc + ic])) / kap;
@@ -1295,6 +1301,7 @@ static void nonloc_pot(const champ_system_t *s, oracle_walker_t *w, double *pe)
                     term_radial = term_radial + yl0(l, costh) * w->vps[((size_t)i * nc + ic) * CHAMP_MPS_L + (l - 1)];
                 double trj = term_radial * s->wq[iq] * exp(fsumn);
                 w->vpsp_det[iab] = w->vpsp_det[iab] + trj * det_ratio;
+                w->tvpsp[((size_t)i * nc + ic) * s->nquad + iq] = trj * det_ratio;
             }
         }
     }
@@ -1759,6 +1766,7 @@ static void walker_copy_state(oracle_walker_t *d, const oracle_walker_t *s, cons
 
 struct orc_dmc {
     int cap, nwalk, nconf, nfprod, ipass, first;
+    int icasula;  /* 0: locality approximation; -1: one T-move per walker after the step */
     double tau, etrial, eigv, eest, fprod, wdsumo;
     double *ff, *wtgen, *wt, *fratio;
     int *age;
@@ -1816,6 +1824,49 @@ void orc_dmc_free(orc_dmc_t *d)
     free(d->w); free(d->ff); free(d->wtgen); free(d->wt); free(d->fratio); free(d->age);
     free(d);
 }
+
+/* One T-move of the walker after its step (icasula = -1): nonloc_grid with
+ * all electrons (nonloc_grid.f90:60-137), heat-bath choice among the
+ * quadrature points with negative t_vpsp in the order (electron, point,
+ * centre), tauprim = tau, then psiedmc + jassav + detsav of the chosen
+ * electron at its point (dmc_ps_mov1.f90:701-730). */
+static void dmc_tmove(const champ_system_t *s, oracle_walker_t *w, double tau)
+{
+    const int ne = w->nelec, nc = w->ncent, nq = s->nquad;
+    double t_norm = 0.0;
+    for (int i = 0; i < ne; i++)
+        for (int iq = 0; iq < nq; iq++)
+            for (int ic = 0; ic < nc; ic++) {
+                double t = w->tvpsp[((size_t)i * nc + ic) * nq + iq];
+                if (t > 0.0) t = 0.0;
+                t_norm = t_norm - t;
+            }
+    t_norm = 1.0 + t_norm * tau;
+    if (t_norm == 1.0) return;
+    const double t_normi = 1.0 / t_norm;
+    const double p = orc_random_dp(w->rng);
+    double t_cum = 0.0;
+    for (int i = 0; i < ne; i++)
+        for (int iq = 0; iq < nq; iq++)
+            for (int ic = 0; ic < nc; ic++) {
+                double t = w->tvpsp[((size_t)i * nc + ic) * nq + iq];
+                if (t > 0.0) t = 0.0;
+                t_cum = t_cum - tau * t * t_normi;
+                if (t_cum > p) {
+                    const double ren = w->r_en[(size_t)i * nc + ic];
+                    const double q[3] = {w->xq[iq], w->yq[iq], w->zq[iq]};
+                    double x[3], psidn, psijn;
+                    for (int k = 0; k < 3; k++) x[k] = ren * q[k] + s->cent[3 * ic + k];
+                    orc_psie(s, w, i, x, &psidn, &psijn, 0);
+                    w->psido = psidn; w->psijo = psijn;
+                    w->psi2o = 2 * (log(fabs(psidn)) + psijn);
+                    orc_accept(s, w, i);
+                    return;
+                }
+            }
+}
+
+void orc_dmc_set_icasula(orc_dmc_t *d, int icasula) { d->icasula = icasula; }
 
 /* splitj (splitj.f90:3-233) with deep copies of the walker states */
 static int dmc_splitj(const champ_system_t *s, orc_dmc_t *d)
@@ -1951,6 +2002,7 @@ int orc_dmc_step(const champ_system_t *s, orc_dmc_t *d, double out[8])
         d->tausum = d->tausum + wtg * taunow;
         d->fratio[iw] = fration;
         if (!iaccept) d->age[iw]++;
+        if (d->icasula < 0 && s->nloc > 0) dmc_tmove(s, w, tau);
         wsum1 = wsum1 + wtnow;
         esum1 = esum1 + wtnow * enew;
     }

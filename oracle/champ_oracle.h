@@ -74,6 +74,7 @@ typedef struct orc_dmc orc_dmc_t;
 orc_dmc_t *orc_dmc_new(const champ_system_t *s, int cap, const double *x0, const uint64_t *rng0, int nconf, double tau,
                        double etrial, int nfprod, const uint64_t pop[4]);
 void   orc_dmc_free(orc_dmc_t *d);
+void   orc_dmc_set_icasula(orc_dmc_t *d, int icasula);
 int    orc_dmc_step(const champ_system_t *s, orc_dmc_t *d, double out[8]);
 double orc_dmc_block(orc_dmc_t *d);
 int    orc_dmc_get(const orc_dmc_t *d, double *wt, double *x, double *eold);

@@ -51,6 +51,7 @@ def lib():
         L.orc_dmc_new.argtypes = [vp, C.c_int, dp, u64p, C.c_int, C.c_double, C.c_double, C.c_int, u64p]
         L.orc_dmc_new.restype = vp
         L.orc_dmc_free.argtypes = [vp]
+        L.orc_dmc_set_icasula.argtypes = [vp, C.c_int]
         L.orc_dmc_step.argtypes = [vp, vp, dp]
         L.orc_dmc_step.restype = C.c_int
         L.orc_dmc_block.argtypes = [vp]
@@ -276,7 +277,7 @@ class Dmc:
     """The oracle's DMC population (orc_dmc_*): cap walker slots with their own
     streams, the first nconf live, and the population stream of splitj."""
 
-    def __init__(self, system, x0, rng0, nconf, tau, etrial, pop, nfprod=100, csys=None):
+    def __init__(self, system, x0, rng0, nconf, tau, etrial, pop, nfprod=100, csys=None, icasula=0):
         self.L = lib()
         self.sys = system
         self.csys = csys if csys is not None else system.as_ctypes()
@@ -286,6 +287,7 @@ class Dmc:
         self.cap = x0.shape[0]
         self.ptr = self.L.orc_dmc_new(C.byref(self.csys), self.cap, P(x0), U(rng0), int(nconf), float(tau),
                                       float(etrial), int(nfprod), U(pop))
+        self.L.orc_dmc_set_icasula(self.ptr, int(icasula))
 
     def __del__(self):
         if getattr(self, "ptr", None):
