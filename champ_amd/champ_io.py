@@ -277,6 +277,20 @@ def read_csf(path: str):
     return ccsf, terms
 
 
+def read_symmetry(path: str, norb: int) -> np.ndarray:
+    """Orbital symmetry file -> irrep [norb] (1-based labels, as in the file;
+    read_data_parser.f90:1836-1866: 'sym_labels nsym norb', a line of text
+    labels, then one irrep index per orbital)."""
+    r = _records(path)
+    w = r.line().split()
+    if len(w) < 3 or w[0] != "sym_labels":
+        raise ValueError(f"{path}: not a symmetry file")
+    if int(w[2]) != norb:
+        raise ValueError(f"{path}: {w[2]} orbitals, the orbital file has {norb}")
+    r.line()
+    return np.array(r.read(norb, int), dtype=np.int32)
+
+
 def read_jastrow_der(path: str, nctype: int, nspin2b: int = 1):
     """The jasderiv block (read_data_parser.f90:1521-1700): which Jastrow
     coefficients are optimised, CHAMP's 1-based indices per centre type (a,
@@ -385,9 +399,12 @@ def _tables_from_grids(grids: List[Dict], nloc: int):
 
 
 def load_input(inp: str, orbitals: Optional[np.ndarray] = None, nquad: Optional[int] = None,
-               name: Optional[str] = None) -> S.System:
+               name: Optional[str] = None, nadorb: int = 0) -> S.System:
     """A CHAMP VMC input file -> System.  ``orbitals`` replaces the file the
-    input names (e.g. after :func:`convert_lcao_order`)."""
+    input names (e.g. after :func:`convert_lcao_order`).  The System keeps the
+    orbitals the determinants use and the next ``nadorb`` of the orbital file
+    (-1: all of them), the external orbitals of orbital optimisation
+    (nextorb, optorb_define's nadorb)."""
     cfg = read_input(inp)
     keys, loads, base = cfg["keys"], cfg["loads"], cfg["dir"]
     pool = os.path.join(base, keys.get("pool", "./"))
@@ -420,7 +437,8 @@ def load_input(inp: str, orbitals: Optional[np.ndarray] = None, nquad: Optional[
     norb = int(iworbd.max()) + 1
     if coef.shape[0] < norb:
         raise ValueError(f"determinants use {norb} orbitals, the orbital file has {coef.shape[0]}")
-    coef = np.ascontiguousarray(coef[:norb])
+    nkeep = coef.shape[0] if nadorb < 0 else min(coef.shape[0], norb + nadorb)
+    coef = np.ascontiguousarray(coef[:nkeep])
     nbasis = int(sum(nbastyp[t] for t in iwctype))
     if coef.shape[1] != nbasis:
         raise ValueError(f"orbital file has {coef.shape[1]} AOs, basis pointers {nbasis}")

@@ -44,6 +44,61 @@ def cusp_fixed_terms(nordc: int) -> List[int]:
     return [t for t in range(nt) if p[t, t] == 0.0]
 
 
+def optorb_define(iworbd, nup: int, irrep, ncore: int = 0, nadorb: int = -1, no_active: int = 0,
+                  iwmix_virt=None) -> np.ndarray:
+    """CHAMP's list of orbital excitations phi_io += kappa phi_jo (optorb_define,
+    optorb.f90:838-1060), 0-based [nterm][2] (io, jo) in its order: io over
+    the orbitals the determinants use (ncore..ndetorb-1), jo over those and
+    the first ``nadorb`` external orbitals (nextorb; -1 or more than there are
+    = all of them, optorb.f90:888), keeping a pair when io is occupied in some
+    determinant, io != jo, both have the same irrep, not both are doubly
+    occupied in every determinant, not (no_active and io partly occupied and
+    jo occupied somewhere), ``iwmix_virt`` [norb][norb] (None = every pair)
+    allows it, and some determinant has io occupied and jo empty in one spin.
+    ``irrep`` has one entry per orbital of the orbital file."""
+    iw = np.asarray(iworbd, dtype=np.int64)
+    irrep = np.asarray(irrep)
+    ndet, nelec = iw.shape
+    norb_tot = len(irrep)
+    ndetorb = int(iw.max()) + 1
+    if ndetorb > norb_tot:
+        raise ValueError(f"determinants use {ndetorb} orbitals, the symmetry list has {norb_tot}")
+    next_max = norb_tot - ndetorb
+    if nadorb < 0 or nadorb > next_max:
+        nadorb = next_max
+    nall = ndetorb + nadorb
+    # per orbital: occupied in some determinant; doubly occupied in every one
+    cnt = np.zeros((ndet, nall), dtype=np.int64)
+    for k in range(ndet):
+        np.add.at(cnt[k], iw[k], 1)
+    used = np.zeros(nall, dtype=bool)
+    used[:ndetorb] = (cnt[:, :ndetorb] > 0).any(axis=0)
+    not_double = np.ones(nall, dtype=bool)
+    not_double[:ndetorb] = (cnt[:, :ndetorb] != 2).any(axis=0)
+    # occupation per spin block: occ[s][k][o]
+    occ = np.zeros((2, ndet, nall), dtype=bool)
+    for k in range(ndet):
+        occ[0, k, iw[k, :nup]] = True
+        occ[1, k, iw[k, nup:]] = True
+    terms = []
+    for io in range(ncore, ndetorb):
+        if not used[io]:
+            continue
+        for jo in range(ncore, nall):
+            if io == jo or irrep[io] != irrep[jo]:
+                continue
+            if not not_double[io] and not not_double[jo]:
+                continue
+            if no_active and not_double[io] and used[jo]:
+                continue
+            if iwmix_virt is not None and not iwmix_virt[io][jo]:
+                continue
+            if not (occ[:, :, io] & ~occ[:, :, jo]).any():
+                continue
+            terms.append((io, jo))
+    return np.array(terms, dtype=np.int32).reshape(-1, 2)
+
+
 @dataclass
 class ChampParams:
     kinds: int

@@ -10,6 +10,10 @@ the energies the CI scripts compare against:
           the CHAMP-v2 orbital file and basis info its conversion step reads)
   C4H6_dmc  tests/CI_test/DMC-C4H6-ci1010_pVTZ-500-dets (dmc.inp; energies from
           DMC_C4H6_ci1010_pVTZ_500-dets_butadiene.sh:8-9, :16-17)
+  C4H6_5k   tests/CI_test/VMC-C4H6-ci1010_pVTZ-05000-dets (vmc_optimization_5000.inp,
+          one sr_n iteration over the Jastrow, determinant and orbital
+          parameters; energies of the closing VMC run from
+          VMC_C4H6_ci1010_pVTZ_5000-dets_butadiene_optimization.sh:8-9, :18-19)
 
 Run in the build container only (the reference tree is not on GPU hosts):
     python tests/golden/champ_inputs/make_fixtures.py
@@ -39,6 +43,11 @@ CASES = {
                  ["dmc.inp", "TZ_1M_500.det", "champ_v3_trexio_order_ci1010_pVTZ_1_orbitals.lcao",
                   "jastrow_good_b3lyp.0", "pool/champ_v3_butadiene.xyz", "pool/champ_v3_BFD-T_basis_pointers.bfinfo",
                   "pool/BFD-T.basis.C", "pool/BFD-T.basis.H", "pool/BFD.gauss_ecp.dat.C", "pool/BFD.gauss_ecp.dat.H"]),
+    "C4H6_5k": ("VMC-C4H6-ci1010_pVTZ-05000-dets", "VMC_C4H6_ci1010_pVTZ_5000-dets_butadiene_optimization.sh",
+                ["vmc_optimization_5000.inp", "TZ_1M_5k.det", "champ_v3_trexio_order_ci1010_pVTZ_1_orbitals.lcao",
+                 "ci1010_pVTZ_1_symmetry.sym", "jastrow_good_b3lyp.0", "jastrow.der", "pool/champ_v3_butadiene.xyz",
+                 "pool/champ_v3_BFD-T_basis_pointers.bfinfo", "pool/BFD-T.basis.C", "pool/BFD-T.basis.H",
+                 "pool/BFD.gauss_ecp.dat.C", "pool/BFD.gauss_ecp.dat.H"]),
 }
 
 

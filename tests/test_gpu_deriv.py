@@ -323,3 +323,79 @@ def test_sr_lowers_the_h2_ci_energy(require_gpu):
     assert r1["efin"] < r0["efin"] - 20.0 * np.hypot(r0["eerr"], r1["eerr"])
     assert r1["sigma"] < 0.7 * r0["sigma"]
     assert r1["nflagged"] == 0
+
+
+def test_sr_iteration_on_the_5000_determinant_case(require_gpu):
+    """The reference's optimisation CI case with 5000 determinants
+    (tests/CI_test/VMC-C4H6-ci1010_pVTZ-05000-dets: butadiene, BFD, method
+    sr_n with ioptjas, ioptci and ioptorb, nextorb 280, nopt_iter 1, sr_tau
+    0.025, sr_eps 0.001, sr_adiag 0.01) run as optwf_sr runs it
+    (optwf_sr.f90:164-353): sampling with metrop6, the SR sums over its
+    parameters (the 13 Jastrow coefficients of its jastrow.der, 4999
+    determinant coefficients and the 7021 orbital excitations optorb_define
+    builds from its symmetry file over all 226 orbitals: 12033), one SR step,
+    the new Jastrow, determinant and orbital coefficients, and the closing VMC
+    run, whose energy the case's CI script compares with
+    tools/compare_value.py's rule |E - E_ref| <= 2 sqrt(err^2 + err_ref^2)
+    (against both of its reference runs)."""
+    import os
+    import time
+
+    from champ_amd import champ_io as io
+    from champ_amd.engine import sr_solve
+    from tests.test_champ_io import G, reference_energy
+
+    d = os.path.join(G, "C4H6_5k")
+    inp = os.path.join(d, "vmc_optimization_5000.inp")
+    sy = io.load_input(inp, name="C4H6_5k", nadorb=280)
+    assert sy.coef.shape == (226, 226) and len(sy.cdet) == 5000
+    der = io.read_jastrow_der(os.path.join(d, "jastrow.der"), sy.nctype, sy.nspin2b)
+    irrep = io.read_symmetry(os.path.join(d, "ci1010_pVTZ_1_symmetry.sym"), sy.coef.shape[0])
+    terms = optwf.optorb_define(sy.iworbd, sy.nup, irrep, ncore=0, nadorb=280)
+    p = optwf.ChampParams.build(sy, der, ci=True, orb_terms=terms)
+    assert len(terms) == 7021 and p.nparm == 13 + 4999 + 7021
+    W, nsample = 2048, 20
+    t0 = time.time()
+    eng = Engine(sy, W, lib_path=ENGINE_LIB)
+    eng.set_walkers(sy.initial_walkers(W, seed=3))
+    eng.seed_rng(S.seed_from_string())
+    eng.hpsi()
+    eng.set_orb_terms(terms)  # before the map: the raw layout includes the terms
+    eng.set_deriv_map(p.kinds, p.m)
+    r0 = eng.vmc(20, 5, nblkeq=2, imetro=6)
+    print(f"[sr 5k] set-up and first VMC {time.time() - t0:.1f} s: E = {r0['efin']:.5f} +- {r0['eerr']:.5f} "
+          f"(sigma {r0['sigma']:.3f})", flush=True)
+    t0 = time.time()
+    acc = None
+    for _ in range(nsample):
+        eng.metrop6(2)
+        sums = eng.sr_sums(DERIV_MAPPED)
+        if acc is None:
+            acc = [np.array(x, dtype=np.float64) for x in sums]
+        else:
+            for a, x in zip(acc, sums):
+                a += x
+    print(f"[sr 5k] {nsample} samples of {W} walkers, {p.nparm} parameters: {time.time() - t0:.1f} s", flush=True)
+    t0 = time.time()
+    dparm, info = sr_solve(acc[0], acc[1], acc[2], float(acc[3]), nsample * W, sr_tau=0.025, sr_eps=1e-3,
+                           sr_adiag=0.01, lib_path=ENGINE_LIB)
+    print(f"[sr 5k] solve {time.time() - t0:.1f} s: rms step {info['dparm_norm']:.3e}, cg {info['cg_iters']}, "
+          f"adiag {info['adiag']}", flush=True)
+    assert np.all(np.isfinite(dparm)) and info["dparm_norm"] <= 1.0
+    eng.set_params(*p.apply(sy, dparm))
+    eng.set_orbitals(p.orbitals(sy, dparm))
+    t0 = time.time()
+    r = eng.vmc(20, 5, nblkeq=2, imetro=6)
+    st = eng.status()
+    print(f"[sr 5k] closing VMC {time.time() - t0:.1f} s: E = {r['efin']:.5f} +- {r['eerr']:.5f} "
+          f"(sigma {r['sigma']:.3f}, acceptance {r['accept']:.3f}); status words {np.unique(st).tolist()} "
+          f"on {int(np.count_nonzero(st))} walkers", flush=True)
+    # A proposal where every orbital vanishes (beyond the numerical basis'
+    # range) has a determinant ratio of exactly 0: determinante.f90:46 returns
+    # it and the move is rejected, so the walker is unchanged.  That is the
+    # only mark allowed here.
+    assert np.all((st & ~2) == 0) and np.count_nonzero(st) <= W // 100
+    for cores in (1, 2):
+        eref, err_ref = reference_energy("C4H6_5k", cores)
+        print(f"[sr 5k] reference ({cores} cores) {eref} +- {err_ref}", flush=True)
+        assert abs(r["efin"] - eref) <= 2.0 * np.hypot(r["eerr"], err_ref)

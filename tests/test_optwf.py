@@ -132,3 +132,84 @@ def test_orbital_step_is_the_map_column():
     exp[2] += 0.2 * ref[7]
     exp[5] -= 0.3 * ref[0]
     np.testing.assert_array_equal(coef, exp)
+
+
+def test_optorb_define_hand_worked_list():
+    """optorb_define on two determinants of four electrons, worked by hand
+    from the rules of optorb.f90:911-1018: orbital 0 doubly occupied in both,
+    1 and 2 partly, 3 external.  Pins the order (io outer, jo inner), the
+    symmetry rule, ncore and no_active."""
+    iw = [[0, 1, 0, 1], [0, 2, 0, 1]]  # [up up | down down]
+    same = [1, 1, 1, 1]
+    t = optwf.optorb_define(iw, 2, same)
+    assert t.tolist() == [[0, 1], [0, 2], [0, 3], [1, 2], [1, 3], [2, 1], [2, 3]]
+    assert optwf.optorb_define(iw, 2, [1, 1, 2, 1]).tolist() == [[0, 1], [0, 3], [1, 3]]
+    assert optwf.optorb_define(iw, 2, same, ncore=1).tolist() == [[1, 2], [1, 3], [2, 1], [2, 3]]
+    assert optwf.optorb_define(iw, 2, same, no_active=1).tolist() == [[0, 1], [0, 2], [0, 3], [1, 3], [2, 3]]
+    # nextorb 0: no external orbital; the doubly occupied pair rule
+    assert optwf.optorb_define(iw, 2, same, nadorb=0).tolist() == [[0, 1], [0, 2], [1, 2], [2, 1]]
+    assert optwf.optorb_define([[0, 1, 0, 1]], 2, [1, 1, 1]).tolist() == [[0, 2], [1, 2]]
+
+
+def test_optorb_define_keeps_exactly_the_pairs_that_change_psi():
+    """The pair rule of optorb_define against the mathematics: phi_io +=
+    kappa phi_jo leaves every determinant unchanged unless, in some
+    determinant and spin, io is occupied and jo is not (a column plus a
+    multiple of another column, or a column not present).  So among pairs of
+    one irrep, the list holds exactly those whose d ln psi / d kappa, by
+    finite differences of the oracle's ln |psi| on the first three
+    determinants of small_md with three external orbitals added (46 pairs
+    kept, 16 of the same irrep left out), is not zero; random irreps pin the
+    symmetry rule."""
+    import dataclasses
+
+    from oracle import ffi
+
+    sy = S.config("small_md")
+    rng = np.random.default_rng(5)
+    coef0 = np.ascontiguousarray(np.vstack([sy.coef, rng.normal(0.0, 0.3, size=(3, sy.coef.shape[1]))]))
+    sy = dataclasses.replace(sy, coef=coef0, cdet=np.array(sy.cdet[:3]), iworbd=np.array(np.asarray(sy.iworbd)[:3]))
+    norb = coef0.shape[0]
+    irrep = rng.integers(1, 3, size=norb)
+    iw = np.asarray(sy.iworbd)
+    x = sy.initial_walkers(2, seed=9)
+
+    def lnpsi(coef):
+        s2 = dataclasses.replace(sy, coef=coef)
+        cs = s2.as_ctypes()
+        out = []
+        for xw in x:
+            w = ffi.walker(s2, cs)
+            w.set_x(xw)
+            h = w.hpsi()
+            out.append(np.log(abs(h["psid"])) + h["psij"])
+        return np.array(out)
+
+    terms = {tuple(t) for t in optwf.optorb_define(iw, sy.nup, irrep).tolist()}
+    assert len(terms) == 46
+    h = 1e-4
+    ndetorb = int(iw.max()) + 1
+    nzero = 0
+    for io in range(ndetorb):
+        for jo in range(norb):
+            if io == jo or irrep[io] != irrep[jo]:
+                assert (io, jo) not in terms
+                continue
+            cp, cm = coef0.copy(), coef0.copy()
+            cp[io] += h * coef0[jo]
+            cm[io] -= h * coef0[jo]
+            d = np.max(np.abs((lnpsi(cp) - lnpsi(cm)) / (2 * h)))
+            assert ((io, jo) in terms) == (d > 1e-6), (io, jo, d)
+            nzero += d <= 1e-6
+    assert nzero == 16
+
+
+def test_symmetry_reader():
+    """The symmetry file of the reference's 5000-determinant butadiene case:
+    226 orbitals in four irreps, the first ones as the file lists them."""
+    d = os.path.join(FIX, "C4H6_5k")
+    ir = io.read_symmetry(os.path.join(d, "ci1010_pVTZ_1_symmetry.sym"), 226)
+    assert ir.shape == (226,) and set(ir.tolist()) == {1, 2, 3, 4}
+    assert ir[:12].tolist() == [1, 4, 4, 1, 1, 4, 1, 4, 1, 2, 3, 2]
+    with pytest.raises(ValueError):
+        io.read_symmetry(os.path.join(d, "ci1010_pVTZ_1_symmetry.sym"), 225)
