@@ -37,6 +37,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <vector>
 
 namespace champ {
@@ -67,9 +68,19 @@ static void deriv_free_scratch(champ_ctx *c)
     c->der_cap = c->der_scap = c->der_om_cap = 0;
 }
 
+static void sr_free(champ_ctx *c)
+{
+    void *ps[] = {c->sr_acc, c->sr_accv, c->sr_z};
+    for (void *p : ps) if (p) hipFree(p);
+    c->sr_acc = c->sr_accv = c->sr_z = nullptr;
+    c->sr_n = c->sr_kinds = 0;
+    c->sr_wsum = c->sr_esum = 0.0;
+}
+
 void deriv_free(champ_ctx *c)
 {
     deriv_free_scratch(c);
+    sr_free(c);
     if (c->der_map) hipFree(c->der_map);
     if (c->der_orb) hipFree(c->der_orb);
     if (c->der_occ) hipFree(c->der_occ);
@@ -262,6 +273,34 @@ __global__ void k_deriv_sums(int n, int W, const double *ot, const double *eold,
     v[n + k] = se;
 }
 
+// dst += src over n elements
+__global__ void k_sr_add(size_t n, const double *__restrict__ src, double *__restrict__ dst)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] += src[i];
+}
+
+// t = S z for the accumulated S [n][n] (symmetric): a wavefront per row, the
+// row read once with consecutive lanes on consecutive columns
+__global__ void __launch_bounds__(256) k_sr_gemv(int n, const double *__restrict__ S, const double *__restrict__ z,
+                                                 double *__restrict__ t)
+{
+    const int lane = threadIdx.x % WAVE;
+    const int row = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (row >= n) return;
+    const double *r = S + (size_t)row * n;
+    double acc = 0.0;
+    for (int l = lane; l < n; l += WAVE) acc += r[l] * z[l];
+    acc = wave_sum(acc);
+    if (lane == 0) t[row] = acc;
+}
+
+__global__ void k_sr_diag(int n, const double *__restrict__ S, double *__restrict__ d)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) d[k] = S[(size_t)k * n + k];
+}
+
 template <typename T>
 static int grow(T **p, int *cap, int need, size_t per)
 {
@@ -351,15 +390,20 @@ static int deriv_compute(champ_ctx *c, int kinds, const double **ot, int *n)
 // optwf_sr_more.f90:31-160, on the dense S of the sums.
 struct SrSystem {
     int n;
-    std::vector<double> oo, o, s_diag, s_ii_inv, h;  // oo [n][n] = <O_k O_l>, o = <O_k>
+    std::vector<double> o, s_diag, s_ii_inv, h;  // o = <O_k>
+    // t = <O O> z (the sums divided by the weight): a dense host product or a
+    // matrix-vector product over the sums on the GPU
+    std::function<int(const double *, double *)> oo_times;
 };
 
-static void sr_hs(SrSystem &A, double adiag, const std::vector<double> &eo, double e)
+// oo_diag = <O_k O_k>, eo = <E O_k>, e = <E>
+static void sr_hs(SrSystem &A, double adiag, const std::vector<double> &oo_diag, const std::vector<double> &eo,
+                  double e)
 {
     const int n = A.n;
     double smax = 0.0;
     for (int k = 0; k < n; k++) {
-        const double aux = A.oo[(size_t)k * n + k] - A.o[k] * A.o[k];
+        const double aux = oo_diag[k] - A.o[k] * A.o[k];
         A.s_diag[k] = aux * adiag;
         A.s_ii_inv[k] = aux + A.s_diag[k];
         A.h[k] = -2.0 * (eo[k] - A.o[k] * e);
@@ -370,20 +414,18 @@ static void sr_hs(SrSystem &A, double adiag, const std::vector<double> &eo, doub
 }
 
 // r = S z with S_kl = <O_k O_l> - <O_k><O_l> + s_diag_k delta_kl
-static void sr_atimes(const SrSystem &A, const double *z, double *r)
+static int sr_atimes(const SrSystem &A, const double *z, double *r)
 {
     const int n = A.n;
     double oz = 0.0;
     for (int k = 0; k < n; k++) oz += A.o[k] * z[k];
-    for (int k = 0; k < n; k++) {
-        double t = 0.0;
-        const double *row = &A.oo[(size_t)k * n];
-        for (int l = 0; l < n; l++) t += row[l] * z[l];
-        r[k] = t - A.o[k] * oz + A.s_diag[k] * z[k];
-    }
+    int rc = A.oo_times(z, r);
+    if (rc) return rc;
+    for (int k = 0; k < n; k++) r[k] = r[k] - A.o[k] * oz + A.s_diag[k] * z[k];
+    return CHAMP_OK;
 }
 
-static int sr_pcg(const SrSystem &A, double eps, std::vector<double> &x)
+static int sr_pcg(const SrSystem &A, double eps, std::vector<double> &x, int *iters)
 {
     const int n = A.n, imax = n, imod = 50;
     std::vector<double> r(n), d(n), q(n), sv(n);
@@ -392,20 +434,21 @@ static int sr_pcg(const SrSystem &A, double eps, std::vector<double> &x)
         for (int k = 0; k < n; k++) t += u[k] * v[k];
         return t;
     };
+    int rc;
     x.assign(n, 0.0);
-    sr_atimes(A, x.data(), r.data());
+    if ((rc = sr_atimes(A, x.data(), r.data()))) return rc;
     for (int k = 0; k < n; k++) r[k] = A.h[k] - r[k];
     for (int k = 0; k < n; k++) d[k] = r[k] * A.s_ii_inv[k];
     double delta_new = dot(d.data(), r.data());
     const double delta_0 = delta_new * eps * eps;
     int i = 0;
     for (; i < imax; i++) {
-        if (delta_new < delta_0) return i;
-        sr_atimes(A, d.data(), q.data());
+        if (delta_new < delta_0) break;
+        if ((rc = sr_atimes(A, d.data(), q.data()))) return rc;
         const double alpha = delta_new / dot(d.data(), q.data());
         for (int k = 0; k < n; k++) x[k] += alpha * d[k];
         if (i % imod == 0) {
-            sr_atimes(A, x.data(), r.data());
+            if ((rc = sr_atimes(A, x.data(), r.data()))) return rc;
             for (int k = 0; k < n; k++) r[k] = A.h[k] - r[k];
         } else {
             for (int k = 0; k < n; k++) r[k] -= alpha * q[k];
@@ -416,7 +459,36 @@ static int sr_pcg(const SrSystem &A, double eps, std::vector<double> &x)
         const double beta = delta_new / delta_old;
         for (int k = 0; k < n; k++) d[k] = sv[k] + beta * d[k];
     }
-    return i;
+    *iters = i;
+    return CHAMP_OK;
+}
+
+// optwf_sr.f90:201-231: solve, dparm = -sr_tau x, and test_solution_parm
+// (optwf_handle_wf.f90:1338-1365): a step whose rms exceeds dparm_norm_min is
+// redone with ten times the diagonal shift
+static int sr_step(SrSystem &A, const std::vector<double> &oo_diag, const std::vector<double> &eo, double e,
+                   double sr_tau, double sr_eps, double sr_adiag, double dparm_norm_min, double *dparm,
+                   champ_sr_info *info)
+{
+    const int n = A.n;
+    A.s_diag.resize(n);
+    A.s_ii_inv.resize(n);
+    A.h.resize(n);
+    double adiag = sr_adiag, norm = 0.0;
+    int iters = 0, tries = 0, rc;
+    std::vector<double> x;
+    for (;; tries++) {
+        sr_hs(A, adiag, oo_diag, eo, e);
+        if ((rc = sr_pcg(A, sr_eps, x, &iters))) return rc;
+        norm = 0.0;
+        for (int k = 0; k < n; k++) { dparm[k] = -sr_tau * x[k]; norm += dparm[k] * dparm[k]; }
+        norm = std::sqrt(norm / n);
+        if (!(adiag > 0.0) || norm <= dparm_norm_min) break;
+        if (tries == 30) { set_error("sr_solve: no step below dparm_norm_min"); return CHAMP_ERR_NUMERIC; }
+        adiag *= 10.0;
+    }
+    if (info) { info->adiag = adiag; info->dparm_norm = norm; info->cg_iters = iters; info->retries = tries; }
+    return CHAMP_OK;
 }
 
 }  // namespace champ
@@ -450,8 +522,11 @@ extern "C" int champ_gpu_set_deriv_map(champ_ctx *c, int32_t kinds, int32_t nout
     const int nraw = der_layout(c, kinds).n;
     if (nraw == 0) { set_error("set_deriv_map: no parameters selected (kinds) for this trial function"); return CHAMP_ERR_ARG; }
     std::vector<double> mt((size_t)nout * nraw);
-    for (int k = 0; k < nraw; k++)
-        for (int j = 0; j < nout; j++) mt[(size_t)j * nraw + k] = map[(size_t)k * nout + j];
+    constexpr int TB = 64;  // transposed in tiles, so both sides stay in cache lines
+    for (int k0 = 0; k0 < nraw; k0 += TB)
+        for (int j0 = 0; j0 < nout; j0 += TB)
+            for (int k = k0; k < std::min(k0 + TB, nraw); k++)
+                for (int j = j0; j < std::min(j0 + TB, nout); j++) mt[(size_t)j * nraw + k] = map[(size_t)k * nout + j];
     HIPCHECK(hipMalloc(&c->der_map, mt.size() * sizeof(double)));
     HIPCHECK(hipMemcpy(c->der_map, mt.data(), mt.size() * sizeof(double), hipMemcpyHostToDevice));
     c->der_map_kinds = kinds;
@@ -542,33 +617,95 @@ extern "C" int champ_sr_solve(int32_t nparm, double wsum, const double *s, const
                               double *dparm, champ_sr_info *info)
 {
     if (nparm <= 0 || !(wsum > 0.0) || !s || !osum || !eosum || !dparm) { set_error("sr_solve: bad arguments"); return CHAMP_ERR_ARG; }
-    SrSystem A;
     const int n = nparm;
+    SrSystem A;
     A.n = n;
-    A.oo.resize((size_t)n * n);
     A.o.resize(n);
-    A.s_diag.resize(n);
-    A.s_ii_inv.resize(n);
-    A.h.resize(n);
-    std::vector<double> eo(n);
-    for (size_t k = 0; k < (size_t)n * n; k++) A.oo[k] = s[k] / wsum;
-    for (int k = 0; k < n; k++) { A.o[k] = osum[k] / wsum; eo[k] = eosum[k] / wsum; }
-    const double e = esum / wsum;
-    double adiag = sr_adiag, norm = 0.0;
-    int iters = 0, tries = 0;
-    std::vector<double> x;
-    for (;; tries++) {
-        sr_hs(A, adiag, eo, e);
-        iters = sr_pcg(A, sr_eps, x);
-        norm = 0.0;
-        for (int k = 0; k < n; k++) { dparm[k] = -sr_tau * x[k]; norm += dparm[k] * dparm[k]; }
-        norm = std::sqrt(norm / n);
-        // test_solution_parm (optwf_handle_wf.f90:1338-1365): a step whose rms
-        // exceeds dparm_norm_min is redone with ten times the diagonal shift
-        if (!(adiag > 0.0) || norm <= dparm_norm_min) break;
-        if (tries == 30) { set_error("sr_solve: no step below dparm_norm_min"); return CHAMP_ERR_NUMERIC; }
-        adiag *= 10.0;
+    std::vector<double> eo(n), diag(n), oo((size_t)n * n);
+    for (size_t k = 0; k < (size_t)n * n; k++) oo[k] = s[k] / wsum;
+    for (int k = 0; k < n; k++) { A.o[k] = osum[k] / wsum; eo[k] = eosum[k] / wsum; diag[k] = oo[(size_t)k * n + k]; }
+    A.oo_times = [&oo, n](const double *z, double *t) {
+        for (int k = 0; k < n; k++) {
+            double u = 0.0;
+            const double *row = &oo[(size_t)k * n];
+            for (int l = 0; l < n; l++) u += row[l] * z[l];
+            t[k] = u;
+        }
+        return (int)CHAMP_OK;
+    };
+    return sr_step(A, diag, eo, esum / wsum, sr_tau, sr_eps, sr_adiag, dparm_norm_min, dparm, info);
+}
+
+extern "C" int champ_gpu_sr_accumulate(champ_ctx *c, int32_t kinds, int32_t reset, double *wsum, double *esum)
+{
+    if (!c) return CHAMP_ERR_ARG;
+    const double *ot = nullptr;
+    int n = 0;
+    int rc = deriv_compute(c, kinds, &ot, &n);
+    if (rc) return rc;
+    if (!reset && c->sr_acc && (c->sr_n != n || c->sr_kinds != kinds)) {
+        set_error("sr_accumulate: the parameters differ from the accumulated ones (start again with reset)");
+        return CHAMP_ERR_ARG;
     }
-    if (info) { info->adiag = adiag; info->dparm_norm = norm; info->cg_iters = iters; info->retries = tries; }
+    if (reset || !c->sr_acc) {
+        if (c->sr_n != n) {
+            sr_free(c);
+            HIPCHECK(hipMalloc(&c->sr_acc, (size_t)n * n * sizeof(double)));
+            HIPCHECK(hipMalloc(&c->sr_accv, (size_t)2 * n * sizeof(double)));
+            HIPCHECK(hipMalloc(&c->sr_z, (size_t)2 * n * sizeof(double)));
+        }
+        HIPCHECK(hipMemsetAsync(c->sr_acc, 0, (size_t)n * n * sizeof(double), c->stream));
+        HIPCHECK(hipMemsetAsync(c->sr_accv, 0, (size_t)2 * n * sizeof(double), c->stream));
+        c->sr_n = n;
+        c->sr_kinds = kinds;
+        c->sr_wsum = c->sr_esum = 0.0;
+    }
+    const int W = c->W;
+    if ((rc = launch_gemm(c, true, n, n, W, ot, W, 0, ot, W, 0, c->der_s, n, 0, 1))) return rc;
+    hipLaunchKernelGGL(k_sr_add, dim3(4096), dim3(256), 0, c->stream, (size_t)n * n, c->der_s, c->sr_acc);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_deriv_sums, dim3((n + 63) / 64), dim3(64), 0, c->stream, n, W, ot, c->eold, c->der_v);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_sr_add, dim3((2 * n + 255) / 256), dim3(256), 0, c->stream, (size_t)2 * n, c->der_v,
+                       c->sr_accv);
+    HIPCHECK(hipGetLastError());
+    std::vector<double> e(W);
+    HIPCHECK(hipMemcpyAsync(e.data(), c->eold, (size_t)W * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    double t = 0.0;
+    for (int w = 0; w < W; w++) t += e[w];
+    c->sr_esum += t;
+    c->sr_wsum += W;
+    if (wsum) *wsum = c->sr_wsum;
+    if (esum) *esum = c->sr_esum;
     return CHAMP_OK;
+}
+
+extern "C" int champ_gpu_sr_solve(champ_ctx *c, double sr_tau, double sr_eps, double sr_adiag, double dparm_norm_min,
+                                  double *dparm, champ_sr_info *info)
+{
+    if (!c || !dparm) return CHAMP_ERR_ARG;
+    if (!c->sr_acc || !(c->sr_wsum > 0.0)) { set_error("sr_solve: no sums accumulated"); return CHAMP_ERR_ARG; }
+    const int n = c->sr_n;
+    const double wsum = c->sr_wsum;
+    SrSystem A;
+    A.n = n;
+    A.o.resize(n);
+    std::vector<double> v((size_t)2 * n), eo(n), diag(n);
+    hipLaunchKernelGGL(k_sr_diag, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->sr_acc, c->sr_z);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(diag.data(), c->sr_z, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(v.data(), c->sr_accv, v.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < n; k++) { A.o[k] = v[k] / wsum; eo[k] = v[(size_t)n + k] / wsum; diag[k] /= wsum; }
+    A.oo_times = [c, n, wsum](const double *z, double *t) {
+        HIPCHECK(hipMemcpyAsync(c->sr_z, z, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_sr_gemv, dim3((n + 3) / 4), dim3(256), 0, c->stream, n, c->sr_acc, c->sr_z, c->sr_z + n);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(t, c->sr_z + n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        for (int k = 0; k < n; k++) t[k] /= wsum;
+        return (int)CHAMP_OK;
+    };
+    return sr_step(A, diag, eo, c->sr_esum / wsum, sr_tau, sr_eps, sr_adiag, dparm_norm_min, dparm, info);
 }

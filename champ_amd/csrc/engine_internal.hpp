@@ -132,6 +132,11 @@ struct champ_ctx {
     int *der_orb = nullptr, *der_occ = nullptr;
     int der_norb = 0;
     int der_map_kinds = 0, der_map_nraw = 0, der_map_nout = 0, der_om_cap = 0;
+    // SR sums kept on the device (champ_gpu_sr_accumulate): S [n][n], [O | E O]
+    // [2n], the conjugate-gradient vectors [2n]; weight and E_L totals
+    double *sr_acc = nullptr, *sr_accv = nullptr, *sr_z = nullptr;
+    int sr_n = 0, sr_kinds = 0;
+    double sr_wsum = 0.0, sr_esum = 0.0;
     unsigned char *acc = nullptr;
     double *esum_dev = nullptr;  // [4] accepted, esum, esum2, nflag
 

@@ -106,6 +106,8 @@ def load_library(path: str = LIBPATH):
     L.champ_gpu_set_orbitals.argtypes = [vp, dp]
     L.champ_sr_solve.argtypes = [i32, C.c_double, dp, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                  C.c_double, dp, C.POINTER(SrInfo)]
+    L.champ_gpu_sr_accumulate.argtypes = [vp, i32, i32, dp, dp]
+    L.champ_gpu_sr_solve.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, dp, C.POINTER(SrInfo)]
     L.champ_gpu_set_metrop6.argtypes = [vp, C.c_double, C.c_double, C.c_double]
     L.champ_gpu_get_walker_scalars.argtypes = [vp, dp, dp, dp, dp]
     L.champ_gpu_get_status.argtypes = [vp, C.POINTER(C.c_int32)]
@@ -339,6 +341,25 @@ class Engine:
         e = C.c_double()
         self._check(self.L.champ_gpu_sr_sums(self.ptr, int(kinds), _p(s), _p(os_), _p(eo), C.byref(e)), "sr_sums")
         return s, os_, eo, e.value
+
+    def sr_accumulate(self, kinds: int, reset: bool = False):
+        """Adds the current walkers' SR sums to the accumulators on the GPU
+        (champ_gpu_sr_accumulate); returns the running (wsum, esum)."""
+        w, e = C.c_double(), C.c_double()
+        self._check(self.L.champ_gpu_sr_accumulate(self.ptr, int(kinds), int(bool(reset)), C.byref(w), C.byref(e)),
+                    "sr_accumulate")
+        self._sr_nparm = self.deriv_count(kinds)
+        return w.value, e.value
+
+    def sr_solve(self, sr_tau: float = 0.02, sr_eps: float = 1e-3, sr_adiag: float = 0.01,
+                 dparm_norm_min: float = 1.0):
+        """The SR step from the accumulated sums, S z on the GPU
+        (champ_gpu_sr_solve): returns (dparm, info)."""
+        info = SrInfo()
+        dparm = np.zeros(getattr(self, "_sr_nparm", 0))
+        self._check(self.L.champ_gpu_sr_solve(self.ptr, float(sr_tau), float(sr_eps), float(sr_adiag),
+                                              float(dparm_norm_min), _p(dparm), C.byref(info)), "sr_solve")
+        return dparm, dict(adiag=info.adiag, dparm_norm=info.dparm_norm, cg_iters=info.cg_iters, retries=info.retries)
 
     def set_metrop6(self, deltar: float = 5.0, deltat: float = 1.0, fbias: float = 1.0):
         self._check(self.L.champ_gpu_set_metrop6(self.ptr, float(deltar), float(deltat), float(fbias)), "set_metrop6")

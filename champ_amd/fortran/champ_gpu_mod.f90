@@ -256,6 +256,21 @@ module champ_gpu_mod
       real(c_double), intent(out) :: dparm(*)
       type(champ_sr_info), intent(out) :: info
     end function
+    integer(c_int) function champ_gpu_sr_accumulate(ctx, kinds, reset, wsum, esum) &
+        bind(c, name="champ_gpu_sr_accumulate")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: kinds, reset
+      real(c_double), intent(out) :: wsum, esum
+    end function
+    integer(c_int) function champ_gpu_sr_solve(ctx, sr_tau, sr_eps, sr_adiag, dparm_norm_min, dparm, info) &
+        bind(c, name="champ_gpu_sr_solve")
+      import :: c_int, c_ptr, c_double, champ_sr_info
+      type(c_ptr), value :: ctx
+      real(c_double), value :: sr_tau, sr_eps, sr_adiag, dparm_norm_min
+      real(c_double), intent(out) :: dparm(*)
+      type(champ_sr_info), intent(out) :: info
+    end function
   end interface
 
   public :: champ_gpu_init, champ_gpu_free, champ_gpu_set_walkers, champ_gpu_get_walkers, champ_gpu_seed_rng
@@ -266,6 +281,7 @@ module champ_gpu_mod
   public :: champ_gpu_dmc_set_icasula
   public :: champ_gpu_deriv_count, champ_gpu_deriv_psi, champ_gpu_sr_sums, champ_gpu_set_deriv_map
   public :: champ_gpu_set_params, champ_sr_solve, champ_gpu_set_orb_terms, champ_gpu_set_orbitals
+  public :: champ_gpu_sr_accumulate, champ_gpu_sr_solve
   integer(c_int32_t), parameter, public :: CHAMP_DERIV_JAS_EN = 1, CHAMP_DERIV_JAS_EE = 2
   integer(c_int32_t), parameter, public :: CHAMP_DERIV_JAS_EEN = 4, CHAMP_DERIV_CI = 8, CHAMP_DERIV_MAPPED = 16
   integer(c_int32_t), parameter, public :: CHAMP_DERIV_ORB = 32

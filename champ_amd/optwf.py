@@ -151,12 +151,10 @@ class ChampParams:
                 off[k] = o
                 o += size
         nraw = o
-        cols, names = [], []
+        cols, names = [], []  # a column is a raw index (a unit vector) or a dense vector
 
         def unit(i):
-            v = np.zeros(nraw)
-            v[i] = 1.0
-            return v
+            return int(i)
 
         for it in range(nct):
             for iw in iwjasa[it]:
@@ -202,8 +200,13 @@ class ChampParams:
             for k, (io, jo) in enumerate(terms):
                 cols.append(unit(off[DERIV_ORB] + k))
                 names.append(f"orb{io + 1}->{jo + 1}")
-        m = np.array(cols).T if cols else np.zeros((nraw, 0))
-        return ChampParams(kinds, np.ascontiguousarray(m), names, iwjasa, iwjasb, iwjasc, bool(kinds & DERIV_CI),
+        m = np.zeros((nraw, len(cols)))
+        for j, col in enumerate(cols):
+            if isinstance(col, int):
+                m[col, j] = 1.0
+            else:
+                m[:, j] = col
+        return ChampParams(kinds, m, names, iwjasa, iwjasb, iwjasc, bool(kinds & DERIV_CI),
                            csf, maxcsf, terms if kinds & DERIV_ORB else None)
 
     def apply(self, sy: S.System, dparm: np.ndarray):

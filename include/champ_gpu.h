@@ -401,6 +401,22 @@ int champ_sr_solve(int32_t nparm, double wsum, const double *s, const double *os
                    double esum, double sr_tau, double sr_eps, double sr_adiag, double dparm_norm_min,
                    double *dparm, champ_sr_info *info);
 
+/* SR sums kept on the GPU, for parameter sets whose S (nparm^2 doubles) is
+ * too large to bring to the host at every sample: adds the sums of
+ * champ_gpu_sr_sums over the current walkers (weight 1 each) to accumulators
+ * on the device; reset = 1 clears them first (needed when kinds or the
+ * parameter count change).  wsum and esum (may be NULL) receive the running
+ * total weight and sum of E_L.  The reference accumulates the same sums over
+ * its samples (sr_hs's inputs, optwf_sr.f90:504-620).                        */
+int champ_gpu_sr_accumulate(champ_ctx *ctx, int32_t kinds, int32_t reset, double *wsum, double *esum);
+
+/* The SR step of champ_sr_solve from the sums champ_gpu_sr_accumulate holds
+ * (same equations, shift retries and info), with the products S z of the
+ * conjugate gradients (atimes_n, optwf_sr_more.f90:31-160) computed on the
+ * GPU over the resident S.  dparm [nparm] on the host.                       */
+int champ_gpu_sr_solve(champ_ctx *ctx, double sr_tau, double sr_eps, double sr_adiag, double dparm_norm_min,
+                       double *dparm, champ_sr_info *info);
+
 /* metrop1 over n populations at once, typically different molecules (each
  * context its own system, walker count and precision): every context's
  * kernels run on its own stream in the order of its own champ_gpu_metrop1,

@@ -14,6 +14,10 @@ the energies the CI scripts compare against:
           one sr_n iteration over the Jastrow, determinant and orbital
           parameters; energies of the closing VMC run from
           VMC_C4H6_ci1010_pVTZ_5000-dets_butadiene_optimization.sh:8-9, :18-19)
+  C4H6_500  tests/CI_test/VMC-C4H6-ci1010_pVTZ-00500-dets (the same with 500
+          determinants; VMC_C4H6_ci1010_pVTZ_500-dets_butadiene_optimization.sh)
+  C4H6_15k  tests/CI_test/VMC-C4H6-ci1010_pVTZ-15000-dets (the same with 15000
+          determinants; VMC_C4H6_ci1010_pVTZ_15000-dets_butadiene_optimization.sh)
 
 Run in the build container only (the reference tree is not on GPU hosts):
     python tests/golden/champ_inputs/make_fixtures.py
@@ -48,6 +52,16 @@ CASES = {
                  "ci1010_pVTZ_1_symmetry.sym", "jastrow_good_b3lyp.0", "jastrow.der", "pool/champ_v3_butadiene.xyz",
                  "pool/champ_v3_BFD-T_basis_pointers.bfinfo", "pool/BFD-T.basis.C", "pool/BFD-T.basis.H",
                  "pool/BFD.gauss_ecp.dat.C", "pool/BFD.gauss_ecp.dat.H"]),
+    "C4H6_500": ("VMC-C4H6-ci1010_pVTZ-00500-dets", "VMC_C4H6_ci1010_pVTZ_500-dets_butadiene_optimization.sh",
+                 ["vmc_optimization_500.inp", "TZ_1M_500.det", "champ_v3_trexio_order_ci1010_pVTZ_1_orbitals.lcao",
+                  "ci1010_pVTZ_1_symmetry.sym", "jastrow_good_b3lyp.0", "jastrow.der", "pool/champ_v3_butadiene.xyz",
+                  "pool/champ_v3_BFD-T_basis_pointers.bfinfo", "pool/BFD-T.basis.C", "pool/BFD-T.basis.H",
+                  "pool/BFD.gauss_ecp.dat.C", "pool/BFD.gauss_ecp.dat.H"]),
+    "C4H6_15k": ("VMC-C4H6-ci1010_pVTZ-15000-dets", "VMC_C4H6_ci1010_pVTZ_15000-dets_butadiene_optimization.sh",
+                 ["vmc_optimization_15000.inp", "TZ_1M_15k.det", "champ_v3_trexio_order_ci1010_pVTZ_1_orbitals.lcao",
+                  "ci1010_pVTZ_1_symmetry.sym", "jastrow_good_b3lyp.0", "jastrow.der", "pool/champ_v3_butadiene.xyz",
+                  "pool/champ_v3_BFD-T_basis_pointers.bfinfo", "pool/BFD-T.basis.C", "pool/BFD-T.basis.H",
+                  "pool/BFD.gauss_ecp.dat.C", "pool/BFD.gauss_ecp.dat.H"]),
 }
 
 

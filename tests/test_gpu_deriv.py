@@ -202,6 +202,45 @@ def test_sr_step_through_the_map(require_gpu):
               f"cg {info['cg_iters']}", flush=True)
 
 
+def test_device_sr_matches_the_host_step(require_gpu):
+    """champ_gpu_sr_accumulate + champ_gpu_sr_solve against the same samples'
+    champ_gpu_sr_sums added on the host and champ_sr_solve: c22_md with
+    CHAMP's Jastrow and CI list, three samples; then a new accumulation after
+    reset.  The weight and E_L totals agree exactly, the conjugate gradients
+    take the same number of iterations, and the steps agree to 1e-4 relative:
+    the S z products are summed in another order, and the iterations, which
+    stop at a relative residual of sr_eps = 1e-3, carry that rounding
+    difference amplified by the conditioning of S (4.5e-6 measured), far
+    below the solver's own tolerance; a missing or wrong term would differ at
+    order one."""
+    from champ_amd.engine import sr_solve
+    sy = S.config("c22_md")
+    nw = 128
+    p = _champ_list(sy, True)
+    eng = Engine(sy, nw, lib_path=ENGINE_LIB)
+    eng.set_walkers(sy.initial_walkers(nw, seed=8))
+    eng.seed_rng(S.seed_from_string())
+    eng.hpsi()
+    eng.set_deriv_map(p.kinds, p.m)
+    for rnd in range(2):
+        acc = None
+        for k in range(3):
+            eng.metrop1(3, 0.3)
+            sums = eng.sr_sums(DERIV_MAPPED)
+            acc = [np.array(x, dtype=np.float64) for x in sums] if acc is None else [a + x for a, x in zip(acc, sums)]
+            wsum, esum = eng.sr_accumulate(DERIV_MAPPED, reset=(k == 0))
+            assert wsum == nw * (k + 1) and esum == float(acc[3])
+        d_host, i_host = sr_solve(acc[0], acc[1], acc[2], float(acc[3]), 3 * nw, sr_tau=0.02, sr_eps=1e-3,
+                                  sr_adiag=0.01, lib_path=ENGINE_LIB)
+        d_dev, i_dev = eng.sr_solve(sr_tau=0.02, sr_eps=1e-3, sr_adiag=0.01)
+        assert d_dev.shape == (p.nparm,)
+        assert i_dev["cg_iters"] == i_host["cg_iters"] and i_dev["retries"] == i_host["retries"]
+        assert np.max(np.abs(d_dev - d_host)) <= 1e-4 * np.max(np.abs(d_host)), rnd
+        print(f"[sr device] round {rnd}: {p.nparm} parameters, cg {i_dev['cg_iters']}, "
+              f"max |d_dev - d_host| {np.max(np.abs(d_dev - d_host)):.2e} of {np.max(np.abs(d_host)):.2e}", flush=True)
+        eng.set_params(*p.apply(sy, d_dev))
+
+
 def _orb_system(name):
     """The test system; a single determinant gets three extra (virtual)
     orbitals so that excitations out of the occupied ones exist."""
@@ -325,77 +364,120 @@ def test_sr_lowers_the_h2_ci_energy(require_gpu):
     assert r1["nflagged"] == 0
 
 
-def test_sr_iteration_on_the_5000_determinant_case(require_gpu):
-    """The reference's optimisation CI case with 5000 determinants
-    (tests/CI_test/VMC-C4H6-ci1010_pVTZ-05000-dets: butadiene, BFD, method
-    sr_n with ioptjas, ioptci and ioptorb, nextorb 280, nopt_iter 1, sr_tau
-    0.025, sr_eps 0.001, sr_adiag 0.01) run as optwf_sr runs it
-    (optwf_sr.f90:164-353): sampling with metrop6, the SR sums over its
-    parameters (the 13 Jastrow coefficients of its jastrow.der, 4999
-    determinant coefficients and the 7021 orbital excitations optorb_define
-    builds from its symmetry file over all 226 orbitals: 12033), one SR step,
-    the new Jastrow, determinant and orbital coefficients, and the closing VMC
-    run, whose energy the case's CI script compares with
-    tools/compare_value.py's rule |E - E_ref| <= 2 sqrt(err^2 + err_ref^2)
-    (against both of its reference runs)."""
+# fixture, input, determinants, orbital excitations of optorb_define
+SR_CASES = [("C4H6_500", "vmc_optimization_500.inp", 500, 3404),
+            ("C4H6_5k", "vmc_optimization_5000.inp", 5000, 7021),
+            ("C4H6_15k", "vmc_optimization_15000.inp", 15469, 9337)]
+
+
+def _sr_case(name, inp_name, ndet, nterm):
     import os
-    import time
 
     from champ_amd import champ_io as io
-    from champ_amd.engine import sr_solve
-    from tests.test_champ_io import G, reference_energy
+    from tests.test_champ_io import G
 
-    d = os.path.join(G, "C4H6_5k")
-    inp = os.path.join(d, "vmc_optimization_5000.inp")
-    sy = io.load_input(inp, name="C4H6_5k", nadorb=280)
-    assert sy.coef.shape == (226, 226) and len(sy.cdet) == 5000
+    d = os.path.join(G, name)
+    sy = io.load_input(os.path.join(d, inp_name), name=name, nadorb=280)
+    assert sy.coef.shape == (226, 226) and len(sy.cdet) == ndet
     der = io.read_jastrow_der(os.path.join(d, "jastrow.der"), sy.nctype, sy.nspin2b)
     irrep = io.read_symmetry(os.path.join(d, "ci1010_pVTZ_1_symmetry.sym"), sy.coef.shape[0])
     terms = optwf.optorb_define(sy.iworbd, sy.nup, irrep, ncore=0, nadorb=280)
     p = optwf.ChampParams.build(sy, der, ci=True, orb_terms=terms)
-    assert len(terms) == 7021 and p.nparm == 13 + 4999 + 7021
-    W, nsample = 2048, 20
-    t0 = time.time()
+    assert len(terms) == nterm and p.nparm == 13 + ndet - 1 + nterm
+    return sy, terms, p
+
+
+def _sr_sample_and_step(sy, terms, p, W, nsample, gap, seed):
+    """optwf_sr's sampling and step: W walkers, nsample samples gap sweeps
+    apart (metrop6), the sums on the GPU, one step with the inputs' sr_tau,
+    sr_eps and sr_adiag; returns (dparm, info, <E> of the samples)."""
     eng = Engine(sy, W, lib_path=ENGINE_LIB)
-    eng.set_walkers(sy.initial_walkers(W, seed=3))
+    eng.set_walkers(sy.initial_walkers(W, seed=seed))
     eng.seed_rng(S.seed_from_string())
     eng.hpsi()
     eng.set_orb_terms(terms)  # before the map: the raw layout includes the terms
     eng.set_deriv_map(p.kinds, p.m)
-    r0 = eng.vmc(20, 5, nblkeq=2, imetro=6)
-    print(f"[sr 5k] set-up and first VMC {time.time() - t0:.1f} s: E = {r0['efin']:.5f} +- {r0['eerr']:.5f} "
-          f"(sigma {r0['sigma']:.3f})", flush=True)
-    t0 = time.time()
-    acc = None
-    for _ in range(nsample):
-        eng.metrop6(2)
-        sums = eng.sr_sums(DERIV_MAPPED)
-        if acc is None:
-            acc = [np.array(x, dtype=np.float64) for x in sums]
-        else:
-            for a, x in zip(acc, sums):
-                a += x
-    print(f"[sr 5k] {nsample} samples of {W} walkers, {p.nparm} parameters: {time.time() - t0:.1f} s", flush=True)
-    t0 = time.time()
-    dparm, info = sr_solve(acc[0], acc[1], acc[2], float(acc[3]), nsample * W, sr_tau=0.025, sr_eps=1e-3,
-                           sr_adiag=0.01, lib_path=ENGINE_LIB)
-    print(f"[sr 5k] solve {time.time() - t0:.1f} s: rms step {info['dparm_norm']:.3e}, cg {info['cg_iters']}, "
-          f"adiag {info['adiag']}", flush=True)
+    eng.metrop6(40)
+    for k in range(nsample):
+        eng.metrop6(gap)
+        wsum, esum = eng.sr_accumulate(DERIV_MAPPED, reset=(k == 0))
+    assert wsum == W * nsample
+    dparm, info = eng.sr_solve(sr_tau=0.025, sr_eps=1e-3, sr_adiag=0.01)
+    eng.close()
     assert np.all(np.isfinite(dparm)) and info["dparm_norm"] <= 1.0
-    eng.set_params(*p.apply(sy, dparm))
-    eng.set_orbitals(p.orbitals(sy, dparm))
-    t0 = time.time()
-    r = eng.vmc(20, 5, nblkeq=2, imetro=6)
+    return dparm, info, esum / wsum
+
+
+def _closing_vmc(sy, p, dparm, W=2048):
+    a4, b, c, cdet = p.apply(sy, dparm)
+    sy2 = dataclasses.replace(sy, a4=a4, b=b, c=c, cdet=cdet, coef=p.orbitals(sy, dparm))
+    eng = Engine(sy2, W, lib_path=ENGINE_LIB)
+    eng.set_walkers(sy2.initial_walkers(W, seed=3))
+    eng.seed_rng(S.seed_from_string())
+    eng.hpsi()
+    r = eng.vmc(20, 5, nblkeq=3, imetro=6)
     st = eng.status()
-    print(f"[sr 5k] closing VMC {time.time() - t0:.1f} s: E = {r['efin']:.5f} +- {r['eerr']:.5f} "
-          f"(sigma {r['sigma']:.3f}, acceptance {r['accept']:.3f}); status words {np.unique(st).tolist()} "
-          f"on {int(np.count_nonzero(st))} walkers", flush=True)
+    eng.close()
     # A proposal where every orbital vanishes (beyond the numerical basis'
     # range) has a determinant ratio of exactly 0: determinante.f90:46 returns
     # it and the move is rejected, so the walker is unchanged.  That is the
     # only mark allowed here.
     assert np.all((st & ~2) == 0) and np.count_nonzero(st) <= W // 100
+    return r
+
+
+@pytest.mark.parametrize("name,inp_name,ndet,nterm", SR_CASES, ids=[c[0] for c in SR_CASES])
+def test_sr_iteration_on_the_optimisation_ci_cases(require_gpu, name, inp_name, ndet, nterm):
+    """The reference's optimisation CI cases with 500, 5000 and 15469
+    determinants (tests/CI_test/VMC-C4H6-ci1010_pVTZ-00500-dets, -05000-dets
+    and -15000-dets: butadiene, BFD, method sr_n with ioptjas, ioptci and
+    ioptorb, nextorb 280, nopt_iter 1, sr_tau 0.025, sr_eps 0.001, sr_adiag
+    0.01) run as optwf_sr runs them (optwf_sr.f90:164-353) and as their CI
+    scripts run it: on 1 and on 2 cores, that is one walker chain per core,
+    sampled at every step of its vmc_nblk 100 x vmc_nstep 20 (vmc_mov1 runs
+    one walker per process).  The SR sums over their parameters (the 13
+    Jastrow coefficients of jastrow.der, ndet - 1 determinant coefficients
+    and the orbital excitations optorb_define builds from the symmetry file
+    over all 226 orbitals: 3916, 12033 and 24818) accumulated on the GPU, one
+    SR step (S z on the GPU), and the closing VMC run (2048 walkers) with the
+    new Jastrow, determinant and orbital coefficients, whose energy each
+    case's CI script compares with tools/compare_value.py's rule
+    |E - E_ref| <= 2 sqrt(err^2 + err_ref^2).  With 2000 or 4000 samples for
+    thousands of parameters the step is mostly noise and raises the energy,
+    in the reference as here."""
+    import time
+
+    from tests.test_champ_io import reference_energy
+
+    sy, terms, p = _sr_case(name, inp_name, ndet, nterm)
     for cores in (1, 2):
-        eref, err_ref = reference_energy("C4H6_5k", cores)
-        print(f"[sr 5k] reference ({cores} cores) {eref} +- {err_ref}", flush=True)
+        t0 = time.time()
+        dparm, info, es = _sr_sample_and_step(sy, terms, p, cores, 2000, 1, seed=5)
+        r = _closing_vmc(sy, p, dparm)
+        eref, err_ref = reference_energy(name, cores)
+        print(f"[sr {name}] {cores} walker chain(s) x 2000 samples, {p.nparm} parameters: <E> {es:.5f}, rms step "
+              f"{info['dparm_norm']:.3e}, cg {info['cg_iters']}; closing VMC E = {r['efin']:.5f} +- {r['eerr']:.5f} "
+              f"(sigma {r['sigma']:.3f}); reference ({cores} cores) {eref} +- {err_ref}; {time.time() - t0:.1f} s",
+              flush=True)
         assert abs(r["efin"] - eref) <= 2.0 * np.hypot(r["eerr"], err_ref)
+
+
+def test_well_sampled_sr_step_lowers_the_energy(require_gpu):
+    """The 500-determinant optimisation case with the step drawn from 40960
+    samples (2048 walkers, 20 samples two sweeps apart) instead of 2000: the
+    closing VMC energy lies more than three standard errors below the
+    starting wave function's (-26.2205 +- 0.0023 -> -26.2336 +- 0.0021
+    measured)."""
+    sy, terms, p = _sr_case(*SR_CASES[0])
+    W = 2048
+    eng = Engine(sy, W, lib_path=ENGINE_LIB)
+    eng.set_walkers(sy.initial_walkers(W, seed=3))
+    eng.seed_rng(S.seed_from_string())
+    eng.hpsi()
+    r0 = eng.vmc(20, 5, nblkeq=3, imetro=6)
+    eng.close()
+    dparm, info, es = _sr_sample_and_step(sy, terms, p, W, 20, 2, seed=5)
+    r1 = _closing_vmc(sy, p, dparm)
+    print(f"[sr well sampled] E {r0['efin']:.5f} +- {r0['eerr']:.5f} (sigma {r0['sigma']:.3f}) -> {r1['efin']:.5f} "
+          f"+- {r1['eerr']:.5f} (sigma {r1['sigma']:.3f}); rms step {info['dparm_norm']:.3e}", flush=True)
+    assert r1["efin"] < r0["efin"] - 3.0 * np.hypot(r0["eerr"], r1["eerr"])
