@@ -1,5 +1,7 @@
 // engine_dmc.hip -- diffusion Monte Carlo of the population: CHAMP's dmc_ps
-// mover (dmc/dmc_ps_mov1.f90:4-838) with the default controls (idmc = 2,
+// mover (dmc/dmc_ps_mov1.f90:4-838), or for all-electron systems (nloc = 0)
+// its dmc_ae mover (dmc/dmc_ae_mov1.f90, dmc.f90:98-102; dmc_ae.hpp), with
+// the default controls (idmc = 2,
 // icasula = 0: the non-local pseudopotential in the locality approximation,
 // ipq = 1, itau_eff = 1, iacc_rej = 1, icross = 1, icut_br = 0, icut_e = 0,
 // limit_wt_dmc = 0, nforce = 1, node_cutoff = 0), the velocity-ratio
@@ -20,6 +22,7 @@
 // from one stream per MPI rank in walker order, which a population moved in
 // parallel cannot reproduce; the oracle's DMC follows these stream rules.
 #include "engine_internal.hpp"
+#include "dmc_ae.hpp"
 #include "hpsi_args.hpp"
 
 #include <algorithm>
@@ -93,16 +96,21 @@ __global__ void k_dmc_begin(int W, const double *eold, double *eprev, double *ac
 }
 
 struct DmcArgs {
-    int W, NE;
+    int W, NE, NC, nloc;
     double tau, etrial, eest, ffi, fprod;
     const double *ddx, *vjT, *eold, *eprev, *ekin, *acc;
+    const double *xT, *cent;  // positions [W][3][NE], centres [NC][3] (the all-electron ratio)
+    const int *iwctype;
+    const EcpParams *ecp;
     double *wt, *fratio, *out;
     int *age;
 };
 
-// The velocity ratio of dmc_eloc_cutoff for nloc > 0 (branching.f90:226-238,
-// tratio = 1): sqrt(sum vavvn^2 v^2 / sum v^2) over the electrons' drifts
-// v = grad ln psi at the walker's configuration.  One wavefront per walker.
+// The velocity ratio of dmc_eloc_cutoff (branching.f90:226-278, tratio = 1):
+// sqrt(sum vavvn^2 v^2 / sum v^2) over the electrons' drifts v = grad ln psi
+// at the walker's configuration, with adrift 0.5 for nloc > 0 and, for
+// nloc = 0, the nearest nucleus' adrift of the all-electron mover (adrift0
+// 0.1).  One wavefront per walker.
 // WEIGHT 0: only fratio (the first step, dmc_ps_mov1.f90:162-172);
 // WEIGHT 1: the reweighting of the step (dmc_ps_mov1.f90:491-617:
 // calculate_fratio, calculate_reweight with icut_e = 0, the population factor
@@ -120,6 +128,12 @@ __global__ void __launch_bounds__(256) k_dmc_weight(DmcArgs a)
         const double *dx = a.ddx + ((size_t)w * NE + e) * 3;
         const double *vj = a.vjT + (size_t)w * 3 * NE;
         const double v0 = dx[0] + vj[e], v1 = dx[1] + vj[NE + e], v2c = dx[2] + vj[2 * NE + e];
+        if (a.nloc == 0) {
+            const double *xw = a.xT + (size_t)w * 3 * NE;
+            const double xe[3] = {xw[e], xw[NE + e], xw[2 * NE + e]}, ve[3] = {v0, v1, v2c};
+            ae_fratio_terms(a.cent, a.iwctype, a.ecp->znuc, a.NC, tau, xe, ve, vav2sum, v2sum);
+            continue;
+        }
         const double v2 = v0 * v0 + v1 * v1 + v2c * v2c;
         const double vavvt = (sqrt(1.0 + 2.0 * adrift * v2 * tau) - 1.0) / (adrift * v2);
         const double vavvn = vavvt / tau;
@@ -179,6 +193,7 @@ static DmcArgs dmc_args(champ_ctx *c)
     a.W = c->W; a.NE = c->NE; a.tau = D.tau; a.etrial = D.etrial; a.eest = D.eest; a.ffi = 1.0; a.fprod = D.fprod;
     a.ddx = c->ddx; a.vjT = c->vjT; a.eold = c->eold; a.eprev = c->dmc_eprev; a.ekin = c->ekin; a.acc = c->dmc_acc;
     a.wt = c->dmc_wt; a.fratio = c->dmc_fratio; a.out = c->dmc_out; a.age = c->dmc_age;
+    a.NC = c->NC; a.nloc = c->nloc; a.xT = c->xT; a.cent = c->cent; a.iwctype = c->iwctype; a.ecp = c->ecp;
     return a;
 }
 
@@ -270,10 +285,6 @@ extern "C" int champ_gpu_dmc_init(champ_ctx *c, int32_t nconf, double tau, doubl
         nfprod < 1 || !pop_state) {
         set_error("dmc_init: 1 <= nconf <= walkers, tau > 0, finite etrial, nfprod >= 1 and a population stream required");
         return CHAMP_ERR_ARG;
-    }
-    if (c->nloc == 0) {
-        set_error("dmc_init: all-electron DMC (dmc_ae) is not provided; the dmc_ps mover needs a pseudopotential");
-        return CHAMP_ERR_UNSUPPORTED;
     }
     if (!(pop_state[0] | pop_state[1] | pop_state[2] | pop_state[3])) {
         set_error("dmc_init: the population stream is all-zero");

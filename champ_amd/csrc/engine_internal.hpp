@@ -202,7 +202,8 @@ struct champ_ctx {
 
 namespace champ {
 // DMC move accumulators per walker (engine_kernels.hip k_move_b, engine_dmc.hip)
-enum { DMC_PP = 0, DMC_DFUS2AC, DMC_DFUS2UN, DMC_PSUM, DMC_NODECR, DMC_IACCEPT, DMC_ACC = 8 };
+// DMC_FNORMO: the all-electron mover's forward normalisation of the current move
+enum { DMC_PP = 0, DMC_DFUS2AC, DMC_DFUS2UN, DMC_PSUM, DMC_NODECR, DMC_IACCEPT, DMC_FNORMO, DMC_ACC = 8 };
 // per-walker estimator terms of a DMC step
 enum { DMC_O_WT = 0, DMC_O_WE, DMC_O_PE, DMC_O_TPB, DMC_O_TAU, DMC_O_PSUM, DMC_O_NODECR, DMC_OUT = 8 };
 // kernel launchers (engine_kernels.hip)

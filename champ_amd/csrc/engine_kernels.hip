@@ -9,6 +9,7 @@
 // AO->MO contractions are population-wide GEMMs.
 #include "een_poly.hpp"
 #include "engine_internal.hpp"
+#include "dmc_ae.hpp"
 #include "metrop6.hpp"
 
 #include <algorithm>
@@ -188,6 +189,18 @@ __global__ void __launch_bounds__(256) CHAMP_WAVES_PER_EU(CHAMP_MOVE_WPE) k_move
                            a.m6rec + (size_t)w * M6_REC);
                 for (int k = 0; k < 4; k++) a.rng[4 * (size_t)w + k] = s[k];
                 for (int k = 0; k < 3; k++) { a.xnew[3 * w + k] = xn[k]; a.vold[3 * w + k] = vo[k]; }
+                xn0 = xn[0]; xn1 = xn[1]; xn2 = xn[2];
+            } else if (lane == 0 && a.imetro == 2 && a.ecp->nloc == 0) {
+                // dmc_ae proposal (dmc_ae_mov1.f90:226-353)
+                uint64_t s[4];
+                for (int k = 0; k < 4; k++) s[k] = a.rng[4 * (size_t)w + k];
+                const double xo[3] = {xw[iel], xw[NE + iel], xw[2 * NE + iel]};
+                double xn[3], dfus2o, fnormo;
+                ae_propose(a.cent, a.iwctype, a.ecp->znuc, NC, a.tau, xo, vo, s, xn, dfus2o, fnormo);
+                for (int k = 0; k < 4; k++) a.rng[4 * (size_t)w + k] = s[k];
+                for (int k = 0; k < 3; k++) { a.xnew[3 * w + k] = xn[k]; a.vold[3 * w + k] = vo[k]; }
+                a.dfus2o[w] = dfus2o;
+                a.dmc_acc[(size_t)w * DMC_ACC + DMC_FNORMO] = fnormo;
                 xn0 = xn[0]; xn1 = xn[1]; xn2 = xn[2];
             } else if (lane == 0 && a.imetro == 2) {
                 // dmc_ps drift-diffusion proposal (dmc_ps_mov1.f90:291-303): the
@@ -587,18 +600,27 @@ __global__ void __launch_bounds__(256) k_move_b(MoveArgs a)
                 double *ac = a.dmc_acc + (size_t)w * DMC_ACC;
                 const double psido = a.psido[w];
                 if (psidn * psido <= 0.0) ac[DMC_NODECR] += 1.0;
-                const double v2new = vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2];
-                const double vavvt = (sqrt(1.0 + 2.0 * adrift * v2new * tau) - 1.0) / (adrift * v2new);
-                double dfus2n = 0.0;
-                for (int k = 0; k < 3; k++) {
-                    const double drift = vavvt * vn[k];
-                    const double xbac = a.xnew[3 * w + k] + drift;
-                    const double dfus = xbac - xw[k * NE + iel];
-                    dfus2n = dfus2n + dfus * dfus;
+                double dfus2n = 0.0, fnorm_ratio = 1.0;
+                if (a.ecp->nloc == 0) {
+                    // dmc_ae's reverse Green function (dmc_ae_mov1.f90:388-490)
+                    const double xo[3] = {xw[iel], xw[NE + iel], xw[2 * NE + iel]};
+                    const double xn[3] = {a.xnew[3 * w], a.xnew[3 * w + 1], a.xnew[3 * w + 2]};
+                    double fnormn;
+                    ae_reverse(a.cent, a.iwctype, a.ecp->znuc, a.NC, tau, xo, xn, vn, dfus2n, fnormn);
+                    fnorm_ratio = fnormn / ac[DMC_FNORMO];
+                } else {
+                    const double v2new = vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2];
+                    const double vavvt = (sqrt(1.0 + 2.0 * adrift * v2new * tau) - 1.0) / (adrift * v2new);
+                    for (int k = 0; k < 3; k++) {
+                        const double drift = vavvt * vn[k];
+                        const double xbac = a.xnew[3 * w + k] + drift;
+                        const double dfus = xbac - xw[k * NE + iel];
+                        dfus2n = dfus2n + dfus * dfus;
+                    }
                 }
                 const double dfus2o = a.dfus2o[w];
                 const double rat = psidn / psido;
-                double p = rat * rat * exp(2 * (psijn - a.psijo[w])) * exp((dfus2o - dfus2n) / (2.0 * tau));
+                double p = rat * rat * exp(2 * (psijn - a.psijo[w])) * exp((dfus2o - dfus2n) / (2.0 * tau)) * fnorm_ratio;
                 const int age = a.dmc_age[w];
                 if (age > 50) p = p * pow(1.1, (double)(age - 50));
                 ac[DMC_PP] = ac[DMC_PP] * p;

@@ -251,13 +251,14 @@ int champ_gpu_vmc(champ_ctx *ctx, int32_t nblk, int32_t nstep, int32_t nblkeq, d
                   double *enow, champ_vmc_result *res);
 
 /* --------------------------------------------------------------------------
- * Diffusion Monte Carlo (dmc.f90 with the dmc_ps mover, dmc_ps_mov1.f90:4-838)
- * at CHAMP's default controls: idmc = 2, icasula = 0 (the non-local
+ * Diffusion Monte Carlo (dmc.f90 with the dmc_ps mover, dmc_ps_mov1.f90:4-838,
+ * or for all-electron systems (nloc = 0) the dmc_ae mover, dmc_ae_mov1.f90,
+ * as dmc.f90:98-102 chooses) at CHAMP's default controls: idmc = 2,
+ * icasula = 0 (the non-local
  * pseudopotential in the locality approximation; -1, T-moves, through
  * champ_gpu_dmc_set_icasula), ipq = 1, itau_eff = 1,
  * iacc_rej = 1, icross = 1, icut_br = 0, icut_e = 0, nforce = 1,
- * node_cutoff = 0.  Systems with a pseudopotential only (CHAMP's all-electron
- * mover dmc_ae is not provided: CHAMP_ERR_UNSUPPORTED).  The context's walker
+ * node_cutoff = 0.  The context's walker
  * count at the first champ_gpu_dmc_init is the population's capacity (MWALK);
  * from then on the context's walkers are the live population, slots
  * 0 .. nwalk-1.  Each walker slot draws its moves from its own stream; the

@@ -1731,7 +1731,8 @@ int orc_metrop6_sweep(const champ_system_t *s, oracle_walker_t *w, double deltar
 }
 
 /* ------------------------------------------------------------------------ */
-/* DMC (dmc.f90 with dmc_ps, dmc_ps_mov1.f90:4-838) at the default controls */
+/* DMC (dmc.f90 with dmc_ps, dmc_ps_mov1.f90:4-838, or dmc_ae,           */
+/* dmc_ae_mov1.f90, when nloc = 0) at the default controls                   */
 /* idmc = 2, icasula = 0, ipq = 1, itau_eff = 1, iacc_rej = 1, icross = 1,  */
 /* icut_br = 0, icut_e = 0, nforce = 1, node_cutoff = 0.  One stream per    */
 /* walker slot (moves and quadrature rotation) and one population stream    */
@@ -1789,6 +1790,156 @@ static double dmc_fratio(const oracle_walker_t *w, double tau)
         const double vavvn = vavvt / tau;
         vav2sum = vav2sum + orc_powi(vavvn, 2) * v2;
         v2sum = v2sum + v2;
+    }
+    return sqrt(vav2sum / v2sum);
+}
+
+/* ---- all-electron mover (dmc_ae_mov1.f90, dmc.f90:98-102 for nloc = 0) ---- */
+
+typedef struct {
+    int ic;
+    double rmin, vr, va, vp, v2, zeta, zeff, xaxis[3], zaxis[3];
+} ae_frame_t;
+
+/* frame at x with drift v: nearest nucleus, radial and perpendicular drift
+   (dmc_ae_mov1.f90:226-270, :388-440) */
+static void ae_frame(const champ_system_t *s, double tau, const double x[3], const double v[3], ae_frame_t *f)
+{
+    const double eps = 1.e-10;
+    double ren2mn = 1.e100;
+    f->ic = 0;
+    for (int ic = 0; ic < s->ncent; ic++) {
+        const double ren2 = orc_powi(x[0] - s->cent[3 * ic], 2) + orc_powi(x[1] - s->cent[3 * ic + 1], 2) +
+                            orc_powi(x[2] - s->cent[3 * ic + 2], 2);
+        if (ren2 < ren2mn) { ren2mn = ren2; f->ic = ic; }
+    }
+    double rv[3], rmin = 0, vr = 0, v2 = 0;
+    for (int k = 0; k < 3; k++) {
+        rv[k] = x[k] - s->cent[3 * f->ic + k];
+        rmin = rmin + rv[k] * rv[k];
+        vr = vr + v[k] * rv[k];
+        v2 = v2 + v[k] * v[k];
+    }
+    rmin = sqrt(rmin);
+    f->rmin = rmin; f->vr = vr / rmin; f->v2 = v2; f->va = sqrt(v2);
+    f->zeff = s->znuc[s->iwctype[f->ic]];
+    f->zeta = sqrt(1.0 / tau + f->zeff * f->zeff);
+    double vp = 0;
+    for (int k = 0; k < 3; k++) {
+        f->zaxis[k] = rv[k] / rmin;
+        f->xaxis[k] = v[k] - f->vr * f->zaxis[k];
+        vp = vp + f->xaxis[k] * f->xaxis[k];
+    }
+    vp = sqrt(vp);
+    if (vp < eps) {
+        f->xaxis[0] = eps * (1 - f->zaxis[0] * f->zaxis[0]);
+        f->xaxis[1] = eps * (-f->zaxis[0] * f->zaxis[1]);
+        f->xaxis[2] = eps * (-f->zaxis[0] * f->zaxis[2]);
+        vp = eps * sqrt(1 + eps - f->zaxis[0] * f->zaxis[0]);
+    }
+    for (int k = 0; k < 3; k++) f->xaxis[k] = f->xaxis[k] / vp;
+    f->vp = vp;
+}
+
+/* adrift near the nucleus and vavvt (dmc_ae_mov1.f90:273-277, adrift0 = 0.1) */
+static double ae_vavvt(const ae_frame_t *f, double tau)
+{
+    const double eps = 1.e-10, adrift0 = 0.1;
+    const double hafzr2 = orc_powi(0.5 * f->zeff * f->rmin, 2);
+    const double adrift = (0.5 * (1 + eps + f->vr / f->va)) + adrift0 * hafzr2 / (1 + hafzr2);
+    return (sqrt(1 + 2 * adrift * f->v2 * tau) - 1) / (adrift * f->v2);
+}
+
+static double ae_term(double tau)
+{
+    const double pi = 4 * atan(1.0);
+    return orc_powi(sqrt(2 * pi * tau), 3) / pi;
+}
+
+/* forward move (dmc_ae_mov1.f90:272-353): Gaussian around the drifted point
+   with probability pgaus, else exponential around the nucleus */
+static void ae_propose(const champ_system_t *s, double tau, const double xo[3], const double vo[3], uint64_t rng[4],
+                       double xn[3], double *dfus2o, double *fnormo)
+{
+    const double pi = 4 * atan(1.0), rttau = sqrt(tau);
+    ae_frame_t f;
+    ae_frame(s, tau, xo, vo, &f);
+    const double *c = s->cent + 3 * f.ic;
+    const double vavvt = ae_vavvt(&f, tau);
+    double rtry = f.rmin + vavvt * f.vr, qgaus;
+    if (rtry > 0) {
+        qgaus = 0.5 * erfc(rtry / sqrt(2 * tau));
+        const double xprime = vavvt * f.vp * rtry / (0.5 * (f.rmin + rtry)), zprime = rtry;
+        for (int k = 0; k < 3; k++) xn[k] = c[k] + f.xaxis[k] * xprime + f.zaxis[k] * zprime;
+    } else {
+        qgaus = 0.5 * (1 + erf(-rtry / sqrt(2 * tau)));
+        for (int k = 0; k < 3; k++) xn[k] = c[k];
+    }
+    const double pgaus = 1 - qgaus;
+    double dfus2a = 0, dfusb;
+    if (orc_random_dp(rng) < pgaus) {
+        double dfus2b = 0;
+        for (int k = 0; k < 3; k++) {
+            const double dfus = orc_gauss(rng) * rttau;
+            dfus2a = dfus2a + dfus * dfus;
+            xn[k] = xn[k] + dfus;
+            dfus2b = dfus2b + orc_powi(xn[k] - c[k], 2);
+        }
+        dfusb = sqrt(dfus2b);
+    } else {
+        const double u1 = orc_random_dp(rng), u2 = orc_random_dp(rng), u3 = orc_random_dp(rng);
+        dfusb = (-0.5 / f.zeta) * log(u1 * u2 * u3);
+        const double costht = 2 * (orc_random_dp(rng) - 0.5), sintht = sqrt(1 - costht * costht);
+        const double phi = 2 * pi * orc_random_dp(rng);
+        const double dfus3[3] = {dfusb * sintht * cos(phi), dfusb * sintht * sin(phi), dfusb * costht};
+        for (int k = 0; k < 3; k++) {
+            dfus2a = dfus2a + orc_powi(c[k] + dfus3[k] - xn[k], 2);
+            xn[k] = c[k] + dfus3[k];
+        }
+    }
+    *dfus2o = dfus2a;
+    *fnormo = pgaus + qgaus * ae_term(tau) * orc_powi(f.zeta, 3) * exp(-2 * f.zeta * dfusb + 0.5 * dfus2a / tau);
+}
+
+/* reverse Green function from xn back to xo (dmc_ae_mov1.f90:388-480) */
+static void ae_reverse(const champ_system_t *s, double tau, const double xo[3], const double xn[3], const double vn[3],
+                       double *dfus2n, double *fnormn)
+{
+    ae_frame_t f;
+    ae_frame(s, tau, xn, vn, &f);
+    const double *c = s->cent + 3 * f.ic;
+    const double vavvt = ae_vavvt(&f, tau);
+    double rtry = f.rmin + vavvt * f.vr, qgaus, dfus2a = 0, dfus2b = 0;
+    if (rtry > 0) {
+        qgaus = 0.5 * erfc(rtry / sqrt(2 * tau));
+        const double xprime = vavvt * f.vp * rtry / (0.5 * (f.rmin + rtry)), zprime = rtry;
+        for (int k = 0; k < 3; k++) {
+            const double xbac = c[k] + f.xaxis[k] * xprime + f.zaxis[k] * zprime;
+            dfus2b = dfus2b + orc_powi(c[k] - xo[k], 2);
+            dfus2a = dfus2a + orc_powi(xbac - xo[k], 2);
+        }
+    } else {
+        qgaus = 0.5 * (1 + erf(-rtry / sqrt(2 * tau)));
+        for (int k = 0; k < 3; k++) dfus2b = dfus2b + orc_powi(c[k] - xo[k], 2);
+        dfus2a = dfus2b;
+    }
+    *dfus2n = dfus2a;
+    const double pgaus = 1 - qgaus, dfusb = sqrt(dfus2b);
+    *fnormn = pgaus + qgaus * ae_term(tau) * orc_powi(f.zeta, 3) * exp(-2 * f.zeta * dfusb + 0.5 * dfus2a / tau);
+}
+
+/* dmc_eloc_cutoff for nloc = 0 (branching.f90:240-278), adrift = adrift0 = 0.1, tratio = 1 */
+static double dmc_fratio_ae(const champ_system_t *s, const oracle_walker_t *w, double tau)
+{
+    double vav2sum = 0, v2sum = 0;
+    for (int i = 0; i < w->nelec; i++) {
+        double v[3], x[3];
+        for (int k = 0; k < 3; k++) { v[k] = w->ddx[3 * i + k] + w->vj[3 * i + k]; x[k] = w->x[3 * i + k]; }
+        ae_frame_t f;
+        ae_frame(s, tau, x, v, &f);
+        const double vavvn = ae_vavvt(&f, tau) / tau;
+        vav2sum = vav2sum + vavvn * vavvn * f.v2;
+        v2sum = v2sum + f.v2;
     }
     return sqrt(vav2sum / v2sum);
 }
@@ -1938,7 +2089,8 @@ int orc_dmc_step(const champ_system_t *s, orc_dmc_t *d, double out[8])
     d->fprod = d->fprod * ffn / d->ff[ipmod];
     d->ff[ipmod] = ffn;
     if (d->first) {
-        for (int iw = 0; iw < d->nwalk; iw++) d->fratio[iw] = dmc_fratio(d->w[iw], tau);
+        for (int iw = 0; iw < d->nwalk; iw++)
+            d->fratio[iw] = s->nloc > 0 ? dmc_fratio(d->w[iw], tau) : dmc_fratio_ae(s, d->w[iw], tau);
         d->first = 0;
     }
     double wsum1 = 0, esum1 = 0, nod = 0;
@@ -1951,29 +2103,38 @@ int orc_dmc_step(const champ_system_t *s, orc_dmc_t *d, double out[8])
             double vold[3], vnew[3], xold[3], xnew[3];
             for (int k = 0; k < 3; k++) xold[k] = w->x[3 * i + k];
             orc_determinante_grad(s, w, i, 1, vold);
-            const double v2old = orc_powi(vold[0], 2) + orc_powi(vold[1], 2) + orc_powi(vold[2], 2);
-            double vavvt = (sqrt(1.0 + 2.0 * adrift * v2old * tau) - 1.0) / (adrift * v2old);
-            double dfus2o = 0;
-            for (int k = 0; k < 3; k++) {
-                const double drift = vavvt * vold[k];
-                const double dfus = orc_gauss(w->rng) * rttau;
-                const double dx = drift + dfus;
-                dfus2o = dfus2o + orc_powi(dfus, 2);
-                xnew[k] = xold[k] + dx;
+            double dfus2o = 0, fnormo = 1, fnormn = 1;
+            if (s->nloc == 0) {
+                ae_propose(s, tau, xold, vold, w->rng, xnew, &dfus2o, &fnormo);
+            } else {
+                const double v2old = orc_powi(vold[0], 2) + orc_powi(vold[1], 2) + orc_powi(vold[2], 2);
+                const double vavvt = (sqrt(1.0 + 2.0 * adrift * v2old * tau) - 1.0) / (adrift * v2old);
+                for (int k = 0; k < 3; k++) {
+                    const double drift = vavvt * vold[k];
+                    const double dfus = orc_gauss(w->rng) * rttau;
+                    const double dx = drift + dfus;
+                    dfus2o = dfus2o + orc_powi(dfus, 2);
+                    xnew[k] = xold[k] + dx;
+                }
             }
             double psidn, psijn;
             orc_psie(s, w, i, xnew, &psidn, &psijn, 0);
             orc_determinante_grad(s, w, i, 0, vnew);
             if (psidn * w->psido <= 0) nod++;
-            const double v2new = orc_powi(vnew[0], 2) + orc_powi(vnew[1], 2) + orc_powi(vnew[2], 2);
-            vavvt = (sqrt(1.0 + 2.0 * adrift * v2new * tau) - 1.0) / (adrift * v2new);
             double dfus2n = 0;
-            for (int k = 0; k < 3; k++) {
-                const double drift = vavvt * vnew[k];
-                const double dfus = xnew[k] + drift - xold[k];
-                dfus2n = dfus2n + orc_powi(dfus, 2);
+            if (s->nloc == 0) {
+                ae_reverse(s, tau, xold, xnew, vnew, &dfus2n, &fnormn);
+            } else {
+                const double v2new = orc_powi(vnew[0], 2) + orc_powi(vnew[1], 2) + orc_powi(vnew[2], 2);
+                const double vavvt = (sqrt(1.0 + 2.0 * adrift * v2new * tau) - 1.0) / (adrift * v2new);
+                for (int k = 0; k < 3; k++) {
+                    const double drift = vavvt * vnew[k];
+                    const double dfus = xnew[k] + drift - xold[k];
+                    dfus2n = dfus2n + orc_powi(dfus, 2);
+                }
             }
-            double p = orc_powi(psidn / w->psido, 2) * exp(2 * (psijn - w->psijo)) * exp((dfus2o - dfus2n) / (2 * tau));
+            double p = orc_powi(psidn / w->psido, 2) * exp(2 * (psijn - w->psijo)) * exp((dfus2o - dfus2n) / (2 * tau)) *
+                       (fnormn / fnormo);
             if (d->age[iw] > 50) p = p * pow(1.1, d->age[iw] - 50);
             p = p < 1.0 ? p : (p >= 1.0 ? 1.0 : 0.0);
             dfus2ac = dfus2ac + p * dfus2o;
@@ -1992,7 +2153,7 @@ int orc_dmc_step(const champ_system_t *s, orc_dmc_t *d, double out[8])
         const double eprev = w->eold;
         double psid, psij, ekin, enew;
         orc_hpsi(s, w, &psid, &psij, &ekin, &enew);
-        const double fration = dmc_fratio(w, tau);
+        const double fration = s->nloc > 0 ? dmc_fratio(w, tau) : dmc_fratio_ae(s, w, tau);
         const double deo = d->eest - eprev, den = d->eest - enew;
         const double ewto = d->eest - deo * d->fratio[iw], ewtn = d->eest - den * fration;
         const double expon = (d->etrial - 0.5 * (ewto + ewtn)) * taunow;

@@ -1,5 +1,5 @@
-"""DMC on the GPU (champ_gpu_dmc_*: dmc.f90 with the dmc_ps mover, splitj,
-acues1/acuest at CHAMP's default controls).
+"""DMC on the GPU (champ_gpu_dmc_*: dmc.f90 with the dmc_ps mover, or dmc_ae
+for all-electron systems, splitj, acues1/acuest at CHAMP's default controls).
 
 * the population against the oracle's DMC step by step: walker counts after
   branching, the weight and energy sums, eest, eigv, fprod, node crossings,
@@ -39,13 +39,17 @@ def _etrial(sy, x):
     return float(np.mean(es))
 
 
-@pytest.mark.parametrize("name,icasula,valu", [("small", 0, False), ("small_md", 0, False), ("small", -1, False),
-                                               ("small_md", -1, False), ("small", -1, True)])
-def test_dmc_population_matches_oracle(require_gpu, name, icasula, valu):
+@pytest.mark.parametrize("name,icasula,valu,tau", [("small", 0, False, 0.05), ("small_md", 0, False, 0.05),
+                                                   ("small", -1, False, 0.05), ("small_md", -1, False, 0.05),
+                                                   ("small", -1, True, 0.05), ("c22", 0, False, 0.002),
+                                                   ("small_ae_md", 0, False, 0.01)])
+def test_dmc_population_matches_oracle(require_gpu, name, icasula, valu, tau):
     """valu: the quadrature on its vector-pipe form (the form systems without
-    an e-e-n term take), which keeps the T-move weights too."""
+    an e-e-n term take), which keeps the T-move weights too.  c22 and
+    small_ae_md are all-electron: the dmc_ae mover (its time steps keep the
+    first steps' weights, from unequilibrated walkers, within the capacity)."""
     sy = S.config(name)
-    cap, nconf, tau, nsteps = 24, 12, 0.05, 5
+    cap, nconf, nsteps = 24, 12, 5
     x0 = sy.initial_walkers(cap, seed=5)
     st = ffi.seeded_streams(S.seed_from_string(), cap + 1)
     etrial = _etrial(sy, x0[:nconf])
@@ -109,3 +113,43 @@ def test_dmc_energy_matches_reference_ci(require_gpu, icasula):
         assert abs(r["egave"] - ref["energy"]) <= 2.0 * np.hypot(r["egerr"], ref["error"])
         assert 0.9 < r["accept"] < 1.0 and 0.5 * nconf < r["nwalk"] < 2 * nconf
         eng.close()
+
+
+def test_dmc_all_electron_h2_does_not_depend_on_the_jastrow(require_gpu):
+    """All-electron DMC (dmc_ae) on the reference's H2 CI input
+    (tests/CI_test/VMC-H2): the ground state of H2 (one electron of each spin)
+    has no node, so fixed-node DMC is exact and its energy cannot depend on
+    the trial function, apart from the time-step error.  With the input's
+    Jastrow and with one whose e-e Pade denominator is changed by 1, the two
+    VMC energies differ by many standard errors, the two DMC energies agree
+    within three standard errors and 5e-4 Ha (time step 0.01, 10 a.u. of
+    projection before 20 a.u. of averaging), and both lie below their VMC
+    energies."""
+    import dataclasses
+
+    from tests.test_champ_io import load_case
+
+    sy0 = load_case("H2")
+    b1 = np.array(sy0.b, dtype=np.float64).copy()
+    b1[0, 1] += 1.0
+    res = []
+    for sy in (sy0, dataclasses.replace(sy0, b=b1)):
+        cap, nconf = 8192, 4096
+        eng = Engine(sy, cap, lib_path=ENGINE_LIB)
+        eng.set_walkers(sy.initial_walkers(cap, seed=3))
+        eng.seed_rng(S.seed_from_string())
+        eng.hpsi()
+        rv = eng.vmc(20, 10, nblkeq=5, tau=0.5)
+        pop = ffi.seeded_streams(S.seed_from_string(), cap + 1)[cap]
+        eng.dmc_init(nconf, 0.01, rv["efin"], pop)
+        r = eng.dmc(40, 50, nblkeq=20)
+        eng.close()
+        print(f"[dmc H2 all-electron] b2 {sy.b[0][1]:.4f}: VMC {rv['efin']:.5f} +- {rv['eerr']:.5f}, DMC {r['egave']:.5f} "
+              f"+- {r['egerr']:.5f}; acceptance {r['accept']:.4f}, walkers {r['nwalk']}, node crossings {r['nodecr']}",
+              flush=True)
+        assert r["egave"] < rv["efin"] - 5.0 * np.hypot(rv["eerr"], r["egerr"])
+        assert 0.99 < r["accept"] <= 1.0 and 0.5 * nconf < r["nwalk"] < 2 * nconf
+        res.append((rv, r))
+    (v0, d0), (v1, d1) = res
+    assert abs(v0["efin"] - v1["efin"]) > 5.0 * np.hypot(v0["eerr"], v1["eerr"])
+    assert abs(d0["egave"] - d1["egave"]) <= 3.0 * np.hypot(d0["egerr"], d1["egerr"]) + 5e-4
