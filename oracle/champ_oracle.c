@@ -43,6 +43,17 @@
  *                                    and drift vs a fresh evaluation; a reordered
  *                                    copy of the reference determinant gives
  *                                    (c0 +- c1) D_ref by the parity of the order.
+ *   metrop6 (m6_*)                -- gammai against scipy's incomplete gamma;
+ *                                    the radial normalisation against quadrature
+ *                                    of its density; VMC energies of the
+ *                                    reference's H2 and butadiene CI cases.
+ *   DMC (orc_dmc_*, dmc_tmove,     -- through the GPU engine, which follows it step
+ *        ae_propose/ae_reverse)      by step: the reference's DMC CI energy (with
+ *                                    and without T-moves) and, for the
+ *                                    all-electron mover, the nodeless H2 ground
+ *                                    state's DMC energy being the same for two
+ *                                    Jastrows.  The mixture normalisation of
+ *                                    ae_propose has no check of its own.
  * The end-to-end per-walker values of the Fortran reference itself are not
  * available (the Fortran build needs an MPI module this image lacks), so the
  * composition of the pinned pieces is what fixes hpsi/psie.
