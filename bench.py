@@ -301,8 +301,210 @@ def cpu_baseline(sy, args):
                       f"thread on {nth} threads, {tc:.1f} s"}
 
 
+def _kernel_table(kt):
+    tot = sum(ms for ms, n in kt.values() if n)
+    return {k: {"ms": round(ms, 3), "launches": n, "ms_per_launch": round(ms / n, 4), "share": round(ms / tot, 4)}
+            for k, (ms, n) in sorted(kt.items(), key=lambda kv: -kv[1][0]) if n}
+
+
+def _cpu_threads(args):
+    from oracle import ffi
+    model, ncpu, navail, _ = ffi.host_cpu()
+    return model, ncpu, navail, args.cpu_threads or min(16, navail)
+
+
+def bench_metrop6(args):
+    """metrop6 (CHAMP's default VMC mover) on the headline system: the same
+    population as the metrop1 headline, blocks of one sweep, FP64 and mixed;
+    the oracle's metrop6 sweeps on the host cores beside it."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from champ_amd import system as S
+    from champ_amd.engine import PREC_FP64, PREC_MIXED, Engine
+    from oracle import ffi
+
+    sy = S.config(args.system)
+    W = args.walkers
+    res = {}
+    for prec, label in ((PREC_FP64, "fp64"), (PREC_MIXED, "mixed")):
+        eng = Engine(sy, W, precision=prec)
+        eng.set_walkers(sy.initial_walkers(W, seed=1))
+        eng.seed_rng(S.seed_from_string())
+        eng.hpsi()
+        if args.autotune:
+            eng.autotune()
+        eng.metrop6(args.warmup)
+        eng.enable_timing(True)
+        t0 = time.perf_counter()
+        vr = eng.vmc(args.steps, 1, nblkeq=0, imetro=6)
+        wall = time.perf_counter() - t0
+        kt = eng.kernel_times()
+        eng.close()
+        res[label] = {"value": W * sy.nelec * args.steps / wall, "wall": wall, "vmc": vr, "kernels": _kernel_table(kt)}
+        print(f"[bench] metrop6 {label}: {args.steps} sweeps in {wall:.3f}s, acceptance {vr['accept']:.3f}", flush=True)
+    cpu = None
+    if not args.no_cpu:
+        model, ncpu, navail, nth = _cpu_threads(args)
+        cs = sy.as_ctypes()
+        x0 = sy.initial_walkers(nth, seed=1)
+        st = ffi.seeded_streams(S.seed_from_string(), nth)
+        walkers = []
+        for k in range(nth):
+            w = ffi.walker(sy, cs)
+            w.set_x(x0[k])
+            w.set_rng(st[k])
+            w.set_quad(sy.xq0, sy.yq0, sy.zq0)
+            w.hpsi()
+            walkers.append(w)
+        nsw = max(1, args.cpu_sweeps // 2)
+
+        def run(w):
+            for _ in range(nsw):
+                w.sweep6()
+                w.hpsi()
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(nth) as ex:
+            list(ex.map(run, walkers))
+        tc = time.perf_counter() - t0
+        cpu = {"value": round(nth * sy.nelec * nsw / tc, 2), "unit": "walker-moves/s", "cores": nth, "kind": "port",
+               "cpu_model": model, "nproc": ncpu, "cpus_available": navail,
+               "sample": f"{nth} walker chains x {nsw} metrop6 sweeps (each followed by hpsi), one chain per thread, "
+                         f"{tc:.1f} s"}
+    out = {"metric": "walker-moves/sec, metrop6 (CHAMP's default VMC mover)", "value": round(res["mixed"]["value"], 1),
+           "unit": "walker-moves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(1e3 * res["mixed"]["wall"] / args.steps, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": f"metrop6 sweep, {sy.nelec} e, {sy.ncent} nuclei, {sy.nbasis} AOs, {W} walkers",
+                      "walkers": W, "electrons": sy.nelec},
+           "fp64_value": round(res["fp64"]["value"], 1),
+           "mixed_over_fp64": round(res["mixed"]["value"] / res["fp64"]["value"], 3),
+           "acceptance": {k: round(v["vmc"]["accept"], 4) for k, v in res.items()},
+           "per_kernel": {k: v["kernels"] for k, v in res.items()}, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+def bench_dmc(args):
+    """DMC on the reference's own DMC CI case (butadiene, 500 determinants,
+    BFD, tau 0.05, T-moves as its input asks): walker-steps per second of the
+    population after equilibration, FP64 and mixed; the oracle's DMC on a few
+    walkers beside it."""
+    from champ_amd import champ_io as io
+    from champ_amd import system as S
+    from champ_amd.engine import PREC_FP64, PREC_MIXED, Engine
+    from oracle import ffi
+
+    sy = io.load_input(os.path.join(ROOT, "tests", "golden", "champ_inputs", "C4H6_dmc", "dmc.inp"), name="C4H6_dmc")
+    cap = args.walkers
+    nconf = cap // 2
+    res = {}
+    for prec, label in ((PREC_FP64, "fp64"), (PREC_MIXED, "mixed")):
+        eng = Engine(sy, cap, precision=prec)
+        eng.set_walkers(sy.initial_walkers(cap, seed=3))
+        eng.seed_rng(S.seed_from_string())
+        eng.hpsi()
+        eng.vmc(2, 10, nblkeq=0, tau=0.2)
+        pop = ffi.seeded_streams(S.seed_from_string(), cap + 1)[cap]
+        eng.dmc_init(nconf, 0.05, -26.3, pop)
+        eng.dmc_set_icasula(-1)
+        for _ in range(args.warmup):
+            eng.dmc_step()
+        eng.enable_timing(True)
+        nws = 0
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            nws += eng.dmc_step()["nwalk"]
+        wall = time.perf_counter() - t0
+        kt = eng.kernel_times()
+        eng.close()
+        res[label] = {"value": nws / wall, "wall": wall, "kernels": _kernel_table(kt), "walkers": nws / args.steps}
+        print(f"[bench] dmc {label}: {args.steps} steps in {wall:.3f}s, {nws / args.steps:.0f} walkers", flush=True)
+    cpu = None
+    if not args.no_cpu:
+        model, ncpu, navail, _ = _cpu_threads(args)
+        ncw, nst = 16, 30
+        x0 = sy.initial_walkers(ncw, seed=3)
+        st = ffi.seeded_streams(S.seed_from_string(), 2 * ncw + 1)
+        orc = ffi.Dmc(sy, np.concatenate([x0, x0]), st[:2 * ncw], ncw, 0.05, -26.3, st[2 * ncw], icasula=-1)
+        t0 = time.perf_counter()
+        nws = 0
+        for _ in range(nst):
+            nws += orc.step()["nwalk"]
+        tc = time.perf_counter() - t0
+        cpu = {"value": round(nws / tc, 3), "unit": "walker-steps/s", "cores": 1, "kind": "port",
+               "cpu_model": model, "nproc": ncpu, "cpus_available": navail,
+               "sample": f"the oracle's DMC (one thread), {ncw} walkers x {nst} steps of the same case, {tc:.1f} s"}
+    ne = sy.nelec
+    out = {"metric": "DMC walker-steps/sec (dmc_ps with T-moves), reference DMC CI case",
+           "value": round(res["mixed"]["value"], 1), "unit": "walker-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(1e3 * res["mixed"]["wall"] / args.steps, 2),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+           "data": "the reference's DMC-C4H6-ci1010_pVTZ-500-dets input; walkers from a short VMC",
+           "config": {"workload": f"DMC step, butadiene {ne} e, 500 determinants, BFD, tau 0.05, icasula -1, "
+                                  f"{nconf} target walkers", "capacity": cap, "electrons": ne},
+           "walker_moves_per_s": round(res["mixed"]["value"] * ne, 1),
+           "fp64_value": round(res["fp64"]["value"], 1),
+           "mixed_over_fp64": round(res["mixed"]["value"] / res["fp64"]["value"], 3),
+           "per_kernel": {k: v["kernels"] for k, v in res.items()}, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+def bench_sr(args):
+    """SR sampling on the reference's 5000-determinant optimisation case: the
+    derivatives of all 12033 parameters (13 Jastrow, 4999 determinant, 7021
+    orbital excitations), their map and the S, <O>, <E O> sums accumulated on
+    the GPU per sample, then the SR step (S z on the GPU)."""
+    from champ_amd import champ_io as io
+    from champ_amd import optwf
+    from champ_amd import system as S
+    from champ_amd.engine import DERIV_MAPPED, Engine
+
+    d = os.path.join(ROOT, "tests", "golden", "champ_inputs", "C4H6_5k")
+    sy = io.load_input(os.path.join(d, "vmc_optimization_5000.inp"), name="C4H6_5k", nadorb=280)
+    der = io.read_jastrow_der(os.path.join(d, "jastrow.der"), sy.nctype, sy.nspin2b)
+    irrep = io.read_symmetry(os.path.join(d, "ci1010_pVTZ_1_symmetry.sym"), sy.coef.shape[0])
+    terms = optwf.optorb_define(sy.iworbd, sy.nup, irrep, nadorb=280)
+    p = optwf.ChampParams.build(sy, der, ci=True, orb_terms=terms)
+    W = args.walkers
+    eng = Engine(sy, W)
+    eng.set_walkers(sy.initial_walkers(W, seed=3))
+    eng.seed_rng(S.seed_from_string())
+    eng.hpsi()
+    eng.set_orb_terms(terms)
+    eng.set_deriv_map(p.kinds, p.m)
+    eng.metrop6(args.warmup)
+    eng.sr_accumulate(DERIV_MAPPED, reset=True)
+    eng.enable_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.sr_accumulate(DERIV_MAPPED)
+    ts = time.perf_counter() - t0
+    kt = eng.kernel_times()
+    t0 = time.perf_counter()
+    dparm, info = eng.sr_solve(sr_tau=0.025, sr_eps=1e-3, sr_adiag=0.01)
+    tsol = time.perf_counter() - t0
+    eng.close()
+    print(f"[bench] sr: {args.steps} samples in {ts:.3f}s; solve {tsol:.3f}s ({info['cg_iters']} CG iterations)",
+          flush=True)
+    out = {"metric": "SR samples/sec (derivatives of every parameter + S, <O>, <E O> sums), "
+                     "5000-determinant optimisation case",
+           "value": round(W * args.steps / ts, 1), "unit": "walker-samples/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(1e3 * ts / args.steps, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "the reference's VMC-C4H6-ci1010_pVTZ-05000-dets input",
+           "config": {"workload": f"SR sample of {W} walkers, {p.nparm} parameters", "walkers": W,
+                      "parameters": p.nparm},
+           "solve_s": round(tsol, 3), "cg_iters": info["cg_iters"], "per_kernel": _kernel_table(kt),
+           "cpu_baseline": None,
+           "cpu_baseline_note": "the oracle has no derivative path (its checks are finite differences of ln psi, "
+                                "2 nparm evaluations per walker)"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="metrop1", choices=["metrop1", "metrop6", "dmc", "sr"],
+                    help="metrop1: the headline; the others measure the VMC default mover, DMC and SR sampling")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=10,
@@ -327,6 +529,10 @@ def main():
         print("bench.py: this engine runs the walker population on one GPU; "
               "multi-GPU runs (walker sharding, see DESIGN.md) are not part of this build.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "metrop1":
+        {"metrop6": bench_metrop6, "dmc": bench_dmc, "sr": bench_sr}[args.workload](args)
+        return
 
     from champ_amd import system as S
     from champ_amd.engine import PREC_FP64, PREC_MIXED
