@@ -77,17 +77,26 @@ static void sr_free(champ_ctx *c)
     c->sr_wsum = c->sr_esum = 0.0;
 }
 
+static void map_free(champ_ctx *c)
+{
+    if (c->der_map) hipFree(c->der_map);
+    if (c->der_src) hipFree(c->der_src);
+    if (c->der_dt) hipFree(c->der_dt);
+    c->der_map = c->der_dt = nullptr;
+    c->der_src = nullptr;
+    c->der_nd = c->der_dt_cap = 0;
+    c->der_map_kinds = c->der_map_nraw = c->der_map_nout = 0;
+}
+
 void deriv_free(champ_ctx *c)
 {
     deriv_free_scratch(c);
     sr_free(c);
-    if (c->der_map) hipFree(c->der_map);
+    map_free(c);
     if (c->der_orb) hipFree(c->der_orb);
     if (c->der_occ) hipFree(c->der_occ);
     c->der_orb = c->der_occ = nullptr;
     c->der_norb = 0;
-    c->der_map = nullptr;
-    c->der_map_kinds = c->der_map_nraw = c->der_map_nout = 0;
 }
 
 // d/dp of the e-n (e-e) form f(r) = q0 r / (1 + q1 r) + sum_{p>=2} q_p r^p
@@ -280,6 +289,44 @@ __global__ void k_sr_add(size_t n, const double *__restrict__ src, double *__res
         dst[i] += src[i];
 }
 
+// O' [nout][W] of the map: row j copies raw row src[j], or, for src[j] < 0,
+// row -1 - src[j] of the dense columns' product dt
+__global__ void k_deriv_assemble(int nout, int W, const int *__restrict__ src, const double *__restrict__ ot,
+                                 const double *__restrict__ dt, double *__restrict__ om)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nout * W) return;
+    const int j = (int)(t / W), w = (int)(t % W), sj = src[j];
+    om[t] = sj >= 0 ? ot[(size_t)sj * W + w] : dt[(size_t)(-1 - sj) * W + w];
+}
+
+// dst += src on the lower triangle (j <= i) of n x n: row i = blockIdx.y
+__global__ void k_sr_add_lower(int n, const double *__restrict__ src, double *__restrict__ dst)
+{
+    const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > i || j >= n) return;
+    dst[(size_t)i * n + j] += src[(size_t)i * n + j];
+}
+
+// S[i][j] = S[j][i] for j > i, through a 32 x 32 LDS tile: block (bx, by)
+// with bx >= by writes the upper tile (by, bx) from the lower tile (bx, by)
+__global__ void __launch_bounds__(256) k_sr_symm(int n, double *S)
+{
+    __shared__ double tile[32][33];
+    const int bx = blockIdx.x, by = blockIdx.y;
+    if (bx < by) return;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+    for (int r = ty; r < 32; r += 8) {
+        const int gi = bx * 32 + r, gj = by * 32 + tx;
+        tile[r][tx] = (gi < n && gj < n) ? S[(size_t)gi * n + gj] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int gi = by * 32 + r, gj = bx * 32 + tx;  // upper element (gi, gj) = lower (gj, gi)
+        if (gi < n && gj < n && gj > gi) S[(size_t)gi * n + gj] = tile[tx][r];
+    }
+}
+
 // t = S z for the accumulated S [n][n] (symmetric): a wavefront per row, the
 // row read once with consecutive lanes on consecutive columns
 __global__ void __launch_bounds__(256) k_sr_gemv(int n, const double *__restrict__ S, const double *__restrict__ z,
@@ -319,7 +366,7 @@ static int grow(T **p, int *cap, int need, size_t per)
 static int deriv_compute(champ_ctx *c, int kinds, const double **ot, int *n)
 {
     const bool mapped = (kinds & CHAMP_DERIV_MAPPED) != 0;
-    if (mapped && !c->der_map) { set_error("deriv: CHAMP_DERIV_MAPPED without a parameter map"); return CHAMP_ERR_ARG; }
+    if (mapped && !c->der_map_nout) { set_error("deriv: CHAMP_DERIV_MAPPED without a parameter map"); return CHAMP_ERR_ARG; }
     const int raw = mapped ? c->der_map_kinds : kinds;
     DerLayout L = der_layout(c, raw);
     if (L.n == 0) { set_error("deriv: no parameters selected (kinds) for this trial function"); return CHAMP_ERR_ARG; }
@@ -374,9 +421,18 @@ static int deriv_compute(champ_ctx *c, int kinds, const double **ot, int *n)
         }
     }
     if (mapped) {
+        // dense columns on the matrix cores, unit columns copied
+        if (c->der_nd > 0) {
+            if ((rc = grow(&c->der_dt, &c->der_dt_cap, c->der_nd, (size_t)c->W * sizeof(double)))) return rc;
+            if ((rc = launch_gemm(c, false, c->der_nd, c->W, L.n, c->der_map, L.n, 0, c->der_ot, c->W, 0, c->der_dt,
+                                  c->W, 0, 1)))
+                return rc;
+        }
         if ((rc = grow(&c->der_om, &c->der_om_cap, nout, (size_t)c->W * sizeof(double)))) return rc;
-        if ((rc = launch_gemm(c, false, nout, c->W, L.n, c->der_map, L.n, 0, c->der_ot, c->W, 0, c->der_om, c->W, 0, 1)))
-            return rc;
+        const long long nt = (long long)nout * c->W;
+        hipLaunchKernelGGL(k_deriv_assemble, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, nout, c->W,
+                           c->der_src, c->der_ot, c->der_dt, c->der_om);
+        HIPCHECK(hipGetLastError());
         *ot = c->der_om;
     } else {
         *ot = c->der_ot;
@@ -499,7 +555,7 @@ extern "C" int champ_gpu_deriv_count(champ_ctx *c, int32_t kinds, int32_t *nparm
 {
     if (!c || !nparm) return CHAMP_ERR_ARG;
     if (kinds & CHAMP_DERIV_MAPPED) {
-        if (!c->der_map) { set_error("deriv: CHAMP_DERIV_MAPPED without a parameter map"); return CHAMP_ERR_ARG; }
+        if (!c->der_map_nout) { set_error("deriv: CHAMP_DERIV_MAPPED without a parameter map"); return CHAMP_ERR_ARG; }
         *nparm = c->der_map_nout;
     } else {
         *nparm = der_layout(c, kinds).n;
@@ -510,9 +566,7 @@ extern "C" int champ_gpu_deriv_count(champ_ctx *c, int32_t kinds, int32_t *nparm
 extern "C" int champ_gpu_set_deriv_map(champ_ctx *c, int32_t kinds, int32_t nout, const double *map)
 {
     if (!c) return CHAMP_ERR_ARG;
-    if (c->der_map) hipFree(c->der_map);
-    c->der_map = nullptr;
-    c->der_map_kinds = c->der_map_nraw = c->der_map_nout = 0;
+    map_free(c);
     if (!map || nout == 0) return CHAMP_OK;
     const int raw_bits = CHAMP_DERIV_JAS_EN | CHAMP_DERIV_JAS_EE | CHAMP_DERIV_JAS_EEN | CHAMP_DERIV_CI | CHAMP_DERIV_ORB;
     if (nout < 0 || (kinds & ~raw_bits)) {
@@ -521,14 +575,33 @@ extern "C" int champ_gpu_set_deriv_map(champ_ctx *c, int32_t kinds, int32_t nout
     }
     const int nraw = der_layout(c, kinds).n;
     if (nraw == 0) { set_error("set_deriv_map: no parameters selected (kinds) for this trial function"); return CHAMP_ERR_ARG; }
-    std::vector<double> mt((size_t)nout * nraw);
-    constexpr int TB = 64;  // transposed in tiles, so both sides stay in cache lines
-    for (int k0 = 0; k0 < nraw; k0 += TB)
-        for (int j0 = 0; j0 < nout; j0 += TB)
-            for (int k = k0; k < std::min(k0 + TB, nraw); k++)
-                for (int j = j0; j < std::min(j0 + TB, nout); j++) mt[(size_t)j * nraw + k] = map[(size_t)k * nout + j];
-    HIPCHECK(hipMalloc(&c->der_map, mt.size() * sizeof(double)));
-    HIPCHECK(hipMemcpy(c->der_map, mt.data(), mt.size() * sizeof(double), hipMemcpyHostToDevice));
+    // a column with a single non-zero equal to 1 copies that raw parameter;
+    // the others keep their coefficients, transposed ([nd][nraw])
+    std::vector<int> nnz(nout, 0), one(nout, -1);
+    for (int k = 0; k < nraw; k++) {
+        const double *row = map + (size_t)k * nout;
+        for (int j = 0; j < nout; j++)
+            if (row[j] != 0.0) {
+                nnz[j]++;
+                one[j] = row[j] == 1.0 ? k : -2;
+            }
+    }
+    std::vector<int> src(nout);
+    int nd = 0;
+    for (int j = 0; j < nout; j++) src[j] = (nnz[j] == 1 && one[j] >= 0) ? one[j] : -1 - nd++;
+    std::vector<double> mt((size_t)nd * nraw);
+    for (int k = 0; k < nraw; k++) {
+        const double *row = map + (size_t)k * nout;
+        for (int j = 0; j < nout; j++)
+            if (src[j] < 0) mt[(size_t)(-1 - src[j]) * nraw + k] = row[j];
+    }
+    HIPCHECK(hipMalloc(&c->der_src, (size_t)nout * sizeof(int)));
+    HIPCHECK(hipMemcpy(c->der_src, src.data(), (size_t)nout * sizeof(int), hipMemcpyHostToDevice));
+    if (nd > 0) {
+        HIPCHECK(hipMalloc(&c->der_map, mt.size() * sizeof(double)));
+        HIPCHECK(hipMemcpy(c->der_map, mt.data(), mt.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    c->der_nd = nd;
     c->der_map_kinds = kinds;
     c->der_map_nraw = nraw;
     c->der_map_nout = nout;
@@ -558,11 +631,7 @@ extern "C" int champ_gpu_set_orb_terms(champ_ctx *c, int32_t nterm, const int32_
     HIPCHECK(hipMemcpy(c->der_occ, occ.data(), occ.size() * sizeof(int), hipMemcpyHostToDevice));
     c->der_norb = nterm;
     // a map over the previous raw layout no longer fits
-    if (c->der_map && (c->der_map_kinds & CHAMP_DERIV_ORB)) {
-        hipFree(c->der_map);
-        c->der_map = nullptr;
-        c->der_map_kinds = c->der_map_nraw = c->der_map_nout = 0;
-    }
+    if (c->der_map_nout && (c->der_map_kinds & CHAMP_DERIV_ORB)) map_free(c);
     return CHAMP_OK;
 }
 
@@ -643,6 +712,7 @@ extern "C" int champ_gpu_sr_accumulate(champ_ctx *c, int32_t kinds, int32_t rese
     int n = 0;
     int rc = deriv_compute(c, kinds, &ot, &n);
     if (rc) return rc;
+    if (n > 65535) { set_error("sr_accumulate: more than 65535 parameters"); return CHAMP_ERR_UNSUPPORTED; }
     if (!reset && c->sr_acc && (c->sr_n != n || c->sr_kinds != kinds)) {
         set_error("sr_accumulate: the parameters differ from the accumulated ones (start again with reset)");
         return CHAMP_ERR_ARG;
@@ -661,8 +731,9 @@ extern "C" int champ_gpu_sr_accumulate(champ_ctx *c, int32_t kinds, int32_t rese
         c->sr_wsum = c->sr_esum = 0.0;
     }
     const int W = c->W;
-    if ((rc = launch_gemm(c, true, n, n, W, ot, W, 0, ot, W, 0, c->der_s, n, 0, 1))) return rc;
-    hipLaunchKernelGGL(k_sr_add, dim3(4096), dim3(256), 0, c->stream, (size_t)n * n, c->der_s, c->sr_acc);
+    // S is symmetric: its lower triangle here, the upper one mirrored by champ_gpu_sr_solve
+    if ((rc = launch_syrk_lower(c, n, W, ot, W, c->der_s, n))) return rc;
+    hipLaunchKernelGGL(k_sr_add_lower, dim3((n + 255) / 256, n), dim3(256), 0, c->stream, n, c->der_s, c->sr_acc);
     HIPCHECK(hipGetLastError());
     hipLaunchKernelGGL(k_deriv_sums, dim3((n + 63) / 64), dim3(64), 0, c->stream, n, W, ot, c->eold, c->der_v);
     HIPCHECK(hipGetLastError());
@@ -692,6 +763,9 @@ extern "C" int champ_gpu_sr_solve(champ_ctx *c, double sr_tau, double sr_eps, do
     A.n = n;
     A.o.resize(n);
     std::vector<double> v((size_t)2 * n), eo(n), diag(n);
+    const int nt = (n + 31) / 32;
+    hipLaunchKernelGGL(k_sr_symm, dim3(nt, nt), dim3(256), 0, c->stream, n, c->sr_acc);
+    HIPCHECK(hipGetLastError());
     hipLaunchKernelGGL(k_sr_diag, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->sr_acc, c->sr_z);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(diag.data(), c->sr_z, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));

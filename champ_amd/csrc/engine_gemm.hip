@@ -237,13 +237,17 @@ int launch_gemm_f32(champ_ctx *c, int M, int N, int K, const float *A, int lda, 
 // value it loads feeds WN instructions; the block's four wavefronts share the
 // B chunk of 16 k x (16 WN) columns through LDS (double-buffered, one barrier
 // per chunk).  Same k re-pairing and accumulator layout as k_gemm_mfma.
-template <bool BT, int WN>
+// LOWER: blocks entirely above the diagonal (first column beyond the last
+// row) return at once, for symmetric products of which the lower triangle
+// is kept; blocks across the diagonal write their whole tile.
+template <bool BT, int WN, bool LOWER = false>
 __global__ void __launch_bounds__(256) k_gemm_mfma_strip(int M, int N, int K, const double *__restrict__ A, int lda,
                                                          long long sA, const double *__restrict__ B, int ldb,
                                                          long long sB, double *__restrict__ Cm, int ldc, long long sC)
 {
     constexpr int BN = 16 * WN;
     __shared__ double Bs[2][16][BN + 1];
+    if (LOWER && (long long)blockIdx.y * BN > (long long)blockIdx.x * 64 + 63) return;  // the whole block
     const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
     const int m0 = blockIdx.x * 64 + wv * 16, n0 = blockIdx.y * BN;
     A += (long long)blockIdx.z * sA;
@@ -293,6 +297,23 @@ __global__ void __launch_bounds__(256) k_gemm_mfma_strip(int M, int N, int K, co
             if (ro < M) Cm[(long long)ro * ldc + col] = acc[t][r];
         }
     }
+}
+
+// The lower triangle (and the tiles across the diagonal) of C = A A^T, C
+// (n x n, ldc), A (n x K, lda): the strip kernel without its upper blocks.
+int launch_syrk_lower(champ_ctx *c, int n, int K, const double *A, int lda, double *Cm, int ldc)
+{
+    if (n <= 0) return CHAMP_OK;
+    constexpr int WN = 5, BN = 16 * WN;
+    const long long gx = (n + 63) / 64, gy = (n + BN - 1) / BN;
+    if (gx > 0x7fffffffLL || gy > 65535) {
+        set_error("gemm: problem too large for one launch");
+        return CHAMP_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL((k_gemm_mfma_strip<true, WN, true>), dim3((unsigned)gx, (unsigned)gy, 1), dim3(256), 0, c->stream,
+                       n, n, K, A, lda, 0LL, A, lda, 0LL, Cm, ldc, 0LL);
+    HIPCHECK(hipGetLastError());
+    return CHAMP_OK;
 }
 
 // C[b] (M x N, row-major, ldc) = A[b] (M x K, lda) * op(B[b]); op = B^T when

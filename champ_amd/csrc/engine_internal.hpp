@@ -127,6 +127,12 @@ struct champ_ctx {
     // parameter map (champ_gpu_set_deriv_map): Mt [nout][nraw] over the raw
     // parameters of der_map_kinds, and the mapped O [nout][W]
     double *der_map = nullptr, *der_om = nullptr;
+    // the map's unit columns are copies: der_src [nout] holds the raw
+    // parameter a column copies, or -(1 + d) for the d-th dense column, whose
+    // coefficients der_map holds ([der_nd][nraw]); der_dt [der_nd][W]
+    int *der_src = nullptr;
+    double *der_dt = nullptr;
+    int der_nd = 0, der_dt_cap = 0;
     // orbital terms (champ_gpu_set_orb_terms): [nterm][2] (io, jo) and the slot
     // of each orbital in the reference determinant per spin [2][NO] (-1: empty)
     int *der_orb = nullptr, *der_occ = nullptr;
@@ -223,6 +229,7 @@ int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int
                 int ldb, long long sB, double *Cm, int ldc, long long sC, int batch, int kind = GEMM_AUTO);
 int launch_wmat(champ_ctx *c);
 // FP32 A [M][K] times FP32 B^T (B stored [N][K]) into FP64 C (matrix cores)
+int launch_syrk_lower(champ_ctx *c, int n, int K, const double *A, int lda, double *Cm, int ldc);
 int launch_gemm_f32(champ_ctx *c, int M, int N, int K, const float *A, int lda, const float *B, int ldb, double *Cm,
                     int ldc);
 void dmc_free(champ_ctx *c);
