@@ -42,7 +42,7 @@ namespace champ {
 constexpr int MD_MAXR = 4;  // excitation rank per spin handled by the closed-form cofactors
 
 struct MdArgs {
-    int W, NE, NUP, NM, NO, NV, NU, ndet, w0, iel, mode, pad;
+    int W, NE, NUP, NM, NO, NV, NU, ndet, w0, iel, mode, lds_aa;  // lds_aa: the per-move kernels read aa from LDS
     int nu[2], nv[2];
     const int *du, *dd;
     const double *cdet;
@@ -60,7 +60,7 @@ static MdArgs md_args(champ_ctx *c, int iel)
 {
     MdArgs m;
     m.W = c->W; m.NE = c->NE; m.NUP = c->NUP; m.NM = c->NM; m.NO = c->NO; m.NV = c->md_NV; m.NU = c->md_NU;
-    m.ndet = c->md_ndet; m.w0 = 0; m.iel = iel; m.mode = 0; m.pad = 0;
+    m.ndet = c->md_ndet; m.w0 = 0; m.iel = iel; m.mode = 0; m.lds_aa = 0;
     for (int s = 0; s < 2; s++) {
         m.nu[s] = c->md_nu[s]; m.nv[s] = c->md_nv[s];
         m.rank[s] = c->md_rank[s]; m.hole[s] = c->md_hole[s]; m.part[s] = c->md_part[s]; m.virt[s] = c->md_virt[s];
@@ -413,9 +413,13 @@ __global__ void __launch_bounds__(256) k_md_pre(MdArgs m)
     const int NE = m.NE, NO = m.NO, NV = m.NV, NM = m.NM, iel = m.iel;
     const int s = iel < m.NUP ? 0 : 1, nel = s ? NE - m.NUP : m.NUP, ish = s ? m.NUP : 0, i = iel - ish;
     const int b = m.cur[2 * w + s], bo = m.cur[2 * w + 1 - s], nv = m.nv[s];
-    (void)nel;
+    HIP_DYNAMIC_SHARED(double, sAA)
     const double *A = m.slmi + ((size_t)w * 2 + s) * NM * NM;
     const double *aa = md_aa(m, w, s, b);
+    if (m.lds_aa) {  // the nel rows of aa, gathered below once per unique determinant
+        for (int t = tid; t < nel * NV; t += bd) sAA[t] = aa[t];
+        aa = sAA;  // md_weights ends with a barrier
+    }
     md_weights(m, w, s, md_D(m, w, 1 - s, bo), m.S[w]);
     double *z = m.Z + (size_t)w * NV * NM;
     md_tb(m, w, s, aa, A, i, nullptr);
@@ -470,17 +474,22 @@ __global__ void __launch_bounds__(256) k_md_post(MdArgs m)
     __syncthreads();
     const double rinv = 1.0 / R0;
     // elements t = tid, tid + bd, ... of the nel x nv block, (j, v) stepped
-    // incrementally (a spin without particle orbitals has no block)
+    // incrementally (a spin without particle orbitals has no block); with
+    // lds_aa also into LDS, where the gathers below read it
+    HIP_DYNAMIC_SHARED(double, sAA)
     if (nv > 0) {
         for (int j = tid / nv, v = tid % nv; j < nel;) {
-            aan[(size_t)j * NV + v] = aa[(size_t)j * NV + v] + A[(size_t)i * NM + j] * rinv * gw[v];
+            const double x = aa[(size_t)j * NV + v] + A[(size_t)i * NM + j] * rinv * gw[v];
+            aan[(size_t)j * NV + v] = x;
+            if (m.lds_aa) sAA[(size_t)j * NV + v] = x;
             j += bd / nv; v += bd % nv;
             if (v >= nv) { v -= nv; j++; }
         }
     }
     __syncthreads();
+    const double *aar = m.lds_aa ? sAA : aan;
     // the spin's determinants and t_u from the same cofactors of aa'
-    md_tb(m, w, s, aan, A, i, md_D(m, w, s, bn));
+    md_tb(m, w, s, aar, A, i, md_D(m, w, s, bn));
     const double *Dn = md_D(m, w, s, bn), *Do = md_D(m, w, 1 - s, bo);
     const double Sn = s == 0 ? md_sum(m, Dn, Do, red) : md_sum(m, Do, Dn, red);
     if (tid == 0) m.Sn[w] = Sn;
@@ -496,7 +505,7 @@ __global__ void __launch_bounds__(256) k_md_post(MdArgs m)
         double x = 0;
         if (j >= 0) {
             double c = 0;
-            for (int vv = 0; vv < nv; vv++) c += aan[(size_t)j * NV + vv] * z[vv];
+            for (int vv = 0; vv < nv; vv++) c += aar[(size_t)j * NV + vv] * z[vv];
             x = A[(size_t)i * NM + j] * rinv - c;
         } else if (v >= 0) {
             x = z[v];
@@ -517,10 +526,23 @@ int launch_md_hpsi(champ_ctx *c, int w0, int nw)
     return CHAMP_OK;
 }
 
+// The per-move kernels stage the moved spin's aa (nel x NV doubles) in LDS
+// when it fits in 64 KiB (two or more blocks per CU); else they read it from HBM.
+static MdArgs md_move_args(champ_ctx *c, int iel, size_t *lds)
+{
+    MdArgs m = md_args(c, iel);
+    const size_t bytes = (size_t)c->NM * c->md_NV * sizeof(double);
+    m.lds_aa = bytes > 0 && bytes <= 64 * 1024;
+    *lds = m.lds_aa ? bytes : 0;
+    return m;
+}
+
 int launch_md_pre(champ_ctx *c, int iel)
 {
     if (!c->md) return CHAMP_OK;
-    hipLaunchKernelGGL(k_md_pre, dim3(c->W), dim3(256), 0, c->stream, md_args(c, iel));
+    size_t lds;
+    const MdArgs m = md_move_args(c, iel, &lds);
+    hipLaunchKernelGGL(k_md_pre, dim3(c->W), dim3(256), lds, c->stream, m);
     HIPCHECK(hipGetLastError());
     return CHAMP_OK;
 }
@@ -528,7 +550,9 @@ int launch_md_pre(champ_ctx *c, int iel)
 int launch_md_post(champ_ctx *c, int iel)
 {
     if (!c->md) return CHAMP_OK;
-    hipLaunchKernelGGL(k_md_post, dim3(c->W), dim3(256), 0, c->stream, md_args(c, iel));
+    size_t lds;
+    const MdArgs m = md_move_args(c, iel, &lds);
+    hipLaunchKernelGGL(k_md_post, dim3(c->W), dim3(256), lds, c->stream, m);
     HIPCHECK(hipGetLastError());
     return CHAMP_OK;
 }
