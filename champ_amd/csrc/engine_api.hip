@@ -442,7 +442,7 @@ extern "C" int champ_gpu_free(champ_ctx *c)
                     c->esum_dev, c->aoall, c->orball, c->wmat, c->pair_count, c->pair_off, c->pair_total, c->grp_off,
                     c->pair_e, c->pair_c, c->pair_vps, c->pair_val, c->pair_psi, c->rrT, c->nxquad, c->ri_cent, c->ri_type,
                     c->ri_rad, c->ri_rmax, c->ao_ri, c->gref, c->lu_ws, c->m6rec, c->ri_perm,
-                    c->ri_aoff, c->ri_aol, c->enp};
+                    c->ri_aoff, c->ri_aol, c->enp, c->gemm_tmp};
     for (void *p : ptrs) if (p) hipFree(p);
     md_free(c);
     dmc_free(c);
@@ -835,9 +835,14 @@ extern "C" int champ_gpu_autotune(champ_ctx *c, int32_t *choices)
     for (int s = 0; s < nsite && !rc; s++) {
         float best = 1e30f;
         int pick = GEMM_AUTO;
-        // the mixed path's per-move contraction may also run on FP32 operands
-        const int nk = (s == 0 && c->prec == CHAMP_PREC_MIXED) ? 4 : 3;
-        const int kinds[4] = {GEMM_TILE, GEMM_STRIP, GEMM_VALU, GEMM_F32};
+        // the per-move contraction may also run split over K, and on the mixed
+        // path on FP32 operands
+        int kinds[5] = {GEMM_TILE, GEMM_STRIP, GEMM_VALU}, nk = 3;
+        if (s == 0) {
+            if ((rc = gemm_reserve_splitk(c, 4 * c->W, c->NO))) break;
+            kinds[nk++] = GEMM_SPLITK;
+            if (c->prec == CHAMP_PREC_MIXED) kinds[nk++] = GEMM_F32;
+        }
         for (int ik = 0; ik < nk; ik++) {
             c->tune_gemm[s] = kinds[ik];
             float ms = 0;
@@ -874,7 +879,7 @@ extern "C" int champ_gpu_set_tuning(champ_ctx *c, const int32_t *choices)
     for (int s = 0; s < 3; s++) {
         const int k = choices[s];
         const bool f32_ok = s == 0 && c->prec == CHAMP_PREC_MIXED;
-        if (k < GEMM_AUTO || k > GEMM_F32 || (k == GEMM_F32 && !f32_ok)) {
+        if (k < GEMM_AUTO || k > GEMM_SPLITK || (k == GEMM_F32 && !f32_ok) || (k == GEMM_SPLITK && s != 0)) {
             set_error("set_tuning: GEMM kind " + std::to_string(k) + " not available at site " + std::to_string(s));
             return CHAMP_ERR_ARG;
         }
@@ -883,6 +888,10 @@ extern "C" int champ_gpu_set_tuning(champ_ctx *c, const int32_t *choices)
     if (c->nloc == 0 && (choices[2] != GEMM_AUTO || choices[3] != -1)) {
         set_error("set_tuning: no ECP sites on an all-electron system");
         return CHAMP_ERR_ARG;
+    }
+    if (choices[0] == GEMM_SPLITK) {
+        const int rc = gemm_reserve_splitk(c, 4 * c->W, c->NO);
+        if (rc) return rc;
     }
     for (int s = 0; s < 3; s++) c->tune_gemm[s] = choices[s];
     c->tune_nonloc = choices[3];

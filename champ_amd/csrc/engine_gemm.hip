@@ -299,6 +299,29 @@ __global__ void __launch_bounds__(256) k_gemm_mfma_strip(int M, int N, int K, co
     }
 }
 
+// Cm[i][j] = t[i][j] + t[M + i][j]: the two split-K partials (t ld N)
+__global__ void k_gemm_add_halves(int M, int N, const double *__restrict__ t, double *__restrict__ Cm, int ldc)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)M * N) return;
+    const int i = (int)(e / N), j = (int)(e % N);
+    Cm[(long long)i * ldc + j] = t[e] + t[(long long)M * N + e];
+}
+
+// Scratch for a split-K product of M x N (reserved before any launch that may
+// be captured in a graph; launch_gemm falls back to the tile form without it).
+int gemm_reserve_splitk(champ_ctx *c, int M, int N)
+{
+    const size_t need = (size_t)2 * M * N * sizeof(double);
+    if (need <= c->gemm_tmp_cap) return CHAMP_OK;
+    if (c->gemm_tmp) HIPCHECK(hipFree(c->gemm_tmp));
+    c->gemm_tmp = nullptr;
+    c->gemm_tmp_cap = 0;
+    HIPCHECK(hipMalloc(&c->gemm_tmp, need));
+    c->gemm_tmp_cap = need;
+    return CHAMP_OK;
+}
+
 // The lower triangle (and the tiles across the diagonal) of C = A A^T, C
 // (n x n, ldc), A (n x K, lda): the strip kernel without its upper blocks.
 int launch_syrk_lower(champ_ctx *c, int n, int K, const double *A, int lda, double *Cm, int ldc)
@@ -322,6 +345,34 @@ int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int
                 int ldb, long long sB, double *Cm, int ldc, long long sC, int batch, int kind)
 {
     if (M <= 0 || N <= 0 || batch <= 0) return CHAMP_OK;
+    if (kind == GEMM_SPLITK) {
+        // the tile form on k in [0, K/2) and [K/2, K) as a batch of two, so twice
+        // as many wavefronts share the work; the partials are then added in a
+        // fixed order (reproducible, no atomics)
+        constexpr int TWN = CHAMP_GEMM_TILE_WN;
+        const long long waves = (long long)((M + 15) / 16) * ((N + 16 * TWN - 1) / (16 * TWN));
+        const long long blocks = (waves + 3) / 4;
+        const bool ok = CHAMP_GEMM_MFMA && batch == 1 && K >= 32 && K % 2 == 0 && blocks <= 0x7fffffffLL &&
+                        c->gemm_tmp && c->gemm_tmp_cap >= (size_t)2 * M * N * sizeof(double);
+        if (ok) {
+            const int Kh = K / 2;
+            const long long sBh = bt ? (long long)Kh : (long long)Kh * ldb;
+            dim3 grid((unsigned)blocks, 2);
+            if (bt)
+                hipLaunchKernelGGL((k_gemm_mfma<true, TWN>), grid, dim3(256), 0, c->stream, M, N, Kh, A, lda, (long long)Kh,
+                                   B, ldb, sBh, c->gemm_tmp, N, (long long)M * N);
+            else
+                hipLaunchKernelGGL((k_gemm_mfma<false, TWN>), grid, dim3(256), 0, c->stream, M, N, Kh, A, lda,
+                                   (long long)Kh, B, ldb, sBh, c->gemm_tmp, N, (long long)M * N);
+            HIPCHECK(hipGetLastError());
+            const long long ne = (long long)M * N;
+            hipLaunchKernelGGL(k_gemm_add_halves, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, c->stream, M, N,
+                               c->gemm_tmp, Cm, ldc);
+            HIPCHECK(hipGetLastError());
+            return CHAMP_OK;
+        }
+        kind = GEMM_TILE;
+    }
     if (!CHAMP_GEMM_MFMA || kind == GEMM_VALU) {
         if (batch > 65535) {
             set_error("gemm: batch too large for one launch");

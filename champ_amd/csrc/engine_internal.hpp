@@ -141,6 +141,9 @@ struct champ_ctx {
     // SR sums kept on the device (champ_gpu_sr_accumulate): S [n][n], [O | E O]
     // [2n], the conjugate-gradient vectors [2n]; weight and E_L totals
     double *sr_acc = nullptr, *sr_accv = nullptr, *sr_z = nullptr;
+    // scratch of the split-K GEMM: two M x N partials (gemm_reserve_splitk)
+    double *gemm_tmp = nullptr;
+    size_t gemm_tmp_cap = 0;
     int sr_n = 0, sr_kinds = 0;
     double sr_wsum = 0.0, sr_esum = 0.0;
     unsigned char *acc = nullptr;
@@ -223,8 +226,11 @@ int launch_slater_refresh(champ_ctx *c);
 int launch_rotqua(champ_ctx *c);
 int launch_basis_points(champ_ctx *c, int npt, const double *dpos, double *dout);
 // kind: GEMM_AUTO (shape rule), GEMM_TILE (16 x 16 per wavefront), GEMM_STRIP
-// (16 x 80 per wavefront, B chunks in LDS) or GEMM_VALU
-enum { GEMM_AUTO = -1, GEMM_TILE = 0, GEMM_STRIP = 1, GEMM_VALU = 2, GEMM_F32 = 3 };
+// (16 x 80 per wavefront, B chunks in LDS), GEMM_VALU, GEMM_F32 (the mixed
+// path's FP32 move contraction) or GEMM_SPLITK (the tile form over the two
+// halves of K into a reserved scratch, then their sum in a fixed order)
+enum { GEMM_AUTO = -1, GEMM_TILE = 0, GEMM_STRIP = 1, GEMM_VALU = 2, GEMM_F32 = 3, GEMM_SPLITK = 4 };
+int gemm_reserve_splitk(champ_ctx *c, int M, int N);
 int launch_gemm(champ_ctx *c, bool bt, int M, int N, int K, const double *A, int lda, long long sA, const double *B,
                 int ldb, long long sB, double *Cm, int ldc, long long sC, int batch, int kind = GEMM_AUTO);
 int launch_wmat(champ_ctx *c);

@@ -434,7 +434,9 @@ int champ_gpu_metrop1_batch(champ_ctx *const *ctxs, int32_t n, int32_t nsweep,
  * calls.  choices (may be NULL) [4]: the kernel kind of the AO->MO GEMM of the
  * moves, of the hpsi orbitals, of the ECP coefficient rows (0: 16 x 16 tiles
  * per wavefront, 1: 16 x 80 strips with LDS-shared B chunks, 2: VALU, 3: FP32
- * operands and accumulation, a candidate for the moves of the mixed path),
+ * operands and accumulation, a candidate for the moves of the mixed path,
+ * 4: the tile form split over two halves of K with their partials added in a
+ * fixed order, a candidate for the moves),
  * and the ECP quadrature form (0: matrix cores, 1: VALU); -1 where nothing was
  * tuned.  Leaves the walker state as
  * it was (hpsi is recomputed).  No counterpart in the reference.              */
@@ -442,8 +444,8 @@ int champ_gpu_autotune(champ_ctx *ctx, int32_t *choices);
 
 /* Applies kernel choices without timing them, in the layout autotune returns
  * (choices [4]; -1 restores the shape rule of that site).  GEMM kind 3 (FP32)
- * is accepted for the moves of the mixed path only.  No counterpart in the
- * reference.                                                                  */
+ * is accepted for the moves of the mixed path only, kind 4 (split K) for the
+ * moves only.  No counterpart in the reference.                              */
 int champ_gpu_set_tuning(champ_ctx *ctx, const int32_t *choices);
 
 /* on = 1: metrop1 replays each sweep as a HIP graph captured on first use

@@ -346,6 +346,34 @@ def test_fp32_move_contraction(require_gpu, name):
         assert relnorm(out[3][0][w], ow.orbn()) < 5e-5
 
 
+@pytest.mark.parametrize("name,prec", [("c22", PREC_FP64), ("c160", PREC_FP64), ("c160", PREC_MIXED)])
+def test_splitk_move_contraction(require_gpu, name, prec):
+    """The per-move contraction split over the two halves of K (GEMM kind 4:
+    the halves' partials added in a fixed order) against the one-pass tile
+    form on the same AO values (the sums differ only in their association,
+    1e-13 relative), and a second call giving the same bits."""
+    sy = S.config(name)
+    nw = 4
+    x = sy.initial_walkers(nw, seed=12)
+    iel = 7
+    xnew = x[:, iel, :] + 0.3
+    out = {}
+    for kind in (0, 4, 4):
+        e = Engine(sy, nw, precision=prec, lib_path=ENGINE_LIB)
+        e.set_walkers(x)
+        e.hpsi()
+        e.set_tuning(move_gemm=kind, ecp_rows_gemm=-1, quadrature=-1)
+        pd, _ = e.psie(iel, xnew)
+        out.setdefault(kind, []).append((e.array("orbn"), pd))
+        e.close()
+    (o0, p0), = out[0]
+    (o4, p4), (o4b, p4b) = out[4]
+    d = relnorm(o4, o0)
+    print(f"[split-K gemm {name}] orbn relnorm {d:.2e}, psidn rel {rel(p4, p0):.2e}", flush=True)
+    assert d < 1e-13 and rel(p4, p0) < 1e-11
+    assert np.array_equal(o4, o4b) and np.array_equal(p4, p4b)
+
+
 def test_set_tuning_rejects_fp32_on_fp64_path(require_gpu):
     e = Engine(S.config("small"), 2, lib_path=ENGINE_LIB)
     with pytest.raises(RuntimeError, match="not available"):

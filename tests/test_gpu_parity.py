@@ -186,7 +186,7 @@ def test_autotune_keeps_parity(require_gpu, name, nwalk):
     pr.quad()
     ch = pr.eng.autotune()
     print(f"[autotune {name}] {ch}", flush=True)
-    assert ch["move_gemm"] in (0, 1, 2) and ch["orbital_gemm"] in (0, 1, 2)
+    assert ch["move_gemm"] in (0, 1, 2, 4) and ch["orbital_gemm"] in (0, 1, 2)
     # all-electron systems have no ECP coefficient rows and no quadrature to tune
     ecp = pr.sys.nloc > 0
     assert ch["ecp_rows_gemm"] in ((0, 1, 2) if ecp else (-1,))
