@@ -321,7 +321,6 @@ def bench_metrop6(args):
 
     from champ_amd import system as S
     from champ_amd.engine import PREC_FP64, PREC_MIXED, Engine
-    from oracle import ffi
 
     sy = S.config(args.system)
     W = args.walkers
@@ -344,6 +343,7 @@ def bench_metrop6(args):
         print(f"[bench] metrop6 {label}: {args.steps} sweeps in {wall:.3f}s, acceptance {vr['accept']:.3f}", flush=True)
     cpu = None
     if not args.no_cpu:
+        from oracle import ffi  # the CPU baseline leg only
         model, ncpu, navail, nth = _cpu_threads(args)
         cs = sy.as_ctypes()
         x0 = sy.initial_walkers(nth, seed=1)
@@ -392,7 +392,6 @@ def bench_dmc(args):
     from champ_amd import champ_io as io
     from champ_amd import system as S
     from champ_amd.engine import PREC_FP64, PREC_MIXED, Engine
-    from oracle import ffi
 
     sy = io.load_input(os.path.join(ROOT, "tests", "golden", "champ_inputs", "C4H6_dmc", "dmc.inp"), name="C4H6_dmc")
     cap = args.walkers
@@ -404,7 +403,8 @@ def bench_dmc(args):
         eng.seed_rng(S.seed_from_string())
         eng.hpsi()
         eng.vmc(2, 10, nblkeq=0, tau=0.2)
-        pop = ffi.seeded_streams(S.seed_from_string(), cap + 1)[cap]
+        # the branching stream: any non-zero xoshiro state (a fixed one, from numpy)
+        pop = np.random.default_rng(2024).integers(1, 2 ** 63 - 1, size=4, dtype=np.uint64)
         eng.dmc_init(nconf, 0.05, -26.3, pop)
         eng.dmc_set_icasula(-1)
         for _ in range(args.warmup):
@@ -421,6 +421,7 @@ def bench_dmc(args):
         print(f"[bench] dmc {label}: {args.steps} steps in {wall:.3f}s, {nws / args.steps:.0f} walkers", flush=True)
     cpu = None
     if not args.no_cpu:
+        from oracle import ffi  # the CPU baseline leg only
         model, ncpu, navail, _ = _cpu_threads(args)
         ncw, nst = 16, 30
         x0 = sy.initial_walkers(ncw, seed=3)
