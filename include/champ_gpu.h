@@ -494,8 +494,12 @@ int champ_gpu_basis_fns(champ_ctx *ctx, int32_t npt, const double *pos,
 int champ_gpu_basis_fns_noalloc(champ_ctx *ctx, int32_t npt, const double *pos,
                                 double *phin);
 
-/* Timing of the last metrop1 call: per-kernel-class milliseconds measured
- * with HIP events on the context stream.  names/ms sized by *n on input.      */
+/* Per-kernel-class timing with HIP events on the context stream.  With names
+ * and ms both NULL the call switches timing on (*n > 0) or off (*n = 0) and
+ * clears the sums, returning the number of classes in *n.  Otherwise it
+ * returns the milliseconds and launch counts summed since then: names
+ * (n x 32 chars), ms, launches (any may be NULL) sized by *n on input.  No
+ * counterpart in the reference (m_timer times on the host).                   */
 int champ_gpu_kernel_times(champ_ctx *ctx, int32_t *n, char *names /* n*32 */,
                            double *ms, int64_t *launches);
 
